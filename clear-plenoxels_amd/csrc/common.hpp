@@ -1,0 +1,107 @@
+// Shared device helpers and error plumbing of libpxl.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+
+#include "pxl.h"
+
+namespace pxl {
+
+// Per-thread last-error text returned by pxl_last_error().
+void set_error(const char *fmt, ...);
+void clear_error();
+
+// Checks the launch that was just issued; records the HIP error text.
+int32_t check_launch(const char *what);
+
+#define PXL_REQUIRE(cond, ...)                 \
+    do {                                       \
+        if (!(cond)) {                         \
+            ::pxl::set_error(__VA_ARGS__);     \
+            return PXL_EINVAL;                 \
+        }                                      \
+    } while (0)
+
+inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+constexpr int kWave = 64;
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) {
+    return v < lo ? lo : (v > hi ? hi : v);
+}
+
+// getSceneIndex (RenderUtils.cu:3-13): clamp each coordinate to
+// [0, dim-1], linear z-y-x index.
+__device__ __forceinline__ uint32_t scene_index(int x, int y, int z, int W,
+                                                int H, int D) {
+    x = clampi(x, 0, W - 1);
+    y = clampi(y, 0, H - 1);
+    z = clampi(z, 0, D - 1);
+    return (uint32_t)z * (uint32_t)W * (uint32_t)H + (uint32_t)y * (uint32_t)W +
+           (uint32_t)x;
+}
+
+// Wave-uniform broadcast of a lane's 32-bit value.
+__device__ __forceinline__ float readlane_f(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+__device__ __forceinline__ double readlane_d(double v, int lane) {
+    const uint64_t u = (uint64_t)__double_as_longlong(v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, lane);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), lane);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+__device__ __forceinline__ int uniform_i(int v) {
+    return __builtin_amdgcn_readfirstlane(v);
+}
+
+// DPP row rotate (16-lane rows): ctrl 0x120 + n = row_ror:n.
+template <int N>
+__device__ __forceinline__ float row_ror_f(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(
+        0, __float_as_int(v), 0x120 + N, 0xF, 0xF, false));
+}
+template <int N>
+__device__ __forceinline__ double row_ror_d(double v) {
+    const uint64_t u = (uint64_t)__double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, 0x120 + N, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), 0x120 + N, 0xF, 0xF, false);
+    return __longlong_as_double(
+        (long long)(((uint64_t)(uint32_t)hi << 32) | (uint64_t)(uint32_t)lo));
+}
+
+// Sum over the 16 lanes of each DPP row; every lane of the row gets the sum.
+__device__ __forceinline__ float row_sum_f(float v) {
+    v += row_ror_f<8>(v);
+    v += row_ror_f<4>(v);
+    v += row_ror_f<2>(v);
+    v += row_ror_f<1>(v);
+    return v;
+}
+__device__ __forceinline__ double row_sum_d(double v) {
+    v += row_ror_d<8>(v);
+    v += row_ror_d<4>(v);
+    v += row_ror_d<2>(v);
+    v += row_ror_d<1>(v);
+    return v;
+}
+
+// Wave-wide sums through cross-lane shuffles (used off the hot loop).
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+__device__ __forceinline__ float wave_sum_f(float v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+__device__ __forceinline__ double wave_sum_d(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+}  // namespace pxl

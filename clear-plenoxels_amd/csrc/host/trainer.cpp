@@ -1,0 +1,224 @@
+// The reference's operator API over the C ABI, and the batch loop of
+// Render::objectSceneLearner (Render.cu:352-503) as a host object.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "common.hpp"
+#include "pxl_host.hpp"
+
+namespace pxl {
+namespace host {
+
+int32_t RenderUtils::renderRays(const pxl_rays &rays, const pxl_grid &scene,
+                                const int32_t *sampleIndex, int64_t startSampleIndex,
+                                int64_t len, hipStream_t stream) {
+    return pxl_render_fwd(&scene, &rays, sampleIndex, startSampleIndex, len, stream);
+}
+
+int32_t RenderUtils::renderRays_withGradients_RGB(const pxl_optim &optimizer,
+                                                  const pxl_rays &rays, const pxl_grid &scene,
+                                                  double *losses, const int32_t *sampleIndex,
+                                                  int64_t startSampleIndex, int64_t len,
+                                                  double maskRayLossCoefficient,
+                                                  float huberDelta, hipStream_t stream) {
+    return pxl_render_bwd_rgb(&scene, &optimizer, &rays, losses, sampleIndex, startSampleIndex,
+                              len, maskRayLossCoefficient, huberDelta, stream);
+}
+
+int32_t RenderUtils::addVarianceLoss(const pxl_grid &scene, const pxl_optim &optimizer,
+                                     double densityCoefficient, double shCoefficient,
+                                     int32_t varianceType, int32_t neighbourNums,
+                                     int32_t optimizeStep, hipStream_t stream) {
+    return pxl_add_variance_loss(&scene, &optimizer, densityCoefficient, shCoefficient,
+                                 varianceType, neighbourNums, optimizeStep, nullptr, 0, stream);
+}
+
+int32_t Optimizer::updateRGBScene(const pxl_optim &optimizer, const pxl_grid &scene,
+                                  double learningRate, double shLearningRate, int32_t timeStep,
+                                  hipStream_t stream) {
+    return pxl_adam_step(&scene, &optimizer, learningRate, shLearningRate, timeStep, stream);
+}
+
+}  // namespace host
+}  // namespace pxl
+
+struct pxl_trainer {
+    pxl_train_config cfg;
+    pxl_grid grid;
+    pxl_optim optim;
+    pxl_rays rays;
+    const int32_t *order;
+    int64_t n_rays;
+    double *losses;
+    double *loss_out;
+    hipStream_t stream;
+    int64_t batch_count;
+    int64_t t;
+    double lr, sh_lr;
+    // Optional HIP events around each batch's render launch.
+    std::vector<hipEvent_t> ev;
+    int64_t ev_used;
+};
+
+using namespace pxl;
+using namespace pxl::host;
+
+extern "C" pxl_trainer *pxl_trainer_create(const pxl_train_config *cfg, const pxl_grid *grid,
+                                           const pxl_optim *optim, const pxl_rays *rays,
+                                           const int32_t *order, int64_t n_rays,
+                                           double *losses, double *loss_out, void *stream) {
+    clear_error();
+    if (!cfg || !grid || !optim || !rays || !order || !losses || !loss_out) {
+        set_error("pxl_trainer_create: null argument");
+        return nullptr;
+    }
+    if (cfg->batch_size <= 0 || n_rays <= 0 || n_rays > rays->n || cfg->variance_interval <= 0) {
+        set_error("pxl_trainer_create: bad batch size %lld / ray count %lld / interval %d",
+                  (long long)cfg->batch_size, (long long)n_rays, cfg->variance_interval);
+        return nullptr;
+    }
+    pxl_trainer *tr = new (std::nothrow) pxl_trainer();
+    if (!tr) {
+        set_error("pxl_trainer_create: out of host memory");
+        return nullptr;
+    }
+    tr->cfg = *cfg;
+    tr->grid = *grid;
+    tr->optim = *optim;
+    tr->rays = *rays;
+    tr->order = order;
+    tr->n_rays = n_rays;
+    tr->losses = losses;
+    tr->loss_out = loss_out;
+    tr->stream = as_stream(stream);
+    tr->batch_count = (n_rays + cfg->batch_size - 1) / cfg->batch_size;
+    tr->t = 0;
+    tr->lr = cfg->learning_rate;
+    tr->sh_lr = cfg->sh_learning_rate;
+    tr->ev_used = 0;
+    return tr;
+}
+
+extern "C" int32_t pxl_trainer_time_render(pxl_trainer *tr, int32_t max_steps) {
+    clear_error();
+    PXL_REQUIRE(tr && max_steps >= 0, "pxl_trainer_time_render: bad argument");
+    for (hipEvent_t e : tr->ev) (void)hipEventDestroy(e);
+    tr->ev.assign((size_t)max_steps * 2, nullptr);
+    for (auto &e : tr->ev) {
+        if (hipEventCreate(&e) != hipSuccess) {
+            set_error("pxl_trainer_time_render: hipEventCreate failed");
+            return PXL_EHIP;
+        }
+    }
+    tr->ev_used = 0;
+    return PXL_OK;
+}
+
+// One batch (Render.cu:399-500).
+extern "C" int32_t pxl_trainer_step(pxl_trainer *tr) {
+    PXL_REQUIRE(tr != nullptr, "pxl_trainer_step: null trainer");
+    const pxl_train_config &c = tr->cfg;
+    const int64_t b = tr->t % tr->batch_count;
+    const int64_t epoch = tr->t / tr->batch_count;
+    const int64_t start = b * c.batch_size;
+    const int64_t len = std::min<int64_t>(c.batch_size, tr->n_rays - start);
+    const bool timed = tr->ev_used * 2 + 1 < (int64_t)tr->ev.size();
+    if (timed) (void)hipEventRecord(tr->ev[tr->ev_used * 2], tr->stream);
+    int32_t st;
+    if (c.fused) {
+        st = pxl_train_rays(&tr->grid, &tr->optim, &tr->rays, tr->losses, tr->order, start, len,
+                            c.mask_coefficient, c.huber_delta, tr->stream);
+    } else {
+        st = RenderUtils::renderRays(tr->rays, tr->grid, tr->order, start, len, tr->stream);
+        if (st == PXL_OK)
+            st = RenderUtils::renderRays_withGradients_RGB(tr->optim, tr->rays, tr->grid,
+                                                           tr->losses, tr->order, start, len,
+                                                           c.mask_coefficient, c.huber_delta,
+                                                           tr->stream);
+    }
+    if (st != PXL_OK) return st;
+    if (timed) {
+        (void)hipEventRecord(tr->ev[tr->ev_used * 2 + 1], tr->stream);
+        tr->ev_used++;
+    }
+    // Batch loss on the GPU in FP64 instead of a D2H copy and a host loop
+    // (Render.cu:443-449).
+    st = pxl_loss_reduce_f64(tr->losses, len, tr->loss_out, tr->stream);
+    if (st != PXL_OK) return st;
+    // Regulariser every batch when dense, every interval-th batch when
+    // sparse (Render.cu:469).
+    if (!tr->grid.sparse || b % c.variance_interval == 0) {
+        st = RenderUtils::addVarianceLoss(tr->grid, tr->optim, c.variance_coefficient,
+                                          c.sh_coefficient, c.variance_type, c.neighbour_nums,
+                                          c.optimize_step, tr->stream);
+        if (st != PXL_OK) return st;
+    }
+    const int32_t time_step = (int32_t)(epoch * tr->batch_count + b + 1);
+    st = Optimizer::updateRGBScene(tr->optim, tr->grid, tr->lr, tr->sh_lr, time_step, tr->stream);
+    if (st != PXL_OK) return st;
+    // LR decay (Render.cu:499-500).
+    const double decay = std::pow(2.0, 1.0 / (double)tr->batch_count / ((double)epoch + 1.0));
+    tr->lr /= decay;
+    tr->sh_lr /= decay;
+    tr->t++;
+    return PXL_OK;
+}
+
+// Average duration of the timed render launches; synchronises on them.
+extern "C" int32_t pxl_trainer_render_ms(pxl_trainer *tr, double *avg_ms, int64_t *count) {
+    PXL_REQUIRE(tr && avg_ms && count, "pxl_trainer_render_ms: null argument");
+    double tot = 0.0;
+    for (int64_t i = 0; i < tr->ev_used; i++) {
+        if (hipEventSynchronize(tr->ev[2 * i + 1]) != hipSuccess) {
+            set_error("pxl_trainer_render_ms: hipEventSynchronize failed");
+            return PXL_EHIP;
+        }
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, tr->ev[2 * i], tr->ev[2 * i + 1]) != hipSuccess) {
+            set_error("pxl_trainer_render_ms: hipEventElapsedTime failed");
+            return PXL_EHIP;
+        }
+        tot += ms;
+    }
+    *count = tr->ev_used;
+    *avg_ms = tr->ev_used ? tot / (double)tr->ev_used : 0.0;
+    return PXL_OK;
+}
+
+// Copies the last batch's FP64 MSE and PSNR to the host; PXL_ENAN when the
+// loss is not finite (the reference aborts training, Render.cu:450-457).
+extern "C" int32_t pxl_trainer_loss(pxl_trainer *tr, double *mse, double *psnr) {
+    PXL_REQUIRE(tr && mse && psnr, "pxl_trainer_loss: null argument");
+    double h[2];
+    if (hipMemcpyAsync(h, tr->loss_out, sizeof(h), hipMemcpyDeviceToHost, tr->stream) !=
+            hipSuccess ||
+        hipStreamSynchronize(tr->stream) != hipSuccess) {
+        set_error("pxl_trainer_loss: copy failed");
+        return PXL_EHIP;
+    }
+    *mse = h[0];
+    *psnr = h[1];
+    if (std::isnan(h[0])) {
+        set_error("pxl_trainer_loss: batch loss is NaN");
+        return PXL_ENAN;
+    }
+    return PXL_OK;
+}
+
+extern "C" int32_t pxl_trainer_state(const pxl_trainer *tr, int64_t *t, double *lr,
+                                     double *sh_lr) {
+    PXL_REQUIRE(tr != nullptr, "pxl_trainer_state: null trainer");
+    if (t) *t = tr->t;
+    if (lr) *lr = tr->lr;
+    if (sh_lr) *sh_lr = tr->sh_lr;
+    return PXL_OK;
+}
+
+extern "C" void pxl_trainer_destroy(pxl_trainer *tr) {
+    if (!tr) return;
+    for (hipEvent_t e : tr->ev) (void)hipEventDestroy(e);
+    delete tr;
+}

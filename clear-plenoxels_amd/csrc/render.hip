@@ -1,0 +1,634 @@
+// Ray marching: forward compositing, analytic backward, their fusion, and
+// two bookkeeping variants (index trace, touched-record bitmap).
+//
+// Work mapping (CDNA4, wave64): one wavefront marches one ray.  The ray's
+// samples are visited in order; per sample the 64 lanes split the record's
+// parameters: DPP row 0/1/2 hold the red/green/blue SH coefficient k at lane
+// k (k < S), row 3 lane 0 holds the density.  Each lane gathers its
+// parameter from the 8 cell corners (8 loads per lane, all lanes of a load
+// hitting one 128-byte record), interpolates, multiplies by its SH basis
+// value, and a 4-step DPP rotate-sum per row gives sigma_raw and the three
+// raw colours.  The compositing scalars are wave-uniform; transmittance,
+// opacity and colour accumulate in FP64 as in volumeRender_RGB_train
+// (RenderUtils.cu:1582-1718).  The backward re-marches the ray and each
+// lane owns the gradient of its parameter: per-corner sums are merged in
+// registers while consecutive samples stay in the same cell and flushed
+// with one coalesced global_atomic_add_f32 per corner (instead of
+// 8 x (1+3S) scattered atomicAdds per sample, RenderUtils.cu:1074-1364).
+#include <type_traits>
+
+#include "common.hpp"
+
+namespace pxl {
+namespace {
+
+enum Mode { kFwd = 0, kBwd = 1, kFused = 2, kTrace = 3, kTouch = 4 };
+
+// Hard cap on marching iterations per ray.  Legitimate rays need at most a
+// few thousand (diagonal / step); the cap only bounds rays whose far
+// distance is infinite or whose adaptive step |step - w| collapses.
+constexpr int kMaxSteps = 1 << 20;
+
+struct RenderArgs {
+    const float *params;
+    int64_t n_vox;
+    const uint32_t *occ;
+    const uint32_t *io;
+    int W, H, D, stride;
+    float step;
+    int white_bg;
+    pxl_rays rays;
+    const int32_t *sample_index;
+    int64_t start, len;
+    float *grad;
+    double *losses;
+    double mask_coef;
+    float huber;
+    uint32_t *trace;
+    uint32_t *bitmap;
+};
+
+// Activations (RenderUtils.cu:480-515) in float or double.
+template <typename R> __device__ __forceinline__ R softplus_shift(R raw);
+template <> __device__ __forceinline__ float softplus_shift<float>(float raw) {
+    const float x = raw - 13.0f;
+    return x > 15.0f ? x + log1pf(expf(-x)) : log1pf(expf(x));
+}
+template <> __device__ __forceinline__ double softplus_shift<double>(double raw) {
+    const double x = raw - 13.0;
+    return x > 30.0 ? x + log1p(exp(-x)) : log1p(exp(x));
+}
+template <typename R> __device__ __forceinline__ R dsoftplus_shift(R raw);
+template <> __device__ __forceinline__ float dsoftplus_shift<float>(float raw) {
+    return 1.0f / (1.0f + expf(13.0f - raw));
+}
+template <> __device__ __forceinline__ double dsoftplus_shift<double>(double raw) {
+    return 1.0 / (1.0 + exp(13.0 - raw));
+}
+template <typename R> __device__ __forceinline__ R sigmoid01(R raw);
+template <> __device__ __forceinline__ float sigmoid01<float>(float raw) {
+    return 1.0f / (1.0f + expf(-0.1f * raw));
+}
+template <> __device__ __forceinline__ double sigmoid01<double>(double raw) {
+    return 1.0 / (1.0 + exp(-0.1 * raw));
+}
+// 1 - exp(-x) without cancellation for small x.
+template <typename R> __device__ __forceinline__ R one_minus_exp_neg(R x);
+template <> __device__ __forceinline__ float one_minus_exp_neg<float>(float x) {
+    return -expm1f(-x);
+}
+template <> __device__ __forceinline__ double one_minus_exp_neg<double>(double x) {
+    return -expm1(-x);
+}
+
+template <typename R> __device__ __forceinline__ R row_sum(R v);
+template <> __device__ __forceinline__ float row_sum<float>(float v) { return row_sum_f(v); }
+template <> __device__ __forceinline__ double row_sum<double>(double v) { return row_sum_d(v); }
+template <typename R> __device__ __forceinline__ R readlane(R v, int l);
+template <> __device__ __forceinline__ float readlane<float>(float v, int l) { return readlane_f(v, l); }
+template <> __device__ __forceinline__ double readlane<double>(double v, int l) { return readlane_d(v, l); }
+
+// Unnormalised real SH basis (RenderUtils.cu:1639-1670), FP32 as in the
+// reference; entry k of the first 16.
+__device__ __forceinline__ float sh_basis(int k, float x, float y, float z) {
+    switch (k) {
+        case 0: return 1.0f;
+        case 1: return y;
+        case 2: return z;
+        case 3: return x;
+        case 4: return x * y;
+        case 5: return y * z;
+        case 6: return 3.0f * z * z - 1.0f;
+        case 7: return x * z;
+        case 8: return x * x - y * y;
+        case 9: return y * (3.0f * x * x - y * y);
+        case 10: return x * y * z;
+        case 11: return y * (5.0f * z * z - 1.0f);
+        case 12: return 5.0f * z * z * z - 3.0f * z;
+        case 13: return x * (5.0f * z * z - 1.0f);
+        case 14: return (x * x - y * y) * z;
+        case 15: return x * (x * x - 3.0f * y * y);
+        default: return 0.0f;
+    }
+}
+
+// One sample position: base cell, corner records and trilinear weights.
+template <typename R>
+struct Sample {
+    int bx, by, bz;
+    uint32_t v[8];
+    R w[8];
+};
+
+// Base cell of a position: the reference truncates with (unsigned int),
+// which maps (-1, 0) to 0 (RenderUtils.cu:828-832).
+__device__ __forceinline__ int base_of(double p) {
+    if (!(p > 0.0)) return 0;
+    if (p >= 1073741824.0) return 1073741824;
+    return (int)p;
+}
+
+// Returns false if the sample is skipped (sparse grid, empty cell:
+// RenderUtils.cu:835-842 and 1681-1686).
+template <typename R, bool SPARSE>
+__device__ __forceinline__ bool locate(const RenderArgs &a, double px, double py,
+                                       double pz, Sample<R> &s) {
+    s.bx = base_of(px);
+    s.by = base_of(py);
+    s.bz = base_of(pz);
+    if (SPARSE) {
+        const int cx = clampi(s.bx, 0, a.W - 2);
+        const int cy = clampi(s.by, 0, a.H - 2);
+        const int cz = clampi(s.bz, 0, a.D - 2);
+        const uint32_t cell = (uint32_t)cz * (uint32_t)(a.W - 1) * (uint32_t)(a.H - 1) +
+                              (uint32_t)cy * (uint32_t)(a.W - 1) + (uint32_t)cx;
+        const uint32_t word = a.occ[cell >> 5];
+        if (((word >> (cell & 31)) & 1u) == 0) return false;
+    }
+    const R fx = (R)(px - (double)s.bx);
+    const R fy = (R)(py - (double)s.by);
+    const R fz = (R)(pz - (double)s.bz);
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        const int dx = j & 1, dy = (j >> 1) & 1, dz = j >> 2;
+        uint32_t v = scene_index(s.bx + dx, s.by + dy, s.bz + dz, a.W, a.H, a.D);
+        if (SPARSE) v = min(a.io[v], (uint32_t)(a.n_vox - 1));
+        s.v[j] = v;
+        const R wx = dx ? fx : (R)1 - fx;
+        const R wy = dy ? fy : (R)1 - fy;
+        const R wz = dz ? fz : (R)1 - fz;
+        s.w[j] = wx * wy * wz;
+    }
+    return true;
+}
+
+// Interpolated raw density and raw colours of a located sample.
+template <typename R>
+__device__ __forceinline__ void interpolate(const RenderArgs &a, const Sample<R> &s,
+                                            int p, R ylane, R &sraw, R &c0, R &c1,
+                                            R &c2) {
+    R acc = 0;
+    if (p >= 0) {
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+            acc += s.w[j] * (R)a.params[(size_t)s.v[j] * (size_t)a.stride + p];
+    }
+    R val = acc * ylane;
+    val = row_sum<R>(val);
+    c0 = readlane<R>(val, 0);
+    c1 = readlane<R>(val, 16);
+    c2 = readlane<R>(val, 32);
+    sraw = readlane<R>(val, 48);
+}
+
+__device__ __forceinline__ uint32_t fnv_mix(uint32_t h, uint32_t x) {
+    for (int i = 0; i < 4; i++) {
+        h ^= (x >> (8 * i)) & 0xFFu;
+        h *= 16777619u;
+    }
+    return h;
+}
+
+template <int S, bool SPARSE, bool ADAPT, int MODE>
+__global__ __launch_bounds__(256) void k_render(const RenderArgs a) {
+    using R = typename std::conditional<ADAPT, double, float>::type;
+    const int lane = threadIdx.x & 63;
+    const int64_t b = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (b >= a.len) return;
+    const int ray = uniform_i(a.sample_index[a.start + b]);
+    if (ray < 0 || (int64_t)ray >= a.rays.n) {
+        // Out-of-range sample index: no outputs, zero loss.
+        if ((MODE == kBwd || MODE == kFused) && lane == 0) a.losses[b] = 0.0;
+        if (MODE == kTrace && lane == 0) {
+            a.trace[2 * b] = 0u;
+            a.trace[2 * b + 1] = 2166136261u;
+        }
+        return;
+    }
+
+    const double ox = a.rays.ox[ray], oy = a.rays.oy[ray], oz = a.rays.oz[ray];
+    const float fdx = a.rays.dx[ray], fdy = a.rays.dy[ray], fdz = a.rays.dz[ray];
+    const double dx = fdx, dy = fdy, dz = fdz;
+    const double tnear = a.rays.near_t[ray], tfar = a.rays.far_t[ray];
+    const double step = (double)a.step;
+    const R rstep = (R)a.step;
+
+    // Lane role: rows 0..2 = colour channel, lane k = SH coefficient k;
+    // row 3 lane 0 = density.
+    const int row = lane >> 4, kk = lane & 15;
+    int p = -1;
+    if (row < 3 && kk < S) p = 1 + row * S + kk;
+    else if (row == 3 && kk == 0) p = 0;
+    const R ylane = row < 3 ? (R)sh_basis(kk, fdx, fdy, fdz) : (R)1;
+
+    const bool do_fwd = MODE == kFwd || MODE == kFused || MODE == kTrace || MODE == kTouch;
+    const bool do_bwd = MODE == kBwd || MODE == kFused;
+
+    // Forward totals (FP64).
+    double C0 = 0.0, C1 = 0.0, C2 = 0.0, A = 0.0, sigsum = 0.0;
+
+    if (tnear > tfar) {
+        if (MODE == kFwd || MODE == kFused || MODE == kTrace) {
+            if (lane == 0) {
+                a.rays.acc_r[ray] = 0.0f;
+                a.rays.acc_g[ray] = 0.0f;
+                a.rays.acc_b[ray] = 0.0f;
+                a.rays.acc_w[ray] = 0.0f;
+                a.rays.acc_density[ray] = 0.0f;
+            }
+        }
+        if (MODE == kTrace && lane == 0) {
+            a.trace[2 * b] = 0u;
+            a.trace[2 * b + 1] = 2166136261u;
+        }
+        if (do_bwd && lane == 0) a.losses[b] = 0.0;
+        return;
+    }
+
+    if (do_fwd) {
+        double t = tnear, T = 1.0;
+        int k = 0;
+        uint32_t hcount = 0, hash = 2166136261u;
+        int last_bx = -1, last_by = -1, last_bz = -1;
+        while (t < tfar && k < kMaxSteps) {
+            const double px = fma(t, dx, ox), py = fma(t, dy, oy), pz = fma(t, dz, oz);
+            Sample<R> s;
+            if (!locate<R, SPARSE>(a, px, py, pz, s)) {
+                if (ADAPT) t += step;
+                ++k;
+                if (!ADAPT) t = fma((double)k, step, tnear);
+                continue;
+            }
+            if (MODE == kTrace) {
+                hcount++;
+                hash = fnv_mix(hash, (uint32_t)k);
+                hash = fnv_mix(hash, (uint32_t)s.bx);
+                hash = fnv_mix(hash, (uint32_t)s.by);
+                hash = fnv_mix(hash, (uint32_t)s.bz);
+            }
+            if (MODE == kTouch) {
+                if (s.bx != last_bx || s.by != last_by || s.bz != last_bz) {
+                    if (lane < 8) {
+                        uint32_t r8 = s.v[0];
+#pragma unroll
+                        for (int j = 1; j < 8; j++)
+                            if (lane == j) r8 = s.v[j];
+                        atomicOr(a.bitmap + (r8 >> 5), 1u << (r8 & 31));
+                    }
+                    last_bx = s.bx;
+                    last_by = s.by;
+                    last_bz = s.bz;
+                }
+                // The touched set does not depend on the marching weights
+                // under the fixed rule; the adaptive rule needs them.
+                if (!ADAPT) {
+                    ++k;
+                    t = fma((double)k, step, tnear);
+                    continue;
+                }
+            }
+            R sraw, cr0, cr1, cr2;
+            interpolate<R>(a, s, p, ylane, sraw, cr0, cr1, cr2);
+            const R sigma = softplus_shift<R>(sraw);
+            const R col0 = sigmoid01<R>(cr0), col1 = sigmoid01<R>(cr1), col2 = sigmoid01<R>(cr2);
+            const R alpha = one_minus_exp_neg<R>(sigma * rstep);
+            const double w = T * (double)alpha;
+            A += w;
+            C0 += w * (double)col0;
+            C1 += w * (double)col1;
+            C2 += w * (double)col2;
+            sigsum += (double)sigma;
+            T = T * (1.0 - (double)alpha);
+            ++k;
+            if (ADAPT) t += fabs(step - w);
+            else t = fma((double)k, step, tnear);
+        }
+        if (a.white_bg) {
+            C0 += 1.0 - A;
+            C1 += 1.0 - A;
+            C2 += 1.0 - A;
+        }
+        if (MODE == kFwd || MODE == kFused || MODE == kTrace) {
+            if (lane == 0) {
+                a.rays.acc_r[ray] = (float)C0;
+                a.rays.acc_g[ray] = (float)C1;
+                a.rays.acc_b[ray] = (float)C2;
+                a.rays.acc_w[ray] = (float)A;
+                a.rays.acc_density[ray] = (float)sigsum;
+            }
+        }
+        if (MODE == kTrace && lane == 0) {
+            a.trace[2 * b] = hcount;
+            a.trace[2 * b + 1] = hash;
+        }
+    }
+
+    if (!do_bwd) return;
+
+    // Backward (renderRays_withGradients_RGBGPU, RenderUtils.cu:2149-2254).
+    if (MODE == kBwd) {
+        C0 = a.rays.acc_r[ray];
+        C1 = a.rays.acc_g[ray];
+        C2 = a.rays.acc_b[ray];
+        A = a.rays.acc_w[ray];
+    }
+    double r0 = C0 - (double)a.rays.gt_r[ray] / 255.0;
+    double r1 = C1 - (double)a.rays.gt_g[ray] / 255.0;
+    double r2 = C2 - (double)a.rays.gt_b[ray] / 255.0;
+    double loss = (fabs(r0) + fabs(r1) + fabs(r2)) * 255.0 / 3.0;
+    if (isnan(loss)) {
+        if (lane == 0) a.losses[b] = 255.0;
+        return;
+    }
+    if (lane == 0) a.losses[b] = loss;
+    const double hd = (double)a.huber;
+    const double hi = hd - 0.5 * hd * hd, lo = 0.5 * hd * hd - hd;
+    r0 = r0 > hd ? hi : (r0 < -hd ? lo : r0);
+    r1 = r1 > hd ? hi : (r1 < -hd ? lo : r1);
+    r2 = r2 > hd ? hi : (r2 < -hd ? lo : r2);
+
+    // Mask and opacity-logit term (RenderUtils.cu:2188-2193).
+    double mterm = 0.0;
+    const int gm = a.rays.gt_mask[ray];
+    if (gm != 0 && A < 1.0) mterm = ((double)gm / 255.0 - A) * a.mask_coef;
+    double wpre = A > 1e-6 ? A : 1e-6;
+    wpre = wpre < 1.0 - 1e-6 ? wpre : 1.0 - 1e-6;
+    mterm += (log(wpre) - log(1.0 - wpre)) * 1e-3;
+
+    double t = tnear, T = 1.0, Ai = 0.0, H0 = 0.0, H1 = 0.0, H2 = 0.0;
+    int k = 0;
+    const bool active = p >= 0;
+    float acc[8];
+    uint32_t cv[8];
+    int cbx = -1, cby = -1, cbz = -1;
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        acc[j] = 0.0f;
+        cv[j] = 0u;
+    }
+    float *gbase = a.grad + p;
+    const size_t stride = (size_t)a.stride;
+
+    while (t < tfar && k < kMaxSteps) {
+        const double px = fma(t, dx, ox), py = fma(t, dy, oy), pz = fma(t, dz, oz);
+        Sample<R> s;
+        if (!locate<R, SPARSE>(a, px, py, pz, s)) {
+            if (ADAPT) t += step;
+            ++k;
+            if (!ADAPT) t = fma((double)k, step, tnear);
+            continue;
+        }
+        R sraw, cr0, cr1, cr2;
+        interpolate<R>(a, s, p, ylane, sraw, cr0, cr1, cr2);
+        const R sigma = softplus_shift<R>(sraw);
+        const R col0 = sigmoid01<R>(cr0), col1 = sigmoid01<R>(cr1), col2 = sigmoid01<R>(cr2);
+        const R alpha = one_minus_exp_neg<R>(sigma * rstep);
+        const double w = T * (double)alpha;
+        Ai += w;
+        H0 += w * (double)col0;
+        H1 += w * (double)col1;
+        H2 += w * (double)col2;
+        // Colour gradients: 0.6667 * w * r * sigmoid'(raw).
+        // (0.6667f is a float literal in the reference, RenderUtils.cu:2220-2222.)
+        const double kc = (double)0.6667f;
+        const double g0 = kc * w * r0 * (double)((R)0.1 * col0 * ((R)1 - col0));
+        const double g1 = kc * w * r1 * (double)((R)0.1 * col1 * ((R)1 - col1));
+        const double g2 = kc * w * r2 * (double)((R)0.1 * col2 * ((R)1 - col2));
+        const double Tn = T * (1.0 - (double)alpha);
+        const double ds = (double)dsoftplus_shift<R>(sraw);
+        double gs = r0 * (Tn * (double)col0 - (C0 - H0)) + r1 * (Tn * (double)col1 - (C1 - H1)) +
+                    r2 * (Tn * (double)col2 - (C2 - H2));
+        gs *= 0.6667 * step * ds;
+        gs += step * ds * (A - Ai - Tn) * mterm;
+        T = Tn;
+
+        // This lane's coefficient of d(param)/d(corner value).
+        const double gch = row == 0 ? g0 : (row == 1 ? g1 : g2);
+        const float coef = (float)(row == 3 ? gs : gch * (double)ylane);
+
+        if (s.bx != cbx || s.by != cby || s.bz != cbz) {
+            if (cbx >= 0 && active) {
+#pragma unroll
+                for (int j = 0; j < 8; j++)
+                    unsafeAtomicAdd(gbase + (size_t)cv[j] * stride, acc[j]);
+            }
+            cbx = s.bx;
+            cby = s.by;
+            cbz = s.bz;
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                cv[j] = s.v[j];
+                acc[j] = 0.0f;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 8; j++) acc[j] += coef * (float)s.w[j];
+
+        ++k;
+        if (ADAPT) t += fabs(step - w);
+        else t = fma((double)k, step, tnear);
+    }
+    if (cbx >= 0 && active) {
+#pragma unroll
+        for (int j = 0; j < 8; j++) unsafeAtomicAdd(gbase + (size_t)cv[j] * stride, acc[j]);
+    }
+}
+
+template <int S, bool SPARSE, bool ADAPT, int MODE>
+int32_t launch3(const RenderArgs &a, hipStream_t st) {
+    const unsigned blocks = (unsigned)((a.len + 3) / 4);
+    hipLaunchKernelGGL((k_render<S, SPARSE, ADAPT, MODE>), dim3(blocks), dim3(256), 0, st, a);
+    return PXL_OK;
+}
+
+template <int MODE>
+int32_t dispatch(int S, bool sparse, bool adapt, const RenderArgs &a, hipStream_t st) {
+#define PXL_CASE(SV)                                                              \
+    case SV:                                                                      \
+        if (sparse) {                                                             \
+            return adapt ? launch3<SV, true, true, MODE>(a, st)                   \
+                         : launch3<SV, true, false, MODE>(a, st);                 \
+        } else {                                                                  \
+            return adapt ? launch3<SV, false, true, MODE>(a, st)                  \
+                         : launch3<SV, false, false, MODE>(a, st);                \
+        }
+    switch (S) {
+        PXL_CASE(1)
+        PXL_CASE(4)
+        PXL_CASE(9)
+        PXL_CASE(16)
+        default:
+            set_error("render: unsupported sh_dim %d (1, 4, 9 or 16)", S);
+            return PXL_EINVAL;
+    }
+#undef PXL_CASE
+}
+
+int32_t check_grid(const pxl_grid *g, const char *who) {
+    PXL_REQUIRE(g != nullptr, "%s: null grid", who);
+    PXL_REQUIRE(g->W >= 2 && g->H >= 2 && g->D >= 2, "%s: grid %dx%dx%d too small", who, g->W,
+                g->H, g->D);
+    PXL_REQUIRE((int64_t)g->W * g->H * g->D < (int64_t)1 << 32,
+                "%s: grid of %lld vertices exceeds 32-bit indexing", who,
+                (long long)g->W * g->H * g->D);
+    PXL_REQUIRE(g->sh_dim == 1 || g->sh_dim == 4 || g->sh_dim == 9 || g->sh_dim == 16,
+                "%s: unsupported sh_dim %d (1, 4, 9 or 16)", who, g->sh_dim);
+    PXL_REQUIRE(g->stride >= 1 + 3 * g->sh_dim && g->stride % 4 == 0,
+                "%s: stride %d must be a multiple of 4 and >= %d", who, g->stride,
+                1 + 3 * g->sh_dim);
+    PXL_REQUIRE(g->params != nullptr && g->n_vox > 0, "%s: empty grid", who);
+    PXL_REQUIRE(g->sparse || g->n_vox == (int64_t)g->W * g->H * g->D,
+                "%s: dense grid needs n_vox = W*H*D", who);
+    PXL_REQUIRE(!g->sparse || (g->occupancy && g->index_offsets),
+                "%s: sparse grid needs occupancy and index_offsets", who);
+    PXL_REQUIRE(g->step > 0.0f, "%s: step must be positive", who);
+    PXL_REQUIRE(g->march_mode == PXL_MARCH_FIXED || g->march_mode == PXL_MARCH_REF_ADAPTIVE,
+                "%s: bad march_mode %d", who, g->march_mode);
+    return PXL_OK;
+}
+
+int32_t prepare(const pxl_grid *g, const pxl_rays *r, const int32_t *sample_index,
+                int64_t start, int64_t len, const char *who, RenderArgs &a) {
+    int32_t st = check_grid(g, who);
+    if (st != PXL_OK) return st;
+    PXL_REQUIRE(r != nullptr && sample_index != nullptr, "%s: null rays or sample_index", who);
+    PXL_REQUIRE(start >= 0 && len >= 0, "%s: negative start/len", who);
+    PXL_REQUIRE(r->ox && r->oy && r->oz && r->dx && r->dy && r->dz && r->near_t && r->far_t,
+                "%s: ray geometry planes must be non-null", who);
+    memset(&a, 0, sizeof(a));
+    a.params = g->params;
+    a.n_vox = g->n_vox;
+    a.occ = g->occupancy;
+    a.io = g->index_offsets;
+    a.W = g->W;
+    a.H = g->H;
+    a.D = g->D;
+    a.stride = g->stride;
+    a.step = g->step;
+    a.white_bg = r->white_bg;
+    a.rays = *r;
+    a.sample_index = sample_index;
+    a.start = start;
+    a.len = len;
+    return PXL_OK;
+}
+
+}  // namespace
+}  // namespace pxl
+
+using namespace pxl;
+
+extern "C" int32_t pxl_render_fwd(const pxl_grid *grid, const pxl_rays *rays,
+                                  const int32_t *sample_index, int64_t start, int64_t len,
+                                  void *stream) {
+    clear_error();
+    RenderArgs a;
+    int32_t st = prepare(grid, rays, sample_index, start, len, "pxl_render_fwd", a);
+    if (st != PXL_OK) return st;
+    PXL_REQUIRE(rays->acc_r && rays->acc_g && rays->acc_b && rays->acc_w && rays->acc_density,
+                "pxl_render_fwd: output planes must be non-null");
+    if (len == 0) return PXL_OK;
+    st = dispatch<kFwd>(grid->sh_dim, grid->sparse, grid->march_mode == PXL_MARCH_REF_ADAPTIVE, a,
+                        as_stream(stream));
+    return st != PXL_OK ? st : check_launch("pxl_render_fwd");
+}
+
+extern "C" int32_t pxl_render_fwd_trace(const pxl_grid *grid, const pxl_rays *rays,
+                                        const int32_t *sample_index, int64_t start,
+                                        int64_t len, uint32_t *trace, void *stream) {
+    clear_error();
+    RenderArgs a;
+    int32_t st = prepare(grid, rays, sample_index, start, len, "pxl_render_fwd_trace", a);
+    if (st != PXL_OK) return st;
+    PXL_REQUIRE(trace != nullptr, "pxl_render_fwd_trace: null trace");
+    PXL_REQUIRE(rays->acc_r && rays->acc_g && rays->acc_b && rays->acc_w && rays->acc_density,
+                "pxl_render_fwd_trace: output planes must be non-null");
+    if (len == 0) return PXL_OK;
+    a.trace = trace;
+    st = dispatch<kTrace>(grid->sh_dim, grid->sparse,
+                          grid->march_mode == PXL_MARCH_REF_ADAPTIVE, a, as_stream(stream));
+    return st != PXL_OK ? st : check_launch("pxl_render_fwd_trace");
+}
+
+static int32_t bwd_common(const pxl_grid *grid, const pxl_optim *optim, const pxl_rays *rays,
+                          double *losses, const int32_t *sample_index, int64_t start,
+                          int64_t len, double mask_coef, float huber_delta, void *stream,
+                          bool fused) {
+    const char *who = fused ? "pxl_train_rays" : "pxl_render_bwd_rgb";
+    clear_error();
+    RenderArgs a;
+    int32_t st = prepare(grid, rays, sample_index, start, len, who, a);
+    if (st != PXL_OK) return st;
+    PXL_REQUIRE(optim && optim->grad, "%s: null gradient", who);
+    PXL_REQUIRE(losses != nullptr, "%s: null losses", who);
+    PXL_REQUIRE(rays->gt_r && rays->gt_g && rays->gt_b && rays->gt_mask,
+                "%s: ground-truth planes must be non-null", who);
+    PXL_REQUIRE(rays->acc_r && rays->acc_g && rays->acc_b && rays->acc_w && rays->acc_density,
+                "%s: forward planes must be non-null", who);
+    if (len == 0) return PXL_OK;
+    a.grad = optim->grad;
+    a.losses = losses;
+    a.mask_coef = mask_coef;
+    a.huber = huber_delta;
+    const bool adapt = grid->march_mode == PXL_MARCH_REF_ADAPTIVE;
+    st = fused ? dispatch<kFused>(grid->sh_dim, grid->sparse, adapt, a, as_stream(stream))
+               : dispatch<kBwd>(grid->sh_dim, grid->sparse, adapt, a, as_stream(stream));
+    return st != PXL_OK ? st : check_launch(who);
+}
+
+extern "C" int32_t pxl_render_bwd_rgb(const pxl_grid *grid, const pxl_optim *optim,
+                                      const pxl_rays *rays, double *losses,
+                                      const int32_t *sample_index, int64_t start, int64_t len,
+                                      double mask_coef, float huber_delta, void *stream) {
+    return bwd_common(grid, optim, rays, losses, sample_index, start, len, mask_coef,
+                      huber_delta, stream, false);
+}
+
+extern "C" int32_t pxl_train_rays(const pxl_grid *grid, const pxl_optim *optim,
+                                  const pxl_rays *rays, double *losses,
+                                  const int32_t *sample_index, int64_t start, int64_t len,
+                                  double mask_coef, float huber_delta, void *stream) {
+    return bwd_common(grid, optim, rays, losses, sample_index, start, len, mask_coef,
+                      huber_delta, stream, true);
+}
+
+namespace pxl {
+namespace {
+__global__ void k_fill_u32(uint32_t *p, int64_t n, uint32_t v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+__global__ void k_popcount(const uint32_t *bm, int64_t nwords, unsigned long long *count) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t c = 0;
+    for (; i < nwords; i += (int64_t)gridDim.x * blockDim.x) c += __popc(bm[i]);
+    c = wave_sum_u32(c);
+    if ((threadIdx.x & 63) == 0 && c) atomicAdd(count, (unsigned long long)c);
+}
+}  // namespace
+}  // namespace pxl
+
+extern "C" int32_t pxl_count_touched(const pxl_grid *grid, const pxl_rays *rays,
+                                     const int32_t *sample_index, int64_t start, int64_t len,
+                                     uint32_t *bitmap, uint64_t *count, void *stream) {
+    clear_error();
+    RenderArgs a;
+    int32_t st = prepare(grid, rays, sample_index, start, len, "pxl_count_touched", a);
+    if (st != PXL_OK) return st;
+    PXL_REQUIRE(bitmap && count, "pxl_count_touched: null bitmap/count");
+    const int64_t nwords = (grid->n_vox + 31) / 32;
+    hipStream_t s = as_stream(stream);
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((nwords + 255) / 256)), dim3(256), 0, s,
+                       bitmap, nwords, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, reinterpret_cast<uint32_t *>(count),
+                       (int64_t)2, 0u);
+    if (len > 0) {
+        a.bitmap = bitmap;
+        st = dispatch<kTouch>(grid->sh_dim, grid->sparse,
+                              grid->march_mode == PXL_MARCH_REF_ADAPTIVE, a, s);
+        if (st != PXL_OK) return st;
+    }
+    hipLaunchKernelGGL(k_popcount, dim3(256), dim3(256), 0, s, bitmap, nwords,
+                       reinterpret_cast<unsigned long long *>(count));
+    return check_launch("pxl_count_touched");
+}

@@ -1,0 +1,466 @@
+"""Python binding of libpxl.so, the MI355X-native Plenoxel train-step library.
+
+Thin ctypes layer over the C ABI declared in include/pxl.h.  Device memory
+comes from torch (ROCm) tensors; every call is queued on torch's current HIP
+stream.  There is no fallback: if libpxl.so is missing or a call fails, an
+exception is raised.
+
+The entry points mirror the reference's operator API
+(SparseVolumeRadianceField/RenderUtils.cuh:7-85, Optimizer.cuh:35-37,
+SceneUtils.cuh:86-90, Rays.cuh:48-53); see each function's docstring.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG_DIR = os.path.dirname(HERE)
+LIB_PATH = os.path.join(PKG_DIR, "libpxl.so")
+
+PXL_OK, PXL_EINVAL, PXL_ENOMEM, PXL_EHIP, PXL_ENAN = 0, 1, 2, 3, 4
+MARCH_FIXED, MARCH_REF_ADAPTIVE = 0, 1
+TV_NEIGHBOUR, TV_DISPARITY = 0, 1
+
+# Every symbol include/pxl.h declares.
+EXPORTS = (
+    "pxl_last_error", "pxl_version", "pxl_camera_from_pose", "pxl_rays_init",
+    "pxl_render_fwd", "pxl_render_fwd_trace", "pxl_render_bwd_rgb", "pxl_train_rays",
+    "pxl_tv_scratch_bytes", "pxl_add_variance_loss", "pxl_adam_step",
+    "pxl_build_occupancy", "pxl_sparse_index_scratch_bytes", "pxl_build_sparse_index",
+    "pxl_grid_compact", "pxl_grid_import_planes", "pxl_grid_export_planes",
+    "pxl_loss_reduce_f64", "pxl_count_touched", "pxl_trainer_create", "pxl_trainer_step",
+    "pxl_trainer_time_render", "pxl_trainer_render_ms", "pxl_trainer_loss",
+    "pxl_trainer_state", "pxl_trainer_destroy",
+)
+
+
+class PxlError(RuntimeError):
+    def __init__(self, fn, code, msg):
+        super().__init__(f"{fn} failed with status {code}: {msg}")
+        self.code = code
+
+
+def build():
+    """Compile libpxl.so in-tree (hipcc, gfx950)."""
+    import subprocess
+
+    jobs = str(min(16, os.cpu_count() or 4))
+    subprocess.run(["make", "-s", "-C", PKG_DIR, "-j" + jobs], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"libpxl.so not built at {LIB_PATH}; run __graft_entry__.build()")
+        _lib = ctypes.CDLL(LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+_P = ctypes.c_void_p
+_I32, _I64, _F32, _F64, _SZ = (ctypes.c_int32, ctypes.c_int64, ctypes.c_float, ctypes.c_double,
+                               ctypes.c_size_t)
+
+
+class Camera(ctypes.Structure):
+    _fields_ = [("c2w", ctypes.c_double * 16), ("centre", ctypes.c_double * 4),
+                ("fx", ctypes.c_double), ("fy", ctypes.c_double),
+                ("cx", ctypes.c_double), ("cy", ctypes.c_double)]
+
+
+class BBox(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int32) for n in ("x_min", "x_max", "y_min", "y_max", "z_min", "z_max")]
+
+
+_RAY_FLOAT = ("ox", "oy", "oz", "dx", "dy", "dz", "near_t", "far_t")
+_RAY_U8 = ("gt_r", "gt_g", "gt_b", "gt_mask")
+_RAY_OUT = ("acc_r", "acc_g", "acc_b", "acc_w", "acc_density")
+
+
+class RaysC(ctypes.Structure):
+    _fields_ = ([(n, _P) for n in _RAY_FLOAT + _RAY_U8 + _RAY_OUT] +
+                [("n", ctypes.c_int64), ("white_bg", ctypes.c_int32), ("reserved", ctypes.c_int32)])
+
+
+class GridC(ctypes.Structure):
+    _fields_ = [("W", _I32), ("H", _I32), ("D", _I32), ("sh_dim", _I32), ("stride", _I32),
+                ("sparse", _I32), ("n_vox", _I64), ("params", _P), ("occupancy", _P),
+                ("index_offsets", _P), ("in_cage", _P), ("compact_to_dense", _P),
+                ("bbox", BBox), ("step", _F32), ("march_mode", _I32)]
+
+
+class OptimC(ctypes.Structure):
+    _fields_ = [("grad", _P), ("m", _P), ("v", _P), ("adaptive_lr", _P)]
+
+
+class TrainConfigC(ctypes.Structure):
+    _fields_ = [("batch_size", _I64), ("learning_rate", _F64), ("sh_learning_rate", _F64),
+                ("variance_coefficient", _F64), ("sh_coefficient", _F64),
+                ("variance_type", _I32), ("neighbour_nums", _I32),
+                ("variance_interval", _I32), ("optimize_step", _I32),
+                ("mask_coefficient", _F64), ("huber_delta", _F32), ("fused", _I32)]
+
+
+def _declare(L):
+    G, R, O = ctypes.POINTER(GridC), ctypes.POINTER(RaysC), ctypes.POINTER(OptimC)
+    sig = {
+        "pxl_last_error": ([], ctypes.c_char_p),
+        "pxl_version": ([], _I32),
+        "pxl_camera_from_pose": ([_P, _P, _F64, _I32, _I32, _F64, _F64, _F64, _F64,
+                                  ctypes.POINTER(Camera), _P], _I32),
+        "pxl_rays_init": ([ctypes.POINTER(Camera), ctypes.POINTER(BBox), _I32, _I32, _P, _P, _P,
+                           _P, _I64, R, _P], _I32),
+        "pxl_render_fwd": ([G, R, _P, _I64, _I64, _P], _I32),
+        "pxl_render_fwd_trace": ([G, R, _P, _I64, _I64, _P, _P], _I32),
+        "pxl_render_bwd_rgb": ([G, O, R, _P, _P, _I64, _I64, _F64, _F32, _P], _I32),
+        "pxl_train_rays": ([G, O, R, _P, _P, _I64, _I64, _F64, _F32, _P], _I32),
+        "pxl_tv_scratch_bytes": ([G], _SZ),
+        "pxl_add_variance_loss": ([G, O, _F64, _F64, _I32, _I32, _I32, _P, _SZ, _P], _I32),
+        "pxl_adam_step": ([G, O, _F64, _F64, _I32, _P], _I32),
+        "pxl_build_occupancy": ([_P, _I32, _I32, _I32, _F32, _P, _P, _P, _P], _I32),
+        "pxl_sparse_index_scratch_bytes": ([_I64], _SZ),
+        "pxl_build_sparse_index": ([_P, _I32, _I32, _I32, _P, _P, _P, _P, _SZ, _P], _I32),
+        "pxl_grid_compact": ([_P, _I32, _P, _I64, _P, _P], _I32),
+        "pxl_grid_import_planes": ([_P, _I32, _I64, _P, _I32, _P], _I32),
+        "pxl_grid_export_planes": ([_P, _I32, _I64, _P, _I32, _P], _I32),
+        "pxl_loss_reduce_f64": ([_P, _I64, _P, _P], _I32),
+        "pxl_count_touched": ([G, R, _P, _I64, _I64, _P, _P, _P], _I32),
+        "pxl_trainer_create": ([ctypes.POINTER(TrainConfigC), G, O, R, _P, _I64, _P, _P, _P],
+                               _P),
+        "pxl_trainer_step": ([_P], _I32),
+        "pxl_trainer_time_render": ([_P, _I32], _I32),
+        "pxl_trainer_render_ms": ([_P, ctypes.POINTER(_F64), ctypes.POINTER(_I64)], _I32),
+        "pxl_trainer_loss": ([_P, ctypes.POINTER(_F64), ctypes.POINTER(_F64)], _I32),
+        "pxl_trainer_state": ([_P, ctypes.POINTER(_I64), ctypes.POINTER(_F64),
+                               ctypes.POINTER(_F64)], _I32),
+        "pxl_trainer_destroy": ([_P], None),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+
+
+def _check(fn, code):
+    if code != PXL_OK:
+        raise PxlError(fn, code, lib().pxl_last_error().decode())
+
+
+def _ptr(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def _stream(stream=None):
+    import torch
+
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+# ---------------------------------------------------------------- data ---
+
+
+class Rays:
+    """Ray set in device memory (SoA, Rays.cuh:7-56)."""
+
+    def __init__(self, n, device="cuda", white_bg=True):
+        import torch
+
+        self.n = int(n)
+        self.white_bg = bool(white_bg)
+        for k in _RAY_FLOAT + _RAY_OUT:
+            setattr(self, k, torch.zeros(self.n, dtype=torch.float32, device=device))
+        for k in _RAY_U8:
+            setattr(self, k, torch.zeros(self.n, dtype=torch.uint8, device=device))
+
+    @classmethod
+    def from_arrays(cls, arrays, device="cuda", white_bg=True):
+        import torch
+
+        n = len(arrays["ox"])
+        r = cls(n, device=device, white_bg=white_bg)
+        for k, v in arrays.items():
+            if hasattr(r, k):
+                getattr(r, k).copy_(torch.as_tensor(v).to(getattr(r, k).dtype))
+        return r
+
+    def c(self):
+        s = RaysC()
+        for k in _RAY_FLOAT + _RAY_U8 + _RAY_OUT:
+            setattr(s, k, getattr(self, k).data_ptr())
+        s.n = self.n
+        s.white_bg = int(self.white_bg)
+        return s
+
+
+def record_stride(S):
+    P = 1 + 3 * S
+    return 32 if P <= 32 else ((P + 3) // 4) * 4
+
+
+class Grid:
+    """Voxel grid in device memory: records [n_vox, stride] float32 plus the
+    caged-layout index arrays when sparse (SceneUtils.cuh:4-98)."""
+
+    def __init__(self, W, H, D, S, params, step=0.5, bbox=None, march_mode=MARCH_FIXED,
+                 occupancy=None, index_offsets=None, in_cage=None, compact_to_dense=None):
+        self.W, self.H, self.D, self.S = int(W), int(H), int(D), int(S)
+        self.params = params
+        self.stride = int(params.shape[1])
+        self.n_vox = int(params.shape[0])
+        self.step = float(step)
+        self.bbox = tuple(bbox) if bbox is not None else (0, W - 1, 0, H - 1, 0, D - 1)
+        self.march_mode = int(march_mode)
+        self.occupancy = occupancy
+        self.index_offsets = index_offsets
+        self.in_cage = in_cage
+        self.compact_to_dense = compact_to_dense
+        self.sparse = occupancy is not None
+
+    @property
+    def P(self):
+        return 1 + 3 * self.S
+
+    def c(self):
+        g = GridC()
+        g.W, g.H, g.D, g.sh_dim, g.stride = self.W, self.H, self.D, self.S, self.stride
+        g.sparse = int(self.sparse)
+        g.n_vox = self.n_vox
+        g.params = self.params.data_ptr()
+        g.occupancy = None if self.occupancy is None else self.occupancy.data_ptr()
+        g.index_offsets = None if self.index_offsets is None else self.index_offsets.data_ptr()
+        g.in_cage = None if self.in_cage is None else self.in_cage.data_ptr()
+        g.compact_to_dense = (None if self.compact_to_dense is None
+                              else self.compact_to_dense.data_ptr())
+        g.bbox = BBox(*self.bbox)
+        g.step = self.step
+        g.march_mode = self.march_mode
+        return g
+
+
+class Optim:
+    """Gradient and Adam moments, same record layout as the grid."""
+
+    def __init__(self, grid, adaptive_lr=None):
+        import torch
+
+        shape = (grid.n_vox, grid.stride)
+        dev = grid.params.device
+        self.grad = torch.zeros(shape, dtype=torch.float32, device=dev)
+        self.m = torch.zeros(shape, dtype=torch.float32, device=dev)
+        self.v = torch.zeros(shape, dtype=torch.float32, device=dev)
+        self.adaptive_lr = adaptive_lr
+
+    def c(self):
+        o = OptimC()
+        o.grad = self.grad.data_ptr()
+        o.m = self.m.data_ptr()
+        o.v = self.v.data_ptr()
+        o.adaptive_lr = None if self.adaptive_lr is None else self.adaptive_lr.data_ptr()
+        return o
+
+
+# ---------------------------------------------------------- entry points --
+
+
+def camera_from_pose(translation, object_centre, camera_distance, fx, fy, cx, cy,
+                     need_rescale=True, z_reversed=True):
+    """Camera of one pose (Camera::readFromMyBlender per view,
+    CameraUtils.cu:339-377).  Returns (Camera, w2c list of 16)."""
+    t = (ctypes.c_double * 3)(*translation)
+    oc = (ctypes.c_double * 3)(*object_centre)
+    cam = Camera()
+    w2c = (ctypes.c_double * 16)()
+    _check("pxl_camera_from_pose",
+           lib().pxl_camera_from_pose(t, oc, camera_distance, int(need_rescale), int(z_reversed),
+                                      fx, fy, cx, cy, ctypes.byref(cam), w2c))
+    return cam, list(w2c)
+
+
+def rays_init(cam, bbox, img_w, img_h, rays, start_index=0, img=None, stream=None):
+    """Rays of one view (_rays::initialSelf / initialRays, Rays.cu:77-240).
+    img = (r, g, b, mask) uint8 device tensors of img_h*img_w or None."""
+    ims = [None] * 4 if img is None else [_ptr(x) for x in img]
+    rc = rays.c()
+    _check("pxl_rays_init",
+           lib().pxl_rays_init(ctypes.byref(cam), ctypes.byref(BBox(*bbox)), img_w, img_h,
+                               *ims, start_index, ctypes.byref(rc), _stream(stream)))
+
+
+def render_fwd(grid, rays, sample_index, start=0, length=None, trace=None, stream=None):
+    """RenderUtils::renderRays (RenderUtils.cu:1961-1971)."""
+    length = sample_index.numel() - start if length is None else length
+    g, r = grid.c(), rays.c()
+    if trace is None:
+        _check("pxl_render_fwd", lib().pxl_render_fwd(ctypes.byref(g), ctypes.byref(r),
+                                                      _ptr(sample_index), start, length,
+                                                      _stream(stream)))
+    else:
+        _check("pxl_render_fwd_trace",
+               lib().pxl_render_fwd_trace(ctypes.byref(g), ctypes.byref(r), _ptr(sample_index),
+                                          start, length, _ptr(trace), _stream(stream)))
+
+
+def render_bwd_rgb(grid, optim, rays, losses, sample_index, start=0, length=None,
+                   mask_coef=1.0, huber_delta=0.1, stream=None):
+    """RenderUtils::renderRays_withGradients_RGB (RenderUtils.cu:2258-2278)."""
+    length = sample_index.numel() - start if length is None else length
+    g, o, r = grid.c(), optim.c(), rays.c()
+    _check("pxl_render_bwd_rgb",
+           lib().pxl_render_bwd_rgb(ctypes.byref(g), ctypes.byref(o), ctypes.byref(r),
+                                    _ptr(losses), _ptr(sample_index), start, length, mask_coef,
+                                    huber_delta, _stream(stream)))
+
+
+def train_rays(grid, optim, rays, losses, sample_index, start=0, length=None, mask_coef=1.0,
+               huber_delta=0.1, stream=None):
+    """Fused renderRays + renderRays_withGradients_RGB in one launch."""
+    length = sample_index.numel() - start if length is None else length
+    g, o, r = grid.c(), optim.c(), rays.c()
+    _check("pxl_train_rays",
+           lib().pxl_train_rays(ctypes.byref(g), ctypes.byref(o), ctypes.byref(r), _ptr(losses),
+                                _ptr(sample_index), start, length, mask_coef, huber_delta,
+                                _stream(stream)))
+
+
+def add_variance_loss(grid, optim, lambda_density, lambda_sh, tv_type=TV_DISPARITY,
+                      neighbours=6, optimize_step=0, stream=None):
+    """RenderUtils::addVarianceLoss (RenderUtils.cu:2815-2851)."""
+    g, o = grid.c(), optim.c()
+    _check("pxl_add_variance_loss",
+           lib().pxl_add_variance_loss(ctypes.byref(g), ctypes.byref(o), lambda_density,
+                                       lambda_sh, tv_type, neighbours, optimize_step, None, 0,
+                                       _stream(stream)))
+
+
+def adam_step(grid, optim, lr, sh_lr, t, stream=None):
+    """Optimizer::updateRGBScene (Optimizer.cu:451-479)."""
+    g, o = grid.c(), optim.c()
+    _check("pxl_adam_step", lib().pxl_adam_step(ctypes.byref(g), ctypes.byref(o), lr, sh_lr, t,
+                                                _stream(stream)))
+
+
+def build_occupancy(density, W, H, D, threshold, occupancy, in_cage, n_occupied=None,
+                    stream=None):
+    """objectScene::pruneSelf (SceneUtils.cu:883-913)."""
+    _check("pxl_build_occupancy",
+           lib().pxl_build_occupancy(_ptr(density), W, H, D, threshold, _ptr(occupancy),
+                                     _ptr(in_cage), _ptr(n_occupied), _stream(stream)))
+
+
+def sparse_index_scratch_bytes(n_vertices):
+    return lib().pxl_sparse_index_scratch_bytes(n_vertices)
+
+
+def build_sparse_index(in_cage, W, H, D, index_offsets, compact_to_dense, in_cage_nums,
+                       scratch, stream=None):
+    """objectScene::getCagedNums (SceneUtils.cu:1081-1120)."""
+    nbytes = 0 if scratch is None else scratch.numel() * scratch.element_size()
+    _check("pxl_build_sparse_index",
+           lib().pxl_build_sparse_index(_ptr(in_cage), W, H, D, _ptr(index_offsets),
+                                        _ptr(compact_to_dense), _ptr(in_cage_nums),
+                                        _ptr(scratch), nbytes, _stream(stream)))
+
+
+def grid_compact(dense_records, compact_to_dense, n_vox, out, stream=None):
+    _check("pxl_grid_compact",
+           lib().pxl_grid_compact(_ptr(dense_records), dense_records.shape[1],
+                                  _ptr(compact_to_dense), n_vox, _ptr(out), _stream(stream)))
+
+
+def grid_import_planes(planes, records, stream=None):
+    """planes: list of P device float32 tensors [n_vox] (reference plane table)."""
+    arr = (ctypes.c_void_p * len(planes))(*[p.data_ptr() for p in planes])
+    _check("pxl_grid_import_planes",
+           lib().pxl_grid_import_planes(arr, len(planes), records.shape[0], _ptr(records),
+                                        records.shape[1], _stream(stream)))
+
+
+def grid_export_planes(records, planes, stream=None):
+    arr = (ctypes.c_void_p * len(planes))(*[p.data_ptr() for p in planes])
+    _check("pxl_grid_export_planes",
+           lib().pxl_grid_export_planes(_ptr(records), records.shape[1], records.shape[0], arr,
+                                        len(planes), _stream(stream)))
+
+
+def loss_reduce(losses, out, length=None, stream=None):
+    """FP64 MSE and PSNR of a batch (getMSEError, Utils.cu:311-320)."""
+    length = losses.numel() if length is None else length
+    _check("pxl_loss_reduce_f64",
+           lib().pxl_loss_reduce_f64(_ptr(losses), length, _ptr(out), _stream(stream)))
+
+
+def count_touched(grid, rays, sample_index, bitmap, count, start=0, length=None, stream=None):
+    length = sample_index.numel() - start if length is None else length
+    g, r = grid.c(), rays.c()
+    _check("pxl_count_touched",
+           lib().pxl_count_touched(ctypes.byref(g), ctypes.byref(r), _ptr(sample_index), start,
+                                   length, _ptr(bitmap), _ptr(count), _stream(stream)))
+
+
+class Trainer:
+    """Batch loop of Render::objectSceneLearner (Render.cu:352-503), run by
+    the C++ host layer; all buffers stay owned by the Python objects passed
+    in, which must outlive the trainer."""
+
+    def __init__(self, grid, optim, rays, order, n_rays=None, batch_size=65536,
+                 learning_rate=0.3, sh_learning_rate=1.0, variance_coefficient=1e-5,
+                 sh_coefficient=1e-6, variance_type=TV_DISPARITY, neighbour_nums=6,
+                 variance_interval=3, optimize_step=0, mask_coefficient=1.0, huber_delta=0.1,
+                 fused=True, stream=None):
+        import torch
+
+        self._keep = (grid, optim, rays, order)
+        dev = grid.params.device
+        self.losses = torch.zeros(batch_size, dtype=torch.float64, device=dev)
+        self.loss_out = torch.zeros(2, dtype=torch.float64, device=dev)
+        cfg = TrainConfigC(batch_size, learning_rate, sh_learning_rate, variance_coefficient,
+                           sh_coefficient, variance_type, neighbour_nums, variance_interval,
+                           optimize_step, mask_coefficient, huber_delta, int(fused))
+        self._g, self._o, self._r = grid.c(), optim.c(), rays.c()
+        n_rays = order.numel() if n_rays is None else n_rays
+        h = lib().pxl_trainer_create(ctypes.byref(cfg), ctypes.byref(self._g),
+                                     ctypes.byref(self._o), ctypes.byref(self._r), _ptr(order),
+                                     n_rays, _ptr(self.losses), _ptr(self.loss_out),
+                                     _stream(stream))
+        if not h:
+            raise PxlError("pxl_trainer_create", PXL_EINVAL, lib().pxl_last_error().decode())
+        self._h = ctypes.c_void_p(h)
+
+    def step(self):
+        _check("pxl_trainer_step", lib().pxl_trainer_step(self._h))
+
+    def time_render(self, max_steps):
+        _check("pxl_trainer_time_render", lib().pxl_trainer_time_render(self._h, max_steps))
+
+    def render_ms(self):
+        avg, n = ctypes.c_double(), ctypes.c_int64()
+        _check("pxl_trainer_render_ms",
+               lib().pxl_trainer_render_ms(self._h, ctypes.byref(avg), ctypes.byref(n)))
+        return avg.value, n.value
+
+    def loss(self):
+        mse, psnr = ctypes.c_double(), ctypes.c_double()
+        _check("pxl_trainer_loss", lib().pxl_trainer_loss(self._h, ctypes.byref(mse),
+                                                          ctypes.byref(psnr)))
+        return mse.value, psnr.value
+
+    def state(self):
+        t, lr, sl = ctypes.c_int64(), ctypes.c_double(), ctypes.c_double()
+        _check("pxl_trainer_state", lib().pxl_trainer_state(self._h, ctypes.byref(t),
+                                                            ctypes.byref(lr), ctypes.byref(sl)))
+        return t.value, lr.value, sl.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().pxl_trainer_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,310 @@
+/*
+ * pxl.h - C ABI of the MI355X-native Plenoxel train-step library (libpxl.so).
+ *
+ * One train step of the reference trainer is four calls in
+ * Render::objectSceneLearner (SparseVolumeRadianceField/Render.cu:407, 430,
+ * 471, 492).  Each has an entry point here with the same argument meaning;
+ * the reference's structs-of-device-pointers (Rays, ObjectScene, Optimizer)
+ * become the POD descriptors below.  Every buffer is caller-owned device
+ * memory; no entry point allocates.  Every entry point is asynchronous on the
+ * HIP stream passed as `stream` (NULL = default stream) and returns a status
+ * code instead of exiting (the reference prints and calls exit(-1) in
+ * checkGPUStatus, Utils.cu:700-717).
+ *
+ * Grid layout in HBM: one record of `stride` floats per stored vertex,
+ * record[0] = raw density, record[1 + ch*S + k] = SH coefficient k of colour
+ * channel ch (R, G, B).  This is the reference's plane order
+ * (SceneUtils.cu:148-171) packed per vertex; pxl_grid_import_planes /
+ * pxl_grid_export_planes convert from / to the reference's planes.
+ */
+#ifndef PXL_H
+#define PXL_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. */
+enum {
+    PXL_OK = 0,
+    PXL_EINVAL = 1,   /* bad argument (null pointer, size, unsupported S) */
+    PXL_ENOMEM = 2,   /* caller-provided scratch too small                */
+    PXL_EHIP = 3,     /* HIP runtime error (launch failure)              */
+    PXL_ENAN = 4      /* non-finite batch loss (Render.cu:450-457)       */
+};
+
+/* Marching rule. */
+enum {
+    /* t_k = t_near + k * step; the rule the benchmark configurations name. */
+    PXL_MARCH_FIXED = 0,
+    /* t += |step - w| after every evaluated sample, t += step after a
+     * skipped one (RenderUtils.cu:1681-1705); evaluated in FP64. */
+    PXL_MARCH_REF_ADAPTIVE = 1
+};
+
+/* Variance-loss (TV) types of RenderUtils::addVarianceLoss
+ * (RenderUtils.cu:2815-2851). */
+enum {
+    PXL_TV_NEIGHBOUR = 0,   /* totalVarianceLoss, 6-neighbour branch      */
+    PXL_TV_DISPARITY = 1    /* addVarianceLossGPU, per-cell mean disparity */
+};
+
+/* Pinhole camera, host memory.  Mirrors the fields of Camera that
+ * initialRays reads (CameraUtils.cuh:4-84): c2wExternalMat, cameraCenter,
+ * alpha and principle. */
+typedef struct pxl_camera {
+    double c2w[16];     /* row-major camera-to-world 4x4 */
+    double centre[4];   /* camera centre; centre[3] = 1  */
+    double fx, fy;      /* focal lengths in pixels       */
+    double cx, cy;      /* principal point in pixels     */
+} pxl_camera;
+
+/* Integer bounding box in voxel units (objectScene::bbox_*,
+ * SceneUtils.cuh:13-15). */
+typedef struct pxl_bbox {
+    int32_t x_min, x_max, y_min, y_max, z_min, z_max;
+} pxl_bbox;
+
+/* Ray set, structure of arrays as in Rays (Rays.cuh:7-56).  Geometry and
+ * ground truth are written by pxl_rays_init; the five acc_* planes are the
+ * forward outputs. */
+typedef struct pxl_rays {
+    float *ox, *oy, *oz;          /* origin (cameraCenter*_Device)      */
+    float *dx, *dy, *dz;          /* unit direction                     */
+    float *near_t, *far_t;        /* slab-test interval                 */
+    uint8_t *gt_r, *gt_g, *gt_b;  /* ground-truth colour, 0..255        */
+    uint8_t *gt_mask;             /* ground-truth mask, 0..255          */
+    float *acc_r, *acc_g, *acc_b; /* rendered colour                    */
+    float *acc_w;                 /* accumulated weights (opacity)      */
+    float *acc_density;           /* sum of activated density samples   */
+    int64_t n;                    /* number of rays in every plane      */
+    int32_t white_bg;             /* composite onto white (WhiteBG:1)   */
+    int32_t reserved;
+} pxl_rays;
+
+/* Voxel grid.  Dense: n_vox = W*H*D records in z-y-x order.  Sparse (the
+ * reference's caged layout, SceneUtils.cu:516-640): n_vox = inCageNums
+ * records, `index_offsets` maps a dense vertex to its record,
+ * `occupancy` is one bit per cell, `in_cage` one byte per dense vertex and
+ * `compact_to_dense` maps a record back to its dense vertex. */
+typedef struct pxl_grid {
+    int32_t W, H, D;              /* vertices per axis                 */
+    int32_t sh_dim;               /* S = (band+1)^2: 1, 4, 9 or 16     */
+    int32_t stride;               /* floats per record, multiple of 4, >= 1+3S */
+    int32_t sparse;               /* 0 dense, 1 caged                  */
+    int64_t n_vox;                /* number of records                 */
+    float *params;                /* [n_vox][stride]                   */
+    const uint32_t *occupancy;    /* bit c = cell z(W-1)(H-1)+y(W-1)+x */
+    const uint32_t *index_offsets;/* [W*H*D]                           */
+    const uint8_t *in_cage;       /* [W*H*D]                           */
+    const uint32_t *compact_to_dense; /* [n_vox]                       */
+    pxl_bbox bbox;
+    float step;                   /* marching step in voxels (StepSize) */
+    int32_t march_mode;           /* PXL_MARCH_*                       */
+} pxl_grid;
+
+/* Optimiser state, same record layout as the grid (Optimizer.cuh:5-37). */
+typedef struct pxl_optim {
+    float *grad;                  /* [n_vox][stride] gradient          */
+    float *m;                     /* [n_vox][stride] Adam first moment */
+    float *v;                     /* [n_vox][stride] Adam second moment*/
+    const float *adaptive_lr;     /* dense only: per-vertex LR factor or NULL */
+} pxl_optim;
+
+/* Last error message of the calling thread ("" if none). */
+const char *pxl_last_error(void);
+
+/* Library version, (major << 16) | minor. */
+int32_t pxl_version(void);
+
+/* Host-side camera of one pose (readFromMyBlender per view,
+ * CameraUtils.cu:339-377; formulateProjectionMat :45-106; centre as in
+ * transferSelf2GPU :577-608).  The pose translation is rescaled into voxel
+ * units (t * object_centre / camera_distance + object_centre) when
+ * need_rescale is set and |t| < 10; the rotation looks at object_centre
+ * with +z up (setCameraLookAt, Utils.cu:448-486) and is flipped 180 degrees
+ * about x when z_reversed.  w2c_out (optional) receives the 4x4
+ * world-to-camera matrix.  No GPU work. */
+int32_t pxl_camera_from_pose(const double translation[3], const double object_centre[3],
+                             double camera_distance, int32_t need_rescale,
+                             int32_t z_reversed, double fx, double fy, double cx, double cy,
+                             pxl_camera *out, double w2c_out[16]);
+
+/* Rays of one view (replaces the per-view initialRays launch in
+ * _rays::initialSelf, Rays.cu:77-144 and 187-240; getBound Rays.cu:22-75).
+ * Pixel (row, col) of the img_w x img_h image becomes ray
+ * start_index + row*img_w + col.  img_* are device planes of img_w*img_h
+ * bytes; all four may be NULL (ground truth then 0). */
+int32_t pxl_rays_init(const pxl_camera *cam, const pxl_bbox *bbox,
+                      int32_t img_w, int32_t img_h,
+                      const uint8_t *img_r, const uint8_t *img_g,
+                      const uint8_t *img_b, const uint8_t *img_mask,
+                      int64_t start_index, const pxl_rays *rays, void *stream);
+
+/* Forward render of rays sample_index[start .. start+len)
+ * (RenderUtils::renderRays, RenderUtils.cu:1919-1971). */
+int32_t pxl_render_fwd(const pxl_grid *grid, const pxl_rays *rays,
+                       const int32_t *sample_index, int64_t start,
+                       int64_t len, void *stream);
+
+/* Same as pxl_render_fwd and also writes, per batch position i, trace[2i] =
+ * number of evaluated samples and trace[2i+1] = a 32-bit FNV-1a hash of the
+ * (sample number, base cell x, y, z) sequence.  For index-exactness tests. */
+int32_t pxl_render_fwd_trace(const pxl_grid *grid, const pxl_rays *rays,
+                             const int32_t *sample_index, int64_t start,
+                             int64_t len, uint32_t *trace, void *stream);
+
+/* Loss, Huber clip, mask/logit term and gradient scatter of a batch whose
+ * forward outputs are in rays->acc_* (RenderUtils::renderRays_withGradients_RGB,
+ * RenderUtils.cu:2051-2278).  losses[i] is the per-ray L1 loss x 255/3 of
+ * batch position i. Gradients are added into optim->grad. */
+int32_t pxl_render_bwd_rgb(const pxl_grid *grid, const pxl_optim *optim,
+                           const pxl_rays *rays, double *losses,
+                           const int32_t *sample_index, int64_t start,
+                           int64_t len, double mask_coef, float huber_delta,
+                           void *stream);
+
+/* Fused forward + backward of one batch: one launch, forward results kept
+ * in FP64 registers between the two marches.  Writes the same outputs as
+ * pxl_render_fwd followed by pxl_render_bwd_rgb. */
+int32_t pxl_train_rays(const pxl_grid *grid, const pxl_optim *optim,
+                       const pxl_rays *rays, double *losses,
+                       const int32_t *sample_index, int64_t start,
+                       int64_t len, double mask_coef, float huber_delta,
+                       void *stream);
+
+/* Variance (TV) loss gradient added into optim->grad
+ * (RenderUtils::addVarianceLoss, RenderUtils.cu:2815-2851).
+ * type PXL_TV_DISPARITY: addVarianceLossGPU (RenderUtils.cu:2469-2547),
+ *   computed race-free as a per-vertex gather; SH terms only when
+ *   optimize_step <= 1 (RenderUtils.cu:2493).
+ * type PXL_TV_NEIGHBOUR: totalVarianceLoss 6-neighbour branch
+ *   (RenderUtils.cu:2660-2763); neighbours must be 6.
+ * scratch: caller-provided device memory of pxl_tv_scratch_bytes() bytes
+ *   (may be NULL when that is 0). */
+size_t pxl_tv_scratch_bytes(const pxl_grid *grid);
+int32_t pxl_add_variance_loss(const pxl_grid *grid, const pxl_optim *optim,
+                              double lambda_density, double lambda_sh,
+                              int32_t type, int32_t neighbours,
+                              int32_t optimize_step, void *scratch,
+                              size_t scratch_bytes, void *stream);
+
+/* Adam step over every record, then gradient := 0
+ * (Optimizer::updateRGBScene / updateRGBSceneGPU_ADAM, Optimizer.cu:359-479).
+ * t is the 1-based time step; density is clamped at 0. */
+int32_t pxl_adam_step(const pxl_grid *grid, const pxl_optim *optim,
+                      double lr, double sh_lr, int32_t t, void *stream);
+
+/* Cell occupancy and in-cage flags from a dense raw-density plane
+ * (objectScene::pruneSelf / pruneScene / occupiedVoxel,
+ * SceneUtils.cu:800-913).  occupancy: ceil((W-1)(H-1)(D-1)/32) words.
+ * in_cage: W*H*D bytes, cleared by this call.  n_occupied: optional device
+ * uint64 receiving the number of occupied cells. */
+int32_t pxl_build_occupancy(const float *density, int32_t W, int32_t H,
+                            int32_t D, float threshold, uint32_t *occupancy,
+                            uint8_t *in_cage, uint64_t *n_occupied,
+                            void *stream);
+
+/* Sparse index: index_offsets[v] = number of in-cage vertices before v if
+ * v is in cage, else 0; *in_cage_nums = total (objectScene::getCagedNums,
+ * SceneUtils.cu:1081-1120).  compact_to_dense (optional) receives the
+ * inverse map.  in_cage must be 16-byte aligned. */
+size_t pxl_sparse_index_scratch_bytes(int64_t n_vertices);
+int32_t pxl_build_sparse_index(const uint8_t *in_cage, int32_t W, int32_t H,
+                               int32_t D, uint32_t *index_offsets,
+                               uint32_t *compact_to_dense,
+                               uint32_t *in_cage_nums, void *scratch,
+                               size_t scratch_bytes, void *stream);
+
+/* Compaction of dense records into caged order (the host loop in
+ * objectScene::initSelfByUpsample, SceneUtils.cu:583-635). */
+int32_t pxl_grid_compact(const float *dense_records, int32_t stride,
+                         const uint32_t *compact_to_dense, int64_t n_vox,
+                         float *sparse_records, void *stream);
+
+/* Reference plane layout <-> packed records.  `planes` is a host array of
+ * n_planes device pointers to n_vox floats each (objectScene plane table,
+ * SceneUtils.cu:148-171).  Import zero-fills the padding floats. */
+int32_t pxl_grid_import_planes(const float *const *planes, int32_t n_planes,
+                               int64_t n_vox, float *records, int32_t stride,
+                               void *stream);
+int32_t pxl_grid_export_planes(const float *records, int32_t stride,
+                               int64_t n_vox, float *const *planes,
+                               int32_t n_planes, void *stream);
+
+/* FP64 batch loss on the GPU: out[0] = mean(losses^2) (getMSEError,
+ * Utils.cu:311-320), out[1] = 10 log10(65025 / out[0]) (Render.cu:466). */
+int32_t pxl_loss_reduce_f64(const double *losses, int64_t len, double *out,
+                            void *stream);
+
+/* Number of distinct records touched by the gathers of a batch (untimed
+ * bookkeeping for the roofline's algorithmic bytes).  bitmap: caller
+ * scratch of ceil(n_vox/32) words, cleared by this call; *count: device
+ * uint64. */
+int32_t pxl_count_touched(const pxl_grid *grid, const pxl_rays *rays,
+                          const int32_t *sample_index, int64_t start,
+                          int64_t len, uint32_t *bitmap, uint64_t *count,
+                          void *stream);
+
+/* ------------------------------------------------------------------ */
+/* Trainer: the batch loop of Render::objectSceneLearner                 */
+/* (Render.cu:352-503) as a host object driving the entry points above.  */
+/* ------------------------------------------------------------------ */
+
+/* The config_linux.txt keys the batch loop reads (BatchSize, LearningRate,
+ * SHLearningRate, VarianceCoefficient, ColorDisparityCoefficient,
+ * VarianceType, NeighboorNumsForTVLoss, VarianceLossInterval,
+ * MaskRayCoefficient, HuberDelta; Utils.cu:765-932). */
+typedef struct pxl_train_config {
+    int64_t batch_size;
+    double learning_rate;
+    double sh_learning_rate;
+    double variance_coefficient;
+    double sh_coefficient;
+    int32_t variance_type;
+    int32_t neighbour_nums;
+    int32_t variance_interval;
+    int32_t optimize_step;   /* current hierarchy level (Render.cu:264) */
+    double mask_coefficient;
+    float huber_delta;
+    int32_t fused;           /* 1: fused forward+backward launch per batch */
+} pxl_train_config;
+
+typedef struct pxl_trainer pxl_trainer;
+
+/* Trainer over caller-owned buffers: rays order[0 .. n_rays) are visited in
+ * batches of batch_size; losses holds batch_size doubles, loss_out 2.
+ * Returns NULL on a bad argument (see pxl_last_error). */
+pxl_trainer *pxl_trainer_create(const pxl_train_config *cfg, const pxl_grid *grid,
+                                const pxl_optim *optim, const pxl_rays *rays,
+                                const int32_t *order, int64_t n_rays, double *losses,
+                                double *loss_out, void *stream);
+
+/* One batch: render + backward (fused or the two reference launches), FP64
+ * batch loss, variance loss (every batch when dense, every
+ * variance_interval-th when sparse, Render.cu:469), Adam with time step
+ * epoch*batchCount + batch + 1, LR decay lr /= 2^(1/(batchCount (epoch+1)))
+ * (Render.cu:499-500).  Asynchronous. */
+int32_t pxl_trainer_step(pxl_trainer *tr);
+
+/* Record HIP events around the render launch of the next max_steps
+ * batches; pxl_trainer_render_ms returns their average duration. */
+int32_t pxl_trainer_time_render(pxl_trainer *tr, int32_t max_steps);
+int32_t pxl_trainer_render_ms(pxl_trainer *tr, double *avg_ms, int64_t *count);
+
+/* Last batch's FP64 MSE and PSNR (synchronises); PXL_ENAN if not finite. */
+int32_t pxl_trainer_loss(pxl_trainer *tr, double *mse, double *psnr);
+
+/* Steps taken and current learning rates. */
+int32_t pxl_trainer_state(const pxl_trainer *tr, int64_t *t, double *lr, double *sh_lr);
+
+void pxl_trainer_destroy(pxl_trainer *tr);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PXL_H */

@@ -1,0 +1,263 @@
+"""CPU oracle of the reference's train-step maths (ctypes over liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker and never as the thing
+measured or shipped.  The product path (libpxl.so) does not use it.
+
+Arrays follow the reference's layout: P = 1 + 3S parameter planes of n_vox
+values, plane-major (SparseVolumeRadianceField/SceneUtils.cu:148-171).
+
+Parity status of each function (how it is pinned; see tests/test_oracle.py):
+  camera_from_pose  - Poses.zip lego centres and w2c rotations, closed forms
+  rays_init         - closed forms (optical axis, slab distances), invariants
+  build_occupancy   - brute force from the textual rule on tiny grids
+  build_index       - brute force / prefix-sum identity
+  render_fwd        - closed forms (constant field, slab), testRender inputs
+  render_bwd        - central finite differences of a differentiable restatement
+  tv_disparity      - closed forms (constant / affine field, single spike)
+  tv_neighbour6     - closed forms (constant field, single spike)
+  adam              - closed form at t = 1 and zero-gradient decay
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+
+def build():
+    """Compile liboracle.so with the committed Makefile."""
+    import subprocess
+
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(_LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+class OrcGrid(ctypes.Structure):
+    _fields_ = [
+        ("W", ctypes.c_int32), ("H", ctypes.c_int32), ("D", ctypes.c_int32),
+        ("S", ctypes.c_int32), ("sparse", ctypes.c_int32), ("n_vox", ctypes.c_int64),
+        ("planes", ctypes.c_void_p), ("occupancy", ctypes.c_void_p),
+        ("index_offsets", ctypes.c_void_p), ("bbox", ctypes.c_int32 * 6),
+        ("step", ctypes.c_float), ("adaptive", ctypes.c_int32),
+    ]
+
+
+class OrcRays(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in
+                ("ox", "oy", "oz", "dx", "dy", "dz", "near_t", "far_t",
+                 "gt_r", "gt_g", "gt_b", "gt_mask")] + [("white_bg", ctypes.c_int32)]
+
+
+_P = ctypes.c_void_p
+_I32, _I64, _F32, _F64 = ctypes.c_int32, ctypes.c_int64, ctypes.c_float, ctypes.c_double
+
+
+def _declare(L):
+    L.orc_camera_from_pose.argtypes = [_P, _P, _F64, _I32, _I32, _F64, _F64, _F64, _F64, _P, _P, _P]
+    L.orc_camera_from_pose.restype = None
+    L.orc_rays_init.argtypes = [_P, _P, _F64, _F64, _F64, _F64, _P, _I32, _I32, _P, _P, _P, _P,
+                                _I32] + [_P] * 12
+    L.orc_rays_init.restype = None
+    L.orc_build_occupancy.argtypes = [_P, _I32, _I32, _I32, _F32, _P, _P]
+    L.orc_build_occupancy.restype = _I64
+    L.orc_build_index.argtypes = [_P, _I32, _I32, _I32, _P]
+    L.orc_build_index.restype = _I64
+    L.orc_render_fwd.argtypes = [ctypes.POINTER(OrcGrid), ctypes.POINTER(OrcRays), _P, _I64, _I64,
+                                 _P, _P, _P, _P, _P, _P]
+    L.orc_render_fwd.restype = None
+    L.orc_render_bwd.argtypes = [ctypes.POINTER(OrcGrid), ctypes.POINTER(OrcRays), _P, _I64, _I64,
+                                 _P, _P, _P, _P, _F64, _F32, _P, _P]
+    L.orc_render_bwd.restype = None
+    L.orc_tv_disparity.argtypes = [ctypes.POINTER(OrcGrid), _F64, _F64, _I32, _P]
+    L.orc_tv_disparity.restype = None
+    L.orc_tv_neighbour6.argtypes = [ctypes.POINTER(OrcGrid), _F64, _F64, _P]
+    L.orc_tv_neighbour6.restype = None
+    L.orc_adam.argtypes = [_P, _P, _P, _P, _P, _I32, _I64, _F64, _F64, _I32]
+    L.orc_adam.restype = None
+    L.orc_mse.argtypes = [_P, _I64]
+    L.orc_mse.restype = _F64
+    L.orc_now.argtypes = []
+    L.orc_now.restype = _F64
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _c(a, dtype):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+def camera_from_pose(translation, object_centre, camera_distance, fx, fy, cx, cy,
+                     need_rescale=True, z_reversed=True):
+    """Returns (c2w[4,4], w2c[4,4], centre[4]) of one pose."""
+    t = _c(translation, np.float64)
+    oc = _c(object_centre, np.float64)
+    c2w = np.zeros(16, np.float64)
+    w2c = np.zeros(16, np.float64)
+    ctr = np.zeros(4, np.float64)
+    lib().orc_camera_from_pose(_p(t), _p(oc), camera_distance, int(need_rescale),
+                               int(z_reversed), fx, fy, cx, cy, _p(c2w), _p(w2c), _p(ctr))
+    return c2w.reshape(4, 4), w2c.reshape(4, 4), ctr
+
+
+def rays_init(c2w, centre, fx, fy, cx, cy, bbox, img_w, img_h, img=None, white_bg=True):
+    """Rays of one view; img = (r, g, b, mask) uint8 arrays of img_h*img_w or None."""
+    n = img_w * img_h
+    c2w = _c(c2w, np.float64).reshape(16)
+    centre = _c(centre, np.float64)
+    bb = _c(bbox, np.int32)
+    out = {k: np.zeros(n, np.float32) for k in ("ox", "oy", "oz", "dx", "dy", "dz", "near_t", "far_t")}
+    out.update({k: np.zeros(n, np.uint8) for k in ("gt_r", "gt_g", "gt_b", "gt_mask")})
+    ims = [None] * 4 if img is None else [_c(x, np.uint8).reshape(n) for x in img]
+    lib().orc_rays_init(_p(c2w), _p(centre), fx, fy, cx, cy, _p(bb), img_w, img_h,
+                        *[_p(x) for x in ims], int(white_bg),
+                        *[_p(out[k]) for k in ("ox", "oy", "oz", "dx", "dy", "dz", "near_t",
+                                               "far_t", "gt_r", "gt_g", "gt_b", "gt_mask")])
+    return out
+
+
+def build_occupancy(density, W, H, D, threshold):
+    """Returns (occupancy bytes per cell, in_cage bytes per vertex, n_occupied)."""
+    d = _c(density, np.float32).reshape(-1)
+    occ = np.zeros((W - 1) * (H - 1) * (D - 1), np.uint8)
+    cage = np.zeros(W * H * D, np.uint8)
+    n = lib().orc_build_occupancy(_p(d), W, H, D, float(threshold), _p(occ), _p(cage))
+    return occ, cage, n
+
+
+def build_index(in_cage, W, H, D):
+    """Returns (index_offsets uint32[W*H*D], inCageNums)."""
+    c = _c(in_cage, np.uint8).reshape(-1)
+    io = np.zeros(W * H * D, np.uint32)
+    n = lib().orc_build_index(_p(c), W, H, D, _p(io))
+    return io, n
+
+
+class Grid:
+    """Grid for the oracle: planes float32[P, n_vox] (reference layout)."""
+
+    def __init__(self, W, H, D, S, planes, step, bbox=None, occupancy=None,
+                 index_offsets=None, adaptive=False):
+        self.W, self.H, self.D, self.S = W, H, D, S
+        self.planes = _c(planes, np.float32).reshape(1 + 3 * S, -1)
+        self.n_vox = self.planes.shape[1]
+        self.sparse = occupancy is not None
+        self.occupancy = None if occupancy is None else _c(occupancy, np.uint8)
+        self.index_offsets = None if index_offsets is None else _c(index_offsets, np.uint32)
+        self.step = step
+        self.bbox = bbox if bbox is not None else (0, W - 1, 0, H - 1, 0, D - 1)
+        self.adaptive = adaptive
+
+    def c(self):
+        g = OrcGrid()
+        g.W, g.H, g.D, g.S = self.W, self.H, self.D, self.S
+        g.sparse = int(self.sparse)
+        g.n_vox = self.n_vox
+        g.planes = _p(self.planes)
+        g.occupancy = _p(self.occupancy)
+        g.index_offsets = _p(self.index_offsets)
+        g.bbox = (ctypes.c_int32 * 6)(*self.bbox)
+        g.step = self.step
+        g.adaptive = int(self.adaptive)
+        return g
+
+
+class Rays:
+    """Ray set for the oracle: dict of numpy arrays as produced by rays_init."""
+
+    def __init__(self, d, white_bg=True):
+        self.d = {k: _c(v, np.uint8 if k.startswith("gt_") else np.float32) for k, v in d.items()}
+        self.white_bg = white_bg
+        self.n = len(self.d["ox"])
+
+    def c(self):
+        r = OrcRays()
+        for k in ("ox", "oy", "oz", "dx", "dy", "dz", "near_t", "far_t", "gt_r", "gt_g", "gt_b",
+                  "gt_mask"):
+            setattr(r, k, _p(self.d[k]))
+        r.white_bg = int(self.white_bg)
+        return r
+
+
+def render_fwd(grid, rays, sample_index, start=0, length=None, trace=False):
+    """Returns dict r, g, b, w, density (float64 per ray) and optional trace[len, 2]."""
+    si = _c(sample_index, np.int32)
+    if length is None:
+        length = len(si) - start
+    out = {k: np.zeros(rays.n, np.float64) for k in ("r", "g", "b", "w", "density")}
+    tr = np.zeros((length, 2), np.uint32) if trace else None
+    g, r = grid.c(), rays.c()
+    lib().orc_render_fwd(ctypes.byref(g), ctypes.byref(r), _p(si), start, length,
+                         _p(out["r"]), _p(out["g"]), _p(out["b"]), _p(out["w"]),
+                         _p(out["density"]), _p(tr))
+    if trace:
+        out["trace"] = tr
+    return out
+
+
+def render_bwd(grid, rays, sample_index, fwd, start=0, length=None, mask_coef=1.0,
+               huber_delta=0.1, grad=None):
+    """Returns (losses float64[len], grad float64[P, n_vox])."""
+    si = _c(sample_index, np.int32)
+    if length is None:
+        length = len(si) - start
+    P = 1 + 3 * grid.S
+    if grad is None:
+        grad = np.zeros((P, grid.n_vox), np.float64)
+    losses = np.zeros(length, np.float64)
+    f = {k: _c(fwd[k], np.float64) for k in ("r", "g", "b", "w")}
+    g, r = grid.c(), rays.c()
+    lib().orc_render_bwd(ctypes.byref(g), ctypes.byref(r), _p(si), start, length,
+                         _p(f["r"]), _p(f["g"]), _p(f["b"]), _p(f["w"]), mask_coef,
+                         huber_delta, _p(losses), _p(grad))
+    return losses, grad
+
+
+def tv_disparity(grid, lambda_density, lambda_sh, optimize_step=0, grad=None):
+    P = 1 + 3 * grid.S
+    if grad is None:
+        grad = np.zeros((P, grid.n_vox), np.float64)
+    g = grid.c()
+    lib().orc_tv_disparity(ctypes.byref(g), lambda_density, lambda_sh, optimize_step, _p(grad))
+    return grad
+
+
+def tv_neighbour6(grid, lambda_density, lambda_sh, grad=None):
+    P = 1 + 3 * grid.S
+    if grad is None:
+        grad = np.zeros((P, grid.n_vox), np.float64)
+    g = grid.c()
+    lib().orc_tv_neighbour6(ctypes.byref(g), lambda_density, lambda_sh, _p(grad))
+    return grad
+
+
+def adam(params, grad, m, v, lr, sh_lr, t, adaptive_lr=None):
+    """In-place Adam on float32 planes [P, n] (params, grad, m, v)."""
+    for a in (params, grad, m, v):
+        assert a.dtype == np.float32 and a.flags.c_contiguous
+    P, n = params.shape
+    alr = None if adaptive_lr is None else _c(adaptive_lr, np.float32)
+    lib().orc_adam(_p(params), _p(grad), _p(m), _p(v), _p(alr), P, n, lr, sh_lr, t)
+
+
+def mse(losses):
+    a = _c(losses, np.float64)
+    return lib().orc_mse(_p(a), len(a))
+
+
+def now():
+    return lib().orc_now()

@@ -1,0 +1,102 @@
+"""Shared scene builders for the parity tests (numpy on the host, torch for
+device memory)."""
+import numpy as np
+
+import oracle
+import pxl
+from pxl import scenes
+
+
+def c1_planes(N=128, S=9, n_blobs=16, seed=1):
+    """Dense grid of the small plain-method workload: density = sum of seeded
+    anisotropic Gaussian blobs, SH ~ N(0, 0.3).  float32 planes [P, N^3]."""
+    P = 1 + 3 * S
+    planes = np.empty((P, N ** 3), np.float32)
+    planes[0] = scenes.blob_density(N, n_blobs, seed).reshape(-1)
+    planes[1:] = scenes.sh_planes(N, S, seed + 100)
+    return planes
+
+
+def cameras(n_views, img, radius, fov, N, upper_only=False):
+    """Product cameras (pxl.camera_from_pose) and the oracle's for the same poses."""
+    f, _, c, _ = scenes.view_intrinsics(img, fov)
+    oc = [N / 2.0] * 3
+    out = []
+    for t in scenes.fibonacci_sphere(n_views, radius, upper_only):
+        cam, _ = pxl.camera_from_pose(t, oc, 1.0, f, f, c, c)
+        c2w, _, ctr = oracle.camera_from_pose(t, oc, 1.0, f, f, c, c)
+        out.append((cam, c2w, ctr, f, c))
+    return out
+
+
+def oracle_rays(cams, img, bbox, seed=0, white_bg=True):
+    """All rays of the views (oracle) with random seeded ground truth."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    for (_, c2w, ctr, f, c) in cams:
+        ims = tuple(rng.integers(0, 256, img * img, dtype=np.uint8) for _ in range(4))
+        r = oracle.rays_init(c2w, ctr, f, f, c, c, bbox, img, img, img=ims, white_bg=white_bg)
+        r["_img"] = ims
+        parts.append(r)
+    keys = [k for k in parts[0] if not k.startswith("_")]
+    allr = {k: np.concatenate([p[k] for p in parts]) for k in keys}
+    return allr, [p["_img"] for p in parts]
+
+
+def to_records(planes, dev, stride=None):
+    import torch
+
+    P, n = planes.shape
+    S = (P - 1) // 3
+    stride = stride or pxl.record_stride(S)
+    rec = torch.zeros((n, stride), dtype=torch.float32, device=dev)
+    rec[:, :P] = torch.from_numpy(np.ascontiguousarray(planes.T)).to(dev)
+    return rec
+
+
+def from_records(rec, P):
+    return rec[:, :P].T.contiguous().cpu().numpy().astype(np.float64)
+
+
+def sparse_from_dense(planes, N, tau, dev):
+    """Occupancy, index and compact planes built with the oracle; returns
+    (oracle Grid kwargs, device arrays)."""
+    occ, cage, _ = oracle.build_occupancy(planes[0], N, N, N, tau)
+    io, n = oracle.build_index(cage, N, N, N)
+    c2d = np.flatnonzero(cage).astype(np.uint32)
+    compact = np.ascontiguousarray(planes[:, c2d])
+    return occ, cage, io, n, c2d, compact
+
+
+def pack_bits(occ_bytes):
+    """Cell bytes -> little-endian 32-bit words (bit c of word c // 32)."""
+    n = len(occ_bytes)
+    pad = (-n) % 32
+    b = np.concatenate([occ_bytes.astype(np.uint8) != 0, np.zeros(pad, bool)])
+    return np.packbits(b.reshape(-1, 8), axis=1, bitorder="little").reshape(-1).view(np.uint32)
+
+
+def device_sparse_grid(N, S, compact, occ, io, cage, c2d, dev, step=0.5, march=0):
+    import torch
+
+    rec = to_records(compact, dev)
+    return pxl.Grid(N, N, N, S, rec, step=step, march_mode=march,
+                    occupancy=torch.from_numpy(pack_bits(occ).view(np.int32)).to(dev),
+                    index_offsets=torch.from_numpy(io.view(np.int32)).to(dev),
+                    in_cage=torch.from_numpy(cage).to(dev),
+                    compact_to_dense=torch.from_numpy(c2d.view(np.int32)).to(dev))
+
+
+def grad_close(g, ref, rel=1e-4, floor=1e-3):
+    """Gradient parity bar: per plane p, |g - ref| <= rel * (|ref| +
+    floor * max|ref[p]|) for every element.  Returns the worst ratio."""
+    worst = 0.0
+    for p in range(ref.shape[0]):
+        scale = np.abs(ref[p]).max()
+        if scale == 0:
+            assert np.abs(g[p]).max() == 0, p
+            continue
+        tol = rel * (np.abs(ref[p]) + floor * scale)
+        ratio = (np.abs(g[p] - ref[p]) / tol).max()
+        worst = max(worst, ratio)
+    return worst
