@@ -98,8 +98,9 @@ class Workload:
         gt_rec = torch.zeros((n_vox, stride), dtype=torch.float32, device=dev)
         pxl.grid_compact(dense_rec, c2d, n_vox, gt_rec)
         del dense_rec, dens
+        bricks = pxl.build_brick_occupancy(occ, N, N, N)
         grid_kw = dict(step=0.5, occupancy=occ, index_offsets=io, in_cage=cage,
-                       compact_to_dense=c2d)
+                       compact_to_dense=c2d, brick_occupancy=bricks)
         self.gt_grid = pxl.Grid(N, N, N, S, gt_rec, **grid_kw)
         log(f"grid {N}^3: tau={tau:.3f}, occupied cells {self.occ_frac:.4f}, "
             f"caged vertices {n_vox} ({time.time() - t0:.1f}s)")

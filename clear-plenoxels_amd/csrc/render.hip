@@ -33,6 +33,8 @@ struct RenderArgs {
     const float *params;
     int64_t n_vox;
     const uint32_t *occ;
+    const uint32_t *bricks;
+    int nbx, nby;
     const uint32_t *io;
     int W, H, D, stride;
     float step;
@@ -113,12 +115,22 @@ __device__ __forceinline__ float sh_basis(int k, float x, float y, float z) {
 }
 
 // One sample position: base cell, corner records and trilinear weights.
+// The cell part (bx, by, bz, occ, v) is kept across samples: consecutive
+// samples that stay in one cell reuse its occupancy test and the 8
+// corner-record lookups.
 template <typename R>
 struct Sample {
     int bx, by, bz;
+    bool occ;
     uint32_t v[8];
     R w[8];
 };
+
+template <typename R>
+__device__ __forceinline__ void sample_reset(Sample<R> &s) {
+    s.bx = s.by = s.bz = -2;
+    s.occ = false;
+}
 
 // Base cell of a position: the reference truncates with (unsigned int),
 // which maps (-1, 0) to 0 (RenderUtils.cu:828-832).
@@ -129,31 +141,52 @@ __device__ __forceinline__ int base_of(double p) {
 }
 
 // Returns false if the sample is skipped (sparse grid, empty cell:
-// RenderUtils.cu:835-842 and 1681-1686).
+// RenderUtils.cu:835-842 and 1681-1686).  A set bit of the optional
+// 8x8x8-cell brick map is necessary for any occupied cell inside the
+// brick, so a clear brick bit skips the sample without reading the
+// cell bitfield; the set of evaluated samples is unchanged.
 template <typename R, bool SPARSE>
 __device__ __forceinline__ bool locate(const RenderArgs &a, double px, double py,
                                        double pz, Sample<R> &s) {
-    s.bx = base_of(px);
-    s.by = base_of(py);
-    s.bz = base_of(pz);
-    if (SPARSE) {
-        const int cx = clampi(s.bx, 0, a.W - 2);
-        const int cy = clampi(s.by, 0, a.H - 2);
-        const int cz = clampi(s.bz, 0, a.D - 2);
-        const uint32_t cell = (uint32_t)cz * (uint32_t)(a.W - 1) * (uint32_t)(a.H - 1) +
-                              (uint32_t)cy * (uint32_t)(a.W - 1) + (uint32_t)cx;
-        const uint32_t word = a.occ[cell >> 5];
-        if (((word >> (cell & 31)) & 1u) == 0) return false;
+    const int bx = base_of(px), by = base_of(py), bz = base_of(pz);
+    if (bx != s.bx || by != s.by || bz != s.bz) {
+        s.bx = bx;
+        s.by = by;
+        s.bz = bz;
+        bool occ = true;
+        if (SPARSE) {
+            const int cx = clampi(bx, 0, a.W - 2);
+            const int cy = clampi(by, 0, a.H - 2);
+            const int cz = clampi(bz, 0, a.D - 2);
+            if (a.bricks != nullptr) {
+                const uint32_t br = (uint32_t)(cz >> 3) * (uint32_t)a.nbx * (uint32_t)a.nby +
+                                    (uint32_t)(cy >> 3) * (uint32_t)a.nbx + (uint32_t)(cx >> 3);
+                occ = (a.bricks[br >> 5] >> (br & 31)) & 1u;
+            }
+            if (occ) {
+                const uint32_t cell = (uint32_t)cz * (uint32_t)(a.W - 1) * (uint32_t)(a.H - 1) +
+                                      (uint32_t)cy * (uint32_t)(a.W - 1) + (uint32_t)cx;
+                occ = (a.occ[cell >> 5] >> (cell & 31)) & 1u;
+            }
+        }
+        s.occ = occ;
+        if (occ) {
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                const int dx = j & 1, dy = (j >> 1) & 1, dz = j >> 2;
+                uint32_t v = scene_index(bx + dx, by + dy, bz + dz, a.W, a.H, a.D);
+                if (SPARSE) v = min(a.io[v], (uint32_t)(a.n_vox - 1));
+                s.v[j] = v;
+            }
+        }
     }
-    const R fx = (R)(px - (double)s.bx);
-    const R fy = (R)(py - (double)s.by);
-    const R fz = (R)(pz - (double)s.bz);
+    if (!s.occ) return false;
+    const R fx = (R)(px - (double)bx);
+    const R fy = (R)(py - (double)by);
+    const R fz = (R)(pz - (double)bz);
 #pragma unroll
     for (int j = 0; j < 8; j++) {
         const int dx = j & 1, dy = (j >> 1) & 1, dz = j >> 2;
-        uint32_t v = scene_index(s.bx + dx, s.by + dy, s.bz + dz, a.W, a.H, a.D);
-        if (SPARSE) v = min(a.io[v], (uint32_t)(a.n_vox - 1));
-        s.v[j] = v;
         const R wx = dx ? fx : (R)1 - fx;
         const R wy = dy ? fy : (R)1 - fy;
         const R wz = dz ? fz : (R)1 - fz;
@@ -250,9 +283,10 @@ __global__ __launch_bounds__(256) void k_render(const RenderArgs a) {
         int k = 0;
         uint32_t hcount = 0, hash = 2166136261u;
         int last_bx = -1, last_by = -1, last_bz = -1;
+        Sample<R> s;
+        sample_reset(s);
         while (t < tfar && k < kMaxSteps) {
             const double px = fma(t, dx, ox), py = fma(t, dy, oy), pz = fma(t, dz, oz);
-            Sample<R> s;
             if (!locate<R, SPARSE>(a, px, py, pz, s)) {
                 if (ADAPT) t += step;
                 ++k;
@@ -369,9 +403,10 @@ __global__ __launch_bounds__(256) void k_render(const RenderArgs a) {
     float *gbase = a.grad + p;
     const size_t stride = (size_t)a.stride;
 
+    Sample<R> s;
+    sample_reset(s);
     while (t < tfar && k < kMaxSteps) {
         const double px = fma(t, dx, ox), py = fma(t, dy, oy), pz = fma(t, dz, oz);
-        Sample<R> s;
         if (!locate<R, SPARSE>(a, px, py, pz, s)) {
             if (ADAPT) t += step;
             ++k;
@@ -499,6 +534,9 @@ int32_t prepare(const pxl_grid *g, const pxl_rays *r, const int32_t *sample_inde
     a.params = g->params;
     a.n_vox = g->n_vox;
     a.occ = g->occupancy;
+    a.bricks = g->sparse ? g->brick_occupancy : nullptr;
+    a.nbx = (g->W - 1 + 7) / 8;
+    a.nby = (g->H - 1 + 7) / 8;
     a.io = g->index_offsets;
     a.W = g->W;
     a.H = g->H;

@@ -51,6 +51,33 @@ __global__ __launch_bounds__(256) void k_occupancy(const float *__restrict__ den
     }
 }
 
+// One thread per 8x8x8-cell brick; a wave's 64 bricks are packed by ballot.
+__global__ __launch_bounds__(256) void k_bricks(const uint32_t *__restrict__ occ, int W, int H,
+                                                int D, int nbx, int nby, int64_t nbricks,
+                                                int64_t nwords, uint32_t *__restrict__ bricks) {
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool any = false;
+    if (b < nbricks) {
+        const int bx = (int)(b % nbx), by = (int)((b / nbx) % nby), bz = (int)(b / ((int64_t)nbx * nby));
+        const int cw = W - 1, ch = H - 1, cd = D - 1;
+        for (int z = bz * 8; z < min(bz * 8 + 8, cd) && !any; z++)
+            for (int y = by * 8; y < min(by * 8 + 8, ch) && !any; y++)
+                for (int x = bx * 8; x < min(bx * 8 + 8, cw); x++) {
+                    const int64_t c = ((int64_t)z * ch + y) * cw + x;
+                    if ((occ[c >> 5] >> (c & 31)) & 1u) {
+                        any = true;
+                        break;
+                    }
+                }
+    }
+    const uint64_t bal = __ballot(any);
+    if ((threadIdx.x & 63) == 0) {
+        const int64_t w0 = b >> 5;
+        if (w0 < nwords) bricks[w0] = (uint32_t)bal;
+        if (w0 + 1 < nwords) bricks[w0 + 1] = (uint32_t)(bal >> 32);
+    }
+}
+
 constexpr int kScanThreads = 256;
 constexpr int kScanPerThread = 16;
 constexpr int64_t kScanTile = (int64_t)kScanThreads * kScanPerThread;
@@ -210,6 +237,25 @@ extern "C" int32_t pxl_build_occupancy(const float *density, int32_t W, int32_t 
                        occupancy, in_cage, reinterpret_cast<unsigned long long *>(n_occupied),
                        ncell, nwords);
     return check_launch("pxl_build_occupancy");
+}
+
+extern "C" int64_t pxl_brick_words(int32_t W, int32_t H, int32_t D) {
+    if (W < 2 || H < 2 || D < 2) return 0;
+    const int64_t n = (int64_t)((W - 1 + 7) / 8) * ((H - 1 + 7) / 8) * ((D - 1 + 7) / 8);
+    return (n + 31) / 32;
+}
+
+extern "C" int32_t pxl_build_brick_occupancy(const uint32_t *occupancy, int32_t W, int32_t H,
+                                             int32_t D, uint32_t *bricks, void *stream) {
+    clear_error();
+    PXL_REQUIRE(occupancy && bricks, "pxl_build_brick_occupancy: null argument");
+    PXL_REQUIRE(W >= 2 && H >= 2 && D >= 2, "pxl_build_brick_occupancy: grid too small");
+    const int nbx = (W - 1 + 7) / 8, nby = (H - 1 + 7) / 8, nbz = (D - 1 + 7) / 8;
+    const int64_t nb = (int64_t)nbx * nby * nbz;
+    const int64_t nwords = (nb + 31) / 32;
+    hipLaunchKernelGGL(k_bricks, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0,
+                       as_stream(stream), occupancy, W, H, D, nbx, nby, nb, nwords, bricks);
+    return check_launch("pxl_build_brick_occupancy");
 }
 
 extern "C" size_t pxl_sparse_index_scratch_bytes(int64_t n_vertices) {

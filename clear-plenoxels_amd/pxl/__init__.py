@@ -29,7 +29,8 @@ EXPORTS = (
     "pxl_grid_compact", "pxl_grid_import_planes", "pxl_grid_export_planes",
     "pxl_loss_reduce_f64", "pxl_count_touched", "pxl_trainer_create", "pxl_trainer_step",
     "pxl_trainer_time_render", "pxl_trainer_render_ms", "pxl_trainer_loss",
-    "pxl_trainer_state", "pxl_trainer_destroy",
+    "pxl_trainer_state", "pxl_trainer_destroy", "pxl_brick_words",
+    "pxl_build_brick_occupancy",
 )
 
 
@@ -89,7 +90,8 @@ class GridC(ctypes.Structure):
     _fields_ = [("W", _I32), ("H", _I32), ("D", _I32), ("sh_dim", _I32), ("stride", _I32),
                 ("sparse", _I32), ("n_vox", _I64), ("params", _P), ("occupancy", _P),
                 ("index_offsets", _P), ("in_cage", _P), ("compact_to_dense", _P),
-                ("bbox", BBox), ("step", _F32), ("march_mode", _I32)]
+                ("brick_occupancy", _P), ("bbox", BBox), ("step", _F32),
+                ("march_mode", _I32)]
 
 
 class OptimC(ctypes.Structure):
@@ -137,6 +139,8 @@ def _declare(L):
         "pxl_trainer_state": ([_P, ctypes.POINTER(_I64), ctypes.POINTER(_F64),
                                ctypes.POINTER(_F64)], _I32),
         "pxl_trainer_destroy": ([_P], None),
+        "pxl_brick_words": ([_I32, _I32, _I32], _I64),
+        "pxl_build_brick_occupancy": ([_P, _I32, _I32, _I32, _P, _P], _I32),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -206,7 +210,8 @@ class Grid:
     caged-layout index arrays when sparse (SceneUtils.cuh:4-98)."""
 
     def __init__(self, W, H, D, S, params, step=0.5, bbox=None, march_mode=MARCH_FIXED,
-                 occupancy=None, index_offsets=None, in_cage=None, compact_to_dense=None):
+                 occupancy=None, index_offsets=None, in_cage=None, compact_to_dense=None,
+                 brick_occupancy=None):
         self.W, self.H, self.D, self.S = int(W), int(H), int(D), int(S)
         self.params = params
         self.stride = int(params.shape[1])
@@ -218,6 +223,7 @@ class Grid:
         self.index_offsets = index_offsets
         self.in_cage = in_cage
         self.compact_to_dense = compact_to_dense
+        self.brick_occupancy = brick_occupancy
         self.sparse = occupancy is not None
 
     @property
@@ -235,6 +241,8 @@ class Grid:
         g.in_cage = None if self.in_cage is None else self.in_cage.data_ptr()
         g.compact_to_dense = (None if self.compact_to_dense is None
                               else self.compact_to_dense.data_ptr())
+        g.brick_occupancy = (None if self.brick_occupancy is None
+                             else self.brick_occupancy.data_ptr())
         g.bbox = BBox(*self.bbox)
         g.step = self.step
         g.march_mode = self.march_mode
@@ -349,6 +357,17 @@ def build_occupancy(density, W, H, D, threshold, occupancy, in_cage, n_occupied=
     _check("pxl_build_occupancy",
            lib().pxl_build_occupancy(_ptr(density), W, H, D, threshold, _ptr(occupancy),
                                      _ptr(in_cage), _ptr(n_occupied), _stream(stream)))
+
+
+def build_brick_occupancy(occupancy, W, H, D, stream=None):
+    """Brick map (one bit per 8^3 cells) of a cell bitfield; returns it."""
+    import torch
+
+    words = lib().pxl_brick_words(W, H, D)
+    out = torch.zeros(max(1, words), dtype=torch.int32, device=occupancy.device)
+    _check("pxl_build_brick_occupancy",
+           lib().pxl_build_brick_occupancy(_ptr(occupancy), W, H, D, _ptr(out), _stream(stream)))
+    return out
 
 
 def sparse_index_scratch_bytes(n_vertices):

@@ -89,7 +89,10 @@ typedef struct pxl_rays {
  * reference's caged layout, SceneUtils.cu:516-640): n_vox = inCageNums
  * records, `index_offsets` maps a dense vertex to its record,
  * `occupancy` is one bit per cell, `in_cage` one byte per dense vertex and
- * `compact_to_dense` maps a record back to its dense vertex. */
+ * `compact_to_dense` maps a record back to its dense vertex.
+ * `brick_occupancy` (optional, sparse only, from pxl_build_brick_occupancy)
+ * has one bit per brick of 8x8x8 cells, set when any cell in it is
+ * occupied; marching then skips empty bricks without reading `occupancy`. */
 typedef struct pxl_grid {
     int32_t W, H, D;              /* vertices per axis                 */
     int32_t sh_dim;               /* S = (band+1)^2: 1, 4, 9 or 16     */
@@ -101,6 +104,7 @@ typedef struct pxl_grid {
     const uint32_t *index_offsets;/* [W*H*D]                           */
     const uint8_t *in_cage;       /* [W*H*D]                           */
     const uint32_t *compact_to_dense; /* [n_vox]                       */
+    const uint32_t *brick_occupancy;  /* optional: bit per 8^3-cell brick */
     pxl_bbox bbox;
     float step;                   /* marching step in voxels (StepSize) */
     int32_t march_mode;           /* PXL_MARCH_*                       */
@@ -207,6 +211,13 @@ int32_t pxl_build_occupancy(const float *density, int32_t W, int32_t H,
                             int32_t D, float threshold, uint32_t *occupancy,
                             uint8_t *in_cage, uint64_t *n_occupied,
                             void *stream);
+
+/* Brick map of a cell bitfield: bit b of brick (bx, by, bz) =
+ * bz*nbx*nby + by*nbx + bx, nbx = ceil((W-1)/8) (likewise y, z), set when
+ * any of its cells is occupied.  bricks: pxl_brick_words(W, H, D) words. */
+int64_t pxl_brick_words(int32_t W, int32_t H, int32_t D);
+int32_t pxl_build_brick_occupancy(const uint32_t *occupancy, int32_t W, int32_t H,
+                                  int32_t D, uint32_t *bricks, void *stream);
 
 /* Sparse index: index_offsets[v] = number of in-cage vertices before v if
  * v is in cage, else 0; *in_cage_nums = total (objectScene::getCagedNums,

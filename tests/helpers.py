@@ -76,15 +76,29 @@ def pack_bits(occ_bytes):
     return np.packbits(b.reshape(-1, 8), axis=1, bitorder="little").reshape(-1).view(np.uint32)
 
 
-def device_sparse_grid(N, S, compact, occ, io, cage, c2d, dev, step=0.5, march=0):
+def device_sparse_grid(N, S, compact, occ, io, cage, c2d, dev, step=0.5, march=0,
+                       bricks=True):
     import torch
 
     rec = to_records(compact, dev)
-    return pxl.Grid(N, N, N, S, rec, step=step, march_mode=march,
-                    occupancy=torch.from_numpy(pack_bits(occ).view(np.int32)).to(dev),
+    occ_d = torch.from_numpy(pack_bits(occ).view(np.int32)).to(dev)
+    br = pxl.build_brick_occupancy(occ_d, N, N, N) if bricks else None
+    return pxl.Grid(N, N, N, S, rec, step=step, march_mode=march, occupancy=occ_d,
                     index_offsets=torch.from_numpy(io.view(np.int32)).to(dev),
                     in_cage=torch.from_numpy(cage).to(dev),
-                    compact_to_dense=torch.from_numpy(c2d.view(np.int32)).to(dev))
+                    compact_to_dense=torch.from_numpy(c2d.view(np.int32)).to(dev),
+                    brick_occupancy=br)
+
+
+def brick_bits(occ_bytes, W, H, D):
+    """Reference brick map from cell bytes: any occupied cell per 8^3 brick."""
+    cw, ch, cd = W - 1, H - 1, D - 1
+    c = occ_bytes.reshape(cd, ch, cw) != 0
+    nbx, nby, nbz = -(-cw // 8), -(-ch // 8), -(-cd // 8)
+    pad = np.zeros((nbz * 8, nby * 8, nbx * 8), bool)
+    pad[:cd, :ch, :cw] = c
+    b = pad.reshape(nbz, 8, nby, 8, nbx, 8).any(axis=(1, 3, 5)).reshape(-1)
+    return pack_bits(b.astype(np.uint8))
 
 
 def grad_close(g, ref, rel=1e-4, floor=1e-3):

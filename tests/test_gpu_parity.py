@@ -13,8 +13,8 @@ import pytest
 
 import oracle
 import pxl
-from helpers import (c1_planes, cameras, device_sparse_grid, from_records, grad_close,
-                     oracle_rays, pack_bits, sparse_from_dense, to_records)
+from helpers import (brick_bits, c1_planes, cameras, device_sparse_grid, from_records,
+                     grad_close, oracle_rays, pack_bits, sparse_from_dense, to_records)
 
 pytestmark = pytest.mark.gpu
 
@@ -111,6 +111,10 @@ def test_occupancy_and_index_bit_exact(dev, c1, case):
     assert np.array_equal(io.cpu().numpy().view(np.uint32), io_o)
     assert np.array_equal(c2d.cpu().numpy()[:cnt_o].view(np.uint32),
                           np.flatnonzero(cage_o).astype(np.uint32))
+    br = pxl.build_brick_occupancy(occ, W, H, D)
+    torch.cuda.synchronize()
+    exp = brick_bits(occ_o, W, H, D)
+    assert np.array_equal(br.cpu().numpy().view(np.uint32)[:len(exp)], exp)
 
 
 def _fwd_compare(grid_o, rays_np, batch, got, trace_gpu, white=True):
@@ -157,11 +161,13 @@ def sparse1(c1):
     return dict(occ=occ, cage=cage, io=io, n=n, c2d=c2d, compact=compact)
 
 
-def test_render_fwd_sparse(c1, sparse1):
+@pytest.mark.parametrize("bricks", [True, False])
+def test_render_fwd_sparse(c1, sparse1, bricks):
     s = sparse1
-    say(f"[fwd] sparse 128^3, {s['n']} caged vertices")
+    say(f"[fwd] sparse 128^3, {s['n']} caged vertices, brick map {bricks}")
     dev = c1["dev"]
-    grid = device_sparse_grid(N1, S1, s["compact"], s["occ"], s["io"], s["cage"], s["c2d"], dev)
+    grid = device_sparse_grid(N1, S1, s["compact"], s["occ"], s["io"], s["cage"], s["c2d"], dev,
+                              bricks=bricks)
     rays = device_rays(c1["rays_np"], dev)
     got, tr = _gpu_fwd(grid, rays, c1["si"], dev)
     grid_o = oracle.Grid(N1, N1, N1, S1, s["compact"], 0.5, occupancy=s["occ"],
@@ -376,3 +382,49 @@ def test_train_step_c1(c1):
     say(f"    max |dparam| = {diff.max():.3g}, fraction beyond 1e-5 = {frac_off:.3g}")
     assert frac_off <= 1e-4
     assert diff.max() <= 0.2 + 1e-6
+
+
+def test_trainer_matches_stage_calls(c1):
+    """The C++ batch loop (pxl_trainer_step) issues the same stages as the
+    Python-driven sequence: dense grid (TV every batch), 3 batches, LR decay
+    lr /= 2^(1/(batchCount (epoch+1))) per batch (Render.cu:499-500)."""
+    import math
+
+    import torch
+
+    say("[trainer] 3 batches vs stage calls")
+    dev = c1["dev"]
+    B = 1024
+    order = c1["si"]
+    n = order.numel()
+    grids, optims = [], []
+    for _ in range(2):
+        rec = c1["rec"].clone()
+        g = pxl.Grid(N1, N1, N1, S1, rec, step=0.5)
+        grids.append(g)
+        optims.append(pxl.Optim(g))
+    rays_a = device_rays(c1["rays_np"], dev)
+    rays_b = device_rays(c1["rays_np"], dev)
+    tr = pxl.Trainer(grids[0], optims[0], rays_a, order, batch_size=B, learning_rate=0.1,
+                     sh_learning_rate=0.2, variance_coefficient=1e-5, sh_coefficient=1e-6,
+                     variance_interval=3, fused=True)
+    for _ in range(3):
+        tr.step()
+    losses = torch.zeros(B, dtype=torch.float64, device=dev)
+    lr, sl = 0.1, 0.2
+    bc = (n + B - 1) // B
+    for t in range(3):
+        pxl.train_rays(grids[1], optims[1], rays_b, losses, order, start=t * B, length=B)
+        pxl.add_variance_loss(grids[1], optims[1], 1e-5, 1e-6, pxl.TV_DISPARITY, 6, 0)
+        pxl.adam_step(grids[1], optims[1], lr, sl, t + 1)
+        d = math.pow(2.0, 1.0 / bc / 1.0)
+        lr, sl = lr / d, sl / d
+    torch.cuda.synchronize()
+    t_, lr_, sl_ = tr.state()
+    assert t_ == 3 and abs(lr_ - lr) < 1e-15 and abs(sl_ - sl) < 1e-15
+    mse, psnr = tr.loss()
+    assert np.isfinite(mse)
+    diff = (grids[0].params - grids[1].params).abs()
+    # identical kernels; only the order of float atomics differs
+    assert float((diff > 1e-4).float().mean()) <= 1e-4, float(diff.max())
+    tr.close()

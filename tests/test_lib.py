@@ -36,7 +36,7 @@ def test_struct_layouts_match_header(pxl_lib):
     assert ctypes.sizeof(pxl_lib.Camera) == 24 * 8
     assert ctypes.sizeof(pxl_lib.BBox) == 24
     assert ctypes.sizeof(pxl_lib.RaysC) == 17 * 8 + 8 + 8
-    assert ctypes.sizeof(pxl_lib.GridC) == 24 + 8 + 5 * 8 + 24 + 8
+    assert ctypes.sizeof(pxl_lib.GridC) == 24 + 8 + 6 * 8 + 24 + 8
     assert ctypes.sizeof(pxl_lib.OptimC) == 32
 
 
