@@ -48,6 +48,9 @@ def parse():
     ap.add_argument("--cpu-rays", type=int, default=1024,
                     help="rays of the bounded CPU-baseline sample (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="run one device-to-device copy of this many MiB first (a kernel of "
+                         "known traffic for calibrating profiler byte counters)")
     return ap.parse_args()
 
 
@@ -123,9 +126,12 @@ class Workload:
         # white-background render.
         allidx = torch.arange(self.n_rays, dtype=torch.int32, device=dev)
         chunk = 1 << 22
-        for s0 in range(0, self.n_rays, chunk):
+        for i, s0 in enumerate(range(0, self.n_rays, chunk)):
             pxl.render_fwd(self.gt_grid, rays, allidx, start=s0,
                            length=min(chunk, self.n_rays - s0))
+            if i % 4 == 3:
+                torch.cuda.synchronize()
+                log(f"targets: {s0 + chunk} / {self.n_rays} rays ({time.time() - t0:.1f}s)")
         torch.cuda.synchronize()
         A = rays.acc_w.clamp(0, 1)
         inv = 1.0 / A.clamp_min(1e-6)
@@ -233,6 +239,15 @@ def main():
     import torch
 
     dev = torch.device("cuda:0")
+    if args.copy_mib > 0:
+        n = args.copy_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {args.copy_mib} MiB read + {args.copy_mib} MiB written")
+        del src, dst
     w = Workload(args, dev)
     for _ in range(args.warmup):
         w.step()

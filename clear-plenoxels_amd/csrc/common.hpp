@@ -1,6 +1,7 @@
 // Shared device helpers and error plumbing of libpxl.
 #pragma once
 
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>

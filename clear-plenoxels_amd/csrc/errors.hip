@@ -31,4 +31,13 @@ int32_t check_launch(const char *what) {
 
 extern "C" const char *pxl_last_error(void) { return pxl::g_err; }
 
-extern "C" int32_t pxl_version(void) { return (1 << 16) | 0; }
+extern "C" int32_t pxl_version(void) { return (1 << 16) | 1; }
+
+extern "C" int32_t pxl_abi_sizes(int64_t *out, int32_t n) {
+    const int64_t sz[6] = {(int64_t)sizeof(pxl_camera), (int64_t)sizeof(pxl_bbox),
+                           (int64_t)sizeof(pxl_rays),   (int64_t)sizeof(pxl_grid),
+                           (int64_t)sizeof(pxl_optim),  (int64_t)sizeof(pxl_train_config)};
+    if (!out) return 6;
+    for (int i = 0; i < n && i < 6; i++) out[i] = sz[i];
+    return 6;
+}

@@ -6,8 +6,9 @@ namespace pxl {
 namespace {
 
 struct TvArgs {
-    const float *params;
+    const float *params;  // elements of the storage type
     float *grad;
+    int gstride;          // floats per gradient record
     const uint32_t *occ;
     const uint32_t *io;
     const uint32_t *c2d;
@@ -16,6 +17,14 @@ struct TvArgs {
     int64_t n_vox;
     float lam_d, lam_sh;
 };
+
+template <int STORE>
+__device__ __forceinline__ float ld(const float *base, size_t idx) {
+    if (STORE == PXL_STORE_F32) return base[idx];
+    const uint16_t h = reinterpret_cast<const uint16_t *>(base)[idx];
+    if (STORE == PXL_STORE_BF16) return __uint_as_float((uint32_t)h << 16);
+    return __half2float(__ushort_as_half(h));
+}
 
 __device__ __forceinline__ bool cell_occupied(const TvArgs &a, int cx, int cy, int cz) {
     const uint32_t cell = (uint32_t)cz * (uint32_t)(a.W - 1) * (uint32_t)(a.H - 1) +
@@ -36,7 +45,7 @@ __device__ __forceinline__ uint32_t record_of(const TvArgs &a, int x, int y, int
 // same terms from the (up to 8) cells that have its vertex as a corner:
 // one group of LPV lanes per record, one lane per parameter, 27-neighbour
 // values loaded once.
-template <int LPV>
+template <int LPV, int STORE>
 __global__ __launch_bounds__(256) void k_tv_disparity(const TvArgs a) {
     constexpr int kGroups = 256 / LPV;
     const int g = threadIdx.x / LPV, p = threadIdx.x % LPV;
@@ -50,40 +59,42 @@ __global__ __launch_bounds__(256) void k_tv_disparity(const TvArgs a) {
     const int y = (int)((d / (uint32_t)a.W) % (uint32_t)a.H);
     const int z = (int)(d / ((uint32_t)a.W * (uint32_t)a.H));
 
-    // Neighbour values u[(dz+1)*9 + (dy+1)*3 + (dx+1)].
-    float u[27];
+    // Neighbour values u[(dz+1)*9 + (dy+1)*3 + (dx+1)].  Sums run in FP64:
+    // a vertex value and the mean of a cell around it nearly cancel.
+    double u[27];
 #pragma unroll
     for (int i = 0; i < 27; i++) {
         const int dx = i % 3 - 1, dy = (i / 3) % 3 - 1, dz = i / 9 - 1;
         const int nx = x + dx, ny = y + dy, nz = z + dz;
         float val = 0.0f;
         if (nx >= 0 && nx < a.W && ny >= 0 && ny < a.H && nz >= 0 && nz < a.D)
-            val = a.params[(size_t)record_of(a, nx, ny, nz) * (size_t)a.stride + p];
-        u[i] = val;
+            val = ld<STORE>(a.params, (size_t)record_of(a, nx, ny, nz) * (size_t)a.stride + p);
+        u[i] = (double)val;
     }
-    const float self = a.params[(size_t)r * (size_t)a.stride + p];
-    float tsum = 0.0f;
+    const double self = u[13];
+    double tsum = 0.0;
 #pragma unroll
     for (int c = 0; c < 8; c++) {
         const int ax = c & 1, ay = (c >> 1) & 1, az = c >> 2;
         const int cx = x - ax, cy = y - ay, cz = z - az;
         if (cx < 0 || cx > a.W - 2 || cy < 0 || cy > a.H - 2 || cz < 0 || cz > a.D - 2) continue;
         if (a.sparse && !cell_occupied(a, cx, cy, cz)) continue;
-        float s = 0.0f;
+        double s = 0.0;
 #pragma unroll
         for (int j = 0; j < 8; j++) {
             const int jx = j & 1, jy = (j >> 1) & 1, jz = j >> 2;
             s += u[(jz - az + 1) * 9 + (jy - ay + 1) * 3 + (jx - ax + 1)];
         }
-        tsum += self - s * 0.125f;
+        tsum += self - s * 0.125;
     }
-    a.grad[(size_t)r * (size_t)a.stride + p] += lam * tsum;
+    a.grad[(size_t)r * (size_t)a.gstride + p] += (float)((double)lam * tsum);
 }
 
 // Caged grids: a cell corner that is not in cage reads and writes record 0
 // (index_offsets = 0, SceneUtils.cu:1106-1113).  Those (cell, corner) pairs
 // are not found by the gather above; this kernel adds them into record 0.
 // One thread per cell; waves without such a cell exit after one ballot.
+template <int STORE>
 __global__ __launch_bounds__(256) void k_tv_alias0(const TvArgs a, int64_t ncell) {
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int n_alias = 0;
@@ -107,15 +118,17 @@ __global__ __launch_bounds__(256) void k_tv_alias0(const TvArgs a, int64_t ncell
     for (int p = 0; p < a.P; p++) {
         const float lam = p == 0 ? a.lam_d : a.lam_sh;
         if (lam == 0.0f) continue;
-        float contrib = 0.0f;
+        double contrib = 0.0;
         if (n_alias > 0) {
-            float s = 0.0f;
+            double s = 0.0;
 #pragma unroll
-            for (int j = 0; j < 8; j++) s += a.params[(size_t)rec[j] * (size_t)a.stride + p];
-            contrib = (float)n_alias * lam * (a.params[p] - s * 0.125f);
+            for (int j = 0; j < 8; j++)
+                s += (double)ld<STORE>(a.params, (size_t)rec[j] * (size_t)a.stride + p);
+            contrib = (double)n_alias * (double)lam *
+                      ((double)ld<STORE>(a.params, (size_t)p) - s * 0.125);
         }
-        contrib = wave_sum_f(contrib);
-        if ((threadIdx.x & 63) == 0 && contrib != 0.0f) unsafeAtomicAdd(a.grad + p, contrib);
+        contrib = wave_sum_d(contrib);
+        if ((threadIdx.x & 63) == 0 && contrib != 0.0) unsafeAtomicAdd(a.grad + p, (float)contrib);
     }
 }
 
@@ -123,7 +136,7 @@ __global__ __launch_bounds__(256) void k_tv_alias0(const TvArgs a, int64_t ncell
 // grids: interior vertices, per parameter plane
 // grad += lambda / (sqrt(sum (u - u_n)^2) / 6 + 1e-6) * sum (u - u_n);
 // a neighbour whose linear index is 0 counts as the vertex itself.
-template <int LPV>
+template <int LPV, int STORE>
 __global__ __launch_bounds__(256) void k_tv_neighbour6(const TvArgs a) {
     constexpr int kGroups = 256 / LPV;
     const int g = threadIdx.x / LPV, p = threadIdx.x % LPV;
@@ -133,34 +146,64 @@ __global__ __launch_bounds__(256) void k_tv_neighbour6(const TvArgs a) {
     if (lam == 0.0f) return;
     const int x = (int)(r % a.W), y = (int)((r / a.W) % a.H), z = (int)(r / ((int64_t)a.W * a.H));
     if (x == 0 || x == a.W - 1 || y == 0 || y == a.H - 1 || z == 0 || z == a.D - 1) return;
-    const float self = a.params[(size_t)r * a.stride + p];
+    const float self = ld<STORE>(a.params, (size_t)r * a.stride + p);
     const int off[6][3] = {{0, 0, -1}, {0, -1, 0}, {-1, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
-    float sq = 0.0f, dis = 0.0f;
+    double sq = 0.0, dis = 0.0;
 #pragma unroll
     for (int i = 0; i < 6; i++) {
         const uint32_t n = record_of(a, x + off[i][0], y + off[i][1], z + off[i][2]);
-        const float un = n != 0u ? a.params[(size_t)n * a.stride + p] : self;
-        sq += (self - un) * (self - un);
-        dis += self - un;
+        const float un = n != 0u ? ld<STORE>(a.params, (size_t)n * a.stride + p) : self;
+        const double d = (double)self - (double)un;
+        sq += d * d;
+        dis += d;
     }
-    const float s = sqrtf(sq) / 6.0f;
-    a.grad[(size_t)r * a.stride + p] += (float)((double)lam / ((double)s + 1e-6) * (double)dis);
+    const double s = sqrt(sq) / 6.0;
+    a.grad[(size_t)r * a.gstride + p] += (float)((double)lam / (s + 1e-6) * dis);
 }
 
 // updateRGBSceneGPU_ADAM (Optimizer.cu:359-415).  One thread per 4
 // consecutive parameters of a record; the float/double mix of each update
-// follows the reference expression by expression.
+// follows the reference expression by expression.  With 16-bit storage the
+// update runs on the FP32 master copy when there is one (else on the
+// widened stored value) and the grid receives the rounded result.
+__device__ __forceinline__ uint16_t to_bf16(float f) {
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+template <int STORE>
+__device__ __forceinline__ void store_param(float *base, size_t idx, float v) {
+    if (STORE == PXL_STORE_F32) base[idx] = v;
+    else if (STORE == PXL_STORE_BF16) reinterpret_cast<uint16_t *>(base)[idx] = to_bf16(v);
+    else reinterpret_cast<uint16_t *>(base)[idx] = __half_as_ushort(__float2half_rn(v));
+}
+
+template <int STORE>
 __global__ __launch_bounds__(256) void k_adam(float *__restrict__ prm, float *__restrict__ grd,
                                               float *__restrict__ mom, float *__restrict__ var,
+                                              float *__restrict__ master,
                                               const float *__restrict__ alr, int64_t n_vox,
-                                              int stride, int P, int Q, double lr, double sh_lr,
-                                              double bc1, double bc2) {
+                                              int pstride, int gstride, int P, int Q, double lr,
+                                              double sh_lr, double bc1, double bc2) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_vox * Q) return;
     const int64_t r = i / Q;
     const int q = (int)(i % Q);
-    const size_t off = (size_t)r * stride + 4 * q;
-    float4 th = *reinterpret_cast<const float4 *>(prm + off);
+    const size_t off = (size_t)r * gstride + 4 * q;
+    const size_t poff = (size_t)r * pstride + 4 * q;
+    float4 th;
+    if (STORE == PXL_STORE_F32) {
+        th = *reinterpret_cast<const float4 *>(prm + poff);
+    } else if (master != nullptr) {
+        th = *reinterpret_cast<const float4 *>(master + off);
+    } else {
+        th.x = ld<STORE>(prm, poff);
+        th.y = ld<STORE>(prm, poff + 1);
+        th.z = ld<STORE>(prm, poff + 2);
+        th.w = ld<STORE>(prm, poff + 3);
+    }
     float4 g4 = *reinterpret_cast<const float4 *>(grd + off);
     float4 m4 = *reinterpret_cast<const float4 *>(mom + off);
     float4 v4 = *reinterpret_cast<const float4 *>(var + off);
@@ -184,7 +227,12 @@ __global__ __launch_bounds__(256) void k_adam(float *__restrict__ prm, float *__
         vp[e] = vv;
         gp[e] = 0.0f;
     }
-    *reinterpret_cast<float4 *>(prm + off) = th;
+    if (STORE == PXL_STORE_F32) {
+        *reinterpret_cast<float4 *>(prm + poff) = th;
+    } else {
+        if (master != nullptr) *reinterpret_cast<float4 *>(master + off) = th;
+        for (int e = 0; e < 4 && 4 * q + e < P; e++) store_param<STORE>(prm, poff + e, tp[e]);
+    }
     *reinterpret_cast<float4 *>(grd + off) = g4;
     *reinterpret_cast<float4 *>(mom + off) = m4;
     *reinterpret_cast<float4 *>(var + off) = v4;
@@ -210,14 +258,38 @@ int32_t tv_check(const pxl_grid *g, const pxl_optim *o, const char *who) {
     PXL_REQUIRE(g && o && o->grad && g->params, "%s: null argument", who);
     PXL_REQUIRE(g->W >= 2 && g->H >= 2 && g->D >= 2, "%s: grid too small", who);
     PXL_REQUIRE(g->sh_dim >= 1 && g->sh_dim <= 21, "%s: bad sh_dim %d", who, g->sh_dim);
-    PXL_REQUIRE(g->stride >= 1 + 3 * g->sh_dim && g->stride % 4 == 0, "%s: bad stride %d", who,
-                g->stride);
+    PXL_REQUIRE(g->storage == PXL_STORE_F32 || g->storage == PXL_STORE_BF16 ||
+                    g->storage == PXL_STORE_F16,
+                "%s: unknown storage %d", who, g->storage);
+    PXL_REQUIRE(g->stride >= 1 + 3 * g->sh_dim && g->stride % (g->storage ? 8 : 4) == 0,
+                "%s: bad stride %d", who, g->stride);
+    PXL_REQUIRE(g->storage == PXL_STORE_F32 || o->stride > 0,
+                "%s: 16-bit grids need an explicit optimiser stride", who);
     PXL_REQUIRE(g->sparse || g->n_vox == (int64_t)g->W * g->H * g->D,
                 "%s: dense grid needs n_vox = W*H*D", who);
     PXL_REQUIRE(!g->sparse || (g->occupancy && g->index_offsets && g->compact_to_dense),
                 "%s: sparse grid needs occupancy, index_offsets and compact_to_dense", who);
     PXL_REQUIRE(g->n_vox > 0, "%s: empty grid", who);
     return PXL_OK;
+}
+
+template <int STORE>
+void tv_launch(const TvArgs &a, int type, hipStream_t s) {
+    const bool wide = a.P > 32;
+    const int groups = wide ? 4 : 8;
+    const unsigned blocks = (unsigned)((a.n_vox + groups - 1) / groups);
+    if (type == PXL_TV_DISPARITY) {
+        if (wide) hipLaunchKernelGGL((k_tv_disparity<64, STORE>), dim3(blocks), dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((k_tv_disparity<32, STORE>), dim3(blocks), dim3(256), 0, s, a);
+        if (a.sparse) {
+            const int64_t ncell = (int64_t)(a.W - 1) * (a.H - 1) * (a.D - 1);
+            hipLaunchKernelGGL(k_tv_alias0<STORE>, dim3((unsigned)((ncell + 255) / 256)),
+                               dim3(256), 0, s, a, ncell);
+        }
+    } else {
+        if (wide) hipLaunchKernelGGL((k_tv_neighbour6<64, STORE>), dim3(blocks), dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((k_tv_neighbour6<32, STORE>), dim3(blocks), dim3(256), 0, s, a);
+    }
 }
 
 }  // namespace
@@ -244,6 +316,7 @@ extern "C" int32_t pxl_add_variance_loss(const pxl_grid *grid, const pxl_optim *
     TvArgs a;
     a.params = grid->params;
     a.grad = optim->grad;
+    a.gstride = optim->stride > 0 ? optim->stride : grid->stride;
     a.occ = grid->occupancy;
     a.io = grid->index_offsets;
     a.c2d = grid->compact_to_dense;
@@ -256,26 +329,15 @@ extern "C" int32_t pxl_add_variance_loss(const pxl_grid *grid, const pxl_optim *
     a.n_vox = grid->n_vox;
     a.lam_d = (float)lambda_density;
     hipStream_t s = as_stream(stream);
-    const bool wide = a.P > 32;
-    if (type == PXL_TV_DISPARITY) {
-        a.lam_sh = optimize_step <= 1 ? (float)lambda_sh : 0.0f;
-        const int groups = wide ? 4 : 8;
-        const unsigned blocks = (unsigned)((a.n_vox + groups - 1) / groups);
-        if (wide) hipLaunchKernelGGL(k_tv_disparity<64>, dim3(blocks), dim3(256), 0, s, a);
-        else hipLaunchKernelGGL(k_tv_disparity<32>, dim3(blocks), dim3(256), 0, s, a);
-        if (a.sparse) {
-            const int64_t ncell = (int64_t)(a.W - 1) * (a.H - 1) * (a.D - 1);
-            hipLaunchKernelGGL(k_tv_alias0, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0,
-                               s, a, ncell);
-        }
-    } else {
+    if (type == PXL_TV_NEIGHBOUR) {
         PXL_REQUIRE(neighbours == 6, "pxl_add_variance_loss: only the 6-neighbour TV is built");
         PXL_REQUIRE(!grid->sparse, "pxl_add_variance_loss: 6-neighbour TV needs a dense grid");
-        a.lam_sh = (float)lambda_sh;
-        const int groups = wide ? 4 : 8;
-        const unsigned blocks = (unsigned)((a.n_vox + groups - 1) / groups);
-        if (wide) hipLaunchKernelGGL(k_tv_neighbour6<64>, dim3(blocks), dim3(256), 0, s, a);
-        else hipLaunchKernelGGL(k_tv_neighbour6<32>, dim3(blocks), dim3(256), 0, s, a);
+    }
+    a.lam_sh = (type == PXL_TV_NEIGHBOUR || optimize_step <= 1) ? (float)lambda_sh : 0.0f;
+    switch (grid->storage) {
+        case PXL_STORE_F32: tv_launch<PXL_STORE_F32>(a, type, s); break;
+        case PXL_STORE_BF16: tv_launch<PXL_STORE_BF16>(a, type, s); break;
+        default: tv_launch<PXL_STORE_F16>(a, type, s); break;
     }
     return check_launch("pxl_add_variance_loss");
 }
@@ -286,23 +348,42 @@ extern "C" int32_t pxl_adam_step(const pxl_grid *grid, const pxl_optim *optim, d
     PXL_REQUIRE(grid && optim && grid->params && optim->grad && optim->m && optim->v,
                 "pxl_adam_step: null argument");
     PXL_REQUIRE(t >= 1, "pxl_adam_step: time step %d must be >= 1", t);
-    PXL_REQUIRE(grid->stride >= 1 + 3 * grid->sh_dim && grid->stride % 4 == 0 && grid->sh_dim >= 1,
-                "pxl_adam_step: bad stride %d / sh_dim %d", grid->stride, grid->sh_dim);
-    PXL_REQUIRE(grid->n_vox >= 0, "pxl_adam_step: bad n_vox");
-    PXL_REQUIRE(((uintptr_t)grid->params | (uintptr_t)optim->grad | (uintptr_t)optim->m |
-                 (uintptr_t)optim->v) % 16 == 0,
-                "pxl_adam_step: parameter and state buffers must be 16-byte aligned");
-    if (grid->n_vox == 0) return PXL_OK;
+    PXL_REQUIRE(grid->sh_dim >= 1, "pxl_adam_step: bad sh_dim %d", grid->sh_dim);
+    PXL_REQUIRE(grid->storage == PXL_STORE_F32 || grid->storage == PXL_STORE_BF16 ||
+                    grid->storage == PXL_STORE_F16,
+                "pxl_adam_step: unknown storage %d", grid->storage);
     const int P = 1 + 3 * grid->sh_dim;
+    const int pstride = grid->stride;
+    const int gstride = optim->stride > 0 ? optim->stride : grid->stride;
+    PXL_REQUIRE(grid->storage == PXL_STORE_F32 || optim->stride > 0,
+                "pxl_adam_step: 16-bit grids need an explicit optimiser stride");
+    PXL_REQUIRE(pstride >= P && pstride % (grid->storage ? 8 : 4) == 0 && gstride >= P &&
+                    gstride % 4 == 0,
+                "pxl_adam_step: bad strides %d / %d", pstride, gstride);
+    PXL_REQUIRE(((uintptr_t)grid->params | (uintptr_t)optim->grad | (uintptr_t)optim->m |
+                 (uintptr_t)optim->v | (uintptr_t)optim->master) % 16 == 0,
+                "pxl_adam_step: parameter and state buffers must be 16-byte aligned");
+    PXL_REQUIRE(grid->n_vox >= 0, "pxl_adam_step: bad n_vox");
+    if (grid->n_vox == 0) return PXL_OK;
     const int Q = (P + 3) / 4;
     // Bias corrections with float pow as in the reference (Optimizer.cu:384-385).
     const double bc1 = 1.0 - (double)powf(0.9f, (float)t);
     const double bc2 = 1.0 - (double)powf(0.999f, (float)t);
     const int64_t total = grid->n_vox * Q;
     const float *alr = grid->sparse ? nullptr : optim->adaptive_lr;
-    hipLaunchKernelGGL(k_adam, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
-                       as_stream(stream), grid->params, optim->grad, optim->m, optim->v, alr,
-                       grid->n_vox, grid->stride, P, Q, lr, sh_lr, bc1, bc2);
+    float *master = grid->storage == PXL_STORE_F32 ? nullptr : optim->master;
+    const dim3 gdim((unsigned)((total + 255) / 256));
+    hipStream_t s = as_stream(stream);
+#define PXL_ADAM(ST)                                                                            \
+    hipLaunchKernelGGL(k_adam<ST>, gdim, dim3(256), 0, s, grid->params, optim->grad, optim->m, \
+                       optim->v, master, alr, grid->n_vox, pstride, gstride, P, Q, lr, sh_lr,  \
+                       bc1, bc2)
+    switch (grid->storage) {
+        case PXL_STORE_F32: PXL_ADAM(PXL_STORE_F32); break;
+        case PXL_STORE_BF16: PXL_ADAM(PXL_STORE_BF16); break;
+        default: PXL_ADAM(PXL_STORE_F16); break;
+    }
+#undef PXL_ADAM
     return check_launch("pxl_adam_step");
 }
 

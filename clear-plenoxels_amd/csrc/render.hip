@@ -37,6 +37,7 @@ struct RenderArgs {
     int nbx, nby;
     const uint32_t *io;
     int W, H, D, stride;
+    int gstride;  // floats per gradient record
     float step;
     int white_bg;
     pxl_rays rays;
@@ -195,8 +196,18 @@ __device__ __forceinline__ bool locate(const RenderArgs &a, double px, double py
     return true;
 }
 
+// One stored parameter as float: FP32, bfloat16 (upper half of an FP32)
+// or IEEE half.
+template <int STORE>
+__device__ __forceinline__ float load_param(const float *base, size_t idx) {
+    if (STORE == PXL_STORE_F32) return base[idx];
+    const uint16_t h = reinterpret_cast<const uint16_t *>(base)[idx];
+    if (STORE == PXL_STORE_BF16) return __uint_as_float((uint32_t)h << 16);
+    return __half2float(__ushort_as_half(h));
+}
+
 // Interpolated raw density and raw colours of a located sample.
-template <typename R>
+template <typename R, int STORE>
 __device__ __forceinline__ void interpolate(const RenderArgs &a, const Sample<R> &s,
                                             int p, R ylane, R &sraw, R &c0, R &c1,
                                             R &c2) {
@@ -204,7 +215,7 @@ __device__ __forceinline__ void interpolate(const RenderArgs &a, const Sample<R>
     if (p >= 0) {
 #pragma unroll
         for (int j = 0; j < 8; j++)
-            acc += s.w[j] * (R)a.params[(size_t)s.v[j] * (size_t)a.stride + p];
+            acc += s.w[j] * (R)load_param<STORE>(a.params, (size_t)s.v[j] * (size_t)a.stride + p);
     }
     R val = acc * ylane;
     val = row_sum<R>(val);
@@ -222,7 +233,7 @@ __device__ __forceinline__ uint32_t fnv_mix(uint32_t h, uint32_t x) {
     return h;
 }
 
-template <int S, bool SPARSE, bool ADAPT, int MODE>
+template <int S, bool SPARSE, bool ADAPT, int MODE, int STORE>
 __global__ __launch_bounds__(256) void k_render(const RenderArgs a) {
     using R = typename std::conditional<ADAPT, double, float>::type;
     const int lane = threadIdx.x & 63;
@@ -322,7 +333,7 @@ __global__ __launch_bounds__(256) void k_render(const RenderArgs a) {
                 }
             }
             R sraw, cr0, cr1, cr2;
-            interpolate<R>(a, s, p, ylane, sraw, cr0, cr1, cr2);
+            interpolate<R, STORE>(a, s, p, ylane, sraw, cr0, cr1, cr2);
             const R sigma = softplus_shift<R>(sraw);
             const R col0 = sigmoid01<R>(cr0), col1 = sigmoid01<R>(cr1), col2 = sigmoid01<R>(cr2);
             const R alpha = one_minus_exp_neg<R>(sigma * rstep);
@@ -401,7 +412,7 @@ __global__ __launch_bounds__(256) void k_render(const RenderArgs a) {
         cv[j] = 0u;
     }
     float *gbase = a.grad + p;
-    const size_t stride = (size_t)a.stride;
+    const size_t stride = (size_t)a.gstride;
 
     Sample<R> s;
     sample_reset(s);
@@ -414,7 +425,7 @@ __global__ __launch_bounds__(256) void k_render(const RenderArgs a) {
             continue;
         }
         R sraw, cr0, cr1, cr2;
-        interpolate<R>(a, s, p, ylane, sraw, cr0, cr1, cr2);
+        interpolate<R, STORE>(a, s, p, ylane, sraw, cr0, cr1, cr2);
         const R sigma = softplus_shift<R>(sraw);
         const R col0 = sigmoid01<R>(cr0), col1 = sigmoid01<R>(cr1), col2 = sigmoid01<R>(cr2);
         const R alpha = one_minus_exp_neg<R>(sigma * rstep);
@@ -469,24 +480,40 @@ __global__ __launch_bounds__(256) void k_render(const RenderArgs a) {
     }
 }
 
-template <int S, bool SPARSE, bool ADAPT, int MODE>
+template <int S, bool SPARSE, bool ADAPT, int MODE, int STORE>
 int32_t launch3(const RenderArgs &a, hipStream_t st) {
     const unsigned blocks = (unsigned)((a.len + 3) / 4);
-    hipLaunchKernelGGL((k_render<S, SPARSE, ADAPT, MODE>), dim3(blocks), dim3(256), 0, st, a);
+    hipLaunchKernelGGL((k_render<S, SPARSE, ADAPT, MODE, STORE>), dim3(blocks), dim3(256), 0, st,
+                       a);
     return PXL_OK;
 }
 
+template <int S, bool SPARSE, int MODE>
+int32_t launch_store(int store, bool adapt, const RenderArgs &a, hipStream_t st) {
+    if (adapt) {
+        if (store != PXL_STORE_F32) {
+            set_error("render: the adaptive marching rule needs FP32 storage");
+            return PXL_EINVAL;
+        }
+        return launch3<S, SPARSE, true, MODE, PXL_STORE_F32>(a, st);
+    }
+    switch (store) {
+        case PXL_STORE_F32: return launch3<S, SPARSE, false, MODE, PXL_STORE_F32>(a, st);
+        case PXL_STORE_BF16: return launch3<S, SPARSE, false, MODE, PXL_STORE_BF16>(a, st);
+        case PXL_STORE_F16: return launch3<S, SPARSE, false, MODE, PXL_STORE_F16>(a, st);
+        default:
+            set_error("render: unknown storage %d", store);
+            return PXL_EINVAL;
+    }
+}
+
 template <int MODE>
-int32_t dispatch(int S, bool sparse, bool adapt, const RenderArgs &a, hipStream_t st) {
+int32_t dispatch(int S, bool sparse, bool adapt, int store, const RenderArgs &a,
+                 hipStream_t st) {
 #define PXL_CASE(SV)                                                              \
     case SV:                                                                      \
-        if (sparse) {                                                             \
-            return adapt ? launch3<SV, true, true, MODE>(a, st)                   \
-                         : launch3<SV, true, false, MODE>(a, st);                 \
-        } else {                                                                  \
-            return adapt ? launch3<SV, false, true, MODE>(a, st)                  \
-                         : launch3<SV, false, false, MODE>(a, st);                \
-        }
+        return sparse ? launch_store<SV, true, MODE>(store, adapt, a, st)         \
+                      : launch_store<SV, false, MODE>(store, adapt, a, st);
     switch (S) {
         PXL_CASE(1)
         PXL_CASE(4)
@@ -508,8 +535,12 @@ int32_t check_grid(const pxl_grid *g, const char *who) {
                 (long long)g->W * g->H * g->D);
     PXL_REQUIRE(g->sh_dim == 1 || g->sh_dim == 4 || g->sh_dim == 9 || g->sh_dim == 16,
                 "%s: unsupported sh_dim %d (1, 4, 9 or 16)", who, g->sh_dim);
-    PXL_REQUIRE(g->stride >= 1 + 3 * g->sh_dim && g->stride % 4 == 0,
-                "%s: stride %d must be a multiple of 4 and >= %d", who, g->stride,
+    PXL_REQUIRE(g->storage == PXL_STORE_F32 || g->storage == PXL_STORE_BF16 ||
+                    g->storage == PXL_STORE_F16,
+                "%s: unknown storage %d", who, g->storage);
+    const int align = g->storage == PXL_STORE_F32 ? 4 : 8;
+    PXL_REQUIRE(g->stride >= 1 + 3 * g->sh_dim && g->stride % align == 0,
+                "%s: stride %d must be a multiple of %d and >= %d", who, g->stride, align,
                 1 + 3 * g->sh_dim);
     PXL_REQUIRE(g->params != nullptr && g->n_vox > 0, "%s: empty grid", who);
     PXL_REQUIRE(g->sparse || g->n_vox == (int64_t)g->W * g->H * g->D,
@@ -542,6 +573,7 @@ int32_t prepare(const pxl_grid *g, const pxl_rays *r, const int32_t *sample_inde
     a.H = g->H;
     a.D = g->D;
     a.stride = g->stride;
+    a.gstride = g->stride;
     a.step = g->step;
     a.white_bg = r->white_bg;
     a.rays = *r;
@@ -566,8 +598,8 @@ extern "C" int32_t pxl_render_fwd(const pxl_grid *grid, const pxl_rays *rays,
     PXL_REQUIRE(rays->acc_r && rays->acc_g && rays->acc_b && rays->acc_w && rays->acc_density,
                 "pxl_render_fwd: output planes must be non-null");
     if (len == 0) return PXL_OK;
-    st = dispatch<kFwd>(grid->sh_dim, grid->sparse, grid->march_mode == PXL_MARCH_REF_ADAPTIVE, a,
-                        as_stream(stream));
+    st = dispatch<kFwd>(grid->sh_dim, grid->sparse, grid->march_mode == PXL_MARCH_REF_ADAPTIVE,
+                        grid->storage, a, as_stream(stream));
     return st != PXL_OK ? st : check_launch("pxl_render_fwd");
 }
 
@@ -584,7 +616,8 @@ extern "C" int32_t pxl_render_fwd_trace(const pxl_grid *grid, const pxl_rays *ra
     if (len == 0) return PXL_OK;
     a.trace = trace;
     st = dispatch<kTrace>(grid->sh_dim, grid->sparse,
-                          grid->march_mode == PXL_MARCH_REF_ADAPTIVE, a, as_stream(stream));
+                          grid->march_mode == PXL_MARCH_REF_ADAPTIVE, grid->storage, a,
+                          as_stream(stream));
     return st != PXL_OK ? st : check_launch("pxl_render_fwd_trace");
 }
 
@@ -605,12 +638,20 @@ static int32_t bwd_common(const pxl_grid *grid, const pxl_optim *optim, const px
                 "%s: forward planes must be non-null", who);
     if (len == 0) return PXL_OK;
     a.grad = optim->grad;
+    a.gstride = optim->stride > 0 ? optim->stride : grid->stride;
+    PXL_REQUIRE(a.gstride >= 1 + 3 * grid->sh_dim && a.gstride % 4 == 0,
+                "%s: gradient stride %d must be a multiple of 4 and >= %d", who, a.gstride,
+                1 + 3 * grid->sh_dim);
+    PXL_REQUIRE(grid->storage == PXL_STORE_F32 || optim->stride > 0,
+                "%s: 16-bit grids need an explicit optimiser stride", who);
     a.losses = losses;
     a.mask_coef = mask_coef;
     a.huber = huber_delta;
     const bool adapt = grid->march_mode == PXL_MARCH_REF_ADAPTIVE;
-    st = fused ? dispatch<kFused>(grid->sh_dim, grid->sparse, adapt, a, as_stream(stream))
-               : dispatch<kBwd>(grid->sh_dim, grid->sparse, adapt, a, as_stream(stream));
+    st = fused ? dispatch<kFused>(grid->sh_dim, grid->sparse, adapt, grid->storage, a,
+                                  as_stream(stream))
+               : dispatch<kBwd>(grid->sh_dim, grid->sparse, adapt, grid->storage, a,
+                                as_stream(stream));
     return st != PXL_OK ? st : check_launch(who);
 }
 
@@ -663,7 +704,7 @@ extern "C" int32_t pxl_count_touched(const pxl_grid *grid, const pxl_rays *rays,
     if (len > 0) {
         a.bitmap = bitmap;
         st = dispatch<kTouch>(grid->sh_dim, grid->sparse,
-                              grid->march_mode == PXL_MARCH_REF_ADAPTIVE, a, s);
+                              grid->march_mode == PXL_MARCH_REF_ADAPTIVE, grid->storage, a, s);
         if (st != PXL_OK) return st;
     }
     hipLaunchKernelGGL(k_popcount, dim3(256), dim3(256), 0, s, bitmap, nwords,

@@ -190,6 +190,30 @@ __global__ void k_compact(const float4 *__restrict__ dense, int q_per_rec,
     out[i] = dense[(int64_t)c2d[r] * q_per_rec + q];
 }
 
+__global__ void k_to16(const float *__restrict__ rec32, int s32, int64_t n, uint16_t *rec16,
+                       int s16, int storage) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n * s16) return;
+    const int64_t r = i / s16;
+    const int e = (int)(i % s16);
+    uint16_t h = 0;
+    if (e < s32) {
+        const float f = rec32[r * s32 + e];
+        if (storage == PXL_STORE_BF16) {
+            uint32_t u = __float_as_uint(f);
+            if ((u & 0x7f800000u) == 0x7f800000u) {
+                h = (uint16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+            } else {
+                u += 0x7fffu + ((u >> 16) & 1u);
+                h = (uint16_t)(u >> 16);
+            }
+        } else {
+            h = __half_as_ushort(__float2half_rn(f));
+        }
+    }
+    rec16[i] = h;
+}
+
 constexpr int kMaxPlanes = 80;
 struct PlaneTable {
     const float *p[kMaxPlanes];
@@ -346,4 +370,20 @@ extern "C" int32_t pxl_grid_export_planes(const float *records, int32_t stride, 
     hipLaunchKernelGGL(k_export, dim3((unsigned)((n_vox + 255) / 256)), dim3(256), 0,
                        as_stream(stream), records, stride, n_vox, t, n_planes);
     return check_launch("pxl_grid_export_planes");
+}
+
+extern "C" int32_t pxl_records_to_16bit(const float *rec32, int32_t stride32, int64_t n_vox,
+                                        void *rec16, int32_t stride16, int32_t storage,
+                                        void *stream) {
+    clear_error();
+    PXL_REQUIRE(rec32 && rec16, "pxl_records_to_16bit: null argument");
+    PXL_REQUIRE(storage == PXL_STORE_BF16 || storage == PXL_STORE_F16,
+                "pxl_records_to_16bit: storage must be PXL_STORE_BF16 or PXL_STORE_F16");
+    PXL_REQUIRE(stride32 > 0 && stride16 > 0 && n_vox >= 0, "pxl_records_to_16bit: bad sizes");
+    if (n_vox == 0) return PXL_OK;
+    const int64_t total = n_vox * stride16;
+    hipLaunchKernelGGL(k_to16, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                       as_stream(stream), rec32, stride32, n_vox,
+                       reinterpret_cast<uint16_t *>(rec16), stride16, storage);
+    return check_launch("pxl_records_to_16bit");
 }

@@ -19,10 +19,11 @@ LIB_PATH = os.path.join(PKG_DIR, "libpxl.so")
 PXL_OK, PXL_EINVAL, PXL_ENOMEM, PXL_EHIP, PXL_ENAN = 0, 1, 2, 3, 4
 MARCH_FIXED, MARCH_REF_ADAPTIVE = 0, 1
 TV_NEIGHBOUR, TV_DISPARITY = 0, 1
+STORE_F32, STORE_BF16, STORE_F16 = 0, 1, 2
 
 # Every symbol include/pxl.h declares.
 EXPORTS = (
-    "pxl_last_error", "pxl_version", "pxl_camera_from_pose", "pxl_rays_init",
+    "pxl_last_error", "pxl_version", "pxl_abi_sizes", "pxl_camera_from_pose", "pxl_rays_init",
     "pxl_render_fwd", "pxl_render_fwd_trace", "pxl_render_bwd_rgb", "pxl_train_rays",
     "pxl_tv_scratch_bytes", "pxl_add_variance_loss", "pxl_adam_step",
     "pxl_build_occupancy", "pxl_sparse_index_scratch_bytes", "pxl_build_sparse_index",
@@ -30,7 +31,7 @@ EXPORTS = (
     "pxl_loss_reduce_f64", "pxl_count_touched", "pxl_trainer_create", "pxl_trainer_step",
     "pxl_trainer_time_render", "pxl_trainer_render_ms", "pxl_trainer_loss",
     "pxl_trainer_state", "pxl_trainer_destroy", "pxl_brick_words",
-    "pxl_build_brick_occupancy",
+    "pxl_build_brick_occupancy", "pxl_records_to_16bit",
 )
 
 
@@ -56,8 +57,14 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"libpxl.so not built at {LIB_PATH}; run __graft_entry__.build()")
-        _lib = ctypes.CDLL(LIB_PATH)
-        _declare(_lib)
+        L = ctypes.CDLL(LIB_PATH)
+        _declare(L)
+        sizes = (ctypes.c_int64 * 6)()
+        L.pxl_abi_sizes(sizes, 6)
+        mine = [ctypes.sizeof(c) for c in (Camera, BBox, RaysC, GridC, OptimC, TrainConfigC)]
+        if list(sizes) != mine:
+            raise ImportError(f"libpxl.so struct sizes {list(sizes)} != binding {mine}")
+        _lib = L
     return _lib
 
 
@@ -91,11 +98,12 @@ class GridC(ctypes.Structure):
                 ("sparse", _I32), ("n_vox", _I64), ("params", _P), ("occupancy", _P),
                 ("index_offsets", _P), ("in_cage", _P), ("compact_to_dense", _P),
                 ("brick_occupancy", _P), ("bbox", BBox), ("step", _F32),
-                ("march_mode", _I32)]
+                ("march_mode", _I32), ("storage", _I32), ("reserved", _I32)]
 
 
 class OptimC(ctypes.Structure):
-    _fields_ = [("grad", _P), ("m", _P), ("v", _P), ("adaptive_lr", _P)]
+    _fields_ = [("grad", _P), ("m", _P), ("v", _P), ("adaptive_lr", _P), ("master", _P),
+                ("stride", _I32), ("reserved", _I32)]
 
 
 class TrainConfigC(ctypes.Structure):
@@ -111,6 +119,7 @@ def _declare(L):
     sig = {
         "pxl_last_error": ([], ctypes.c_char_p),
         "pxl_version": ([], _I32),
+        "pxl_abi_sizes": ([_P, _I32], _I32),
         "pxl_camera_from_pose": ([_P, _P, _F64, _I32, _I32, _F64, _F64, _F64, _F64,
                                   ctypes.POINTER(Camera), _P], _I32),
         "pxl_rays_init": ([ctypes.POINTER(Camera), ctypes.POINTER(BBox), _I32, _I32, _P, _P, _P,
@@ -141,6 +150,7 @@ def _declare(L):
         "pxl_trainer_destroy": ([_P], None),
         "pxl_brick_words": ([_I32, _I32, _I32], _I64),
         "pxl_build_brick_occupancy": ([_P, _I32, _I32, _I32, _P, _P], _I32),
+        "pxl_records_to_16bit": ([_P, _I32, _I64, _P, _I32, _I32, _P], _I32),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -200,9 +210,14 @@ class Rays:
         return s
 
 
-def record_stride(S):
+def record_stride(S, storage=STORE_F32):
+    """Elements per record: 32 (128 B FP32 / 64 B 16-bit) up to 32
+    parameters, else rounded up to 4 (FP32) or 8 (16-bit) elements."""
     P = 1 + 3 * S
-    return 32 if P <= 32 else ((P + 3) // 4) * 4
+    if P <= 32:
+        return 32
+    a = 4 if storage == STORE_F32 else 8
+    return ((P + a - 1) // a) * a
 
 
 class Grid:
@@ -211,7 +226,7 @@ class Grid:
 
     def __init__(self, W, H, D, S, params, step=0.5, bbox=None, march_mode=MARCH_FIXED,
                  occupancy=None, index_offsets=None, in_cage=None, compact_to_dense=None,
-                 brick_occupancy=None):
+                 brick_occupancy=None, storage=STORE_F32):
         self.W, self.H, self.D, self.S = int(W), int(H), int(D), int(S)
         self.params = params
         self.stride = int(params.shape[1])
@@ -224,6 +239,9 @@ class Grid:
         self.in_cage = in_cage
         self.compact_to_dense = compact_to_dense
         self.brick_occupancy = brick_occupancy
+        self.storage = int(storage)
+        if self.storage != STORE_F32:
+            assert params.element_size() == 2, "16-bit storage needs a 2-byte tensor"
         self.sparse = occupancy is not None
 
     @property
@@ -246,21 +264,30 @@ class Grid:
         g.bbox = BBox(*self.bbox)
         g.step = self.step
         g.march_mode = self.march_mode
+        g.storage = self.storage
         return g
 
 
 class Optim:
-    """Gradient and Adam moments, same record layout as the grid."""
+    """Gradient and Adam moments (FP32 records).  For a 16-bit grid,
+    `master` (FP32 parameters that Adam updates) is created from the
+    given FP32 records when master_from is passed."""
 
-    def __init__(self, grid, adaptive_lr=None):
+    def __init__(self, grid, adaptive_lr=None, master_from=None):
         import torch
 
-        shape = (grid.n_vox, grid.stride)
+        stride = grid.stride if grid.storage == STORE_F32 else record_stride(grid.S)
+        shape = (grid.n_vox, stride)
         dev = grid.params.device
+        self.stride = stride
         self.grad = torch.zeros(shape, dtype=torch.float32, device=dev)
         self.m = torch.zeros(shape, dtype=torch.float32, device=dev)
         self.v = torch.zeros(shape, dtype=torch.float32, device=dev)
         self.adaptive_lr = adaptive_lr
+        self.master = None
+        if master_from is not None:
+            self.master = torch.zeros(shape, dtype=torch.float32, device=dev)
+            self.master[:, :master_from.shape[1]] = master_from
 
     def c(self):
         o = OptimC()
@@ -268,6 +295,8 @@ class Optim:
         o.m = self.m.data_ptr()
         o.v = self.v.data_ptr()
         o.adaptive_lr = None if self.adaptive_lr is None else self.adaptive_lr.data_ptr()
+        o.master = None if self.master is None else self.master.data_ptr()
+        o.stride = self.stride
         return o
 
 
@@ -403,6 +432,18 @@ def grid_export_planes(records, planes, stream=None):
     _check("pxl_grid_export_planes",
            lib().pxl_grid_export_planes(_ptr(records), records.shape[1], records.shape[0], arr,
                                         len(planes), _stream(stream)))
+
+
+def records_to_16bit(rec32, storage, stride16=None, stream=None):
+    """FP32 records -> new 16-bit records (round to nearest even)."""
+    import torch
+
+    stride16 = rec32.shape[1] if stride16 is None else stride16
+    out = torch.empty((rec32.shape[0], stride16), dtype=torch.int16, device=rec32.device)
+    _check("pxl_records_to_16bit",
+           lib().pxl_records_to_16bit(_ptr(rec32), rec32.shape[1], rec32.shape[0], _ptr(out),
+                                      stride16, storage, _stream(stream)))
+    return out
 
 
 def loss_reduce(losses, out, length=None, stream=None):

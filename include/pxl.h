@@ -45,6 +45,14 @@ enum {
     PXL_MARCH_REF_ADAPTIVE = 1
 };
 
+/* Storage of the grid parameters.  Interpolation, compositing, gradients
+ * and Adam moments are FP32/FP64 in every case. */
+enum {
+    PXL_STORE_F32 = 0,
+    PXL_STORE_BF16 = 1,   /* bfloat16, round to nearest even   */
+    PXL_STORE_F16 = 2     /* IEEE binary16, round to nearest even */
+};
+
 /* Variance-loss (TV) types of RenderUtils::addVarianceLoss
  * (RenderUtils.cu:2815-2851). */
 enum {
@@ -96,7 +104,8 @@ typedef struct pxl_rays {
 typedef struct pxl_grid {
     int32_t W, H, D;              /* vertices per axis                 */
     int32_t sh_dim;               /* S = (band+1)^2: 1, 4, 9 or 16     */
-    int32_t stride;               /* floats per record, multiple of 4, >= 1+3S */
+    int32_t stride;               /* elements per record, >= 1+3S; a multiple of 4
+                                     (FP32) or 8 (16-bit storage) */
     int32_t sparse;               /* 0 dense, 1 caged                  */
     int64_t n_vox;                /* number of records                 */
     float *params;                /* [n_vox][stride]                   */
@@ -108,14 +117,22 @@ typedef struct pxl_grid {
     pxl_bbox bbox;
     float step;                   /* marching step in voxels (StepSize) */
     int32_t march_mode;           /* PXL_MARCH_*                       */
+    int32_t storage;              /* PXL_STORE_*: element type of params */
+    int32_t reserved;
 } pxl_grid;
 
-/* Optimiser state, same record layout as the grid (Optimizer.cuh:5-37). */
+/* Optimiser state (Optimizer.cuh:5-37): FP32 records of `stride` floats
+ * (0 = the grid's stride, FP32 grids only).  With 16-bit grid storage,
+ * `master` (optional) holds the FP32 parameters Adam updates; the grid
+ * then receives their rounded copy. */
 typedef struct pxl_optim {
     float *grad;                  /* [n_vox][stride] gradient          */
     float *m;                     /* [n_vox][stride] Adam first moment */
     float *v;                     /* [n_vox][stride] Adam second moment*/
     const float *adaptive_lr;     /* dense only: per-vertex LR factor or NULL */
+    float *master;                /* 16-bit grids: FP32 parameters or NULL */
+    int32_t stride;               /* floats per record, multiple of 4  */
+    int32_t reserved;
 } pxl_optim;
 
 /* Last error message of the calling thread ("" if none). */
@@ -123,6 +140,10 @@ const char *pxl_last_error(void);
 
 /* Library version, (major << 16) | minor. */
 int32_t pxl_version(void);
+
+/* sizeof of pxl_camera, pxl_bbox, pxl_rays, pxl_grid, pxl_optim and
+ * pxl_train_config, for bindings to check their mirrors; returns 6. */
+int32_t pxl_abi_sizes(int64_t *out, int32_t n);
 
 /* Host-side camera of one pose (readFromMyBlender per view,
  * CameraUtils.cu:339-377; formulateProjectionMat :45-106; centre as in
@@ -245,6 +266,13 @@ int32_t pxl_grid_import_planes(const float *const *planes, int32_t n_planes,
 int32_t pxl_grid_export_planes(const float *records, int32_t stride,
                                int64_t n_vox, float *const *planes,
                                int32_t n_planes, void *stream);
+
+/* FP32 records (stride32 floats) -> 16-bit records (stride16 elements,
+ * storage PXL_STORE_BF16 or PXL_STORE_F16), round to nearest even; the
+ * first min(stride32, stride16) elements are converted, the rest of each
+ * 16-bit record is zeroed. */
+int32_t pxl_records_to_16bit(const float *rec32, int32_t stride32, int64_t n_vox,
+                             void *rec16, int32_t stride16, int32_t storage, void *stream);
 
 /* FP64 batch loss on the GPU: out[0] = mean(losses^2) (getMSEError,
  * Utils.cu:311-320), out[1] = 10 log10(65025 / out[0]) (Render.cu:466). */

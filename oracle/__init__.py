@@ -209,8 +209,8 @@ def render_fwd(grid, rays, sample_index, start=0, length=None, trace=False):
     return out
 
 
-def render_bwd(grid, rays, sample_index, fwd, start=0, length=None, mask_coef=1.0,
-               huber_delta=0.1, grad=None):
+def render_bwd(grid, rays, sample_index, fwd, mask_coef=1.0, huber_delta=0.1, start=0,
+               length=None, grad=None):
     """Returns (losses float64[len], grad float64[P, n_vox])."""
     si = _c(sample_index, np.int32)
     if length is None:

@@ -114,3 +114,21 @@ def grad_close(g, ref, rel=1e-4, floor=1e-3):
         ratio = (np.abs(g[p] - ref[p]) / tol).max()
         worst = max(worst, ratio)
     return worst
+
+
+def round_bf16(x):
+    """float32 -> bfloat16 bits (round to nearest even), as uint16."""
+    u = np.ascontiguousarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    return u.astype(np.uint16)
+
+
+def widen_bf16(h):
+    return (h.astype(np.uint32) << 16).view(np.float32)
+
+
+def to_16bit_planes(planes, storage):
+    """The values a 16-bit grid holds, widened back to float32 (exact)."""
+    if storage == pxl.STORE_BF16:
+        return widen_bf16(round_bf16(planes))
+    return planes.astype(np.float16).astype(np.float32)

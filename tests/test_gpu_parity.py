@@ -201,10 +201,12 @@ def _bwd_case(c1, grid, grid_o, nr, fused, huber=0.1):
         torch.cuda.synchronize()
         fwd = oracle.render_fwd(grid_o, rays_o, batch)
     else:
+        # Each side's backward reads its own forward result rounded to the
+        # FP32 ray planes, as the reference's separate kernels do.
         pxl.render_fwd(grid, rays, si)
         torch.cuda.synchronize()
-        fwd = {k_o: getattr(rays, k_g).cpu().numpy().astype(np.float64)
-               for k_g, k_o in (("acc_r", "r"), ("acc_g", "g"), ("acc_b", "b"), ("acc_w", "w"))}
+        fwd = oracle.render_fwd(grid_o, rays_o, batch)
+        fwd = {k: fwd[k].astype(np.float32).astype(np.float64) for k in ("r", "g", "b", "w")}
         pxl.render_bwd_rgb(grid, optim, rays, losses, si, mask_coef=1.0, huber_delta=huber)
         torch.cuda.synchronize()
     l_ref, g_ref = oracle.render_bwd(grid_o, rays_o, batch, fwd, mask_coef=1.0,
@@ -428,3 +430,86 @@ def test_trainer_matches_stage_calls(c1):
     # identical kernels; only the order of float atomics differs
     assert float((diff > 1e-4).float().mean()) <= 1e-4, float(diff.max())
     tr.close()
+
+
+# ------------------------------------------------------- 16-bit storage
+
+
+@pytest.mark.parametrize("storage", [pxl.STORE_BF16, pxl.STORE_F16])
+def test_render_16bit_storage(c1, storage):
+    """16-bit grids: the GPU result matches the FP64 oracle run on the
+    widened 16-bit values to the same bars as FP32; the storage error
+    itself (vs the FP32 grid) is reported."""
+    import torch
+
+    from helpers import to_16bit_planes
+
+    name = "bf16" if storage == pxl.STORE_BF16 else "f16"
+    say(f"[{name}] dense 128^3 forward + fused backward, 1024 rays")
+    dev = c1["dev"]
+    P = 1 + 3 * S1
+    rec16 = pxl.records_to_16bit(c1["rec"], storage)
+    grid = pxl.Grid(N1, N1, N1, S1, rec16, step=0.5, storage=storage)
+    planes16 = to_16bit_planes(c1["planes"], storage)
+    got16 = rec16[:, :P].T.contiguous().cpu().numpy()
+    if storage == pxl.STORE_BF16:
+        from helpers import widen_bf16
+        got16 = widen_bf16(got16.view(np.uint16))
+    else:
+        got16 = got16.view(np.float16).astype(np.float32)
+    assert np.array_equal(got16, planes16), "16-bit conversion differs from RNE"
+    rays = device_rays(c1["rays_np"], dev)
+    si = c1["si"][:1024]
+    batch = c1["batch"][:1024]
+    got, tr = _gpu_fwd(grid, rays, si, dev)
+    grid_o = oracle.Grid(N1, N1, N1, S1, planes16, 0.5)
+    ref = _fwd_compare(grid_o, c1["rays_np"], batch, got, tr)
+    ref32 = oracle.render_fwd(oracle.Grid(N1, N1, N1, S1, c1["planes"], 0.5),
+                              oracle.Rays(c1["rays_np"]), batch)
+    serr = max(np.abs(got[k][batch] - ref32[o][batch]).max()
+               for k, o in (("acc_r", "r"), ("acc_g", "g"), ("acc_b", "b")))
+    say(f"    pixel error vs FP32 grid (storage rounding): {serr:.3g}")
+    optim = pxl.Optim(grid)
+    losses = torch.zeros(1024, dtype=torch.float64, device=dev)
+    pxl.train_rays(grid, optim, rays, losses, si)
+    torch.cuda.synchronize()
+    _, g_ref = oracle.render_bwd(grid_o, oracle.Rays(c1["rays_np"]), batch,
+                                 oracle.render_fwd(grid_o, oracle.Rays(c1["rays_np"]), batch),
+                                 1.0, 0.1)
+    worst = grad_close(from_records(optim.grad, P), g_ref)
+    say(f"    worst gradient error / tolerance = {worst:.3g}")
+    assert worst <= 1.0
+    # TV on the 16-bit grid
+    optim.grad.zero_()
+    pxl.add_variance_loss(grid, optim, 1.0, 0.5, pxl.TV_DISPARITY, 6, 0)
+    torch.cuda.synchronize()
+    assert grad_close(from_records(optim.grad, P), oracle.tv_disparity(grid_o, 1.0, 0.5)) <= 1.0
+
+
+def test_adam_bf16_master(dev):
+    """Adam on a bfloat16 grid with an FP32 master copy: the master follows
+    the oracle's FP32 Adam; the grid holds its round-to-nearest-even copy."""
+    import torch
+
+    from helpers import round_bf16
+
+    say("[bf16] Adam with master copy")
+    rng = np.random.default_rng(12)
+    P, n = 28, 20_000
+    planes = rng.normal(size=(P, n)).astype(np.float32)
+    planes[0] = np.abs(planes[0])
+    rec32 = to_records(planes, dev)
+    rec16 = pxl.records_to_16bit(rec32, pxl.STORE_BF16)
+    grid = pxl.Grid(n, 1, 1, 9, rec16, storage=pxl.STORE_BF16)
+    optim = pxl.Optim(grid, master_from=rec32)
+    p_o, m, v = planes.copy(), np.zeros_like(planes), np.zeros_like(planes)
+    for t in range(1, 4):
+        g = rng.normal(size=(P, n)).astype(np.float32)
+        optim.grad[:, :P] = torch.from_numpy(g.T.copy()).to(dev)
+        pxl.adam_step(grid, optim, 0.1, 0.2, t)
+        oracle.adam(p_o, g.copy(), m, v, 0.1, 0.2, t)
+    torch.cuda.synchronize()
+    master = from_records(optim.master, P).astype(np.float32)
+    np.testing.assert_allclose(master, p_o, rtol=1e-6, atol=1e-7)
+    stored = rec16[:, :P].T.contiguous().cpu().numpy().view(np.uint16)
+    assert np.array_equal(stored, round_bf16(master))

@@ -28,6 +28,7 @@ def test_exports_every_declared_symbol(pxl_lib):
 
 def test_version_and_error_string(pxl_lib):
     assert pxl_lib.lib().pxl_version() >> 16 == 1
+    assert pxl_lib.lib().pxl_abi_sizes(None, 0) == 6
     assert isinstance(pxl_lib.lib().pxl_last_error(), bytes)
 
 
@@ -36,8 +37,8 @@ def test_struct_layouts_match_header(pxl_lib):
     assert ctypes.sizeof(pxl_lib.Camera) == 24 * 8
     assert ctypes.sizeof(pxl_lib.BBox) == 24
     assert ctypes.sizeof(pxl_lib.RaysC) == 17 * 8 + 8 + 8
-    assert ctypes.sizeof(pxl_lib.GridC) == 24 + 8 + 6 * 8 + 24 + 8
-    assert ctypes.sizeof(pxl_lib.OptimC) == 32
+    assert ctypes.sizeof(pxl_lib.GridC) == 24 + 8 + 6 * 8 + 24 + 16
+    assert ctypes.sizeof(pxl_lib.OptimC) == 48
 
 
 def test_argument_validation_without_device(pxl_lib):
