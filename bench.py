@@ -48,6 +48,9 @@ def parse():
     ap.add_argument("--cpu-rays", type=int, default=1024,
                     help="rays of the bounded CPU-baseline sample (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batch-order", choices=("shuffled", "sorted"), default="sorted",
+                    help="order of the rays inside each batch (the batches themselves are "
+                         "the same consecutive chunks of one seeded shuffle)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="run one device-to-device copy of this many MiB first (a kernel of "
                          "known traffic for calibrating profiler byte counters)")
@@ -147,6 +150,13 @@ class Workload:
         self.rays = rays
         gen.manual_seed(3)
         self.order = torch.randperm(self.n_rays, generator=gen, device=dev).to(torch.int32)
+        if args.batch_order == "sorted":
+            # Same batches (consecutive chunks of the shuffle), rays inside
+            # each batch in ray-index order: a view's rays become neighbours.
+            B = args.batch
+            full = (self.n_rays // B) * B
+            head = self.order[:full].view(-1, B).sort(dim=1).values.reshape(-1)
+            self.order = torch.cat([head, self.order[full:].sort().values])
         torch.cuda.synchronize()
         log(f"rays {self.n_rays} built with targets ({time.time() - t0:.1f}s)")
 
@@ -304,7 +314,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"plain FP32, {args.grid}^3 SH-2 caged grid at "
+        "config": {"batch_order": args.batch_order,
+                   "workload": f"plain FP32, {args.grid}^3 SH-2 caged grid at "
                                f"{w.occ_frac:.4f} cell occupancy, {args.views} views "
                                f"{args.img}x{args.img}, {args.batch} rays/batch, step 0.5",
                    "global_batch": args.batch, "caged_records": w.n_vox,

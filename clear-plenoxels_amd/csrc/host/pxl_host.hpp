@@ -31,7 +31,8 @@ struct RenderUtils {
     static int32_t addVarianceLoss(const pxl_grid &scene, const pxl_optim &optimizer,
                                    double densityCoefficient, double shCoefficient,
                                    int32_t varianceType, int32_t neighbourNums,
-                                   int32_t optimizeStep, hipStream_t stream);
+                                   int32_t optimizeStep, void *scratch, size_t scratchBytes,
+                                   hipStream_t stream);
 };
 
 // Optimizer::updateRGBScene (Optimizer.cuh:35-37).

@@ -31,9 +31,11 @@ int32_t RenderUtils::renderRays_withGradients_RGB(const pxl_optim &optimizer,
 int32_t RenderUtils::addVarianceLoss(const pxl_grid &scene, const pxl_optim &optimizer,
                                      double densityCoefficient, double shCoefficient,
                                      int32_t varianceType, int32_t neighbourNums,
-                                     int32_t optimizeStep, hipStream_t stream) {
+                                     int32_t optimizeStep, void *scratch, size_t scratchBytes,
+                                     hipStream_t stream) {
     return pxl_add_variance_loss(&scene, &optimizer, densityCoefficient, shCoefficient,
-                                 varianceType, neighbourNums, optimizeStep, nullptr, 0, stream);
+                                 varianceType, neighbourNums, optimizeStep, scratch,
+                                 scratchBytes, stream);
 }
 
 int32_t Optimizer::updateRGBScene(const pxl_optim &optimizer, const pxl_grid &scene,
@@ -54,6 +56,8 @@ struct pxl_trainer {
     int64_t n_rays;
     double *losses;
     double *loss_out;
+    void *tv_scratch;
+    size_t tv_scratch_bytes;
     hipStream_t stream;
     int64_t batch_count;
     int64_t t;
@@ -69,7 +73,8 @@ using namespace pxl::host;
 extern "C" pxl_trainer *pxl_trainer_create(const pxl_train_config *cfg, const pxl_grid *grid,
                                            const pxl_optim *optim, const pxl_rays *rays,
                                            const int32_t *order, int64_t n_rays,
-                                           double *losses, double *loss_out, void *stream) {
+                                           double *losses, double *loss_out, void *tv_scratch,
+                                           size_t tv_scratch_bytes, void *stream) {
     clear_error();
     if (!cfg || !grid || !optim || !rays || !order || !losses || !loss_out) {
         set_error("pxl_trainer_create: null argument");
@@ -94,6 +99,8 @@ extern "C" pxl_trainer *pxl_trainer_create(const pxl_train_config *cfg, const px
     tr->losses = losses;
     tr->loss_out = loss_out;
     tr->stream = as_stream(stream);
+    tr->tv_scratch = tv_scratch;
+    tr->tv_scratch_bytes = tv_scratch_bytes;
     tr->batch_count = (n_rays + cfg->batch_size - 1) / cfg->batch_size;
     tr->t = 0;
     tr->lr = cfg->learning_rate;
@@ -153,7 +160,8 @@ extern "C" int32_t pxl_trainer_step(pxl_trainer *tr) {
     if (!tr->grid.sparse || b % c.variance_interval == 0) {
         st = RenderUtils::addVarianceLoss(tr->grid, tr->optim, c.variance_coefficient,
                                           c.sh_coefficient, c.variance_type, c.neighbour_nums,
-                                          c.optimize_step, tr->stream);
+                                          c.optimize_step, tr->tv_scratch,
+                                          tr->tv_scratch_bytes, tr->stream);
         if (st != PXL_OK) return st;
     }
     const int32_t time_step = (int32_t)(epoch * tr->batch_count + b + 1);

@@ -92,43 +92,95 @@ __global__ __launch_bounds__(256) void k_tv_disparity(const TvArgs a) {
 
 // Caged grids: a cell corner that is not in cage reads and writes record 0
 // (index_offsets = 0, SceneUtils.cu:1106-1113).  Those (cell, corner) pairs
-// are not found by the gather above; this kernel adds them into record 0.
-// One thread per cell; waves without such a cell exit after one ballot.
-template <int STORE>
-__global__ __launch_bounds__(256) void k_tv_alias0(const TvArgs a, int64_t ncell) {
+// are not found by the gather above.  Their sum into record 0 is
+//   lambda * (N * u_0 - sum_c n_c * mu_c) = lambda * (N * u_0 - sum_r w_r * u_r)
+// where n_c counts the aliased corners of occupied cell c, N = sum_c n_c and
+// w_r = sum over (c, corner j) with record(j) = r of n_c / 8.  N and w depend
+// only on the cage, so pxl_tv_prepare computes them once; each TV call then
+// needs one weighted reduction over the records with w_r != 0.
+struct AliasHeader {
+    uint32_t magic;
+    uint32_t n_alias;
+    int64_t n_vox;
+};
+constexpr uint32_t kAliasMagic = 0x70786131u;
+constexpr int kAliasBlocks = 256;
+constexpr size_t kAliasHdrBytes = 256;
+
+__host__ __device__ inline size_t alias_weights_off() { return kAliasHdrBytes; }
+__host__ __device__ inline size_t alias_partials_off(int64_t n_vox) {
+    return (kAliasHdrBytes + (size_t)n_vox * 4 + 255) / 256 * 256;
+}
+
+__global__ __launch_bounds__(256) void k_alias_weights(const TvArgs a, int64_t ncell,
+                                                       AliasHeader *hdr, float *w) {
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int n_alias = 0;
-    uint32_t rec[8];
+    if (c >= ncell) return;
+    if (!((a.occ[c >> 5] >> (c & 31)) & 1u)) return;
     const uint32_t d0 = a.c2d[0];
-    if (c < ncell) {
-        const int64_t cw = a.W - 1, ch = a.H - 1;
-        const int x = (int)(c % cw), y = (int)((c / cw) % ch), z = (int)(c / (cw * ch));
-        if ((a.occ[c >> 5] >> (c & 31)) & 1u) {
+    const int64_t cw = a.W - 1, ch = a.H - 1;
+    const int x = (int)(c % cw), y = (int)((c / cw) % ch), z = (int)(c / (cw * ch));
+    uint32_t rec[8];
+    int n = 0;
 #pragma unroll
-            for (int j = 0; j < 8; j++) {
-                const int jx = j & 1, jy = (j >> 1) & 1, jz = j >> 2;
-                const uint32_t d = (uint32_t)(z + jz) * (uint32_t)a.W * (uint32_t)a.H +
-                                   (uint32_t)(y + jy) * (uint32_t)a.W + (uint32_t)(x + jx);
-                rec[j] = a.io[d];
-                if (rec[j] == 0u && d != d0) n_alias++;
-            }
+    for (int j = 0; j < 8; j++) {
+        const int jx = j & 1, jy = (j >> 1) & 1, jz = j >> 2;
+        const uint32_t d = (uint32_t)(z + jz) * (uint32_t)a.W * (uint32_t)a.H +
+                           (uint32_t)(y + jy) * (uint32_t)a.W + (uint32_t)(x + jx);
+        rec[j] = a.io[d];
+        if (rec[j] == 0u && d != d0) n++;
+    }
+    if (n == 0) return;
+#pragma unroll
+    for (int j = 0; j < 8; j++) atomicAdd(w + rec[j], (float)n * 0.125f);
+    atomicAdd(&hdr->n_alias, (uint32_t)n);
+}
+
+// Per-block partial sums of w_r * u_r[p]; LPV lanes per record, one per
+// parameter.
+template <int LPV, int STORE>
+__global__ __launch_bounds__(256) void k_alias_partial(const TvArgs a, const float *w,
+                                                       double *partials) {
+    constexpr int kGroups = 256 / LPV;
+    __shared__ double red[256];
+    const int g = threadIdx.x / LPV, p = threadIdx.x % LPV;
+    double acc = 0.0;
+    if (p < a.P) {
+        for (int64_t r = (int64_t)blockIdx.x * kGroups + g; r < a.n_vox;
+             r += (int64_t)gridDim.x * kGroups) {
+            const float wr = w[r];
+            if (wr != 0.0f) acc += (double)wr * (double)ld<STORE>(a.params, (size_t)r * a.stride + p);
         }
     }
-    if (__ballot(n_alias > 0) == 0) return;
-    for (int p = 0; p < a.P; p++) {
-        const float lam = p == 0 ? a.lam_d : a.lam_sh;
-        if (lam == 0.0f) continue;
-        double contrib = 0.0;
-        if (n_alias > 0) {
-            double s = 0.0;
-#pragma unroll
-            for (int j = 0; j < 8; j++)
-                s += (double)ld<STORE>(a.params, (size_t)rec[j] * (size_t)a.stride + p);
-            contrib = (double)n_alias * (double)lam *
-                      ((double)ld<STORE>(a.params, (size_t)p) - s * 0.125);
-        }
-        contrib = wave_sum_d(contrib);
-        if ((threadIdx.x & 63) == 0 && contrib != 0.0) unsafeAtomicAdd(a.grad + p, (float)contrib);
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    if (threadIdx.x < LPV) {
+        double t = 0.0;
+        for (int i = 0; i < kGroups; i++) t += red[i * LPV + threadIdx.x];
+        partials[(size_t)blockIdx.x * 64 + threadIdx.x] = t;
+    }
+}
+
+template <int STORE>
+__global__ void k_alias_final(const TvArgs a, const AliasHeader *hdr, const double *partials,
+                              int nblocks) {
+    const int p = threadIdx.x;
+    if (p >= a.P) return;
+    const float lam = p == 0 ? a.lam_d : a.lam_sh;
+    if (lam == 0.0f || hdr->magic != kAliasMagic || hdr->n_vox != a.n_vox) return;
+    double s = 0.0;
+    for (int b = 0; b < nblocks; b++) s += partials[(size_t)b * 64 + p];
+    const double u0 = (double)ld<STORE>(a.params, (size_t)p);
+    a.grad[p] += (float)((double)lam * ((double)hdr->n_alias * u0 - s));
+}
+
+__global__ void k_alias_init(AliasHeader *hdr, float *w, int64_t n_vox) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n_vox) w[i] = 0.0f;
+    if (i == 0) {
+        hdr->magic = kAliasMagic;
+        hdr->n_alias = 0u;
+        hdr->n_vox = n_vox;
     }
 }
 
@@ -274,7 +326,7 @@ int32_t tv_check(const pxl_grid *g, const pxl_optim *o, const char *who) {
 }
 
 template <int STORE>
-void tv_launch(const TvArgs &a, int type, hipStream_t s) {
+void tv_launch(const TvArgs &a, int type, void *scratch, hipStream_t s) {
     const bool wide = a.P > 32;
     const int groups = wide ? 4 : 8;
     const unsigned blocks = (unsigned)((a.n_vox + groups - 1) / groups);
@@ -282,9 +334,18 @@ void tv_launch(const TvArgs &a, int type, hipStream_t s) {
         if (wide) hipLaunchKernelGGL((k_tv_disparity<64, STORE>), dim3(blocks), dim3(256), 0, s, a);
         else hipLaunchKernelGGL((k_tv_disparity<32, STORE>), dim3(blocks), dim3(256), 0, s, a);
         if (a.sparse) {
-            const int64_t ncell = (int64_t)(a.W - 1) * (a.H - 1) * (a.D - 1);
-            hipLaunchKernelGGL(k_tv_alias0<STORE>, dim3((unsigned)((ncell + 255) / 256)),
-                               dim3(256), 0, s, a, ncell);
+            char *base = reinterpret_cast<char *>(scratch);
+            const AliasHeader *hdr = reinterpret_cast<const AliasHeader *>(base);
+            const float *w = reinterpret_cast<const float *>(base + alias_weights_off());
+            double *part = reinterpret_cast<double *>(base + alias_partials_off(a.n_vox));
+            if (wide)
+                hipLaunchKernelGGL((k_alias_partial<64, STORE>), dim3(kAliasBlocks), dim3(256), 0,
+                                   s, a, w, part);
+            else
+                hipLaunchKernelGGL((k_alias_partial<32, STORE>), dim3(kAliasBlocks), dim3(256), 0,
+                                   s, a, w, part);
+            hipLaunchKernelGGL(k_alias_final<STORE>, dim3(1), dim3(64), 0, s, a, hdr, part,
+                               kAliasBlocks);
         }
     } else {
         if (wide) hipLaunchKernelGGL((k_tv_neighbour6<64, STORE>), dim3(blocks), dim3(256), 0, s, a);
@@ -298,25 +359,14 @@ void tv_launch(const TvArgs &a, int type, hipStream_t s) {
 using namespace pxl;
 
 extern "C" size_t pxl_tv_scratch_bytes(const pxl_grid *grid) {
-    (void)grid;
-    return 0;
+    if (!grid || !grid->sparse || grid->n_vox <= 0) return 0;
+    return alias_partials_off(grid->n_vox) + (size_t)kAliasBlocks * 64 * sizeof(double);
 }
 
-extern "C" int32_t pxl_add_variance_loss(const pxl_grid *grid, const pxl_optim *optim,
-                                         double lambda_density, double lambda_sh, int32_t type,
-                                         int32_t neighbours, int32_t optimize_step,
-                                         void *scratch, size_t scratch_bytes, void *stream) {
-    (void)scratch;
-    (void)scratch_bytes;
-    clear_error();
-    int32_t st = tv_check(grid, optim, "pxl_add_variance_loss");
-    if (st != PXL_OK) return st;
-    PXL_REQUIRE(type == PXL_TV_DISPARITY || type == PXL_TV_NEIGHBOUR,
-                "pxl_add_variance_loss: unsupported type %d", type);
-    TvArgs a;
+static void fill_tv_args(const pxl_grid *grid, const pxl_optim *optim, TvArgs &a) {
     a.params = grid->params;
-    a.grad = optim->grad;
-    a.gstride = optim->stride > 0 ? optim->stride : grid->stride;
+    a.grad = optim ? optim->grad : nullptr;
+    a.gstride = optim && optim->stride > 0 ? optim->stride : grid->stride;
     a.occ = grid->occupancy;
     a.io = grid->index_offsets;
     a.c2d = grid->compact_to_dense;
@@ -327,6 +377,55 @@ extern "C" int32_t pxl_add_variance_loss(const pxl_grid *grid, const pxl_optim *
     a.P = 1 + 3 * grid->sh_dim;
     a.sparse = grid->sparse;
     a.n_vox = grid->n_vox;
+    a.lam_d = 0.0f;
+    a.lam_sh = 0.0f;
+}
+
+extern "C" int32_t pxl_tv_prepare(const pxl_grid *grid, void *scratch, size_t scratch_bytes,
+                                  void *stream) {
+    clear_error();
+    PXL_REQUIRE(grid && grid->sparse && grid->occupancy && grid->index_offsets &&
+                    grid->compact_to_dense && grid->n_vox > 0,
+                "pxl_tv_prepare: needs a non-empty sparse grid");
+    const size_t need = pxl_tv_scratch_bytes(grid);
+    if (!scratch || scratch_bytes < need || ((uintptr_t)scratch & 255)) {
+        set_error("pxl_tv_prepare: scratch of %zu bytes (256-byte aligned) needed, got %zu",
+                  need, scratch_bytes);
+        return PXL_ENOMEM;
+    }
+    TvArgs a;
+    fill_tv_args(grid, nullptr, a);
+    char *base = reinterpret_cast<char *>(scratch);
+    AliasHeader *hdr = reinterpret_cast<AliasHeader *>(base);
+    float *w = reinterpret_cast<float *>(base + alias_weights_off());
+    hipStream_t s = as_stream(stream);
+    hipLaunchKernelGGL(k_alias_init, dim3((unsigned)((a.n_vox + 255) / 256)), dim3(256), 0, s,
+                       hdr, w, a.n_vox);
+    const int64_t ncell = (int64_t)(a.W - 1) * (a.H - 1) * (a.D - 1);
+    hipLaunchKernelGGL(k_alias_weights, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, s, a,
+                       ncell, hdr, w);
+    return check_launch("pxl_tv_prepare");
+}
+
+extern "C" int32_t pxl_add_variance_loss(const pxl_grid *grid, const pxl_optim *optim,
+                                         double lambda_density, double lambda_sh, int32_t type,
+                                         int32_t neighbours, int32_t optimize_step,
+                                         void *scratch, size_t scratch_bytes, void *stream) {
+    clear_error();
+    int32_t st = tv_check(grid, optim, "pxl_add_variance_loss");
+    if (st != PXL_OK) return st;
+    PXL_REQUIRE(type == PXL_TV_DISPARITY || type == PXL_TV_NEIGHBOUR,
+                "pxl_add_variance_loss: unsupported type %d", type);
+    if (type == PXL_TV_DISPARITY && grid->sparse) {
+        const size_t need = pxl_tv_scratch_bytes(grid);
+        if (!scratch || scratch_bytes < need || ((uintptr_t)scratch & 255)) {
+            set_error("pxl_add_variance_loss: sparse grids need %zu bytes of scratch prepared "
+                      "by pxl_tv_prepare, got %zu", need, scratch_bytes);
+            return PXL_ENOMEM;
+        }
+    }
+    TvArgs a;
+    fill_tv_args(grid, optim, a);
     a.lam_d = (float)lambda_density;
     hipStream_t s = as_stream(stream);
     if (type == PXL_TV_NEIGHBOUR) {
@@ -335,9 +434,9 @@ extern "C" int32_t pxl_add_variance_loss(const pxl_grid *grid, const pxl_optim *
     }
     a.lam_sh = (type == PXL_TV_NEIGHBOUR || optimize_step <= 1) ? (float)lambda_sh : 0.0f;
     switch (grid->storage) {
-        case PXL_STORE_F32: tv_launch<PXL_STORE_F32>(a, type, s); break;
-        case PXL_STORE_BF16: tv_launch<PXL_STORE_BF16>(a, type, s); break;
-        default: tv_launch<PXL_STORE_F16>(a, type, s); break;
+        case PXL_STORE_F32: tv_launch<PXL_STORE_F32>(a, type, scratch, s); break;
+        case PXL_STORE_BF16: tv_launch<PXL_STORE_BF16>(a, type, scratch, s); break;
+        default: tv_launch<PXL_STORE_F16>(a, type, scratch, s); break;
     }
     return check_launch("pxl_add_variance_loss");
 }

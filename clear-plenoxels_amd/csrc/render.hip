@@ -15,6 +15,7 @@
 // registers while consecutive samples stay in the same cell and flushed
 // with one coalesced global_atomic_add_f32 per corner (instead of
 // 8 x (1+3S) scattered atomicAdds per sample, RenderUtils.cu:1074-1364).
+#include <cstdlib>
 #include <type_traits>
 
 #include "common.hpp"
@@ -215,7 +216,8 @@ __device__ __forceinline__ void interpolate(const RenderArgs &a, const Sample<R>
     if (p >= 0) {
 #pragma unroll
         for (int j = 0; j < 8; j++)
-            acc += s.w[j] * (R)load_param<STORE>(a.params, (size_t)s.v[j] * (size_t)a.stride + p);
+            acc = fma(s.w[j], (R)load_param<STORE>(a.params, (size_t)s.v[j] * (size_t)a.stride + p),
+                      acc);
     }
     R val = acc * ylane;
     val = row_sum<R>(val);
@@ -233,11 +235,24 @@ __device__ __forceinline__ uint32_t fnv_mix(uint32_t h, uint32_t x) {
     return h;
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (block i -> XCD
+// i mod 8).  Give XCD x a contiguous slice of the batch instead, so rays that
+// are neighbours in the batch order share one XCD's L2 for their gathers and
+// gradient atomics.  A bijection on [0, gridDim.x).
+__device__ __forceinline__ unsigned xcd_slice_block(unsigned blk, unsigned nb) {
+    constexpr unsigned kXcd = 8;
+    if (nb < kXcd) return blk;
+    const unsigned per = nb / kXcd, rem = nb % kXcd;
+    const unsigned x = blk % kXcd, idx = blk / kXcd;
+    return x < rem ? x * (per + 1) + idx : rem * (per + 1) + (x - rem) * per + idx;
+}
+
 template <int S, bool SPARSE, bool ADAPT, int MODE, int STORE>
 __global__ __launch_bounds__(256) void k_render(const RenderArgs a) {
     using R = typename std::conditional<ADAPT, double, float>::type;
     const int lane = threadIdx.x & 63;
-    const int64_t b = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int64_t b =
+        (int64_t)xcd_slice_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
     if (b >= a.len) return;
     const int ray = uniform_i(a.sample_index[a.start + b]);
     if (ray < 0 || (int64_t)ray >= a.rays.n) {
@@ -453,10 +468,30 @@ __global__ __launch_bounds__(256) void k_render(const RenderArgs a) {
         const float coef = (float)(row == 3 ? gs : gch * (double)ylane);
 
         if (s.bx != cbx || s.by != cby || s.bz != cbz) {
-            if (cbx >= 0 && active) {
+            // New cell.  Corners it shares with the previous cell (same
+            // vertex coordinates) keep accumulating; the others are flushed
+            // with one coalesced atomic per corner.  The shift (db) is
+            // wave-uniform, so every branch here is uniform.
+            const int dbx = s.bx - cbx, dby = s.by - cby, dbz = s.bz - cbz;
+            float nacc[8];
 #pragma unroll
-                for (int j = 0; j < 8; j++)
-                    unsafeAtomicAdd(gbase + (size_t)cv[j] * stride, acc[j]);
+            for (int j = 0; j < 8; j++) {
+                const int ix = (j & 1) + dbx, iy = ((j >> 1) & 1) + dby, iz = (j >> 2) + dbz;
+                const bool ok = cbx >= 0 && (unsigned)ix < 2u && (unsigned)iy < 2u &&
+                                (unsigned)iz < 2u;
+                const int src = iz * 4 + iy * 2 + ix;
+                float v = 0.0f;
+#pragma unroll
+                for (int i = 0; i < 8; i++) v = (ok && src == i) ? acc[i] : v;
+                nacc[j] = v;
+            }
+            if (cbx >= 0) {
+#pragma unroll
+                for (int i = 0; i < 8; i++) {
+                    const int jx = (i & 1) - dbx, jy = ((i >> 1) & 1) - dby, jz = (i >> 2) - dbz;
+                    const bool kept = (unsigned)jx < 2u && (unsigned)jy < 2u && (unsigned)jz < 2u;
+                    if (!kept && active) unsafeAtomicAdd(gbase + (size_t)cv[i] * stride, acc[i]);
+                }
             }
             cbx = s.bx;
             cby = s.by;
@@ -464,7 +499,7 @@ __global__ __launch_bounds__(256) void k_render(const RenderArgs a) {
 #pragma unroll
             for (int j = 0; j < 8; j++) {
                 cv[j] = s.v[j];
-                acc[j] = 0.0f;
+                acc[j] = nacc[j];
             }
         }
 #pragma unroll
@@ -480,9 +515,27 @@ __global__ __launch_bounds__(256) void k_render(const RenderArgs a) {
     }
 }
 
+#include "march8.hpp"
+
+// Marcher selection: the eight-samples-per-wave kernel serves the fixed rule
+// for S <= 9 in the forward, backward, fused and trace modes; the one-sample
+// kernel serves the adaptive rule, S = 16 and the touched-record count.
+// PXL_MARCHER=wave forces the one-sample kernel (both are tested).
+bool use_march8() {
+    const char *e = getenv("PXL_MARCHER");
+    return !(e && strcmp(e, "wave") == 0);
+}
+
 template <int S, bool SPARSE, bool ADAPT, int MODE, int STORE>
 int32_t launch3(const RenderArgs &a, hipStream_t st) {
     const unsigned blocks = (unsigned)((a.len + 3) / 4);
+    constexpr bool kGroup = !ADAPT && S <= 9 && MODE != kTouch;
+    if (kGroup && use_march8()) {
+        hipLaunchKernelGGL((k_march8<(S <= 9 ? S : 9), SPARSE, (MODE == kTouch ? kFwd : MODE),
+                                     STORE>),
+                           dim3(blocks), dim3(256), 0, st, a);
+        return PXL_OK;
+    }
     hipLaunchKernelGGL((k_render<S, SPARSE, ADAPT, MODE, STORE>), dim3(blocks), dim3(256), 0, st,
                        a);
     return PXL_OK;
@@ -543,6 +596,7 @@ int32_t check_grid(const pxl_grid *g, const char *who) {
                 "%s: stride %d must be a multiple of %d and >= %d", who, g->stride, align,
                 1 + 3 * g->sh_dim);
     PXL_REQUIRE(g->params != nullptr && g->n_vox > 0, "%s: empty grid", who);
+    PXL_REQUIRE(((uintptr_t)g->params & 15) == 0, "%s: params must be 16-byte aligned", who);
     PXL_REQUIRE(g->sparse || g->n_vox == (int64_t)g->W * g->H * g->D,
                 "%s: dense grid needs n_vox = W*H*D", who);
     PXL_REQUIRE(!g->sparse || (g->occupancy && g->index_offsets),

@@ -31,7 +31,7 @@ EXPORTS = (
     "pxl_loss_reduce_f64", "pxl_count_touched", "pxl_trainer_create", "pxl_trainer_step",
     "pxl_trainer_time_render", "pxl_trainer_render_ms", "pxl_trainer_loss",
     "pxl_trainer_state", "pxl_trainer_destroy", "pxl_brick_words",
-    "pxl_build_brick_occupancy", "pxl_records_to_16bit",
+    "pxl_build_brick_occupancy", "pxl_records_to_16bit", "pxl_tv_prepare",
 )
 
 
@@ -139,8 +139,9 @@ def _declare(L):
         "pxl_grid_export_planes": ([_P, _I32, _I64, _P, _I32, _P], _I32),
         "pxl_loss_reduce_f64": ([_P, _I64, _P, _P], _I32),
         "pxl_count_touched": ([G, R, _P, _I64, _I64, _P, _P, _P], _I32),
-        "pxl_trainer_create": ([ctypes.POINTER(TrainConfigC), G, O, R, _P, _I64, _P, _P, _P],
-                               _P),
+        "pxl_trainer_create": ([ctypes.POINTER(TrainConfigC), G, O, R, _P, _I64, _P, _P, _P,
+                                _SZ, _P], _P),
+        "pxl_tv_prepare": ([G, _P, _SZ, _P], _I32),
         "pxl_trainer_step": ([_P], _I32),
         "pxl_trainer_time_render": ([_P, _I32], _I32),
         "pxl_trainer_render_ms": ([_P, ctypes.POINTER(_F64), ctypes.POINTER(_I64)], _I32),
@@ -363,14 +364,33 @@ def train_rays(grid, optim, rays, losses, sample_index, start=0, length=None, ma
                                 _stream(stream)))
 
 
+def tv_scratch(grid, stream=None):
+    """The grid's prepared variance-loss scratch (sparse grids; created and
+    filled by pxl_tv_prepare on first use, kept on the Grid object)."""
+    import torch
+
+    if not grid.sparse:
+        return None
+    if getattr(grid, "_tv_scratch", None) is None:
+        g = grid.c()
+        nbytes = lib().pxl_tv_scratch_bytes(ctypes.byref(g))
+        buf = torch.zeros(nbytes // 8 + 64, dtype=torch.float64, device=grid.params.device)
+        _check("pxl_tv_prepare", lib().pxl_tv_prepare(ctypes.byref(g), _ptr(buf), nbytes,
+                                                      _stream(stream)))
+        grid._tv_scratch = (buf, nbytes)
+    return grid._tv_scratch
+
+
 def add_variance_loss(grid, optim, lambda_density, lambda_sh, tv_type=TV_DISPARITY,
                       neighbours=6, optimize_step=0, stream=None):
     """RenderUtils::addVarianceLoss (RenderUtils.cu:2815-2851)."""
     g, o = grid.c(), optim.c()
+    sc = tv_scratch(grid, stream) if tv_type == TV_DISPARITY else None
+    buf, nbytes = (None, 0) if sc is None else sc
     _check("pxl_add_variance_loss",
            lib().pxl_add_variance_loss(ctypes.byref(g), ctypes.byref(o), lambda_density,
-                                       lambda_sh, tv_type, neighbours, optimize_step, None, 0,
-                                       _stream(stream)))
+                                       lambda_sh, tv_type, neighbours, optimize_step,
+                                       _ptr(buf), nbytes, _stream(stream)))
 
 
 def adam_step(grid, optim, lr, sh_lr, t, stream=None):
@@ -482,10 +502,12 @@ class Trainer:
                            optimize_step, mask_coefficient, huber_delta, int(fused))
         self._g, self._o, self._r = grid.c(), optim.c(), rays.c()
         n_rays = order.numel() if n_rays is None else n_rays
+        sc = tv_scratch(grid, stream) if variance_type == TV_DISPARITY else None
+        buf, nbytes = (None, 0) if sc is None else sc
         h = lib().pxl_trainer_create(ctypes.byref(cfg), ctypes.byref(self._g),
                                      ctypes.byref(self._o), ctypes.byref(self._r), _ptr(order),
-                                     n_rays, _ptr(self.losses), _ptr(self.loss_out),
-                                     _stream(stream))
+                                     n_rays, _ptr(self.losses), _ptr(self.loss_out), _ptr(buf),
+                                     nbytes, _stream(stream))
         if not h:
             raise PxlError("pxl_trainer_create", PXL_EINVAL, lib().pxl_last_error().decode())
         self._h = ctypes.c_void_p(h)

@@ -208,9 +208,14 @@ int32_t pxl_train_rays(const pxl_grid *grid, const pxl_optim *optim,
  *   optimize_step <= 1 (RenderUtils.cu:2493).
  * type PXL_TV_NEIGHBOUR: totalVarianceLoss 6-neighbour branch
  *   (RenderUtils.cu:2660-2763); neighbours must be 6.
- * scratch: caller-provided device memory of pxl_tv_scratch_bytes() bytes
- *   (may be NULL when that is 0). */
+ * scratch: sparse grids only - pxl_tv_scratch_bytes() bytes of device
+ *   memory, 256-byte aligned, filled once by pxl_tv_prepare for the grid's
+ *   cage (it holds the per-record weights of the corners that alias record
+ *   0); NULL / 0 for dense grids.  Calls that share one scratch on one
+ *   stream are safe; calls on different streams must not share it. */
 size_t pxl_tv_scratch_bytes(const pxl_grid *grid);
+int32_t pxl_tv_prepare(const pxl_grid *grid, void *scratch, size_t scratch_bytes,
+                       void *stream);
 int32_t pxl_add_variance_loss(const pxl_grid *grid, const pxl_optim *optim,
                               double lambda_density, double lambda_sh,
                               int32_t type, int32_t neighbours,
@@ -315,12 +320,15 @@ typedef struct pxl_train_config {
 typedef struct pxl_trainer pxl_trainer;
 
 /* Trainer over caller-owned buffers: rays order[0 .. n_rays) are visited in
- * batches of batch_size; losses holds batch_size doubles, loss_out 2.
- * Returns NULL on a bad argument (see pxl_last_error). */
+ * batches of batch_size; losses holds batch_size doubles, loss_out 2;
+ * tv_scratch as for pxl_add_variance_loss (already prepared by
+ * pxl_tv_prepare for sparse grids).  Returns NULL on a bad argument (see
+ * pxl_last_error). */
 pxl_trainer *pxl_trainer_create(const pxl_train_config *cfg, const pxl_grid *grid,
                                 const pxl_optim *optim, const pxl_rays *rays,
                                 const int32_t *order, int64_t n_rays, double *losses,
-                                double *loss_out, void *stream);
+                                double *loss_out, void *tv_scratch, size_t tv_scratch_bytes,
+                                void *stream);
 
 /* One batch: render + backward (fused or the two reference launches), FP64
  * batch loss, variance loss (every batch when dense, every

@@ -105,14 +105,21 @@ def grad_close(g, ref, rel=1e-4, floor=1e-3):
     """Gradient parity bar: per plane p, |g - ref| <= rel * (|ref| +
     floor * max|ref[p]|) for every element.  Returns the worst ratio."""
     worst = 0.0
+    where = None
     for p in range(ref.shape[0]):
         scale = np.abs(ref[p]).max()
         if scale == 0:
             assert np.abs(g[p]).max() == 0, p
             continue
         tol = rel * (np.abs(ref[p]) + floor * scale)
-        ratio = (np.abs(g[p] - ref[p]) / tol).max()
-        worst = max(worst, ratio)
+        r = np.abs(g[p] - ref[p]) / tol
+        i = int(np.argmax(r))
+        if r[i] > worst:
+            worst = float(r[i])
+            where = (p, i, float(g[p][i]), float(ref[p][i]), float(scale))
+    if worst > 1.0:
+        print(f"    worst at plane {where[0]} record {where[1]}: got {where[2]:.6g} "
+              f"ref {where[3]:.6g} (plane max {where[4]:.3g})", flush=True)
     return worst
 
 

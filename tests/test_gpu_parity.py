@@ -44,6 +44,14 @@ def c1(dev):
                 si=torch.from_numpy(batch).to(dev))
 
 
+@pytest.fixture(params=["group", "wave"])
+def marcher(request, monkeypatch):
+    """Fixed-rule marcher under test: eight samples per wave (default) or
+    one sample per wave (PXL_MARCHER=wave)."""
+    monkeypatch.setenv("PXL_MARCHER", request.param)
+    return request.param
+
+
 def device_rays(rays_np, dev, white_bg=True):
     return pxl.Rays.from_arrays(rays_np, device=dev, white_bg=white_bg)
 
@@ -143,8 +151,8 @@ def _gpu_fwd(grid, rays, si, dev):
     return got, tr.cpu().numpy().view(np.uint32)
 
 
-def test_render_fwd_dense_c1(c1):
-    say("[fwd] dense 128^3 SH-2, 4096 rays")
+def test_render_fwd_dense_c1(c1, marcher):
+    say(f"[fwd] dense 128^3 SH-2, 4096 rays, {marcher} marcher")
     dev = c1["dev"]
     grid = pxl.Grid(N1, N1, N1, S1, c1["rec"], step=0.5)
     rays = device_rays(c1["rays_np"], dev)
@@ -162,9 +170,9 @@ def sparse1(c1):
 
 
 @pytest.mark.parametrize("bricks", [True, False])
-def test_render_fwd_sparse(c1, sparse1, bricks):
+def test_render_fwd_sparse(c1, sparse1, bricks, marcher):
     s = sparse1
-    say(f"[fwd] sparse 128^3, {s['n']} caged vertices, brick map {bricks}")
+    say(f"[fwd] sparse 128^3, {s['n']} caged vertices, brick map {bricks}, {marcher} marcher")
     dev = c1["dev"]
     grid = device_sparse_grid(N1, S1, s["compact"], s["occ"], s["io"], s["cage"], s["c2d"], dev,
                               bricks=bricks)
@@ -201,12 +209,13 @@ def _bwd_case(c1, grid, grid_o, nr, fused, huber=0.1):
         torch.cuda.synchronize()
         fwd = oracle.render_fwd(grid_o, rays_o, batch)
     else:
-        # Each side's backward reads its own forward result rounded to the
-        # FP32 ray planes, as the reference's separate kernels do.
+        # The separate kernels exchange C and A through the FP32 ray planes
+        # (RenderUtils.cu:1709-1716), as in the reference; the oracle reads
+        # the same FP32 values.
         pxl.render_fwd(grid, rays, si)
         torch.cuda.synchronize()
-        fwd = oracle.render_fwd(grid_o, rays_o, batch)
-        fwd = {k: fwd[k].astype(np.float32).astype(np.float64) for k in ("r", "g", "b", "w")}
+        fwd = {k_o: getattr(rays, k_g).cpu().numpy().astype(np.float64)
+               for k_g, k_o in (("acc_r", "r"), ("acc_g", "g"), ("acc_b", "b"), ("acc_w", "w"))}
         pxl.render_bwd_rgb(grid, optim, rays, losses, si, mask_coef=1.0, huber_delta=huber)
         torch.cuda.synchronize()
     l_ref, g_ref = oracle.render_bwd(grid_o, rays_o, batch, fwd, mask_coef=1.0,
@@ -214,24 +223,27 @@ def _bwd_case(c1, grid, grid_o, nr, fused, huber=0.1):
     l_got = losses.cpu().numpy()
     assert np.abs(l_got - l_ref).max() <= 255 * 1e-5
     g = from_records(optim.grad, grid.P)
-    worst = grad_close(g, g_ref)
+    # The separate backward reads C and A rounded to FP32 by the forward, so
+    # the suffix C - C_i of late samples carries +-ulp(C)/2 (~6e-8) on
+    # either side of the comparison: its floor is 2e-3 of the plane max.
+    worst = grad_close(g, g_ref, floor=1e-3 if fused else 2e-3)
     say(f"    worst gradient error / tolerance = {worst:.3g}")
     assert worst <= 1.0
     return optim
 
 
 @pytest.mark.parametrize("fused", [False, True])
-def test_backward_dense(c1, fused):
-    say(f"[bwd] dense 128^3, 1024 rays, fused={fused}")
+def test_backward_dense(c1, fused, marcher):
+    say(f"[bwd] dense 128^3, 1024 rays, fused={fused}, {marcher} marcher")
     grid = pxl.Grid(N1, N1, N1, S1, c1["rec"], step=0.5)
     grid_o = oracle.Grid(N1, N1, N1, S1, c1["planes"], 0.5)
     _bwd_case(c1, grid, grid_o, 1024, fused)
 
 
 @pytest.mark.parametrize("fused", [False, True])
-def test_backward_sparse(c1, sparse1, fused):
+def test_backward_sparse(c1, sparse1, fused, marcher):
     s = sparse1
-    say(f"[bwd] sparse 128^3, 1024 rays, fused={fused}")
+    say(f"[bwd] sparse 128^3, 1024 rays, fused={fused}, {marcher} marcher")
     grid = device_sparse_grid(N1, S1, s["compact"], s["occ"], s["io"], s["cage"], s["c2d"],
                               c1["dev"])
     grid_o = oracle.Grid(N1, N1, N1, S1, s["compact"], 0.5, occupancy=s["occ"],
@@ -299,15 +311,18 @@ def test_adam_matches_oracle(dev):
     v = np.zeros_like(planes)
     p_o = planes.copy()
     for t in range(1, 6):
-        g = rng.normal(size=(P, n)).astype(np.float32) * (10.0 ** rng.integers(-4, 1))
+        g = (rng.normal(size=(P, n)) * (10.0 ** rng.integers(-4, 1))).astype(np.float32)
         optim.grad[:, :P] = torch.from_numpy(g.T.copy()).to(dev)
         pxl.adam_step(grid, optim, 0.3, 1.0, t)
         go = g.copy()
         oracle.adam(p_o, go, m, v, 0.3, 1.0, t, adaptive_lr=alr)
     torch.cuda.synchronize()
-    np.testing.assert_allclose(from_records(grid.params, P), p_o, rtol=1e-6, atol=1e-7)
-    np.testing.assert_allclose(from_records(optim.m, P), m, rtol=1e-6, atol=1e-12)
-    np.testing.assert_allclose(from_records(optim.v, P), v, rtol=1e-6, atol=1e-14)
+    for name, got, exp in (("params", grid.params, p_o), ("m", optim.m, m), ("v", optim.v, v)):
+        g = from_records(got, P)
+        err = np.abs(g - exp) / (np.abs(exp) + 1e-30)
+        i = np.unravel_index(np.argmax(err), err.shape)
+        say(f"    {name}: worst rel err {err[i]:.3g} at {i}: gpu {g[i]!r} oracle {exp[i]!r}")
+        np.testing.assert_allclose(g, exp, rtol=1e-6, atol=1e-7 if name == "params" else 1e-20)
     assert float(optim.grad.abs().max()) == 0.0
 
 
@@ -436,7 +451,7 @@ def test_trainer_matches_stage_calls(c1):
 
 
 @pytest.mark.parametrize("storage", [pxl.STORE_BF16, pxl.STORE_F16])
-def test_render_16bit_storage(c1, storage):
+def test_render_16bit_storage(c1, storage, marcher):
     """16-bit grids: the GPU result matches the FP64 oracle run on the
     widened 16-bit values to the same bars as FP32; the storage error
     itself (vs the FP32 grid) is reported."""
