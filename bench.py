@@ -240,6 +240,26 @@ def cpu_baseline(w, dev, nrays):
             "sample": sample}
 
 
+TRAFFIC_FILE = os.path.join(REPO, "profiles", "train_step_256_sparse_f32.md")
+
+
+def profiled_traffic(kernel, args):
+    """HBM bytes per launch of `kernel` on the default workload, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (lines of the form
+    `traffic_bytes_per_launch <kernel> <bytes>`); None for any other
+    workload or kernel, or when the summary holds no figure for it."""
+    default = (args.grid, args.views, args.img, args.batch, args.occupancy,
+               args.batch_order) == (256, 100, 800, 65536, 0.10, "sorted")
+    if not default or not os.path.exists(TRAFFIC_FILE):
+        return None
+    with open(TRAFFIC_FILE) as f:
+        for line in f:
+            parts = line.strip().strip("`").split()
+            if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
+                return int(parts[2])
+    return None
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -247,6 +267,7 @@ def main():
               "multi-GPU design is described, not built.", file=sys.stderr)
         sys.exit(2)
     import torch
+    import pxl
 
     dev = torch.device("cuda:0")
     if args.copy_mib > 0:
@@ -292,9 +313,11 @@ def main():
     vt = touched_vertices(w, dev)
     alg = args.batch * 44 + vt * 448
     achieved = alg / (kern_ms / 1e3) / 1e9
+    kname = pxl.render_kernel_name(w.grid, "fused")
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-            "kernel": "k_render<9,sparse,fixed,fused>", "kernel_ms": round(kern_ms, 4),
+            "frac": round(achieved / HBM_PEAK_GBS, 5),
+            "traffic": profiled_traffic(kname, args),
+            "kernel": kname, "kernel_ms": round(kern_ms, 4),
             "touched_records": int(vt), "alg_bytes": int(alg)}
 
     cpu = None

@@ -105,4 +105,17 @@ __device__ __forceinline__ double wave_sum_d(double v) {
     return v;
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (block i -> XCD
+// i mod 8).  Give XCD x a contiguous slice of the launch instead, so work
+// items that are neighbours in launch order (rays of a sorted batch, records
+// of one grid region) share one XCD's L2 for their gathers and atomics.  A
+// bijection on [0, gridDim.x).
+__device__ __forceinline__ unsigned xcd_slice_block(unsigned blk, unsigned nb) {
+    constexpr unsigned kXcd = 8;
+    if (nb < kXcd) return blk;
+    const unsigned per = nb / kXcd, rem = nb % kXcd;
+    const unsigned x = blk % kXcd, idx = blk / kXcd;
+    return x < rem ? x * (per + 1) + idx : rem * (per + 1) + (x - rem) * per + idx;
+}
+
 }  // namespace pxl

@@ -17,13 +17,14 @@
 //     product-scan of (1 - alpha) over the slots; opacity and colour
 //     prefixes are sum-scans, so the per-sample scalar maths runs once per
 //     8 samples of wave time instead of once per sample;
-//   * backward: lane (s, j) owns the gradient of corner j of sample s.
-//     Consecutive samples in the same cell (same eight records) form a run;
-//     each lane adds its 1 + 3S values into the LDS row (run, j), and a
-//     finished run is flushed to the gradient planes with lanes over
-//     parameters, two records per instruction.  The last run of a group stays
-//     in LDS to merge with the next group, so a record gets one global atomic
-//     per parameter per run instead of one per sample.
+//   * backward: the gradient of the current cell's 8 corners lives in
+//     registers with lanes over parameters (4 values per lane); the slots'
+//     scalars (trilinear fractions, dL/dsigma, dL/dcolour) are broadcast with
+//     readlane and accumulated in order.  Leaving a cell keeps the corners the
+//     next cell shares (a face when the ray steps to a neighbour) and adds the
+//     rest to the gradient planes, two records per atomic instruction, so a
+//     record gets one global atomic per parameter per visit of the ray
+//     instead of one per sample.
 // The backward recomputes every forward quantity; render.hip is compiled
 // with -ffp-contract=off (FMAs written out) so both passes produce the same
 // bits and C - C_i stays exact.
@@ -105,20 +106,43 @@ __device__ __forceinline__ void load_record(const float *params, size_t rec, int
     }
 }
 
-constexpr int kRuns = 9;  // LDS row blocks per wave: the kept run + 8 new
+// Leaves the current cell for one displaced by (dx, dy, dz) cells.  Lane
+// (lh, pp) holds in acc[h] parameter pp of corner j = 2h + lh = (jx, jy, jz).
+// Old corner j sits at the new cell's corner j - d when that is in {0,1}^3:
+// its value moves there; every other old corner is added to the gradient
+// planes.  New corners not reached by a move start at zero.
+template <int P>
+__device__ __forceinline__ void retire_cell(const RenderArgs &a, float (&acc)[4],
+                                            const uint32_t (&rec)[4], int dx, int dy, int dz,
+                                            int lh, int pp) {
+    const auto in01 = [](int v) { return v == 0 || v == 1; };
+#pragma unroll
+    for (int h = 0; h < 4; h++) {
+        const bool kept = in01(lh - dx) && in01((h & 1) - dy) && in01((h >> 1) - dz);
+        if (!kept && pp < P)
+            unsafeAtomicAdd(a.grad + (size_t)rec[h] * (size_t)a.gstride + pp, acc[h]);
+    }
+    // New corner j' = 2h' + lh takes old corner j' + d: register
+    // (h' & 1) + dy + 2 ((h' >> 1) + dz), lane half lh + dx.
+    const bool x_ok = in01(lh + dx);
+    float moved[4];
+#pragma unroll
+    for (int h = 0; h < 4; h++) {
+        const int yy = (h & 1) + dy, zz = (h >> 1) + dz;
+        const bool yz_ok = in01(yy) && in01(zz);
+        const int src = yz_ok ? yy + 2 * zz : 0;
+        float v = src == 0 ? acc[0] : (src == 1 ? acc[1] : (src == 2 ? acc[2] : acc[3]));
+        if (dx != 0) v = __shfl_xor(v, 32, 64);
+        moved[h] = (yz_ok && x_ok) ? v : 0.0f;
+    }
+#pragma unroll
+    for (int h = 0; h < 4; h++) acc[h] = moved[h];
+}
 
 template <int S, bool SPARSE, int MODE, int STORE>
 __global__ __launch_bounds__(256) void k_march8(const RenderArgs a) {
     constexpr int P = 1 + 3 * S;
     const int lane = threadIdx.x & 63;
-    // Per-wave gradient rows [run][corner][32 floats] (backward only).
-    constexpr int kRowFloats = (MODE == kBwd || MODE == kFused) ? kRuns * 8 * 32 : 1;
-    __shared__ float lds_rows[4][kRowFloats];
-    float *rows = lds_rows[threadIdx.x >> 6];
-    if (MODE == kBwd || MODE == kFused) {
-        for (int i = lane; i < kRowFloats; i += 64) rows[i] = 0.0f;
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    }
     const int slot = lane >> 3, corner = lane & 7;
     const int64_t b =
         (int64_t)xcd_slice_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
@@ -161,6 +185,15 @@ __global__ __launch_bounds__(256) void k_march8(const RenderArgs a) {
 #pragma unroll
     for (int k = 0; k < S; k++) Y[k] = sh_basis(k, fdx, fdy, fdz);
 
+    // Gradient layout (backward): lane (lh = lane / 32, pp = lane % 32) holds
+    // in acc[h] parameter pp of corner 2h + lh of the current cell; ysel is
+    // the SH basis value parameter pp multiplies, pch its colour channel.
+    const int lh = lane >> 5, pp = lane & 31;
+    const int pch = pp == 0 ? -1 : (pp - 1) / S;
+    float ysel = 0.0f;
+#pragma unroll
+    for (int k = 0; k < S; k++) ysel = (pp >= 1 && (pp - 1) % S == k) ? Y[k] : ysel;
+
     // Backward coefficients (set after the forward / from the ray planes).
     double C0 = 0.0, C1 = 0.0, C2 = 0.0, A = 0.0;
     double r0 = 0.0, r1 = 0.0, r2 = 0.0, mterm = 0.0;
@@ -199,11 +232,12 @@ __global__ __launch_bounds__(256) void k_march8(const RenderArgs a) {
 
         // Running totals before the current group (uniform, FP64).
         double T = 1.0, Acc = 0.0, Cr = 0.0, Cg = 0.0, Cb = 0.0, sig = 0.0;
-        // Backward gradient merge state: whether a run is kept, its LDS row
-        // block and (lane l) the record of its corner l & 7.
-        bool keep_valid = false;
-        int keep_base = 0;
-        uint32_t keep_rec = 0u;
+        // Backward: the current cell (base coordinates), its corner records
+        // in the acc layout, and the accumulated gradient.
+        bool has_cur = false;
+        int cur_x = 0, cur_y = 0, cur_z = 0;
+        uint32_t cur_rec[4] = {0u, 0u, 0u, 0u};
+        float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
         uint32_t hcount = 0, hash = 2166136261u;
         int k0 = 0;
         uint64_t pending = 0;  // occupied positions of [k0, k0 + 64) not yet done
@@ -335,66 +369,43 @@ __global__ __launch_bounds__(256) void k_march8(const RenderArgs a) {
                             r2 * (Tn * (double)col2 - (C2 - H2));
                 gs *= 0.6667 * step * ds;
                 gs += step * ds * (A - Ai - Tn) * mterm;
-                // Runs of consecutive slots whose eight corner records are
-                // identical.  Run 0 continues the run kept from the previous
-                // group; a slot whose records differ from its predecessor's
-                // (or from the kept ones) opens a new run.  The masks, run
-                // numbers and run starts are wave-uniform.
-                const uint32_t up_rec = (uint32_t)__shfl_up((int)rec, 8, 64);
-                const uint32_t prev_rec = slot == 0 ? keep_rec : up_rec;
-                const bool differs = active && (rec != prev_rec || (slot == 0 && !keep_valid));
-                const uint64_t dmask = __ballot(differs);
-                uint32_t starts = 0u;  // bit s: slot s opens a run
+                // Accumulate the slots in order.  A slot in the current cell
+                // adds to acc; a slot in another cell first retires the
+                // current one: corners the two cells share (same vertex, so
+                // the same record) move to their new corner position, the
+                // others go to the gradient planes with one float atomic per
+                // parameter.  All control flow here is wave-uniform.
+                const float gsf = (float)gs;
 #pragma unroll
-                for (int s2 = 0; s2 < 8; s2++)
-                    starts |= ((dmask >> (8 * s2)) & 0xFFull) != 0 ? (1u << s2) : 0u;
-                const int R = 1 + __builtin_popcount(starts);
-                const int my_run = __builtin_popcount(starts & ((2u << slot) - 1u));
-                // Merge this lane's gradient into its (run, corner) row.
-                if (active) {
-                    float *row = rows + ((keep_base + my_run) % kRuns) * (8 * 32) + corner * 32;
-                    atomicAdd(row, (float)gs * w);
-                    const float gch[3] = {g0 * w, g1 * w, g2 * w};
+                for (int s2 = 0; s2 < 8; s2++) {
+                    if (s2 >= n) break;
+                    const int sb = 8 * s2;
+                    const int nx = __builtin_amdgcn_readlane(bx, sb);
+                    const int ny = __builtin_amdgcn_readlane(by, sb);
+                    const int nz = __builtin_amdgcn_readlane(bz, sb);
+                    if (!has_cur || nx != cur_x || ny != cur_y || nz != cur_z) {
+                        if (has_cur) retire_cell<P>(a, acc, cur_rec, nx - cur_x, ny - cur_y,
+                                                    nz - cur_z, lh, pp);
+                        has_cur = true;
+                        cur_x = nx;
+                        cur_y = ny;
+                        cur_z = nz;
 #pragma unroll
-                    for (int ch = 0; ch < 3; ch++)
-#pragma unroll
-                        for (int k = 0; k < S; k++) atomicAdd(row + 1 + ch * S + k, gch[ch] * Y[k]);
-                }
-                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-                // Flush runs 0 .. R-2 (run R-1 stays for the next group):
-                // two rows per instruction, lanes over parameters.  Run r >= 1
-                // starts at the r-th set bit of starts.
-                uint32_t rest = starts;
-                for (int r = 0; r < R - 1; r++) {
-                    int src = -1;
-                    if (r > 0) {
-                        src = 8 * __builtin_ctz(rest);
-                        rest &= rest - 1u;
+                        for (int h = 0; h < 4; h++)
+                            cur_rec[h] = (uint32_t)__shfl((int)rec, sb + 2 * h + lh, 64);
                     }
-                    if (r == 0 && !keep_valid) continue;  // nothing kept yet
+                    const float sfx = readlane_f(fx, sb), sfy = readlane_f(fy, sb),
+                                sfz = readlane_f(fz, sb);
+                    const float sgs = readlane_f(gsf, sb), sg0 = readlane_f(g0, sb),
+                                sg1 = readlane_f(g1, sb), sg2 = readlane_f(g2, sb);
+                    const float coef =
+                        pch < 0 ? sgs : (pch == 0 ? sg0 : (pch == 1 ? sg1 : sg2)) * ysel;
+                    const float wx = lh ? sfx : 1.0f - sfx;
 #pragma unroll
-                    for (int half = 0; half < 4; half++) {
-                        const int j = 2 * half + (lane >> 5);
-                        const int pp = lane & 31;
-                        const uint32_t rj =
-                            src < 0 ? (uint32_t)__shfl((int)keep_rec, j, 64)
-                                    : (uint32_t)__shfl((int)rec, src + j, 64);
-                        float *row = rows + ((keep_base + r) % kRuns) * (8 * 32) + j * 32;
-                        if (pp < P) {
-                            const float v = row[pp];
-                            row[pp] = 0.0f;
-                            unsafeAtomicAdd(a.grad + (size_t)rj * (size_t)a.gstride + pp, v);
-                        }
+                    for (int h = 0; h < 4; h++) {
+                        const float wyz = ((h & 1) ? sfy : 1.0f - sfy) * ((h >> 1) ? sfz : 1.0f - sfz);
+                        acc[h] = fmaf(wx * wyz, coef, acc[h]);
                     }
-                }
-                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-                // The last run becomes the kept one; lane l keeps the record
-                // of corner l & 7.
-                if (R > 1) {
-                    const int fs = 31 - __builtin_clz(starts);
-                    keep_rec = (uint32_t)__shfl((int)rec, 8 * fs + (lane & 7), 64);
-                    keep_valid = true;
-                    keep_base = (keep_base + R - 1) % kRuns;
                 }
                 Acc = __shfl(Ai, 63, 64);
                 Cr = __shfl(H0, 63, 64);
@@ -404,19 +415,9 @@ __global__ __launch_bounds__(256) void k_march8(const RenderArgs a) {
             }
         }
 
-        if (bwd && keep_valid) {
-#pragma unroll
-            for (int half = 0; half < 4; half++) {
-                const int j = 2 * half + (lane >> 5);
-                const int pp = lane & 31;
-                const uint32_t rj = (uint32_t)__shfl((int)keep_rec, j, 64);
-                float *row = rows + keep_base * (8 * 32) + j * 32;
-                if (pp < P) {
-                    const float v = row[pp];
-                    row[pp] = 0.0f;
-                    unsafeAtomicAdd(a.grad + (size_t)rj * (size_t)a.gstride + pp, v);
-                }
-            }
+        if (bwd && has_cur) {
+            // Retire the last cell: a displacement of 2 shares no corner.
+            retire_cell<P>(a, acc, cur_rec, 2, 2, 2, lh, pp);
         }
 
         if (!bwd) {

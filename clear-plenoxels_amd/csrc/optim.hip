@@ -43,42 +43,69 @@ __device__ __forceinline__ uint32_t record_of(const TvArgs &a, int x, int y, int
 // mean mu of its 8 corners and adds lambda * (u_corner - mu) to each
 // corner's gradient with unsynchronised +=.  Here each record gathers the
 // same terms from the (up to 8) cells that have its vertex as a corner:
-// one group of LPV lanes per record, one lane per parameter, 27-neighbour
-// values loaded once.
+// one group of LPV lanes per record, one lane per parameter.  Lane i < 27 of
+// the group looks up the record of neighbour i and lane c < 8 tests cell c,
+// so the index and occupancy gathers are one load each; a neighbour value is
+// loaded only when an occupied cell around the vertex contains it.
 template <int LPV, int STORE>
 __global__ __launch_bounds__(256) void k_tv_disparity(const TvArgs a) {
     constexpr int kGroups = 256 / LPV;
     const int g = threadIdx.x / LPV, p = threadIdx.x % LPV;
-    const int64_t r = (int64_t)blockIdx.x * kGroups + g;
-    if (r >= a.n_vox) return;
-    if (p >= a.P) return;
-    const float lam = p == 0 ? a.lam_d : a.lam_sh;
-    if (lam == 0.0f) return;
+    const int64_t r = (int64_t)xcd_slice_block(blockIdx.x, gridDim.x) * kGroups + g;
+    if (r >= a.n_vox) return;  // the whole group leaves together
     const uint32_t d = a.sparse ? a.c2d[r] : (uint32_t)r;
     const int x = (int)(d % (uint32_t)a.W);
     const int y = (int)((d / (uint32_t)a.W) % (uint32_t)a.H);
     const int z = (int)(d / ((uint32_t)a.W * (uint32_t)a.H));
 
-    // Neighbour values u[(dz+1)*9 + (dy+1)*3 + (dx+1)].  Sums run in FP64:
-    // a vertex value and the mean of a cell around it nearly cancel.
+    // Neighbour i = (dz+1)*9 + (dy+1)*3 + (dx+1), looked up by lane i.
+    uint32_t nrec = 0u;
+    if (p < 27) {
+        const int nx = x + p % 3 - 1, ny = y + (p / 3) % 3 - 1, nz = z + p / 9 - 1;
+        if (nx >= 0 && nx < a.W && ny >= 0 && ny < a.H && nz >= 0 && nz < a.D)
+            nrec = record_of(a, nx, ny, nz);
+    }
+    // Cell c = (x - (c&1), y - (c>>1&1), z - (c>>2)), tested by lane c.
+    bool cocc = false;
+    if (p < 8) {
+        const int cx = x - (p & 1), cy = y - ((p >> 1) & 1), cz = z - (p >> 2);
+        cocc = cx >= 0 && cx <= a.W - 2 && cy >= 0 && cy <= a.H - 2 && cz >= 0 &&
+               cz <= a.D - 2 && (!a.sparse || cell_occupied(a, cx, cy, cz));
+    }
+    const int seg = (int)(threadIdx.x & 63) & ~(LPV - 1);
+    const uint32_t cmask = (uint32_t)(__ballot(cocc) >> seg) & 0xFFu;
+    if (cmask == 0u) return;  // no occupied cell around this vertex
+
+    const float lam = p == 0 ? a.lam_d : a.lam_sh;
+    const bool mine = p < a.P && lam != 0.0f;
+    // Sums run in FP64: a vertex value and the mean of a cell around it
+    // nearly cancel.
     double u[27];
 #pragma unroll
     for (int i = 0; i < 27; i++) {
         const int dx = i % 3 - 1, dy = (i / 3) % 3 - 1, dz = i / 9 - 1;
-        const int nx = x + dx, ny = y + dy, nz = z + dz;
+        // Cells (ax, ay, az) containing neighbour i: dx in {-ax, 1-ax} etc.
+        uint32_t cells = 0u;
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+            const int ax = c & 1, ay = (c >> 1) & 1, az = c >> 2;
+            if ((dx == -ax || dx == 1 - ax) && (dy == -ay || dy == 1 - ay) &&
+                (dz == -az || dz == 1 - az))
+                cells |= 1u << c;
+        }
+        const uint32_t ri = (uint32_t)__shfl((int)nrec, i, LPV);
         float val = 0.0f;
-        if (nx >= 0 && nx < a.W && ny >= 0 && ny < a.H && nz >= 0 && nz < a.D)
-            val = ld<STORE>(a.params, (size_t)record_of(a, nx, ny, nz) * (size_t)a.stride + p);
+        if (mine && (cells & cmask) != 0u)
+            val = ld<STORE>(a.params, (size_t)ri * (size_t)a.stride + p);
         u[i] = (double)val;
     }
+    if (!mine) return;
     const double self = u[13];
     double tsum = 0.0;
 #pragma unroll
     for (int c = 0; c < 8; c++) {
+        if (!((cmask >> c) & 1u)) continue;
         const int ax = c & 1, ay = (c >> 1) & 1, az = c >> 2;
-        const int cx = x - ax, cy = y - ay, cz = z - az;
-        if (cx < 0 || cx > a.W - 2 || cy < 0 || cy > a.H - 2 || cz < 0 || cz > a.D - 2) continue;
-        if (a.sparse && !cell_occupied(a, cx, cy, cz)) continue;
         double s = 0.0;
 #pragma unroll
         for (int j = 0; j < 8; j++) {
@@ -290,16 +317,27 @@ __global__ __launch_bounds__(256) void k_adam(float *__restrict__ prm, float *__
     *reinterpret_cast<float4 *>(var + off) = v4;
 }
 
-__global__ __launch_bounds__(256) void k_loss_reduce(const double *__restrict__ l, int64_t n,
-                                                     double *__restrict__ out) {
-    __shared__ double part[4];
-    double s = 0.0;
-    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) s += l[i] * l[i];
-    s = wave_sum_d(s);
+__global__ __launch_bounds__(1024) void k_loss_reduce(const double *__restrict__ l, int64_t n,
+                                                      double *__restrict__ out) {
+    __shared__ double part[16];
+    // Four independent accumulators per lane keep several loads in flight.
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    const int64_t T = blockDim.x;
+    int64_t i = threadIdx.x;
+    for (; i + 3 * T < n; i += 4 * T) {
+        const double a = l[i], b = l[i + T], c = l[i + 2 * T], d = l[i + 3 * T];
+        s0 += a * a;
+        s1 += b * b;
+        s2 += c * c;
+        s3 += d * d;
+    }
+    for (; i < n; i += T) s0 += l[i] * l[i];
+    double s = wave_sum_d((s0 + s1) + (s2 + s3));
     if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
     __syncthreads();
     if (threadIdx.x == 0) {
-        const double tot = part[0] + part[1] + part[2] + part[3];
+        double tot = 0.0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) tot += part[w];
         const double mse = tot / (double)n;
         out[0] = mse;
         out[1] = 10.0 * log10(65025.0 / mse);
@@ -491,6 +529,6 @@ extern "C" int32_t pxl_loss_reduce_f64(const double *losses, int64_t len, double
     clear_error();
     PXL_REQUIRE(losses && out, "pxl_loss_reduce_f64: null argument");
     PXL_REQUIRE(len > 0, "pxl_loss_reduce_f64: empty batch");
-    hipLaunchKernelGGL(k_loss_reduce, dim3(1), dim3(256), 0, as_stream(stream), losses, len, out);
+    hipLaunchKernelGGL(k_loss_reduce, dim3(1), dim3(1024), 0, as_stream(stream), losses, len, out);
     return check_launch("pxl_loss_reduce_f64");
 }

@@ -235,18 +235,6 @@ __device__ __forceinline__ uint32_t fnv_mix(uint32_t h, uint32_t x) {
     return h;
 }
 
-// Workgroups are dispatched round-robin over the 8 XCDs (block i -> XCD
-// i mod 8).  Give XCD x a contiguous slice of the batch instead, so rays that
-// are neighbours in the batch order share one XCD's L2 for their gathers and
-// gradient atomics.  A bijection on [0, gridDim.x).
-__device__ __forceinline__ unsigned xcd_slice_block(unsigned blk, unsigned nb) {
-    constexpr unsigned kXcd = 8;
-    if (nb < kXcd) return blk;
-    const unsigned per = nb / kXcd, rem = nb % kXcd;
-    const unsigned x = blk % kXcd, idx = blk / kXcd;
-    return x < rem ? x * (per + 1) + idx : rem * (per + 1) + (x - rem) * per + idx;
-}
-
 template <int S, bool SPARSE, bool ADAPT, int MODE, int STORE>
 __global__ __launch_bounds__(256) void k_render(const RenderArgs a) {
     using R = typename std::conditional<ADAPT, double, float>::type;
@@ -764,4 +752,27 @@ extern "C" int32_t pxl_count_touched(const pxl_grid *grid, const pxl_rays *rays,
     hipLaunchKernelGGL(k_popcount, dim3(256), dim3(256), 0, s, bitmap, nwords,
                        reinterpret_cast<unsigned long long *>(count));
     return check_launch("pxl_count_touched");
+}
+
+extern "C" const char *pxl_render_kernel_name(const pxl_grid *grid, int32_t mode) {
+    clear_error();
+    if (!grid || mode < 0 || mode > 2) {
+        set_error("pxl_render_kernel_name: bad argument");
+        return nullptr;
+    }
+    static thread_local char name[96];
+    const bool adapt = grid->march_mode == PXL_MARCH_REF_ADAPTIVE;
+    const bool group = !adapt && grid->sh_dim <= 9 && use_march8();
+    const char *modes[3] = {"fwd", "bwd", "fused"};
+    const char *store = grid->storage == PXL_STORE_BF16  ? "bf16"
+                        : grid->storage == PXL_STORE_F16 ? "f16"
+                                                         : "f32";
+    if (group)
+        snprintf(name, sizeof(name), "k_march8<%d,%s,%s,%s>", grid->sh_dim,
+                 grid->sparse ? "sparse" : "dense", modes[mode], store);
+    else
+        snprintf(name, sizeof(name), "k_render<%d,%s,%s,%s,%s>", grid->sh_dim,
+                 grid->sparse ? "sparse" : "dense", adapt ? "adaptive" : "fixed", modes[mode],
+                 store);
+    return name;
 }

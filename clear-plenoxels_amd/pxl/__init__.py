@@ -32,6 +32,7 @@ EXPORTS = (
     "pxl_trainer_time_render", "pxl_trainer_render_ms", "pxl_trainer_loss",
     "pxl_trainer_state", "pxl_trainer_destroy", "pxl_brick_words",
     "pxl_build_brick_occupancy", "pxl_records_to_16bit", "pxl_tv_prepare",
+    "pxl_render_kernel_name",
 )
 
 
@@ -152,6 +153,7 @@ def _declare(L):
         "pxl_brick_words": ([_I32, _I32, _I32], _I64),
         "pxl_build_brick_occupancy": ([_P, _I32, _I32, _I32, _P, _P], _I32),
         "pxl_records_to_16bit": ([_P, _I32, _I64, _P, _I32, _I32, _P], _I32),
+        "pxl_render_kernel_name": ([G, _I32], ctypes.c_char_p),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -471,6 +473,17 @@ def loss_reduce(losses, out, length=None, stream=None):
     length = losses.numel() if length is None else length
     _check("pxl_loss_reduce_f64",
            lib().pxl_loss_reduce_f64(_ptr(losses), length, _ptr(out), _stream(stream)))
+
+
+def render_kernel_name(grid, mode="fused"):
+    """Name of the kernel the render entry points launch for `grid`
+    (mode "fwd", "bwd" or "fused")."""
+    m = {"fwd": 0, "bwd": 1, "fused": 2}[mode]
+    g = grid.c()
+    name = lib().pxl_render_kernel_name(ctypes.byref(g), m)
+    if name is None:
+        raise PxlError("pxl_render_kernel_name", PXL_EINVAL, lib().pxl_last_error().decode())
+    return name.decode()
 
 
 def count_touched(grid, rays, sample_index, bitmap, count, start=0, length=None, stream=None):

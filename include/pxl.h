@@ -293,6 +293,13 @@ int32_t pxl_count_touched(const pxl_grid *grid, const pxl_rays *rays,
                           int64_t len, uint32_t *bitmap, uint64_t *count,
                           void *stream);
 
+/* Name of the kernel the render entry points launch for this grid:
+ * mode 0 = pxl_render_fwd, 1 = pxl_render_bwd_rgb, 2 = pxl_train_rays.
+ * Returns a thread-local string such as "k_march8<9,sparse,fused,f32>" (or
+ * NULL for a bad argument); for profilers and benchmark reports.  No
+ * counterpart in the reference. */
+const char *pxl_render_kernel_name(const pxl_grid *grid, int32_t mode);
+
 /* ------------------------------------------------------------------ */
 /* Trainer: the batch loop of Render::objectSceneLearner                 */
 /* (Render.cu:352-503) as a host object driving the entry points above.  */

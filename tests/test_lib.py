@@ -70,6 +70,25 @@ def test_argument_validation_without_device(pxl_lib):
     assert L.pxl_adam_step(None, None, 0.1, 0.1, 1, None) == pxl_lib.PXL_EINVAL
 
 
+def test_render_kernel_name(pxl_lib, monkeypatch):
+    L = pxl_lib.lib()
+    g = pxl_lib.GridC()
+    g.sh_dim, g.sparse, g.storage = 9, 1, pxl_lib.STORE_F32
+    monkeypatch.delenv("PXL_MARCHER", raising=False)
+    assert L.pxl_render_kernel_name(ctypes.byref(g), 2) == b"k_march8<9,sparse,fused,f32>"
+    g.storage = pxl_lib.STORE_BF16
+    assert L.pxl_render_kernel_name(ctypes.byref(g), 0) == b"k_march8<9,sparse,fwd,bf16>"
+    g.sh_dim, g.sparse = 16, 0
+    assert L.pxl_render_kernel_name(ctypes.byref(g), 1) == b"k_render<16,dense,fixed,bwd,bf16>"
+    g.sh_dim, g.march_mode, g.storage = 9, pxl_lib.MARCH_REF_ADAPTIVE, pxl_lib.STORE_F32
+    assert L.pxl_render_kernel_name(ctypes.byref(g), 2) == b"k_render<9,dense,adaptive,fused,f32>"
+    g.march_mode = pxl_lib.MARCH_FIXED
+    monkeypatch.setenv("PXL_MARCHER", "wave")
+    assert L.pxl_render_kernel_name(ctypes.byref(g), 2) == b"k_render<9,dense,fixed,fused,f32>"
+    assert L.pxl_render_kernel_name(ctypes.byref(g), 3) is None
+    assert L.pxl_render_kernel_name(None, 0) is None
+
+
 def test_camera_equals_oracle(pxl_lib, orc):
     rng = np.random.default_rng(7)
     oc = np.array([64.0, 64.0, 64.0])
