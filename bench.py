@@ -51,6 +51,8 @@ def parse():
     ap.add_argument("--batch-order", choices=("shuffled", "sorted"), default="sorted",
                     help="order of the rays inside each batch (the batches themselves are "
                          "the same consecutive chunks of one seeded shuffle)")
+    ap.add_argument("--log-samples", type=int, default=512,
+                    help="samples per ray the two-kernel fused step logs (0: one fused kernel)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="run one device-to-device copy of this many MiB first (a kernel of "
                          "known traffic for calibrating profiler byte counters)")
@@ -173,7 +175,8 @@ class Workload:
                                    sh_learning_rate=1.0, variance_coefficient=1e-5,
                                    sh_coefficient=1e-6, variance_type=pxl.TV_DISPARITY,
                                    neighbour_nums=6, variance_interval=3, mask_coefficient=1.0,
-                                   huber_delta=0.1, fused=True)
+                                   huber_delta=0.1, fused=True,
+                                   log_samples=args.log_samples)
 
     def step(self):
         self.trainer.step()

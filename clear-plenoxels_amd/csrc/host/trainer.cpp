@@ -65,6 +65,9 @@ struct pxl_trainer {
     // Optional HIP events around each batch's render launch.
     std::vector<hipEvent_t> ev;
     int64_t ev_used;
+    // Sample-log workspace of the two-kernel fused step (owned).
+    void *march_ws;
+    size_t march_ws_bytes;
 };
 
 using namespace pxl;
@@ -106,6 +109,19 @@ extern "C" pxl_trainer *pxl_trainer_create(const pxl_train_config *cfg, const px
     tr->lr = cfg->learning_rate;
     tr->sh_lr = cfg->sh_learning_rate;
     tr->ev_used = 0;
+    tr->march_ws = nullptr;
+    tr->march_ws_bytes = 0;
+    if (cfg->fused && cfg->log_samples > 0) {
+        tr->march_ws_bytes = pxl_train_rays_workspace_bytes(cfg->batch_size, cfg->log_samples);
+        if (tr->march_ws_bytes == 0 ||
+            hipMalloc(&tr->march_ws, tr->march_ws_bytes) != hipSuccess) {
+            set_error("pxl_trainer_create: cannot allocate a %zu-byte sample log for %d samples "
+                      "per ray",
+                      tr->march_ws_bytes, cfg->log_samples);
+            delete tr;
+            return nullptr;
+        }
+    }
     return tr;
 }
 
@@ -135,7 +151,11 @@ extern "C" int32_t pxl_trainer_step(pxl_trainer *tr) {
     const bool timed = tr->ev_used * 2 + 1 < (int64_t)tr->ev.size();
     if (timed) (void)hipEventRecord(tr->ev[tr->ev_used * 2], tr->stream);
     int32_t st;
-    if (c.fused) {
+    if (c.fused && tr->march_ws) {
+        st = pxl_train_rays_ws(&tr->grid, &tr->optim, &tr->rays, tr->losses, tr->order, start,
+                               len, c.mask_coefficient, c.huber_delta, tr->march_ws,
+                               tr->march_ws_bytes, c.log_samples, tr->stream);
+    } else if (c.fused) {
         st = pxl_train_rays(&tr->grid, &tr->optim, &tr->rays, tr->losses, tr->order, start, len,
                             c.mask_coefficient, c.huber_delta, tr->stream);
     } else {
@@ -228,5 +248,9 @@ extern "C" int32_t pxl_trainer_state(const pxl_trainer *tr, int64_t *t, double *
 extern "C" void pxl_trainer_destroy(pxl_trainer *tr) {
     if (!tr) return;
     for (hipEvent_t e : tr->ev) (void)hipEventDestroy(e);
+    if (tr->march_ws) {
+        (void)hipStreamSynchronize(tr->stream);
+        (void)hipFree(tr->march_ws);
+    }
     delete tr;
 }

@@ -106,20 +106,76 @@ __device__ __forceinline__ void load_record(const float *params, size_t rec, int
     }
 }
 
+// Loss and clipped residuals of one ray from its forward totals
+// (renderRays_withGradients_RGBGPU, RenderUtils.cu:2149-2193: Huber-clipped
+// colour residuals, the mask term gated on A < 1 and the 1e-3 log-odds
+// opacity prior).  False when the loss
+// is NaN (the reference then skips the ray and books a loss of 255).
+struct RayResid {
+    double r0, r1, r2, mterm, loss;
+};
+__device__ __forceinline__ bool ray_residuals(const RenderArgs &a, int ray, double C0, double C1,
+                                              double C2, double A, RayResid &o) {
+    double r0 = C0 - (double)a.rays.gt_r[ray] / 255.0;
+    double r1 = C1 - (double)a.rays.gt_g[ray] / 255.0;
+    double r2 = C2 - (double)a.rays.gt_b[ray] / 255.0;
+    o.loss = (fabs(r0) + fabs(r1) + fabs(r2)) * 255.0 / 3.0;
+    if (isnan(o.loss)) return false;
+    const double hd = (double)a.huber;
+    const double hi = hd - 0.5 * hd * hd, lo = 0.5 * hd * hd - hd;
+    o.r0 = r0 > hd ? hi : (r0 < -hd ? lo : r0);
+    o.r1 = r1 > hd ? hi : (r1 < -hd ? lo : r1);
+    o.r2 = r2 > hd ? hi : (r2 < -hd ? lo : r2);
+    const int gm = a.rays.gt_mask[ray];
+    double mterm = 0.0;
+    if (gm != 0 && A < 1.0) mterm = ((double)gm / 255.0 - A) * a.mask_coef;
+    double wpre = A > 1e-6 ? A : 1e-6;
+    wpre = wpre < 1.0 - 1e-6 ? wpre : 1.0 - 1e-6;
+    o.mterm = mterm + (log(wpre) - log(1.0 - wpre)) * 1e-3;
+    return true;
+}
+
+// Per-sample gradient scalars (RenderUtils.cu:2229-2241): dL/draw-colour per
+// channel (g0..g2, before the SH basis and trilinear weight) and dL/draw-
+// density (gs, before the trilinear weight).  ws = T_i alpha_i, Tn =
+// T_{i+1}, Ai and H* the opacity / colour prefixes including sample i.
+__device__ __forceinline__ void sample_grads(const RayResid &rr, double C0, double C1,
+                                             double C2, double A, double step, double ws,
+                                             double Tn, double Ai, double H0, double H1,
+                                             double H2, float col0, float col1, float col2,
+                                             float sraw, float &g0, float &g1, float &g2,
+                                             double &gs) {
+    const double kc = (double)0.6667f;
+    g0 = (float)(kc * ws * rr.r0 * (double)(0.1f * col0 * (1.0f - col0)));
+    g1 = (float)(kc * ws * rr.r1 * (double)(0.1f * col1 * (1.0f - col1)));
+    g2 = (float)(kc * ws * rr.r2 * (double)(0.1f * col2 * (1.0f - col2)));
+    const double ds = (double)dsoftplus_shift<float>(sraw);
+    gs = rr.r0 * (Tn * (double)col0 - (C0 - H0)) + rr.r1 * (Tn * (double)col1 - (C1 - H1)) +
+         rr.r2 * (Tn * (double)col2 - (C2 - H2));
+    gs *= 0.6667 * step * ds;
+    gs += step * ds * (A - Ai - Tn) * rr.mterm;
+}
+
 // Leaves the current cell for one displaced by (dx, dy, dz) cells.  Lane
 // (lh, pp) holds in acc[h] parameter pp of corner j = 2h + lh = (jx, jy, jz).
 // Old corner j sits at the new cell's corner j - d when that is in {0,1}^3:
 // its value moves there; every other old corner is added to the gradient
 // planes.  New corners not reached by a move start at zero.
+// Record 0 is also where every corner outside the cage of a caged grid
+// lands (index_offsets 0, SceneUtils.cu:1106-1113), so it collects a flush
+// from a large share of all cells: those go to acc0, a per-lane register
+// added to the gradient once per ray (flush_record0), instead of each being
+// an atomic on the same cache line.
 template <int P>
 __device__ __forceinline__ void retire_cell(const RenderArgs &a, float (&acc)[4],
-                                            const uint32_t (&rec)[4], int dx, int dy, int dz,
-                                            int lh, int pp) {
+                                            const uint32_t (&rec)[4], float &acc0, int dx,
+                                            int dy, int dz, int lh, int pp) {
     const auto in01 = [](int v) { return v == 0 || v == 1; };
 #pragma unroll
     for (int h = 0; h < 4; h++) {
         const bool kept = in01(lh - dx) && in01((h & 1) - dy) && in01((h >> 1) - dz);
-        if (!kept && pp < P)
+        if (!kept && rec[h] == 0u) acc0 += acc[h];
+        else if (!kept && pp < P)
             unsafeAtomicAdd(a.grad + (size_t)rec[h] * (size_t)a.gstride + pp, acc[h]);
     }
     // New corner j' = 2h' + lh takes old corner j' + d: register
@@ -139,6 +195,13 @@ __device__ __forceinline__ void retire_cell(const RenderArgs &a, float (&acc)[4]
     for (int h = 0; h < 4; h++) acc[h] = moved[h];
 }
 
+// Adds the ray's record-0 sum (both lane halves) to the gradient.
+template <int P>
+__device__ __forceinline__ void flush_record0(const RenderArgs &a, float acc0, int lh, int pp) {
+    const float v = acc0 + __shfl_xor(acc0, 32, 64);
+    if (lh == 0 && pp < P && v != 0.0f) unsafeAtomicAdd(a.grad + pp, v);
+}
+
 template <int S, bool SPARSE, int MODE, int STORE>
 __global__ __launch_bounds__(256) void k_march8(const RenderArgs a) {
     constexpr int P = 1 + 3 * S;
@@ -148,10 +211,15 @@ __global__ __launch_bounds__(256) void k_march8(const RenderArgs a) {
         (int64_t)xcd_slice_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
     if (b >= a.len) return;
     const int ray = uniform_i(a.sample_index[a.start + b]);
-    const bool do_fwd = MODE == kFwd || MODE == kFused || MODE == kTrace;
+    const bool do_fwd = MODE == kFwd || MODE == kFused || MODE == kTrace || MODE == kLog;
     const bool do_bwd = MODE == kBwd || MODE == kFused;
+    const bool logging = MODE == kLog;
+    // Second fused launch of the logged step: only the rays the log could
+    // not hold.
+    if (MODE == kFused && a.log_redo && uniform_i(a.log_count[b]) != -1) return;
     if (ray < 0 || (int64_t)ray >= a.rays.n) {
-        if (do_bwd && lane == 0) a.losses[b] = 0.0;
+        if ((do_bwd || logging) && lane == 0) a.losses[b] = 0.0;
+        if (logging && lane == 0) a.log_count[b] = 0;
         if (MODE == kTrace && lane == 0) {
             a.trace[2 * b] = 0u;
             a.trace[2 * b + 1] = 2166136261u;
@@ -166,7 +234,7 @@ __global__ __launch_bounds__(256) void k_march8(const RenderArgs a) {
     const float fstep = a.step;
 
     if (tnear > tfar) {
-        if ((MODE == kFwd || MODE == kFused || MODE == kTrace) && lane == 0) {
+        if (do_fwd && lane == 0) {
             a.rays.acc_r[ray] = 0.0f;
             a.rays.acc_g[ray] = 0.0f;
             a.rays.acc_b[ray] = 0.0f;
@@ -177,7 +245,8 @@ __global__ __launch_bounds__(256) void k_march8(const RenderArgs a) {
             a.trace[2 * b] = 0u;
             a.trace[2 * b + 1] = 2166136261u;
         }
-        if (do_bwd && lane == 0) a.losses[b] = 0.0;
+        if ((do_bwd || logging) && lane == 0) a.losses[b] = 0.0;
+        if (logging && lane == 0) a.log_count[b] = 0;
         return;
     }
 
@@ -196,7 +265,8 @@ __global__ __launch_bounds__(256) void k_march8(const RenderArgs a) {
 
     // Backward coefficients (set after the forward / from the ray planes).
     double C0 = 0.0, C1 = 0.0, C2 = 0.0, A = 0.0;
-    double r0 = 0.0, r1 = 0.0, r2 = 0.0, mterm = 0.0;
+    RayResid rr = {0.0, 0.0, 0.0, 0.0, 0.0};
+    int logged = 0;  // samples written to the log (logging mode)
 
     // pass 0 = forward, pass 1 = backward re-march.
     for (int pass = do_fwd ? 0 : 1; pass < (do_bwd ? 2 : 1); pass++) {
@@ -208,26 +278,11 @@ __global__ __launch_bounds__(256) void k_march8(const RenderArgs a) {
                 C2 = a.rays.acc_b[ray];
                 A = a.rays.acc_w[ray];
             }
-            r0 = C0 - (double)a.rays.gt_r[ray] / 255.0;
-            r1 = C1 - (double)a.rays.gt_g[ray] / 255.0;
-            r2 = C2 - (double)a.rays.gt_b[ray] / 255.0;
-            const double loss = (fabs(r0) + fabs(r1) + fabs(r2)) * 255.0 / 3.0;
-            if (isnan(loss)) {
+            if (!ray_residuals(a, ray, C0, C1, C2, A, rr)) {
                 if (lane == 0) a.losses[b] = 255.0;
                 return;
             }
-            if (lane == 0) a.losses[b] = loss;
-            const double hd = (double)a.huber;
-            const double hi = hd - 0.5 * hd * hd, lo = 0.5 * hd * hd - hd;
-            r0 = r0 > hd ? hi : (r0 < -hd ? lo : r0);
-            r1 = r1 > hd ? hi : (r1 < -hd ? lo : r1);
-            r2 = r2 > hd ? hi : (r2 < -hd ? lo : r2);
-            const int gm = a.rays.gt_mask[ray];
-            mterm = 0.0;
-            if (gm != 0 && A < 1.0) mterm = ((double)gm / 255.0 - A) * a.mask_coef;
-            double wpre = A > 1e-6 ? A : 1e-6;
-            wpre = wpre < 1.0 - 1e-6 ? wpre : 1.0 - 1e-6;
-            mterm += (log(wpre) - log(1.0 - wpre)) * 1e-3;
+            if (lane == 0) a.losses[b] = rr.loss;
         }
 
         // Running totals before the current group (uniform, FP64).
@@ -238,6 +293,7 @@ __global__ __launch_bounds__(256) void k_march8(const RenderArgs a) {
         int cur_x = 0, cur_y = 0, cur_z = 0;
         uint32_t cur_rec[4] = {0u, 0u, 0u, 0u};
         float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        float acc0 = 0.0f;
         uint32_t hcount = 0, hash = 2166136261u;
         int k0 = 0;
         uint64_t pending = 0;  // occupied positions of [k0, k0 + 64) not yet done
@@ -345,6 +401,22 @@ __global__ __launch_bounds__(256) void k_march8(const RenderArgs a) {
             const double total_keep = __shfl(incl, 63, 64);
 
             if (!bwd) {
+                if (logging) {
+                    // Lane (s, j): record of corner j, and word j of the
+                    // sample's (sraw, col0, col1, col2, k) row.
+                    const int i = logged + slot;
+                    if (active && i < a.log_cap) {
+                        const size_t at = ((size_t)b * (size_t)a.log_cap + (size_t)i) * 8 + corner;
+                        a.log_rec[at] = rec;
+                        if (corner < 5)
+                            a.log_val[at] = corner == 0   ? sraw
+                                            : corner == 1 ? col0
+                                            : corner == 2 ? col1
+                                            : corner == 3 ? col2
+                                                          : __int_as_float(kslot);
+                    }
+                    logged += n;
+                }
                 // Forward totals: xor-sums over the slots of one corner column
                 // (every column holds the same per-slot values).
                 Acc += slot_sum(ws);
@@ -360,15 +432,10 @@ __global__ __launch_bounds__(256) void k_march8(const RenderArgs a) {
                 const double H1 = Cg + slot_scan_add(ws * (double)col1, lane);
                 const double H2 = Cb + slot_scan_add(ws * (double)col2, lane);
                 const double Tn = Ts * keep;
-                const double kc = (double)0.6667f;
-                const float g0 = (float)(kc * ws * r0 * (double)(0.1f * col0 * (1.0f - col0)));
-                const float g1 = (float)(kc * ws * r1 * (double)(0.1f * col1 * (1.0f - col1)));
-                const float g2 = (float)(kc * ws * r2 * (double)(0.1f * col2 * (1.0f - col2)));
-                const double ds = (double)dsoftplus_shift<float>(sraw);
-                double gs = r0 * (Tn * (double)col0 - (C0 - H0)) + r1 * (Tn * (double)col1 - (C1 - H1)) +
-                            r2 * (Tn * (double)col2 - (C2 - H2));
-                gs *= 0.6667 * step * ds;
-                gs += step * ds * (A - Ai - Tn) * mterm;
+                float g0, g1, g2;
+                double gs;
+                sample_grads(rr, C0, C1, C2, A, step, ws, Tn, Ai, H0, H1, H2, col0, col1, col2,
+                             sraw, g0, g1, g2, gs);
                 // Accumulate the slots in order.  A slot in the current cell
                 // adds to acc; a slot in another cell first retires the
                 // current one: corners the two cells share (same vertex, so
@@ -384,8 +451,8 @@ __global__ __launch_bounds__(256) void k_march8(const RenderArgs a) {
                     const int ny = __builtin_amdgcn_readlane(by, sb);
                     const int nz = __builtin_amdgcn_readlane(bz, sb);
                     if (!has_cur || nx != cur_x || ny != cur_y || nz != cur_z) {
-                        if (has_cur) retire_cell<P>(a, acc, cur_rec, nx - cur_x, ny - cur_y,
-                                                    nz - cur_z, lh, pp);
+                        if (has_cur) retire_cell<P>(a, acc, cur_rec, acc0, nx - cur_x,
+                                                    ny - cur_y, nz - cur_z, lh, pp);
                         has_cur = true;
                         cur_x = nx;
                         cur_y = ny;
@@ -417,7 +484,8 @@ __global__ __launch_bounds__(256) void k_march8(const RenderArgs a) {
 
         if (bwd && has_cur) {
             // Retire the last cell: a displacement of 2 shares no corner.
-            retire_cell<P>(a, acc, cur_rec, 2, 2, 2, lh, pp);
+            retire_cell<P>(a, acc, cur_rec, acc0, 2, 2, 2, lh, pp);
+            flush_record0<P>(a, acc0, lh, pp);
         }
 
         if (!bwd) {
@@ -441,6 +509,166 @@ __global__ __launch_bounds__(256) void k_march8(const RenderArgs a) {
                     a.trace[2 * b + 1] = hash;
                 }
             }
+            if (logging) {
+                if (logged <= a.log_cap) {
+                    // Whole ray logged: the loss here, the gradient in
+                    // k_log_grad.
+                    const bool ok = ray_residuals(a, ray, C0, C1, C2, A, rr);
+                    if (lane == 0) {
+                        a.losses[b] = ok ? rr.loss : 255.0;
+                        a.log_count[b] = ok ? logged : 0;
+                        a.log_tot[4 * b + 0] = C0;
+                        a.log_tot[4 * b + 1] = C1;
+                        a.log_tot[4 * b + 2] = C2;
+                        a.log_tot[4 * b + 3] = A;
+                    }
+                    return;
+                }
+                // Longer than the log: the fused marcher redoes this ray.
+                if (lane == 0) a.log_count[b] = -1;
+                return;
+            }
         }
+    }
+}
+
+// Inclusive scans over the 64 lanes of a wave in FP64.
+__device__ __forceinline__ double wave_scan_add64(double v, int lane) {
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const double y = __shfl_up(v, off, 64);
+        if (lane >= off) v += y;
+    }
+    return v;
+}
+__device__ __forceinline__ double wave_scan_mul64(double v, int lane) {
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const double y = __shfl_up(v, off, 64);
+        if (lane >= off) v *= y;
+    }
+    return v;
+}
+
+// Gradient of the rays k_march8 logged (second kernel of the two-kernel
+// fused step).  One wave per batch entry; 64 logged samples at a time, one
+// per lane: sigma, alpha and the colours come from the logged raw values
+// through the same float functions as the forward, transmittance and the
+// opacity / colour prefixes are FP64 wave scans, the gradient scalars are
+// those of sample_grads.  The scattering then runs as in the marcher's own
+// backward (register accumulators, corner carry), but with every input
+// already in registers: no global load follows an atomic until the next 64
+// samples, so the atomics never hold up the wave.
+template <int S>
+__global__ __launch_bounds__(256) void k_log_grad(const RenderArgs a) {
+    constexpr int P = 1 + 3 * S;
+    const int lane = threadIdx.x & 63;
+    const int64_t b =
+        (int64_t)xcd_slice_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
+    if (b >= a.len) return;
+    const int cnt = uniform_i(a.log_count[b]);
+    if (cnt <= 0) return;  // nothing logged, or done by the marcher
+    const int ray = uniform_i(a.sample_index[a.start + b]);
+    const double ox = a.rays.ox[ray], oy = a.rays.oy[ray], oz = a.rays.oz[ray];
+    const float fdx = a.rays.dx[ray], fdy = a.rays.dy[ray], fdz = a.rays.dz[ray];
+    const double dx = fdx, dy = fdy, dz = fdz;
+    const double tnear = a.rays.near_t[ray];
+    const double step = (double)a.step;
+    const float fstep = a.step;
+    const double C0 = a.log_tot[4 * b + 0], C1 = a.log_tot[4 * b + 1],
+                 C2 = a.log_tot[4 * b + 2], A = a.log_tot[4 * b + 3];
+    RayResid rr;
+    ray_residuals(a, ray, C0, C1, C2, A, rr);  // finite: the marcher logged it
+
+    float Y[S];
+#pragma unroll
+    for (int k = 0; k < S; k++) Y[k] = sh_basis(k, fdx, fdy, fdz);
+    const int lh = lane >> 5, pp = lane & 31;
+    const int pch = pp == 0 ? -1 : (pp - 1) / S;
+    float ysel = 0.0f;
+#pragma unroll
+    for (int k = 0; k < S; k++) ysel = (pp >= 1 && (pp - 1) % S == k) ? Y[k] : ysel;
+
+    double T = 1.0, Acc = 0.0, Cr = 0.0, Cg = 0.0, Cb = 0.0;
+    bool has_cur = false;
+    int cur_x = 0, cur_y = 0, cur_z = 0;
+    uint32_t cur_rec[4] = {0u, 0u, 0u, 0u};
+    float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    float acc0 = 0.0f;
+    for (int c0 = 0; c0 < cnt; c0 += 64) {
+        const int m = min(64, cnt - c0);
+        const bool act = lane < m;
+        const size_t at = ((size_t)b * (size_t)a.log_cap + (size_t)(c0 + (act ? lane : 0))) * 8;
+        const float4 v = *reinterpret_cast<const float4 *>(a.log_val + at);
+        const int k = __float_as_int(a.log_val[at + 4]);
+        const uint4 q0 = *reinterpret_cast<const uint4 *>(a.log_rec + at);
+        const uint4 q1 = *reinterpret_cast<const uint4 *>(a.log_rec + at + 4);
+        const uint32_t recs[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
+        const float sraw = v.x, col0 = v.y, col1 = v.z, col2 = v.w;
+
+        const float sigma = act ? softplus_shift<float>(sraw) : 0.0f;
+        const float alpha = act ? one_minus_exp_neg<float>(sigma * fstep) : 0.0f;
+        const double keep = 1.0 - (double)alpha;
+        const double incl = wave_scan_mul64(keep, lane);
+        double excl = __shfl_up(incl, 1, 64);
+        if (lane == 0) excl = 1.0;
+        const double Ts = T * excl;
+        const double ws = Ts * (double)alpha;
+        const double Ai = Acc + wave_scan_add64(ws, lane);
+        const double H0 = Cr + wave_scan_add64(ws * (double)col0, lane);
+        const double H1 = Cg + wave_scan_add64(ws * (double)col1, lane);
+        const double H2 = Cb + wave_scan_add64(ws * (double)col2, lane);
+        const double Tn = Ts * keep;
+        float g0, g1, g2;
+        double gs;
+        sample_grads(rr, C0, C1, C2, A, step, ws, Tn, Ai, H0, H1, H2, col0, col1, col2, sraw, g0,
+                     g1, g2, gs);
+        const float gsf = (float)gs;
+        // Position of sample k exactly as the marcher computed it.
+        const double t = fma((double)(act ? k : 0), step, tnear);
+        const double px = fma(t, dx, ox), py = fma(t, dy, oy), pz = fma(t, dz, oz);
+        const int bx = base_of(px), by = base_of(py), bz = base_of(pz);
+        const float fx = (float)(px - (double)bx), fy = (float)(py - (double)by),
+                    fz = (float)(pz - (double)bz);
+
+        for (int i = 0; i < m; i++) {
+            const int nx = __builtin_amdgcn_readlane(bx, i);
+            const int ny = __builtin_amdgcn_readlane(by, i);
+            const int nz = __builtin_amdgcn_readlane(bz, i);
+            if (!has_cur || nx != cur_x || ny != cur_y || nz != cur_z) {
+                if (has_cur)
+                    retire_cell<P>(a, acc, cur_rec, acc0, nx - cur_x, ny - cur_y, nz - cur_z, lh,
+                                   pp);
+                has_cur = true;
+                cur_x = nx;
+                cur_y = ny;
+                cur_z = nz;
+#pragma unroll
+                for (int h = 0; h < 4; h++) {
+                    const uint32_t e = (uint32_t)__builtin_amdgcn_readlane((int)recs[2 * h], i);
+                    const uint32_t o = (uint32_t)__builtin_amdgcn_readlane((int)recs[2 * h + 1], i);
+                    cur_rec[h] = lh ? o : e;
+                }
+            }
+            const float sfx = readlane_f(fx, i), sfy = readlane_f(fy, i), sfz = readlane_f(fz, i);
+            const float sgs = readlane_f(gsf, i), sg0 = readlane_f(g0, i),
+                        sg1 = readlane_f(g1, i), sg2 = readlane_f(g2, i);
+            const float coef = pch < 0 ? sgs : (pch == 0 ? sg0 : (pch == 1 ? sg1 : sg2)) * ysel;
+            const float wx = lh ? sfx : 1.0f - sfx;
+#pragma unroll
+            for (int h = 0; h < 4; h++) {
+                const float wyz = ((h & 1) ? sfy : 1.0f - sfy) * ((h >> 1) ? sfz : 1.0f - sfz);
+                acc[h] = fmaf(wx * wyz, coef, acc[h]);
+            }
+        }
+        T *= readlane_d(incl, 63);
+        Acc = readlane_d(Ai, 63);
+        Cr = readlane_d(H0, 63);
+        Cg = readlane_d(H1, 63);
+        Cb = readlane_d(H2, 63);
+    }
+    if (has_cur) {
+        retire_cell<P>(a, acc, cur_rec, acc0, 2, 2, 2, lh, pp);
+        flush_record0<P>(a, acc0, lh, pp);
     }
 }

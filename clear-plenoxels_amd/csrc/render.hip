@@ -23,7 +23,9 @@
 namespace pxl {
 namespace {
 
-enum Mode { kFwd = 0, kBwd = 1, kFused = 2, kTrace = 3, kTouch = 4 };
+// kLog: forward plus the sample log of the two-kernel fused step (group
+// marcher only).
+enum Mode { kFwd = 0, kBwd = 1, kFused = 2, kTrace = 3, kTouch = 4, kLog = 5 };
 
 // Hard cap on marching iterations per ray.  Legitimate rays need at most a
 // few thousand (diagonal / step); the cap only bounds rays whose far
@@ -50,6 +52,16 @@ struct RenderArgs {
     float huber;
     uint32_t *trace;
     uint32_t *bitmap;
+    // Sample log of the two-kernel fused path (all null when unused):
+    // FP64 forward totals C0, C1, C2, A per batch entry, logged sample count
+    // (-1: backward done by the marcher itself), and per logged sample the 8
+    // corner records and (sraw, col0, col1, col2, k) padded to 8 words.
+    double *log_tot;
+    int32_t *log_count;
+    uint32_t *log_rec;
+    float *log_val;
+    int log_cap;
+    int log_redo;  // kFused: only the batch entries whose count is -1
 };
 
 // Activations (RenderUtils.cu:480-515) in float or double.
@@ -663,11 +675,78 @@ extern "C" int32_t pxl_render_fwd_trace(const pxl_grid *grid, const pxl_rays *ra
     return st != PXL_OK ? st : check_launch("pxl_render_fwd_trace");
 }
 
+namespace pxl {
+namespace {
+// Workspace of the two-kernel fused step: per batch entry 4 FP64 totals and a
+// count, per logged sample 8 records and 8 value words; 256-byte aligned.
+struct LogLayout {
+    size_t tot, count, rec, val, total;
+};
+inline size_t align256(size_t x) { return (x + 255) / 256 * 256; }
+LogLayout log_layout(int64_t len, int32_t cap) {
+    LogLayout L;
+    size_t o = 0;
+    L.tot = o;
+    o += align256((size_t)len * 4 * sizeof(double));
+    L.count = o;
+    o += align256((size_t)len * sizeof(int32_t));
+    L.rec = o;
+    o += align256((size_t)len * (size_t)cap * 8 * sizeof(uint32_t));
+    L.val = o;
+    o += align256((size_t)len * (size_t)cap * 8 * sizeof(float));
+    L.total = o;
+    return L;
+}
+constexpr int32_t kMaxLogSamples = 1 << 16;
+
+// Forward + log, gradient of the logged rays, then the fused marcher again
+// for the rays longer than the log (its waves leave at once for the others).
+template <int S, bool SPARSE, int STORE>
+void launch_logged(const RenderArgs &a, hipStream_t st) {
+    const unsigned blocks = (unsigned)((a.len + 3) / 4);
+    hipLaunchKernelGGL((k_march8<S, SPARSE, kLog, STORE>), dim3(blocks), dim3(256), 0, st, a);
+    hipLaunchKernelGGL((k_log_grad<S>), dim3(blocks), dim3(256), 0, st, a);
+    RenderArgs redo = a;
+    redo.log_redo = 1;
+    hipLaunchKernelGGL((k_march8<S, SPARSE, kFused, STORE>), dim3(blocks), dim3(256), 0, st,
+                       redo);
+}
+template <int S>
+int32_t dispatch_logged_s(bool sparse, int store, const RenderArgs &a, hipStream_t st) {
+#define PXL_LOGGED(SP, ST)                      \
+    if (sparse == SP && store == ST) {          \
+        launch_logged<S, SP, ST>(a, st);        \
+        return PXL_OK;                          \
+    }
+    PXL_LOGGED(true, PXL_STORE_F32)
+    PXL_LOGGED(true, PXL_STORE_BF16)
+    PXL_LOGGED(true, PXL_STORE_F16)
+    PXL_LOGGED(false, PXL_STORE_F32)
+    PXL_LOGGED(false, PXL_STORE_BF16)
+    PXL_LOGGED(false, PXL_STORE_F16)
+#undef PXL_LOGGED
+    set_error("render: unknown storage %d", store);
+    return PXL_EINVAL;
+}
+int32_t dispatch_logged(int S, bool sparse, int store, const RenderArgs &a, hipStream_t st) {
+    switch (S) {
+        case 1: return dispatch_logged_s<1>(sparse, store, a, st);
+        case 4: return dispatch_logged_s<4>(sparse, store, a, st);
+        case 9: return dispatch_logged_s<9>(sparse, store, a, st);
+        default:
+            set_error("render: the logged fused step supports sh_dim 1, 4 and 9");
+            return PXL_EINVAL;
+    }
+}
+}  // namespace
+}  // namespace pxl
+
 static int32_t bwd_common(const pxl_grid *grid, const pxl_optim *optim, const pxl_rays *rays,
                           double *losses, const int32_t *sample_index, int64_t start,
                           int64_t len, double mask_coef, float huber_delta, void *stream,
-                          bool fused) {
-    const char *who = fused ? "pxl_train_rays" : "pxl_render_bwd_rgb";
+                          bool fused, void *workspace = nullptr, size_t workspace_bytes = 0,
+                          int32_t max_samples = 0) {
+    const char *who = workspace ? "pxl_train_rays_ws" : fused ? "pxl_train_rays" : "pxl_render_bwd_rgb";
     clear_error();
     RenderArgs a;
     int32_t st = prepare(grid, rays, sample_index, start, len, who, a);
@@ -690,6 +769,24 @@ static int32_t bwd_common(const pxl_grid *grid, const pxl_optim *optim, const px
     a.mask_coef = mask_coef;
     a.huber = huber_delta;
     const bool adapt = grid->march_mode == PXL_MARCH_REF_ADAPTIVE;
+    if (workspace && !adapt && grid->sh_dim <= 9 && use_march8()) {
+        PXL_REQUIRE(max_samples > 0 && max_samples <= kMaxLogSamples,
+                    "%s: max_samples %d outside [1, %d]", who, max_samples, kMaxLogSamples);
+        const LogLayout L = log_layout(len, max_samples);
+        PXL_REQUIRE(workspace_bytes >= L.total,
+                    "%s: workspace of %zu bytes, %zu needed for %lld rays x %d samples", who,
+                    workspace_bytes, L.total, (long long)len, max_samples);
+        PXL_REQUIRE(((uintptr_t)workspace & 255) == 0, "%s: workspace must be 256-byte aligned",
+                    who);
+        char *w = static_cast<char *>(workspace);
+        a.log_tot = reinterpret_cast<double *>(w + L.tot);
+        a.log_count = reinterpret_cast<int32_t *>(w + L.count);
+        a.log_rec = reinterpret_cast<uint32_t *>(w + L.rec);
+        a.log_val = reinterpret_cast<float *>(w + L.val);
+        a.log_cap = max_samples;
+        st = dispatch_logged(grid->sh_dim, grid->sparse, grid->storage, a, as_stream(stream));
+        return st != PXL_OK ? st : check_launch(who);
+    }
     st = fused ? dispatch<kFused>(grid->sh_dim, grid->sparse, adapt, grid->storage, a,
                                   as_stream(stream))
                : dispatch<kBwd>(grid->sh_dim, grid->sparse, adapt, grid->storage, a,
@@ -711,6 +808,25 @@ extern "C" int32_t pxl_train_rays(const pxl_grid *grid, const pxl_optim *optim,
                                   double mask_coef, float huber_delta, void *stream) {
     return bwd_common(grid, optim, rays, losses, sample_index, start, len, mask_coef,
                       huber_delta, stream, true);
+}
+
+extern "C" size_t pxl_train_rays_workspace_bytes(int64_t len, int32_t max_samples) {
+    if (len <= 0 || max_samples <= 0 || max_samples > kMaxLogSamples) return 0;
+    return log_layout(len, max_samples).total;
+}
+
+extern "C" int32_t pxl_train_rays_ws(const pxl_grid *grid, const pxl_optim *optim,
+                                     const pxl_rays *rays, double *losses,
+                                     const int32_t *sample_index, int64_t start, int64_t len,
+                                     double mask_coef, float huber_delta, void *workspace,
+                                     size_t workspace_bytes, int32_t max_samples, void *stream) {
+    if (!workspace) {
+        clear_error();
+        set_error("pxl_train_rays_ws: null workspace");
+        return PXL_EINVAL;
+    }
+    return bwd_common(grid, optim, rays, losses, sample_index, start, len, mask_coef,
+                      huber_delta, stream, true, workspace, workspace_bytes, max_samples);
 }
 
 namespace pxl {

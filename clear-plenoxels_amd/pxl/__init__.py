@@ -32,7 +32,7 @@ EXPORTS = (
     "pxl_trainer_time_render", "pxl_trainer_render_ms", "pxl_trainer_loss",
     "pxl_trainer_state", "pxl_trainer_destroy", "pxl_brick_words",
     "pxl_build_brick_occupancy", "pxl_records_to_16bit", "pxl_tv_prepare",
-    "pxl_render_kernel_name",
+    "pxl_render_kernel_name", "pxl_train_rays_workspace_bytes", "pxl_train_rays_ws",
 )
 
 
@@ -112,7 +112,8 @@ class TrainConfigC(ctypes.Structure):
                 ("variance_coefficient", _F64), ("sh_coefficient", _F64),
                 ("variance_type", _I32), ("neighbour_nums", _I32),
                 ("variance_interval", _I32), ("optimize_step", _I32),
-                ("mask_coefficient", _F64), ("huber_delta", _F32), ("fused", _I32)]
+                ("mask_coefficient", _F64), ("huber_delta", _F32), ("fused", _I32),
+                ("log_samples", _I32), ("reserved", _I32)]
 
 
 def _declare(L):
@@ -154,6 +155,9 @@ def _declare(L):
         "pxl_build_brick_occupancy": ([_P, _I32, _I32, _I32, _P, _P], _I32),
         "pxl_records_to_16bit": ([_P, _I32, _I64, _P, _I32, _I32, _P], _I32),
         "pxl_render_kernel_name": ([G, _I32], ctypes.c_char_p),
+        "pxl_train_rays_workspace_bytes": ([_I64, _I32], _SZ),
+        "pxl_train_rays_ws": ([G, O, R, _P, _P, _I64, _I64, _F64, _F32, _P, _SZ, _I32, _P],
+                              _I32),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -366,6 +370,31 @@ def train_rays(grid, optim, rays, losses, sample_index, start=0, length=None, ma
                                 _stream(stream)))
 
 
+def train_workspace(length, max_samples, device="cuda"):
+    """Device workspace for train_rays_ws over `length` rays logging up to
+    `max_samples` samples per ray."""
+    import torch
+
+    nbytes = lib().pxl_train_rays_workspace_bytes(length, max_samples)
+    if nbytes == 0:
+        raise PxlError("pxl_train_rays_workspace_bytes", PXL_EINVAL,
+                       f"bad length {length} / max_samples {max_samples}")
+    return torch.empty(nbytes, dtype=torch.uint8, device=device)
+
+
+def train_rays_ws(grid, optim, rays, losses, sample_index, workspace, max_samples, start=0,
+                  length=None, mask_coef=1.0, huber_delta=0.1, stream=None):
+    """train_rays as two kernels with a sample log in `workspace`
+    (train_workspace(length, max_samples))."""
+    length = sample_index.numel() - start if length is None else length
+    g, o, r = grid.c(), optim.c(), rays.c()
+    _check("pxl_train_rays_ws",
+           lib().pxl_train_rays_ws(ctypes.byref(g), ctypes.byref(o), ctypes.byref(r),
+                                   _ptr(losses), _ptr(sample_index), start, length, mask_coef,
+                                   huber_delta, _ptr(workspace), workspace.numel(), max_samples,
+                                   _stream(stream)))
+
+
 def tv_scratch(grid, stream=None):
     """The grid's prepared variance-loss scratch (sparse grids; created and
     filled by pxl_tv_prepare on first use, kept on the Grid object)."""
@@ -503,7 +532,7 @@ class Trainer:
                  learning_rate=0.3, sh_learning_rate=1.0, variance_coefficient=1e-5,
                  sh_coefficient=1e-6, variance_type=TV_DISPARITY, neighbour_nums=6,
                  variance_interval=3, optimize_step=0, mask_coefficient=1.0, huber_delta=0.1,
-                 fused=True, stream=None):
+                 fused=True, log_samples=0, stream=None):
         import torch
 
         self._keep = (grid, optim, rays, order)
@@ -512,7 +541,8 @@ class Trainer:
         self.loss_out = torch.zeros(2, dtype=torch.float64, device=dev)
         cfg = TrainConfigC(batch_size, learning_rate, sh_learning_rate, variance_coefficient,
                            sh_coefficient, variance_type, neighbour_nums, variance_interval,
-                           optimize_step, mask_coefficient, huber_delta, int(fused))
+                           optimize_step, mask_coefficient, huber_delta, int(fused),
+                           int(log_samples), 0)
         self._g, self._o, self._r = grid.c(), optim.c(), rays.c()
         n_rays = order.numel() if n_rays is None else n_rays
         sc = tv_scratch(grid, stream) if variance_type == TV_DISPARITY else None

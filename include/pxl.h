@@ -293,6 +293,25 @@ int32_t pxl_count_touched(const pxl_grid *grid, const pxl_rays *rays,
                           int64_t len, uint32_t *bitmap, uint64_t *count,
                           void *stream);
 
+/* The fused step as two kernels with a sample log in between (same
+ * arguments and results as pxl_train_rays).  The marcher renders each ray
+ * and writes, per sample, its 8 corner records and raw density / colours
+ * into `workspace`; a second kernel then scatters the gradient of every
+ * logged ray with no global load between its atomics.  A ray with more than
+ * max_samples evaluated samples is differentiated by the marcher itself.
+ * Grids the eight-sample marcher does not serve (adaptive rule, S = 16) run
+ * as pxl_train_rays.  workspace: 256-byte aligned device memory of at least
+ * pxl_train_rays_workspace_bytes(len, max_samples) bytes; calls sharing a
+ * workspace must not overlap.  No counterpart in the reference (an
+ * implementation of renderRays_withGradients_RGB, RenderUtils.cu:2257-2278,
+ * fused with renderRays). */
+size_t pxl_train_rays_workspace_bytes(int64_t len, int32_t max_samples);
+int32_t pxl_train_rays_ws(const pxl_grid *grid, const pxl_optim *optim,
+                          const pxl_rays *rays, double *losses,
+                          const int32_t *sample_index, int64_t start, int64_t len,
+                          double mask_coef, float huber_delta, void *workspace,
+                          size_t workspace_bytes, int32_t max_samples, void *stream);
+
 /* Name of the kernel the render entry points launch for this grid:
  * mode 0 = pxl_render_fwd, 1 = pxl_render_bwd_rgb, 2 = pxl_train_rays.
  * Returns a thread-local string such as "k_march8<9,sparse,fused,f32>" (or
@@ -322,6 +341,10 @@ typedef struct pxl_train_config {
     double mask_coefficient;
     float huber_delta;
     int32_t fused;           /* 1: fused forward+backward launch per batch */
+    int32_t log_samples;     /* fused only: > 0 runs pxl_train_rays_ws with a
+                                trainer-owned workspace logging up to this
+                                many samples per ray; 0 runs pxl_train_rays */
+    int32_t reserved;
 } pxl_train_config;
 
 typedef struct pxl_trainer pxl_trainer;

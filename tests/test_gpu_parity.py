@@ -8,6 +8,8 @@ plane, the floor covering entries that cancel to ~0 under FP32 atomics).
 """
 import sys
 
+import os
+
 import numpy as np
 import pytest
 
@@ -194,9 +196,17 @@ def test_render_fwd_adaptive(c1):
     _fwd_compare(grid_o, c1["rays_np"], c1["batch"][:1024], got, tr)
 
 
-def _bwd_case(c1, grid, grid_o, nr, fused, huber=0.1):
+# Backward entry points: the separate forward + backward kernels, the fused
+# marcher, and the two-kernel logged step with a log that holds every ray
+# (512 samples, several 64-sample chunks per ray) or only short rays (24
+# samples: the longer rays go through the fused marcher's redo launch).
+BWD_PATHS = ["separate", "fused", "logged", "logged_24"]
+
+
+def _bwd_case(c1, grid, grid_o, nr, path, huber=0.1):
     import torch
 
+    fused = path != "separate"
     dev = c1["dev"]
     rays = device_rays(c1["rays_np"], dev)
     si = c1["si"][:nr]
@@ -204,7 +214,25 @@ def _bwd_case(c1, grid, grid_o, nr, fused, huber=0.1):
     optim = pxl.Optim(grid)
     losses = torch.zeros(nr, dtype=torch.float64, device=dev)
     rays_o = oracle.Rays(c1["rays_np"])
-    if fused:
+    if path.startswith("logged"):
+        cap = 512 if path == "logged" else 24
+        ws = pxl.train_workspace(nr, cap, dev)
+        ws.fill_(0xAB)  # stale contents must not leak into the result
+        pxl.train_rays_ws(grid, optim, rays, losses, si, ws, cap, mask_coef=1.0,
+                          huber_delta=huber)
+        torch.cuda.synchronize()
+        fwd = oracle.render_fwd(grid_o, rays_o, batch)
+        if (os.environ.get("PXL_MARCHER", "group") != "wave"
+                and grid.march_mode == pxl.MARCH_FIXED):
+            # Per-ray counts follow the FP64 totals (32 bytes per ray, 256-byte
+            # aligned): both the logged and the redo route must be taken.
+            off = (nr * 32 + 255) // 256 * 256
+            cnt = ws[off:off + 4 * nr].view(torch.int32).cpu().numpy()
+            n_logged, n_redo = int((cnt > 0).sum()), int((cnt == -1).sum())
+            say(f"    logged rays {n_logged}, redone rays {n_redo}")
+            assert n_logged > 0 and set(np.unique(cnt)) <= set(range(-1, cap + 1))
+            assert (n_redo > 0) == (cap == 24)
+    elif fused:
         pxl.train_rays(grid, optim, rays, losses, si, mask_coef=1.0, huber_delta=huber)
         torch.cuda.synchronize()
         fwd = oracle.render_fwd(grid_o, rays_o, batch)
@@ -232,30 +260,32 @@ def _bwd_case(c1, grid, grid_o, nr, fused, huber=0.1):
     return optim
 
 
-@pytest.mark.parametrize("fused", [False, True])
-def test_backward_dense(c1, fused, marcher):
-    say(f"[bwd] dense 128^3, 1024 rays, fused={fused}, {marcher} marcher")
+@pytest.mark.parametrize("path", BWD_PATHS)
+def test_backward_dense(c1, path, marcher):
+    say(f"[bwd] dense 128^3, 1024 rays, {path}, {marcher} marcher")
     grid = pxl.Grid(N1, N1, N1, S1, c1["rec"], step=0.5)
     grid_o = oracle.Grid(N1, N1, N1, S1, c1["planes"], 0.5)
-    _bwd_case(c1, grid, grid_o, 1024, fused)
+    _bwd_case(c1, grid, grid_o, 1024, path)
 
 
-@pytest.mark.parametrize("fused", [False, True])
-def test_backward_sparse(c1, sparse1, fused, marcher):
+@pytest.mark.parametrize("path", BWD_PATHS)
+def test_backward_sparse(c1, sparse1, path, marcher):
     s = sparse1
-    say(f"[bwd] sparse 128^3, 1024 rays, fused={fused}, {marcher} marcher")
+    say(f"[bwd] sparse 128^3, 1024 rays, {path}, {marcher} marcher")
     grid = device_sparse_grid(N1, S1, s["compact"], s["occ"], s["io"], s["cage"], s["c2d"],
                               c1["dev"])
     grid_o = oracle.Grid(N1, N1, N1, S1, s["compact"], 0.5, occupancy=s["occ"],
                          index_offsets=s["io"])
-    _bwd_case(c1, grid, grid_o, 1024, fused)
+    _bwd_case(c1, grid, grid_o, 1024, path)
 
 
 def test_backward_adaptive_unclipped(c1):
     say("[bwd] dense adaptive, 256 rays, no clipping")
     grid = pxl.Grid(N1, N1, N1, S1, c1["rec"], step=0.5, march_mode=pxl.MARCH_REF_ADAPTIVE)
     grid_o = oracle.Grid(N1, N1, N1, S1, c1["planes"], 0.5, adaptive=True)
-    _bwd_case(c1, grid, grid_o, 256, True, huber=10.0)
+    _bwd_case(c1, grid, grid_o, 256, "fused", huber=10.0)
+    # The logged step serves only the fixed rule: it runs as the fused one.
+    _bwd_case(c1, grid, grid_o, 256, "logged", huber=10.0)
 
 
 @pytest.mark.parametrize("kind", ["dense", "sparse"])
@@ -401,15 +431,17 @@ def test_train_step_c1(c1):
     assert diff.max() <= 0.2 + 1e-6
 
 
-def test_trainer_matches_stage_calls(c1):
+@pytest.mark.parametrize("log_samples", [0, 512])
+def test_trainer_matches_stage_calls(c1, log_samples):
     """The C++ batch loop (pxl_trainer_step) issues the same stages as the
     Python-driven sequence: dense grid (TV every batch), 3 batches, LR decay
-    lr /= 2^(1/(batchCount (epoch+1))) per batch (Render.cu:499-500)."""
+    lr /= 2^(1/(batchCount (epoch+1))) per batch (Render.cu:499-500); with
+    log_samples the trainer runs the two-kernel logged step."""
     import math
 
     import torch
 
-    say("[trainer] 3 batches vs stage calls")
+    say(f"[trainer] 3 batches vs stage calls, log_samples={log_samples}")
     dev = c1["dev"]
     B = 1024
     order = c1["si"]
@@ -424,7 +456,7 @@ def test_trainer_matches_stage_calls(c1):
     rays_b = device_rays(c1["rays_np"], dev)
     tr = pxl.Trainer(grids[0], optims[0], rays_a, order, batch_size=B, learning_rate=0.1,
                      sh_learning_rate=0.2, variance_coefficient=1e-5, sh_coefficient=1e-6,
-                     variance_interval=3, fused=True)
+                     variance_interval=3, fused=True, log_samples=log_samples)
     for _ in range(3):
         tr.step()
     losses = torch.zeros(B, dtype=torch.float64, device=dev)
