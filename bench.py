@@ -252,7 +252,7 @@ def profiled_traffic(kernel, args):
     `traffic_bytes_per_launch <kernel> <bytes>`); None for any other
     workload or kernel, or when the summary holds no figure for it."""
     default = (args.grid, args.views, args.img, args.batch, args.occupancy,
-               args.batch_order) == (256, 100, 800, 65536, 0.10, "sorted")
+               args.batch_order, args.log_samples) == (256, 100, 800, 65536, 0.10, "sorted", 512)
     if not default or not os.path.exists(TRAFFIC_FILE):
         return None
     with open(TRAFFIC_FILE) as f:
@@ -310,18 +310,24 @@ def main():
     log(f"{K} steps: {ms_step:.3f} ms/step, fused render {kern_ms:.3f} ms, wall {wall:.2f}s, "
         f"batch MSE {mse:.3f} PSNR {psnr:.2f}")
 
-    # Algorithmic bytes of the fused render kernel: 44 B per ray (geometry,
-    # target, outputs, loss) + 448 B per distinct touched record (parameters
-    # read by both marches, gradient read-modify-write).
+    # Algorithmic bytes of the render stage (the fused forward + backward,
+    # one kernel or the logged pair): 44 B per ray (geometry, target,
+    # outputs, loss) + 448 B per distinct touched record (its parameters read
+    # by the forward and the backward, its gradient read and written).  The
+    # step figure adds Adam's 896 B per record (parameters, gradient, two
+    # moments: read + write).
     vt = touched_vertices(w, dev)
     alg = args.batch * 44 + vt * 448
     achieved = alg / (kern_ms / 1e3) / 1e9
-    kname = pxl.render_kernel_name(w.grid, "fused")
+    step_alg = alg + w.n_vox * 896
+    kname = pxl.render_kernel_name(w.grid, "logged" if args.log_samples > 0 else "fused")
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5),
             "traffic": profiled_traffic(kname, args),
             "kernel": kname, "kernel_ms": round(kern_ms, 4),
-            "touched_records": int(vt), "alg_bytes": int(alg)}
+            "touched_records": int(vt), "alg_bytes": int(alg),
+            "step_alg_bytes": int(step_alg),
+            "step_frac": round(step_alg / (ms_step / 1e3) / 1e9 / HBM_PEAK_GBS, 5)}
 
     cpu = None
     if not args.no_cpu_baseline and args.cpu_rays > 0:

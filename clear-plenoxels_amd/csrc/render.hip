@@ -419,6 +419,7 @@ __global__ __launch_bounds__(256) void k_render(const RenderArgs a) {
     int k = 0;
     const bool active = p >= 0;
     float acc[8];
+    float acc0 = 0.0f;
     uint32_t cv[8];
     int cbx = -1, cby = -1, cbz = -1;
 #pragma unroll
@@ -490,7 +491,11 @@ __global__ __launch_bounds__(256) void k_render(const RenderArgs a) {
                 for (int i = 0; i < 8; i++) {
                     const int jx = (i & 1) - dbx, jy = ((i >> 1) & 1) - dby, jz = (i >> 2) - dbz;
                     const bool kept = (unsigned)jx < 2u && (unsigned)jy < 2u && (unsigned)jz < 2u;
-                    if (!kept && active) unsafeAtomicAdd(gbase + (size_t)cv[i] * stride, acc[i]);
+                    // Record 0 (also every corner outside the cage) is summed
+                    // in acc0 and added once per ray.
+                    if (!kept && cv[i] == 0u) acc0 += acc[i];
+                    else if (!kept && active)
+                        unsafeAtomicAdd(gbase + (size_t)cv[i] * stride, acc[i]);
                 }
             }
             cbx = s.bx;
@@ -509,10 +514,14 @@ __global__ __launch_bounds__(256) void k_render(const RenderArgs a) {
         if (ADAPT) t += fabs(step - w);
         else t = fma((double)k, step, tnear);
     }
-    if (cbx >= 0 && active) {
+    if (cbx >= 0) {
 #pragma unroll
-        for (int j = 0; j < 8; j++) unsafeAtomicAdd(gbase + (size_t)cv[j] * stride, acc[j]);
+        for (int j = 0; j < 8; j++) {
+            if (cv[j] == 0u) acc0 += acc[j];
+            else if (active) unsafeAtomicAdd(gbase + (size_t)cv[j] * stride, acc[j]);
+        }
     }
+    if (active && acc0 != 0.0f) unsafeAtomicAdd(gbase, acc0);
 }
 
 #include "march8.hpp"
@@ -872,18 +881,21 @@ extern "C" int32_t pxl_count_touched(const pxl_grid *grid, const pxl_rays *rays,
 
 extern "C" const char *pxl_render_kernel_name(const pxl_grid *grid, int32_t mode) {
     clear_error();
-    if (!grid || mode < 0 || mode > 2) {
+    if (!grid || mode < 0 || mode > 3) {
         set_error("pxl_render_kernel_name: bad argument");
         return nullptr;
     }
     static thread_local char name[96];
     const bool adapt = grid->march_mode == PXL_MARCH_REF_ADAPTIVE;
     const bool group = !adapt && grid->sh_dim <= 9 && use_march8();
-    const char *modes[3] = {"fwd", "bwd", "fused"};
+    const char *modes[4] = {"fwd", "bwd", "fused", "fused"};
     const char *store = grid->storage == PXL_STORE_BF16  ? "bf16"
                         : grid->storage == PXL_STORE_F16 ? "f16"
                                                          : "f32";
-    if (group)
+    if (group && mode == 3)
+        snprintf(name, sizeof(name), "k_march8<%d,%s,log,%s>+k_log_grad<%d>", grid->sh_dim,
+                 grid->sparse ? "sparse" : "dense", store, grid->sh_dim);
+    else if (group)
         snprintf(name, sizeof(name), "k_march8<%d,%s,%s,%s>", grid->sh_dim,
                  grid->sparse ? "sparse" : "dense", modes[mode], store);
     else

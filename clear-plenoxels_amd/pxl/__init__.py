@@ -506,8 +506,8 @@ def loss_reduce(losses, out, length=None, stream=None):
 
 def render_kernel_name(grid, mode="fused"):
     """Name of the kernel the render entry points launch for `grid`
-    (mode "fwd", "bwd" or "fused")."""
-    m = {"fwd": 0, "bwd": 1, "fused": 2}[mode]
+    (mode "fwd", "bwd", "fused" or "logged" for train_rays_ws)."""
+    m = {"fwd": 0, "bwd": 1, "fused": 2, "logged": 3}[mode]
     g = grid.c()
     name = lib().pxl_render_kernel_name(ctypes.byref(g), m)
     if name is None:

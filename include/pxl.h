@@ -313,7 +313,8 @@ int32_t pxl_train_rays_ws(const pxl_grid *grid, const pxl_optim *optim,
                           size_t workspace_bytes, int32_t max_samples, void *stream);
 
 /* Name of the kernel the render entry points launch for this grid:
- * mode 0 = pxl_render_fwd, 1 = pxl_render_bwd_rgb, 2 = pxl_train_rays.
+ * mode 0 = pxl_render_fwd, 1 = pxl_render_bwd_rgb, 2 = pxl_train_rays,
+ * 3 = pxl_train_rays_ws (two names joined by '+' when it runs as two kernels).
  * Returns a thread-local string such as "k_march8<9,sparse,fused,f32>" (or
  * NULL for a bad argument); for profilers and benchmark reports.  No
  * counterpart in the reference. */

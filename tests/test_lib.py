@@ -86,7 +86,11 @@ def test_render_kernel_name(pxl_lib, monkeypatch):
     g.march_mode = pxl_lib.MARCH_FIXED
     monkeypatch.setenv("PXL_MARCHER", "wave")
     assert L.pxl_render_kernel_name(ctypes.byref(g), 2) == b"k_render<9,dense,fixed,fused,f32>"
-    assert L.pxl_render_kernel_name(ctypes.byref(g), 3) is None
+    assert L.pxl_render_kernel_name(ctypes.byref(g), 3) == b"k_render<9,dense,fixed,fused,f32>"
+    monkeypatch.delenv("PXL_MARCHER")
+    assert (L.pxl_render_kernel_name(ctypes.byref(g), 3) ==
+            b"k_march8<9,dense,log,f32>+k_log_grad<9>")
+    assert L.pxl_render_kernel_name(ctypes.byref(g), 4) is None
     assert L.pxl_render_kernel_name(None, 0) is None
 
 
