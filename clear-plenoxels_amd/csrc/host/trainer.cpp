@@ -1,5 +1,6 @@
 // The reference's operator API over the C ABI, and the batch loop of
 // Render::objectSceneLearner (Render.cu:352-503) as a host object.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -68,6 +69,13 @@ struct pxl_trainer {
     // Sample-log workspace of the two-kernel fused step (owned).
     void *march_ws;
     size_t march_ws_bytes;
+    // Batch order buffer and sort scratch (owned), and the schedule in use.
+    int32_t *order_buf;
+    void *order_ws;
+    size_t order_ws_bytes;
+    int32_t order_mode;
+    pxl_schedule sched;
+    bool tuned;
 };
 
 using namespace pxl;
@@ -111,18 +119,112 @@ extern "C" pxl_trainer *pxl_trainer_create(const pxl_train_config *cfg, const px
     tr->ev_used = 0;
     tr->march_ws = nullptr;
     tr->march_ws_bytes = 0;
-    if (cfg->fused && cfg->log_samples > 0) {
+    tr->order_buf = nullptr;
+    tr->order_ws = nullptr;
+    tr->order_ws_bytes = 0;
+    tr->order_mode = cfg->batch_order;
+    tr->sched.waves_per_block = cfg->waves_per_block > 0 ? cfg->waves_per_block : 4;
+    tr->sched.persistent = cfg->persistent;
+    tr->sched.reserved[0] = tr->sched.reserved[1] = 0;
+    tr->tuned = !cfg->autotune;
+    const bool logged = cfg->fused && cfg->log_samples > 0;
+    if (!logged && (cfg->autotune || cfg->batch_order || cfg->persistent ||
+                    (cfg->waves_per_block && cfg->waves_per_block != 4))) {
+        set_error("pxl_trainer_create: batch orders and schedules need the logged step "
+                  "(fused with log_samples > 0)");
+        delete tr;
+        return nullptr;
+    }
+    if (logged) {
         tr->march_ws_bytes = pxl_train_rays_workspace_bytes(cfg->batch_size, cfg->log_samples);
         if (tr->march_ws_bytes == 0 ||
             hipMalloc(&tr->march_ws, tr->march_ws_bytes) != hipSuccess) {
             set_error("pxl_trainer_create: cannot allocate a %zu-byte sample log for %d samples "
                       "per ray",
                       tr->march_ws_bytes, cfg->log_samples);
-            delete tr;
+            pxl_trainer_destroy(tr);
             return nullptr;
+        }
+        if (cfg->autotune || cfg->batch_order) {
+            tr->order_ws_bytes = pxl_batch_order_scratch_bytes(cfg->batch_size);
+            if (hipMalloc(&tr->order_buf, (size_t)cfg->batch_size * sizeof(int32_t)) !=
+                    hipSuccess ||
+                hipMalloc(&tr->order_ws, tr->order_ws_bytes) != hipSuccess) {
+                set_error("pxl_trainer_create: cannot allocate the batch-order buffers");
+                pxl_trainer_destroy(tr);
+                return nullptr;
+            }
         }
     }
     return tr;
+}
+
+// The render call of one batch under a given order and schedule.
+static int32_t render_batch(pxl_trainer *tr, int64_t start, int64_t len, int32_t order,
+                            const pxl_schedule &sched) {
+    const pxl_train_config &c = tr->cfg;
+    const int32_t *si = tr->order;
+    int64_t s0 = start;
+    if (order != 0) {
+        const int32_t st = pxl_batch_order(&tr->grid, &tr->rays, tr->order, start, len, order,
+                                           tr->order_buf, tr->order_ws, tr->order_ws_bytes,
+                                           tr->stream);
+        if (st != PXL_OK) return st;
+        si = tr->order_buf;
+        s0 = 0;
+    }
+    return pxl_train_rays_sched(&tr->grid, &tr->optim, &tr->rays, tr->losses, si, s0, len,
+                                c.mask_coefficient, c.huber_delta, tr->march_ws,
+                                tr->march_ws_bytes, c.log_samples, &sched, tr->stream);
+}
+
+// Times every batch order x schedule twice on the first batch (HIP events
+// on the trainer's stream), keeps the fastest, and zeroes the gradient the
+// trial runs added.
+static int32_t autotune(pxl_trainer *tr, int64_t start, int64_t len) {
+    struct Cand {
+        int32_t order;
+        pxl_schedule s;
+    };
+    std::vector<Cand> cands;
+    for (int32_t order : {0, (int32_t)PXL_ORDER_MORTON, (int32_t)PXL_ORDER_LENGTH_MORTON})
+        for (int32_t wpb : {1, 2, 4})
+            for (int32_t pers : {0, 1}) cands.push_back({order, {wpb, pers, {0, 0}}});
+    constexpr int kReps = 2;
+    std::vector<hipEvent_t> ev(cands.size() * kReps * 2, nullptr);
+    int32_t st = PXL_OK;
+    for (auto &e : ev)
+        if (hipEventCreate(&e) != hipSuccess) st = PXL_EHIP;
+    for (size_t i = 0; st == PXL_OK && i < cands.size() * kReps; i++) {
+        const Cand &cd = cands[i % cands.size()];
+        (void)hipEventRecord(ev[2 * i], tr->stream);
+        st = render_batch(tr, start, len, cd.order, cd.s);
+        (void)hipEventRecord(ev[2 * i + 1], tr->stream);
+    }
+    if (st == PXL_OK && hipStreamSynchronize(tr->stream) != hipSuccess) st = PXL_EHIP;
+    if (st == PXL_OK) {
+        std::vector<float> best(cands.size(), 1e30f);
+        for (size_t i = 0; i < cands.size() * kReps; i++) {
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]);
+            best[i % cands.size()] = std::min(best[i % cands.size()], ms);
+        }
+        size_t k = 0;
+        for (size_t i = 1; i < cands.size(); i++)
+            if (best[i] < best[k]) k = i;
+        tr->order_mode = cands[k].order;
+        tr->sched = cands[k].s;
+        const int stride = tr->optim.stride > 0 ? tr->optim.stride : tr->grid.stride;
+        if (hipMemsetAsync(tr->optim.grad, 0,
+                           (size_t)tr->grid.n_vox * (size_t)stride * sizeof(float),
+                           tr->stream) != hipSuccess)
+            st = PXL_EHIP;
+    }
+    for (auto &e : ev)
+        if (e) (void)hipEventDestroy(e);
+    if (st == PXL_EHIP) set_error("pxl_trainer_step: autotuning failed");
+    tr->tuned = true;
+    return st;
 }
 
 extern "C" int32_t pxl_trainer_time_render(pxl_trainer *tr, int32_t max_steps) {
@@ -152,9 +254,13 @@ extern "C" int32_t pxl_trainer_step(pxl_trainer *tr) {
     if (timed) (void)hipEventRecord(tr->ev[tr->ev_used * 2], tr->stream);
     int32_t st;
     if (c.fused && tr->march_ws) {
-        st = pxl_train_rays_ws(&tr->grid, &tr->optim, &tr->rays, tr->losses, tr->order, start,
-                               len, c.mask_coefficient, c.huber_delta, tr->march_ws,
-                               tr->march_ws_bytes, c.log_samples, tr->stream);
+        if (!tr->tuned) {
+            if (timed) (void)hipEventRecord(tr->ev[tr->ev_used * 2], tr->stream);
+            st = autotune(tr, start, len);
+            if (st != PXL_OK) return st;
+            if (timed) (void)hipEventRecord(tr->ev[tr->ev_used * 2], tr->stream);
+        }
+        st = render_batch(tr, start, len, tr->order_mode, tr->sched);
     } else if (c.fused) {
         st = pxl_train_rays(&tr->grid, &tr->optim, &tr->rays, tr->losses, tr->order, start, len,
                             c.mask_coefficient, c.huber_delta, tr->stream);
@@ -248,9 +354,17 @@ extern "C" int32_t pxl_trainer_state(const pxl_trainer *tr, int64_t *t, double *
 extern "C" void pxl_trainer_destroy(pxl_trainer *tr) {
     if (!tr) return;
     for (hipEvent_t e : tr->ev) (void)hipEventDestroy(e);
-    if (tr->march_ws) {
-        (void)hipStreamSynchronize(tr->stream);
-        (void)hipFree(tr->march_ws);
-    }
+    if (tr->march_ws || tr->order_buf || tr->order_ws) (void)hipStreamSynchronize(tr->stream);
+    if (tr->march_ws) (void)hipFree(tr->march_ws);
+    if (tr->order_buf) (void)hipFree(tr->order_buf);
+    if (tr->order_ws) (void)hipFree(tr->order_ws);
     delete tr;
+}
+
+extern "C" int32_t pxl_trainer_schedule(const pxl_trainer *tr, int32_t *batch_order,
+                                        pxl_schedule *schedule) {
+    PXL_REQUIRE(tr != nullptr, "pxl_trainer_schedule: null trainer");
+    if (batch_order) *batch_order = tr->order_mode;
+    if (schedule) *schedule = tr->sched;
+    return PXL_OK;
 }

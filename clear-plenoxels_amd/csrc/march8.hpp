@@ -202,14 +202,51 @@ __device__ __forceinline__ void flush_record0(const RenderArgs &a, float acc0, i
     if (lh == 0 && pp < P && v != 0.0f) unsafeAtomicAdd(a.grad + pp, v);
 }
 
+// Batch entry of the next ray a wave of a persistent launch processes.  The
+// batch is cut into 8 slices, one per XCD (a block's XCD taken as blockIdx
+// mod 8, the dispatch order); a wave claims entries of its XCD's slice from
+// that slice's counter and, once the slice is used up, of the next slices in
+// turn.  Returns len when every slice is used up; `slice` carries the slice
+// the wave is on between calls (start: blockIdx mod 8, `visited` 0).
+__device__ __forceinline__ int64_t claim_ray(unsigned long long *ctr, int64_t len, int &slice,
+                                             int &visited) {
+    while (visited < 8) {
+        const int64_t lo = len * slice / 8, hi = len * (slice + 1) / 8;
+        unsigned long long got = 0;
+        if ((threadIdx.x & 63) == 0) got = atomicAdd(ctr + slice, 1ull);
+        const int64_t b = lo + (int64_t)__builtin_amdgcn_readfirstlane((int)got);
+        if (b < hi) return b;
+        slice = (slice + 1) & 7;
+        visited++;
+    }
+    return len;
+}
+
 template <int S, bool SPARSE, int MODE, int STORE>
+__device__ __forceinline__ void march8_ray(const RenderArgs &a, const int64_t b);
+
+// One wave per ray: either the block's waves take consecutive batch entries
+// of an XCD-sliced block order, or (a.persist) a fixed grid of waves claims
+// entries until the batch is done.
+template <int S, bool SPARSE, int MODE, int STORE, bool PERSIST = false>
 __global__ __launch_bounds__(256) void k_march8(const RenderArgs a) {
+    if (PERSIST) {
+        int slice = blockIdx.x & 7, visited = 0;
+        for (int64_t b = claim_ray(a.work, a.len, slice, visited); b < a.len;
+             b = claim_ray(a.work, a.len, slice, visited))
+            march8_ray<S, SPARSE, MODE, STORE>(a, b);
+    } else {
+        const int64_t b = (int64_t)xcd_slice_block(blockIdx.x, gridDim.x) * (blockDim.x >> 6) +
+                          (threadIdx.x >> 6);
+        if (b < a.len) march8_ray<S, SPARSE, MODE, STORE>(a, b);
+    }
+}
+
+template <int S, bool SPARSE, int MODE, int STORE>
+__device__ __forceinline__ void march8_ray(const RenderArgs &a, const int64_t b) {
     constexpr int P = 1 + 3 * S;
     const int lane = threadIdx.x & 63;
     const int slot = lane >> 3, corner = lane & 7;
-    const int64_t b =
-        (int64_t)xcd_slice_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
-    if (b >= a.len) return;
     const int ray = uniform_i(a.sample_index[a.start + b]);
     const bool do_fwd = MODE == kFwd || MODE == kFused || MODE == kTrace || MODE == kLog;
     const bool do_bwd = MODE == kBwd || MODE == kFused;
@@ -560,12 +597,26 @@ __device__ __forceinline__ double wave_scan_mul64(double v, int lane) {
 // already in registers: no global load follows an atomic until the next 64
 // samples, so the atomics never hold up the wave.
 template <int S>
+__device__ __forceinline__ void log_grad_ray(const RenderArgs &a, const int64_t b);
+
+template <int S, bool PERSIST = false>
 __global__ __launch_bounds__(256) void k_log_grad(const RenderArgs a) {
+    if (PERSIST) {
+        int slice = blockIdx.x & 7, visited = 0;
+        for (int64_t b = claim_ray(a.work + 8, a.len, slice, visited); b < a.len;
+             b = claim_ray(a.work + 8, a.len, slice, visited))
+            log_grad_ray<S>(a, b);
+    } else {
+        const int64_t b = (int64_t)xcd_slice_block(blockIdx.x, gridDim.x) * (blockDim.x >> 6) +
+                          (threadIdx.x >> 6);
+        if (b < a.len) log_grad_ray<S>(a, b);
+    }
+}
+
+template <int S>
+__device__ __forceinline__ void log_grad_ray(const RenderArgs &a, const int64_t b) {
     constexpr int P = 1 + 3 * S;
     const int lane = threadIdx.x & 63;
-    const int64_t b =
-        (int64_t)xcd_slice_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
-    if (b >= a.len) return;
     const int cnt = uniform_i(a.log_count[b]);
     if (cnt <= 0) return;  // nothing logged, or done by the marcher
     const int ray = uniform_i(a.sample_index[a.start + b]);

@@ -15,6 +15,7 @@
 // registers while consecutive samples stay in the same cell and flushed
 // with one coalesced global_atomic_add_f32 per corner (instead of
 // 8 x (1+3S) scattered atomicAdds per sample, RenderUtils.cu:1074-1364).
+#include <algorithm>
 #include <cstdlib>
 #include <type_traits>
 
@@ -62,6 +63,8 @@ struct RenderArgs {
     float *log_val;
     int log_cap;
     int log_redo;  // kFused: only the batch entries whose count is -1
+    // Persistent launches: 8 claim counters per kernel (zeroed before launch).
+    unsigned long long *work;
 };
 
 // Activations (RenderUtils.cu:480-515) in float or double.
@@ -689,12 +692,14 @@ namespace {
 // Workspace of the two-kernel fused step: per batch entry 4 FP64 totals and a
 // count, per logged sample 8 records and 8 value words; 256-byte aligned.
 struct LogLayout {
-    size_t tot, count, rec, val, total;
+    size_t work, tot, count, rec, val, total;
 };
 inline size_t align256(size_t x) { return (x + 255) / 256 * 256; }
 LogLayout log_layout(int64_t len, int32_t cap) {
     LogLayout L;
     size_t o = 0;
+    L.work = o;  // 16 claim counters of the persistent launches
+    o += 256;
     L.tot = o;
     o += align256((size_t)len * 4 * sizeof(double));
     L.count = o;
@@ -708,24 +713,56 @@ LogLayout log_layout(int64_t len, int32_t cap) {
 }
 constexpr int32_t kMaxLogSamples = 1 << 16;
 
+__global__ void k_zero_u64(unsigned long long *p, int n) {
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
+}
+
+// Blocks of `threads` threads of `kernel` that fit on the whole device at
+// once (the grid of a persistent launch).
+template <typename K>
+unsigned resident_blocks(K kernel, int threads) {
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess ||
+        cus <= 0 || per_cu <= 0)
+        return 1024;
+    return (unsigned)cus * (unsigned)per_cu;
+}
+
 // Forward + log, gradient of the logged rays, then the fused marcher again
 // for the rays longer than the log (its waves leave at once for the others).
+// wpb waves per block; persistent launches size the grid to what is
+// resident and let the waves claim batch entries.
 template <int S, bool SPARSE, int STORE>
-void launch_logged(const RenderArgs &a, hipStream_t st) {
-    const unsigned blocks = (unsigned)((a.len + 3) / 4);
-    hipLaunchKernelGGL((k_march8<S, SPARSE, kLog, STORE>), dim3(blocks), dim3(256), 0, st, a);
-    hipLaunchKernelGGL((k_log_grad<S>), dim3(blocks), dim3(256), 0, st, a);
+void launch_logged(const RenderArgs &a, int wpb, bool persist, hipStream_t st) {
+    const int threads = 64 * wpb;
+    const unsigned blocks = (unsigned)((a.len + wpb - 1) / wpb);
+    if (persist) {
+        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, a.work, 16);
+        const unsigned ga =
+            std::min(blocks, resident_blocks(k_march8<S, SPARSE, kLog, STORE, true>, threads));
+        const unsigned gb = std::min(blocks, resident_blocks(k_log_grad<S, true>, threads));
+        hipLaunchKernelGGL((k_march8<S, SPARSE, kLog, STORE, true>), dim3(ga), dim3(threads), 0,
+                           st, a);
+        hipLaunchKernelGGL((k_log_grad<S, true>), dim3(gb), dim3(threads), 0, st, a);
+    } else {
+        hipLaunchKernelGGL((k_march8<S, SPARSE, kLog, STORE>), dim3(blocks), dim3(threads), 0, st,
+                           a);
+        hipLaunchKernelGGL((k_log_grad<S>), dim3(blocks), dim3(threads), 0, st, a);
+    }
     RenderArgs redo = a;
     redo.log_redo = 1;
-    hipLaunchKernelGGL((k_march8<S, SPARSE, kFused, STORE>), dim3(blocks), dim3(256), 0, st,
+    hipLaunchKernelGGL((k_march8<S, SPARSE, kFused, STORE>), dim3(blocks), dim3(threads), 0, st,
                        redo);
 }
 template <int S>
-int32_t dispatch_logged_s(bool sparse, int store, const RenderArgs &a, hipStream_t st) {
-#define PXL_LOGGED(SP, ST)                      \
-    if (sparse == SP && store == ST) {          \
-        launch_logged<S, SP, ST>(a, st);        \
-        return PXL_OK;                          \
+int32_t dispatch_logged_s(bool sparse, int store, const RenderArgs &a, int wpb, bool persist,
+                          hipStream_t st) {
+#define PXL_LOGGED(SP, ST)                                  \
+    if (sparse == SP && store == ST) {                      \
+        launch_logged<S, SP, ST>(a, wpb, persist, st);      \
+        return PXL_OK;                                      \
     }
     PXL_LOGGED(true, PXL_STORE_F32)
     PXL_LOGGED(true, PXL_STORE_BF16)
@@ -737,11 +774,12 @@ int32_t dispatch_logged_s(bool sparse, int store, const RenderArgs &a, hipStream
     set_error("render: unknown storage %d", store);
     return PXL_EINVAL;
 }
-int32_t dispatch_logged(int S, bool sparse, int store, const RenderArgs &a, hipStream_t st) {
+int32_t dispatch_logged(int S, bool sparse, int store, const RenderArgs &a, int wpb,
+                        bool persist, hipStream_t st) {
     switch (S) {
-        case 1: return dispatch_logged_s<1>(sparse, store, a, st);
-        case 4: return dispatch_logged_s<4>(sparse, store, a, st);
-        case 9: return dispatch_logged_s<9>(sparse, store, a, st);
+        case 1: return dispatch_logged_s<1>(sparse, store, a, wpb, persist, st);
+        case 4: return dispatch_logged_s<4>(sparse, store, a, wpb, persist, st);
+        case 9: return dispatch_logged_s<9>(sparse, store, a, wpb, persist, st);
         default:
             set_error("render: the logged fused step supports sh_dim 1, 4 and 9");
             return PXL_EINVAL;
@@ -754,7 +792,7 @@ static int32_t bwd_common(const pxl_grid *grid, const pxl_optim *optim, const px
                           double *losses, const int32_t *sample_index, int64_t start,
                           int64_t len, double mask_coef, float huber_delta, void *stream,
                           bool fused, void *workspace = nullptr, size_t workspace_bytes = 0,
-                          int32_t max_samples = 0) {
+                          int32_t max_samples = 0, const pxl_schedule *sched = nullptr) {
     const char *who = workspace ? "pxl_train_rays_ws" : fused ? "pxl_train_rays" : "pxl_render_bwd_rgb";
     clear_error();
     RenderArgs a;
@@ -793,7 +831,12 @@ static int32_t bwd_common(const pxl_grid *grid, const pxl_optim *optim, const px
         a.log_rec = reinterpret_cast<uint32_t *>(w + L.rec);
         a.log_val = reinterpret_cast<float *>(w + L.val);
         a.log_cap = max_samples;
-        st = dispatch_logged(grid->sh_dim, grid->sparse, grid->storage, a, as_stream(stream));
+        a.work = reinterpret_cast<unsigned long long *>(w + L.work);
+        const int wpb = sched ? sched->waves_per_block : 4;
+        PXL_REQUIRE(wpb == 1 || wpb == 2 || wpb == 4, "%s: waves_per_block %d (1, 2 or 4)", who,
+                    wpb);
+        st = dispatch_logged(grid->sh_dim, grid->sparse, grid->storage, a, wpb,
+                             sched && sched->persistent, as_stream(stream));
         return st != PXL_OK ? st : check_launch(who);
     }
     st = fused ? dispatch<kFused>(grid->sh_dim, grid->sparse, adapt, grid->storage, a,
@@ -836,6 +879,21 @@ extern "C" int32_t pxl_train_rays_ws(const pxl_grid *grid, const pxl_optim *opti
     }
     return bwd_common(grid, optim, rays, losses, sample_index, start, len, mask_coef,
                       huber_delta, stream, true, workspace, workspace_bytes, max_samples);
+}
+
+extern "C" int32_t pxl_train_rays_sched(const pxl_grid *grid, const pxl_optim *optim,
+                                        const pxl_rays *rays, double *losses,
+                                        const int32_t *sample_index, int64_t start, int64_t len,
+                                        double mask_coef, float huber_delta, void *workspace,
+                                        size_t workspace_bytes, int32_t max_samples,
+                                        const pxl_schedule *sched, void *stream) {
+    if (!workspace || !sched) {
+        clear_error();
+        set_error("pxl_train_rays_sched: null workspace or schedule");
+        return PXL_EINVAL;
+    }
+    return bwd_common(grid, optim, rays, losses, sample_index, start, len, mask_coef,
+                      huber_delta, stream, true, workspace, workspace_bytes, max_samples, sched);
 }
 
 namespace pxl {

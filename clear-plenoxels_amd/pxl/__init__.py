@@ -33,6 +33,8 @@ EXPORTS = (
     "pxl_trainer_state", "pxl_trainer_destroy", "pxl_brick_words",
     "pxl_build_brick_occupancy", "pxl_records_to_16bit", "pxl_tv_prepare",
     "pxl_render_kernel_name", "pxl_train_rays_workspace_bytes", "pxl_train_rays_ws",
+    "pxl_train_rays_sched", "pxl_batch_order_scratch_bytes", "pxl_batch_order",
+    "pxl_trainer_schedule",
 )
 
 
@@ -113,7 +115,12 @@ class TrainConfigC(ctypes.Structure):
                 ("variance_type", _I32), ("neighbour_nums", _I32),
                 ("variance_interval", _I32), ("optimize_step", _I32),
                 ("mask_coefficient", _F64), ("huber_delta", _F32), ("fused", _I32),
-                ("log_samples", _I32), ("reserved", _I32)]
+                ("log_samples", _I32), ("autotune", _I32), ("batch_order", _I32),
+                ("waves_per_block", _I32), ("persistent", _I32), ("reserved", _I32)]
+
+
+class ScheduleC(ctypes.Structure):
+    _fields_ = [("waves_per_block", _I32), ("persistent", _I32), ("reserved", _I32 * 2)]
 
 
 def _declare(L):
@@ -158,6 +165,11 @@ def _declare(L):
         "pxl_train_rays_workspace_bytes": ([_I64, _I32], _SZ),
         "pxl_train_rays_ws": ([G, O, R, _P, _P, _I64, _I64, _F64, _F32, _P, _SZ, _I32, _P],
                               _I32),
+        "pxl_train_rays_sched": ([G, O, R, _P, _P, _I64, _I64, _F64, _F32, _P, _SZ, _I32,
+                                  ctypes.POINTER(ScheduleC), _P], _I32),
+        "pxl_batch_order_scratch_bytes": ([_I64], _SZ),
+        "pxl_batch_order": ([G, R, _P, _I64, _I64, _I32, _P, _P, _SZ, _P], _I32),
+        "pxl_trainer_schedule": ([_P, ctypes.POINTER(_I32), ctypes.POINTER(ScheduleC)], _I32),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -395,6 +407,40 @@ def train_rays_ws(grid, optim, rays, losses, sample_index, workspace, max_sample
                                    _stream(stream)))
 
 
+ORDER_NONE, ORDER_MORTON, ORDER_LENGTH_MORTON = 0, 1, 2
+
+
+def train_rays_sched(grid, optim, rays, losses, sample_index, workspace, max_samples,
+                     waves_per_block=4, persistent=False, start=0, length=None, mask_coef=1.0,
+                     huber_delta=0.1, stream=None):
+    """train_rays_ws with an explicit launch schedule."""
+    length = sample_index.numel() - start if length is None else length
+    g, o, r = grid.c(), optim.c(), rays.c()
+    sc = ScheduleC(waves_per_block, int(persistent), (_I32 * 2)(0, 0))
+    _check("pxl_train_rays_sched",
+           lib().pxl_train_rays_sched(ctypes.byref(g), ctypes.byref(o), ctypes.byref(r),
+                                      _ptr(losses), _ptr(sample_index), start, length, mask_coef,
+                                      huber_delta, _ptr(workspace), workspace.numel(),
+                                      max_samples, ctypes.byref(sc), _stream(stream)))
+
+
+def batch_order(grid, rays, sample_index, out, mode=ORDER_MORTON, start=0, length=None,
+                scratch=None, stream=None):
+    """Ray ids sample_index[start:start+length] in `mode` order, into out[:length]."""
+    import torch
+
+    length = sample_index.numel() - start if length is None else length
+    nbytes = lib().pxl_batch_order_scratch_bytes(length)
+    if scratch is None:
+        scratch = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=sample_index.device)
+    g, r = grid.c(), rays.c()
+    _check("pxl_batch_order",
+           lib().pxl_batch_order(ctypes.byref(g), ctypes.byref(r), _ptr(sample_index), start,
+                                 length, mode, _ptr(out), _ptr(scratch), scratch.numel(),
+                                 _stream(stream)))
+    return out
+
+
 def tv_scratch(grid, stream=None):
     """The grid's prepared variance-loss scratch (sparse grids; created and
     filled by pxl_tv_prepare on first use, kept on the Grid object)."""
@@ -532,7 +578,8 @@ class Trainer:
                  learning_rate=0.3, sh_learning_rate=1.0, variance_coefficient=1e-5,
                  sh_coefficient=1e-6, variance_type=TV_DISPARITY, neighbour_nums=6,
                  variance_interval=3, optimize_step=0, mask_coefficient=1.0, huber_delta=0.1,
-                 fused=True, log_samples=0, stream=None):
+                 fused=True, log_samples=0, autotune=False, batch_order=0,
+                 waves_per_block=0, persistent=False, stream=None):
         import torch
 
         self._keep = (grid, optim, rays, order)
@@ -542,7 +589,8 @@ class Trainer:
         cfg = TrainConfigC(batch_size, learning_rate, sh_learning_rate, variance_coefficient,
                            sh_coefficient, variance_type, neighbour_nums, variance_interval,
                            optimize_step, mask_coefficient, huber_delta, int(fused),
-                           int(log_samples), 0)
+                           int(log_samples), int(autotune), int(batch_order),
+                           int(waves_per_block), int(persistent), 0)
         self._g, self._o, self._r = grid.c(), optim.c(), rays.c()
         n_rays = order.numel() if n_rays is None else n_rays
         sc = tv_scratch(grid, stream) if variance_type == TV_DISPARITY else None
@@ -578,6 +626,13 @@ class Trainer:
         _check("pxl_trainer_state", lib().pxl_trainer_state(self._h, ctypes.byref(t),
                                                             ctypes.byref(lr), ctypes.byref(sl)))
         return t.value, lr.value, sl.value
+
+    def schedule(self):
+        """(batch order, waves per block, persistent) the trainer runs."""
+        order, sc = _I32(), ScheduleC()
+        _check("pxl_trainer_schedule",
+               lib().pxl_trainer_schedule(self._h, ctypes.byref(order), ctypes.byref(sc)))
+        return order.value, sc.waves_per_block, bool(sc.persistent)
 
     def close(self):
         if getattr(self, "_h", None):

@@ -306,11 +306,50 @@ int32_t pxl_count_touched(const pxl_grid *grid, const pxl_rays *rays,
  * implementation of renderRays_withGradients_RGB, RenderUtils.cu:2257-2278,
  * fused with renderRays). */
 size_t pxl_train_rays_workspace_bytes(int64_t len, int32_t max_samples);
+
+/* Launch schedule of the two-kernel step: waves (rays) per workgroup, 1, 2
+ * or 4; persistent != 0 launches as many workgroups as fit on the device at
+ * once and lets each wave claim batch entries (XCD-sliced claim counters in
+ * the workspace) until the batch is done, for batches of very uneven ray
+ * lengths.  The results do not depend on the schedule beyond the order of
+ * float atomics. */
+typedef struct pxl_schedule {
+    int32_t waves_per_block;
+    int32_t persistent;
+    int32_t reserved[2];
+} pxl_schedule;
 int32_t pxl_train_rays_ws(const pxl_grid *grid, const pxl_optim *optim,
                           const pxl_rays *rays, double *losses,
                           const int32_t *sample_index, int64_t start, int64_t len,
                           double mask_coef, float huber_delta, void *workspace,
                           size_t workspace_bytes, int32_t max_samples, void *stream);
+/* pxl_train_rays_ws with an explicit schedule (pxl_train_rays_ws uses 4
+ * waves per block, not persistent).  No counterpart in the reference. */
+int32_t pxl_train_rays_sched(const pxl_grid *grid, const pxl_optim *optim,
+                             const pxl_rays *rays, double *losses,
+                             const int32_t *sample_index, int64_t start, int64_t len,
+                             double mask_coef, float huber_delta, void *workspace,
+                             size_t workspace_bytes, int32_t max_samples,
+                             const pxl_schedule *sched, void *stream);
+
+/* Batch orders (pxl_batch_order). */
+enum {
+    PXL_ORDER_MORTON = 1,        /* Morton code of the grid entry point */
+    PXL_ORDER_LENGTH_MORTON = 2  /* longest-first length buckets, Morton inside */
+};
+
+/* The ray ids sample_index[start .. start+len) in the given order, written
+ * to out_index[0 .. len) (use it as the sample index of the train call with
+ * start 0).  The same rays as the reference's batch (Render.cu:399-406), in
+ * an order that keeps neighbouring rays on one XCD and, for uneven batches,
+ * starts the longest rays first; rays that miss the grid box go last.
+ * scratch: 256-byte aligned, pxl_batch_order_scratch_bytes(len) bytes.  No
+ * counterpart in the reference. */
+size_t pxl_batch_order_scratch_bytes(int64_t len);
+int32_t pxl_batch_order(const pxl_grid *grid, const pxl_rays *rays,
+                        const int32_t *sample_index, int64_t start, int64_t len,
+                        int32_t mode, int32_t *out_index, void *scratch,
+                        size_t scratch_bytes, void *stream);
 
 /* Name of the kernel the render entry points launch for this grid:
  * mode 0 = pxl_render_fwd, 1 = pxl_render_bwd_rgb, 2 = pxl_train_rays,
@@ -342,9 +381,15 @@ typedef struct pxl_train_config {
     double mask_coefficient;
     float huber_delta;
     int32_t fused;           /* 1: fused forward+backward launch per batch */
-    int32_t log_samples;     /* fused only: > 0 runs pxl_train_rays_ws with a
-                                trainer-owned workspace logging up to this
-                                many samples per ray; 0 runs pxl_train_rays */
+    int32_t log_samples;     /* fused only: > 0 runs the two-kernel step
+                                (pxl_train_rays_sched) with a trainer-owned
+                                workspace logging up to this many samples per
+                                ray; 0 runs pxl_train_rays */
+    int32_t autotune;        /* log_samples > 0: time every batch order x
+                                schedule on the first batch, keep the fastest */
+    int32_t batch_order;     /* 0 as given, PXL_ORDER_* (when not autotuned) */
+    int32_t waves_per_block; /* 1, 2 or 4 (when not autotuned; 0 = 4) */
+    int32_t persistent;      /* when not autotuned */
     int32_t reserved;
 } pxl_train_config;
 
@@ -378,6 +423,10 @@ int32_t pxl_trainer_loss(pxl_trainer *tr, double *mse, double *psnr);
 
 /* Steps taken and current learning rates. */
 int32_t pxl_trainer_state(const pxl_trainer *tr, int64_t *t, double *lr, double *sh_lr);
+/* The batch order and launch schedule the trainer runs (after autotuning,
+ * the chosen one); any pointer may be NULL. */
+int32_t pxl_trainer_schedule(const pxl_trainer *tr, int32_t *batch_order,
+                             pxl_schedule *schedule);
 
 void pxl_trainer_destroy(pxl_trainer *tr);
 

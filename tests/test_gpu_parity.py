@@ -224,9 +224,10 @@ def _bwd_case(c1, grid, grid_o, nr, path, huber=0.1):
         fwd = oracle.render_fwd(grid_o, rays_o, batch)
         if (os.environ.get("PXL_MARCHER", "group") != "wave"
                 and grid.march_mode == pxl.MARCH_FIXED):
-            # Per-ray counts follow the FP64 totals (32 bytes per ray, 256-byte
-            # aligned): both the logged and the redo route must be taken.
-            off = (nr * 32 + 255) // 256 * 256
+            # Per-ray counts follow 256 bytes of claim counters and the FP64
+            # totals (32 bytes per ray, 256-byte aligned): both the logged
+            # and the redo route must be taken.
+            off = 256 + (nr * 32 + 255) // 256 * 256
             cnt = ws[off:off + 4 * nr].view(torch.int32).cpu().numpy()
             n_logged, n_redo = int((cnt > 0).sum()), int((cnt == -1).sum())
             say(f"    logged rays {n_logged}, redone rays {n_redo}")
@@ -277,6 +278,91 @@ def test_backward_sparse(c1, sparse1, path, marcher):
     grid_o = oracle.Grid(N1, N1, N1, S1, s["compact"], 0.5, occupancy=s["occ"],
                          index_offsets=s["io"])
     _bwd_case(c1, grid, grid_o, 1024, path)
+
+
+def _order_keys_np(rays_np, ids, N, step, mode):
+    """The batch-order key of each ray id, restated in numpy."""
+    ox, oy, oz = (rays_np[k][ids].astype(np.float64) for k in ("ox", "oy", "oz"))
+    dx, dy, dz = (rays_np[k][ids].astype(np.float32).astype(np.float64) for k in ("dx", "dy", "dz"))
+    tn, tf = rays_np["near_t"][ids].astype(np.float64), rays_np["far_t"][ids].astype(np.float64)
+    t0 = np.maximum(tn, 0.0)
+
+    def q(p):
+        return np.clip(p / (N - 1) * 1023.0, 0.0, 1023.0).astype(np.uint64)
+
+    def spread(v):
+        out = np.zeros_like(v)
+        for bit in range(10):
+            out |= ((v >> np.uint64(bit)) & np.uint64(1)) << np.uint64(3 * bit)
+        return out
+
+    key = spread(q(ox + t0 * dx)) | (spread(q(oy + t0 * dy)) << np.uint64(1)) | \
+        (spread(q(oz + t0 * dz)) << np.uint64(2))
+    if mode == pxl.ORDER_LENGTH_MORTON:
+        n = (tf - tn) / step
+        steps = np.where(n < 1.0, 0, np.minimum(n, 4.0e9)).astype(np.uint64)
+        bits = np.array([int(v).bit_length() for v in steps], dtype=np.uint64)
+        key |= (np.uint64(15) - np.minimum(bits, np.uint64(15))) << np.uint64(30)
+    key[tn > tf] = np.uint64(1) << np.uint64(40)
+    return key
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_batch_order(c1, mode):
+    """pxl_batch_order returns a permutation of the batch whose keys, restated
+    in numpy, are non-decreasing (bit-exact keys)."""
+    import torch
+
+    say(f"[order] batch order mode {mode}, 4096 rays")
+    dev = c1["dev"]
+    rays = device_rays(c1["rays_np"], dev)
+    grid = pxl.Grid(N1, N1, N1, S1, c1["rec"], step=0.5)
+    si = c1["si"]
+    out = torch.empty_like(si)
+    pxl.batch_order(grid, rays, si, out, mode=mode)
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    assert sorted(got.tolist()) == sorted(si.cpu().numpy().tolist())
+    keys = _order_keys_np(c1["rays_np"], got, N1, 0.5, mode)
+    assert np.all(keys[1:] >= keys[:-1])
+
+
+@pytest.mark.parametrize("sched", [(1, False, 0), (2, True, 1), (4, True, 2), (4, False, 1)])
+def test_logged_schedules(c1, sparse1, sched):
+    """The two-kernel step under each schedule and batch order matches the
+    oracle (sparse 128^3, 1024 rays, log of 64 samples: some rays redone)."""
+    import torch
+
+    wpb, persist, mode = sched
+    say(f"[sched] waves/block {wpb}, persistent {persist}, order {mode}")
+    s = sparse1
+    dev = c1["dev"]
+    grid = device_sparse_grid(N1, S1, s["compact"], s["occ"], s["io"], s["cage"], s["c2d"], dev)
+    grid_o = oracle.Grid(N1, N1, N1, S1, s["compact"], 0.5, occupancy=s["occ"],
+                         index_offsets=s["io"])
+    nr = 1024
+    rays = device_rays(c1["rays_np"], dev)
+    si = c1["si"][:nr]
+    if mode:
+        ordered = torch.empty_like(si)
+        pxl.batch_order(grid, rays, si, ordered, mode=mode)
+        si = ordered
+    torch.cuda.synchronize()
+    batch = si.cpu().numpy().astype(np.int32)
+    optim = pxl.Optim(grid)
+    losses = torch.zeros(nr, dtype=torch.float64, device=dev)
+    ws = pxl.train_workspace(nr, 64, dev)
+    pxl.train_rays_sched(grid, optim, rays, losses, si, ws, 64, waves_per_block=wpb,
+                         persistent=persist)
+    torch.cuda.synchronize()
+    rays_o = oracle.Rays(c1["rays_np"])
+    fwd = oracle.render_fwd(grid_o, rays_o, batch)
+    l_ref, g_ref = oracle.render_bwd(grid_o, rays_o, batch, fwd, mask_coef=1.0, huber_delta=0.1)
+    assert np.abs(losses.cpu().numpy() - l_ref).max() <= 255 * 1e-5
+    assert np.abs(rays.acc_r.cpu().numpy()[batch] - fwd["r"][batch]).max() <= 1e-5
+    worst = grad_close(from_records(optim.grad, grid.P), g_ref, floor=1e-3)
+    say(f"    worst gradient error / tolerance = {worst:.3g}")
+    assert worst <= 1.0
 
 
 def test_backward_adaptive_unclipped(c1):
@@ -431,8 +517,8 @@ def test_train_step_c1(c1):
     assert diff.max() <= 0.2 + 1e-6
 
 
-@pytest.mark.parametrize("log_samples", [0, 512])
-def test_trainer_matches_stage_calls(c1, log_samples):
+@pytest.mark.parametrize("log_samples,autotune", [(0, False), (512, False), (512, True)])
+def test_trainer_matches_stage_calls(c1, log_samples, autotune):
     """The C++ batch loop (pxl_trainer_step) issues the same stages as the
     Python-driven sequence: dense grid (TV every batch), 3 batches, LR decay
     lr /= 2^(1/(batchCount (epoch+1))) per batch (Render.cu:499-500); with
@@ -441,7 +527,7 @@ def test_trainer_matches_stage_calls(c1, log_samples):
 
     import torch
 
-    say(f"[trainer] 3 batches vs stage calls, log_samples={log_samples}")
+    say(f"[trainer] 3 batches vs stage calls, log_samples={log_samples}, autotune={autotune}")
     dev = c1["dev"]
     B = 1024
     order = c1["si"]
@@ -456,7 +542,8 @@ def test_trainer_matches_stage_calls(c1, log_samples):
     rays_b = device_rays(c1["rays_np"], dev)
     tr = pxl.Trainer(grids[0], optims[0], rays_a, order, batch_size=B, learning_rate=0.1,
                      sh_learning_rate=0.2, variance_coefficient=1e-5, sh_coefficient=1e-6,
-                     variance_interval=3, fused=True, log_samples=log_samples)
+                     variance_interval=3, fused=True, log_samples=log_samples,
+                     autotune=autotune)
     for _ in range(3):
         tr.step()
     losses = torch.zeros(B, dtype=torch.float64, device=dev)
@@ -469,6 +556,10 @@ def test_trainer_matches_stage_calls(c1, log_samples):
         d = math.pow(2.0, 1.0 / bc / 1.0)
         lr, sl = lr / d, sl / d
     torch.cuda.synchronize()
+    if autotune:
+        order_, wpb_, pers_ = tr.schedule()
+        say(f"    autotuned: order {order_}, waves/block {wpb_}, persistent {pers_}")
+        assert order_ in (0, 1, 2) and wpb_ in (1, 2, 4)
     t_, lr_, sl_ = tr.state()
     assert t_ == 3 and abs(lr_ - lr) < 1e-15 and abs(sl_ - sl) < 1e-15
     mse, psnr = tr.loss()

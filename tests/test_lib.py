@@ -38,7 +38,8 @@ def test_struct_layouts_match_header(pxl_lib):
     assert ctypes.sizeof(pxl_lib.BBox) == 24
     assert ctypes.sizeof(pxl_lib.RaysC) == 17 * 8 + 8 + 8
     assert ctypes.sizeof(pxl_lib.GridC) == 24 + 8 + 6 * 8 + 24 + 16
-    assert ctypes.sizeof(pxl_lib.TrainConfigC) == 8 + 4 * 8 + 4 * 4 + 8 + 4 * 4
+    assert ctypes.sizeof(pxl_lib.TrainConfigC) == 8 + 4 * 8 + 4 * 4 + 8 + 4 * 2 + 4 * 6
+    assert ctypes.sizeof(pxl_lib.ScheduleC) == 16
     assert ctypes.sizeof(pxl_lib.OptimC) == 48
 
 
