@@ -40,11 +40,17 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--grid", type=int, default=256)
+    ap.add_argument("--workload", choices=("plain256", "uneven512"), default="plain256",
+                    help="plain256: the 256^3 FP32 configuration the metric is quoted on; "
+                         "uneven512: 512^3 shells + blobs, BF16 records, 262144 "
+                         "rays/batch from a radius-6 sphere, autotuned schedule")
+    ap.add_argument("--grid", type=int, default=None)
     ap.add_argument("--views", type=int, default=100)
     ap.add_argument("--img", type=int, default=800)
-    ap.add_argument("--batch", type=int, default=65536)
-    ap.add_argument("--occupancy", type=float, default=0.10)
+    ap.add_argument("--batch", type=int, default=None)
+    ap.add_argument("--occupancy", type=float, default=None)
+    ap.add_argument("--autotune", type=int, default=None,
+                    help="1: the trainer times every batch order x schedule on the first batch")
     ap.add_argument("--cpu-rays", type=int, default=1024,
                     help="rays of the bounded CPU-baseline sample (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -56,7 +62,164 @@ def parse():
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="run one device-to-device copy of this many MiB first (a kernel of "
                          "known traffic for calibrating profiler byte counters)")
-    return ap.parse_args()
+    a = ap.parse_args()
+    uneven = a.workload == "uneven512"
+    a.grid = a.grid or (512 if uneven else 256)
+    a.batch = a.batch or (262144 if uneven else 65536)
+    a.occupancy = a.occupancy if a.occupancy is not None else (0.01 if uneven else 0.10)
+    a.autotune = a.autotune if a.autotune is not None else (1 if uneven else 0)
+    if uneven and a.log_samples == 512:
+        a.log_samples = 1024
+    return a
+
+
+def occupancy_threshold(dens, N, frac):
+    """Density threshold that leaves `frac` of the (N-1)^3 cells occupied
+    (a cell is occupied when a corner exceeds it)."""
+    import torch
+
+    ncell = (N - 1) ** 3
+    cellmax = torch.nn.functional.max_pool3d(dens.view(1, 1, N, N, N), 2, stride=1).reshape(-1)
+    k = int(round((1.0 - frac) * ncell))
+    tau = float(torch.kthvalue(cellmax.float().cpu(), k).values)
+    return tau
+
+
+def caged_grid(dens, N, tau, dev):
+    """Occupancy bits, cage, caged index and brick bits of a dense density."""
+    import torch
+
+    import pxl
+
+    ncell = (N - 1) ** 3
+    occ = torch.zeros((ncell + 31) // 32, dtype=torch.int32, device=dev)
+    cage = torch.zeros(N ** 3, dtype=torch.uint8, device=dev)
+    nocc = torch.zeros(1, dtype=torch.int64, device=dev)
+    pxl.build_occupancy(dens, N, N, N, tau, occ, cage, nocc)
+    io = torch.zeros(N ** 3, dtype=torch.int32, device=dev)
+    c2d = torch.zeros(N ** 3, dtype=torch.int32, device=dev)
+    cnt = torch.zeros(1, dtype=torch.int32, device=dev)
+    scratch = torch.zeros(pxl.sparse_index_scratch_bytes(N ** 3) // 4 + 1, dtype=torch.int32,
+                          device=dev)
+    pxl.build_sparse_index(cage, N, N, N, io, c2d, cnt, scratch)
+    torch.cuda.synchronize()
+    n_vox = int(cnt.item())
+    c2d = c2d[:n_vox].clone()
+    bricks = pxl.build_brick_occupancy(occ, N, N, N)
+    kw = dict(step=0.5, occupancy=occ, index_offsets=io, in_cage=cage, compact_to_dense=c2d,
+              brick_occupancy=bricks)
+    return kw, n_vox, int(nocc.item()) / ncell
+
+
+def set_targets(rays, grid, n_rays, dev):
+    """Render `grid` without background and store the un-premultiplied
+    colours and the opacity mask as the rays' 8-bit targets, so the
+    white-background compositing of the train step reproduces the render."""
+    import torch
+
+    import pxl
+
+    allidx = torch.arange(n_rays, dtype=torch.int32, device=dev)
+    chunk = 1 << 22
+    for s0 in range(0, n_rays, chunk):
+        pxl.render_fwd(grid, rays, allidx, start=s0, length=min(chunk, n_rays - s0))
+    torch.cuda.synchronize()
+    A = rays.acc_w.clamp(0, 1)
+    inv = 1.0 / A.clamp_min(1e-6)
+    to_u8 = lambda x: (x.clamp(0, 1) * 255.0 + 0.5).floor().to(torch.uint8)
+    rays.gt_r.copy_(to_u8(rays.acc_r * inv))
+    rays.gt_g.copy_(to_u8(rays.acc_g * inv))
+    rays.gt_b.copy_(to_u8(rays.acc_b * inv))
+    rays.gt_mask.copy_(to_u8(A))
+    rays.white_bg = True
+
+
+class UnevenWorkload:
+    """512^3 thin shells + blobs (seed 41), cells occupied where a corner
+    exceeds half the shell amplitude (a few % of cells), caged,
+    SH-2 stored in BF16 with an FP32 master copy; rays with origins uniform
+    on a radius-6 sphere (grid half-width 1) aimed at uniform points of
+    [-0.5, 0.5]^3, so per-ray sample counts run from 0 to ~2000; four
+    262144-ray batches with targets rendered from the seeded ground truth."""
+
+    def __init__(self, args, dev):
+        import torch
+
+        import pxl
+        from pxl import scenes
+
+        self.args = args
+        N, S = args.grid, 9
+        P = 1 + 3 * S
+        self.N, self.S, self.P = N, S, P
+        t0 = time.time()
+        dens = scenes.shell_density(N, 41, device=dev).reshape(-1)
+        # Half the shell amplitude: every shell cell and the blob cores are
+        # occupied (a quantile of the cell maxima would fall above the
+        # shells, whose density is one constant).
+        tau = 20.0
+        grid_kw, n_vox, self.occ_frac = caged_grid(dens, N, tau, dev)
+        self.n_vox = n_vox
+        c2d = grid_kw["compact_to_dense"]
+        gen = torch.Generator(device=dev)
+        gen.manual_seed(42)
+        rec32 = torch.zeros((n_vox, pxl.record_stride(S)), dtype=torch.float32, device=dev)
+        rec32[:, 0] = dens[c2d.long()]
+        rec32[:, 1:P].normal_(0.0, 0.3, generator=gen)
+        del dens
+        gt16 = pxl.records_to_16bit(rec32, pxl.STORE_BF16)
+        self.gt_grid = pxl.Grid(N, N, N, S, gt16, storage=pxl.STORE_BF16, **grid_kw)
+        log(f"grid {N}^3: occupied cells {self.occ_frac:.4f}, caged vertices {n_vox} "
+            f"({time.time() - t0:.1f}s)")
+
+        B = args.batch
+        self.n_rays = 4 * B
+        gen.manual_seed(43)
+        half = N / 2.0
+        u = torch.randn((self.n_rays, 3), generator=gen, device=dev, dtype=torch.float64)
+        origin = 6.0 * u / u.norm(dim=1, keepdim=True)
+        target = torch.rand((self.n_rays, 3), generator=gen, device=dev, dtype=torch.float64) - 0.5
+        o = origin * half + half
+        d = (target - origin)
+        d = (d / d.norm(dim=1, keepdim=True)).float()
+        lo, hi = 0.0, float(N - 1)
+        dd = d.double()
+        inv = 1.0 / torch.where(dd == 0, torch.full_like(dd, 1e-300), dd)
+        t1, t2 = (lo - o) * inv, (hi - o) * inv
+        tmin = torch.minimum(t1, t2).amax(dim=1).clamp_min(0.0)
+        tmax = torch.maximum(t1, t2).amin(dim=1)
+        rays = pxl.Rays(self.n_rays, device=dev, white_bg=False)
+        for i, k in enumerate(("ox", "oy", "oz")):
+            getattr(rays, k).copy_(o[:, i].float())
+        for i, k in enumerate(("dx", "dy", "dz")):
+            getattr(rays, k).copy_(d[:, i])
+        rays.near_t.copy_(tmin.float())
+        rays.far_t.copy_(tmax.float())
+        set_targets(rays, self.gt_grid, self.n_rays, dev)
+        self.rays = rays
+        self.order = torch.arange(self.n_rays, dtype=torch.int32, device=dev)
+        tr = torch.zeros(2 * B, dtype=torch.int32, device=dev)
+        pxl.render_fwd(self.gt_grid, rays, self.order, length=B, trace=tr)
+        cnt = tr.view(-1, 2)[:, 0].double().cpu()
+        self.samples = {"mean": round(cnt.mean().item(), 2), "max": int(cnt.max().item()),
+                        "zero_frac": round((cnt == 0).double().mean().item(), 4)}
+        log(f"rays {self.n_rays}, samples/ray {self.samples} ({time.time() - t0:.1f}s)")
+
+        zero32 = torch.zeros_like(rec32)
+        self.rec = pxl.records_to_16bit(zero32, pxl.STORE_BF16)
+        self.grid = pxl.Grid(N, N, N, S, self.rec, storage=pxl.STORE_BF16, **grid_kw)
+        self.optim = pxl.Optim(self.grid, master_from=zero32)
+        del rec32
+        self.trainer = pxl.Trainer(self.grid, self.optim, self.rays, self.order, batch_size=B,
+                                   learning_rate=0.3, sh_learning_rate=1.0,
+                                   variance_coefficient=1e-5, sh_coefficient=1e-6,
+                                   variance_type=pxl.TV_DISPARITY, neighbour_nums=6,
+                                   variance_interval=3, mask_coefficient=1.0, huber_delta=0.1,
+                                   fused=True, log_samples=args.log_samples,
+                                   autotune=bool(args.autotune))
+
+    def step(self):
+        self.trainer.step()
 
 
 class Workload:
@@ -176,7 +339,8 @@ class Workload:
                                    sh_coefficient=1e-6, variance_type=pxl.TV_DISPARITY,
                                    neighbour_nums=6, variance_interval=3, mask_coefficient=1.0,
                                    huber_delta=0.1, fused=True,
-                                   log_samples=args.log_samples)
+                                   log_samples=args.log_samples,
+                                   autotune=bool(args.autotune))
 
     def step(self):
         self.trainer.step()
@@ -282,7 +446,8 @@ def main():
         torch.cuda.synchronize()
         log(f"calibration copy: {args.copy_mib} MiB read + {args.copy_mib} MiB written")
         del src, dst
-    w = Workload(args, dev)
+    uneven = args.workload == "uneven512"
+    w = UnevenWorkload(args, dev) if uneven else Workload(args, dev)
     for _ in range(args.warmup):
         w.step()
     torch.cuda.synchronize()
@@ -316,10 +481,13 @@ def main():
     # by the forward and the backward, its gradient read and written).  The
     # step figure adds Adam's 896 B per record (parameters, gradient, two
     # moments: read + write).
+    # 16-bit records: 2 P s_p + 8 P = 336 B per touched record and
+    # P (2 s_p + 8 + 16) = 784 B per record for Adam.
+    s_p = 2 if uneven else 4
     vt = touched_vertices(w, dev)
-    alg = args.batch * 44 + vt * 448
+    alg = args.batch * 44 + vt * (2 * w.P * s_p + 8 * w.P)
     achieved = alg / (kern_ms / 1e3) / 1e9
-    step_alg = alg + w.n_vox * 896
+    step_alg = alg + w.n_vox * w.P * (2 * s_p + 8 + 16)
     kname = pxl.render_kernel_name(w.grid, "logged" if args.log_samples > 0 else "fused")
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5),
@@ -330,7 +498,7 @@ def main():
             "step_frac": round(step_alg / (ms_step / 1e3) / 1e9 / HBM_PEAK_GBS, 5)}
 
     cpu = None
-    if not args.no_cpu_baseline and args.cpu_rays > 0:
+    if not args.no_cpu_baseline and args.cpu_rays > 0 and not uneven:
         log("CPU baseline (oracle) ...")
         cpu = cpu_baseline(w, dev, args.cpu_rays)
     out = {
@@ -355,6 +523,17 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    order, wpb, pers = w.trainer.schedule()
+    out["config"]["schedule"] = {"batch_order": ["given", "morton", "length+morton"][order],
+                                 "waves_per_block": wpb, "persistent": pers,
+                                 "autotuned": bool(args.autotune)}
+    if uneven:
+        out["metric"] = "train rays/sec (fwd+bwd+TV+Adam) 512^3 uneven SH2 BF16"
+        out["config"]["workload"] = (
+            f"uneven BF16-stored, {args.grid}^3 SH-2 caged shells + blobs at {w.occ_frac:.4f} "
+            f"cell occupancy, {args.batch} rays/batch from a radius-6 sphere, step 0.5, "
+            f"samples/ray {w.samples}")
+        out["config"].pop("batch_order")
     print(json.dumps(out), flush=True)
 
 
