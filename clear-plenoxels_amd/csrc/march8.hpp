@@ -290,6 +290,14 @@ __device__ __forceinline__ void march8_ray(const RenderArgs &a, const int64_t b)
     float Y[S];
 #pragma unroll
     for (int k = 0; k < S; k++) Y[k] = sh_basis(k, fdx, fdy, fdz);
+    // Proposal mask of this batch entry (all bins when none is given).
+    uint64_t bmask = ~0ull;
+    if (a.bin_mask) {
+        const uint64_t m = a.bin_mask[b];
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)m);
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(m >> 32));
+        bmask = ((uint64_t)hi << 32) | (uint64_t)lo;
+    }
 
     // Gradient layout (backward): lane (lh = lane / 32, pp = lane % 32) holds
     // in acc[h] parameter pp of corner 2h + lh of the current cell; ysel is
@@ -359,6 +367,7 @@ __device__ __forceinline__ void march8_ray(const RenderArgs &a, const int64_t b)
                         occ = (a.occ[cell >> 5] >> (cell & 31)) & 1u;
                     }
                 }
+                if (bmask != ~0ull && occ) occ = (bmask >> bin_of(k, step, tnear, tfar)) & 1ull;
                 pending = __ballot(occ);
                 more = __ballot(valid) == ~0ull;
                 if (pending == 0) {
@@ -596,29 +605,36 @@ __device__ __forceinline__ double wave_scan_mul64(double v, int lane) {
 // backward (register accumulators, corner carry), but with every input
 // already in registers: no global load follows an atomic until the next 64
 // samples, so the atomics never hold up the wave.
-template <int S>
+template <int S, bool TGT>
 __device__ __forceinline__ void log_grad_ray(const RenderArgs &a, const int64_t b);
 
-template <int S, bool PERSIST = false>
+// TGT: also write the proposal targets (a.bin_targets, a.target_valid).
+template <int S, bool PERSIST = false, bool TGT = false>
 __global__ __launch_bounds__(256) void k_log_grad(const RenderArgs a) {
     if (PERSIST) {
         int slice = blockIdx.x & 7, visited = 0;
         for (int64_t b = claim_ray(a.work + 8, a.len, slice, visited); b < a.len;
              b = claim_ray(a.work + 8, a.len, slice, visited))
-            log_grad_ray<S>(a, b);
+            log_grad_ray<S, TGT>(a, b);
     } else {
         const int64_t b = (int64_t)xcd_slice_block(blockIdx.x, gridDim.x) * (blockDim.x >> 6) +
                           (threadIdx.x >> 6);
-        if (b < a.len) log_grad_ray<S>(a, b);
+        if (b < a.len) log_grad_ray<S, TGT>(a, b);
     }
 }
 
-template <int S>
+template <int S, bool TGT>
 __device__ __forceinline__ void log_grad_ray(const RenderArgs &a, const int64_t b) {
     constexpr int P = 1 + 3 * S;
     const int lane = threadIdx.x & 63;
     const int cnt = uniform_i(a.log_count[b]);
-    if (cnt <= 0) return;  // nothing logged, or done by the marcher
+    if (cnt <= 0) {  // nothing logged, or done by the marcher
+        if (TGT && lane == 0) {
+            a.bin_targets[b] = 0ull;
+            a.target_valid[b] = cnt == 0 ? 1 : 0;
+        }
+        return;
+    }
     const int ray = uniform_i(a.sample_index[a.start + b]);
     const double ox = a.rays.ox[ray], oy = a.rays.oy[ray], oz = a.rays.oz[ray];
     const float fdx = a.rays.dx[ray], fdy = a.rays.dy[ray], fdz = a.rays.dz[ray];
@@ -646,6 +662,8 @@ __device__ __forceinline__ void log_grad_ray(const RenderArgs &a, const int64_t 
     uint32_t cur_rec[4] = {0u, 0u, 0u, 0u};
     float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     float acc0 = 0.0f;
+    uint64_t tmask = 0ull;  // bins holding weight (proposal targets)
+    const double tfar = a.rays.far_t[ray];
     for (int c0 = 0; c0 < cnt; c0 += 64) {
         const int m = min(64, cnt - c0);
         const bool act = lane < m;
@@ -675,6 +693,13 @@ __device__ __forceinline__ void log_grad_ray(const RenderArgs &a, const int64_t 
         sample_grads(rr, C0, C1, C2, A, step, ws, Tn, Ai, H0, H1, H2, col0, col1, col2, sraw, g0,
                      g1, g2, gs);
         const float gsf = (float)gs;
+        if (TGT) {
+            uint64_t bit = act && ws > a.target_wmin ? 1ull << bin_of(k, step, tnear, tfar) : 0ull;
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1)
+                bit |= (uint64_t)__shfl_xor((unsigned long long)bit, off, 64);
+            tmask |= bit;
+        }
         // Position of sample k exactly as the marcher computed it.
         const double t = fma((double)(act ? k : 0), step, tnear);
         const double px = fma(t, dx, ox), py = fma(t, dy, oy), pz = fma(t, dz, oz);
@@ -721,5 +746,9 @@ __device__ __forceinline__ void log_grad_ray(const RenderArgs &a, const int64_t 
     if (has_cur) {
         retire_cell<P>(a, acc, cur_rec, acc0, 2, 2, 2, lh, pp);
         flush_record0<P>(a, acc0, lh, pp);
+    }
+    if (TGT && lane == 0) {
+        a.bin_targets[b] = tmask;
+        a.target_valid[b] = 1;
     }
 }

@@ -65,7 +65,22 @@ struct RenderArgs {
     int log_redo;  // kFused: only the batch entries whose count is -1
     // Persistent launches: 8 claim counters per kernel (zeroed before launch).
     unsigned long long *work;
+    // Learned proposal: per batch entry, the bins a ray may evaluate (null:
+    // all), and the bins where the logged samples carry weight > target_wmin
+    // (written by k_log_grad; target_valid 0 for rays it does not see).
+    const uint64_t *bin_mask;
+    uint64_t *bin_targets;
+    uint8_t *target_valid;
+    double target_wmin;
 };
+
+// Proposal bin of step k: 64 equal bins of [t_near, t_far).
+__device__ __forceinline__ int bin_of(int k, double step, double tn, double tf) {
+    const double span = tf - tn;
+    if (!(span > 0.0)) return 0;
+    const double q = floor((double)k * step * 64.0 / span);
+    return q > 63.0 ? 63 : (int)q;
+}
 
 // Activations (RenderUtils.cu:480-515) in float or double.
 template <typename R> __device__ __forceinline__ R softplus_shift(R raw);
@@ -669,6 +684,28 @@ extern "C" int32_t pxl_render_fwd(const pxl_grid *grid, const pxl_rays *rays,
     return st != PXL_OK ? st : check_launch("pxl_render_fwd");
 }
 
+extern "C" int32_t pxl_render_fwd_masked(const pxl_grid *grid, const pxl_rays *rays,
+                                         const int32_t *sample_index, int64_t start, int64_t len,
+                                         const uint64_t *bin_mask, uint32_t *trace,
+                                         void *stream) {
+    clear_error();
+    RenderArgs a;
+    int32_t st = prepare(grid, rays, sample_index, start, len, "pxl_render_fwd_masked", a);
+    if (st != PXL_OK) return st;
+    PXL_REQUIRE(rays->acc_r && rays->acc_g && rays->acc_b && rays->acc_w && rays->acc_density,
+                "pxl_render_fwd_masked: output planes must be non-null");
+    PXL_REQUIRE(grid->march_mode == PXL_MARCH_FIXED && grid->sh_dim <= 9 && use_march8(),
+                "pxl_render_fwd_masked: needs the eight-sample marcher (fixed rule, sh_dim <= 9)");
+    if (len == 0) return PXL_OK;
+    a.bin_mask = bin_mask;
+    a.trace = trace;
+    st = trace ? dispatch<kTrace>(grid->sh_dim, grid->sparse, false, grid->storage, a,
+                                  as_stream(stream))
+               : dispatch<kFwd>(grid->sh_dim, grid->sparse, false, grid->storage, a,
+                                as_stream(stream));
+    return st != PXL_OK ? st : check_launch("pxl_render_fwd_masked");
+}
+
 extern "C" int32_t pxl_render_fwd_trace(const pxl_grid *grid, const pxl_rays *rays,
                                         const int32_t *sample_index, int64_t start,
                                         int64_t len, uint32_t *trace, void *stream) {
@@ -738,18 +775,28 @@ template <int S, bool SPARSE, int STORE>
 void launch_logged(const RenderArgs &a, int wpb, bool persist, hipStream_t st) {
     const int threads = 64 * wpb;
     const unsigned blocks = (unsigned)((a.len + wpb - 1) / wpb);
+    const bool tgt = a.bin_targets != nullptr;
     if (persist) {
         hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, a.work, 16);
         const unsigned ga =
             std::min(blocks, resident_blocks(k_march8<S, SPARSE, kLog, STORE, true>, threads));
-        const unsigned gb = std::min(blocks, resident_blocks(k_log_grad<S, true>, threads));
         hipLaunchKernelGGL((k_march8<S, SPARSE, kLog, STORE, true>), dim3(ga), dim3(threads), 0,
                            st, a);
-        hipLaunchKernelGGL((k_log_grad<S, true>), dim3(gb), dim3(threads), 0, st, a);
+        if (tgt) {
+            const unsigned gb =
+                std::min(blocks, resident_blocks(k_log_grad<S, true, true>, threads));
+            hipLaunchKernelGGL((k_log_grad<S, true, true>), dim3(gb), dim3(threads), 0, st, a);
+        } else {
+            const unsigned gb = std::min(blocks, resident_blocks(k_log_grad<S, true>, threads));
+            hipLaunchKernelGGL((k_log_grad<S, true>), dim3(gb), dim3(threads), 0, st, a);
+        }
     } else {
         hipLaunchKernelGGL((k_march8<S, SPARSE, kLog, STORE>), dim3(blocks), dim3(threads), 0, st,
                            a);
-        hipLaunchKernelGGL((k_log_grad<S>), dim3(blocks), dim3(threads), 0, st, a);
+        if (tgt)
+            hipLaunchKernelGGL((k_log_grad<S, false, true>), dim3(blocks), dim3(threads), 0, st, a);
+        else
+            hipLaunchKernelGGL((k_log_grad<S>), dim3(blocks), dim3(threads), 0, st, a);
     }
     RenderArgs redo = a;
     redo.log_redo = 1;
@@ -792,7 +839,9 @@ static int32_t bwd_common(const pxl_grid *grid, const pxl_optim *optim, const px
                           double *losses, const int32_t *sample_index, int64_t start,
                           int64_t len, double mask_coef, float huber_delta, void *stream,
                           bool fused, void *workspace = nullptr, size_t workspace_bytes = 0,
-                          int32_t max_samples = 0, const pxl_schedule *sched = nullptr) {
+                          int32_t max_samples = 0, const pxl_schedule *sched = nullptr,
+                          const uint64_t *bin_mask = nullptr, uint64_t *bin_targets = nullptr,
+                          uint8_t *target_valid = nullptr, double target_wmin = 0.0) {
     const char *who = workspace ? "pxl_train_rays_ws" : fused ? "pxl_train_rays" : "pxl_render_bwd_rgb";
     clear_error();
     RenderArgs a;
@@ -816,7 +865,16 @@ static int32_t bwd_common(const pxl_grid *grid, const pxl_optim *optim, const px
     a.mask_coef = mask_coef;
     a.huber = huber_delta;
     const bool adapt = grid->march_mode == PXL_MARCH_REF_ADAPTIVE;
-    if (workspace && !adapt && grid->sh_dim <= 9 && use_march8()) {
+    const bool group = !adapt && grid->sh_dim <= 9 && use_march8();
+    PXL_REQUIRE(!bin_mask || group,
+                "%s: proposal masks need the eight-sample marcher (fixed rule, sh_dim <= 9)", who);
+    PXL_REQUIRE(!bin_targets || (group && workspace && target_valid),
+                "%s: proposal targets need the logged step and a validity array", who);
+    a.bin_mask = bin_mask;
+    a.bin_targets = bin_targets;
+    a.target_valid = target_valid;
+    a.target_wmin = target_wmin;
+    if (workspace && group) {
         PXL_REQUIRE(max_samples > 0 && max_samples <= kMaxLogSamples,
                     "%s: max_samples %d outside [1, %d]", who, max_samples, kMaxLogSamples);
         const LogLayout L = log_layout(len, max_samples);
@@ -879,6 +937,25 @@ extern "C" int32_t pxl_train_rays_ws(const pxl_grid *grid, const pxl_optim *opti
     }
     return bwd_common(grid, optim, rays, losses, sample_index, start, len, mask_coef,
                       huber_delta, stream, true, workspace, workspace_bytes, max_samples);
+}
+
+extern "C" int32_t pxl_train_rays_prop(const pxl_grid *grid, const pxl_optim *optim,
+                                       const pxl_rays *rays, double *losses,
+                                       const int32_t *sample_index, int64_t start, int64_t len,
+                                       double mask_coef, float huber_delta, void *workspace,
+                                       size_t workspace_bytes, int32_t max_samples,
+                                       const pxl_schedule *sched, const uint64_t *bin_mask,
+                                       uint64_t *bin_targets, uint8_t *target_valid,
+                                       double target_wmin, void *stream) {
+    if (!workspace) {
+        clear_error();
+        set_error("pxl_train_rays_prop: null workspace");
+        return PXL_EINVAL;
+    }
+    const pxl_schedule dflt = {4, 0, {0, 0}};
+    return bwd_common(grid, optim, rays, losses, sample_index, start, len, mask_coef,
+                      huber_delta, stream, true, workspace, workspace_bytes, max_samples,
+                      sched ? sched : &dflt, bin_mask, bin_targets, target_valid, target_wmin);
 }
 
 extern "C" int32_t pxl_train_rays_sched(const pxl_grid *grid, const pxl_optim *optim,

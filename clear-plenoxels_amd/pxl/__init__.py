@@ -34,8 +34,14 @@ EXPORTS = (
     "pxl_build_brick_occupancy", "pxl_records_to_16bit", "pxl_tv_prepare",
     "pxl_render_kernel_name", "pxl_train_rays_workspace_bytes", "pxl_train_rays_ws",
     "pxl_train_rays_sched", "pxl_batch_order_scratch_bytes", "pxl_batch_order",
-    "pxl_trainer_schedule",
+    "pxl_trainer_schedule", "pxl_proposal_workspace_bytes", "pxl_proposal_init",
+    "pxl_proposal_features", "pxl_proposal_forward", "pxl_proposal_train_step",
+    "pxl_proposal_masks", "pxl_render_fwd_masked", "pxl_train_rays_prop",
 )
+
+PROP_IN, PROP_WIDTH, PROP_BINS = 128, 64, 64
+PROP_WEIGHTS = PROP_IN * PROP_WIDTH + 2 * PROP_WIDTH * PROP_WIDTH
+PROP_PARAMS = PROP_WEIGHTS + 3 * PROP_WIDTH
 
 
 class PxlError(RuntimeError):
@@ -119,6 +125,10 @@ class TrainConfigC(ctypes.Structure):
                 ("waves_per_block", _I32), ("persistent", _I32), ("reserved", _I32)]
 
 
+class ProposalC(ctypes.Structure):
+    _fields_ = [("weights", _P), ("master", _P), ("m", _P), ("v", _P), ("grad", _P)]
+
+
 class ScheduleC(ctypes.Structure):
     _fields_ = [("waves_per_block", _I32), ("persistent", _I32), ("reserved", _I32 * 2)]
 
@@ -170,6 +180,16 @@ def _declare(L):
         "pxl_batch_order_scratch_bytes": ([_I64], _SZ),
         "pxl_batch_order": ([G, R, _P, _I64, _I64, _I32, _P, _P, _SZ, _P], _I32),
         "pxl_trainer_schedule": ([_P, ctypes.POINTER(_I32), ctypes.POINTER(ScheduleC)], _I32),
+        "pxl_proposal_workspace_bytes": ([_I64], _SZ),
+        "pxl_proposal_init": ([ctypes.POINTER(ProposalC), ctypes.c_uint64, _P], _I32),
+        "pxl_proposal_features": ([G, R, _P, _I64, _I64, _P, _SZ, _P], _I32),
+        "pxl_proposal_forward": ([ctypes.POINTER(ProposalC), _I64, _P, _P, _SZ, _P], _I32),
+        "pxl_proposal_train_step": ([ctypes.POINTER(ProposalC), _I64, _P, _P, _I64, _F32,
+                                     _F64, _I32, _P, _SZ, _P, _P], _I32),
+        "pxl_proposal_masks": ([_P, _I64, _F32, _I32, _P, _P], _I32),
+        "pxl_render_fwd_masked": ([G, R, _P, _I64, _I64, _P, _P, _P], _I32),
+        "pxl_train_rays_prop": ([G, O, R, _P, _P, _I64, _I64, _F64, _F32, _P, _SZ, _I32,
+                                 ctypes.POINTER(ScheduleC), _P, _P, _P, _F64, _P], _I32),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -439,6 +459,92 @@ def batch_order(grid, rays, sample_index, out, mode=ORDER_MORTON, start=0, lengt
                                  length, mode, _ptr(out), _ptr(scratch), scratch.numel(),
                                  _stream(stream)))
     return out
+
+
+class Proposal:
+    """Learned proposal network (BF16 weights, FP32 master and Adam moments)
+    in device memory, initialised from `seed`."""
+
+    def __init__(self, device="cuda", seed=0, stream=None):
+        import torch
+
+        self.weights = torch.zeros(PROP_WEIGHTS, dtype=torch.int16, device=device)
+        self.master = torch.zeros(PROP_PARAMS, dtype=torch.float32, device=device)
+        self.m = torch.zeros_like(self.master)
+        self.v = torch.zeros_like(self.master)
+        self.grad = torch.zeros_like(self.master)
+        c = self.c()
+        _check("pxl_proposal_init", lib().pxl_proposal_init(ctypes.byref(c), seed,
+                                                            _stream(stream)))
+
+    def c(self):
+        return ProposalC(self.weights.data_ptr(), self.master.data_ptr(), self.m.data_ptr(),
+                         self.v.data_ptr(), self.grad.data_ptr())
+
+
+def proposal_workspace(length, device="cuda"):
+    import torch
+
+    return torch.empty(lib().pxl_proposal_workspace_bytes(length), dtype=torch.uint8,
+                       device=device)
+
+
+def proposal_features(grid, rays, sample_index, workspace, start=0, length=None, stream=None):
+    length = sample_index.numel() - start if length is None else length
+    g, r = grid.c(), rays.c()
+    _check("pxl_proposal_features",
+           lib().pxl_proposal_features(ctypes.byref(g), ctypes.byref(r), _ptr(sample_index),
+                                       start, length, _ptr(workspace), workspace.numel(),
+                                       _stream(stream)))
+
+
+def proposal_forward(prop, length, logits, workspace, stream=None):
+    c = prop.c()
+    _check("pxl_proposal_forward",
+           lib().pxl_proposal_forward(ctypes.byref(c), length, _ptr(logits), _ptr(workspace),
+                                      workspace.numel(), _stream(stream)))
+
+
+def proposal_train_step(prop, length, targets, workspace, t, lr=1e-3, valid=None, n_valid=None,
+                        pos_weight=1.0, loss_out=None, stream=None):
+    c = prop.c()
+    n_valid = length if n_valid is None else n_valid
+    _check("pxl_proposal_train_step",
+           lib().pxl_proposal_train_step(ctypes.byref(c), length, _ptr(targets), _ptr(valid),
+                                         n_valid, pos_weight, lr, t, _ptr(workspace),
+                                         workspace.numel(), _ptr(loss_out), _stream(stream)))
+
+
+def proposal_masks(logits, length, masks, threshold=0.05, dilate=1, stream=None):
+    _check("pxl_proposal_masks",
+           lib().pxl_proposal_masks(_ptr(logits), length, threshold, dilate, _ptr(masks),
+                                    _stream(stream)))
+
+
+def render_fwd_masked(grid, rays, sample_index, bin_mask=None, start=0, length=None,
+                      trace=None, stream=None):
+    length = sample_index.numel() - start if length is None else length
+    g, r = grid.c(), rays.c()
+    _check("pxl_render_fwd_masked",
+           lib().pxl_render_fwd_masked(ctypes.byref(g), ctypes.byref(r), _ptr(sample_index),
+                                       start, length, _ptr(bin_mask), _ptr(trace),
+                                       _stream(stream)))
+
+
+def train_rays_prop(grid, optim, rays, losses, sample_index, workspace, max_samples,
+                    bin_mask=None, bin_targets=None, target_valid=None, target_wmin=1e-4,
+                    waves_per_block=4, persistent=False, start=0, length=None, mask_coef=1.0,
+                    huber_delta=0.1, stream=None):
+    """The two-kernel step with an optional proposal mask and target output."""
+    length = sample_index.numel() - start if length is None else length
+    g, o, r = grid.c(), optim.c(), rays.c()
+    sc = ScheduleC(waves_per_block, int(persistent), (_I32 * 2)(0, 0))
+    _check("pxl_train_rays_prop",
+           lib().pxl_train_rays_prop(ctypes.byref(g), ctypes.byref(o), ctypes.byref(r),
+                                     _ptr(losses), _ptr(sample_index), start, length, mask_coef,
+                                     huber_delta, _ptr(workspace), workspace.numel(), max_samples,
+                                     ctypes.byref(sc), _ptr(bin_mask), _ptr(bin_targets),
+                                     _ptr(target_valid), target_wmin, _stream(stream)))
 
 
 def tv_scratch(grid, stream=None):

@@ -351,6 +351,77 @@ int32_t pxl_batch_order(const pxl_grid *grid, const pxl_rays *rays,
                         int32_t mode, int32_t *out_index, void *scratch,
                         size_t scratch_bytes, void *stream);
 
+/* ------------------------------------------------------------------ */
+/* Learned proposal (no counterpart in the reference): a per-ray MLP that */
+/* predicts which of PXL_PROP_BINS equal bins of [t_near, t_far) hold     */
+/* samples of visible weight.                                            */
+/* ------------------------------------------------------------------ */
+
+#define PXL_PROP_IN 128     /* 126 positional-encoding features + 2 zero */
+#define PXL_PROP_WIDTH 64
+#define PXL_PROP_BINS 64
+/* Weights W1 [128][64], W2 [64][64], W3 [64][64] (row-major, input-major),
+ * then biases b1, b2, b3 [64] each. */
+#define PXL_PROP_WEIGHTS (PXL_PROP_IN * PXL_PROP_WIDTH + 2 * PXL_PROP_WIDTH * PXL_PROP_WIDTH)
+#define PXL_PROP_PARAMS (PXL_PROP_WEIGHTS + 3 * PXL_PROP_WIDTH)
+
+typedef struct pxl_proposal {
+    uint16_t *weights;   /* PXL_PROP_WEIGHTS bfloat16 (the copy the kernels use) */
+    float *master;       /* PXL_PROP_PARAMS FP32: weights then biases */
+    float *m, *v;        /* Adam moments, PXL_PROP_PARAMS each */
+    float *grad;         /* PXL_PROP_PARAMS: gradient of the last train step */
+} pxl_proposal;
+
+/* Workspace of the proposal calls for a batch of len rays (256-byte
+ * aligned).  pxl_proposal_features fills its feature rows; forward and
+ * train_step read them. */
+size_t pxl_proposal_workspace_bytes(int64_t len);
+/* He-uniform weights from `seed`, zero biases and moments. */
+int32_t pxl_proposal_init(const pxl_proposal *p, uint64_t seed, void *stream);
+/* Features of batch entries sample_index[start .. start+len): positional
+ * encoding (10 frequencies) of the grid entry point o + max(t_near, 0) d,
+ * mapped to [-1, 1]^3, and of the direction. */
+int32_t pxl_proposal_features(const pxl_grid *grid, const pxl_rays *rays,
+                              const int32_t *sample_index, int64_t start, int64_t len,
+                              void *workspace, size_t workspace_bytes, void *stream);
+/* Bin logits [len][64] (FP32) of the featured batch. */
+int32_t pxl_proposal_forward(const pxl_proposal *p, int64_t len, float *logits,
+                             void *workspace, size_t workspace_bytes, void *stream);
+/* One training step on the featured batch: binary cross entropy of every
+ * bin against bit j of targets[b] (rows with valid[b] == 0 ignored; valid
+ * may be NULL), positives weighted pos_weight, mean over n_valid rows x 64
+ * bins; gradient into p->grad; then Adam (lr, step t >= 1) when lr > 0.
+ * loss_out (device, optional): the weighted BCE sum (FP64). */
+int32_t pxl_proposal_train_step(const pxl_proposal *p, int64_t len, const uint64_t *targets,
+                                const uint8_t *valid, int64_t n_valid, float pos_weight,
+                                double lr, int32_t t, void *workspace, size_t workspace_bytes,
+                                double *loss_out, void *stream);
+/* Allowed-bin masks from logits: bit j set when sigmoid(logit j) >
+ * threshold, dilated by `dilate` bins on each side. */
+int32_t pxl_proposal_masks(const float *logits, int64_t len, float threshold, int32_t dilate,
+                           uint64_t *masks, void *stream);
+
+/* pxl_render_fwd with a proposal mask: batch entry b evaluates step k only
+ * when bit floor(k step 64 / (t_far - t_near)) (at most 63) of
+ * bin_mask[b] is set (bin_mask NULL: every step, as pxl_render_fwd).  With
+ * trace != NULL, also the per-entry sample count and hash as
+ * pxl_render_fwd_trace.  Eight-sample marcher only (fixed rule, sh_dim <= 9). */
+int32_t pxl_render_fwd_masked(const pxl_grid *grid, const pxl_rays *rays,
+                              const int32_t *sample_index, int64_t start, int64_t len,
+                              const uint64_t *bin_mask, uint32_t *trace, void *stream);
+/* pxl_train_rays_sched with a proposal mask (as above, may be NULL) and, when
+ * bin_targets is not NULL, the proposal's training targets: per batch entry
+ * the bins whose logged samples carry weight T alpha > target_wmin, and
+ * target_valid[b] = 0 for rays longer than the log.  sched may be NULL. */
+int32_t pxl_train_rays_prop(const pxl_grid *grid, const pxl_optim *optim,
+                            const pxl_rays *rays, double *losses,
+                            const int32_t *sample_index, int64_t start, int64_t len,
+                            double mask_coef, float huber_delta, void *workspace,
+                            size_t workspace_bytes, int32_t max_samples,
+                            const pxl_schedule *sched, const uint64_t *bin_mask,
+                            uint64_t *bin_targets, uint8_t *target_valid, double target_wmin,
+                            void *stream);
+
 /* Name of the kernel the render entry points launch for this grid:
  * mode 0 = pxl_render_fwd, 1 = pxl_render_bwd_rgb, 2 = pxl_train_rays,
  * 3 = pxl_train_rays_ws (two names joined by '+' when it runs as two kernels).

@@ -81,6 +81,13 @@ def _declare(L):
     L.orc_render_bwd.argtypes = [ctypes.POINTER(OrcGrid), ctypes.POINTER(OrcRays), _P, _I64, _I64,
                                  _P, _P, _P, _P, _F64, _F32, _P, _P]
     L.orc_render_bwd.restype = None
+    L.orc_render_fwd_masked.argtypes = [ctypes.POINTER(OrcGrid), ctypes.POINTER(OrcRays), _P, _I64,
+                                        _I64, _P, ctypes.c_double, _P, _P, _P, _P, _P, _P, _P]
+    L.orc_render_fwd_masked.restype = None
+    L.orc_render_bwd_masked.argtypes = [ctypes.POINTER(OrcGrid), ctypes.POINTER(OrcRays), _P, _I64,
+                                        _I64, _P, _P, _P, _P, _P, ctypes.c_double,
+                                        ctypes.c_float, _P, _P]
+    L.orc_render_bwd_masked.restype = None
     L.orc_tv_disparity.argtypes = [ctypes.POINTER(OrcGrid), _F64, _F64, _I32, _P]
     L.orc_tv_disparity.restype = None
     L.orc_tv_neighbour6.argtypes = [ctypes.POINTER(OrcGrid), _F64, _F64, _P]
@@ -224,6 +231,48 @@ def render_bwd(grid, rays, sample_index, fwd, mask_coef=1.0, huber_delta=0.1, st
     lib().orc_render_bwd(ctypes.byref(g), ctypes.byref(r), _p(si), start, length,
                          _p(f["r"]), _p(f["g"]), _p(f["b"]), _p(f["w"]), mask_coef,
                          huber_delta, _p(losses), _p(grad))
+    return losses, grad
+
+
+def render_fwd_masked(grid, rays, sample_index, bin_mask=None, target_wmin=None, start=0,
+                      length=None, trace=False):
+    """render_fwd with a per-batch-entry proposal mask (uint64[len] or None);
+    with target_wmin, also out["targets"]: per batch entry the bins holding a
+    sample of weight T alpha > target_wmin."""
+    si = _c(sample_index, np.int32)
+    if length is None:
+        length = len(si) - start
+    out = {k: np.zeros(rays.n, np.float64) for k in ("r", "g", "b", "w", "density")}
+    tr = np.zeros((length, 2), np.uint32) if trace else None
+    bm = None if bin_mask is None else _c(bin_mask, np.uint64)
+    tg = np.zeros(length, np.uint64) if target_wmin is not None else None
+    g, r = grid.c(), rays.c()
+    lib().orc_render_fwd_masked(ctypes.byref(g), ctypes.byref(r), _p(si), start, length, _p(bm),
+                                0.0 if target_wmin is None else float(target_wmin), _p(tg),
+                                _p(out["r"]), _p(out["g"]), _p(out["b"]), _p(out["w"]),
+                                _p(out["density"]), _p(tr))
+    if trace:
+        out["trace"] = tr
+    if tg is not None:
+        out["targets"] = tg
+    return out
+
+
+def render_bwd_masked(grid, rays, sample_index, fwd, bin_mask=None, mask_coef=1.0,
+                      huber_delta=0.1, start=0, length=None, grad=None):
+    si = _c(sample_index, np.int32)
+    if length is None:
+        length = len(si) - start
+    P = 1 + 3 * grid.S
+    if grad is None:
+        grad = np.zeros((P, grid.n_vox), np.float64)
+    losses = np.zeros(length, np.float64)
+    f = {k: _c(fwd[k], np.float64) for k in ("r", "g", "b", "w")}
+    bm = None if bin_mask is None else _c(bin_mask, np.uint64)
+    g, r = grid.c(), rays.c()
+    lib().orc_render_bwd_masked(ctypes.byref(g), ctypes.byref(r), _p(si), start, length, _p(bm),
+                                _p(f["r"]), _p(f["g"]), _p(f["b"]), _p(f["w"]), mask_coef,
+                                huber_delta, _p(losses), _p(grad))
     return losses, grad
 
 

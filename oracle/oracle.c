@@ -427,9 +427,25 @@ static uint32_t fnv_mix(uint32_t h, uint32_t x) {
     return h;
 }
 
-void orc_render_fwd(const orc_grid *g, const orc_rays *r, const int32_t *sample_index,
-                    int64_t start, int64_t len, double *out_r, double *out_g, double *out_b,
-                    double *out_w, double *out_density, uint32_t *trace) {
+/* Proposal bin of step k of a ray: 64 equal bins of [t_near, t_far). */
+static int bin_of(int k, double step, double tn, double tf) {
+    const double span = tf - tn;
+    if (!(span > 0.0)) return 0;
+    const double q = floor((double)k * step * 64.0 / span);
+    return q > 63.0 ? 63 : (int)q;
+}
+
+/* Whether batch entry b may evaluate step k (no mask: always). */
+static int bin_allowed(const uint64_t *bin_mask, int64_t b, int k, double step, double tn,
+                       double tf) {
+    if (!bin_mask) return 1;
+    return (int)((bin_mask[b] >> bin_of(k, step, tn, tf)) & 1u);
+}
+
+static void render_fwd(const orc_grid *g, const orc_rays *r, const int32_t *sample_index,
+                       int64_t start, int64_t len, double *out_r, double *out_g, double *out_b,
+                       double *out_w, double *out_density, uint32_t *trace,
+                       const uint64_t *bin_mask, double target_wmin, uint64_t *targets) {
     const double step = (double)g->step;
     for (int64_t b = 0; b < len; b++) {
         const int ray = sample_index[start + b];
@@ -443,10 +459,11 @@ void orc_render_fwd(const orc_grid *g, const orc_rays *r, const int32_t *sample_
             sh_basis(dx, dy, dz, Y);
             double t = tn, lam = 0.0;
             int k = 0;
+            uint64_t tgt = 0;
             while (t < tf && k < ORC_MAX_STEPS) {
                 const double px = fma(t, dx, ox), py = fma(t, dy, oy), pz = fma(t, dz, oz);
                 sample s;
-                if (!locate(g, px, py, pz, &s)) {
+                if (!locate(g, px, py, pz, &s) || !bin_allowed(bin_mask, b, k, step, tn, tf)) {
                     if (g->adaptive) t += step;
                     ++k;
                     if (!g->adaptive) t = fma((double)k, step, tn);
@@ -462,6 +479,7 @@ void orc_render_fwd(const orc_grid *g, const orc_rays *r, const int32_t *sample_
                 const double sigma = softplus13(sraw);
                 const double T = exp(-lam);
                 const double w = T * (-expm1(-sigma * step));
+                if (w > target_wmin) tgt |= (uint64_t)1 << bin_of(k, step, tn, tf);
                 A += w;
                 for (int ch = 0; ch < 3; ch++) C[ch] += w * sigmoid01(craw[ch]);
                 sig += sigma;
@@ -472,6 +490,9 @@ void orc_render_fwd(const orc_grid *g, const orc_rays *r, const int32_t *sample_
             }
             if (r->white_bg)
                 for (int ch = 0; ch < 3; ch++) C[ch] += 1.0 - A;
+            if (targets) targets[b] = tgt;
+        } else if (targets) {
+            targets[b] = 0;
         }
         out_r[ray] = C[0];
         out_g[ray] = C[1];
@@ -485,12 +506,32 @@ void orc_render_fwd(const orc_grid *g, const orc_rays *r, const int32_t *sample_
     }
 }
 
+void orc_render_fwd(const orc_grid *g, const orc_rays *r, const int32_t *sample_index,
+                    int64_t start, int64_t len, double *out_r, double *out_g, double *out_b,
+                    double *out_w, double *out_density, uint32_t *trace) {
+    render_fwd(g, r, sample_index, start, len, out_r, out_g, out_b, out_w, out_density, trace,
+               NULL, 0.0, NULL);
+}
+
+/* The forward with a proposal mask (steps in bins whose bit of
+ * bin_mask[b] is clear are skipped like unoccupied cells; bin_mask may be
+ * NULL) and, when targets is not NULL, the bins holding a sample of weight
+ * T alpha above target_wmin as a 64-bit mask per batch entry. */
+void orc_render_fwd_masked(const orc_grid *g, const orc_rays *r, const int32_t *sample_index,
+                           int64_t start, int64_t len, const uint64_t *bin_mask,
+                           double target_wmin, uint64_t *targets, double *out_r, double *out_g,
+                           double *out_b, double *out_w, double *out_density, uint32_t *trace) {
+    render_fwd(g, r, sample_index, start, len, out_r, out_g, out_b, out_w, out_density, trace,
+               bin_mask, target_wmin, targets);
+}
+
 /* renderRays_withGradients_RGBGPU + trilerpGradientsRGB
- * (RenderUtils.cu:2149-2254, 1074-1364). */
-void orc_render_bwd(const orc_grid *g, const orc_rays *r, const int32_t *sample_index,
-                    int64_t start, int64_t len, const double *fwd_r, const double *fwd_g,
-                    const double *fwd_b, const double *fwd_w, double mask_coef,
-                    float huber_delta, double *losses, double *grad) {
+ * (RenderUtils.cu:2149-2254, 1074-1364); bin_mask as in the forward. */
+static void render_bwd(const orc_grid *g, const orc_rays *r, const int32_t *sample_index,
+                       int64_t start, int64_t len, const double *fwd_r, const double *fwd_g,
+                       const double *fwd_b, const double *fwd_w, double mask_coef,
+                       float huber_delta, double *losses, double *grad,
+                       const uint64_t *bin_mask) {
     const double step = (double)g->step;
     const int S = g->S;
     const int64_t n = g->n_vox;
@@ -534,7 +575,7 @@ void orc_render_bwd(const orc_grid *g, const orc_rays *r, const int32_t *sample_
         while (t < tf && k < ORC_MAX_STEPS) {
             const double px = fma(t, dx, ox), py = fma(t, dy, oy), pz = fma(t, dz, oz);
             sample s;
-            if (!locate(g, px, py, pz, &s)) {
+            if (!locate(g, px, py, pz, &s) || !bin_allowed(bin_mask, b, k, step, tn, tf)) {
                 if (g->adaptive) t += step;
                 ++k;
                 if (!g->adaptive) t = fma((double)k, step, tn);
@@ -570,6 +611,23 @@ void orc_render_bwd(const orc_grid *g, const orc_rays *r, const int32_t *sample_
             else t = fma((double)k, step, tn);
         }
     }
+}
+
+void orc_render_bwd(const orc_grid *g, const orc_rays *r, const int32_t *sample_index,
+                    int64_t start, int64_t len, const double *fwd_r, const double *fwd_g,
+                    const double *fwd_b, const double *fwd_w, double mask_coef,
+                    float huber_delta, double *losses, double *grad) {
+    render_bwd(g, r, sample_index, start, len, fwd_r, fwd_g, fwd_b, fwd_w, mask_coef, huber_delta,
+               losses, grad, NULL);
+}
+
+void orc_render_bwd_masked(const orc_grid *g, const orc_rays *r, const int32_t *sample_index,
+                           int64_t start, int64_t len, const uint64_t *bin_mask,
+                           const double *fwd_r, const double *fwd_g, const double *fwd_b,
+                           const double *fwd_w, double mask_coef, float huber_delta,
+                           double *losses, double *grad) {
+    render_bwd(g, r, sample_index, start, len, fwd_r, fwd_g, fwd_b, fwd_w, mask_coef, huber_delta,
+               losses, grad, bin_mask);
 }
 
 /* ------------------------------------------------------------------ */

@@ -79,6 +79,19 @@ void orc_render_bwd(const orc_grid *g, const orc_rays *r, const int32_t *sample_
                     int64_t start, int64_t len, const double *fwd_r, const double *fwd_g,
                     const double *fwd_b, const double *fwd_w, double mask_coef,
                     float huber_delta, double *losses, double *grad);
+/* Proposal variants: steps in bins whose bit of bin_mask[b] is clear are
+ * skipped (bin k of a ray: floor(k step 64 / (t_far - t_near)), at most 63);
+ * the forward also returns, per batch entry, the bins holding a sample of
+ * weight above target_wmin.  Fixed marching rule only. */
+void orc_render_fwd_masked(const orc_grid *g, const orc_rays *r, const int32_t *sample_index,
+                           int64_t start, int64_t len, const uint64_t *bin_mask,
+                           double target_wmin, uint64_t *targets, double *out_r, double *out_g,
+                           double *out_b, double *out_w, double *out_density, uint32_t *trace);
+void orc_render_bwd_masked(const orc_grid *g, const orc_rays *r, const int32_t *sample_index,
+                           int64_t start, int64_t len, const uint64_t *bin_mask,
+                           const double *fwd_r, const double *fwd_g, const double *fwd_b,
+                           const double *fwd_w, double mask_coef, float huber_delta,
+                           double *losses, double *grad);
 
 /* addVarianceLossGPU (RenderUtils.cu:2469-2547), cells visited serially so
  * the reference's unsynchronised += becomes an exact sum.  Adds into grad. */
