@@ -51,6 +51,16 @@ def parse():
     ap.add_argument("--occupancy", type=float, default=None)
     ap.add_argument("--autotune", type=int, default=None,
                     help="1: the trainer times every batch order x schedule on the first batch")
+    ap.add_argument("--storage", choices=("f32", "bf16"), default="f32",
+                    help="plain256: grid records in FP32 or BF16 (FP32 master copy for Adam)")
+    ap.add_argument("--proposal", action="store_true",
+                    help="plain256: train with the learned proposal (pretrain untimed, then "
+                         "time pruned steps) and report held-out PSNR of pruned vs plain renders")
+    ap.add_argument("--pretrain", type=int, default=None,
+                    help="untimed training steps before warmup (default 2000 with --proposal, "
+                         "else 0)")
+    ap.add_argument("--prop-interval", type=int, default=8,
+                    help="every n-th timed batch runs unpruned and trains the proposal")
     ap.add_argument("--cpu-rays", type=int, default=1024,
                     help="rays of the bounded CPU-baseline sample (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -70,6 +80,8 @@ def parse():
     a.autotune = a.autotune if a.autotune is not None else (1 if uneven else 0)
     if uneven and a.log_samples == 512:
         a.log_samples = 1024
+    if a.pretrain is None:
+        a.pretrain = 2000 if a.proposal else 0
     return a
 
 
@@ -327,8 +339,14 @@ class Workload:
 
         # Trained grid: same caged structure, InitialDensity 0, InitialColor 0.
         self.rec = torch.zeros_like(gt_rec)
-        self.grid = pxl.Grid(N, N, N, S, self.rec, **grid_kw)
-        self.optim = pxl.Optim(self.grid)
+        if args.storage == "bf16":
+            zero32 = self.rec
+            self.rec = pxl.records_to_16bit(zero32, pxl.STORE_BF16)
+            self.grid = pxl.Grid(N, N, N, S, self.rec, storage=pxl.STORE_BF16, **grid_kw)
+            self.optim = pxl.Optim(self.grid, master_from=zero32)
+        else:
+            self.grid = pxl.Grid(N, N, N, S, self.rec, **grid_kw)
+            self.optim = pxl.Optim(self.grid)
         # config_linux.txt defaults: LearningRate 0.3, SHLearningRate 1,
         # VarianceCoefficient 1e-5, ColorDisparityCoefficient 1e-6,
         # VarianceType 1, VarianceLossInterval 3, MaskRayCoefficient 1,
@@ -427,6 +445,45 @@ def profiled_traffic(kernel, args):
     return None
 
 
+def proposal_quality(w, prop, dev, args):
+    """Held-out batch (the last of the shuffle, never trained on): sample
+    counts and PSNR against the targets of the plain and the pruned render
+    of the current grid."""
+    import torch
+
+    import pxl
+
+    B = args.batch
+    start = (w.n_rays // B - 1) * B
+    idx = w.order[start:start + B]
+    pws = pxl.proposal_workspace(B, dev)
+    pxl.proposal_features(w.grid, w.rays, w.order, pws, start=start, length=B)
+    logits = torch.zeros((B, 64), dtype=torch.float32, device=dev)
+    pxl.proposal_forward(prop, B, logits, pws)
+    masks = torch.zeros(B, dtype=torch.int64, device=dev)
+    pxl.proposal_masks(logits, B, masks, threshold=0.05, dilate=1)
+    gt = torch.stack([w.rays.gt_r[idx.long()], w.rays.gt_g[idx.long()],
+                      w.rays.gt_b[idx.long()]], 1).double() / 255.0
+    # Rays that miss the grid box are rendered black and carry no loss
+    # (the reference's convention); PSNR is over the rays that enter it.
+    hit = w.rays.near_t[idx.long()] <= w.rays.far_t[idx.long()]
+    res = {}
+    for name, m in (("plain", None), ("proposal", masks)):
+        tr = torch.zeros(2 * B, dtype=torch.int32, device=dev)
+        pxl.render_fwd_masked(w.grid, w.rays, w.order, m, start=start, length=B, trace=tr)
+        c = torch.stack([w.rays.acc_r[idx.long()], w.rays.acc_g[idx.long()],
+                         w.rays.acc_b[idx.long()]], 1).double()
+        torch.cuda.synchronize()
+        mse = float(((c - gt)[hit] ** 2).mean())
+        res[name] = {"samples": int(tr.view(-1, 2)[:, 0].sum().item()),
+                     "psnr": round(10.0 * math.log10(1.0 / mse), 4)}
+    res["pruned_frac"] = round(1.0 - res["proposal"]["samples"] / max(1, res["plain"]["samples"]), 4)
+    res["psnr_delta"] = round(res["proposal"]["psnr"] - res["plain"]["psnr"], 4)
+    res["interval"] = args.prop_interval
+    log(f"proposal on held-out batch: {res}")
+    return res
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -448,6 +505,26 @@ def main():
         del src, dst
     uneven = args.workload == "uneven512"
     w = UnevenWorkload(args, dev) if uneven else Workload(args, dev)
+    prop = None
+    if args.proposal:
+        # Pretrain with the proposal learning on every (unpruned) batch, then
+        # prune all but every prop_interval-th batch in the timed steps.
+        prop = pxl.Proposal(dev, seed=1)
+        w.trainer.set_proposal(prop, interval=1, prune=False)
+        t0 = time.time()
+        for i in range(args.pretrain):
+            w.step()
+            if (i + 1) % 500 == 0:
+                torch.cuda.synchronize()
+                mse, psnr = w.trainer.loss()
+                log(f"pretrain {i + 1}: batch PSNR {psnr:.2f}, proposal BCE "
+                    f"{w.trainer.proposal_loss() / (args.batch * 64):.4f} ({time.time() - t0:.1f}s)")
+        w.trainer.set_proposal(prop, interval=args.prop_interval, prune=True)
+    elif args.pretrain:
+        for _ in range(args.pretrain):
+            w.step()
+        torch.cuda.synchronize()
+        log(f"pretrained {args.pretrain} steps, batch PSNR {w.trainer.loss()[1]:.2f}")
     for _ in range(args.warmup):
         w.step()
     torch.cuda.synchronize()
@@ -483,7 +560,7 @@ def main():
     # moments: read + write).
     # 16-bit records: 2 P s_p + 8 P = 336 B per touched record and
     # P (2 s_p + 8 + 16) = 784 B per record for Adam.
-    s_p = 2 if uneven else 4
+    s_p = 2 if (uneven or args.storage == "bf16") else 4
     vt = touched_vertices(w, dev)
     alg = args.batch * 44 + vt * (2 * w.P * s_p + 8 * w.P)
     achieved = alg / (kern_ms / 1e3) / 1e9
@@ -498,7 +575,8 @@ def main():
             "step_frac": round(step_alg / (ms_step / 1e3) / 1e9 / HBM_PEAK_GBS, 5)}
 
     cpu = None
-    if not args.no_cpu_baseline and args.cpu_rays > 0 and not uneven:
+    if (not args.no_cpu_baseline and args.cpu_rays > 0 and not uneven and not args.proposal
+            and args.storage == "f32" and not args.pretrain):
         log("CPU baseline (oracle) ...")
         cpu = cpu_baseline(w, dev, args.cpu_rays)
     out = {
@@ -523,6 +601,12 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    if prop is not None:
+        out["metric"] = ("train rays/sec (fwd+bwd+TV+Adam) 256^3 SH2 with learned proposal"
+                         + (", BF16 grid" if args.storage == "bf16" else ""))
+        out["config"]["proposal"] = proposal_quality(w, prop, dev, args)
+    out["config"]["pretrain_steps"] = args.pretrain
+    out["config"]["storage"] = args.storage
     order, wpb, pers = w.trainer.schedule()
     out["config"]["schedule"] = {"batch_order": ["given", "morton", "length+morton"][order],
                                  "waves_per_block": wpb, "persistent": pers,

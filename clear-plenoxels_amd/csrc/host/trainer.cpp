@@ -76,6 +76,19 @@ struct pxl_trainer {
     int32_t order_mode;
     pxl_schedule sched;
     bool tuned;
+    // Learned proposal (caller-owned network; trainer-owned buffers).
+    bool prop_on;
+    pxl_proposal prop;
+    int32_t prop_interval, prop_dilate, prop_prune;
+    float prop_threshold, prop_pos_weight;
+    double prop_lr, prop_wmin;
+    int32_t prop_t;
+    void *prop_ws;
+    size_t prop_ws_bytes;
+    float *prop_logits;
+    uint64_t *prop_masks, *prop_targets;
+    uint8_t *prop_valid;
+    double *prop_loss;
 };
 
 using namespace pxl;
@@ -127,6 +140,12 @@ extern "C" pxl_trainer *pxl_trainer_create(const pxl_train_config *cfg, const px
     tr->sched.persistent = cfg->persistent;
     tr->sched.reserved[0] = tr->sched.reserved[1] = 0;
     tr->tuned = !cfg->autotune;
+    tr->prop_on = false;
+    tr->prop_ws = nullptr;
+    tr->prop_logits = nullptr;
+    tr->prop_masks = tr->prop_targets = nullptr;
+    tr->prop_valid = nullptr;
+    tr->prop_loss = nullptr;
     const bool logged = cfg->fused && cfg->log_samples > 0;
     if (!logged && (cfg->autotune || cfg->batch_order || cfg->persistent ||
                     (cfg->waves_per_block && cfg->waves_per_block != 4))) {
@@ -159,9 +178,12 @@ extern "C" pxl_trainer *pxl_trainer_create(const pxl_train_config *cfg, const px
     return tr;
 }
 
-// The render call of one batch under a given order and schedule.
+// The render call of one batch under a given order and schedule; with the
+// proposal on, every prop_interval-th batch (or every batch when not
+// pruning) runs unpruned, returns targets and trains the proposal, the
+// others run with the proposal's bin masks.
 static int32_t render_batch(pxl_trainer *tr, int64_t start, int64_t len, int32_t order,
-                            const pxl_schedule &sched) {
+                            const pxl_schedule &sched, bool use_prop = false) {
     const pxl_train_config &c = tr->cfg;
     const int32_t *si = tr->order;
     int64_t s0 = start;
@@ -173,9 +195,83 @@ static int32_t render_batch(pxl_trainer *tr, int64_t start, int64_t len, int32_t
         si = tr->order_buf;
         s0 = 0;
     }
-    return pxl_train_rays_sched(&tr->grid, &tr->optim, &tr->rays, tr->losses, si, s0, len,
-                                c.mask_coefficient, c.huber_delta, tr->march_ws,
-                                tr->march_ws_bytes, c.log_samples, &sched, tr->stream);
+    if (!use_prop || !tr->prop_on)
+        return pxl_train_rays_sched(&tr->grid, &tr->optim, &tr->rays, tr->losses, si, s0, len,
+                                    c.mask_coefficient, c.huber_delta, tr->march_ws,
+                                    tr->march_ws_bytes, c.log_samples, &sched, tr->stream);
+    int32_t st = pxl_proposal_features(&tr->grid, &tr->rays, si, s0, len, tr->prop_ws,
+                                       tr->prop_ws_bytes, tr->stream);
+    if (st != PXL_OK) return st;
+    const bool learn = !tr->prop_prune || tr->t % tr->prop_interval == 0;
+    if (learn) {
+        st = pxl_train_rays_prop(&tr->grid, &tr->optim, &tr->rays, tr->losses, si, s0, len,
+                                 c.mask_coefficient, c.huber_delta, tr->march_ws,
+                                 tr->march_ws_bytes, c.log_samples, &sched, nullptr,
+                                 tr->prop_targets, tr->prop_valid, tr->prop_wmin, tr->stream);
+        if (st != PXL_OK) return st;
+        tr->prop_t++;
+        return pxl_proposal_train_step(&tr->prop, len, tr->prop_targets, tr->prop_valid, len,
+                                       tr->prop_pos_weight, tr->prop_lr, tr->prop_t, tr->prop_ws,
+                                       tr->prop_ws_bytes, tr->prop_loss, tr->stream);
+    }
+    st = pxl_proposal_forward(&tr->prop, len, tr->prop_logits, tr->prop_ws, tr->prop_ws_bytes,
+                              tr->stream);
+    if (st == PXL_OK)
+        st = pxl_proposal_masks(tr->prop_logits, len, tr->prop_threshold, tr->prop_dilate,
+                                tr->prop_masks, tr->stream);
+    if (st != PXL_OK) return st;
+    return pxl_train_rays_prop(&tr->grid, &tr->optim, &tr->rays, tr->losses, si, s0, len,
+                               c.mask_coefficient, c.huber_delta, tr->march_ws,
+                               tr->march_ws_bytes, c.log_samples, &sched, tr->prop_masks,
+                               nullptr, nullptr, 0.0, tr->stream);
+}
+
+extern "C" int32_t pxl_trainer_set_proposal(pxl_trainer *tr, const pxl_proposal *p,
+                                            int32_t interval, int32_t prune, float threshold,
+                                            int32_t dilate, double lr, float pos_weight,
+                                            double target_wmin) {
+    clear_error();
+    PXL_REQUIRE(tr && p && p->weights && p->master && p->m && p->v && p->grad,
+                "pxl_trainer_set_proposal: null argument");
+    PXL_REQUIRE(tr->march_ws != nullptr,
+                "pxl_trainer_set_proposal: needs the logged step (fused, log_samples > 0)");
+    PXL_REQUIRE(interval >= 1 && dilate >= 0 && dilate < 64 && lr >= 0.0,
+                "pxl_trainer_set_proposal: bad interval / dilate / lr");
+    const int64_t B = tr->cfg.batch_size;
+    if (!tr->prop_ws) {
+        tr->prop_ws_bytes = pxl_proposal_workspace_bytes(B);
+        if (hipMalloc(&tr->prop_ws, tr->prop_ws_bytes) != hipSuccess ||
+            hipMalloc((void **)&tr->prop_logits, (size_t)B * 64 * sizeof(float)) != hipSuccess ||
+            hipMalloc((void **)&tr->prop_masks, (size_t)B * sizeof(uint64_t)) != hipSuccess ||
+            hipMalloc((void **)&tr->prop_targets, (size_t)B * sizeof(uint64_t)) != hipSuccess ||
+            hipMalloc((void **)&tr->prop_valid, (size_t)B) != hipSuccess ||
+            hipMalloc((void **)&tr->prop_loss, sizeof(double)) != hipSuccess) {
+            set_error("pxl_trainer_set_proposal: cannot allocate the proposal buffers");
+            return PXL_ENOMEM;
+        }
+    }
+    tr->prop = *p;
+    tr->prop_interval = interval;
+    tr->prop_prune = prune;
+    tr->prop_threshold = threshold;
+    tr->prop_dilate = dilate;
+    tr->prop_lr = lr;
+    tr->prop_pos_weight = pos_weight;
+    tr->prop_wmin = target_wmin;
+    tr->prop_t = 0;
+    tr->prop_on = true;
+    return PXL_OK;
+}
+
+extern "C" int32_t pxl_trainer_proposal_loss(pxl_trainer *tr, double *bce_sum) {
+    PXL_REQUIRE(tr && bce_sum && tr->prop_on, "pxl_trainer_proposal_loss: no proposal");
+    if (hipMemcpyAsync(bce_sum, tr->prop_loss, sizeof(double), hipMemcpyDeviceToHost,
+                       tr->stream) != hipSuccess ||
+        hipStreamSynchronize(tr->stream) != hipSuccess) {
+        set_error("pxl_trainer_proposal_loss: copy failed");
+        return PXL_EHIP;
+    }
+    return PXL_OK;
 }
 
 // Times every batch order x schedule twice on the first batch (HIP events
@@ -260,7 +356,7 @@ extern "C" int32_t pxl_trainer_step(pxl_trainer *tr) {
             if (st != PXL_OK) return st;
             if (timed) (void)hipEventRecord(tr->ev[tr->ev_used * 2], tr->stream);
         }
-        st = render_batch(tr, start, len, tr->order_mode, tr->sched);
+        st = render_batch(tr, start, len, tr->order_mode, tr->sched, true);
     } else if (c.fused) {
         st = pxl_train_rays(&tr->grid, &tr->optim, &tr->rays, tr->losses, tr->order, start, len,
                             c.mask_coefficient, c.huber_delta, tr->stream);
@@ -354,10 +450,13 @@ extern "C" int32_t pxl_trainer_state(const pxl_trainer *tr, int64_t *t, double *
 extern "C" void pxl_trainer_destroy(pxl_trainer *tr) {
     if (!tr) return;
     for (hipEvent_t e : tr->ev) (void)hipEventDestroy(e);
-    if (tr->march_ws || tr->order_buf || tr->order_ws) (void)hipStreamSynchronize(tr->stream);
+    (void)hipStreamSynchronize(tr->stream);
     if (tr->march_ws) (void)hipFree(tr->march_ws);
     if (tr->order_buf) (void)hipFree(tr->order_buf);
     if (tr->order_ws) (void)hipFree(tr->order_ws);
+    for (void *q : {tr->prop_ws, (void *)tr->prop_logits, (void *)tr->prop_masks,
+                    (void *)tr->prop_targets, (void *)tr->prop_valid, (void *)tr->prop_loss})
+        if (q) (void)hipFree(q);
     delete tr;
 }
 

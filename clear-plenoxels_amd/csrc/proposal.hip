@@ -183,16 +183,20 @@ __global__ void k_relu_back(const float *__restrict__ G, const uint16_t *__restr
     dH[i] = f2bf(H[i] != 0 && (H[i] & 0x8000u) == 0 ? G[i] : 0.0f);
 }
 
-// Column sums of a [rows][64] BF16 matrix into FP32 (bias gradients): one
-// block of 64 x 4 threads.
-__global__ void k_colsum(const uint16_t *__restrict__ D, int64_t rows, float *__restrict__ out) {
-    __shared__ float part[4][64];
+// Column sums of a [rows][64] BF16 matrix (bias gradients): block c sums
+// rows [c * chunk_rows, (c+1) * chunk_rows) into part[c][64]; k_sum_partials
+// adds the chunks.
+__global__ void k_colsum(const uint16_t *__restrict__ D, int64_t rows, int chunk_rows,
+                         float *__restrict__ part) {
+    __shared__ float red[4][64];
     const int j = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int64_t r0 = (int64_t)blockIdx.x * chunk_rows;
+    const int64_t r1 = min(rows, r0 + chunk_rows);
     float s = 0.0f;
-    for (int64_t r = w; r < rows; r += 4) s += bf2f(D[r * 64 + j]);
-    part[w][j] = s;
+    for (int64_t r = r0 + w; r < r1; r += 4) s += bf2f(D[r * 64 + j]);
+    red[w][j] = s;
     __syncthreads();
-    if (w == 0) out[j] = part[0][j] + part[1][j] + part[2][j] + part[3][j];
+    if (w == 0) part[(size_t)blockIdx.x * 64 + j] = red[0][j] + red[1][j] + red[2][j] + red[3][j];
 }
 
 // Sum of per-row losses (one block, FP64).
@@ -305,6 +309,13 @@ void gemm(bool ta, bool tb, const uint16_t *A, const uint16_t *B, float *C, int 
     if (!ta && !tb) hipLaunchKernelGGL((k_gemm_bf16<false, false>), grid, dim3(256), 0, st, A, B, C, M, K, kchunk);
     else if (!ta && tb) hipLaunchKernelGGL((k_gemm_bf16<false, true>), grid, dim3(256), 0, st, A, B, C, M, K, kchunk);
     else hipLaunchKernelGGL((k_gemm_bf16<true, false>), grid, dim3(256), 0, st, A, B, C, M, K, kchunk);
+}
+
+// Bias gradient: column sums of D in chunks of kChunkRows rows, then added.
+void colsum(const uint16_t *D, const PropLayout &L, float *part, float *out, hipStream_t st) {
+    hipLaunchKernelGGL(k_colsum, dim3((unsigned)L.chunks), dim3(256), 0, st, D, L.rows,
+                       kChunkRows, part);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, st, part, L.chunks, 64, out);
 }
 
 int32_t check_prop(const pxl_proposal *p, const char *who) {
@@ -423,20 +434,20 @@ extern "C" int32_t pxl_proposal_train_step(const pxl_proposal *p, int64_t len,
     gemm(true, false, H2, dZ, part, kW, M, L.chunks, s);
     hipLaunchKernelGGL(k_sum_partials, dim3(kW3 / 256), dim3(256), 0, s, part, L.chunks, kW3,
                        g + kW1 + kW2);
-    hipLaunchKernelGGL(k_colsum, dim3(1), dim3(256), 0, s, dZ, L.rows, g + kWeights + 2 * kW);
+    colsum(dZ, L, part, g + kWeights + 2 * kW, s);
     gemm(false, true, dZ, W3, C, M, kBins, 1, s);
     hipLaunchKernelGGL(k_relu_back, dim3(n64), dim3(256), 0, s, C, H2, L.rows * 64, dH);
     // Layer 2: dW2 = H1^T dH2, db2, G = dH2 W2^T.
     gemm(true, false, H1, dH, part, kW, M, L.chunks, s);
     hipLaunchKernelGGL(k_sum_partials, dim3(kW2 / 256), dim3(256), 0, s, part, L.chunks, kW2,
                        g + kW1);
-    hipLaunchKernelGGL(k_colsum, dim3(1), dim3(256), 0, s, dH, L.rows, g + kWeights + kW);
+    colsum(dH, L, part, g + kWeights + kW, s);
     gemm(false, true, dH, W2, C, M, kW, 1, s);
     hipLaunchKernelGGL(k_relu_back, dim3(n64), dim3(256), 0, s, C, H1, L.rows * 64, dZ);
     // Layer 1: dW1 = X^T dH1, db1.
     gemm(true, false, X, dZ, part, kIn, M, L.chunks, s);
     hipLaunchKernelGGL(k_sum_partials, dim3(kW1 / 256), dim3(256), 0, s, part, L.chunks, kW1, g);
-    hipLaunchKernelGGL(k_colsum, dim3(1), dim3(256), 0, s, dZ, L.rows, g + kWeights);
+    colsum(dZ, L, part, g + kWeights, s);
     if (loss_out)
         hipLaunchKernelGGL(k_sum_rows, dim3(1), dim3(256), 0, s, lrow, L.rows, loss_out);
     if (lr > 0.0) {

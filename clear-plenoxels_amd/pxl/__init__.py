@@ -37,6 +37,7 @@ EXPORTS = (
     "pxl_trainer_schedule", "pxl_proposal_workspace_bytes", "pxl_proposal_init",
     "pxl_proposal_features", "pxl_proposal_forward", "pxl_proposal_train_step",
     "pxl_proposal_masks", "pxl_render_fwd_masked", "pxl_train_rays_prop",
+    "pxl_trainer_set_proposal", "pxl_trainer_proposal_loss",
 )
 
 PROP_IN, PROP_WIDTH, PROP_BINS = 128, 64, 64
@@ -190,6 +191,9 @@ def _declare(L):
         "pxl_render_fwd_masked": ([G, R, _P, _I64, _I64, _P, _P, _P], _I32),
         "pxl_train_rays_prop": ([G, O, R, _P, _P, _I64, _I64, _F64, _F32, _P, _SZ, _I32,
                                  ctypes.POINTER(ScheduleC), _P, _P, _P, _F64, _P], _I32),
+        "pxl_trainer_set_proposal": ([_P, ctypes.POINTER(ProposalC), _I32, _I32, _F32, _I32,
+                                      _F64, _F32, _F64], _I32),
+        "pxl_trainer_proposal_loss": ([_P, ctypes.POINTER(_F64)], _I32),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -732,6 +736,22 @@ class Trainer:
         _check("pxl_trainer_state", lib().pxl_trainer_state(self._h, ctypes.byref(t),
                                                             ctypes.byref(lr), ctypes.byref(sl)))
         return t.value, lr.value, sl.value
+
+    def set_proposal(self, prop, interval=8, prune=True, threshold=0.05, dilate=1, lr=3e-3,
+                     pos_weight=4.0, target_wmin=1e-5):
+        """Run the learned proposal `prop` (a pxl.Proposal kept alive by the
+        caller) inside the batch loop."""
+        self._prop = prop
+        c = prop.c()
+        _check("pxl_trainer_set_proposal",
+               lib().pxl_trainer_set_proposal(self._h, ctypes.byref(c), interval, int(prune),
+                                              threshold, dilate, lr, pos_weight, target_wmin))
+
+    def proposal_loss(self):
+        v = ctypes.c_double()
+        _check("pxl_trainer_proposal_loss",
+               lib().pxl_trainer_proposal_loss(self._h, ctypes.byref(v)))
+        return v.value
 
     def schedule(self):
         """(batch order, waves per block, persistent) the trainer runs."""

@@ -494,6 +494,20 @@ int32_t pxl_trainer_loss(pxl_trainer *tr, double *mse, double *psnr);
 
 /* Steps taken and current learning rates. */
 int32_t pxl_trainer_state(const pxl_trainer *tr, int64_t *t, double *lr, double *sh_lr);
+/* Turns on the learned proposal for a trainer running the logged step.
+ * Every interval-th batch (every batch when prune == 0) runs unpruned,
+ * takes the bins whose samples carry weight > target_wmin as targets and
+ * makes one proposal training step (Adam lr, positives weighted
+ * pos_weight); the other batches evaluate only the bins the proposal allows
+ * (sigmoid > threshold, dilated by `dilate` bins).  The network in *p stays
+ * owned by the caller; the trainer allocates its own buffers.  No
+ * counterpart in the reference. */
+int32_t pxl_trainer_set_proposal(pxl_trainer *tr, const pxl_proposal *p, int32_t interval,
+                                 int32_t prune, float threshold, int32_t dilate, double lr,
+                                 float pos_weight, double target_wmin);
+/* The weighted BCE sum of the last proposal training batch (synchronises). */
+int32_t pxl_trainer_proposal_loss(pxl_trainer *tr, double *bce_sum);
+
 /* The batch order and launch schedule the trainer runs (after autotuning,
  * the chosen one); any pointer may be NULL. */
 int32_t pxl_trainer_schedule(const pxl_trainer *tr, int32_t *batch_order,
