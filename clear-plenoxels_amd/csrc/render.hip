@@ -26,7 +26,8 @@ namespace {
 
 // kLog: forward plus the sample log of the two-kernel fused step (group
 // marcher only).
-enum Mode { kFwd = 0, kBwd = 1, kFused = 2, kTrace = 3, kTouch = 4, kLog = 5 };
+// kImage: eval render of one camera image (renderImageGPU), adaptive rule.
+enum Mode { kFwd = 0, kBwd = 1, kFused = 2, kTrace = 3, kTouch = 4, kLog = 5, kImage = 6 };
 
 // Hard cap on marching iterations per ray.  Legitimate rays need at most a
 // few thousand (diagonal / step); the cap only bounds rays whose far
@@ -72,7 +73,54 @@ struct RenderArgs {
     uint64_t *bin_targets;
     uint8_t *target_valid;
     double target_wmin;
+    // Eval image (kImage): camera, size, grid box and the output planes
+    // (any may be null): RGB float [3][h*w], RGB bytes [3][h*w], mask and
+    // disparity [h*w].
+    pxl_camera cam;
+    pxl_bbox box;
+    int img_w, img_h;
+    float *img_rgb;
+    uint8_t *img_rgb8, *img_mask;
+    float *img_disp;
 };
+
+// Eval ray of pixel (row, col) (getBound, RenderUtils.cu:34-97): the pixel
+// centre through the camera, c2w x - centre, normalised, all FP64 with the
+// reference's summation order; near / far from the slabs of the integer box
+// (no clamp at 0).
+__device__ inline void eval_ray(const RenderArgs &a, int row, int col, double d[3],
+                                double &tmin, double &tmax) {
+    const double x[4] = {((double)col + 0.5 - a.cam.cx) / a.cam.fx,
+                         ((double)row + 0.5 - a.cam.cy) / a.cam.fy, 1.0, 1.0};
+    double sp[3];
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+        double v = 0.0;
+#pragma unroll
+        for (int c = 0; c < 4; c++) v += a.cam.c2w[r * 4 + c] * x[c];
+        sp[r] = v - a.cam.centre[r];
+    }
+    const double n = sqrt(sp[0] * sp[0] + sp[1] * sp[1] + sp[2] * sp[2]);
+    d[0] = sp[0] / n;
+    d[1] = sp[1] / n;
+    d[2] = sp[2] / n;
+    const double lo[3] = {(double)a.box.x_min, (double)a.box.y_min, (double)a.box.z_min};
+    const double hi[3] = {(double)a.box.x_max, (double)a.box.y_max, (double)a.box.z_max};
+    tmin = -1.0 / 0.0;
+    tmax = 1.0 / 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const double t1 = (hi[i] - a.cam.centre[i]) / d[i], t2 = (lo[i] - a.cam.centre[i]) / d[i];
+        const double mn = t1 < t2 ? t1 : t2, mx = t1 > t2 ? t1 : t2;
+        if (i == 0) {
+            tmin = mn;
+            tmax = mx;
+        } else {
+            tmin = tmin > mn ? tmin : mn;
+            tmax = tmax < mx ? tmax : mx;
+        }
+    }
+}
 
 // Proposal bin of step k: 64 equal bins of [t_near, t_far).
 __device__ __forceinline__ int bin_of(int k, double step, double tn, double tf) {
@@ -272,8 +320,8 @@ __global__ __launch_bounds__(256) void k_render(const RenderArgs a) {
     const int64_t b =
         (int64_t)xcd_slice_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
     if (b >= a.len) return;
-    const int ray = uniform_i(a.sample_index[a.start + b]);
-    if (ray < 0 || (int64_t)ray >= a.rays.n) {
+    const int ray = MODE == kImage ? (int)b : uniform_i(a.sample_index[a.start + b]);
+    if (MODE != kImage && (ray < 0 || (int64_t)ray >= a.rays.n)) {
         // Out-of-range sample index: no outputs, zero loss.
         if ((MODE == kBwd || MODE == kFused) && lane == 0) a.losses[b] = 0.0;
         if (MODE == kTrace && lane == 0) {
@@ -283,10 +331,35 @@ __global__ __launch_bounds__(256) void k_render(const RenderArgs a) {
         return;
     }
 
-    const double ox = a.rays.ox[ray], oy = a.rays.oy[ray], oz = a.rays.oz[ray];
-    const float fdx = a.rays.dx[ray], fdy = a.rays.dy[ray], fdz = a.rays.dz[ray];
-    const double dx = fdx, dy = fdy, dz = fdz;
-    const double tnear = a.rays.near_t[ray], tfar = a.rays.far_t[ray];
+    double ox, oy, oz, dx, dy, dz, tnear, tfar;
+    float fdx, fdy, fdz;
+    if (MODE == kImage) {
+        // Pixel b: row b / w, column b % h (the reference's pixelCol,
+        // RenderUtils.cu:1727; the same as b % w for square images).
+        double d[3];
+        eval_ray(a, (int)(b / a.img_w), (int)(b % a.img_h), d, tnear, tfar);
+        ox = a.cam.centre[0];
+        oy = a.cam.centre[1];
+        oz = a.cam.centre[2];
+        dx = d[0];
+        dy = d[1];
+        dz = d[2];
+        fdx = (float)d[0];
+        fdy = (float)d[1];
+        fdz = (float)d[2];
+    } else {
+        ox = a.rays.ox[ray];
+        oy = a.rays.oy[ray];
+        oz = a.rays.oz[ray];
+        fdx = a.rays.dx[ray];
+        fdy = a.rays.dy[ray];
+        fdz = a.rays.dz[ray];
+        dx = fdx;
+        dy = fdy;
+        dz = fdz;
+        tnear = a.rays.near_t[ray];
+        tfar = a.rays.far_t[ray];
+    }
     const double step = (double)a.step;
     const R rstep = (R)a.step;
 
@@ -298,12 +371,27 @@ __global__ __launch_bounds__(256) void k_render(const RenderArgs a) {
     else if (row == 3 && kk == 0) p = 0;
     const R ylane = row < 3 ? (R)sh_basis(kk, fdx, fdy, fdz) : (R)1;
 
-    const bool do_fwd = MODE == kFwd || MODE == kFused || MODE == kTrace || MODE == kTouch;
+    const bool do_fwd =
+        MODE == kFwd || MODE == kFused || MODE == kTrace || MODE == kTouch || MODE == kImage;
     const bool do_bwd = MODE == kBwd || MODE == kFused;
 
     // Forward totals (FP64).
-    double C0 = 0.0, C1 = 0.0, C2 = 0.0, A = 0.0, sigsum = 0.0;
+    double C0 = 0.0, C1 = 0.0, C2 = 0.0, A = 0.0, sigsum = 0.0, disp = 0.0;
 
+    if (MODE == kImage && tnear > tfar) {
+        // Missed the box: white bytes, float colour 1 on a white background
+        // (else 0), no mask or disparity (RenderUtils.cu:1753-1791).
+        if (lane == 0) {
+            const size_t n = (size_t)a.img_w * (size_t)a.img_h;
+            for (int c = 0; c < 3; c++) {
+                if (a.img_rgb) a.img_rgb[c * n + b] = a.white_bg ? 1.0f : 0.0f;
+                if (a.img_rgb8) a.img_rgb8[c * n + b] = 255;
+            }
+            if (a.img_mask) a.img_mask[b] = 0;
+            if (a.img_disp) a.img_disp[b] = 0.0f;
+        }
+        return;
+    }
     if (tnear > tfar) {
         if (MODE == kFwd || MODE == kFused || MODE == kTrace) {
             if (lane == 0) {
@@ -372,6 +460,7 @@ __global__ __launch_bounds__(256) void k_render(const RenderArgs a) {
             const R alpha = one_minus_exp_neg<R>(sigma * rstep);
             const double w = T * (double)alpha;
             A += w;
+            if (MODE == kImage) disp += w * t;
             C0 += w * (double)col0;
             C1 += w * (double)col1;
             C2 += w * (double)col2;
@@ -380,6 +469,22 @@ __global__ __launch_bounds__(256) void k_render(const RenderArgs a) {
             ++k;
             if (ADAPT) t += fabs(step - w);
             else t = fma((double)k, step, tnear);
+        }
+        if (MODE == kImage) {
+            // Bytes always over white, floats over white when white_bg, the
+            // mask A * 255 truncated (RenderUtils.cu:1565-1579).
+            if (lane == 0) {
+                const size_t n = (size_t)a.img_w * (size_t)a.img_h;
+                const double cc[3] = {C0, C1, C2};
+                for (int c = 0; c < 3; c++) {
+                    if (a.img_rgb8)
+                        a.img_rgb8[c * n + b] = (uint8_t)(int)((cc[c] + 1.0 - A) * 255.0 + 0.5);
+                    if (a.img_rgb) a.img_rgb[c * n + b] = (float)(cc[c] + (a.white_bg ? 1.0 - A : 0.0));
+                }
+                if (a.img_mask) a.img_mask[b] = (uint8_t)(int)(A * 255.0);
+                if (a.img_disp) a.img_disp[b] = (float)disp;
+            }
+            return;
         }
         if (a.white_bg) {
             C0 += 1.0 - A;
@@ -704,6 +809,64 @@ extern "C" int32_t pxl_render_fwd_masked(const pxl_grid *grid, const pxl_rays *r
                : dispatch<kFwd>(grid->sh_dim, grid->sparse, false, grid->storage, a,
                                 as_stream(stream));
     return st != PXL_OK ? st : check_launch("pxl_render_fwd_masked");
+}
+
+extern "C" int32_t pxl_render_image(const pxl_grid *grid, const pxl_camera *cam, int32_t width,
+                                    int32_t height, int32_t white_bg, float *rgb, uint8_t *rgb8,
+                                    uint8_t *mask, float *disparity, void *stream) {
+    clear_error();
+    int32_t st = check_grid(grid, "pxl_render_image");
+    if (st != PXL_OK) return st;
+    PXL_REQUIRE(cam != nullptr && width > 0 && height > 0, "pxl_render_image: bad camera / size");
+    PXL_REQUIRE(grid->storage == PXL_STORE_F32,
+                "pxl_render_image: the adaptive marching rule needs FP32 storage");
+    RenderArgs a;
+    memset(&a, 0, sizeof(a));
+    a.params = grid->params;
+    a.n_vox = grid->n_vox;
+    a.occ = grid->occupancy;
+    a.bricks = grid->sparse ? grid->brick_occupancy : nullptr;
+    a.nbx = (grid->W - 1 + 7) / 8;
+    a.nby = (grid->H - 1 + 7) / 8;
+    a.io = grid->index_offsets;
+    a.W = grid->W;
+    a.H = grid->H;
+    a.D = grid->D;
+    a.stride = grid->stride;
+    a.gstride = grid->stride;
+    a.step = grid->step;
+    a.white_bg = white_bg;
+    a.len = (int64_t)width * (int64_t)height;
+    a.cam = *cam;
+    a.box = grid->bbox;
+    a.img_w = width;
+    a.img_h = height;
+    a.img_rgb = rgb;
+    a.img_rgb8 = rgb8;
+    a.img_mask = mask;
+    a.img_disp = disparity;
+    const unsigned blocks = (unsigned)((a.len + 3) / 4);
+    const hipStream_t s = as_stream(stream);
+#define PXL_IMG(SV)                                                                           \
+    case SV:                                                                                  \
+        if (grid->sparse)                                                                     \
+            hipLaunchKernelGGL((k_render<SV, true, true, kImage, PXL_STORE_F32>), dim3(blocks),  \
+                               dim3(256), 0, s, a);                                           \
+        else                                                                                  \
+            hipLaunchKernelGGL((k_render<SV, false, true, kImage, PXL_STORE_F32>), dim3(blocks), \
+                               dim3(256), 0, s, a);                                           \
+        break;
+    switch (grid->sh_dim) {
+        PXL_IMG(1)
+        PXL_IMG(4)
+        PXL_IMG(9)
+        PXL_IMG(16)
+        default:
+            set_error("pxl_render_image: unsupported sh_dim %d", grid->sh_dim);
+            return PXL_EINVAL;
+    }
+#undef PXL_IMG
+    return check_launch("pxl_render_image");
 }
 
 extern "C" int32_t pxl_render_fwd_trace(const pxl_grid *grid, const pxl_rays *rays,

@@ -37,7 +37,7 @@ EXPORTS = (
     "pxl_trainer_schedule", "pxl_proposal_workspace_bytes", "pxl_proposal_init",
     "pxl_proposal_features", "pxl_proposal_forward", "pxl_proposal_train_step",
     "pxl_proposal_masks", "pxl_render_fwd_masked", "pxl_train_rays_prop",
-    "pxl_trainer_set_proposal", "pxl_trainer_proposal_loss",
+    "pxl_trainer_set_proposal", "pxl_trainer_proposal_loss", "pxl_render_image",
 )
 
 PROP_IN, PROP_WIDTH, PROP_BINS = 128, 64, 64
@@ -194,6 +194,8 @@ def _declare(L):
         "pxl_trainer_set_proposal": ([_P, ctypes.POINTER(ProposalC), _I32, _I32, _F32, _I32,
                                       _F64, _F32, _F64], _I32),
         "pxl_trainer_proposal_loss": ([_P, ctypes.POINTER(_F64)], _I32),
+        "pxl_render_image": ([G, ctypes.POINTER(Camera), _I32, _I32, _I32, _P, _P, _P, _P, _P],
+                             _I32),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -523,6 +525,24 @@ def proposal_masks(logits, length, masks, threshold=0.05, dilate=1, stream=None)
     _check("pxl_proposal_masks",
            lib().pxl_proposal_masks(_ptr(logits), length, threshold, dilate, _ptr(masks),
                                     _stream(stream)))
+
+
+def render_image(grid, cam, width, height, white_bg=True, stream=None, device="cuda"):
+    """Eval render of one image (RenderUtils::renderImage): dict of device
+    planes rgb [3, h*w] float32, rgb8 [3, h*w] uint8, mask, disparity."""
+    import torch
+
+    n = width * height
+    out = {"rgb": torch.zeros((3, n), dtype=torch.float32, device=device),
+           "rgb8": torch.zeros((3, n), dtype=torch.uint8, device=device),
+           "mask": torch.zeros(n, dtype=torch.uint8, device=device),
+           "disparity": torch.zeros(n, dtype=torch.float32, device=device)}
+    g = grid.c()
+    _check("pxl_render_image",
+           lib().pxl_render_image(ctypes.byref(g), ctypes.byref(cam), width, height,
+                                  int(white_bg), _ptr(out["rgb"]), _ptr(out["rgb8"]),
+                                  _ptr(out["mask"]), _ptr(out["disparity"]), _stream(stream)))
+    return out
 
 
 def render_fwd_masked(grid, rays, sample_index, bin_mask=None, start=0, length=None,

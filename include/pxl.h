@@ -401,6 +401,19 @@ int32_t pxl_proposal_train_step(const pxl_proposal *p, int64_t len, const uint64
 int32_t pxl_proposal_masks(const float *logits, int64_t len, float threshold, int32_t dilate,
                            uint64_t *masks, void *stream);
 
+/* Eval render of one width x height image from `cam`
+ * (RenderUtils::renderImage, RenderUtils.cu:1720-1818, RGB path): per pixel
+ * an FP64 camera ray (getBound, RenderUtils.cu:34-97; pixel row i / width,
+ * column i % height as the reference computes it), the adaptive marching
+ * rule t += |step - w|, and the outputs of volumeRender_RGB
+ * (RenderUtils.cu:1443-1580): rgb [3][h*w] float (white-composited when
+ * white_bg), rgb8 [3][h*w] bytes (always white-composited, rounded), mask
+ * [h*w] = A * 255 truncated, disparity [h*w] = sum w t.  Pixels whose ray
+ * misses the grid box are white.  Any output may be NULL.  FP32 grids. */
+int32_t pxl_render_image(const pxl_grid *grid, const pxl_camera *cam, int32_t width,
+                         int32_t height, int32_t white_bg, float *rgb, uint8_t *rgb8,
+                         uint8_t *mask, float *disparity, void *stream);
+
 /* pxl_render_fwd with a proposal mask: batch entry b evaluates step k only
  * when bit floor(k step 64 / (t_far - t_near)) (at most 63) of
  * bin_mask[b] is set (bin_mask NULL: every step, as pxl_render_fwd).  With

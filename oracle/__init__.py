@@ -88,6 +88,10 @@ def _declare(L):
                                         _I64, _P, _P, _P, _P, _P, ctypes.c_double,
                                         ctypes.c_float, _P, _P]
     L.orc_render_bwd_masked.restype = None
+    L.orc_render_image.argtypes = [ctypes.POINTER(OrcGrid), _P, _P, ctypes.c_double,
+                                   ctypes.c_double, ctypes.c_double, ctypes.c_double, _I32, _I32,
+                                   _I32, _P, _P, _P, _P, _P]
+    L.orc_render_image.restype = None
     L.orc_tv_disparity.argtypes = [ctypes.POINTER(OrcGrid), _F64, _F64, _I32, _P]
     L.orc_tv_disparity.restype = None
     L.orc_tv_neighbour6.argtypes = [ctypes.POINTER(OrcGrid), _F64, _F64, _P]
@@ -255,6 +259,23 @@ def render_fwd_masked(grid, rays, sample_index, bin_mask=None, target_wmin=None,
         out["trace"] = tr
     if tg is not None:
         out["targets"] = tg
+    return out
+
+
+def render_image(grid, c2w, centre, fx, fy, cx, cy, width, height, white_bg=True):
+    """Eval image (renderImageGPU): dict rgb [3, h*w] float64, rgb8 [3, h*w]
+    uint8, mask [h*w] uint8, disparity [h*w], rays [h*w, 5] (t_min, t_max,
+    direction)."""
+    n = width * height
+    out = {"rgb": np.zeros((3, n), np.float64), "rgb8": np.zeros((3, n), np.uint8),
+           "mask": np.zeros(n, np.uint8), "disparity": np.zeros(n, np.float64),
+           "rays": np.zeros((n, 5), np.float64)}
+    c2w = _c(np.asarray(c2w, np.float64).reshape(16), np.float64)
+    centre = _c(np.asarray(centre, np.float64).reshape(4), np.float64)
+    g = grid.c()
+    lib().orc_render_image(ctypes.byref(g), _p(c2w), _p(centre), fx, fy, cx, cy, width, height,
+                           int(white_bg), _p(out["rgb"]), _p(out["rgb8"]), _p(out["mask"]),
+                           _p(out["disparity"]), _p(out["rays"]))
     return out
 
 

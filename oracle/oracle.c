@@ -525,6 +525,88 @@ void orc_render_fwd_masked(const orc_grid *g, const orc_rays *r, const int32_t *
                bin_mask, target_wmin, targets);
 }
 
+/* renderImageGPU + getBound + volumeRender_RGB (RenderUtils.cu:1720-1810,
+ * 34-97, 1443-1580): eval image of one camera, adaptive marching rule.
+ * Outputs planar [3][h*w] (any may be NULL). */
+void orc_render_image(const orc_grid *g, const double c2w[16], const double centre[4],
+                      double fx, double fy, double cx, double cy, int32_t width, int32_t height,
+                      int32_t white_bg, double *rgb, uint8_t *rgb8, uint8_t *mask,
+                      double *disparity, double *rays_out) {
+    const double step = (double)g->step;
+    const size_t n = (size_t)width * (size_t)height;
+    for (size_t px = 0; px < n; px++) {
+        const int row = (int)(px / (size_t)width), col = (int)(px % (size_t)height);
+        const double x[4] = {(col + 0.5 - cx) / fx, (row + 0.5 - cy) / fy, 1.0, 1.0};
+        double sp[3];
+        for (int r = 0; r < 3; r++) {
+            double v = 0.0;
+            for (int c = 0; c < 4; c++) v += c2w[r * 4 + c] * x[c];
+            sp[r] = v - centre[r];
+        }
+        const double nn = sqrt(sp[0] * sp[0] + sp[1] * sp[1] + sp[2] * sp[2]);
+        const double d[3] = {sp[0] / nn, sp[1] / nn, sp[2] / nn};
+        const double lo[3] = {g->bbox[0], g->bbox[2], g->bbox[4]};
+        const double hi[3] = {g->bbox[1], g->bbox[3], g->bbox[5]};
+        double tmin = 0.0, tmax = 0.0;
+        for (int i = 0; i < 3; i++) {
+            const double t1 = (hi[i] - centre[i]) / d[i], t2 = (lo[i] - centre[i]) / d[i];
+            const double mn = t1 < t2 ? t1 : t2, mx = t1 > t2 ? t1 : t2;
+            if (i == 0) {
+                tmin = mn;
+                tmax = mx;
+            } else {
+                tmin = tmin > mn ? tmin : mn;
+                tmax = tmax < mx ? tmax : mx;
+            }
+        }
+        if (rays_out) {
+            rays_out[5 * px + 0] = tmin;
+            rays_out[5 * px + 1] = tmax;
+            for (int i = 0; i < 3; i++) rays_out[5 * px + 2 + i] = d[i];
+        }
+        if (tmin > tmax) {
+            for (int c = 0; c < 3; c++) {
+                if (rgb) rgb[c * n + px] = white_bg ? 1.0 : 0.0;
+                if (rgb8) rgb8[c * n + px] = 255;
+            }
+            if (mask) mask[px] = 0;
+            if (disparity) disparity[px] = 0.0;
+            continue;
+        }
+        double Y[16];
+        sh_basis((float)d[0], (float)d[1], (float)d[2], Y);
+        double C[3] = {0.0, 0.0, 0.0}, A = 0.0, disp = 0.0, lam = 0.0, t = tmin;
+        int k = 0;
+        while (t < tmax && k < ORC_MAX_STEPS) {
+            const double p0 = fma(t, d[0], centre[0]), p1 = fma(t, d[1], centre[1]),
+                         p2 = fma(t, d[2], centre[2]);
+            sample s;
+            if (!locate(g, p0, p1, p2, &s)) {
+                t += step;
+                ++k;
+                continue;
+            }
+            double sraw, craw[3];
+            interp(g, &s, Y, &sraw, craw);
+            const double sigma = softplus13(sraw);
+            const double T = exp(-lam);
+            const double w = T * (-expm1(-sigma * step));
+            A += w;
+            disp += w * t;
+            for (int ch = 0; ch < 3; ch++) C[ch] += w * sigmoid01(craw[ch]);
+            lam += sigma * step;
+            ++k;
+            t += fabs(step - w);
+        }
+        for (int c = 0; c < 3; c++) {
+            if (rgb8) rgb8[c * n + px] = (uint8_t)(int)((C[c] + 1.0 - A) * 255.0 + 0.5);
+            if (rgb) rgb[c * n + px] = C[c] + (white_bg ? 1.0 - A : 0.0);
+        }
+        if (mask) mask[px] = (uint8_t)(int)(A * 255.0);
+        if (disparity) disparity[px] = disp;
+    }
+}
+
 /* renderRays_withGradients_RGBGPU + trilerpGradientsRGB
  * (RenderUtils.cu:2149-2254, 1074-1364); bin_mask as in the forward. */
 static void render_bwd(const orc_grid *g, const orc_rays *r, const int32_t *sample_index,

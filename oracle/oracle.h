@@ -83,6 +83,13 @@ void orc_render_bwd(const orc_grid *g, const orc_rays *r, const int32_t *sample_
  * skipped (bin k of a ray: floor(k step 64 / (t_far - t_near)), at most 63);
  * the forward also returns, per batch entry, the bins holding a sample of
  * weight above target_wmin.  Fixed marching rule only. */
+/* Eval image of one camera (renderImageGPU): adaptive rule, planar
+ * outputs [3][h*w] / [h*w], any may be NULL; rays_out (optional) gets
+ * (t_min, t_max, dx, dy, dz) per pixel. */
+void orc_render_image(const orc_grid *g, const double c2w[16], const double centre[4],
+                      double fx, double fy, double cx, double cy, int32_t width, int32_t height,
+                      int32_t white_bg, double *rgb, uint8_t *rgb8, uint8_t *mask,
+                      double *disparity, double *rays_out);
 void orc_render_fwd_masked(const orc_grid *g, const orc_rays *r, const int32_t *sample_index,
                            int64_t start, int64_t len, const uint64_t *bin_mask,
                            double target_wmin, uint64_t *targets, double *out_r, double *out_g,

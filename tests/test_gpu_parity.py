@@ -651,3 +651,68 @@ def test_adam_bf16_master(dev):
     np.testing.assert_allclose(master, p_o, rtol=1e-6, atol=1e-7)
     stored = rec16[:, :P].T.contiguous().cpu().numpy().view(np.uint16)
     assert np.array_equal(stored, round_bf16(master))
+
+
+# ------------------------------------------------------------ eval image
+
+
+@pytest.mark.parametrize("kind", ["dense", "sparse"])
+def test_render_image(c1, sparse1, kind):
+    """pxl_render_image against the oracle's renderImageGPU restatement:
+    float colour within 1e-5, disparity (a weighted sum of depths up to ~220)
+    within 1e-6 relative + 1e-3, bytes and mask within the truncations of the
+    oracle's value -/+ 1e-5."""
+    import math
+
+    import torch
+
+    say(f"[image] eval render {kind}, 96x96")
+    dev = c1["dev"]
+    img = 96
+    f = img / 2.0 / math.tan(math.radians(20.0))
+    cam, _ = pxl.camera_from_pose((2.2, -3.1, 1.7), (N1 / 2.0,) * 3, 1.0, f, f, img / 2.0,
+                                  img / 2.0)
+    if kind == "dense":
+        grid = pxl.Grid(N1, N1, N1, S1, c1["rec"], step=0.5)
+        grid_o = oracle.Grid(N1, N1, N1, S1, c1["planes"], 0.5)
+    else:
+        s = sparse1
+        grid = device_sparse_grid(N1, S1, s["compact"], s["occ"], s["io"], s["cage"], s["c2d"],
+                                  dev)
+        grid_o = oracle.Grid(N1, N1, N1, S1, s["compact"], 0.5, occupancy=s["occ"],
+                             index_offsets=s["io"])
+    out = pxl.render_image(grid, cam, img, img, white_bg=True, device=dev)
+    torch.cuda.synchronize()
+    c2w = np.array(cam.c2w[:], np.float64)
+    centre = np.array(cam.centre[:], np.float64)
+    ref = oracle.render_image(grid_o, c2w, centre, f, f, img / 2.0, img / 2.0, img, img)
+    rgb = out["rgb"].cpu().numpy()
+    assert np.abs(rgb - ref["rgb"]).max() <= 1e-5
+    disp = out["disparity"].cpu().numpy()
+    assert np.all(np.abs(disp - ref["disparity"]) <= 1e-6 * np.abs(ref["disparity"]) + 1e-3)
+    # Bytes are truncations of 255 * value (+ 0.5 for colour): a value within
+    # the 1e-5 pixel bar of an integer step may land on either side, so each
+    # byte must lie between the truncations of the oracle value -/+ 1e-5.  The
+    # accumulated weight A is 1 - (white - black background float colour).
+    black = oracle.render_image(grid_o, c2w, centre, f, f, img / 2.0, img / 2.0, img, img,
+                                white_bg=False)
+    acc = 1.0 - (ref["rgb"][0] - black["rgb"][0])
+    tol = 1e-5
+
+    def byte_range(v, half):
+        lo = np.clip(np.floor((v - tol) * 255.0 + half), 0, 255)
+        hi = np.clip(np.floor((v + tol) * 255.0 + half), 0, 255)
+        return lo, hi
+
+    lo, hi = byte_range(acc, 0.0)
+    m = out["mask"].cpu().numpy().astype(np.float64)
+    assert np.all((m >= lo) & (m <= hi)), int(((m < lo) | (m > hi)).sum())
+    lo, hi = byte_range(ref["rgb"], 0.5)  # over white, as the bytes always are
+    b8 = out["rgb8"].cpu().numpy().astype(np.float64)
+    assert np.all((b8 >= lo) & (b8 <= hi)), int(((b8 < lo) | (b8 > hi)).sum())
+    lo, hi = byte_range(acc, 0.0)  # the derived weight reproduces the oracle's own mask
+    assert np.all((ref["mask"] >= lo) & (ref["mask"] <= hi))
+    hit = ref["rays"][:, 0] <= ref["rays"][:, 1]
+    say(f"    pixels hitting the grid {hit.mean():.3f}, max colour error "
+        f"{np.abs(rgb - ref['rgb']).max():.2e}")
+    assert hit.any() and (~hit).any()

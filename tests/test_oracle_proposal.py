@@ -145,3 +145,63 @@ def test_masked_march_constant_field(orc):
         tf = float(np.float32(far[i]))
         bins = {_bin(k, tf) for k in range(200) if k * 0.5 < tf}
         assert full["targets"][i] == sum(np.uint64(1) << np.uint64(b) for b in bins)
+
+
+# ------------------------------------------------------------ eval image
+
+
+def _eval_camera(orc, N, img):
+    f = img / 2.0 / math.tan(math.radians(20.0))
+    c = img / 2.0
+    c2w, _, centre = orc.camera_from_pose((2.2, -3.1, 1.7), (N / 2.0,) * 3, 1.0, f, f, c, c)
+    return c2w, centre, f, c
+
+
+def test_eval_rays_match_training_rays(orc):
+    """The eval ray of each pixel (getBound in FP64) is the training ray of
+    the same pixel (the FP32 replica of the training ray builder) up to FP32
+    rounding: same direction, near and far."""
+    N, img = 40, 48
+    planes = np.zeros((4, N ** 3), np.float32)
+    g = orc.Grid(N, N, N, 1, planes, 0.5)
+    c2w, centre, f, c = _eval_camera(orc, N, img)
+    ev = orc.render_image(g, c2w, centre, f, f, c, c, img, img)
+    tr = orc.rays_init(c2w.reshape(-1), centre, f, f, c, c, (0, N - 1, 0, N - 1, 0, N - 1),
+                       img, img)
+    d_tr = np.stack([tr["dx"], tr["dy"], tr["dz"]], 1).astype(np.float64)
+    d_ev = ev["rays"][:, 2:5]
+    # FP32 pixel coordinates, c2w product and centre subtraction in the
+    # training builder: ~1e-6 on the unit direction (a swapped row / column,
+    # sign or missing normalisation would be ~1e-1).
+    assert np.abs(d_ev - d_tr).max() < 1e-5
+    hit = ev["rays"][:, 0] <= ev["rays"][:, 1]
+    assert hit.any() and (~hit).any()
+    np.testing.assert_allclose(ev["rays"][hit, 0], tr["near_t"][hit], rtol=1e-5, atol=1e-3)
+    np.testing.assert_allclose(ev["rays"][hit, 1], tr["far_t"][hit], rtol=1e-5, atol=1e-3)
+
+
+def test_eval_image_output_conventions(orc):
+    """Zero density: white bytes, float white only on a white background,
+    mask 0; missed pixels white with zero disparity.  Opaque grid: mask
+    254-255, disparity = weight-averaged depth inside [near, far]."""
+    N, img = 40, 48
+    c2w, centre, f, c = _eval_camera(orc, N, img)
+    empty = orc.Grid(N, N, N, 1, np.full((4, N ** 3), -200.0, np.float32), 0.5)
+    for wb in (True, False):
+        out = orc.render_image(empty, c2w, centre, f, f, c, c, img, img, white_bg=wb)
+        assert np.all(out["rgb8"] == 255) and np.all(out["mask"] == 0)
+        np.testing.assert_allclose(out["rgb"], 1.0 if wb else 0.0, atol=1e-12)
+    planes = np.zeros((4, N ** 3), np.float32)
+    planes[0] = 60.0
+    planes[1] = 10.0
+    opaque = orc.Grid(N, N, N, 1, planes, 0.5)
+    out = orc.render_image(opaque, c2w, centre, f, f, c, c, img, img)
+    hit = out["rays"][:, 0] <= out["rays"][:, 1]
+    assert np.all(out["mask"][hit] >= 254)
+    assert np.all(out["disparity"][~hit] == 0.0)
+    dmean = out["disparity"][hit] / np.maximum(1e-12, out["mask"][hit] / 255.0)
+    assert np.all(dmean >= out["rays"][hit, 0] - 1e-6)
+    assert np.all(dmean <= out["rays"][hit, 0] + 1.0)
+    col = 1.0 / (1.0 + math.exp(-1.0))
+    np.testing.assert_allclose(out["rgb"][0, hit], col, atol=1e-6)
+    assert np.all(out["rgb8"][0, hit] == int(col * 255 + 0.5))
