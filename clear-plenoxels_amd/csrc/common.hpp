@@ -105,6 +105,16 @@ __device__ __forceinline__ double wave_sum_d(double v) {
     return v;
 }
 
+// One element of a record in the grid's storage type (PXL_STORE_*), widened
+// to float.
+template <int STORE>
+__device__ __forceinline__ float load_elem(const float *base, size_t idx) {
+    if (STORE == PXL_STORE_F32) return base[idx];
+    const uint16_t h = reinterpret_cast<const uint16_t *>(base)[idx];
+    if (STORE == PXL_STORE_BF16) return __uint_as_float((uint32_t)h << 16);
+    return __half2float(__ushort_as_half(h));
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs (block i -> XCD
 // i mod 8).  Give XCD x a contiguous slice of the launch instead, so work
 // items that are neighbours in launch order (rays of a sorted batch, records

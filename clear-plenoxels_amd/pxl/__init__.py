@@ -38,6 +38,7 @@ EXPORTS = (
     "pxl_proposal_features", "pxl_proposal_forward", "pxl_proposal_train_step",
     "pxl_proposal_masks", "pxl_render_fwd_masked", "pxl_train_rays_prop",
     "pxl_trainer_set_proposal", "pxl_trainer_proposal_loss", "pxl_render_image",
+    "pxl_upsample_density", "pxl_upsample",
 )
 
 PROP_IN, PROP_WIDTH, PROP_BINS = 128, 64, 64
@@ -196,6 +197,8 @@ def _declare(L):
         "pxl_trainer_proposal_loss": ([_P, ctypes.POINTER(_F64)], _I32),
         "pxl_render_image": ([G, ctypes.POINTER(Camera), _I32, _I32, _I32, _P, _P, _P, _P, _P],
                              _I32),
+        "pxl_upsample_density": ([G, _I32, _I32, _I32, _F32, _P, _P], _I32),
+        "pxl_upsample": ([G, G, _F32, _I32, _P], _I32),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -790,3 +793,68 @@ class Trainer:
             self.close()
         except Exception:
             pass
+
+
+def upsample_density(src, W2, H2, D2, threshold, stream=None):
+    """Thresholded density of every vertex of the W2 x H2 x D2 grid sampled
+    from `src` (upsampleScene into the dense temporary, SceneUtils.cu:540-546);
+    a float32 device tensor [W2 * H2 * D2]."""
+    import torch
+
+    out = torch.empty(W2 * H2 * D2, dtype=torch.float32, device=src.params.device)
+    g = src.c()
+    _check("pxl_upsample_density",
+           lib().pxl_upsample_density(ctypes.byref(g), W2, H2, D2, threshold, _ptr(out),
+                                      _stream(stream)))
+    return out
+
+
+def upsample(src, dst, threshold, low_band=False, stream=None):
+    """upsampleScene (SceneUtils.cu:419-485) into the FP32 records of `dst`
+    (a Grid, dense or caged)."""
+    gs, gd = src.c(), dst.c()
+    _check("pxl_upsample",
+           lib().pxl_upsample(ctypes.byref(gs), ctypes.byref(gd), threshold, int(low_band),
+                              _stream(stream)))
+
+
+def upsample_grid(src, threshold, to_sparse=True, low_band=False, sh_dim=None, stride=None,
+                  step=None, stream=None):
+    """objectScene::initSelfByUpsample (SceneUtils.cu:487-657): a new Grid of
+    twice the source's dimensions.  Caged (to_sparse): the thresholded density
+    is pruned with the same threshold (pxl_build_occupancy), indexed
+    (pxl_build_sparse_index), and the caged records filled by pxl_upsample;
+    the brick map is built as for any caged grid."""
+    import torch
+
+    dev = src.params.device
+    W2, H2, D2 = 2 * src.W, 2 * src.H, 2 * src.D
+    S2 = (1 if low_band else src.S) if sh_dim is None else int(sh_dim)
+    P2 = 1 + 3 * S2
+    stride = stride or ((P2 + 3) // 4 * 4 if P2 > 32 else 32)
+    step = src.step if step is None else step
+    if not to_sparse:
+        rec = torch.empty((W2 * H2 * D2, stride), dtype=torch.float32, device=dev)
+        dst = Grid(W2, H2, D2, S2, rec, step=step, march_mode=src.march_mode)
+        upsample(src, dst, threshold, low_band, stream)
+        return dst
+    dens = upsample_density(src, W2, H2, D2, threshold, stream)
+    ncell = (W2 - 1) * (H2 - 1) * (D2 - 1)
+    occ = torch.zeros((ncell + 31) // 32, dtype=torch.int32, device=dev)
+    cage = torch.zeros(W2 * H2 * D2, dtype=torch.uint8, device=dev)
+    build_occupancy(dens, W2, H2, D2, threshold, occ, cage, stream=stream)
+    del dens
+    io = torch.zeros(W2 * H2 * D2, dtype=torch.int32, device=dev)
+    c2d = torch.zeros(W2 * H2 * D2, dtype=torch.int32, device=dev)
+    cnt = torch.zeros(1, dtype=torch.int32, device=dev)
+    scratch = torch.zeros(sparse_index_scratch_bytes(W2 * H2 * D2) // 4 + 1, dtype=torch.int32,
+                          device=dev)
+    build_sparse_index(cage, W2, H2, D2, io, c2d, cnt, scratch, stream=stream)
+    n = int(cnt.item())
+    rec = torch.empty((n, stride), dtype=torch.float32, device=dev)
+    br = build_brick_occupancy(occ, W2, H2, D2, stream=stream)
+    dst = Grid(W2, H2, D2, S2, rec, step=step, march_mode=src.march_mode, occupancy=occ,
+               index_offsets=io, in_cage=cage, compact_to_dense=c2d[:max(n, 1)].clone(),
+               brick_occupancy=br)
+    upsample(src, dst, threshold, low_band, stream)
+    return dst

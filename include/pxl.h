@@ -262,6 +262,29 @@ int32_t pxl_grid_compact(const float *dense_records, int32_t stride,
                          const uint32_t *compact_to_dense, int64_t n_vox,
                          float *sparse_records, void *stream);
 
+/* Coarse-to-fine upsampling (objectScene::initSelfByUpsample,
+ * SceneUtils.cu:487-657, through upsampleScene + upsample_trilinear,
+ * SceneUtils.cu:250-485).  Destination vertex (x, y, z) samples the source
+ * at (x W / W2, y H / H2, z D / D2) in float, trilinearly, in the
+ * reference's corner order and float accumulation; when the source is
+ * caged, a sample in an unoccupied cell gives zeros and corners outside the
+ * cage read record 0.  Density below `threshold` becomes 0.
+ *
+ * pxl_upsample_density writes that density for every vertex of the
+ * W2 x H2 x D2 grid (the dense temporary the reference prunes, :540-546);
+ * pxl_build_occupancy(density, ..., threshold) and pxl_build_sparse_index
+ * then give the caged layout of the finer grid.
+ *
+ * pxl_upsample fills dst->params (FP32 records, stride <= 64), dense or
+ * caged (record r is vertex compact_to_dense[r]).  low_band_only carries
+ * only SH coefficient 0 of each channel (the reference's lowBoundOnly flag);
+ * dst->sh_dim must hold the carried coefficients; every other element of a
+ * written record, padding included, becomes 0 (initSelf's zero start). */
+int32_t pxl_upsample_density(const pxl_grid *src, int32_t W2, int32_t H2, int32_t D2,
+                             float threshold, float *density, void *stream);
+int32_t pxl_upsample(const pxl_grid *src, const pxl_grid *dst, float threshold,
+                     int32_t low_band_only, void *stream);
+
 /* Reference plane layout <-> packed records.  `planes` is a host array of
  * n_planes device pointers to n_vox floats each (objectScene plane table,
  * SceneUtils.cu:148-171).  Import zero-fills the padding floats. */

@@ -17,6 +17,14 @@ Parity status of each function (how it is pinned; see tests/test_oracle.py):
   tv_disparity      - closed forms (constant / affine field, single spike)
   tv_neighbour6     - closed forms (constant field, single spike)
   adam              - closed form at t = 1 and zero-gradient decay
+  render_fwd_masked - all-ones mask == plain march, one-bin masks on a constant
+                      field (closed form)
+  render_image      - eval rays == training rays; output conventions on empty
+                      and opaque grids (closed forms)
+  upsample_density,
+  upsample          - even vertices copy the source, affine fields reproduced
+                      away from the clamped upper faces, the clamp itself,
+                      unoccupied source cells, the threshold rule, band masks
 """
 import ctypes
 import os
@@ -98,6 +106,11 @@ def _declare(L):
     L.orc_tv_neighbour6.restype = None
     L.orc_adam.argtypes = [_P, _P, _P, _P, _P, _I32, _I64, _F64, _F64, _I32]
     L.orc_adam.restype = None
+    L.orc_upsample_density.argtypes = [ctypes.POINTER(OrcGrid), _I32, _I32, _I32, _F32, _P]
+    L.orc_upsample_density.restype = None
+    L.orc_upsample.argtypes = [ctypes.POINTER(OrcGrid), _I32, _I32, _I32, _I32, _P, _P, _I64,
+                               _F32, _I32, _P]
+    L.orc_upsample.restype = None
     L.orc_mse.argtypes = [_P, _I64]
     L.orc_mse.restype = _F64
     L.orc_now.argtypes = []
@@ -331,3 +344,27 @@ def mse(losses):
 
 def now():
     return lib().orc_now()
+
+
+def upsample_density(grid, W2, H2, D2, threshold):
+    """Thresholded density of every vertex of the W2 x H2 x D2 grid
+    (upsampleScene into the dense temporary, SceneUtils.cu:540-546)."""
+    out = np.zeros(W2 * H2 * D2, np.float32)
+    g = grid.c()
+    lib().orc_upsample_density(ctypes.byref(g), W2, H2, D2, threshold, _p(out))
+    return out
+
+
+def upsample(grid, W2, H2, D2, S2, threshold, low_band=False, in_cage=None, index_offsets=None,
+             n=None):
+    """upsampleScene (SceneUtils.cu:419-485): planes float32[1 + 3 S2, n] of
+    the destination grid, dense (n = W2 H2 D2) or caged (in_cage, index)."""
+    if in_cage is None:
+        n = W2 * H2 * D2
+    out = np.zeros((1 + 3 * S2, n), np.float32)
+    cage = None if in_cage is None else _c(in_cage, np.uint8)
+    io = None if index_offsets is None else _c(index_offsets, np.uint32)
+    g = grid.c()
+    lib().orc_upsample(ctypes.byref(g), W2, H2, D2, S2, _p(cage), _p(io), n, threshold,
+                       int(low_band), _p(out))
+    return out

@@ -800,6 +800,101 @@ void orc_adam(float *params, float *grad, float *m, float *v, const float *adapt
         }
 }
 
+/* ------------------------------------------------------------------ */
+/* Upsampling (SceneUtils.cu:250-485).                                  */
+/* ------------------------------------------------------------------ */
+
+/* getIndex (SceneUtils.cu:250-260): upper clamp only (unsigned inputs). */
+static uint32_t up_index(uint32_t x, uint32_t y, uint32_t z, int W, int H, int D) {
+    if (x > (uint32_t)(W - 1)) x = (uint32_t)(W - 1);
+    if (y > (uint32_t)(H - 1)) y = (uint32_t)(H - 1);
+    if (z > (uint32_t)(D - 1)) z = (uint32_t)(D - 1);
+    return z * (uint32_t)W * (uint32_t)H + y * (uint32_t)W + x;
+}
+
+/* upsample_trilinear (SceneUtils.cu:262-416) at destination vertex
+ * (x, y, z) of a W2 x H2 x D2 grid, sampling the source at
+ * x * 1.f * W / W2 (float operations, upsampleScene :428-433, :448).  Corners
+ * in the reference's order, each added to a float accumulator as
+ * (float)((double)acc + weight * value).  Fills tri[0 .. P_src) (bands
+ * beyond the first when low_band) and returns 0 when the source is caged and
+ * the cell holding the sample is unoccupied. */
+static int up_trilinear(const orc_grid *s, int x, int y, int z, int W2, int H2, int D2,
+                        int low_band, float *tri) {
+    const int S = s->S, P = 1 + 3 * S;
+    float bc[3];
+    bc[0] = (float)x * 1.f * (float)s->W / (float)W2;
+    bc[1] = (float)y * 1.f * (float)s->H / (float)H2;
+    bc[2] = (float)z * 1.f * (float)s->D / (float)D2;
+    const uint32_t b0 = (uint32_t)bc[0], b1 = (uint32_t)bc[1], b2 = (uint32_t)bc[2];
+    if (s->sparse) {
+        const uint32_t cell = up_index(b0, b1, b2, s->W - 1, s->H - 1, s->D - 1);
+        if (!s->occupancy[cell]) return 0;
+    }
+    const double fx = (double)(bc[0] - (float)b0);
+    const double fy = (double)(bc[1] - (float)b1);
+    const double fz = (double)(bc[2] - (float)b2);
+    for (int i = 0; i < P; i++) tri[i] = 0.f;
+    const int nsh = low_band ? 1 : S;
+    /* (dx, dy, dz) in the order the reference visits the corners */
+    static const int ord[8][3] = {{0, 0, 0}, {0, 1, 0}, {1, 0, 0}, {1, 1, 0},
+                                  {0, 0, 1}, {0, 1, 1}, {1, 0, 1}, {1, 1, 1}};
+    for (int c = 0; c < 8; c++) {
+        const int dx = ord[c][0], dy = ord[c][1], dz = ord[c][2];
+        uint32_t idx = up_index(b0 + (uint32_t)dx, b1 + (uint32_t)dy, b2 + (uint32_t)dz, s->W,
+                                s->H, s->D);
+        if (s->sparse) idx = s->index_offsets[idx];
+        const double w = (dx ? fx : 1. - fx) * (dy ? fy : 1. - fy) * (dz ? fz : 1. - fz);
+        const float *pl = s->planes;
+        const int64_t n = s->n_vox;
+        tri[0] = (float)((double)tri[0] + w * (double)pl[idx]);
+        for (int i = 0; i < nsh; i++)
+            for (int ch = 0; ch < 3; ch++) {
+                const int q = 1 + ch * S + i;
+                tri[q] = (float)((double)tri[q] + w * (double)pl[(int64_t)q * n + idx]);
+            }
+    }
+    return 1;
+}
+
+void orc_upsample_density(const orc_grid *src, int32_t W2, int32_t H2, int32_t D2,
+                          float threshold, float *density) {
+    float *tri = (float *)malloc(sizeof(float) * (size_t)(1 + 3 * src->S));
+    for (int z = 0; z < D2; z++)
+        for (int y = 0; y < H2; y++)
+            for (int x = 0; x < W2; x++) {
+                const size_t d = ((size_t)z * H2 + y) * W2 + x;
+                if (up_trilinear(src, x, y, z, W2, H2, D2, 1, tri))
+                    density[d] = tri[0] < threshold ? 0.f : tri[0];
+                else
+                    density[d] = 0.f;
+            }
+    free(tri);
+}
+
+void orc_upsample(const orc_grid *src, int32_t W2, int32_t H2, int32_t D2, int32_t S2,
+                  const uint8_t *dst_in_cage, const uint32_t *dst_index, int64_t dst_n,
+                  float threshold, int32_t low_band, float *dst) {
+    const int S = src->S, ncopy = low_band ? 1 : S;
+    float *tri = (float *)malloc(sizeof(float) * (size_t)(1 + 3 * S));
+    for (int z = 0; z < D2; z++)
+        for (int y = 0; y < H2; y++)
+            for (int x = 0; x < W2; x++) {
+                size_t r = ((size_t)z * H2 + y) * W2 + x;
+                if (dst_in_cage) {
+                    if (!dst_in_cage[r]) continue;
+                    r = dst_index[r];
+                }
+                const int val = up_trilinear(src, x, y, z, W2, H2, D2, low_band, tri);
+                dst[r] = val ? (tri[0] < threshold ? 0.f : tri[0]) : 0.f;
+                for (int i = 0; i < ncopy; i++)
+                    for (int ch = 0; ch < 3; ch++)
+                        dst[(size_t)(1 + ch * S2 + i) * (size_t)dst_n + r] =
+                            val ? tri[1 + ch * S + i] : 0.f;
+            }
+    free(tri);
+}
+
 double orc_mse(const double *losses, int64_t len) {
     double s = 0.0;
     for (int64_t i = 0; i < len; i++) s += losses[i] * losses[i];

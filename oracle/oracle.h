@@ -115,6 +115,17 @@ void orc_adam(float *params, float *grad, float *m, float *v, const float *adapt
               int32_t P, int64_t n, double lr, double sh_lr, int32_t t);
 
 /* getMSEError (Utils.cu:311-320). */
+/* upsampleScene + upsample_trilinear (SceneUtils.cu:262-485).
+ * orc_upsample_density: the thresholded density of every vertex of the
+ * W2 x H2 x D2 grid (the dense temporary initSelfByUpsample prunes,
+ * SceneUtils.cu:540-546).  orc_upsample: dst planes [1 + 3 S2][dst_n]
+ * (zero-initialised by the caller, as initSelf does); caged when
+ * dst_in_cage is given (records via dst_index). */
+void orc_upsample_density(const orc_grid *src, int32_t W2, int32_t H2, int32_t D2,
+                          float threshold, float *density);
+void orc_upsample(const orc_grid *src, int32_t W2, int32_t H2, int32_t D2, int32_t S2,
+                  const uint8_t *dst_in_cage, const uint32_t *dst_index, int64_t dst_n,
+                  float threshold, int32_t low_band, float *dst);
 double orc_mse(const double *losses, int64_t len);
 
 /* Wall-clock seconds (CLOCK_MONOTONIC), for the CPU baseline. */
