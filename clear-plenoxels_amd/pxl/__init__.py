@@ -38,7 +38,8 @@ EXPORTS = (
     "pxl_proposal_features", "pxl_proposal_forward", "pxl_proposal_train_step",
     "pxl_proposal_masks", "pxl_render_fwd_masked", "pxl_train_rays_prop",
     "pxl_trainer_set_proposal", "pxl_trainer_proposal_loss", "pxl_render_image",
-    "pxl_upsample_density", "pxl_upsample",
+    "pxl_upsample_density", "pxl_upsample", "pxl_rays_init_box", "pxl_mask_bounds",
+    "pxl_rays_permute", "pxl_shuffle_rank",
 )
 
 PROP_IN, PROP_WIDTH, PROP_BINS = 128, 64, 64
@@ -199,6 +200,11 @@ def _declare(L):
                              _I32),
         "pxl_upsample_density": ([G, _I32, _I32, _I32, _F32, _P, _P], _I32),
         "pxl_upsample": ([G, G, _F32, _I32, _P], _I32),
+        "pxl_rays_init_box": ([ctypes.POINTER(Camera), ctypes.POINTER(BBox), _I32, _I32, _P, _P,
+                               _P, _P, _P, _I64, R, _P], _I32),
+        "pxl_mask_bounds": ([_P, _I32, _I32, _P, _P], _I32),
+        "pxl_rays_permute": ([R, R, _P, _I64, _P], _I32),
+        "pxl_shuffle_rank": ([_P, _I32, _I32, _I32, _I32], _I32),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -373,6 +379,50 @@ def rays_init(cam, bbox, img_w, img_h, rays, start_index=0, img=None, stream=Non
     _check("pxl_rays_init",
            lib().pxl_rays_init(ctypes.byref(cam), ctypes.byref(BBox(*bbox)), img_w, img_h,
                                *ims, start_index, ctypes.byref(rc), _stream(stream)))
+
+
+def rays_init_box(cam, bbox, img_w, img_h, pixel_box, rays, start_index=0, img=None,
+                  stream=None):
+    """Mask-area rays of one view (MaskAreaOnly, Rays.cu:86-92): the pixels of
+    pixel_box = (row_min, row_max, col_min, col_max), inclusive, row-major."""
+    ims = [None] * 4 if img is None else [_ptr(x) for x in img]
+    box = (ctypes.c_int32 * 4)(*[int(v) for v in pixel_box])
+    rc = rays.c()
+    _check("pxl_rays_init_box",
+           lib().pxl_rays_init_box(ctypes.byref(cam), ctypes.byref(BBox(*bbox)), img_w, img_h,
+                                   box, *ims, start_index, ctypes.byref(rc), _stream(stream)))
+
+
+def mask_bounds(mask, img_w, img_h, stream=None):
+    """readMask's bounds of a device mask: int32 device tensor (row min, row
+    max, col min, col max)."""
+    import torch
+
+    out = torch.empty(4, dtype=torch.int32, device=mask.device)
+    _check("pxl_mask_bounds",
+           lib().pxl_mask_bounds(_ptr(mask), img_w, img_h, _ptr(out), _stream(stream)))
+    return out
+
+
+def rays_permute(src, dst, perm, n=None, stream=None):
+    """_rays::shuffleTo (Rays.cu:271-287): dst ray j = src ray perm[j]."""
+    n = perm.numel() if n is None else n
+    s, d = src.c(), dst.c()
+    _check("pxl_rays_permute",
+           lib().pxl_rays_permute(ctypes.byref(s), ctypes.byref(d), _ptr(perm), n,
+                                  _stream(stream)))
+
+
+def shuffle_rank(dest, max_index, regenerate=True, min_index=0, shuffle=True):
+    """ShuffleRank (Utils.cu:488-506) into a host int32 numpy array, drawing
+    from the C library's rand()."""
+    import numpy as np
+
+    assert dest.dtype == np.int32 and dest.flags["C_CONTIGUOUS"] and len(dest) >= max_index
+    _check("pxl_shuffle_rank",
+           lib().pxl_shuffle_rank(dest.ctypes.data_as(ctypes.c_void_p), max_index,
+                                  int(regenerate), min_index, int(shuffle)))
+    return dest
 
 
 def render_fwd(grid, rays, sample_index, start=0, length=None, trace=None, stream=None):

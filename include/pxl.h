@@ -169,6 +169,42 @@ int32_t pxl_rays_init(const pxl_camera *cam, const pxl_bbox *bbox,
                       const uint8_t *img_b, const uint8_t *img_mask,
                       int64_t start_index, const pxl_rays *rays, void *stream);
 
+/* Mask-area rays of one view (_rays::initialSelf with MaskAreaOnly,
+ * Rays.cu:187-240; initialRays :86-92; Image::processMasksInfo,
+ * ImageUtils.cu:130-141): the pixels of the box rows pixel_box[0] ..
+ * pixel_box[1], columns pixel_box[2] .. pixel_box[3] (inclusive, Image::bbox
+ * order) in row-major order become rays start_index, start_index + 1, ...
+ * Otherwise as pxl_rays_init. */
+int32_t pxl_rays_init_box(const pxl_camera *cam, const pxl_bbox *bbox,
+                          int32_t img_w, int32_t img_h, const int32_t pixel_box[4],
+                          const uint8_t *img_r, const uint8_t *img_g,
+                          const uint8_t *img_b, const uint8_t *img_mask,
+                          int64_t start_index, const pxl_rays *rays, void *stream);
+
+/* Bounding box of the nonzero pixels of a device mask (readMask,
+ * Utils.cu:508-548, from the start values of Image::initSelf,
+ * ImageUtils.cu:12-15): bounds (device int32[4]) = row min, row max, column
+ * min, column max.  An all-zero mask leaves (img_h - 1, 0, img_w - 1, 0),
+ * which processMasksInfo rejects (ImageUtils.cu:132). */
+int32_t pxl_mask_bounds(const uint8_t *mask, int32_t img_w, int32_t img_h,
+                        int32_t *bounds, void *stream);
+
+/* _rays::shuffleTo (Rays.cu:242-287): ray j of dst = ray perm[j] of src for
+ * j < n, geometry and ground truth (the acc_* planes are not copied).  An
+ * index outside [0, src->n) gives a zero ray. */
+int32_t pxl_rays_permute(const pxl_rays *src, const pxl_rays *dst, const int32_t *perm,
+                         int64_t n, void *stream);
+
+/* ShuffleRank (Utils.cu:488-506), host memory: with regenerate, dest[i] = i
+ * on [min_index, max_index); with shuffle, for i from max_index - 1 down to
+ * min_index + 1 swaps dest[i] and dest[rand() % i + min_index].  Draws from
+ * the C library's rand(): after the same srand() and rand() calls it gives
+ * the reference's order on the same C library.  The draw can reach
+ * min_index + max_index - 2, past the range, unless min_index <= 1: larger
+ * values are rejected when shuffling (the reference uses 0).  No GPU work. */
+int32_t pxl_shuffle_rank(int32_t *dest, int32_t max_index, int32_t regenerate,
+                         int32_t min_index, int32_t shuffle);
+
 /* Forward render of rays sample_index[start .. start+len)
  * (RenderUtils::renderRays, RenderUtils.cu:1919-1971). */
 int32_t pxl_render_fwd(const pxl_grid *grid, const pxl_rays *rays,
