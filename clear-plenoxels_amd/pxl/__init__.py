@@ -39,7 +39,8 @@ EXPORTS = (
     "pxl_proposal_masks", "pxl_render_fwd_masked", "pxl_train_rays_prop",
     "pxl_trainer_set_proposal", "pxl_trainer_proposal_loss", "pxl_render_image",
     "pxl_upsample_density", "pxl_upsample", "pxl_rays_init_box", "pxl_mask_bounds",
-    "pxl_rays_permute", "pxl_shuffle_rank",
+    "pxl_rays_permute", "pxl_shuffle_rank", "pxl_read_cameras", "pxl_read_image_text",
+    "pxl_grid_save_text", "pxl_grid_text_records", "pxl_grid_load_text",
 )
 
 PROP_IN, PROP_WIDTH, PROP_BINS = 128, 64, 64
@@ -205,6 +206,14 @@ def _declare(L):
         "pxl_mask_bounds": ([_P, _I32, _I32, _P, _P], _I32),
         "pxl_rays_permute": ([R, R, _P, _I64, _P], _I32),
         "pxl_shuffle_rank": ([_P, _I32, _I32, _I32, _I32], _I32),
+        "pxl_read_cameras": ([ctypes.c_char_p, _P, _I32, _I32, _I32, _F64, _I32, _I32,
+                              ctypes.POINTER(Camera), _I32, ctypes.POINTER(_I32)], _I32),
+        "pxl_read_image_text": ([ctypes.c_char_p, _I32, _I32, _I32, _P, _P, _P, _P, _P], _I32),
+        "pxl_grid_save_text": ([G, ctypes.c_char_p, _I32, _I32, _I32, _P], _I32),
+        "pxl_grid_text_records": ([ctypes.c_char_p, _I32, _I32, _I32, _I32, _I32,
+                                   ctypes.POINTER(_I64)], _I32),
+        "pxl_grid_load_text": ([ctypes.c_char_p, _I32, _I32, _I32, _I32, _I32, _I32, _I32, _I32,
+                                _I64, _P, _P, _P, _P, _P, _P], _I32),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -908,3 +917,77 @@ def upsample_grid(src, threshold, to_sparse=True, low_band=False, sh_dim=None, s
                brick_occupancy=br)
     upsample(src, dst, threshold, low_band, stream)
     return dst
+
+
+def read_cameras(path, object_centre, target_w, target_h, camera_distance, camera_nums=0,
+                 need_rescale=True, z_reversed=True):
+    """Camera::readFromMyBlender (CameraUtils.cu:271-386): list of Camera."""
+    oc = (ctypes.c_double * 3)(*object_centre)
+    n = _I32()
+    p = str(path).encode()
+    _check("pxl_read_cameras",
+           lib().pxl_read_cameras(p, oc, target_w, target_h, camera_nums, camera_distance,
+                                  int(need_rescale), int(z_reversed), None, 0, ctypes.byref(n)))
+    cams = (Camera * max(1, n.value))()
+    _check("pxl_read_cameras",
+           lib().pxl_read_cameras(p, oc, target_w, target_h, camera_nums, camera_distance,
+                                  int(need_rescale), int(z_reversed), cams, n.value,
+                                  ctypes.byref(n)))
+    return [cams[i] for i in range(n.value)]
+
+
+def read_image_text(prefix, width, height, gray_only=False):
+    """Image::initSelfFromFile + readMask: dict of host uint8 planes r, g, b,
+    mask ([height * width]) and the mask bounds."""
+    import numpy as np
+
+    n = width * height
+    planes = {k: np.zeros(n, np.uint8) for k in ("r", "g", "b", "mask")}
+    bounds = (ctypes.c_int32 * 4)()
+    ptr = {k: v.ctypes.data_as(ctypes.c_void_p) for k, v in planes.items()}
+    _check("pxl_read_image_text",
+           lib().pxl_read_image_text(str(prefix).encode(), width, height, int(gray_only),
+                                     ptr["r"], ptr["g"], ptr["b"], ptr["mask"], bounds))
+    planes["bounds"] = list(bounds)
+    return planes
+
+
+def grid_save_text(grid, directory, band, density_only=False, hierarchy_mode=False,
+                   stream=None):
+    """objectScene::saveSelf (SceneUtils.cu:1246-1299)."""
+    g = grid.c()
+    _check("pxl_grid_save_text",
+           lib().pxl_grid_save_text(ctypes.byref(g), str(directory).encode(), band,
+                                    int(density_only), int(hierarchy_mode), _stream(stream)))
+
+
+def grid_load_text(directory, W, H, D, band, sparse, sh_dim=None, have_occupancy=True,
+                   step=0.5, device="cuda", stream=None):
+    """objectScene::initFromFile (SceneUtils.cu:703-790): a new FP32 Grid."""
+    import torch
+
+    S = (band + 1) ** 2 if sh_dim is None else int(sh_dim)
+    stride = record_stride(S)
+    d = str(directory).encode()
+    n = _I64()
+    _check("pxl_grid_text_records",
+           lib().pxl_grid_text_records(d, band, int(sparse), W, H, D, ctypes.byref(n)))
+    nrec = n.value
+    ncell = (W - 1) * (H - 1) * (D - 1)
+    rec = torch.zeros((nrec, stride), dtype=torch.float32, device=device)
+    occ = (torch.zeros((ncell + 31) // 32, dtype=torch.int32, device=device)
+           if have_occupancy else None)
+    cage = io = c2d = None
+    if sparse:
+        cage = torch.zeros(W * H * D, dtype=torch.uint8, device=device)
+        io = torch.zeros(W * H * D, dtype=torch.int32, device=device)
+        c2d = torch.zeros(max(1, nrec), dtype=torch.int32, device=device)
+    _check("pxl_grid_load_text",
+           lib().pxl_grid_load_text(d, band, int(sparse), int(have_occupancy), W, H, D, S, stride,
+                                    nrec, _ptr(rec), _ptr(occ), _ptr(cage), _ptr(io), _ptr(c2d),
+                                    _stream(stream)))
+    if not sparse:
+        return Grid(W, H, D, S, rec, step=step)
+    br = build_brick_occupancy(occ, W, H, D, stream=stream)
+    return Grid(W, H, D, S, rec, step=step, occupancy=occ, index_offsets=io, in_cage=cage,
+                compact_to_dense=c2d, brick_occupancy=br)

@@ -321,6 +321,61 @@ int32_t pxl_upsample_density(const pxl_grid *src, int32_t W2, int32_t H2, int32_
 int32_t pxl_upsample(const pxl_grid *src, const pxl_grid *dst, float threshold,
                      int32_t low_band_only, void *stream);
 
+/* Text formats (host code; grid functions copy to or from the device and
+ * synchronise `stream`).
+ *
+ * pxl_read_cameras: Camera::readFromMyBlender (CameraUtils.cu:271-386) -
+ * "total N", the 3x3 intrinsics (rescaled when the principal point is not
+ * the centre of target_w x target_h), then per view "i:", the translation
+ * and a 4x4 matrix the reference reads and ignores; each view becomes
+ * pxl_camera_from_pose(translation, object_centre, camera_distance,
+ * need_rescale, z_reversed, fx, fy, cx, cy).  camera_nums != 0 overrides N.
+ * With out == NULL only *count is set.
+ *
+ * pxl_read_image_text: Image::initSelfFromFile + readMask
+ * (ImageUtils.cu:84-128, Utils.cu:508-572): host planes from
+ * prefix_mask.txt and prefix_red/_green/_blue.txt (prefix.txt into r when
+ * gray_only), integers per pixel in row-major order; bounds = row min, row
+ * max, column min, column max of the nonzero mask pixels, from
+ * (height - 1, 0, width - 1, 0).
+ *
+ * pxl_grid_save_text: objectScene::saveSelf (SceneUtils.cu:1246-1299) into
+ * an existing directory: occupancy.txt, opacity_band<b>.txt or the caged
+ * sparse_opacity_band<b>.txt, sparse_inCageInfo_band<b>.txt and
+ * sparse_indexOffset.txt, and the SH files <colour>_<i>[_sparse].txt for
+ * i < (band + 1)^2 (from band^2 with hierarchy_mode; none with
+ * density_only); "%.6lf " per float, "%d " per integer, a row per depth
+ * slice (dense) or per record (caged).  The caged record count goes to
+ * InCageNumsInfo.txt; the reference writes it over the in-cage file
+ * instead (SceneUtils.cu:1279-1283).
+ *
+ * pxl_grid_text_records: records of a saved grid (W*H*D dense; caged: the
+ * count of InCageNumsInfo.txt, else the first number of the in-cage file).
+ *
+ * pxl_grid_load_text: objectScene::initFromFile (SceneUtils.cu:703-790)
+ * into caller-allocated device buffers: FP32 records [n][stride] (elements
+ * without a file set to 0), occupancy bits (have_occupancy), and for a
+ * caged grid in_cage, index_offsets and compact_to_dense.  In-cage flags
+ * come from their file, or, when it holds only the count (a directory the
+ * reference saved), from occupancy.txt by the pruning rule; the index file
+ * must agree with them. */
+int32_t pxl_read_cameras(const char *path, const double object_centre[3], int32_t target_w,
+                         int32_t target_h, int32_t camera_nums, double camera_distance,
+                         int32_t need_rescale, int32_t z_reversed, pxl_camera *out,
+                         int32_t capacity, int32_t *count);
+int32_t pxl_read_image_text(const char *prefix, int32_t width, int32_t height, int32_t gray_only,
+                            uint8_t *r, uint8_t *g, uint8_t *b, uint8_t *mask,
+                            int32_t bounds[4]);
+int32_t pxl_grid_save_text(const pxl_grid *g, const char *dir, int32_t band,
+                           int32_t density_only, int32_t hierarchy_mode, void *stream);
+int32_t pxl_grid_text_records(const char *dir, int32_t band, int32_t sparse, int32_t W,
+                              int32_t H, int32_t D, int64_t *n_records);
+int32_t pxl_grid_load_text(const char *dir, int32_t band, int32_t sparse, int32_t have_occupancy,
+                           int32_t W, int32_t H, int32_t D, int32_t sh_dim, int32_t stride,
+                           int64_t n_records, float *params, uint32_t *occupancy,
+                           uint8_t *in_cage, uint32_t *index_offsets,
+                           uint32_t *compact_to_dense, void *stream);
+
 /* Reference plane layout <-> packed records.  `planes` is a host array of
  * n_planes device pointers to n_vox floats each (objectScene plane table,
  * SceneUtils.cu:148-171).  Import zero-fills the padding floats. */
