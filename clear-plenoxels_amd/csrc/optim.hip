@@ -43,44 +43,24 @@ __device__ __forceinline__ uint32_t record_of(const TvArgs &a, int x, int y, int
 // mean mu of its 8 corners and adds lambda * (u_corner - mu) to each
 // corner's gradient with unsynchronised +=.  Here each record gathers the
 // same terms from the (up to 8) cells that have its vertex as a corner:
-// one group of LPV lanes per record, one lane per parameter.  Lane i < 27 of
-// the group looks up the record of neighbour i and lane c < 8 tests cell c,
-// so the index and occupancy gathers are one load each; a neighbour value is
-// loaded only when an occupied cell around the vertex contains it.
+// one group of LPV lanes per record, one lane per parameter; a neighbour
+// value is loaded only when an occupied cell around the vertex contains it.
+//
+// Gradient of one record from its 27 neighbour records (lane i of the group
+// holds neighbour i's record in nrec) and the mask of occupied cells around
+// it (bit c = cell (x - (c&1), y - (c>>1&1), z - (c>>2))).
 template <int LPV, int STORE>
-__global__ __launch_bounds__(256) void k_tv_disparity(const TvArgs a) {
-    constexpr int kGroups = 256 / LPV;
-    const int g = threadIdx.x / LPV, p = threadIdx.x % LPV;
-    const int64_t r = (int64_t)xcd_slice_block(blockIdx.x, gridDim.x) * kGroups + g;
-    if (r >= a.n_vox) return;  // the whole group leaves together
-    const uint32_t d = a.sparse ? a.c2d[r] : (uint32_t)r;
-    const int x = (int)(d % (uint32_t)a.W);
-    const int y = (int)((d / (uint32_t)a.W) % (uint32_t)a.H);
-    const int z = (int)(d / ((uint32_t)a.W * (uint32_t)a.H));
-
-    // Neighbour i = (dz+1)*9 + (dy+1)*3 + (dx+1), looked up by lane i.
-    uint32_t nrec = 0u;
-    if (p < 27) {
-        const int nx = x + p % 3 - 1, ny = y + (p / 3) % 3 - 1, nz = z + p / 9 - 1;
-        if (nx >= 0 && nx < a.W && ny >= 0 && ny < a.H && nz >= 0 && nz < a.D)
-            nrec = record_of(a, nx, ny, nz);
-    }
-    // Cell c = (x - (c&1), y - (c>>1&1), z - (c>>2)), tested by lane c.
-    bool cocc = false;
-    if (p < 8) {
-        const int cx = x - (p & 1), cy = y - ((p >> 1) & 1), cz = z - (p >> 2);
-        cocc = cx >= 0 && cx <= a.W - 2 && cy >= 0 && cy <= a.H - 2 && cz >= 0 &&
-               cz <= a.D - 2 && (!a.sparse || cell_occupied(a, cx, cy, cz));
-    }
-    const int seg = (int)(threadIdx.x & 63) & ~(LPV - 1);
-    const uint32_t cmask = (uint32_t)(__ballot(cocc) >> seg) & 0xFFu;
-    if (cmask == 0u) return;  // no occupied cell around this vertex
-
+__device__ __forceinline__ void tv_record(const TvArgs &a, int64_t r, int p, uint32_t nrec,
+                                          uint32_t cmask) {
     const float lam = p == 0 ? a.lam_d : a.lam_sh;
     const bool mine = p < a.P && lam != 0.0f;
-    // Sums run in FP64: a vertex value and the mean of a cell around it
-    // nearly cancel.
-    double u[27];
+    // Every lane issues all 27 loads unconditionally, so they are in flight
+    // together (a load under a branch is waited for at the branch's end).
+    // A neighbour outside the occupied cells reads the record itself, and a
+    // lane without a parameter reads element 0: both lines are read anyway.
+    const int q = mine ? p : 0;
+    float v[27];
+    uint32_t need = 0u;
 #pragma unroll
     for (int i = 0; i < 27; i++) {
         const int dx = i % 3 - 1, dy = (i / 3) % 3 - 1, dz = i / 9 - 1;
@@ -94,12 +74,16 @@ __global__ __launch_bounds__(256) void k_tv_disparity(const TvArgs a) {
                 cells |= 1u << c;
         }
         const uint32_t ri = (uint32_t)__shfl((int)nrec, i, LPV);
-        float val = 0.0f;
-        if (mine && (cells & cmask) != 0u)
-            val = ld<STORE>(a.params, (size_t)ri * (size_t)a.stride + p);
-        u[i] = (double)val;
+        const bool used = (cells & cmask) != 0u;
+        need |= used ? 1u << i : 0u;
+        v[i] = ld<STORE>(a.params, (size_t)(used ? ri : (uint32_t)r) * (size_t)a.stride + q);
     }
     if (!mine) return;
+    // Sums run in FP64: a vertex value and the mean of a cell around it
+    // nearly cancel.
+    double u[27];
+#pragma unroll
+    for (int i = 0; i < 27; i++) u[i] = ((need >> i) & 1u) ? (double)v[i] : 0.0;
     const double self = u[13];
     double tsum = 0.0;
 #pragma unroll
@@ -117,6 +101,80 @@ __global__ __launch_bounds__(256) void k_tv_disparity(const TvArgs a) {
     a.grad[(size_t)r * (size_t)a.gstride + p] += (float)((double)lam * tsum);
 }
 
+// Neighbour records and cell mask of record r, computed by its group: lane
+// i < 27 looks up neighbour i = (dz+1)*9 + (dy+1)*3 + (dx+1), lane c < 8
+// tests cell c.
+template <int LPV>
+__device__ __forceinline__ uint32_t tv_neighbours(const TvArgs &a, int64_t r, int p,
+                                                  uint32_t &cmask) {
+    const uint32_t d = a.sparse ? a.c2d[r] : (uint32_t)r;
+    const int x = (int)(d % (uint32_t)a.W);
+    const int y = (int)((d / (uint32_t)a.W) % (uint32_t)a.H);
+    const int z = (int)(d / ((uint32_t)a.W * (uint32_t)a.H));
+    uint32_t nrec = 0u;
+    if (p < 27) {
+        const int nx = x + p % 3 - 1, ny = y + (p / 3) % 3 - 1, nz = z + p / 9 - 1;
+        if (nx >= 0 && nx < a.W && ny >= 0 && ny < a.H && nz >= 0 && nz < a.D)
+            nrec = record_of(a, nx, ny, nz);
+    }
+    bool cocc = false;
+    if (p < 8) {
+        const int cx = x - (p & 1), cy = y - ((p >> 1) & 1), cz = z - (p >> 2);
+        cocc = cx >= 0 && cx <= a.W - 2 && cy >= 0 && cy <= a.H - 2 && cz >= 0 &&
+               cz <= a.D - 2 && (!a.sparse || cell_occupied(a, cx, cy, cz));
+    }
+    const int seg = (int)(threadIdx.x & 63) & ~(LPV - 1);
+    cmask = (uint32_t)(__ballot(cocc) >> seg) & 0xFFu;
+    return nrec;
+}
+
+// addVarianceLossGPU as a gather, looking the neighbours up on the fly
+// (dense grids, where a record's neighbours are arithmetic).
+template <int LPV, int STORE>
+__global__ __launch_bounds__(256) void k_tv_disparity(const TvArgs a) {
+    constexpr int kGroups = 256 / LPV;
+    const int g = threadIdx.x / LPV, p = threadIdx.x % LPV;
+    const int64_t r = (int64_t)xcd_slice_block(blockIdx.x, gridDim.x) * kGroups + g;
+    if (r >= a.n_vox) return;  // the whole group leaves together
+    uint32_t cmask;
+    const uint32_t nrec = tv_neighbours<LPV>(a, r, p, cmask);
+    if (cmask == 0u) return;  // no occupied cell around this vertex
+    tv_record<LPV, STORE>(a, r, p, nrec, cmask);
+}
+
+// Caged grids: the neighbour records and the cell mask depend only on the
+// cage, so pxl_tv_prepare stores them, 32 words per record (27 records, the
+// mask, 4 zero words), and each TV call reads them with one coalesced load
+// instead of the compact-to-dense, index and occupancy lookups.
+constexpr int kTableWords = 32;
+
+template <int LPV>
+__global__ __launch_bounds__(256) void k_tv_table_build(const TvArgs a, uint32_t *table) {
+    constexpr int kGroups = 256 / LPV;
+    const int g = threadIdx.x / LPV, p = threadIdx.x % LPV;
+    const int64_t r = (int64_t)blockIdx.x * kGroups + g;
+    if (r >= a.n_vox) return;
+    uint32_t cmask;
+    const uint32_t nrec = tv_neighbours<LPV>(a, r, p, cmask);
+    if (p < kTableWords) table[(size_t)r * kTableWords + p] = p < 27 ? nrec : (p == 27 ? cmask : 0u);
+}
+
+template <int LPV, int STORE>
+__global__ __launch_bounds__(256) void k_tv_disparity_table(const TvArgs a,
+                                                            const uint32_t *__restrict__ table) {
+    constexpr int kGroups = 256 / LPV;
+    const int g = threadIdx.x / LPV, p = threadIdx.x % LPV;
+    const int64_t r = (int64_t)xcd_slice_block(blockIdx.x, gridDim.x) * kGroups + g;
+    if (r >= a.n_vox) return;
+    uint32_t e = p < kTableWords ? table[(size_t)r * kTableWords + p] : 0u;
+    const uint32_t cmask = (uint32_t)__shfl((int)e, 27, LPV) & 0xFFu;
+    if (cmask == 0u) return;
+    // A record index beyond the grid (scratch not prepared for this cage)
+    // reads record 0 instead of faulting.
+    if ((int64_t)e >= a.n_vox) e = 0u;
+    tv_record<LPV, STORE>(a, r, p, e, cmask);
+}
+
 // Caged grids: a cell corner that is not in cage reads and writes record 0
 // (index_offsets = 0, SceneUtils.cu:1106-1113).  Those (cell, corner) pairs
 // are not found by the gather above.  Their sum into record 0 is
@@ -131,12 +189,15 @@ struct AliasHeader {
     int64_t n_vox;
 };
 constexpr uint32_t kAliasMagic = 0x70786131u;
-constexpr int kAliasBlocks = 256;
+constexpr int kAliasBlocks = 1024;
 constexpr size_t kAliasHdrBytes = 256;
 
 __host__ __device__ inline size_t alias_weights_off() { return kAliasHdrBytes; }
 __host__ __device__ inline size_t alias_partials_off(int64_t n_vox) {
     return (kAliasHdrBytes + (size_t)n_vox * 4 + 255) / 256 * 256;
+}
+__host__ __device__ inline size_t tv_table_off(int64_t n_vox) {
+    return alias_partials_off(n_vox) + (size_t)kAliasBlocks * 64 * sizeof(double);
 }
 
 __global__ __launch_bounds__(256) void k_alias_weights(const TvArgs a, int64_t ncell,
@@ -173,10 +234,21 @@ __global__ __launch_bounds__(256) void k_alias_partial(const TvArgs a, const flo
     const int g = threadIdx.x / LPV, p = threadIdx.x % LPV;
     double acc = 0.0;
     if (p < a.P) {
-        for (int64_t r = (int64_t)blockIdx.x * kGroups + g; r < a.n_vox;
-             r += (int64_t)gridDim.x * kGroups) {
-            const float wr = w[r];
-            if (wr != 0.0f) acc += (double)wr * (double)ld<STORE>(a.params, (size_t)r * a.stride + p);
+        // Four records per step with unconditional loads, so the weight and
+        // value loads of a step are in flight together.
+        const int64_t step = (int64_t)gridDim.x * kGroups;
+        for (int64_t r0 = (int64_t)blockIdx.x * kGroups + g; r0 < a.n_vox; r0 += 4 * step) {
+            float wr[4], ur[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const int64_t r = r0 + k * step;
+                const int64_t rc = r < a.n_vox ? r : 0;
+                wr[k] = r < a.n_vox ? w[rc] : 0.0f;
+                ur[k] = ld<STORE>(a.params, (size_t)rc * a.stride + p);
+            }
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                if (wr[k] != 0.0f) acc += (double)wr[k] * (double)ur[k];
         }
     }
     red[threadIdx.x] = acc;
@@ -188,17 +260,25 @@ __global__ __launch_bounds__(256) void k_alias_partial(const TvArgs a, const flo
     }
 }
 
+// One block of 1024 threads: thread t sums the partials of parameter t % 64
+// over the blocks b = t / 64 (mod 16), then 16 lanes per parameter fold in
+// LDS.
 template <int STORE>
-__global__ void k_alias_final(const TvArgs a, const AliasHeader *hdr, const double *partials,
-                              int nblocks) {
-    const int p = threadIdx.x;
-    if (p >= a.P) return;
+__global__ __launch_bounds__(1024) void k_alias_final(const TvArgs a, const AliasHeader *hdr,
+                                                      const double *partials, int nblocks) {
+    __shared__ double red[1024];
+    const int p = threadIdx.x & 63, c = threadIdx.x >> 6;
+    double s = 0.0;
+    for (int b = c; b < nblocks; b += 16) s += partials[(size_t)b * 64 + p];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x >= 64 || p >= a.P) return;
     const float lam = p == 0 ? a.lam_d : a.lam_sh;
     if (lam == 0.0f || hdr->magic != kAliasMagic || hdr->n_vox != a.n_vox) return;
-    double s = 0.0;
-    for (int b = 0; b < nblocks; b++) s += partials[(size_t)b * 64 + p];
+    double t = 0.0;
+    for (int k = 0; k < 16; k++) t += red[k * 64 + p];
     const double u0 = (double)ld<STORE>(a.params, (size_t)p);
-    a.grad[p] += (float)((double)lam * ((double)hdr->n_alias * u0 - s));
+    a.grad[p] += (float)((double)lam * ((double)hdr->n_alias * u0 - t));
 }
 
 __global__ void k_alias_init(AliasHeader *hdr, float *w, int64_t n_vox) {
@@ -227,12 +307,18 @@ __global__ __launch_bounds__(256) void k_tv_neighbour6(const TvArgs a) {
     if (x == 0 || x == a.W - 1 || y == 0 || y == a.H - 1 || z == 0 || z == a.D - 1) return;
     const float self = ld<STORE>(a.params, (size_t)r * a.stride + p);
     const int off[6][3] = {{0, 0, -1}, {0, -1, 0}, {-1, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
-    double sq = 0.0, dis = 0.0;
+    // Unconditional loads (index 0 reads the vertex itself), all in flight
+    // together.
+    float un[6];
 #pragma unroll
     for (int i = 0; i < 6; i++) {
         const uint32_t n = record_of(a, x + off[i][0], y + off[i][1], z + off[i][2]);
-        const float un = n != 0u ? ld<STORE>(a.params, (size_t)n * a.stride + p) : self;
-        const double d = (double)self - (double)un;
+        un[i] = ld<STORE>(a.params, (size_t)(n != 0u ? n : (uint32_t)r) * a.stride + p);
+    }
+    double sq = 0.0, dis = 0.0;
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        const double d = (double)self - (double)un[i];
         sq += d * d;
         dis += d;
     }
@@ -369,8 +455,20 @@ void tv_launch(const TvArgs &a, int type, void *scratch, hipStream_t s) {
     const int groups = wide ? 4 : 8;
     const unsigned blocks = (unsigned)((a.n_vox + groups - 1) / groups);
     if (type == PXL_TV_DISPARITY) {
-        if (wide) hipLaunchKernelGGL((k_tv_disparity<64, STORE>), dim3(blocks), dim3(256), 0, s, a);
-        else hipLaunchKernelGGL((k_tv_disparity<32, STORE>), dim3(blocks), dim3(256), 0, s, a);
+        if (a.sparse) {
+            const uint32_t *table = reinterpret_cast<const uint32_t *>(
+                reinterpret_cast<char *>(scratch) + tv_table_off(a.n_vox));
+            if (wide)
+                hipLaunchKernelGGL((k_tv_disparity_table<64, STORE>), dim3(blocks), dim3(256), 0,
+                                   s, a, table);
+            else
+                hipLaunchKernelGGL((k_tv_disparity_table<32, STORE>), dim3(blocks), dim3(256), 0,
+                                   s, a, table);
+        } else if (wide) {
+            hipLaunchKernelGGL((k_tv_disparity<64, STORE>), dim3(blocks), dim3(256), 0, s, a);
+        } else {
+            hipLaunchKernelGGL((k_tv_disparity<32, STORE>), dim3(blocks), dim3(256), 0, s, a);
+        }
         if (a.sparse) {
             char *base = reinterpret_cast<char *>(scratch);
             const AliasHeader *hdr = reinterpret_cast<const AliasHeader *>(base);
@@ -382,7 +480,7 @@ void tv_launch(const TvArgs &a, int type, void *scratch, hipStream_t s) {
             else
                 hipLaunchKernelGGL((k_alias_partial<32, STORE>), dim3(kAliasBlocks), dim3(256), 0,
                                    s, a, w, part);
-            hipLaunchKernelGGL(k_alias_final<STORE>, dim3(1), dim3(64), 0, s, a, hdr, part,
+            hipLaunchKernelGGL(k_alias_final<STORE>, dim3(1), dim3(1024), 0, s, a, hdr, part,
                                kAliasBlocks);
         }
     } else {
@@ -398,7 +496,7 @@ using namespace pxl;
 
 extern "C" size_t pxl_tv_scratch_bytes(const pxl_grid *grid) {
     if (!grid || !grid->sparse || grid->n_vox <= 0) return 0;
-    return alias_partials_off(grid->n_vox) + (size_t)kAliasBlocks * 64 * sizeof(double);
+    return tv_table_off(grid->n_vox) + (size_t)grid->n_vox * kTableWords * sizeof(uint32_t);
 }
 
 static void fill_tv_args(const pxl_grid *grid, const pxl_optim *optim, TvArgs &a) {
@@ -442,6 +540,9 @@ extern "C" int32_t pxl_tv_prepare(const pxl_grid *grid, void *scratch, size_t sc
     const int64_t ncell = (int64_t)(a.W - 1) * (a.H - 1) * (a.D - 1);
     hipLaunchKernelGGL(k_alias_weights, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, s, a,
                        ncell, hdr, w);
+    uint32_t *table = reinterpret_cast<uint32_t *>(base + tv_table_off(a.n_vox));
+    hipLaunchKernelGGL(k_tv_table_build<32>, dim3((unsigned)((a.n_vox + 7) / 8)), dim3(256), 0, s,
+                       a, table);
     return check_launch("pxl_tv_prepare");
 }
 
