@@ -130,11 +130,40 @@ __device__ __forceinline__ int bin_of(int k, double step, double tn, double tf) 
     return q > 63.0 ? 63 : (int)q;
 }
 
+// Float building blocks on the hardware exp2 / log2 / reciprocal (about one
+// ulp each).  exp(z) = 2^h (1 + l ln 2) with h + l = z log2 e split by an
+// FMA (h the rounded product), so large |z| keeps a few-ulp result.  log1p
+// and expm1 use Kahan's identities log1p(y) = y log(u) / (u - 1), u = 1 + y,
+// and expm1(x) = (u - 1) x / log(u), u = e^x, which keep the relative
+// accuracy of small arguments to a few ulps (W. Kahan, HP-15C Advanced
+// Functions Handbook, 1982).
+__device__ __forceinline__ float fexp(float z) {
+    constexpr float kL2e = 1.44269504088896341f, kL2eLo = 1.925963e-08f;
+    const float h = z * kL2e;
+    const float l = __builtin_fmaf(z, kL2e, -h) + z * kL2eLo;
+    return __builtin_amdgcn_exp2f(h) * (1.0f + l * 0.693147180559945309f);
+}
+__device__ __forceinline__ float flog(float u) {
+    return __builtin_amdgcn_logf(u) * 0.693147180559945309f;
+}
+__device__ __forceinline__ float frcp(float u) { return __builtin_amdgcn_rcpf(u); }
+__device__ __forceinline__ float flog1p(float y) {  // y >= 0
+    const float u = 1.0f + y;
+    return u == 1.0f ? y : flog(u) * (y / (u - 1.0f));
+}
+__device__ __forceinline__ float fexpm1(float x) {  // |x| moderate
+    const float u = fexp(x);
+    if (u == 1.0f) return x;
+    const float um1 = u - 1.0f;
+    return um1 == -1.0f ? -1.0f : um1 * (x / flog(u));
+}
+
 // Activations (RenderUtils.cu:480-515) in float or double.
 template <typename R> __device__ __forceinline__ R softplus_shift(R raw);
 template <> __device__ __forceinline__ float softplus_shift<float>(float raw) {
+    // log(1 + e^x) = max(x, 0) + log1p(e^-|x|)
     const float x = raw - 13.0f;
-    return x > 15.0f ? x + log1pf(expf(-x)) : log1pf(expf(x));
+    return fmaxf(x, 0.0f) + flog1p(fexp(-fabsf(x)));
 }
 template <> __device__ __forceinline__ double softplus_shift<double>(double raw) {
     const double x = raw - 13.0;
@@ -142,14 +171,14 @@ template <> __device__ __forceinline__ double softplus_shift<double>(double raw)
 }
 template <typename R> __device__ __forceinline__ R dsoftplus_shift(R raw);
 template <> __device__ __forceinline__ float dsoftplus_shift<float>(float raw) {
-    return 1.0f / (1.0f + expf(13.0f - raw));
+    return frcp(1.0f + fexp(13.0f - raw));
 }
 template <> __device__ __forceinline__ double dsoftplus_shift<double>(double raw) {
     return 1.0 / (1.0 + exp(13.0 - raw));
 }
 template <typename R> __device__ __forceinline__ R sigmoid01(R raw);
 template <> __device__ __forceinline__ float sigmoid01<float>(float raw) {
-    return 1.0f / (1.0f + expf(-0.1f * raw));
+    return frcp(1.0f + fexp(-0.1f * raw));
 }
 template <> __device__ __forceinline__ double sigmoid01<double>(double raw) {
     return 1.0 / (1.0 + exp(-0.1 * raw));
@@ -157,7 +186,7 @@ template <> __device__ __forceinline__ double sigmoid01<double>(double raw) {
 // 1 - exp(-x) without cancellation for small x.
 template <typename R> __device__ __forceinline__ R one_minus_exp_neg(R x);
 template <> __device__ __forceinline__ float one_minus_exp_neg<float>(float x) {
-    return -expm1f(-x);
+    return -fexpm1(-x);
 }
 template <> __device__ __forceinline__ double one_minus_exp_neg<double>(double x) {
     return -expm1(-x);
