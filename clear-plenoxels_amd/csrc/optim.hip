@@ -345,6 +345,14 @@ __device__ __forceinline__ void store_param(float *base, size_t idx, float v) {
     else reinterpret_cast<uint16_t *>(base)[idx] = __half_as_ushort(__float2half_rn(v));
 }
 
+// Streaming store (non-temporal hint) of the Adam moments: no kernel reads
+// them before the next Adam step, so they need not stay in the caches.
+typedef float f4v __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void st_stream(float *p, const float4 &v) {
+    const f4v w = {v.x, v.y, v.z, v.w};
+    __builtin_nontemporal_store(w, reinterpret_cast<f4v *>(p));
+}
+
 template <int STORE>
 __global__ __launch_bounds__(256) void k_adam(float *__restrict__ prm, float *__restrict__ grd,
                                               float *__restrict__ mom, float *__restrict__ var,
@@ -399,8 +407,8 @@ __global__ __launch_bounds__(256) void k_adam(float *__restrict__ prm, float *__
         for (int e = 0; e < 4 && 4 * q + e < P; e++) store_param<STORE>(prm, poff + e, tp[e]);
     }
     *reinterpret_cast<float4 *>(grd + off) = g4;
-    *reinterpret_cast<float4 *>(mom + off) = m4;
-    *reinterpret_cast<float4 *>(var + off) = v4;
+    st_stream(mom + off, m4);
+    st_stream(var + off, v4);
 }
 
 __global__ __launch_bounds__(1024) void k_loss_reduce(const double *__restrict__ l, int64_t n,
