@@ -111,6 +111,8 @@ def _declare(L):
     L.orc_upsample.argtypes = [ctypes.POINTER(OrcGrid), _I32, _I32, _I32, _I32, _P, _P, _I64,
                                _F32, _I32, _P]
     L.orc_upsample.restype = None
+    L.orc_set_threads.argtypes = [_I32]
+    L.orc_set_threads.restype = None
     L.orc_mse.argtypes = [_P, _I64]
     L.orc_mse.restype = _F64
     L.orc_now.argtypes = []
@@ -368,3 +370,8 @@ def upsample(grid, W2, H2, D2, S2, threshold, low_band=False, in_cage=None, inde
     lib().orc_upsample(ctypes.byref(g), W2, H2, D2, S2, _p(cage), _p(io), n, threshold,
                        int(low_band), _p(out))
     return out
+
+
+def set_threads(n):
+    """Threads for the oracle's loops (1 = reproducible sums, the default)."""
+    lib().orc_set_threads(int(n))

@@ -16,6 +16,22 @@
 #include <string.h>
 #include <time.h>
 
+/* Threads used by the loops over rays, cells and parameters (default 1:
+ * every sum in a fixed order, results reproducible bit for bit).  With more
+ * threads the gradient additions become atomic and their order varies. */
+static int g_threads = 1;
+
+void orc_set_threads(int32_t n) { g_threads = n > 0 ? n : 1; }
+
+static inline void add_d(double *p, double v) {
+    if (g_threads > 1) {
+#pragma omp atomic
+        *p += v;
+    } else {
+        *p += v;
+    }
+}
+
 #define ORC_PI 3.141592653589793
 
 /* ------------------------------------------------------------------ */
@@ -447,6 +463,7 @@ static void render_fwd(const orc_grid *g, const orc_rays *r, const int32_t *samp
                        double *out_w, double *out_density, uint32_t *trace,
                        const uint64_t *bin_mask, double target_wmin, uint64_t *targets) {
     const double step = (double)g->step;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(g_threads) if (g_threads > 1)
     for (int64_t b = 0; b < len; b++) {
         const int ray = sample_index[start + b];
         const double ox = r->ox[ray], oy = r->oy[ray], oz = r->oz[ray];
@@ -618,6 +635,7 @@ static void render_bwd(const orc_grid *g, const orc_rays *r, const int32_t *samp
     const int S = g->S;
     const int64_t n = g->n_vox;
     const double kc = (double)0.6667f, kd = 0.6667;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(g_threads) if (g_threads > 1)
     for (int64_t b = 0; b < len; b++) {
         const int ray = sample_index[start + b];
         const double tn = r->near_t[ray], tf = r->far_t[ray];
@@ -683,10 +701,11 @@ static void render_bwd(const orc_grid *g, const orc_rays *r, const int32_t *samp
             gs *= kd * step * ds;
             gs += step * ds * (Af - Ai - T) * m;
             for (int j = 0; j < 8; j++) {
-                grad[s.v[j]] += gs * s.w[j];
+                add_d(&grad[s.v[j]], gs * s.w[j]);
                 for (int ch = 0; ch < 3; ch++)
                     for (int kk = 0; kk < S; kk++)
-                        grad[(int64_t)(1 + ch * S + kk) * n + s.v[j]] += gc[ch] * s.w[j] * Y[kk];
+                        add_d(&grad[(int64_t)(1 + ch * S + kk) * n + s.v[j]],
+                              gc[ch] * s.w[j] * Y[kk]);
             }
             ++k;
             if (g->adaptive) t += fabs(step - w);
@@ -721,6 +740,7 @@ void orc_tv_disparity(const orc_grid *g, double lambda_density, double lambda_sh
     const int W = g->W, H = g->H, D = g->D, S = g->S;
     const int64_t n = g->n_vox;
     const int P = 1 + 3 * S;
+#pragma omp parallel for collapse(2) schedule(dynamic, 4) num_threads(g_threads) if (g_threads > 1)
     for (int z = 0; z < D - 1; z++)
         for (int y = 0; y < H - 1; y++)
             for (int x = 0; x < W - 1; x++) {
@@ -741,7 +761,8 @@ void orc_tv_disparity(const orc_grid *g, double lambda_density, double lambda_sh
                     double mean = 0.0;
                     for (int j = 0; j < 8; j++) mean += pl[v[j]];
                     mean /= 8.0;
-                    for (int j = 0; j < 8; j++) grad[(int64_t)p * n + v[j]] += lam * (pl[v[j]] - mean);
+                    for (int j = 0; j < 8; j++)
+                        add_d(&grad[(int64_t)p * n + v[j]], lam * (pl[v[j]] - mean));
                 }
             }
 }
@@ -783,6 +804,7 @@ void orc_adam(float *params, float *grad, float *m, float *v, const float *adapt
     const float b1 = 0.9f, b2 = 0.999f, eps = 1.e-6f;
     const double bc1 = 1.0 - (double)powf(b1, (float)t);
     const double bc2 = 1.0 - (double)powf(b2, (float)t);
+#pragma omp parallel for collapse(2) schedule(static) num_threads(g_threads) if (g_threads > 1)
     for (int p = 0; p < P; p++)
         for (int64_t i = 0; i < n; i++) {
             const int64_t k = (int64_t)p * n + i;

@@ -126,6 +126,9 @@ void orc_upsample_density(const orc_grid *src, int32_t W2, int32_t H2, int32_t D
 void orc_upsample(const orc_grid *src, int32_t W2, int32_t H2, int32_t D2, int32_t S2,
                   const uint8_t *dst_in_cage, const uint32_t *dst_index, int64_t dst_n,
                   float threshold, int32_t low_band, float *dst);
+/* Threads for the ray, cell and parameter loops (default 1, bit-reproducible;
+ * more threads make the gradient additions atomic, in varying order). */
+void orc_set_threads(int32_t n);
 double orc_mse(const double *losses, int64_t len);
 
 /* Wall-clock seconds (CLOCK_MONOTONIC), for the CPU baseline. */
