@@ -60,7 +60,6 @@ __device__ __forceinline__ void tv_record(const TvArgs &a, int64_t r, int p, uin
     // lane without a parameter reads element 0: both lines are read anyway.
     const int q = mine ? p : 0;
     float v[27];
-    uint32_t need = 0u;
 #pragma unroll
     for (int i = 0; i < 27; i++) {
         const int dx = i % 3 - 1, dy = (i / 3) % 3 - 1, dz = i / 9 - 1;
@@ -75,29 +74,28 @@ __device__ __forceinline__ void tv_record(const TvArgs &a, int64_t r, int p, uin
         }
         const uint32_t ri = (uint32_t)__shfl((int)nrec, i, LPV);
         const bool used = (cells & cmask) != 0u;
-        need |= used ? 1u << i : 0u;
         v[i] = ld<STORE>(a.params, (size_t)(used ? ri : (uint32_t)r) * (size_t)a.stride + q);
     }
     if (!mine) return;
-    // Sums run in FP64: a vertex value and the mean of a cell around it
-    // nearly cancel.
-    double u[27];
+    // sum over occupied cells c of (u - mean_c) = n u - (1/8) sum_i m_i u_i,
+    // n the occupied cells around the vertex, m_i those that also contain
+    // neighbour i (m_13 = n).  Sums run in FP64: a vertex value and the mean
+    // of a cell around it nearly cancel.
+    double msum = 0.0;
 #pragma unroll
-    for (int i = 0; i < 27; i++) u[i] = ((need >> i) & 1u) ? (double)v[i] : 0.0;
-    const double self = u[13];
-    double tsum = 0.0;
+    for (int i = 0; i < 27; i++) {
+        const int dx = i % 3 - 1, dy = (i / 3) % 3 - 1, dz = i / 9 - 1;
+        uint32_t cells = 0u;
 #pragma unroll
-    for (int c = 0; c < 8; c++) {
-        if (!((cmask >> c) & 1u)) continue;
-        const int ax = c & 1, ay = (c >> 1) & 1, az = c >> 2;
-        double s = 0.0;
-#pragma unroll
-        for (int j = 0; j < 8; j++) {
-            const int jx = j & 1, jy = (j >> 1) & 1, jz = j >> 2;
-            s += u[(jz - az + 1) * 9 + (jy - ay + 1) * 3 + (jx - ax + 1)];
+        for (int c = 0; c < 8; c++) {
+            const int ax = c & 1, ay = (c >> 1) & 1, az = c >> 2;
+            if ((dx == -ax || dx == 1 - ax) && (dy == -ay || dy == 1 - ay) &&
+                (dz == -az || dz == 1 - az))
+                cells |= 1u << c;
         }
-        tsum += self - s * 0.125;
+        msum = fma((double)__builtin_popcount(cells & cmask), (double)v[i], msum);
     }
+    const double tsum = (double)__builtin_popcount(cmask) * (double)v[13] - 0.125 * msum;
     a.grad[(size_t)r * (size_t)a.gstride + p] += (float)((double)lam * tsum);
 }
 
