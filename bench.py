@@ -40,13 +40,17 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", choices=("plain256", "uneven512"), default="plain256",
+    ap.add_argument("--workload", choices=("plain256", "uneven512", "dense128"),
+                    default="plain256",
                     help="plain256: the 256^3 FP32 configuration the metric is quoted on; "
                          "uneven512: 512^3 shells + blobs, BF16 records, 262144 "
-                         "rays/batch from a radius-6 sphere, autotuned schedule")
+                         "rays/batch from a radius-6 sphere, autotuned schedule; "
+                         "dense128: the small plain-method scene (dense 128^3, 16 blobs, "
+                         "20 views 200x200 on a sphere, 4096 rays/batch), the scene of the "
+                         "learned-proposal configuration")
     ap.add_argument("--grid", type=int, default=None)
-    ap.add_argument("--views", type=int, default=100)
-    ap.add_argument("--img", type=int, default=800)
+    ap.add_argument("--views", type=int, default=None)
+    ap.add_argument("--img", type=int, default=None)
     ap.add_argument("--batch", type=int, default=None)
     ap.add_argument("--occupancy", type=float, default=None)
     ap.add_argument("--autotune", type=int, default=None,
@@ -74,8 +78,11 @@ def parse():
                          "known traffic for calibrating profiler byte counters)")
     a = ap.parse_args()
     uneven = a.workload == "uneven512"
-    a.grid = a.grid or (512 if uneven else 256)
-    a.batch = a.batch or (262144 if uneven else 65536)
+    dense = a.workload == "dense128"
+    a.grid = a.grid or (512 if uneven else (128 if dense else 256))
+    a.batch = a.batch or (262144 if uneven else (4096 if dense else 65536))
+    a.views = a.views or (20 if dense else 100)
+    a.img = a.img or (200 if dense else 800)
     a.occupancy = a.occupancy if a.occupancy is not None else (0.01 if uneven else 0.10)
     a.autotune = a.autotune if a.autotune is not None else (1 if uneven else 0)
     if uneven and a.log_samples == 512:
@@ -249,7 +256,43 @@ class Workload:
         P = 1 + 3 * S
         stride = pxl.record_stride(S)
         self.N, self.S, self.P = N, S, P
+        self.dense = args.workload == "dense128"
         t0 = time.time()
+        if self.dense:
+            gt_rec, grid_kw = self._dense_scene(dev, stride)
+        else:
+            gt_rec, grid_kw = self._caged_scene(dev, stride)
+        self.gt_grid = pxl.Grid(N, N, N, S, gt_rec, **grid_kw)
+        if self.dense:
+            log(f"grid {N}^3 dense, {self.n_vox} vertices ({time.time() - t0:.1f}s)")
+        else:
+            log(f"grid {N}^3: tau={self.tau:.3f}, occupied cells {self.occ_frac:.4f}, "
+                f"caged vertices {self.n_vox} ({time.time() - t0:.1f}s)")
+        self._rays_and_trainer(dev, gt_rec, grid_kw, t0)
+
+    def _dense_scene(self, dev, stride):
+        """The small plain-method scene: 16 blobs (seed 1), SH ~ N(0, 0.3)."""
+        import torch
+
+        from pxl import scenes
+
+        N, P = self.N, self.P
+        self.occ_frac, self.tau = 1.0, None
+        self.n_vox = N ** 3
+        gen = torch.Generator(device=dev)
+        gen.manual_seed(101)
+        rec = torch.zeros((N ** 3, stride), dtype=torch.float32, device=dev)
+        rec[:, 0] = scenes.blob_density(N, 16, 1, device=dev).reshape(-1)
+        rec[:, 1:P].normal_(0.0, 0.3, generator=gen)
+        return rec, dict(step=0.5)
+
+    def _caged_scene(self, dev, stride):
+        import torch
+
+        import pxl
+        from pxl import scenes
+
+        N, P, args = self.N, self.P, self.args
         # Ground-truth grid, dense first.
         dens = scenes.blob_density(N, 64, 11, device=dev).reshape(-1)
         ncell = (N - 1) ** 3
@@ -284,15 +327,21 @@ class Workload:
         bricks = pxl.build_brick_occupancy(occ, N, N, N)
         grid_kw = dict(step=0.5, occupancy=occ, index_offsets=io, in_cage=cage,
                        compact_to_dense=c2d, brick_occupancy=bricks)
-        self.gt_grid = pxl.Grid(N, N, N, S, gt_rec, **grid_kw)
-        log(f"grid {N}^3: tau={tau:.3f}, occupied cells {self.occ_frac:.4f}, "
-            f"caged vertices {n_vox} ({time.time() - t0:.1f}s)")
+        return gt_rec, grid_kw
 
-        # Rays of all views.
+    def _rays_and_trainer(self, dev, gt_rec, grid_kw, t0):
+        import torch
+
+        import pxl
+        from pxl import scenes
+
+        N, S, args = self.N, self.S, self.args
+        # Rays of all views (upper hemisphere; the whole sphere for the dense
+        # scene).
         img, V = args.img, args.views
         f, _, c, _ = scenes.view_intrinsics(img, 40.0)
         self.cams = [pxl.camera_from_pose(t, [N / 2.0] * 3, 1.0, f, f, c, c)[0]
-                     for t in scenes.fibonacci_sphere(V, 4.0, upper_only=True)]
+                     for t in scenes.fibonacci_sphere(V, 4.0, upper_only=not self.dense)]
         npix = img * img
         self.n_rays = V * npix
         rays = pxl.Rays(self.n_rays, device=dev, white_bg=False)
@@ -325,6 +374,7 @@ class Workload:
                           img=(im_r[sl], im_g[sl], im_b[sl], im_m[sl]))
         del im_r, im_g, im_b, im_m, A, inv, allidx
         self.rays = rays
+        gen = torch.Generator(device=dev)
         gen.manual_seed(3)
         self.order = torch.randperm(self.n_rays, generator=gen, device=dev).to(torch.int32)
         if args.batch_order == "sorted":
@@ -405,10 +455,13 @@ def cpu_baseline(w, dev, nrays):
 
     P = w.P
     planes = w.rec[:, :P].T.contiguous().cpu().numpy()
-    occ_bits = w.grid.occupancy.cpu().numpy().view(np.uint32)
-    ncell = (w.N - 1) ** 3
-    occ = np.unpackbits(occ_bits.view(np.uint8), bitorder="little")[:ncell].astype(np.uint8)
-    io = w.grid.index_offsets.cpu().numpy().view(np.uint32)
+    if w.dense:
+        occ = io = None
+    else:
+        occ_bits = w.grid.occupancy.cpu().numpy().view(np.uint32)
+        ncell = (w.N - 1) ** 3
+        occ = np.unpackbits(occ_bits.view(np.uint8), bitorder="little")[:ncell].astype(np.uint8)
+        io = w.grid.index_offsets.cpu().numpy().view(np.uint32)
     keys = ("ox", "oy", "oz", "dx", "dy", "dz", "near_t", "far_t", "gt_r", "gt_g", "gt_b",
             "gt_mask")
     idx = w.order[:nrays].long()
@@ -431,11 +484,12 @@ def cpu_baseline(w, dev, nrays):
     oracle.adam(p32, g, m, v, 0.3, 1.0, 1)
     t_adam = oracle.now() - t0
     B = w.args.batch
-    t_step = B * t_rays + t_tv / 3.0 + t_adam
     oracle.set_threads(1)
+    t_step = B * t_rays + (t_tv if w.dense else t_tv / 3.0) + t_adam
     sample = (f"{nrays} rays of the workload (FP64 forward + backward, "
-              f"{t_rays * 1e6:.2f} us/ray on {cores} threads) + TV over the caged grid "
-              f"({t_tv:.2f} s, every 3rd step) + Adam over {w.n_vox} records "
+              f"{t_rays * 1e6:.2f} us/ray on {cores} threads) + TV over the "
+              f"{'dense' if w.dense else 'caged'} grid ({t_tv:.2f} s, "
+              f"{'every' if w.dense else 'every 3rd'} step) + Adam over {w.n_vox} records "
               f"({t_adam:.2f} s); step = {B} rays")
     return {"value": B / t_step, "unit": "rays/s", "cores": cores, "kind": "port",
             "sample": sample}
@@ -596,7 +650,8 @@ def main():
         log("CPU baseline (oracle) ...")
         cpu = cpu_baseline(w, dev, args.cpu_rays)
     out = {
-        "metric": "train rays/sec (fwd+bwd+TV+Adam) 256^3 SH2",
+        "metric": f"train rays/sec (fwd+bwd+TV+Adam) {args.grid}^3 SH2"
+                  + (" dense" if args.workload == "dense128" else ""),
         "value": round(rays_per_s, 1),
         "unit": "rays/s",
         "n_gpus": 1,
@@ -609,17 +664,19 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"batch_order": args.batch_order,
-                   "workload": f"plain FP32, {args.grid}^3 SH-2 caged grid at "
-                               f"{w.occ_frac:.4f} cell occupancy, {args.views} views "
-                               f"{args.img}x{args.img}, {args.batch} rays/batch, step 0.5",
-                   "global_batch": args.batch, "caged_records": w.n_vox,
+                   "workload": f"plain {args.storage.upper()}, {args.grid}^3 SH-2 "
+                               + ("dense grid" if args.workload == "dense128" else
+                                  f"caged grid at {w.occ_frac:.4f} cell occupancy")
+                               + f", {args.views} views {args.img}x{args.img}, "
+                                 f"{args.batch} rays/batch, step 0.5",
+                   "global_batch": args.batch, "records": w.n_vox,
                    "train_psnr_last_batch": round(psnr, 3)},
         "roofline": roof,
         "cpu_baseline": cpu,
     }
     if prop is not None:
-        out["metric"] = ("train rays/sec (fwd+bwd+TV+Adam) 256^3 SH2 with learned proposal"
-                         + (", BF16 grid" if args.storage == "bf16" else ""))
+        out["metric"] += (" with learned proposal"
+                          + (", BF16 grid" if args.storage == "bf16" else ""))
         out["config"]["proposal"] = proposal_quality(w, prop, dev, args)
     out["config"]["pretrain_steps"] = args.pretrain
     out["config"]["storage"] = args.storage
