@@ -55,6 +55,9 @@ def parse():
     ap.add_argument("--occupancy", type=float, default=None)
     ap.add_argument("--autotune", type=int, default=None,
                     help="1: the trainer times every batch order x schedule on the first batch")
+    ap.add_argument("--bf16-master", type=int, default=1,
+                    help="with --storage bf16: Adam updates an FP32 master copy (1) or the "
+                         "widened BF16 values themselves (0)")
     ap.add_argument("--storage", choices=("f32", "bf16"), default="f32",
                     help="plain256: grid records in FP32 or BF16 (FP32 master copy for Adam)")
     ap.add_argument("--proposal", action="store_true",
@@ -227,7 +230,7 @@ class UnevenWorkload:
         zero32 = torch.zeros_like(rec32)
         self.rec = pxl.records_to_16bit(zero32, pxl.STORE_BF16)
         self.grid = pxl.Grid(N, N, N, S, self.rec, storage=pxl.STORE_BF16, **grid_kw)
-        self.optim = pxl.Optim(self.grid, master_from=zero32)
+        self.optim = pxl.Optim(self.grid, master_from=zero32 if args.bf16_master else None)
         del rec32
         self.trainer = pxl.Trainer(self.grid, self.optim, self.rays, self.order, batch_size=B,
                                    learning_rate=0.3, sh_learning_rate=1.0,
@@ -393,7 +396,7 @@ class Workload:
             zero32 = self.rec
             self.rec = pxl.records_to_16bit(zero32, pxl.STORE_BF16)
             self.grid = pxl.Grid(N, N, N, S, self.rec, storage=pxl.STORE_BF16, **grid_kw)
-            self.optim = pxl.Optim(self.grid, master_from=zero32)
+            self.optim = pxl.Optim(self.grid, master_from=zero32 if args.bf16_master else None)
         else:
             self.grid = pxl.Grid(N, N, N, S, self.rec, **grid_kw)
             self.optim = pxl.Optim(self.grid)
@@ -680,6 +683,8 @@ def main():
         out["config"]["proposal"] = proposal_quality(w, prop, dev, args)
     out["config"]["pretrain_steps"] = args.pretrain
     out["config"]["storage"] = args.storage
+    if args.storage == "bf16" or args.workload == "uneven512":
+        out["config"]["fp32_master"] = bool(args.bf16_master)
     order, wpb, pers = w.trainer.schedule()
     out["config"]["schedule"] = {"batch_order": ["given", "morton", "length+morton"][order],
                                  "waves_per_block": wpb, "persistent": pers,

@@ -337,10 +337,14 @@ __device__ __forceinline__ uint16_t to_bf16(float f) {
 }
 
 template <int STORE>
-__device__ __forceinline__ void store_param(float *base, size_t idx, float v) {
-    if (STORE == PXL_STORE_F32) base[idx] = v;
-    else if (STORE == PXL_STORE_BF16) reinterpret_cast<uint16_t *>(base)[idx] = to_bf16(v);
-    else reinterpret_cast<uint16_t *>(base)[idx] = __half_as_ushort(__float2half_rn(v));
+__device__ __forceinline__ float widen16(uint16_t h) {
+    if (STORE == PXL_STORE_BF16) return __uint_as_float((uint32_t)h << 16);
+    return __half2float(__ushort_as_half(h));
+}
+template <int STORE>
+__device__ __forceinline__ uint16_t narrow16(float v) {
+    if (STORE == PXL_STORE_BF16) return to_bf16(v);
+    return __half_as_ushort(__float2half_rn(v));
 }
 
 // Streaming store (non-temporal hint) of the Adam moments: no kernel reads
@@ -370,10 +374,14 @@ __global__ __launch_bounds__(256) void k_adam(float *__restrict__ prm, float *__
     } else if (master != nullptr) {
         th = *reinterpret_cast<const float4 *>(master + off);
     } else {
-        th.x = ld<STORE>(prm, poff);
-        th.y = ld<STORE>(prm, poff + 1);
-        th.z = ld<STORE>(prm, poff + 2);
-        th.w = ld<STORE>(prm, poff + 3);
+        // Four 16-bit values in one 8-byte load (the 16-bit stride is a
+        // multiple of 8 elements, so 4 q is 8-byte aligned).
+        const uint2 h = *reinterpret_cast<const uint2 *>(reinterpret_cast<const uint16_t *>(prm) +
+                                                         poff);
+        th.x = widen16<STORE>((uint16_t)(h.x & 0xFFFFu));
+        th.y = widen16<STORE>((uint16_t)(h.x >> 16));
+        th.z = widen16<STORE>((uint16_t)(h.y & 0xFFFFu));
+        th.w = widen16<STORE>((uint16_t)(h.y >> 16));
     }
     float4 g4 = *reinterpret_cast<const float4 *>(grd + off);
     float4 m4 = *reinterpret_cast<const float4 *>(mom + off);
@@ -402,9 +410,14 @@ __global__ __launch_bounds__(256) void k_adam(float *__restrict__ prm, float *__
         *reinterpret_cast<float4 *>(prm + poff) = th;
     } else {
         if (master != nullptr) *reinterpret_cast<float4 *>(master + off) = th;
-        for (int e = 0; e < 4 && 4 * q + e < P; e++) store_param<STORE>(prm, poff + e, tp[e]);
-    }
-    *reinterpret_cast<float4 *>(grd + off) = g4;
+        // One 8-byte store of the four rounded values (elements past P hold
+        // what was loaded, so they are written back unchanged or from the
+        // master's padding).
+        uint2 h;
+        h.x = (uint32_t)narrow16<STORE>(th.x) | ((uint32_t)narrow16<STORE>(th.y) << 16);
+        h.y = (uint32_t)narrow16<STORE>(th.z) | ((uint32_t)narrow16<STORE>(th.w) << 16);
+        *reinterpret_cast<uint2 *>(reinterpret_cast<uint16_t *>(prm) + poff) = h;
+    }    *reinterpret_cast<float4 *>(grd + off) = g4;
     st_stream(mom + off, m4);
     st_stream(var + off, v4);
 }

@@ -653,6 +653,56 @@ def test_adam_bf16_master(dev):
     assert np.array_equal(stored, round_bf16(master))
 
 
+@pytest.mark.parametrize("storage", ["bf16", "f16"])
+def test_adam_16bit_without_master(dev, storage):
+    """Adam directly on 16-bit records (no FP32 master): each step is the
+    oracle's FP32 Adam on the widened stored values, rounded to nearest
+    even; stored values within one 16-bit step of that (a value at a rounding
+    boundary may land on either side)."""
+    import torch
+
+    from helpers import round_bf16
+
+    say(f"[{storage}] Adam without master copy")
+    st = pxl.STORE_BF16 if storage == "bf16" else pxl.STORE_F16
+    rng = np.random.default_rng(13)
+    P, n = 28, 20_000
+    planes = rng.normal(size=(P, n)).astype(np.float32)
+    planes[0] = np.abs(planes[0]) + 0.5
+    rec16 = pxl.records_to_16bit(to_records(planes, dev), st)
+    grid = pxl.Grid(n, 1, 1, 9, rec16, storage=st)
+    optim = pxl.Optim(grid)
+    m, v = np.zeros_like(planes), np.zeros_like(planes)
+
+    def stored():
+        return rec16[:, :P].T.contiguous().cpu().numpy().view(np.uint16)
+
+    def widen(h):
+        if storage == "bf16":
+            return (h.astype(np.uint32) << 16).view(np.float32)
+        return h.view(np.float16).astype(np.float32)
+
+    def narrow(x):
+        if storage == "bf16":
+            return round_bf16(x)
+        return x.astype(np.float16).view(np.uint16)
+
+    for t in range(1, 4):
+        torch.cuda.synchronize()
+        p_o = widen(stored())
+        g = rng.normal(size=(P, n)).astype(np.float32)
+        optim.grad[:, :P] = torch.from_numpy(g.T.copy()).to(dev)
+        pxl.adam_step(grid, optim, 0.1, 0.2, t)
+        oracle.adam(p_o, g.copy(), m, v, 0.1, 0.2, t)
+        torch.cuda.synchronize()
+        got, exp = stored().astype(np.int32), narrow(p_o).astype(np.int32)
+        d = np.abs(got - exp)
+        assert d.max() <= 1 and (d > 0).mean() < 1e-3, (t, d.max(), (d > 0).mean())
+    # the moments are FP32 and follow the oracle's
+    mom = from_records(optim.m, P).astype(np.float32)
+    np.testing.assert_allclose(mom, m, rtol=1e-6, atol=1e-7)
+
+
 # ------------------------------------------------------------ eval image
 
 
