@@ -557,6 +557,28 @@ def proposal_quality(w, prop, dev, args):
     return res
 
 
+def copy_bandwidth(dev, mib=1024, reps=5):
+    """Device-to-device copy bandwidth (bytes read + written per second, GB/s)
+    of a mib-MiB float buffer: the stream rate this box reaches, reported
+    beside the published HBM peak."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    t1.record()
+    torch.cuda.synchronize()
+    gbs = 2.0 * n * 4 * reps / (t0.elapsed_time(t1) / 1e3) / 1e9
+    del src, dst
+    return gbs
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -639,8 +661,11 @@ def main():
     achieved = alg / (kern_ms / 1e3) / 1e9
     step_alg = alg + w.n_vox * w.P * (2 * s_p + 8 + 16)
     kname = pxl.render_kernel_name(w.grid, "logged" if args.log_samples > 0 else "fused")
+    copy_gbs = copy_bandwidth(dev)
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5),
+            "peak_measured_copy": round(copy_gbs, 1),
+            "frac_of_measured_copy": round(achieved / copy_gbs, 5),
             "traffic": profiled_traffic(kname, args),
             "kernel": kname, "kernel_ms": round(kern_ms, 4),
             "touched_records": int(vt), "alg_bytes": int(alg),

@@ -131,9 +131,12 @@ def _fwd_compare(grid_o, rays_np, batch, got, trace_gpu, white=True):
     rays_o = oracle.Rays(rays_np, white_bg=white)
     ref = oracle.render_fwd(grid_o, rays_o, batch, trace=True)
     idx = batch
+    worst = 0.0
     for k_g, k_o in (("acc_r", "r"), ("acc_g", "g"), ("acc_b", "b"), ("acc_w", "w")):
         err = np.abs(got[k_g][idx].astype(np.float64) - ref[k_o][idx]).max()
         assert err <= 1e-5, (k_g, err)
+        worst = max(worst, err)
+    say(f"    max pixel / opacity error {worst:.2e}")
     dref = ref["density"][idx]
     derr = np.abs(got["acc_density"][idx] - dref) / np.maximum(1.0, np.abs(dref))
     assert derr.max() <= 1e-5, derr.max()
@@ -252,6 +255,10 @@ def _bwd_case(c1, grid, grid_o, nr, path, huber=0.1):
     l_got = losses.cpu().numpy()
     assert np.abs(l_got - l_ref).max() <= 255 * 1e-5
     g = from_records(optim.grad, grid.P)
+    rel = max(float(np.abs(g[p] - g_ref[p]).max() / np.abs(g_ref[p]).max())
+              for p in range(g_ref.shape[0]) if np.abs(g_ref[p]).max() > 0)
+    say(f"    max loss error {np.abs(l_got - l_ref).max():.2e}, max gradient error / plane "
+        f"max {rel:.2e}")
     # The separate backward reads C and A rounded to FP32 by the forward, so
     # the suffix C - C_i of late samples carries +-ulp(C)/2 (~6e-8) on
     # either side of the comparison: its floor is 2e-3 of the plane max.
