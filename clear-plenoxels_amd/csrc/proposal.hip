@@ -1,7 +1,8 @@
 // Learned proposal: a per-ray MLP that predicts which of 64 equal bins of
 // [t_near, t_far) hold samples with visible weight, so the marcher can skip
 // the others.  No counterpart in the reference (a build-only addition); the
-// plain occupancy marcher stays the fallback and the judge.
+// plain occupancy marcher stays the fallback and the reference its error is
+// measured against.
 //
 // Network: 128 input features (positional encoding of the grid entry point
 // and of the direction, 126 used) -> 64 -> 64 -> 64 bin logits, ReLU between

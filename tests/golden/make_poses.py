@@ -1,6 +1,6 @@
 """Writes tests/golden/poses_lego.npz from the reference's Poses.zip.
 
-Runs in the build container only (the reference tree is not on the GPU box):
+Runs where the reference tree is present; the tests read only the fixture:
     python tests/golden/make_poses.py /root/reference/Poses.zip
 The fixture holds, per split (train / val / test) of the lego scene: the
 intrinsic matrix K (cameraInfo.txt line 2-4), each view's camera translation,
