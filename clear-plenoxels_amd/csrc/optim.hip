@@ -1,5 +1,7 @@
 // Regulariser gradients (variance / TV loss), the Adam step and the FP64
 // batch-loss reduction.
+#include <cstdlib>
+
 #include "common.hpp"
 
 namespace pxl {
@@ -38,6 +40,22 @@ __device__ __forceinline__ uint32_t record_of(const TvArgs &a, int x, int y, int
     return a.sparse ? a.io[d] : d;
 }
 
+// Cells around a vertex that contain its neighbour i = (dz+1)*9 + (dy+1)*3 +
+// (dx+1): bit c = cell (x - (c&1), y - (c>>1&1), z - (c>>2)) contains it
+// when dx is -ax or 1-ax, and likewise for y and z.
+__device__ __forceinline__ uint32_t tv_cells_of(int i) {
+    const int dx = i % 3 - 1, dy = (i / 3) % 3 - 1, dz = i / 9 - 1;
+    uint32_t cells = 0u;
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+        const int ax = c & 1, ay = (c >> 1) & 1, az = c >> 2;
+        if ((dx == -ax || dx == 1 - ax) && (dy == -ay || dy == 1 - ay) &&
+            (dz == -az || dz == 1 - az))
+            cells |= 1u << c;
+    }
+    return cells;
+}
+
 // addVarianceLossGPU (RenderUtils.cu:2469-2547) as a race-free gather.  The
 // reference visits every cell (every occupied cell when sparse), takes the
 // mean mu of its 8 corners and adds lambda * (u_corner - mu) to each
@@ -62,16 +80,7 @@ __device__ __forceinline__ void tv_record(const TvArgs &a, int64_t r, int p, uin
     float v[27];
 #pragma unroll
     for (int i = 0; i < 27; i++) {
-        const int dx = i % 3 - 1, dy = (i / 3) % 3 - 1, dz = i / 9 - 1;
-        // Cells (ax, ay, az) containing neighbour i: dx in {-ax, 1-ax} etc.
-        uint32_t cells = 0u;
-#pragma unroll
-        for (int c = 0; c < 8; c++) {
-            const int ax = c & 1, ay = (c >> 1) & 1, az = c >> 2;
-            if ((dx == -ax || dx == 1 - ax) && (dy == -ay || dy == 1 - ay) &&
-                (dz == -az || dz == 1 - az))
-                cells |= 1u << c;
-        }
+        const uint32_t cells = tv_cells_of(i);
         const uint32_t ri = (uint32_t)__shfl((int)nrec, i, LPV);
         const bool used = (cells & cmask) != 0u;
         v[i] = ld<STORE>(a.params, (size_t)(used ? ri : (uint32_t)r) * (size_t)a.stride + q);
@@ -83,18 +92,8 @@ __device__ __forceinline__ void tv_record(const TvArgs &a, int64_t r, int p, uin
     // of a cell around it nearly cancel.
     double msum = 0.0;
 #pragma unroll
-    for (int i = 0; i < 27; i++) {
-        const int dx = i % 3 - 1, dy = (i / 3) % 3 - 1, dz = i / 9 - 1;
-        uint32_t cells = 0u;
-#pragma unroll
-        for (int c = 0; c < 8; c++) {
-            const int ax = c & 1, ay = (c >> 1) & 1, az = c >> 2;
-            if ((dx == -ax || dx == 1 - ax) && (dy == -ay || dy == 1 - ay) &&
-                (dz == -az || dz == 1 - az))
-                cells |= 1u << c;
-        }
-        msum = fma((double)__builtin_popcount(cells & cmask), (double)v[i], msum);
-    }
+    for (int i = 0; i < 27; i++)
+        msum = fma((double)__builtin_popcount(tv_cells_of(i) & cmask), (double)v[i], msum);
     const double tsum = (double)__builtin_popcount(cmask) * (double)v[13] - 0.125 * msum;
     a.grad[(size_t)r * (size_t)a.gstride + p] += (float)((double)lam * tsum);
 }
@@ -126,14 +125,96 @@ __device__ __forceinline__ uint32_t tv_neighbours(const TvArgs &a, int64_t r, in
     return nrec;
 }
 
+// One element of the storage type at a byte offset from base.
+template <int STORE>
+__device__ __forceinline__ float ld_at(const char *base, uint32_t off) {
+    if (STORE == PXL_STORE_F32) return *reinterpret_cast<const float *>(base + off);
+    const uint16_t h = *reinterpret_cast<const uint16_t *>(base + off);
+    if (STORE == PXL_STORE_BF16) return __uint_as_float((uint32_t)h << 16);
+    return __half2float(__ushort_as_half(h));
+}
+
+// One element of the storage type through a buffer resource: byte offset
+// voff (per lane) + soff (wave-uniform).
+template <int STORE>
+__device__ __forceinline__ float ld_buf(__amdgpu_buffer_rsrc_t rs, uint32_t voff, uint32_t soff) {
+    if (STORE == PXL_STORE_F32)
+        return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, (int)voff, (int)soff, 0));
+    const uint16_t h = __builtin_amdgcn_raw_buffer_load_b16(rs, (int)voff, (int)soff, 0);
+    if (STORE == PXL_STORE_BF16) return __uint_as_float((uint32_t)h << 16);
+    return __half2float(__ushort_as_half(h));
+}
+
+// TV gradient of an interior vertex of a dense grid: all 8 cells around it
+// exist, so m_i = 2^(zero components of (dx, dy, dz)) and
+//   n u - (1/8) sum_i m_i u_i = 7 u - F / 2 - E / 4 - C / 8
+// with F, E, C the sums over the 6 face, 12 edge and 8 corner neighbours.
+// Neighbour i lies at a grid-uniform record offset from the vertex's first
+// neighbour r0 = r - (1 + W + W H).  OFF32 (the records span less than
+// 4 GiB): the loads go through a buffer resource with the lane's byte
+// offset from r0 in a vector register, shared by all 27 loads, and the
+// neighbour's offset from r0 in a scalar register.
+template <int STORE, bool OFF32>
+__device__ __forceinline__ void tv_interior(const TvArgs &a, int64_t r, int p) {
+    constexpr uint32_t kElem = STORE == PXL_STORE_F32 ? 4u : 2u;
+    const float lam = p == 0 ? a.lam_d : a.lam_sh;
+    const bool mine = p < a.P && lam != 0.0f;
+    const uint32_t q = mine ? (uint32_t)p : 0u;
+    const int64_t plane = (int64_t)a.W * a.H;
+    const int64_t r0 = r - (1 + a.W + plane);
+    float v[27];
+    if (OFF32) {
+        const uint32_t rec_bytes = (uint32_t)a.stride * kElem;
+        const uint32_t bytes = (uint32_t)a.n_vox * rec_bytes;
+        const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<float *>(a.params), (short)0, (int)bytes, 0x00020000);
+        const uint32_t lane_off = (uint32_t)r0 * rec_bytes + q * kElem;
+#pragma unroll
+        for (int i = 0; i < 27; i++) {
+            const uint32_t off = (uint32_t)(i % 3) + (uint32_t)((i / 3) % 3) * (uint32_t)a.W +
+                                 (uint32_t)(i / 9) * (uint32_t)plane;
+            v[i] = ld_buf<STORE>(rs, lane_off, off * rec_bytes);
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < 27; i++) {
+            const int64_t off = (int64_t)(i % 3) + (int64_t)((i / 3) % 3) * a.W +
+                                (int64_t)(i / 9) * plane;
+            v[i] = ld<STORE>(a.params, (size_t)(r0 + off) * (size_t)a.stride + q);
+        }
+    }
+    if (!mine) return;
+    double f = 0.0, e = 0.0, c = 0.0;
+#pragma unroll
+    for (int i = 0; i < 27; i++) {
+        const int nz = (i % 3 != 1) + ((i / 3) % 3 != 1) + (i / 9 != 1);
+        if (nz == 1) f += (double)v[i];
+        else if (nz == 2) e += (double)v[i];
+        else if (nz == 3) c += (double)v[i];
+    }
+    const double tsum = 7.0 * (double)v[13] - (0.5 * f + 0.25 * e + 0.125 * c);
+    a.grad[(size_t)r * (size_t)a.gstride + p] += (float)((double)lam * tsum);
+}
+
 // addVarianceLossGPU as a gather, looking the neighbours up on the fly
-// (dense grids, where a record's neighbours are arithmetic).
-template <int LPV, int STORE>
+// (dense grids, where a record's neighbours are arithmetic).  Interior
+// vertices take tv_interior; boundary vertices the general gather.
+template <int LPV, int STORE, bool OFF32>
 __global__ __launch_bounds__(256) void k_tv_disparity(const TvArgs a) {
     constexpr int kGroups = 256 / LPV;
     const int g = threadIdx.x / LPV, p = threadIdx.x % LPV;
     const int64_t r = (int64_t)xcd_slice_block(blockIdx.x, gridDim.x) * kGroups + g;
     if (r >= a.n_vox) return;  // the whole group leaves together
+    {
+        const uint32_t d = (uint32_t)r;
+        const uint32_t x = d % (uint32_t)a.W, y = (d / (uint32_t)a.W) % (uint32_t)a.H;
+        const uint32_t z = d / ((uint32_t)a.W * (uint32_t)a.H);
+        if (x - 1u < (uint32_t)a.W - 2u && y - 1u < (uint32_t)a.H - 2u &&
+            z - 1u < (uint32_t)a.D - 2u) {
+            tv_interior<STORE, OFF32>(a, r, p);
+            return;
+        }
+    }
     uint32_t cmask;
     const uint32_t nrec = tv_neighbours<LPV>(a, r, p, cmask);
     if (cmask == 0u) return;  // no occupied cell around this vertex
@@ -141,9 +222,13 @@ __global__ __launch_bounds__(256) void k_tv_disparity(const TvArgs a) {
 }
 
 // Caged grids: the neighbour records and the cell mask depend only on the
-// cage, so pxl_tv_prepare stores them, 32 words per record (27 records, the
-// mask, 4 zero words), and each TV call reads them with one coalesced load
-// instead of the compact-to-dense, index and occupancy lookups.
+// cage, so pxl_tv_prepare stores them, 32 words per record, and each TV call
+// reads them with one coalesced load instead of the compact-to-dense, index
+// and occupancy lookups.  Words 0-26: the record to read for neighbour i
+// (the record itself when no occupied cell around the vertex contains
+// neighbour i); word 27: the occupied-cell mask; words 28-31: the
+// multiplicities m_i (occupied cells containing both the vertex and
+// neighbour i, 0..8) as nibbles, m_i in word 28 + i/8 at bit 4 (i % 8).
 constexpr int kTableWords = 32;
 
 template <int LPV>
@@ -151,16 +236,29 @@ __global__ __launch_bounds__(256) void k_tv_table_build(const TvArgs a, uint32_t
     constexpr int kGroups = 256 / LPV;
     const int g = threadIdx.x / LPV, p = threadIdx.x % LPV;
     const int64_t r = (int64_t)blockIdx.x * kGroups + g;
-    if (r >= a.n_vox) return;
+    if (r >= a.n_vox) return;  // the whole group leaves together
     uint32_t cmask;
     const uint32_t nrec = tv_neighbours<LPV>(a, r, p, cmask);
-    if (p < kTableWords) table[(size_t)r * kTableWords + p] = p < 27 ? nrec : (p == 27 ? cmask : 0u);
+    const uint32_t m = p < 27 ? (uint32_t)__builtin_popcount(tv_cells_of(p) & cmask) : 0u;
+    uint32_t nib = 0u;
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+        nib |= (uint32_t)__shfl((int)m, ((p - 28) & 3) * 8 + k, LPV) << (4 * k);
+    uint32_t word;
+    if (p < 27) word = m != 0u ? nrec : (uint32_t)r;
+    else if (p == 27) word = cmask;
+    else word = nib;
+    if (p < kTableWords) table[(size_t)r * kTableWords + p] = word;
 }
 
-template <int LPV, int STORE>
+// TV gradient of record r from its prepared table row.  OFF24: every record
+// index is below 2^24 and every byte offset into the records below 2^32, so
+// an address is one 24-bit multiply-add on the records' base pointer.
+template <int LPV, int STORE, bool OFF24>
 __global__ __launch_bounds__(256) void k_tv_disparity_table(const TvArgs a,
                                                             const uint32_t *__restrict__ table) {
     constexpr int kGroups = 256 / LPV;
+    constexpr uint32_t kElem = STORE == PXL_STORE_F32 ? 4u : 2u;
     const int g = threadIdx.x / LPV, p = threadIdx.x % LPV;
     const int64_t r = (int64_t)xcd_slice_block(blockIdx.x, gridDim.x) * kGroups + g;
     if (r >= a.n_vox) return;
@@ -169,8 +267,36 @@ __global__ __launch_bounds__(256) void k_tv_disparity_table(const TvArgs a,
     if (cmask == 0u) return;
     // A record index beyond the grid (scratch not prepared for this cage)
     // reads record 0 instead of faulting.
-    if ((int64_t)e >= a.n_vox) e = 0u;
-    tv_record<LPV, STORE>(a, r, p, e, cmask);
+    if (p < 27 && (int64_t)e >= a.n_vox) e = 0u;
+    const float lam = p == 0 ? a.lam_d : a.lam_sh;
+    const bool mine = p < a.P && lam != 0.0f;
+    // Every lane issues all 27 loads unconditionally, so they are in flight
+    // together; a lane without a parameter reads element 0.
+    const uint32_t q = mine ? (uint32_t)p : 0u;
+    const char *base = reinterpret_cast<const char *>(a.params);
+    const uint32_t rec_bytes = (uint32_t)a.stride * kElem;
+    float v[27];
+#pragma unroll
+    for (int i = 0; i < 27; i++) {
+        const uint32_t ri = (uint32_t)__shfl((int)e, i, LPV);
+        if (OFF24)
+            v[i] = ld_at<STORE>(base, (uint32_t)__umul24(ri, rec_bytes) + q * kElem);
+        else
+            v[i] = ld<STORE>(a.params, (size_t)ri * (size_t)a.stride + q);
+    }
+    uint32_t mw[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) mw[j] = (uint32_t)__shfl((int)e, 28 + j, LPV);
+    if (!mine) return;
+    // sum over occupied cells c of (u - mean_c) = n u - (1/8) sum_i m_i u_i
+    // (m_13 = n), in FP64: a vertex value and the mean of a cell around it
+    // nearly cancel.
+    double msum = 0.0;
+#pragma unroll
+    for (int i = 0; i < 27; i++)
+        msum = fma((double)((mw[i >> 3] >> (4 * (i & 7))) & 15u), (double)v[i], msum);
+    const double tsum = (double)__builtin_popcount(cmask) * (double)v[13] - 0.125 * msum;
+    a.grad[(size_t)r * (size_t)a.gstride + p] += (float)((double)lam * tsum);
 }
 
 // Caged grids: a cell corner that is not in cage reads and writes record 0
@@ -468,6 +594,13 @@ int32_t tv_check(const pxl_grid *g, const pxl_optim *o, const char *who) {
     return PXL_OK;
 }
 
+// PXL_TV_OFFSETS=64 forces the 64-bit address form of the TV gathers, which
+// otherwise serves only grids beyond 2^24 records or 4 GiB (both are tested).
+bool offsets64_forced() {
+    const char *e = getenv("PXL_TV_OFFSETS");
+    return e && strcmp(e, "64") == 0;
+}
+
 template <int STORE>
 void tv_launch(const TvArgs &a, int type, void *scratch, hipStream_t s) {
     const bool wide = a.P > 32;
@@ -477,16 +610,38 @@ void tv_launch(const TvArgs &a, int type, void *scratch, hipStream_t s) {
         if (a.sparse) {
             const uint32_t *table = reinterpret_cast<const uint32_t *>(
                 reinterpret_cast<char *>(scratch) + tv_table_off(a.n_vox));
-            if (wide)
-                hipLaunchKernelGGL((k_tv_disparity_table<64, STORE>), dim3(blocks), dim3(256), 0,
-                                   s, a, table);
+            constexpr uint64_t kElem = STORE == PXL_STORE_F32 ? 4u : 2u;
+            const bool off24 = !offsets64_forced() && a.n_vox <= (int64_t)(1u << 24) &&
+                               (uint64_t)a.stride * kElem < (1u << 24) &&
+                               (uint64_t)a.n_vox * (uint64_t)a.stride * kElem <= 0xFFFFFFFFull;
+            if (wide && off24)
+                hipLaunchKernelGGL((k_tv_disparity_table<64, STORE, true>), dim3(blocks),
+                                   dim3(256), 0, s, a, table);
+            else if (wide)
+                hipLaunchKernelGGL((k_tv_disparity_table<64, STORE, false>), dim3(blocks),
+                                   dim3(256), 0, s, a, table);
+            else if (off24)
+                hipLaunchKernelGGL((k_tv_disparity_table<32, STORE, true>), dim3(blocks),
+                                   dim3(256), 0, s, a, table);
             else
-                hipLaunchKernelGGL((k_tv_disparity_table<32, STORE>), dim3(blocks), dim3(256), 0,
-                                   s, a, table);
-        } else if (wide) {
-            hipLaunchKernelGGL((k_tv_disparity<64, STORE>), dim3(blocks), dim3(256), 0, s, a);
+                hipLaunchKernelGGL((k_tv_disparity_table<32, STORE, false>), dim3(blocks),
+                                   dim3(256), 0, s, a, table);
         } else {
-            hipLaunchKernelGGL((k_tv_disparity<32, STORE>), dim3(blocks), dim3(256), 0, s, a);
+            constexpr uint64_t kElem = STORE == PXL_STORE_F32 ? 4u : 2u;
+            const bool off32 = !offsets64_forced() &&
+                               (uint64_t)a.n_vox * (uint64_t)a.stride * kElem <= 0xFFFFFFFFull;
+            if (wide && off32)
+                hipLaunchKernelGGL((k_tv_disparity<64, STORE, true>), dim3(blocks), dim3(256), 0,
+                                   s, a);
+            else if (wide)
+                hipLaunchKernelGGL((k_tv_disparity<64, STORE, false>), dim3(blocks), dim3(256),
+                                   0, s, a);
+            else if (off32)
+                hipLaunchKernelGGL((k_tv_disparity<32, STORE, true>), dim3(blocks), dim3(256), 0,
+                                   s, a);
+            else
+                hipLaunchKernelGGL((k_tv_disparity<32, STORE, false>), dim3(blocks), dim3(256),
+                                   0, s, a);
         }
         if (a.sparse) {
             char *base = reinterpret_cast<char *>(scratch);

@@ -405,6 +405,66 @@ def test_tv_disparity(c1, sparse1, kind):
     assert worst <= 1.0
 
 
+@pytest.mark.parametrize("kind", ["f32", "bf16", "wide", "dense", "dense-wide"])
+@pytest.mark.parametrize("form", ["short", "64"])
+def test_tv_disparity_forms(c1, sparse1, kind, form, monkeypatch):
+    """TV in both address forms: the short one (caged: 24-bit record
+    offsets; dense: 32-bit offsets through a buffer resource) and the
+    64-bit form that grids beyond 2^24 records or 4 GiB of records take
+    (forced here by PXL_TV_OFFSETS=64); on caged FP32 and bfloat16 records,
+    on records wider than 32 parameters (SH degree 3: 49 parameters, one
+    wave per record), and on dense grids (interior vertices take their own
+    path, boundary vertices the general gather)."""
+    import torch
+
+    from helpers import to_16bit_planes
+
+    monkeypatch.setenv("PXL_TV_OFFSETS", form)
+    dev = c1["dev"]
+    if kind.startswith("dense"):
+        N, S = (40, 16) if kind == "dense-wide" else (N1, S1)
+        planes = c1_planes(N, S, n_blobs=6, seed=5) if kind == "dense-wide" else c1["planes"]
+        say(f"[tv] dense {N}^3 SH {S}, {form} offsets")
+        grid = pxl.Grid(N, N, N, S, to_records(planes, dev), step=0.5)
+        optim = pxl.Optim(grid)
+        pxl.add_variance_loss(grid, optim, 1.0, 0.5, pxl.TV_DISPARITY, 6, optimize_step=0)
+        torch.cuda.synchronize()
+        ref = oracle.tv_disparity(oracle.Grid(N, N, N, S, planes, 0.5), 1.0, 0.5,
+                                  optimize_step=0)
+        worst = grad_close(from_records(optim.grad, grid.P), ref)
+        say(f"    worst TV error / tolerance = {worst:.3g}")
+        assert worst <= 1.0
+        return
+    if kind == "wide":
+        N, S = 40, 16
+        planes = c1_planes(N, S, n_blobs=6, seed=5)
+        occ, cage, io, n, c2d, compact = sparse_from_dense(planes, N, 30.0, dev)
+    else:
+        N, S = N1, S1
+        s = sparse1
+        occ, cage, io, n, c2d, compact = (s["occ"], s["cage"], s["io"], s["n"], s["c2d"],
+                                          s["compact"])
+    say(f"[tv] caged table, {kind} records, {form}-bit offsets, {n} records")
+    storage = pxl.STORE_BF16 if kind == "bf16" else pxl.STORE_F32
+    rec = to_records(compact, dev)
+    if storage != pxl.STORE_F32:
+        rec = pxl.records_to_16bit(rec, storage)
+        compact = to_16bit_planes(compact, storage)
+    grid = pxl.Grid(N, N, N, S, rec, step=0.5, storage=storage,
+                    occupancy=torch.from_numpy(pack_bits(occ).view(np.int32)).to(dev),
+                    index_offsets=torch.from_numpy(io.view(np.int32)).to(dev),
+                    in_cage=torch.from_numpy(cage).to(dev),
+                    compact_to_dense=torch.from_numpy(c2d.view(np.int32)).to(dev))
+    optim = pxl.Optim(grid)
+    pxl.add_variance_loss(grid, optim, 1.0, 0.5, pxl.TV_DISPARITY, 6, optimize_step=0)
+    torch.cuda.synchronize()
+    grid_o = oracle.Grid(N, N, N, S, compact, 0.5, occupancy=occ, index_offsets=io)
+    ref = oracle.tv_disparity(grid_o, 1.0, 0.5, optimize_step=0)
+    worst = grad_close(from_records(optim.grad, grid.P), ref)
+    say(f"    worst TV error / tolerance = {worst:.3g}")
+    assert worst <= 1.0
+
+
 def test_tv_neighbour6_dense(dev):
     import torch
 
