@@ -100,7 +100,8 @@ extern "C" int32_t pxl_batch_order(const pxl_grid *grid, const pxl_rays *rays,
                                    int32_t mode, int32_t *out_index, void *scratch,
                                    size_t scratch_bytes, void *stream) {
     clear_error();
-    PXL_REQUIRE(grid && rays && sample_index && out_index, "pxl_batch_order: null argument");
+    PXL_REQUIRE(grid && rays && ((sample_index && out_index) || len == 0),
+                "pxl_batch_order: null argument");
     PXL_REQUIRE(start >= 0 && len >= 0, "pxl_batch_order: negative start/len");
     PXL_REQUIRE(mode == PXL_ORDER_MORTON || mode == PXL_ORDER_LENGTH_MORTON,
                 "pxl_batch_order: unknown mode %d", mode);

@@ -772,7 +772,8 @@ int32_t prepare(const pxl_grid *g, const pxl_rays *r, const int32_t *sample_inde
                 int64_t start, int64_t len, const char *who, RenderArgs &a) {
     int32_t st = check_grid(g, who);
     if (st != PXL_OK) return st;
-    PXL_REQUIRE(r != nullptr && sample_index != nullptr, "%s: null rays or sample_index", who);
+    PXL_REQUIRE(r != nullptr && (sample_index != nullptr || len == 0),
+                "%s: null rays or sample_index", who);
     PXL_REQUIRE(start >= 0 && len >= 0, "%s: negative start/len", who);
     PXL_REQUIRE(r->ox && r->oy && r->oz && r->dx && r->dy && r->dz && r->near_t && r->far_t,
                 "%s: ray geometry planes must be non-null", who);

@@ -206,7 +206,9 @@ int32_t pxl_shuffle_rank(int32_t *dest, int32_t max_index, int32_t regenerate,
                          int32_t min_index, int32_t shuffle);
 
 /* Forward render of rays sample_index[start .. start+len)
- * (RenderUtils::renderRays, RenderUtils.cu:1919-1971). */
+ * (RenderUtils::renderRays, RenderUtils.cu:1919-1971).  In the render and
+ * train entry points (pxl_render_*, pxl_train_rays*) and pxl_batch_order,
+ * len == 0 is a no-op and sample_index (and out_index) may then be null. */
 int32_t pxl_render_fwd(const pxl_grid *grid, const pxl_rays *rays,
                        const int32_t *sample_index, int64_t start,
                        int64_t len, void *stream);
