@@ -196,16 +196,124 @@ __device__ __forceinline__ void tv_interior(const TvArgs &a, int64_t r, int p) {
     a.grad[(size_t)r * (size_t)a.gstride + p] += (float)((double)lam * tsum);
 }
 
+// Interior vertices of a dense grid, marched along z.  With w(0) = 2 and
+// w(+-1) = 1, m_i = w(dx) w(dy) w(dz), so the interior gradient is
+//   8 u - B / 8,  B = sum_z' w(z' - z) Bxy(z'),  Bxy(z') = sum w(dx) w(dy) u
+// over the 3 x 3 vertices around (x, y) in plane z'.  One group of LPV lanes
+// (one per parameter) walks column (x, y) over kTvChunk vertices, keeping
+// Bxy of the planes below and at z: 9 loads per vertex instead of 27.
+// Columns are numbered x fastest, then y, then the z chunk.  Loads go
+// through a buffer resource (the records span less than 4 GiB): lane offset
+// in a vector register, neighbour offset in a scalar one.
+constexpr int kTvChunk = 24;
+
+__device__ __forceinline__ double tv_blur_xy(const float (&v)[9]) {
+    const double e = ((double)v[1] + (double)v[3]) + ((double)v[5] + (double)v[7]);
+    const double c = ((double)v[0] + (double)v[2]) + ((double)v[6] + (double)v[8]);
+    return 4.0 * (double)v[4] + 2.0 * e + c;
+}
+
+template <int LPV, int STORE>
+__global__ __launch_bounds__(256) void k_tv_zmarch(const TvArgs a, int64_t n_cols) {
+    constexpr int kGroups = 256 / LPV;
+    constexpr uint32_t kElem = STORE == PXL_STORE_F32 ? 4u : 2u;
+    const int g = threadIdx.x / LPV, p = threadIdx.x % LPV;
+    const int64_t col = (int64_t)xcd_slice_block(blockIdx.x, gridDim.x) * kGroups + g;
+    if (col >= n_cols) return;
+    const float lam = p == 0 ? a.lam_d : a.lam_sh;
+    if (p >= a.P || lam == 0.0f) return;
+    const int nx = a.W - 2, ny = a.H - 2;
+    const int x = 1 + (int)(col % nx), y = 1 + (int)((col / nx) % ny);
+    const int z0 = 1 + (int)(col / ((int64_t)nx * ny)) * kTvChunk;
+    const int z1 = min(z0 + kTvChunk, a.D - 1);  // vertices z0 .. z1-1, planes to z1
+    const uint32_t rec_bytes = (uint32_t)a.stride * kElem;
+    const uint32_t plane_bytes = (uint32_t)a.W * (uint32_t)a.H * rec_bytes;
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<float *>(a.params), (short)0, (int)((uint32_t)a.n_vox * rec_bytes),
+        0x00020000);
+    // Byte offset of (x - 1, y - 1, z') and of the 3 x 3 neighbours from it.
+    const uint32_t lane0 = ((uint32_t)(y - 1) * (uint32_t)a.W + (uint32_t)(x - 1)) * rec_bytes +
+                           (uint32_t)p * kElem;
+    uint32_t nb[9];
+#pragma unroll
+    for (int i = 0; i < 9; i++) nb[i] = ((uint32_t)(i / 3) * (uint32_t)a.W + (uint32_t)(i % 3)) * rec_bytes;
+    const size_t gplane = (size_t)a.W * a.H * (size_t)a.gstride;
+    float *const gcol = a.grad + ((size_t)y * a.W + x) * (size_t)a.gstride + p;
+    auto plane = [&](int zz, float (&v)[9]) {
+#pragma unroll
+        for (int i = 0; i < 9; i++) v[i] = ld_buf<STORE>(rs, lane0 + (uint32_t)zz * plane_bytes, nb[i]);
+    };
+    double b_lo, b_mid, u;
+    {
+        float v[9];
+        plane(z0 - 1, v);
+        b_lo = tv_blur_xy(v);
+        plane(z0, v);
+        b_mid = tv_blur_xy(v);
+        u = (double)v[4];
+    }
+    // Ring of three slots: the slot of vertex s holds plane s + 1 and the
+    // gradient of s, loaded three steps before it is used (indices clamped
+    // to the chunk: in bounds, unused).  A step uses its slot, then refills
+    // it for vertex s + 3, so two steps' loads stay in flight behind it.
+    float pv[3][9], pg[3];
+    auto fill = [&](int k, int s) {
+        pg[k] = gcol[(size_t)min(s, z1 - 1) * gplane];
+        plane(min(s + 1, z1), pv[k]);
+    };
+    fill(0, z0);
+    fill(1, z0 + 1);
+    fill(2, z0 + 2);
+    auto step = [&](int k, int s, bool refill) {
+        const double b_hi = tv_blur_xy(pv[k]);
+        const double tsum = 8.0 * u - 0.125 * (b_lo + 2.0 * b_mid + b_hi);
+        gcol[(size_t)s * gplane] = pg[k] + (float)((double)lam * tsum);
+        b_lo = b_mid;
+        b_mid = b_hi;
+        u = (double)pv[k][4];
+        if (refill) fill(k, s + 3);
+    };
+    int s = z0;
+    for (; s + 2 < z1; s += 3) {
+        step(0, s, true);
+        step(1, s + 1, true);
+        step(2, s + 2, true);
+    }
+    if (s < z1) step(0, s, false);
+    if (s + 1 < z1) step(1, s + 1, false);
+}
+
+// Record of the b-th boundary vertex of a dense grid (W, H, D >= 3): the
+// planes z = 0 and z = D-1, then the rows y = 0 and y = H-1 of the planes
+// between, then the columns x = 0 and x = W-1 of the rows between.
+__device__ __forceinline__ int64_t tv_boundary_record(const TvArgs &a, int64_t b) {
+    const int64_t W = a.W, H = a.H, D = a.D;
+    const int64_t fz = W * H, fy = W * (D - 2), fx = (H - 2) * (D - 2);
+    if (b < 2 * fz) return (b < fz ? 0 : (D - 1) * fz) + b % fz;
+    b -= 2 * fz;
+    if (b < 2 * fy) {
+        const int64_t y = b < fy ? 0 : H - 1, e = b % fy;
+        return (1 + e / W) * fz + y * W + e % W;
+    }
+    b -= 2 * fy;
+    const int64_t x = b < fx ? 0 : W - 1, e = b % fx;
+    return (1 + e / (H - 2)) * fz + (1 + e % (H - 2)) * W + x;
+}
+
 // addVarianceLossGPU as a gather, looking the neighbours up on the fly
-// (dense grids, where a record's neighbours are arithmetic).  Interior
-// vertices take tv_interior; boundary vertices the general gather.
+// (dense grids, where a record's neighbours are arithmetic).  All vertices
+// (n_items = n_vox): interior ones take tv_interior, the rest the general
+// gather.  boundary_only (n_items boundary vertices, tv_boundary_record):
+// the general gather for the vertices k_tv_zmarch leaves.
 template <int LPV, int STORE, bool OFF32>
-__global__ __launch_bounds__(256) void k_tv_disparity(const TvArgs a) {
+__global__ __launch_bounds__(256) void k_tv_disparity(const TvArgs a, int64_t n_items,
+                                                      int boundary_only) {
     constexpr int kGroups = 256 / LPV;
     const int g = threadIdx.x / LPV, p = threadIdx.x % LPV;
-    const int64_t r = (int64_t)xcd_slice_block(blockIdx.x, gridDim.x) * kGroups + g;
-    if (r >= a.n_vox) return;  // the whole group leaves together
-    {
+    const int64_t item = (int64_t)xcd_slice_block(blockIdx.x, gridDim.x) * kGroups + g;
+    if (item >= n_items) return;  // the whole group leaves together
+    const int64_t r = boundary_only ? tv_boundary_record(a, item) : item;
+    if (!boundary_only) {
         const uint32_t d = (uint32_t)r;
         const uint32_t x = d % (uint32_t)a.W, y = (d / (uint32_t)a.W) % (uint32_t)a.H;
         const uint32_t z = d / ((uint32_t)a.W * (uint32_t)a.H);
@@ -630,18 +738,37 @@ void tv_launch(const TvArgs &a, int type, void *scratch, hipStream_t s) {
             constexpr uint64_t kElem = STORE == PXL_STORE_F32 ? 4u : 2u;
             const bool off32 = !offsets64_forced() &&
                                (uint64_t)a.n_vox * (uint64_t)a.stride * kElem <= 0xFFFFFFFFull;
+            // 32-bit offsets: interior vertices marched along z, the rest
+            // gathered one by one; otherwise every vertex gathered.
+            const int skip = off32 && a.W >= 3 && a.H >= 3 && a.D >= 3;
+            if (skip) {
+                const int64_t n_cols = (int64_t)(a.W - 2) * (a.H - 2) *
+                                       ((a.D - 2 + kTvChunk - 1) / kTvChunk);
+                const unsigned cb = (unsigned)((n_cols + groups - 1) / groups);
+                if (wide)
+                    hipLaunchKernelGGL((k_tv_zmarch<64, STORE>), dim3(cb), dim3(256), 0, s, a,
+                                       n_cols);
+                else
+                    hipLaunchKernelGGL((k_tv_zmarch<32, STORE>), dim3(cb), dim3(256), 0, s, a,
+                                       n_cols);
+            }
+            const int64_t items =
+                skip ? 2 * ((int64_t)a.W * a.H + (int64_t)a.W * (a.D - 2) +
+                            (int64_t)(a.H - 2) * (a.D - 2))
+                     : a.n_vox;
+            const unsigned ib = (unsigned)((items + groups - 1) / groups);
             if (wide && off32)
-                hipLaunchKernelGGL((k_tv_disparity<64, STORE, true>), dim3(blocks), dim3(256), 0,
-                                   s, a);
+                hipLaunchKernelGGL((k_tv_disparity<64, STORE, true>), dim3(ib), dim3(256), 0, s,
+                                   a, items, skip);
             else if (wide)
-                hipLaunchKernelGGL((k_tv_disparity<64, STORE, false>), dim3(blocks), dim3(256),
-                                   0, s, a);
+                hipLaunchKernelGGL((k_tv_disparity<64, STORE, false>), dim3(ib), dim3(256), 0, s,
+                                   a, items, 0);
             else if (off32)
-                hipLaunchKernelGGL((k_tv_disparity<32, STORE, true>), dim3(blocks), dim3(256), 0,
-                                   s, a);
+                hipLaunchKernelGGL((k_tv_disparity<32, STORE, true>), dim3(ib), dim3(256), 0, s,
+                                   a, items, skip);
             else
-                hipLaunchKernelGGL((k_tv_disparity<32, STORE, false>), dim3(blocks), dim3(256),
-                                   0, s, a);
+                hipLaunchKernelGGL((k_tv_disparity<32, STORE, false>), dim3(ib), dim3(256), 0, s,
+                                   a, items, 0);
         }
         if (a.sparse) {
             char *base = reinterpret_cast<char *>(scratch);
