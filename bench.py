@@ -68,6 +68,10 @@ def parse():
                          "else 0)")
     ap.add_argument("--prop-interval", type=int, default=8,
                     help="every n-th timed batch runs unpruned and trains the proposal")
+    ap.add_argument("--prop-threshold", type=float, default=0.05,
+                    help="bin probability below which the proposal prunes the bin")
+    ap.add_argument("--prop-dilate", type=int, default=1,
+                    help="bins kept on each side of a bin the proposal keeps")
     ap.add_argument("--cpu-rays", type=int, default=8192,
                     help="rays of the bounded CPU-baseline sample (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -433,6 +437,23 @@ def touched_vertices(w, dev, nb=4):
     return tot / nb
 
 
+def samples_per_batch(w, dev, nb=2):
+    """Mean samples the plain marcher evaluates per batch (per-ray counts of
+    a traced forward over the first nb batches; untimed)."""
+    import torch
+
+    import pxl
+
+    B = w.args.batch
+    tr = torch.zeros(2 * B, dtype=torch.int32, device=dev)
+    tot = 0
+    for b in range(nb):
+        pxl.render_fwd(w.grid, w.rays, w.order, start=b * B, length=B, trace=tr)
+        torch.cuda.synchronize()
+        tot += int(tr.view(-1, 2)[:, 0].long().sum().item())
+    return tot / nb
+
+
 def cpu_cores():
     """Host threads for the CPU baseline: the cores this process may run on,
     at most 16 (the host share of one GPU)."""
@@ -534,7 +555,7 @@ def proposal_quality(w, prop, dev, args):
     logits = torch.zeros((B, 64), dtype=torch.float32, device=dev)
     pxl.proposal_forward(prop, B, logits, pws)
     masks = torch.zeros(B, dtype=torch.int64, device=dev)
-    pxl.proposal_masks(logits, B, masks, threshold=0.05, dilate=1)
+    pxl.proposal_masks(logits, B, masks, threshold=args.prop_threshold, dilate=args.prop_dilate)
     gt = torch.stack([w.rays.gt_r[idx.long()], w.rays.gt_g[idx.long()],
                       w.rays.gt_b[idx.long()]], 1).double() / 255.0
     # Rays that miss the grid box are rendered black and carry no loss
@@ -553,6 +574,8 @@ def proposal_quality(w, prop, dev, args):
     res["pruned_frac"] = round(1.0 - res["proposal"]["samples"] / max(1, res["plain"]["samples"]), 4)
     res["psnr_delta"] = round(res["proposal"]["psnr"] - res["plain"]["psnr"], 4)
     res["interval"] = args.prop_interval
+    res["threshold"] = args.prop_threshold
+    res["dilate"] = args.prop_dilate
     log(f"proposal on held-out batch: {res}")
     return res
 
@@ -614,7 +637,8 @@ def main():
                 mse, psnr = w.trainer.loss()
                 log(f"pretrain {i + 1}: batch PSNR {psnr:.2f}, proposal BCE "
                     f"{w.trainer.proposal_loss() / (args.batch * 64):.4f} ({time.time() - t0:.1f}s)")
-        w.trainer.set_proposal(prop, interval=args.prop_interval, prune=True)
+        w.trainer.set_proposal(prop, interval=args.prop_interval, prune=True,
+                               threshold=args.prop_threshold, dilate=args.prop_dilate)
     elif args.pretrain:
         for _ in range(args.pretrain):
             w.step()
@@ -671,6 +695,12 @@ def main():
             "touched_records": int(vt), "alg_bytes": int(alg),
             "step_alg_bytes": int(step_alg),
             "step_frac": round(step_alg / (ms_step / 1e3) / 1e9 / HBM_PEAK_GBS, 5)}
+    # The L2-level figure, not HBM: per sample, the 8 corner records
+    # gathered (8 P s_p bytes) and the reference's 8 P float atomics at 8 B.
+    n_samp = samples_per_batch(w, dev)
+    per_sample = 8 * w.P * s_p + 8 * w.P * 8
+    roof["l2_level"] = {"samples_per_batch": int(n_samp), "bytes_per_sample": per_sample,
+                        "GBps": round(n_samp * per_sample / (kern_ms / 1e3) / 1e9, 1)}
 
     cpu = None
     if (not args.no_cpu_baseline and args.cpu_rays > 0 and not uneven and not args.proposal
