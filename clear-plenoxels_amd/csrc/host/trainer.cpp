@@ -138,7 +138,8 @@ extern "C" pxl_trainer *pxl_trainer_create(const pxl_train_config *cfg, const px
     tr->order_mode = cfg->batch_order;
     tr->sched.waves_per_block = cfg->waves_per_block > 0 ? cfg->waves_per_block : 4;
     tr->sched.persistent = cfg->persistent;
-    tr->sched.reserved[0] = tr->sched.reserved[1] = 0;
+    tr->sched.chunked = cfg->chunked;
+    tr->sched.reserved = 0;
     tr->tuned = !cfg->autotune;
     tr->prop_on = false;
     tr->prop_ws = nullptr;
@@ -147,10 +148,15 @@ extern "C" pxl_trainer *pxl_trainer_create(const pxl_train_config *cfg, const px
     tr->prop_valid = nullptr;
     tr->prop_loss = nullptr;
     const bool logged = cfg->fused && cfg->log_samples > 0;
-    if (!logged && (cfg->autotune || cfg->batch_order || cfg->persistent ||
+    if (!logged && (cfg->autotune || cfg->batch_order || cfg->persistent || cfg->chunked ||
                     (cfg->waves_per_block && cfg->waves_per_block != 4))) {
         set_error("pxl_trainer_create: batch orders and schedules need the logged step "
                   "(fused with log_samples > 0)");
+        delete tr;
+        return nullptr;
+    }
+    if (cfg->chunked && cfg->persistent) {
+        set_error("pxl_trainer_create: the chunk-parallel gradient has no persistent form");
         delete tr;
         return nullptr;
     }
@@ -284,8 +290,10 @@ static int32_t autotune(pxl_trainer *tr, int64_t start, int64_t len) {
     };
     std::vector<Cand> cands;
     for (int32_t order : {0, (int32_t)PXL_ORDER_MORTON, (int32_t)PXL_ORDER_LENGTH_MORTON})
-        for (int32_t wpb : {1, 2, 4})
-            for (int32_t pers : {0, 1}) cands.push_back({order, {wpb, pers, {0, 0}}});
+        for (int32_t wpb : {1, 2, 4}) {
+            for (int32_t pers : {0, 1}) cands.push_back({order, {wpb, pers, 0, 0}});
+            cands.push_back({order, {wpb, 0, 1, 0}});
+        }
     constexpr int kReps = 2;
     std::vector<hipEvent_t> ev(cands.size() * kReps * 2, nullptr);
     int32_t st = PXL_OK;

@@ -604,9 +604,12 @@ __device__ __forceinline__ double wave_scan_mul64(double v, int lane) {
 // those of sample_grads.  The scattering then runs as in the marcher's own
 // backward (register accumulators, corner carry), but with every input
 // already in registers: no global load follows an atomic until the next 64
-// samples, so the atomics never hold up the wave.
+// samples, so the atomics never hold up the wave.  chunk >= 0: only the
+// logged samples 64 chunk .. 64 chunk + 63, starting from the prefix
+// k_log_prefix stored for them (chunk-parallel schedule).
 template <int S, bool TGT>
-__device__ __forceinline__ void log_grad_ray(const RenderArgs &a, const int64_t b);
+__device__ __forceinline__ void log_grad_ray(const RenderArgs &a, const int64_t b,
+                                             const int chunk = -1);
 
 // TGT: also write the proposal targets (a.bin_targets, a.target_valid).
 template <int S, bool PERSIST = false, bool TGT = false>
@@ -623,8 +626,61 @@ __global__ __launch_bounds__(256) void k_log_grad(const RenderArgs a) {
     }
 }
 
+// Transmittance and the opacity / colour prefixes (T, A, C0, C1, C2) in
+// front of each 64-sample chunk of a ray's log after the first, for the
+// chunk-parallel schedule: the same float activations and FP64 wave scans,
+// in the same order, as log_grad_ray, so every chunk starts from the values
+// the one-wave-per-ray kernel carries into it.  One wave per batch entry;
+// entries with at most 64 logged samples have nothing to store.
+template <int S>
+__global__ __launch_bounds__(256) void k_log_prefix(const RenderArgs a) {
+    const int64_t b = (int64_t)xcd_slice_block(blockIdx.x, gridDim.x) * (blockDim.x >> 6) +
+                      (threadIdx.x >> 6);
+    if (b >= a.len) return;
+    const int lane = threadIdx.x & 63;
+    const int cnt = uniform_i(a.log_count[b]);
+    const float fstep = a.step;
+    double T = 1.0, Acc = 0.0, Cr = 0.0, Cg = 0.0, Cb = 0.0;
+    for (int c0 = 0; c0 + 64 < cnt; c0 += 64) {
+        const size_t at = ((size_t)b * (size_t)a.log_cap + (size_t)(c0 + lane)) * 8;
+        const float4 v = *reinterpret_cast<const float4 *>(a.log_val + at);
+        const float sigma = softplus_shift<float>(v.x);
+        const float alpha = one_minus_exp_neg<float>(sigma * fstep);
+        const double keep = 1.0 - (double)alpha;
+        const double incl = wave_scan_mul64(keep, lane);
+        double excl = __shfl_up(incl, 1, 64);
+        if (lane == 0) excl = 1.0;
+        const double ws = T * excl * (double)alpha;
+        const double Ai = Acc + wave_scan_add64(ws, lane);
+        const double H0 = Cr + wave_scan_add64(ws * (double)v.y, lane);
+        const double H1 = Cg + wave_scan_add64(ws * (double)v.z, lane);
+        const double H2 = Cb + wave_scan_add64(ws * (double)v.w, lane);
+        T *= readlane_d(incl, 63);
+        Acc = readlane_d(Ai, 63);
+        Cr = readlane_d(H0, 63);
+        Cg = readlane_d(H1, 63);
+        Cb = readlane_d(H2, 63);
+        double *pre = a.log_pre + ((size_t)b * (size_t)a.log_chunks + (size_t)(c0 / 64 + 1)) * 5;
+        if (lane < 5)
+            pre[lane] = lane == 0 ? T : (lane == 1 ? Acc : (lane == 2 ? Cr : (lane == 3 ? Cg : Cb)));
+    }
+}
+
+// Chunk-parallel gradient: one wave per (batch entry, 64-sample chunk of its
+// log), entry = item / log_chunks; the corner accumulators are flushed at
+// the end of each chunk.  For batches too small to fill the chip, where the
+// longest rays' chunks would otherwise run one after another.
+template <int S>
+__global__ __launch_bounds__(256) void k_log_grad_chunked(const RenderArgs a, int64_t n_items) {
+    const int64_t item = (int64_t)xcd_slice_block(blockIdx.x, gridDim.x) * (blockDim.x >> 6) +
+                         (threadIdx.x >> 6);
+    if (item >= n_items) return;
+    log_grad_ray<S, false>(a, item / a.log_chunks, (int)(item % a.log_chunks));
+}
+
 template <int S, bool TGT>
-__device__ __forceinline__ void log_grad_ray(const RenderArgs &a, const int64_t b) {
+__device__ __forceinline__ void log_grad_ray(const RenderArgs &a, const int64_t b,
+                                             const int chunk) {
     constexpr int P = 1 + 3 * S;
     const int lane = threadIdx.x & 63;
     const int cnt = uniform_i(a.log_count[b]);
@@ -635,6 +691,9 @@ __device__ __forceinline__ void log_grad_ray(const RenderArgs &a, const int64_t 
         }
         return;
     }
+    const int c_begin = chunk < 0 ? 0 : 64 * chunk;
+    if (c_begin >= cnt) return;
+    const int c_end = chunk < 0 ? cnt : min(cnt, c_begin + 64);
     const int ray = uniform_i(a.sample_index[a.start + b]);
     const double ox = a.rays.ox[ray], oy = a.rays.oy[ray], oz = a.rays.oz[ray];
     const float fdx = a.rays.dx[ray], fdy = a.rays.dy[ray], fdz = a.rays.dz[ray];
@@ -657,6 +716,14 @@ __device__ __forceinline__ void log_grad_ray(const RenderArgs &a, const int64_t 
     for (int k = 0; k < S; k++) ysel = (pp >= 1 && (pp - 1) % S == k) ? Y[k] : ysel;
 
     double T = 1.0, Acc = 0.0, Cr = 0.0, Cg = 0.0, Cb = 0.0;
+    if (chunk > 0) {
+        const double *pre = a.log_pre + ((size_t)b * (size_t)a.log_chunks + (size_t)chunk) * 5;
+        T = pre[0];
+        Acc = pre[1];
+        Cr = pre[2];
+        Cg = pre[3];
+        Cb = pre[4];
+    }
     bool has_cur = false;
     int cur_x = 0, cur_y = 0, cur_z = 0;
     uint32_t cur_rec[4] = {0u, 0u, 0u, 0u};
@@ -664,8 +731,8 @@ __device__ __forceinline__ void log_grad_ray(const RenderArgs &a, const int64_t 
     float acc0 = 0.0f;
     uint64_t tmask = 0ull;  // bins holding weight (proposal targets)
     const double tfar = a.rays.far_t[ray];
-    for (int c0 = 0; c0 < cnt; c0 += 64) {
-        const int m = min(64, cnt - c0);
+    for (int c0 = c_begin; c0 < c_end; c0 += 64) {
+        const int m = min(64, c_end - c0);
         const bool act = lane < m;
         const size_t at = ((size_t)b * (size_t)a.log_cap + (size_t)(c0 + (act ? lane : 0))) * 8;
         const float4 v = *reinterpret_cast<const float4 *>(a.log_val + at);

@@ -64,6 +64,11 @@ struct RenderArgs {
     float *log_val;
     int log_cap;
     int log_redo;  // kFused: only the batch entries whose count is -1
+    // Chunk-parallel gradient: per batch entry and 64-sample chunk of its log
+    // the FP64 (T, A, C0, C1, C2) in front of the chunk; log_chunks =
+    // ceil(log_cap / 64) chunks per entry.
+    double *log_pre;
+    int log_chunks;
     // Persistent launches: 8 claim counters per kernel (zeroed before launch).
     unsigned long long *work;
     // Learned proposal: per batch entry, the bins a ray may evaluate (null:
@@ -920,9 +925,10 @@ extern "C" int32_t pxl_render_fwd_trace(const pxl_grid *grid, const pxl_rays *ra
 namespace pxl {
 namespace {
 // Workspace of the two-kernel fused step: per batch entry 4 FP64 totals and a
-// count, per logged sample 8 records and 8 value words; 256-byte aligned.
+// count, per logged sample 8 records and 8 value words, per 64-sample chunk
+// of an entry's log 5 FP64 prefixes; 256-byte aligned.
 struct LogLayout {
-    size_t work, tot, count, rec, val, total;
+    size_t work, tot, count, rec, val, pre, total;
 };
 inline size_t align256(size_t x) { return (x + 255) / 256 * 256; }
 LogLayout log_layout(int64_t len, int32_t cap) {
@@ -938,6 +944,8 @@ LogLayout log_layout(int64_t len, int32_t cap) {
     o += align256((size_t)len * (size_t)cap * 8 * sizeof(uint32_t));
     L.val = o;
     o += align256((size_t)len * (size_t)cap * 8 * sizeof(float));
+    L.pre = o;
+    o += align256((size_t)len * (size_t)((cap + 63) / 64) * 5 * sizeof(double));
     L.total = o;
     return L;
 }
@@ -965,7 +973,7 @@ unsigned resident_blocks(K kernel, int threads) {
 // wpb waves per block; persistent launches size the grid to what is
 // resident and let the waves claim batch entries.
 template <int S, bool SPARSE, int STORE>
-void launch_logged(const RenderArgs &a, int wpb, bool persist, hipStream_t st) {
+void launch_logged(const RenderArgs &a, int wpb, bool persist, bool chunked, hipStream_t st) {
     const int threads = 64 * wpb;
     const unsigned blocks = (unsigned)((a.len + wpb - 1) / wpb);
     const bool tgt = a.bin_targets != nullptr;
@@ -983,6 +991,15 @@ void launch_logged(const RenderArgs &a, int wpb, bool persist, hipStream_t st) {
             const unsigned gb = std::min(blocks, resident_blocks(k_log_grad<S, true>, threads));
             hipLaunchKernelGGL((k_log_grad<S, true>), dim3(gb), dim3(threads), 0, st, a);
         }
+    } else if (chunked && !tgt) {
+        // Chunk-parallel gradient (the proposal's target words need the
+        // whole ray in one wave, so learning batches keep the per-ray kernel).
+        hipLaunchKernelGGL((k_march8<S, SPARSE, kLog, STORE>), dim3(blocks), dim3(threads), 0, st,
+                           a);
+        hipLaunchKernelGGL((k_log_prefix<S>), dim3(blocks), dim3(threads), 0, st, a);
+        const int64_t items = a.len * (int64_t)a.log_chunks;
+        hipLaunchKernelGGL((k_log_grad_chunked<S>), dim3((unsigned)((items + wpb - 1) / wpb)),
+                           dim3(threads), 0, st, a, items);
     } else {
         hipLaunchKernelGGL((k_march8<S, SPARSE, kLog, STORE>), dim3(blocks), dim3(threads), 0, st,
                            a);
@@ -998,11 +1015,11 @@ void launch_logged(const RenderArgs &a, int wpb, bool persist, hipStream_t st) {
 }
 template <int S>
 int32_t dispatch_logged_s(bool sparse, int store, const RenderArgs &a, int wpb, bool persist,
-                          hipStream_t st) {
-#define PXL_LOGGED(SP, ST)                                  \
-    if (sparse == SP && store == ST) {                      \
-        launch_logged<S, SP, ST>(a, wpb, persist, st);      \
-        return PXL_OK;                                      \
+                          bool chunked, hipStream_t st) {
+#define PXL_LOGGED(SP, ST)                                          \
+    if (sparse == SP && store == ST) {                              \
+        launch_logged<S, SP, ST>(a, wpb, persist, chunked, st);     \
+        return PXL_OK;                                              \
     }
     PXL_LOGGED(true, PXL_STORE_F32)
     PXL_LOGGED(true, PXL_STORE_BF16)
@@ -1015,11 +1032,11 @@ int32_t dispatch_logged_s(bool sparse, int store, const RenderArgs &a, int wpb, 
     return PXL_EINVAL;
 }
 int32_t dispatch_logged(int S, bool sparse, int store, const RenderArgs &a, int wpb,
-                        bool persist, hipStream_t st) {
+                        bool persist, bool chunked, hipStream_t st) {
     switch (S) {
-        case 1: return dispatch_logged_s<1>(sparse, store, a, wpb, persist, st);
-        case 4: return dispatch_logged_s<4>(sparse, store, a, wpb, persist, st);
-        case 9: return dispatch_logged_s<9>(sparse, store, a, wpb, persist, st);
+        case 1: return dispatch_logged_s<1>(sparse, store, a, wpb, persist, chunked, st);
+        case 4: return dispatch_logged_s<4>(sparse, store, a, wpb, persist, chunked, st);
+        case 9: return dispatch_logged_s<9>(sparse, store, a, wpb, persist, chunked, st);
         default:
             set_error("render: the logged fused step supports sh_dim 1, 4 and 9");
             return PXL_EINVAL;
@@ -1082,12 +1099,17 @@ static int32_t bwd_common(const pxl_grid *grid, const pxl_optim *optim, const px
         a.log_rec = reinterpret_cast<uint32_t *>(w + L.rec);
         a.log_val = reinterpret_cast<float *>(w + L.val);
         a.log_cap = max_samples;
+        a.log_pre = reinterpret_cast<double *>(w + L.pre);
+        a.log_chunks = (max_samples + 63) / 64;
         a.work = reinterpret_cast<unsigned long long *>(w + L.work);
         const int wpb = sched ? sched->waves_per_block : 4;
         PXL_REQUIRE(wpb == 1 || wpb == 2 || wpb == 4, "%s: waves_per_block %d (1, 2 or 4)", who,
                     wpb);
+        PXL_REQUIRE(!sched || !(sched->persistent && sched->chunked),
+                    "%s: the chunk-parallel gradient has no persistent form", who);
         st = dispatch_logged(grid->sh_dim, grid->sparse, grid->storage, a, wpb,
-                             sched && sched->persistent, as_stream(stream));
+                             sched && sched->persistent, sched && sched->chunked,
+                             as_stream(stream));
         return st != PXL_OK ? st : check_launch(who);
     }
     st = fused ? dispatch<kFused>(grid->sh_dim, grid->sparse, adapt, grid->storage, a,
@@ -1145,7 +1167,7 @@ extern "C" int32_t pxl_train_rays_prop(const pxl_grid *grid, const pxl_optim *op
         set_error("pxl_train_rays_prop: null workspace");
         return PXL_EINVAL;
     }
-    const pxl_schedule dflt = {4, 0, {0, 0}};
+    const pxl_schedule dflt = {4, 0, 0, 0};
     return bwd_common(grid, optim, rays, losses, sample_index, start, len, mask_coef,
                       huber_delta, stream, true, workspace, workspace_bytes, max_samples,
                       sched ? sched : &dflt, bin_mask, bin_targets, target_valid, target_wmin);

@@ -126,7 +126,7 @@ class TrainConfigC(ctypes.Structure):
                 ("variance_interval", _I32), ("optimize_step", _I32),
                 ("mask_coefficient", _F64), ("huber_delta", _F32), ("fused", _I32),
                 ("log_samples", _I32), ("autotune", _I32), ("batch_order", _I32),
-                ("waves_per_block", _I32), ("persistent", _I32), ("reserved", _I32)]
+                ("waves_per_block", _I32), ("persistent", _I32), ("chunked", _I32)]
 
 
 class ProposalC(ctypes.Structure):
@@ -134,7 +134,8 @@ class ProposalC(ctypes.Structure):
 
 
 class ScheduleC(ctypes.Structure):
-    _fields_ = [("waves_per_block", _I32), ("persistent", _I32), ("reserved", _I32 * 2)]
+    _fields_ = [("waves_per_block", _I32), ("persistent", _I32), ("chunked", _I32),
+                ("reserved", _I32)]
 
 
 def _declare(L):
@@ -499,12 +500,13 @@ ORDER_NONE, ORDER_MORTON, ORDER_LENGTH_MORTON = 0, 1, 2
 
 
 def train_rays_sched(grid, optim, rays, losses, sample_index, workspace, max_samples,
-                     waves_per_block=4, persistent=False, start=0, length=None, mask_coef=1.0,
-                     huber_delta=0.1, stream=None):
-    """train_rays_ws with an explicit launch schedule."""
+                     waves_per_block=4, persistent=False, chunked=False, start=0, length=None,
+                     mask_coef=1.0, huber_delta=0.1, stream=None):
+    """train_rays_ws with an explicit launch schedule (chunked: one gradient
+    wave per 64-sample chunk of each ray's log)."""
     length = sample_index.numel() - start if length is None else length
     g, o, r = grid.c(), optim.c(), rays.c()
-    sc = ScheduleC(waves_per_block, int(persistent), (_I32 * 2)(0, 0))
+    sc = ScheduleC(waves_per_block, int(persistent), int(chunked), 0)
     _check("pxl_train_rays_sched",
            lib().pxl_train_rays_sched(ctypes.byref(g), ctypes.byref(o), ctypes.byref(r),
                                       _ptr(losses), _ptr(sample_index), start, length, mask_coef,
@@ -771,7 +773,7 @@ class Trainer:
                  sh_coefficient=1e-6, variance_type=TV_DISPARITY, neighbour_nums=6,
                  variance_interval=3, optimize_step=0, mask_coefficient=1.0, huber_delta=0.1,
                  fused=True, log_samples=0, autotune=False, batch_order=0,
-                 waves_per_block=0, persistent=False, stream=None):
+                 waves_per_block=0, persistent=False, chunked=False, stream=None):
         import torch
 
         self._keep = (grid, optim, rays, order)
@@ -782,7 +784,7 @@ class Trainer:
                            sh_coefficient, variance_type, neighbour_nums, variance_interval,
                            optimize_step, mask_coefficient, huber_delta, int(fused),
                            int(log_samples), int(autotune), int(batch_order),
-                           int(waves_per_block), int(persistent), 0)
+                           int(waves_per_block), int(persistent), int(chunked))
         self._g, self._o, self._r = grid.c(), optim.c(), rays.c()
         n_rays = order.numel() if n_rays is None else n_rays
         sc = tv_scratch(grid, stream) if variance_type == TV_DISPARITY else None
@@ -836,11 +838,11 @@ class Trainer:
         return v.value
 
     def schedule(self):
-        """(batch order, waves per block, persistent) the trainer runs."""
+        """(batch order, waves per block, persistent, chunked) the trainer runs."""
         order, sc = _I32(), ScheduleC()
         _check("pxl_trainer_schedule",
                lib().pxl_trainer_schedule(self._h, ctypes.byref(order), ctypes.byref(sc)))
-        return order.value, sc.waves_per_block, bool(sc.persistent)
+        return order.value, sc.waves_per_block, bool(sc.persistent), bool(sc.chunked)
 
     def close(self):
         if getattr(self, "_h", None):

@@ -429,12 +429,17 @@ size_t pxl_train_rays_workspace_bytes(int64_t len, int32_t max_samples);
  * or 4; persistent != 0 launches as many workgroups as fit on the device at
  * once and lets each wave claim batch entries (XCD-sliced claim counters in
  * the workspace) until the batch is done, for batches of very uneven ray
- * lengths.  The results do not depend on the schedule beyond the order of
- * float atomics. */
+ * lengths; chunked != 0 (not with persistent) runs the gradient kernel as one
+ * wave per 64-sample chunk of each ray's log after a pass that stores each
+ * chunk's FP64 prefix, for batches too small to fill the device, where the
+ * longest rays would otherwise set the kernel's duration (batches with
+ * proposal targets keep one wave per ray).  The results do not depend on the
+ * schedule beyond the order of float atomics. */
 typedef struct pxl_schedule {
     int32_t waves_per_block;
     int32_t persistent;
-    int32_t reserved[2];
+    int32_t chunked;
+    int32_t reserved;
 } pxl_schedule;
 int32_t pxl_train_rays_ws(const pxl_grid *grid, const pxl_optim *optim,
                           const pxl_rays *rays, double *losses,
@@ -592,7 +597,8 @@ typedef struct pxl_train_config {
     int32_t batch_order;     /* 0 as given, PXL_ORDER_* (when not autotuned) */
     int32_t waves_per_block; /* 1, 2 or 4 (when not autotuned; 0 = 4) */
     int32_t persistent;      /* when not autotuned */
-    int32_t reserved;
+    int32_t chunked;         /* when not autotuned: chunk-parallel gradient
+                                (pxl_schedule), not with persistent */
 } pxl_train_config;
 
 typedef struct pxl_trainer pxl_trainer;

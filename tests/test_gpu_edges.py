@@ -1,6 +1,6 @@
 """Edge cases of the train step against the oracle, on every launch path
 (fused kernel, logged two-kernel step, logged step with rays redone past a
-short log, persistent single-wave blocks):
+short log, persistent single-wave blocks, chunk-parallel gradient):
 
 - ragged batch sizes: 1, 63, 65 and 257 rays (partial waves and blocks);
 - rays that miss the grid (near > far: outputs zeroed and no sample,
@@ -29,7 +29,7 @@ from helpers import (c1_planes, cameras, device_sparse_grid, from_records, grad_
 pytestmark = pytest.mark.gpu
 
 N, S, IMG = 48, 9, 48
-PATHS = ("fused", "logged", "redo", "persistent")
+PATHS = ("fused", "logged", "redo", "persistent", "chunked")
 
 
 def say(*a):
@@ -73,6 +73,8 @@ def _run(grid, rays, si, path, dev):
         if path == "persistent":
             pxl.train_rays_sched(grid, optim, rays, losses, si, ws, cap, waves_per_block=1,
                                  persistent=True)
+        elif path == "chunked":
+            pxl.train_rays_sched(grid, optim, rays, losses, si, ws, cap, chunked=True)
         else:
             pxl.train_rays_ws(grid, optim, rays, losses, si, ws, cap)
     torch.cuda.synchronize()

@@ -6,9 +6,9 @@ four 800 x 800 views of the upper hemisphere, random 8-bit targets.
   at 256^3;
 - 128 rays of the batch against the FP64 oracle on the full grid (colours
   1e-5, losses 255 * 1e-5, gradients helpers.grad_close), fused and logged;
-- the whole batch: the fused kernel, the logged step and the persistent
-  one-wave schedule give the same per-ray colours and losses, and gradients
-  equal up to the order of FP32 atomics;
+- the whole batch: the fused kernel, the logged step, the persistent
+  one-wave schedule and the chunk-parallel gradient give the same per-ray
+  colours and losses, and gradients equal up to the order of FP32 atomics;
 - the whole batch's gradient is the sum of its two halves' (no cross-ray
   interference, record 0 included);
 - TV over the full caged grid and one Adam step over all records against
@@ -126,6 +126,8 @@ def _train(c2, si, path):
         if path == "persistent":
             pxl.train_rays_sched(grid, optim, c2["rays"], losses, si, ws, 512, waves_per_block=1,
                                  persistent=True)
+        elif path == "chunked":
+            pxl.train_rays_sched(grid, optim, c2["rays"], losses, si, ws, 512, chunked=True)
         else:
             pxl.train_rays_ws(grid, optim, c2["rays"], losses, si, ws, 512)
     torch.cuda.synchronize()
@@ -167,11 +169,11 @@ def test_c2_schedules_agree(c2):
     import torch
 
     si = torch.from_numpy(c2["batch"]).to(c2["dev"])
-    say("[c2] full batch: fused, logged, persistent")
+    say("[c2] full batch: fused, logged, persistent, chunked")
     o_f, l_f, c_f = _train(c2, si, "fused")
     g_f = from_records(o_f.grad, P)
     del o_f
-    for path in ("logged", "persistent"):
+    for path in ("logged", "persistent", "chunked"):
         o, l, c = _train(c2, si, path)
         dc, dl = np.abs(c - c_f).max(), np.abs(l - l_f).max()
         worst = grad_close(from_records(o.grad, P), g_f)

@@ -13,9 +13,11 @@ samples per ray), random 8-bit targets.
   FP32 per-sample arithmetic then differs from FP64 by up to ~3e-3 relative
   on entries ~1e-3 of the plane maximum (~2e-6 of the maximum absolute);
 - the whole batch under every launch schedule the autotuner chooses from
-  (1 or 4 waves per block, persistent or not, given, Morton and
-  length + Morton order): identical per-ray colours and losses, gradients
-  equal up to the order of FP32 atomics;
+  (1 or 4 waves per block, persistent or not, the chunk-parallel gradient;
+  given, Morton and length + Morton order): identical per-ray colours and
+  losses, gradients equal up to the order of FP32 atomics (the bar of the
+  sampled-ray check: the same cancelling density entries also carry the
+  different rounding of differently grouped FP32 sums);
 - one Adam step over all records with the FP32 master copy against the
   oracle's FP32 Adam, the grid holding the master's round-to-nearest-even
   bfloat16 value.
@@ -108,7 +110,7 @@ def _grid(c4):
     return pxl.Grid(N, N, N, S, c4["rec16"], storage=pxl.STORE_BF16, **c4["grid_kw"])
 
 
-def _train(c4, si, wpb=4, persistent=False):
+def _train(c4, si, wpb=4, persistent=False, chunked=False):
     import torch
 
     grid = _grid(c4)
@@ -117,7 +119,7 @@ def _train(c4, si, wpb=4, persistent=False):
     losses = torch.zeros(n, dtype=torch.float64, device=c4["dev"])
     ws = pxl.train_workspace(n, 512, c4["dev"])
     pxl.train_rays_sched(grid, optim, c4["rays"], losses, si, ws, 512, waves_per_block=wpb,
-                         persistent=persistent)
+                         persistent=persistent, chunked=chunked)
     torch.cuda.synchronize()
     r = c4["rays"]
     idx = si.cpu().numpy()
@@ -176,18 +178,20 @@ def test_c4_schedules_agree(c4):
             order = si
         perm = order.cpu().numpy()
         assert np.array_equal(np.sort(perm), np.arange(BATCH))
-        for wpb, persist in ((1, False), (4, True), (1, True)):
-            if mode == 0 and wpb == 4 and not persist:
+        for wpb, persist, chunked in ((1, False, False), (4, True, False), (1, True, False),
+                                      (4, False, True)):
+            if mode == 0 and wpb == 4 and not persist and not chunked:
                 continue
-            o, l, c, idx = _train(c4, order, wpb, persist)
+            o, l, c, idx = _train(c4, order, wpb, persist, chunked)
             # losses are per batch position: map them back to ray ids
             l_by_ray = np.empty(BATCH)
             l_by_ray[idx] = l
             c_by_ray = np.empty_like(c_ref)
             c_by_ray[:, idx] = c
             dc, dl = np.abs(c_by_ray - c_ref).max(), np.abs(l_by_ray - l_ref).max()
-            worst = grad_close(from_records(o.grad, P), g_ref)
-            say(f"    {name}, {wpb} waves/block, persistent {persist}: colour diff {dc:.3g}, "
+            worst = grad_close(from_records(o.grad, P), g_ref, floor=0.1)
+            say(f"    {name}, {wpb} waves/block, persistent {persist}, chunked {chunked}: "
+                f"colour diff {dc:.3g}, "
                 f"loss diff {dl:.3g}, gradient / tolerance {worst:.3g}")
             del o
             assert dc == 0.0 and dl == 0.0

@@ -334,14 +334,16 @@ def test_batch_order(c1, mode):
     assert np.all(keys[1:] >= keys[:-1])
 
 
-@pytest.mark.parametrize("sched", [(1, False, 0), (2, True, 1), (4, True, 2), (4, False, 1)])
+@pytest.mark.parametrize("sched", [(1, False, 0, False), (2, True, 1, False), (4, True, 2, False),
+                                   (4, False, 1, False), (4, False, 0, True), (1, False, 2, True)])
 def test_logged_schedules(c1, sparse1, sched):
     """The two-kernel step under each schedule and batch order matches the
-    oracle (sparse 128^3, 1024 rays, log of 64 samples: some rays redone)."""
+    oracle (sparse 128^3, 1024 rays, log of 64 samples: some rays redone;
+    the chunk-parallel gradient with a log of 256: rays of up to 4 chunks)."""
     import torch
 
-    wpb, persist, mode = sched
-    say(f"[sched] waves/block {wpb}, persistent {persist}, order {mode}")
+    wpb, persist, mode, chunked = sched
+    say(f"[sched] waves/block {wpb}, persistent {persist}, order {mode}, chunked {chunked}")
     s = sparse1
     dev = c1["dev"]
     grid = device_sparse_grid(N1, S1, s["compact"], s["occ"], s["io"], s["cage"], s["c2d"], dev)
@@ -358,9 +360,10 @@ def test_logged_schedules(c1, sparse1, sched):
     batch = si.cpu().numpy().astype(np.int32)
     optim = pxl.Optim(grid)
     losses = torch.zeros(nr, dtype=torch.float64, device=dev)
-    ws = pxl.train_workspace(nr, 64, dev)
-    pxl.train_rays_sched(grid, optim, rays, losses, si, ws, 64, waves_per_block=wpb,
-                         persistent=persist)
+    cap = 256 if chunked else 64
+    ws = pxl.train_workspace(nr, cap, dev)
+    pxl.train_rays_sched(grid, optim, rays, losses, si, ws, cap, waves_per_block=wpb,
+                         chunked=chunked, persistent=persist)
     torch.cuda.synchronize()
     rays_o = oracle.Rays(c1["rays_np"])
     fwd = oracle.render_fwd(grid_o, rays_o, batch)
@@ -624,8 +627,9 @@ def test_trainer_matches_stage_calls(c1, log_samples, autotune):
         lr, sl = lr / d, sl / d
     torch.cuda.synchronize()
     if autotune:
-        order_, wpb_, pers_ = tr.schedule()
-        say(f"    autotuned: order {order_}, waves/block {wpb_}, persistent {pers_}")
+        order_, wpb_, pers_, chunk_ = tr.schedule()
+        say(f"    autotuned: order {order_}, waves/block {wpb_}, persistent {pers_}, "
+            f"chunked {chunk_}")
         assert order_ in (0, 1, 2) and wpb_ in (1, 2, 4)
     t_, lr_, sl_ = tr.state()
     assert t_ == 3 and abs(lr_ - lr) < 1e-15 and abs(sl_ - sl) < 1e-15
