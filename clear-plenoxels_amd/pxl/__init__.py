@@ -621,12 +621,13 @@ def render_fwd_masked(grid, rays, sample_index, bin_mask=None, start=0, length=N
 
 def train_rays_prop(grid, optim, rays, losses, sample_index, workspace, max_samples,
                     bin_mask=None, bin_targets=None, target_valid=None, target_wmin=1e-4,
-                    waves_per_block=4, persistent=False, start=0, length=None, mask_coef=1.0,
-                    huber_delta=0.1, stream=None):
-    """The two-kernel step with an optional proposal mask and target output."""
+                    waves_per_block=4, persistent=False, chunked=False, start=0, length=None,
+                    mask_coef=1.0, huber_delta=0.1, stream=None):
+    """The two-kernel step with an optional proposal mask and target output
+    (batches with targets run the gradient one wave per ray)."""
     length = sample_index.numel() - start if length is None else length
     g, o, r = grid.c(), optim.c(), rays.c()
-    sc = ScheduleC(waves_per_block, int(persistent), (_I32 * 2)(0, 0))
+    sc = ScheduleC(waves_per_block, int(persistent), int(chunked), 0)
     _check("pxl_train_rays_prop",
            lib().pxl_train_rays_prop(ctypes.byref(g), ctypes.byref(o), ctypes.byref(r),
                                      _ptr(losses), _ptr(sample_index), start, length, mask_coef,

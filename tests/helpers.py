@@ -101,6 +101,33 @@ def brick_bits(occ_bytes, W, H, D):
     return pack_bits(b.astype(np.uint8))
 
 
+def grad_close_alias(g, ref, rel=1e-4, floor=1e-3, rel0=1e-3):
+    """grad_close over every record but 0, plus record 0 within rel0 (the
+    floors from each plane's largest entry, record 0 included).  Two GPU
+    runs of a large caged batch differ at record 0 by the order of its FP32
+    atomics: it collects every out-of-cage corner of the batch (the caged
+    index maps them there), a sum of ~10^5 terms that largely cancel.
+    Returns the worst ratio."""
+    worst = 0.0
+    where = None
+    for p in range(ref.shape[0]):
+        scale = np.abs(ref[p]).max()
+        if scale == 0:
+            assert np.abs(g[p]).max() == 0, p
+            continue
+        tol = rel * (np.abs(ref[p]) + floor * scale)
+        tol[0] = rel0 * abs(ref[p][0]) + rel * floor * scale
+        r = np.abs(g[p] - ref[p]) / tol
+        i = int(np.argmax(r))
+        if r[i] > worst:
+            worst = float(r[i])
+            where = (p, i, float(g[p][i]), float(ref[p][i]), float(scale))
+    if worst > 1.0:
+        print(f"    worst at plane {where[0]} record {where[1]}: got {where[2]:.6g} "
+              f"ref {where[3]:.6g} (plane max {where[4]:.3g})", flush=True)
+    return worst
+
+
 def grad_close(g, ref, rel=1e-4, floor=1e-3):
     """Gradient parity bar: per plane p, |g - ref| <= rel * (|ref| +
     floor * max|ref[p]|) for every element.  Returns the worst ratio."""

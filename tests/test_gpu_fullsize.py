@@ -8,7 +8,9 @@ four 800 x 800 views of the upper hemisphere, random 8-bit targets.
   1e-5, losses 255 * 1e-5, gradients helpers.grad_close), fused and logged;
 - the whole batch: the fused kernel, the logged step, the persistent
   one-wave schedule and the chunk-parallel gradient give the same per-ray
-  colours and losses, and gradients equal up to the order of FP32 atomics;
+  colours and losses, and gradients equal up to the order of FP32 atomics
+  (record 0, where every out-of-cage corner lands, within 1e-3 relative:
+  helpers.grad_close_alias);
 - the whole batch's gradient is the sum of its two halves' (no cross-ray
   interference, record 0 included);
 - TV over the full caged grid and one Adam step over all records against
@@ -21,7 +23,7 @@ import pytest
 
 import oracle
 import pxl
-from helpers import from_records, grad_close, pack_bits
+from helpers import from_records, grad_close, grad_close_alias, pack_bits
 
 pytestmark = pytest.mark.gpu
 
@@ -176,7 +178,7 @@ def test_c2_schedules_agree(c2):
     for path in ("logged", "persistent", "chunked"):
         o, l, c = _train(c2, si, path)
         dc, dl = np.abs(c - c_f).max(), np.abs(l - l_f).max()
-        worst = grad_close(from_records(o.grad, P), g_f)
+        worst = grad_close_alias(from_records(o.grad, P), g_f)
         say(f"    {path}: colour diff {dc:.3g}, loss diff {dl:.3g}, gradient / tolerance {worst:.3g}")
         assert dc <= 1e-6 and dl <= 255 * 1e-6
         assert worst <= 1.0
@@ -196,7 +198,7 @@ def test_c2_gradient_additive(c2):
     del o_a
     o_b, _, _ = _train(c2, si[h:].clone(), "logged")
     g_sum += from_records(o_b.grad, P)
-    worst = grad_close(g_full, g_sum)
+    worst = grad_close_alias(g_full, g_sum)
     say(f"    worst difference / tolerance = {worst:.3g}")
     assert worst <= 1.0
 

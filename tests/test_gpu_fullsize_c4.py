@@ -17,7 +17,9 @@ samples per ray), random 8-bit targets.
   given, Morton and length + Morton order): identical per-ray colours and
   losses, gradients equal up to the order of FP32 atomics (the bar of the
   sampled-ray check: the same cancelling density entries also carry the
-  different rounding of differently grouped FP32 sums);
+  different rounding of differently grouped FP32 sums; record 0, the sum of
+  every out-of-cage corner of the batch, within 1e-3 relative:
+  helpers.grad_close_alias);
 - one Adam step over all records with the FP32 master copy against the
   oracle's FP32 Adam, the grid holding the master's round-to-nearest-even
   bfloat16 value.
@@ -29,7 +31,7 @@ import pytest
 
 import oracle
 import pxl
-from helpers import from_records, grad_close, round_bf16, widen_bf16
+from helpers import from_records, grad_close, grad_close_alias, round_bf16, widen_bf16
 
 pytestmark = [pytest.mark.gpu, pytest.mark.timeout(600)]
 
@@ -189,7 +191,7 @@ def test_c4_schedules_agree(c4):
             c_by_ray = np.empty_like(c_ref)
             c_by_ray[:, idx] = c
             dc, dl = np.abs(c_by_ray - c_ref).max(), np.abs(l_by_ray - l_ref).max()
-            worst = grad_close(from_records(o.grad, P), g_ref, floor=0.1)
+            worst = grad_close_alias(from_records(o.grad, P), g_ref, floor=0.1)
             say(f"    {name}, {wpb} waves/block, persistent {persist}, chunked {chunked}: "
                 f"colour diff {dc:.3g}, "
                 f"loss diff {dl:.3g}, gradient / tolerance {worst:.3g}")
