@@ -8,9 +8,11 @@ the plain method from the seeded ground-truth grid; the trained grid starts
 at density 0, SH 0; 65536 rays per batch, step 0.5 voxel.
 
 One step = one batch of Render::objectSceneLearner
-(SparseVolumeRadianceField/Render.cu:352-503): fused forward + backward
-(pxl_train_rays), FP64 batch loss on the GPU, the disparity TV gradient every
-third batch (sparse grids, VarianceLossInterval 3), Adam, LR decay.
+(SparseVolumeRadianceField/Render.cu:352-503): forward + backward as the
+two-kernel logged step (pxl_train_rays_sched; the trainer times every batch
+order and launch schedule on the first, untimed batch and keeps the
+fastest), FP64 batch loss on the GPU, the disparity TV gradient every third
+batch (sparse grids, VarianceLossInterval 3), Adam, LR decay.
 
 Prints one JSON line.  `value` is rays/s over the timed steps with all
 inputs resident in HBM.
@@ -53,8 +55,10 @@ def parse():
     ap.add_argument("--img", type=int, default=None)
     ap.add_argument("--batch", type=int, default=None)
     ap.add_argument("--occupancy", type=float, default=None)
-    ap.add_argument("--autotune", type=int, default=None,
-                    help="1: the trainer times every batch order x schedule on the first batch")
+    ap.add_argument("--autotune", type=int, default=1,
+                    help="1: the trainer times every batch order x schedule on the first "
+                         "(untimed) batch and keeps the fastest; 0: given order, 4 waves per "
+                         "block")
     ap.add_argument("--bf16-master", type=int, default=1,
                     help="with --storage bf16: Adam updates an FP32 master copy (1) or the "
                          "widened BF16 values themselves (0)")
@@ -91,7 +95,6 @@ def parse():
     a.views = a.views or (20 if dense else 100)
     a.img = a.img or (200 if dense else 800)
     a.occupancy = a.occupancy if a.occupancy is not None else (0.01 if uneven else 0.10)
-    a.autotune = a.autotune if a.autotune is not None else (1 if uneven else 0)
     if uneven and a.log_samples == 512:
         a.log_samples = 1024
     if a.pretrain is None:
