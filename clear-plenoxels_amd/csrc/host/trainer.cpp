@@ -139,7 +139,7 @@ extern "C" pxl_trainer *pxl_trainer_create(const pxl_train_config *cfg, const px
     tr->sched.waves_per_block = cfg->waves_per_block > 0 ? cfg->waves_per_block : 4;
     tr->sched.persistent = cfg->persistent;
     tr->sched.chunked = cfg->chunked;
-    tr->sched.reserved = 0;
+    tr->sched.grad_waves_per_block = 0;
     tr->tuned = !cfg->autotune;
     tr->prop_on = false;
     tr->prop_ws = nullptr;
@@ -289,10 +289,14 @@ static int32_t autotune(pxl_trainer *tr, int64_t start, int64_t len) {
         pxl_schedule s;
     };
     std::vector<Cand> cands;
+    // Waves per workgroup of the forward and (separately) of the gradient
+    // kernel; persistent launches share one shape; the chunk-parallel
+    // gradient with 4 waves per workgroup.
     for (int32_t order : {0, (int32_t)PXL_ORDER_MORTON, (int32_t)PXL_ORDER_LENGTH_MORTON})
         for (int32_t wpb : {1, 2, 4}) {
-            for (int32_t pers : {0, 1}) cands.push_back({order, {wpb, pers, 0, 0}});
-            cands.push_back({order, {wpb, 0, 1, 0}});
+            for (int32_t gwpb : {1, 2, 4}) cands.push_back({order, {wpb, 0, 0, gwpb}});
+            cands.push_back({order, {wpb, 1, 0, 0}});
+            cands.push_back({order, {wpb, 0, 1, 4}});
         }
     constexpr int kReps = 2;
     std::vector<hipEvent_t> ev(cands.size() * kReps * 2, nullptr);

@@ -973,9 +973,13 @@ unsigned resident_blocks(K kernel, int threads) {
 // wpb waves per block; persistent launches size the grid to what is
 // resident and let the waves claim batch entries.
 template <int S, bool SPARSE, int STORE>
-void launch_logged(const RenderArgs &a, int wpb, bool persist, bool chunked, hipStream_t st) {
+void launch_logged(const RenderArgs &a, int wpb, int gwpb, bool persist, bool chunked,
+                   hipStream_t st) {
     const int threads = 64 * wpb;
     const unsigned blocks = (unsigned)((a.len + wpb - 1) / wpb);
+    // The gradient kernels' own workgroup shape.
+    const int gthreads = 64 * gwpb;
+    const unsigned gblocks = (unsigned)((a.len + gwpb - 1) / gwpb);
     const bool tgt = a.bin_targets != nullptr;
     if (persist) {
         hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, a.work, 16);
@@ -985,28 +989,29 @@ void launch_logged(const RenderArgs &a, int wpb, bool persist, bool chunked, hip
                            st, a);
         if (tgt) {
             const unsigned gb =
-                std::min(blocks, resident_blocks(k_log_grad<S, true, true>, threads));
-            hipLaunchKernelGGL((k_log_grad<S, true, true>), dim3(gb), dim3(threads), 0, st, a);
+                std::min(gblocks, resident_blocks(k_log_grad<S, true, true>, gthreads));
+            hipLaunchKernelGGL((k_log_grad<S, true, true>), dim3(gb), dim3(gthreads), 0, st, a);
         } else {
-            const unsigned gb = std::min(blocks, resident_blocks(k_log_grad<S, true>, threads));
-            hipLaunchKernelGGL((k_log_grad<S, true>), dim3(gb), dim3(threads), 0, st, a);
+            const unsigned gb = std::min(gblocks, resident_blocks(k_log_grad<S, true>, gthreads));
+            hipLaunchKernelGGL((k_log_grad<S, true>), dim3(gb), dim3(gthreads), 0, st, a);
         }
     } else if (chunked && !tgt) {
         // Chunk-parallel gradient (the proposal's target words need the
         // whole ray in one wave, so learning batches keep the per-ray kernel).
         hipLaunchKernelGGL((k_march8<S, SPARSE, kLog, STORE>), dim3(blocks), dim3(threads), 0, st,
                            a);
-        hipLaunchKernelGGL((k_log_prefix<S>), dim3(blocks), dim3(threads), 0, st, a);
+        hipLaunchKernelGGL((k_log_prefix<S>), dim3(gblocks), dim3(gthreads), 0, st, a);
         const int64_t items = a.len * (int64_t)a.log_chunks;
-        hipLaunchKernelGGL((k_log_grad_chunked<S>), dim3((unsigned)((items + wpb - 1) / wpb)),
-                           dim3(threads), 0, st, a, items);
+        hipLaunchKernelGGL((k_log_grad_chunked<S>), dim3((unsigned)((items + gwpb - 1) / gwpb)),
+                           dim3(gthreads), 0, st, a, items);
     } else {
         hipLaunchKernelGGL((k_march8<S, SPARSE, kLog, STORE>), dim3(blocks), dim3(threads), 0, st,
                            a);
         if (tgt)
-            hipLaunchKernelGGL((k_log_grad<S, false, true>), dim3(blocks), dim3(threads), 0, st, a);
+            hipLaunchKernelGGL((k_log_grad<S, false, true>), dim3(gblocks), dim3(gthreads), 0, st,
+                               a);
         else
-            hipLaunchKernelGGL((k_log_grad<S>), dim3(blocks), dim3(threads), 0, st, a);
+            hipLaunchKernelGGL((k_log_grad<S>), dim3(gblocks), dim3(gthreads), 0, st, a);
     }
     RenderArgs redo = a;
     redo.log_redo = 1;
@@ -1014,12 +1019,12 @@ void launch_logged(const RenderArgs &a, int wpb, bool persist, bool chunked, hip
                        redo);
 }
 template <int S>
-int32_t dispatch_logged_s(bool sparse, int store, const RenderArgs &a, int wpb, bool persist,
-                          bool chunked, hipStream_t st) {
-#define PXL_LOGGED(SP, ST)                                          \
-    if (sparse == SP && store == ST) {                              \
-        launch_logged<S, SP, ST>(a, wpb, persist, chunked, st);     \
-        return PXL_OK;                                              \
+int32_t dispatch_logged_s(bool sparse, int store, const RenderArgs &a, int wpb, int gwpb,
+                          bool persist, bool chunked, hipStream_t st) {
+#define PXL_LOGGED(SP, ST)                                              \
+    if (sparse == SP && store == ST) {                                  \
+        launch_logged<S, SP, ST>(a, wpb, gwpb, persist, chunked, st);   \
+        return PXL_OK;                                                  \
     }
     PXL_LOGGED(true, PXL_STORE_F32)
     PXL_LOGGED(true, PXL_STORE_BF16)
@@ -1031,12 +1036,12 @@ int32_t dispatch_logged_s(bool sparse, int store, const RenderArgs &a, int wpb, 
     set_error("render: unknown storage %d", store);
     return PXL_EINVAL;
 }
-int32_t dispatch_logged(int S, bool sparse, int store, const RenderArgs &a, int wpb,
+int32_t dispatch_logged(int S, bool sparse, int store, const RenderArgs &a, int wpb, int gwpb,
                         bool persist, bool chunked, hipStream_t st) {
     switch (S) {
-        case 1: return dispatch_logged_s<1>(sparse, store, a, wpb, persist, chunked, st);
-        case 4: return dispatch_logged_s<4>(sparse, store, a, wpb, persist, chunked, st);
-        case 9: return dispatch_logged_s<9>(sparse, store, a, wpb, persist, chunked, st);
+        case 1: return dispatch_logged_s<1>(sparse, store, a, wpb, gwpb, persist, chunked, st);
+        case 4: return dispatch_logged_s<4>(sparse, store, a, wpb, gwpb, persist, chunked, st);
+        case 9: return dispatch_logged_s<9>(sparse, store, a, wpb, gwpb, persist, chunked, st);
         default:
             set_error("render: the logged fused step supports sh_dim 1, 4 and 9");
             return PXL_EINVAL;
@@ -1105,9 +1110,12 @@ static int32_t bwd_common(const pxl_grid *grid, const pxl_optim *optim, const px
         const int wpb = sched ? sched->waves_per_block : 4;
         PXL_REQUIRE(wpb == 1 || wpb == 2 || wpb == 4, "%s: waves_per_block %d (1, 2 or 4)", who,
                     wpb);
+        const int gwpb = sched && sched->grad_waves_per_block ? sched->grad_waves_per_block : wpb;
+        PXL_REQUIRE(gwpb == 1 || gwpb == 2 || gwpb == 4,
+                    "%s: grad_waves_per_block %d (0, 1, 2 or 4)", who, gwpb);
         PXL_REQUIRE(!sched || !(sched->persistent && sched->chunked),
                     "%s: the chunk-parallel gradient has no persistent form", who);
-        st = dispatch_logged(grid->sh_dim, grid->sparse, grid->storage, a, wpb,
+        st = dispatch_logged(grid->sh_dim, grid->sparse, grid->storage, a, wpb, gwpb,
                              sched && sched->persistent, sched && sched->chunked,
                              as_stream(stream));
         return st != PXL_OK ? st : check_launch(who);

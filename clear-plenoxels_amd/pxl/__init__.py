@@ -135,7 +135,7 @@ class ProposalC(ctypes.Structure):
 
 class ScheduleC(ctypes.Structure):
     _fields_ = [("waves_per_block", _I32), ("persistent", _I32), ("chunked", _I32),
-                ("reserved", _I32)]
+                ("grad_waves_per_block", _I32)]
 
 
 def _declare(L):
@@ -500,13 +500,14 @@ ORDER_NONE, ORDER_MORTON, ORDER_LENGTH_MORTON = 0, 1, 2
 
 
 def train_rays_sched(grid, optim, rays, losses, sample_index, workspace, max_samples,
-                     waves_per_block=4, persistent=False, chunked=False, start=0, length=None,
-                     mask_coef=1.0, huber_delta=0.1, stream=None):
+                     waves_per_block=4, persistent=False, chunked=False, grad_waves_per_block=0,
+                     start=0, length=None, mask_coef=1.0, huber_delta=0.1, stream=None):
     """train_rays_ws with an explicit launch schedule (chunked: one gradient
-    wave per 64-sample chunk of each ray's log)."""
+    wave per 64-sample chunk of each ray's log; grad_waves_per_block: the
+    gradient kernel's waves per workgroup, 0 = waves_per_block)."""
     length = sample_index.numel() - start if length is None else length
     g, o, r = grid.c(), optim.c(), rays.c()
-    sc = ScheduleC(waves_per_block, int(persistent), int(chunked), 0)
+    sc = ScheduleC(waves_per_block, int(persistent), int(chunked), int(grad_waves_per_block))
     _check("pxl_train_rays_sched",
            lib().pxl_train_rays_sched(ctypes.byref(g), ctypes.byref(o), ctypes.byref(r),
                                       _ptr(losses), _ptr(sample_index), start, length, mask_coef,
@@ -839,11 +840,13 @@ class Trainer:
         return v.value
 
     def schedule(self):
-        """(batch order, waves per block, persistent, chunked) the trainer runs."""
+        """(batch order, waves per block, persistent, chunked, gradient waves
+        per block) the trainer runs."""
         order, sc = _I32(), ScheduleC()
         _check("pxl_trainer_schedule",
                lib().pxl_trainer_schedule(self._h, ctypes.byref(order), ctypes.byref(sc)))
-        return order.value, sc.waves_per_block, bool(sc.persistent), bool(sc.chunked)
+        return (order.value, sc.waves_per_block, bool(sc.persistent), bool(sc.chunked),
+                sc.grad_waves_per_block or sc.waves_per_block)
 
     def close(self):
         if getattr(self, "_h", None):

@@ -426,7 +426,8 @@ int32_t pxl_count_touched(const pxl_grid *grid, const pxl_rays *rays,
 size_t pxl_train_rays_workspace_bytes(int64_t len, int32_t max_samples);
 
 /* Launch schedule of the two-kernel step: waves (rays) per workgroup, 1, 2
- * or 4; persistent != 0 launches as many workgroups as fit on the device at
+ * or 4, of the forward + log kernel and (grad_waves_per_block, 0 = the same)
+ * of the gradient kernel; persistent != 0 launches as many workgroups as fit on the device at
  * once and lets each wave claim batch entries (XCD-sliced claim counters in
  * the workspace) until the batch is done, for batches of very uneven ray
  * lengths; chunked != 0 (not with persistent) runs the gradient kernel as one
@@ -439,7 +440,7 @@ typedef struct pxl_schedule {
     int32_t waves_per_block;
     int32_t persistent;
     int32_t chunked;
-    int32_t reserved;
+    int32_t grad_waves_per_block;
 } pxl_schedule;
 int32_t pxl_train_rays_ws(const pxl_grid *grid, const pxl_optim *optim,
                           const pxl_rays *rays, double *losses,

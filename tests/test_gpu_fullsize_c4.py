@@ -112,7 +112,7 @@ def _grid(c4):
     return pxl.Grid(N, N, N, S, c4["rec16"], storage=pxl.STORE_BF16, **c4["grid_kw"])
 
 
-def _train(c4, si, wpb=4, persistent=False, chunked=False):
+def _train(c4, si, wpb=4, persistent=False, chunked=False, gwpb=0):
     import torch
 
     grid = _grid(c4)
@@ -121,7 +121,7 @@ def _train(c4, si, wpb=4, persistent=False, chunked=False):
     losses = torch.zeros(n, dtype=torch.float64, device=c4["dev"])
     ws = pxl.train_workspace(n, 512, c4["dev"])
     pxl.train_rays_sched(grid, optim, c4["rays"], losses, si, ws, 512, waves_per_block=wpb,
-                         persistent=persistent, chunked=chunked)
+                         persistent=persistent, chunked=chunked, grad_waves_per_block=gwpb)
     torch.cuda.synchronize()
     r = c4["rays"]
     idx = si.cpu().numpy()
@@ -180,11 +180,10 @@ def test_c4_schedules_agree(c4):
             order = si
         perm = order.cpu().numpy()
         assert np.array_equal(np.sort(perm), np.arange(BATCH))
-        for wpb, persist, chunked in ((1, False, False), (4, True, False), (1, True, False),
-                                      (4, False, True)):
-            if mode == 0 and wpb == 4 and not persist and not chunked:
-                continue
-            o, l, c, idx = _train(c4, order, wpb, persist, chunked)
+        for wpb, persist, chunked, gwpb in ((1, False, False, 0), (1, False, False, 4),
+                                            (4, True, False, 0), (1, True, False, 0),
+                                            (4, False, True, 0)):
+            o, l, c, idx = _train(c4, order, wpb, persist, chunked, gwpb)
             # losses are per batch position: map them back to ray ids
             l_by_ray = np.empty(BATCH)
             l_by_ray[idx] = l
@@ -192,7 +191,8 @@ def test_c4_schedules_agree(c4):
             c_by_ray[:, idx] = c
             dc, dl = np.abs(c_by_ray - c_ref).max(), np.abs(l_by_ray - l_ref).max()
             worst = grad_close_alias(from_records(o.grad, P), g_ref, floor=0.1)
-            say(f"    {name}, {wpb} waves/block, persistent {persist}, chunked {chunked}: "
+            say(f"    {name}, {wpb} waves/block (gradient {gwpb or wpb}), persistent {persist}, "
+                f"chunked {chunked}: "
                 f"colour diff {dc:.3g}, "
                 f"loss diff {dl:.3g}, gradient / tolerance {worst:.3g}")
             del o

@@ -334,16 +334,19 @@ def test_batch_order(c1, mode):
     assert np.all(keys[1:] >= keys[:-1])
 
 
-@pytest.mark.parametrize("sched", [(1, False, 0, False), (2, True, 1, False), (4, True, 2, False),
-                                   (4, False, 1, False), (4, False, 0, True), (1, False, 2, True)])
+@pytest.mark.parametrize("sched", [(1, False, 0, False, 0), (2, True, 1, False, 4),
+                                   (4, True, 2, False, 0), (4, False, 1, False, 1),
+                                   (1, False, 0, False, 4), (4, False, 0, True, 0),
+                                   (1, False, 2, True, 2)])
 def test_logged_schedules(c1, sparse1, sched):
     """The two-kernel step under each schedule and batch order matches the
     oracle (sparse 128^3, 1024 rays, log of 64 samples: some rays redone;
     the chunk-parallel gradient with a log of 256: rays of up to 4 chunks)."""
     import torch
 
-    wpb, persist, mode, chunked = sched
-    say(f"[sched] waves/block {wpb}, persistent {persist}, order {mode}, chunked {chunked}")
+    wpb, persist, mode, chunked, gwpb = sched
+    say(f"[sched] waves/block {wpb} (gradient {gwpb or wpb}), persistent {persist}, "
+        f"order {mode}, chunked {chunked}")
     s = sparse1
     dev = c1["dev"]
     grid = device_sparse_grid(N1, S1, s["compact"], s["occ"], s["io"], s["cage"], s["c2d"], dev)
@@ -363,7 +366,7 @@ def test_logged_schedules(c1, sparse1, sched):
     cap = 256 if chunked else 64
     ws = pxl.train_workspace(nr, cap, dev)
     pxl.train_rays_sched(grid, optim, rays, losses, si, ws, cap, waves_per_block=wpb,
-                         chunked=chunked, persistent=persist)
+                         chunked=chunked, persistent=persist, grad_waves_per_block=gwpb)
     torch.cuda.synchronize()
     rays_o = oracle.Rays(c1["rays_np"])
     fwd = oracle.render_fwd(grid_o, rays_o, batch)
@@ -627,7 +630,7 @@ def test_trainer_matches_stage_calls(c1, log_samples, autotune):
         lr, sl = lr / d, sl / d
     torch.cuda.synchronize()
     if autotune:
-        order_, wpb_, pers_, chunk_ = tr.schedule()
+        order_, wpb_, pers_, chunk_, gwpb_ = tr.schedule()
         say(f"    autotuned: order {order_}, waves/block {wpb_}, persistent {pers_}, "
             f"chunked {chunk_}")
         assert order_ in (0, 1, 2) and wpb_ in (1, 2, 4)
