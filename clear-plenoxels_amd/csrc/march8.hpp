@@ -225,6 +225,10 @@ __device__ __forceinline__ int64_t claim_ray(unsigned long long *ctr, int64_t le
 template <int S, bool SPARSE, int MODE, int STORE>
 __device__ __forceinline__ void march8_ray(const RenderArgs &a, const int64_t b);
 
+// Workspace word counting the batch entries longer than the sample log
+// (after the 8 + 8 claim counters of the persistent launches).
+constexpr int kRedoCounter = 16;
+
 // One wave per ray: either the block's waves take consecutive batch entries
 // of an XCD-sliced block order, or (a.persist) a fixed grid of waves claims
 // entries until the batch is done.
@@ -240,6 +244,17 @@ __global__ __launch_bounds__(256) void k_march8(const RenderArgs a) {
                           (threadIdx.x >> 6);
         if (b < a.len) march8_ray<S, SPARSE, MODE, STORE>(a, b);
     }
+}
+
+// The fused marcher over the batch entries the logged step could not hold
+// (the list k_march8<kLog> appends to): a small grid of waves striding over
+// the list, which is empty in most batches.
+template <int S, bool SPARSE, int STORE>
+__global__ __launch_bounds__(256) void k_march8_redo(const RenderArgs a) {
+    const int64_t n = (int64_t)a.work[kRedoCounter];
+    const int64_t nw = (int64_t)gridDim.x * (blockDim.x >> 6);
+    for (int64_t i = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); i < n; i += nw)
+        march8_ray<S, SPARSE, kFused, STORE>(a, uniform_i(a.redo_list[i]));
 }
 
 template <int S, bool SPARSE, int MODE, int STORE>
@@ -570,8 +585,12 @@ __device__ __forceinline__ void march8_ray(const RenderArgs &a, const int64_t b)
                     }
                     return;
                 }
-                // Longer than the log: the fused marcher redoes this ray.
-                if (lane == 0) a.log_count[b] = -1;
+                // Longer than the log: the fused marcher redoes this ray
+                // (k_march8_redo walks the list of such entries).
+                if (lane == 0) {
+                    a.log_count[b] = -1;
+                    a.redo_list[atomicAdd(a.work + kRedoCounter, 1ull)] = (int32_t)b;
+                }
                 return;
             }
         }

@@ -69,6 +69,9 @@ struct RenderArgs {
     // ceil(log_cap / 64) chunks per entry.
     double *log_pre;
     int log_chunks;
+    // Batch entries longer than the log, appended by k_march8<kLog>; their
+    // count is work[kRedoCounter].
+    int32_t *redo_list;
     // Persistent launches: 8 claim counters per kernel (zeroed before launch).
     unsigned long long *work;
     // Learned proposal: per batch entry, the bins a ray may evaluate (null:
@@ -928,7 +931,7 @@ namespace {
 // count, per logged sample 8 records and 8 value words, per 64-sample chunk
 // of an entry's log 5 FP64 prefixes; 256-byte aligned.
 struct LogLayout {
-    size_t work, tot, count, rec, val, pre, total;
+    size_t work, tot, count, rec, val, pre, redo, total;
 };
 inline size_t align256(size_t x) { return (x + 255) / 256 * 256; }
 LogLayout log_layout(int64_t len, int32_t cap) {
@@ -946,6 +949,8 @@ LogLayout log_layout(int64_t len, int32_t cap) {
     o += align256((size_t)len * (size_t)cap * 8 * sizeof(float));
     L.pre = o;
     o += align256((size_t)len * (size_t)((cap + 63) / 64) * 5 * sizeof(double));
+    L.redo = o;
+    o += align256((size_t)len * sizeof(int32_t));
     L.total = o;
     return L;
 }
@@ -969,9 +974,10 @@ unsigned resident_blocks(K kernel, int threads) {
 }
 
 // Forward + log, gradient of the logged rays, then the fused marcher again
-// for the rays longer than the log (its waves leave at once for the others).
+// for the rays longer than the log (the list the forward appended to).
 // wpb waves per block; persistent launches size the grid to what is
-// resident and let the waves claim batch entries.
+// resident and let the waves claim batch entries.  The workspace's first
+// 17 words (claim counters, redo count) start at zero.
 template <int S, bool SPARSE, int STORE>
 void launch_logged(const RenderArgs &a, int wpb, int gwpb, bool persist, bool chunked,
                    hipStream_t st) {
@@ -981,8 +987,8 @@ void launch_logged(const RenderArgs &a, int wpb, int gwpb, bool persist, bool ch
     const int gthreads = 64 * gwpb;
     const unsigned gblocks = (unsigned)((a.len + gwpb - 1) / gwpb);
     const bool tgt = a.bin_targets != nullptr;
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, a.work, kRedoCounter + 1);
     if (persist) {
-        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, a.work, 16);
         const unsigned ga =
             std::min(blocks, resident_blocks(k_march8<S, SPARSE, kLog, STORE, true>, threads));
         hipLaunchKernelGGL((k_march8<S, SPARSE, kLog, STORE, true>), dim3(ga), dim3(threads), 0,
@@ -1015,8 +1021,8 @@ void launch_logged(const RenderArgs &a, int wpb, int gwpb, bool persist, bool ch
     }
     RenderArgs redo = a;
     redo.log_redo = 1;
-    hipLaunchKernelGGL((k_march8<S, SPARSE, kFused, STORE>), dim3(blocks), dim3(threads), 0, st,
-                       redo);
+    hipLaunchKernelGGL((k_march8_redo<S, SPARSE, STORE>), dim3(std::min(blocks, 1024u)),
+                       dim3(threads), 0, st, redo);
 }
 template <int S>
 int32_t dispatch_logged_s(bool sparse, int store, const RenderArgs &a, int wpb, int gwpb,
@@ -1105,6 +1111,7 @@ static int32_t bwd_common(const pxl_grid *grid, const pxl_optim *optim, const px
         a.log_val = reinterpret_cast<float *>(w + L.val);
         a.log_cap = max_samples;
         a.log_pre = reinterpret_cast<double *>(w + L.pre);
+        a.redo_list = reinterpret_cast<int32_t *>(w + L.redo);
         a.log_chunks = (max_samples + 63) / 64;
         a.work = reinterpret_cast<unsigned long long *>(w + L.work);
         const int wpb = sched ? sched->waves_per_block : 4;
