@@ -419,6 +419,7 @@ struct AliasHeader {
     uint32_t magic;
     uint32_t n_alias;
     int64_t n_vox;
+    uint32_t n_list;  // records with w_r != 0 (the compact list)
 };
 constexpr uint32_t kAliasMagic = 0x70786131u;
 constexpr int kAliasBlocks = 1024;
@@ -430,6 +431,92 @@ __host__ __device__ inline size_t alias_partials_off(int64_t n_vox) {
 }
 __host__ __device__ inline size_t tv_table_off(int64_t n_vox) {
     return alias_partials_off(n_vox) + (size_t)kAliasBlocks * 64 * sizeof(double);
+}
+// After the neighbour table: the records with w_r != 0 in record order,
+// (record, w_r) pairs, and the per-block counts of the compaction.
+constexpr int kListPerBlock = 4096;  // 256 threads x 16 records
+__host__ __device__ inline size_t alias_list_off(int64_t n_vox) {
+    return tv_table_off(n_vox) + (size_t)n_vox * kTableWords * sizeof(uint32_t);
+}
+__host__ __device__ inline size_t alias_counts_off(int64_t n_vox) {
+    return alias_list_off(n_vox) + ((size_t)n_vox * 8 + 255) / 256 * 256;
+}
+__host__ __device__ inline size_t tv_scratch_total(int64_t n_vox) {
+    const size_t nb = (size_t)((n_vox + kListPerBlock - 1) / kListPerBlock);
+    return alias_counts_off(n_vox) + (nb * sizeof(uint32_t) + 255) / 256 * 256;
+}
+
+// Compaction of the records with w_r != 0, in record order (so the
+// weighted sums below run in a fixed order).  Thread t of block b looks at
+// records b * 4096 + 16 t .. + 15.
+__device__ __forceinline__ uint32_t alias_flags16(const float *w, int64_t n, int64_t base) {
+    uint32_t f = 0u;
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        const int64_t r = base + k;
+        if (r < n && w[r] != 0.0f) f |= 1u << k;
+    }
+    return f;
+}
+
+__global__ __launch_bounds__(256) void k_alias_count(const float *w, int64_t n,
+                                                     uint32_t *counts) {
+    __shared__ uint32_t part[4];
+    const int64_t base = (int64_t)blockIdx.x * kListPerBlock + 16 * (int64_t)threadIdx.x;
+    const uint32_t c = wave_sum_u32((uint32_t)__builtin_popcount(alias_flags16(w, n, base)));
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) counts[blockIdx.x] = part[0] + part[1] + part[2] + part[3];
+}
+
+// One block: exclusive scan of the block counts in place, total to the header.
+__global__ __launch_bounds__(1024) void k_alias_scan(uint32_t *counts, int64_t nb,
+                                                     AliasHeader *hdr) {
+    __shared__ uint32_t lds[1024];
+    uint32_t carry = 0u;
+    for (int64_t c0 = 0; c0 < nb; c0 += 1024) {
+        const int64_t i = c0 + threadIdx.x;
+        const uint32_t v = i < nb ? counts[i] : 0u;
+        lds[threadIdx.x] = v;
+        __syncthreads();
+        for (int o = 1; o < 1024; o <<= 1) {  // inclusive Hillis-Steele scan
+            const uint32_t x = (int)threadIdx.x >= o ? lds[threadIdx.x - o] : 0u;
+            __syncthreads();
+            lds[threadIdx.x] += x;
+            __syncthreads();
+        }
+        if (i < nb) counts[i] = carry + lds[threadIdx.x] - v;
+        carry += lds[1023];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) hdr->n_list = carry;
+}
+
+__global__ __launch_bounds__(256) void k_alias_list(const float *w, int64_t n,
+                                                    const uint32_t *offsets, uint2 *list) {
+    __shared__ uint32_t wsum[4];
+    const int64_t base = (int64_t)blockIdx.x * kListPerBlock + 16 * (int64_t)threadIdx.x;
+    const uint32_t f = alias_flags16(w, n, base);
+    const uint32_t c = (uint32_t)__builtin_popcount(f);
+    // Exclusive prefix of c over the block, in thread order.
+    uint32_t incl = c;
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t x = (uint32_t)__shfl_up((int)incl, o, 64);
+        if (lane >= o) incl += x;
+    }
+    if (lane == 63) wsum[threadIdx.x >> 6] = incl;
+    __syncthreads();
+    uint32_t before = 0u;
+    for (int k = 0; k < (int)(threadIdx.x >> 6); k++) before += wsum[k];
+    uint32_t at = offsets[blockIdx.x] + before + incl - c;
+#pragma unroll
+    for (int k = 0; k < 16; k++)
+        if ((f >> k) & 1u) {
+            const int64_t r = base + k;
+            list[at++] = make_uint2((uint32_t)r, __float_as_uint(w[r]));
+        }
 }
 
 __global__ __launch_bounds__(256) void k_alias_weights(const TvArgs a, int64_t ncell,
@@ -456,31 +543,40 @@ __global__ __launch_bounds__(256) void k_alias_weights(const TvArgs a, int64_t n
     atomicAdd(&hdr->n_alias, (uint32_t)n);
 }
 
-// Per-block partial sums of w_r * u_r[p]; LPV lanes per record, one per
-// parameter.
+// Per-block partial sums of w_r * u_r[p] over the compact list of records
+// with w_r != 0; LPV lanes per list entry, one per parameter.
 template <int LPV, int STORE>
-__global__ __launch_bounds__(256) void k_alias_partial(const TvArgs a, const float *w,
-                                                       double *partials) {
+__global__ __launch_bounds__(256) void k_alias_partial(const TvArgs a, const AliasHeader *hdr,
+                                                       const uint2 *list, double *partials) {
     constexpr int kGroups = 256 / LPV;
     __shared__ double red[256];
     const int g = threadIdx.x / LPV, p = threadIdx.x % LPV;
     double acc = 0.0;
+    // Scratch not prepared for this grid: no list (k_alias_final adds nothing
+    // either); an entry naming a record beyond the grid reads record 0.
+    const bool ok = hdr->magic == kAliasMagic && hdr->n_vox == a.n_vox;
+    const int64_t n = ok ? min((int64_t)hdr->n_list, a.n_vox) : 0;
     if (p < a.P) {
-        // Four records per step with unconditional loads, so the weight and
-        // value loads of a step are in flight together.
+        // Four entries per step; an entry past the end reads entry 0 with
+        // weight 0, so every step's loads are in flight together.
         const int64_t step = (int64_t)gridDim.x * kGroups;
-        for (int64_t r0 = (int64_t)blockIdx.x * kGroups + g; r0 < a.n_vox; r0 += 4 * step) {
-            float wr[4], ur[4];
+        for (int64_t e0 = (int64_t)blockIdx.x * kGroups + g; e0 < n; e0 += 4 * step) {
+            uint2 ent[4];
 #pragma unroll
             for (int k = 0; k < 4; k++) {
-                const int64_t r = r0 + k * step;
-                const int64_t rc = r < a.n_vox ? r : 0;
-                wr[k] = r < a.n_vox ? w[rc] : 0.0f;
-                ur[k] = ld<STORE>(a.params, (size_t)rc * a.stride + p);
+                const int64_t e = e0 + k * step;
+                ent[k] = list[e < n ? e : 0];
+                if (e >= n) ent[k].y = 0u;
+            }
+            float ur[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const uint32_t rec = (int64_t)ent[k].x < a.n_vox ? ent[k].x : 0u;
+                ur[k] = ld<STORE>(a.params, (size_t)rec * a.stride + p);
             }
 #pragma unroll
             for (int k = 0; k < 4; k++)
-                if (wr[k] != 0.0f) acc += (double)wr[k] * (double)ur[k];
+                if (ent[k].y != 0u) acc += (double)__uint_as_float(ent[k].y) * (double)ur[k];
         }
     }
     red[threadIdx.x] = acc;
@@ -492,25 +588,29 @@ __global__ __launch_bounds__(256) void k_alias_partial(const TvArgs a, const flo
     }
 }
 
-// One block of 1024 threads: thread t sums the partials of parameter t % 64
-// over the blocks b = t / 64 (mod 16), then 16 lanes per parameter fold in
-// LDS.
+// One block of 256 threads per parameter p (blockIdx.x): thread t sums the
+// partials of blocks t, t + 256, t + 512, t + 768 (loads in flight
+// together), then the block folds the 256 sums in LDS in a fixed order.
 template <int STORE>
-__global__ __launch_bounds__(1024) void k_alias_final(const TvArgs a, const AliasHeader *hdr,
-                                                      const double *partials, int nblocks) {
-    __shared__ double red[1024];
-    const int p = threadIdx.x & 63, c = threadIdx.x >> 6;
-    double s = 0.0;
-    for (int b = c; b < nblocks; b += 16) s += partials[(size_t)b * 64 + p];
-    red[threadIdx.x] = s;
+__global__ __launch_bounds__(256) void k_alias_final(const TvArgs a, const AliasHeader *hdr,
+                                                     const double *partials) {
+    static_assert(kAliasBlocks == 4 * 256, "four partials per thread");
+    __shared__ double red[256];
+    const int p = blockIdx.x, t = threadIdx.x;
+    double v[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) v[k] = partials[(size_t)(t + 256 * k) * 64 + p];
+    red[t] = (v[0] + v[1]) + (v[2] + v[3]);
     __syncthreads();
-    if (threadIdx.x >= 64 || p >= a.P) return;
+    for (int h = 128; h > 0; h >>= 1) {
+        if (t < h) red[t] += red[t + h];
+        __syncthreads();
+    }
+    if (t != 0 || p >= a.P) return;
     const float lam = p == 0 ? a.lam_d : a.lam_sh;
     if (lam == 0.0f || hdr->magic != kAliasMagic || hdr->n_vox != a.n_vox) return;
-    double t = 0.0;
-    for (int k = 0; k < 16; k++) t += red[k * 64 + p];
     const double u0 = (double)ld<STORE>(a.params, (size_t)p);
-    a.grad[p] += (float)((double)lam * ((double)hdr->n_alias * u0 - t));
+    a.grad[p] += (float)((double)lam * ((double)hdr->n_alias * u0 - red[0]));
 }
 
 __global__ void k_alias_init(AliasHeader *hdr, float *w, int64_t n_vox) {
@@ -773,16 +873,16 @@ void tv_launch(const TvArgs &a, int type, void *scratch, hipStream_t s) {
         if (a.sparse) {
             char *base = reinterpret_cast<char *>(scratch);
             const AliasHeader *hdr = reinterpret_cast<const AliasHeader *>(base);
-            const float *w = reinterpret_cast<const float *>(base + alias_weights_off());
+            const uint2 *list = reinterpret_cast<const uint2 *>(base + alias_list_off(a.n_vox));
             double *part = reinterpret_cast<double *>(base + alias_partials_off(a.n_vox));
             if (wide)
                 hipLaunchKernelGGL((k_alias_partial<64, STORE>), dim3(kAliasBlocks), dim3(256), 0,
-                                   s, a, w, part);
+                                   s, a, hdr, list, part);
             else
                 hipLaunchKernelGGL((k_alias_partial<32, STORE>), dim3(kAliasBlocks), dim3(256), 0,
-                                   s, a, w, part);
-            hipLaunchKernelGGL(k_alias_final<STORE>, dim3(1), dim3(1024), 0, s, a, hdr, part,
-                               kAliasBlocks);
+                                   s, a, hdr, list, part);
+            hipLaunchKernelGGL(k_alias_final<STORE>, dim3((unsigned)a.P), dim3(256), 0, s, a, hdr,
+                               part);
         }
     } else {
         if (wide) hipLaunchKernelGGL((k_tv_neighbour6<64, STORE>), dim3(blocks), dim3(256), 0, s, a);
@@ -797,7 +897,7 @@ using namespace pxl;
 
 extern "C" size_t pxl_tv_scratch_bytes(const pxl_grid *grid) {
     if (!grid || !grid->sparse || grid->n_vox <= 0) return 0;
-    return tv_table_off(grid->n_vox) + (size_t)grid->n_vox * kTableWords * sizeof(uint32_t);
+    return tv_scratch_total(grid->n_vox);
 }
 
 static void fill_tv_args(const pxl_grid *grid, const pxl_optim *optim, TvArgs &a) {
@@ -841,6 +941,13 @@ extern "C" int32_t pxl_tv_prepare(const pxl_grid *grid, void *scratch, size_t sc
     const int64_t ncell = (int64_t)(a.W - 1) * (a.H - 1) * (a.D - 1);
     hipLaunchKernelGGL(k_alias_weights, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, s, a,
                        ncell, hdr, w);
+    const int64_t nb = (a.n_vox + kListPerBlock - 1) / kListPerBlock;
+    uint32_t *counts = reinterpret_cast<uint32_t *>(base + alias_counts_off(a.n_vox));
+    uint2 *list = reinterpret_cast<uint2 *>(base + alias_list_off(a.n_vox));
+    hipLaunchKernelGGL(k_alias_count, dim3((unsigned)nb), dim3(256), 0, s, w, a.n_vox, counts);
+    hipLaunchKernelGGL(k_alias_scan, dim3(1), dim3(1024), 0, s, counts, nb, hdr);
+    hipLaunchKernelGGL(k_alias_list, dim3((unsigned)nb), dim3(256), 0, s, w, a.n_vox, counts,
+                       list);
     uint32_t *table = reinterpret_cast<uint32_t *>(base + tv_table_off(a.n_vox));
     hipLaunchKernelGGL(k_tv_table_build<32>, dim3((unsigned)((a.n_vox + 7) / 8)), dim3(256), 0, s,
                        a, table);
