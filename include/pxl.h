@@ -249,8 +249,9 @@ int32_t pxl_train_rays(const pxl_grid *grid, const pxl_optim *optim,
  * scratch: sparse grids only - pxl_tv_scratch_bytes() bytes of device
  *   memory, 256-byte aligned, filled once by pxl_tv_prepare for the grid's
  *   cage (it holds the per-record weights of the corners that alias record
- *   0 and each record's 27 neighbour records and occupied-cell mask, about
- *   132 B per record); NULL / 0 for dense grids.  Prepare again after the
+ *   0, the list of records whose weight is not 0, and each record's 27
+ *   neighbour records, occupied-cell mask and multiplicities, at most about
+ *   140 B per record); NULL / 0 for dense grids.  Prepare again after the
  *   cage changes.  Calls that share one scratch on one stream are safe;
  *   calls on different streams must not share it. */
 size_t pxl_tv_scratch_bytes(const pxl_grid *grid);
