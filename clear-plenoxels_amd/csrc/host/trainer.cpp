@@ -60,6 +60,12 @@ struct pxl_trainer {
     void *tv_scratch;
     size_t tv_scratch_bytes;
     hipStream_t stream;
+    // The batch-loss reduction runs on a second stream (default priority),
+    // after the render and beside TV and Adam; the next render waits for it
+    // before it rewrites the losses.  side == nullptr: all on `stream`.
+    hipStream_t side;
+    hipEvent_t ev_rendered, ev_reduced;
+    bool side_used;
     int64_t batch_count;
     int64_t t;
     double lr, sh_lr;
@@ -123,6 +129,18 @@ extern "C" pxl_trainer *pxl_trainer_create(const pxl_train_config *cfg, const px
     tr->losses = losses;
     tr->loss_out = loss_out;
     tr->stream = as_stream(stream);
+    tr->side = nullptr;
+    tr->ev_rendered = tr->ev_reduced = nullptr;
+    tr->side_used = false;
+    if (hipStreamCreateWithFlags(&tr->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&tr->ev_rendered, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&tr->ev_reduced, hipEventDisableTiming) != hipSuccess) {
+        if (tr->ev_rendered) (void)hipEventDestroy(tr->ev_rendered);
+        if (tr->ev_reduced) (void)hipEventDestroy(tr->ev_reduced);
+        if (tr->side) (void)hipStreamDestroy(tr->side);
+        tr->side = nullptr;
+        tr->ev_rendered = tr->ev_reduced = nullptr;
+    }
     tr->tv_scratch = tv_scratch;
     tr->tv_scratch_bytes = tv_scratch_bytes;
     tr->batch_count = (n_rays + cfg->batch_size - 1) / cfg->batch_size;
@@ -152,12 +170,12 @@ extern "C" pxl_trainer *pxl_trainer_create(const pxl_train_config *cfg, const px
                     (cfg->waves_per_block && cfg->waves_per_block != 4))) {
         set_error("pxl_trainer_create: batch orders and schedules need the logged step "
                   "(fused with log_samples > 0)");
-        delete tr;
+        pxl_trainer_destroy(tr);
         return nullptr;
     }
     if (cfg->chunked && cfg->persistent) {
         set_error("pxl_trainer_create: the chunk-parallel gradient has no persistent form");
-        delete tr;
+        pxl_trainer_destroy(tr);
         return nullptr;
     }
     if (logged) {
@@ -359,6 +377,12 @@ extern "C" int32_t pxl_trainer_step(pxl_trainer *tr) {
     const int64_t start = b * c.batch_size;
     const int64_t len = std::min<int64_t>(c.batch_size, tr->n_rays - start);
     const bool timed = tr->ev_used * 2 + 1 < (int64_t)tr->ev.size();
+    // The previous batch's loss reduction reads the losses this render
+    // rewrites.
+    if (tr->side_used && hipStreamWaitEvent(tr->stream, tr->ev_reduced, 0) != hipSuccess) {
+        set_error("pxl_trainer_step: hipStreamWaitEvent failed");
+        return PXL_EHIP;
+    }
     if (timed) (void)hipEventRecord(tr->ev[tr->ev_used * 2], tr->stream);
     int32_t st;
     if (c.fused && tr->march_ws) {
@@ -386,9 +410,24 @@ extern "C" int32_t pxl_trainer_step(pxl_trainer *tr) {
         tr->ev_used++;
     }
     // Batch loss on the GPU in FP64 instead of a D2H copy and a host loop
-    // (Render.cu:443-449).
-    st = pxl_loss_reduce_f64(tr->losses, len, tr->loss_out, tr->stream);
-    if (st != PXL_OK) return st;
+    // (Render.cu:443-449), on the side stream once the render is done.
+    if (tr->side) {
+        if (hipEventRecord(tr->ev_rendered, tr->stream) != hipSuccess ||
+            hipStreamWaitEvent(tr->side, tr->ev_rendered, 0) != hipSuccess) {
+            set_error("pxl_trainer_step: event ordering failed");
+            return PXL_EHIP;
+        }
+        st = pxl_loss_reduce_f64(tr->losses, len, tr->loss_out, tr->side);
+        if (st != PXL_OK) return st;
+        if (hipEventRecord(tr->ev_reduced, tr->side) != hipSuccess) {
+            set_error("pxl_trainer_step: hipEventRecord failed");
+            return PXL_EHIP;
+        }
+        tr->side_used = true;
+    } else {
+        st = pxl_loss_reduce_f64(tr->losses, len, tr->loss_out, tr->stream);
+        if (st != PXL_OK) return st;
+    }
     // Regulariser every batch when dense, every interval-th batch when
     // sparse (Render.cu:469).
     if (!tr->grid.sparse || b % c.variance_interval == 0) {
@@ -435,7 +474,8 @@ extern "C" int32_t pxl_trainer_render_ms(pxl_trainer *tr, double *avg_ms, int64_
 extern "C" int32_t pxl_trainer_loss(pxl_trainer *tr, double *mse, double *psnr) {
     PXL_REQUIRE(tr && mse && psnr, "pxl_trainer_loss: null argument");
     double h[2];
-    if (hipMemcpyAsync(h, tr->loss_out, sizeof(h), hipMemcpyDeviceToHost, tr->stream) !=
+    if ((tr->side_used && hipStreamWaitEvent(tr->stream, tr->ev_reduced, 0) != hipSuccess) ||
+        hipMemcpyAsync(h, tr->loss_out, sizeof(h), hipMemcpyDeviceToHost, tr->stream) !=
             hipSuccess ||
         hipStreamSynchronize(tr->stream) != hipSuccess) {
         set_error("pxl_trainer_loss: copy failed");
@@ -463,6 +503,12 @@ extern "C" void pxl_trainer_destroy(pxl_trainer *tr) {
     if (!tr) return;
     for (hipEvent_t e : tr->ev) (void)hipEventDestroy(e);
     (void)hipStreamSynchronize(tr->stream);
+    if (tr->side) {
+        (void)hipStreamSynchronize(tr->side);
+        (void)hipEventDestroy(tr->ev_rendered);
+        (void)hipEventDestroy(tr->ev_reduced);
+        (void)hipStreamDestroy(tr->side);
+    }
     if (tr->march_ws) (void)hipFree(tr->march_ws);
     if (tr->order_buf) (void)hipFree(tr->order_buf);
     if (tr->order_ws) (void)hipFree(tr->order_ws);
