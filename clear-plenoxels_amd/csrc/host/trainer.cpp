@@ -2,6 +2,7 @@
 // Render::objectSceneLearner (Render.cu:352-503) as a host object.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -66,6 +67,12 @@ struct pxl_trainer {
     hipStream_t side;
     hipEvent_t ev_rendered, ev_reduced;
     bool side_used;
+    // Regulariser batches of the logged step: TV runs on the side stream
+    // beside the forward kernel, after the previous batch's Adam (ev_step);
+    // the gradient kernels wait for it (ev_tv).  PXL_TRAINER_TV_OVERLAP=0
+    // keeps TV on `stream` after the render.
+    hipEvent_t ev_step, ev_tv;
+    bool tv_overlap;
     int64_t batch_count;
     int64_t t;
     double lr, sh_lr;
@@ -130,16 +137,22 @@ extern "C" pxl_trainer *pxl_trainer_create(const pxl_train_config *cfg, const px
     tr->loss_out = loss_out;
     tr->stream = as_stream(stream);
     tr->side = nullptr;
-    tr->ev_rendered = tr->ev_reduced = nullptr;
+    tr->ev_rendered = tr->ev_reduced = tr->ev_step = tr->ev_tv = nullptr;
     tr->side_used = false;
     if (hipStreamCreateWithFlags(&tr->side, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&tr->ev_rendered, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&tr->ev_reduced, hipEventDisableTiming) != hipSuccess) {
-        if (tr->ev_rendered) (void)hipEventDestroy(tr->ev_rendered);
-        if (tr->ev_reduced) (void)hipEventDestroy(tr->ev_reduced);
+        hipEventCreateWithFlags(&tr->ev_reduced, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&tr->ev_step, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&tr->ev_tv, hipEventDisableTiming) != hipSuccess) {
+        for (hipEvent_t e : {tr->ev_rendered, tr->ev_reduced, tr->ev_step, tr->ev_tv})
+            if (e) (void)hipEventDestroy(e);
         if (tr->side) (void)hipStreamDestroy(tr->side);
         tr->side = nullptr;
-        tr->ev_rendered = tr->ev_reduced = nullptr;
+        tr->ev_rendered = tr->ev_reduced = tr->ev_step = tr->ev_tv = nullptr;
+    }
+    {
+        const char *e = std::getenv("PXL_TRAINER_TV_OVERLAP");
+        tr->tv_overlap = tr->side != nullptr && !(e && e[0] == '0');
     }
     tr->tv_scratch = tv_scratch;
     tr->tv_scratch_bytes = tv_scratch_bytes;
@@ -376,15 +389,43 @@ extern "C" int32_t pxl_trainer_step(pxl_trainer *tr) {
     const int64_t epoch = tr->t / tr->batch_count;
     const int64_t start = b * c.batch_size;
     const int64_t len = std::min<int64_t>(c.batch_size, tr->n_rays - start);
-    const bool timed = tr->ev_used * 2 + 1 < (int64_t)tr->ev.size();
     // The previous batch's loss reduction reads the losses this render
     // rewrites.
     if (tr->side_used && hipStreamWaitEvent(tr->stream, tr->ev_reduced, 0) != hipSuccess) {
         set_error("pxl_trainer_step: hipStreamWaitEvent failed");
         return PXL_EHIP;
     }
-    if (timed) (void)hipEventRecord(tr->ev[tr->ev_used * 2], tr->stream);
+    // Regulariser every batch when dense, every interval-th batch when
+    // sparse (Render.cu:469).
+    const bool do_tv = !tr->grid.sparse || b % c.variance_interval == 0;
+    // With the logged step (schedule already chosen, no proposal) TV goes to
+    // the side stream ahead of the render.  Its gradient then enters the
+    // planes before the render's instead of after: the same sum in another
+    // order.
+    const bool tv_side = do_tv && tr->tv_overlap && c.fused && tr->march_ws && tr->tuned &&
+                         !tr->prop_on && len > 0;
+    // Render timing covers the batches whose render runs alone.
+    const bool timed = !tv_side && tr->ev_used * 2 + 1 < (int64_t)tr->ev.size();
     int32_t st;
+    if (tv_side) {
+        if (hipEventRecord(tr->ev_step, tr->stream) != hipSuccess ||
+            hipStreamWaitEvent(tr->side, tr->ev_step, 0) != hipSuccess) {
+            set_error("pxl_trainer_step: event ordering failed");
+            return PXL_EHIP;
+        }
+        st = RenderUtils::addVarianceLoss(tr->grid, tr->optim, c.variance_coefficient,
+                                          c.sh_coefficient, c.variance_type, c.neighbour_nums,
+                                          c.optimize_step, tr->tv_scratch,
+                                          tr->tv_scratch_bytes, tr->side);
+        if (st != PXL_OK) return st;
+        if (hipEventRecord(tr->ev_tv, tr->side) != hipSuccess) {
+            set_error("pxl_trainer_step: hipEventRecord failed");
+            return PXL_EHIP;
+        }
+        tr->side_used = true;
+        pxl::set_grad_wait(tr->ev_tv);
+    }
+    if (timed) (void)hipEventRecord(tr->ev[tr->ev_used * 2], tr->stream);
     if (c.fused && tr->march_ws) {
         if (!tr->tuned) {
             if (timed) (void)hipEventRecord(tr->ev[tr->ev_used * 2], tr->stream);
@@ -403,6 +444,16 @@ extern "C" int32_t pxl_trainer_step(pxl_trainer *tr) {
                                                            tr->losses, tr->order, start, len,
                                                            c.mask_coefficient, c.huber_delta,
                                                            tr->stream);
+    }
+    if (tv_side) {
+        // A launch path that did not take the wait (or failed to enqueue it)
+        // is still ordered after TV before anything else touches the planes.
+        const bool failed = pxl::grad_wait_failed();
+        pxl::set_grad_wait(nullptr);
+        if (failed || hipStreamWaitEvent(tr->stream, tr->ev_tv, 0) != hipSuccess) {
+            set_error("pxl_trainer_step: hipStreamWaitEvent failed");
+            return PXL_EHIP;
+        }
     }
     if (st != PXL_OK) return st;
     if (timed) {
@@ -428,9 +479,7 @@ extern "C" int32_t pxl_trainer_step(pxl_trainer *tr) {
         st = pxl_loss_reduce_f64(tr->losses, len, tr->loss_out, tr->stream);
         if (st != PXL_OK) return st;
     }
-    // Regulariser every batch when dense, every interval-th batch when
-    // sparse (Render.cu:469).
-    if (!tr->grid.sparse || b % c.variance_interval == 0) {
+    if (do_tv && !tv_side) {
         st = RenderUtils::addVarianceLoss(tr->grid, tr->optim, c.variance_coefficient,
                                           c.sh_coefficient, c.variance_type, c.neighbour_nums,
                                           c.optimize_step, tr->tv_scratch,
@@ -505,8 +554,8 @@ extern "C" void pxl_trainer_destroy(pxl_trainer *tr) {
     (void)hipStreamSynchronize(tr->stream);
     if (tr->side) {
         (void)hipStreamSynchronize(tr->side);
-        (void)hipEventDestroy(tr->ev_rendered);
-        (void)hipEventDestroy(tr->ev_reduced);
+        for (hipEvent_t e : {tr->ev_rendered, tr->ev_reduced, tr->ev_step, tr->ev_tv})
+            (void)hipEventDestroy(e);
         (void)hipStreamDestroy(tr->side);
     }
     if (tr->march_ws) (void)hipFree(tr->march_ws);

@@ -926,7 +926,25 @@ extern "C" int32_t pxl_render_fwd_trace(const pxl_grid *grid, const pxl_rays *ra
 }
 
 namespace pxl {
+// Event that the next logged launch on the calling thread waits for between
+// its forward kernel and its gradient kernels, then forgets.  The trainer runs
+// the regulariser on a second stream beside the forward; the regulariser
+// writes the gradient planes that the gradient kernels add into.
+static thread_local hipEvent_t t_grad_wait = nullptr;
+static thread_local bool t_grad_wait_failed = false;
+void set_grad_wait(hipEvent_t ev) {
+    t_grad_wait = ev;
+    t_grad_wait_failed = false;
+}
+bool grad_wait_failed() { return t_grad_wait_failed; }
+
 namespace {
+void wait_before_grad(hipStream_t st) {
+    if (!t_grad_wait) return;
+    if (hipStreamWaitEvent(st, t_grad_wait, 0) != hipSuccess) t_grad_wait_failed = true;
+    t_grad_wait = nullptr;
+}
+
 // Workspace of the two-kernel fused step: per batch entry 4 FP64 totals and a
 // count, per logged sample 8 records and 8 value words, per 64-sample chunk
 // of an entry's log 5 FP64 prefixes; 256-byte aligned.
@@ -993,6 +1011,7 @@ void launch_logged(const RenderArgs &a, int wpb, int gwpb, bool persist, bool ch
             std::min(blocks, resident_blocks(k_march8<S, SPARSE, kLog, STORE, true>, threads));
         hipLaunchKernelGGL((k_march8<S, SPARSE, kLog, STORE, true>), dim3(ga), dim3(threads), 0,
                            st, a);
+        wait_before_grad(st);
         if (tgt) {
             const unsigned gb =
                 std::min(gblocks, resident_blocks(k_log_grad<S, true, true>, gthreads));
@@ -1006,6 +1025,7 @@ void launch_logged(const RenderArgs &a, int wpb, int gwpb, bool persist, bool ch
         // whole ray in one wave, so learning batches keep the per-ray kernel).
         hipLaunchKernelGGL((k_march8<S, SPARSE, kLog, STORE>), dim3(blocks), dim3(threads), 0, st,
                            a);
+        wait_before_grad(st);
         hipLaunchKernelGGL((k_log_prefix<S>), dim3(gblocks), dim3(gthreads), 0, st, a);
         const int64_t items = a.len * (int64_t)a.log_chunks;
         hipLaunchKernelGGL((k_log_grad_chunked<S>), dim3((unsigned)((items + gwpb - 1) / gwpb)),
@@ -1013,6 +1033,7 @@ void launch_logged(const RenderArgs &a, int wpb, int gwpb, bool persist, bool ch
     } else {
         hipLaunchKernelGGL((k_march8<S, SPARSE, kLog, STORE>), dim3(blocks), dim3(threads), 0, st,
                            a);
+        wait_before_grad(st);
         if (tgt)
             hipLaunchKernelGGL((k_log_grad<S, false, true>), dim3(gblocks), dim3(gthreads), 0, st,
                                a);

@@ -620,11 +620,19 @@ pxl_trainer *pxl_trainer_create(const pxl_train_config *cfg, const pxl_grid *gri
  * batch loss, variance loss (every batch when dense, every
  * variance_interval-th when sparse, Render.cu:469), Adam with time step
  * epoch*batchCount + batch + 1, LR decay lr /= 2^(1/(batchCount (epoch+1)))
- * (Render.cu:499-500).  Asynchronous. */
+ * (Render.cu:499-500).  Asynchronous.  With the logged step, once its
+ * schedule is chosen and without the proposal, the variance loss runs on a
+ * second stream beside the forward kernel and the gradient kernels wait for
+ * it (the same sum into the gradient planes in another order);
+ * PXL_TRAINER_TV_OVERLAP=0 in the environment at create keeps it after the
+ * render on the trainer's stream.  The batch loss is reduced on that second
+ * stream too; pxl_trainer_loss waits for it. */
 int32_t pxl_trainer_step(pxl_trainer *tr);
 
 /* Record HIP events around the render launch of the next max_steps
- * batches; pxl_trainer_render_ms returns their average duration. */
+ * batches that render alone (batches whose variance loss overlaps the
+ * forward are not timed); pxl_trainer_render_ms returns their average
+ * duration and count. */
 int32_t pxl_trainer_time_render(pxl_trainer *tr, int32_t max_steps);
 int32_t pxl_trainer_render_ms(pxl_trainer *tr, double *avg_ms, int64_t *count);
 
