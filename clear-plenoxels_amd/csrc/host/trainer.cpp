@@ -404,8 +404,10 @@ extern "C" int32_t pxl_trainer_step(pxl_trainer *tr) {
     // order.
     const bool tv_side = do_tv && tr->tv_overlap && c.fused && tr->march_ws && tr->tuned &&
                          !tr->prop_on && len > 0;
-    // Render timing covers the batches whose render runs alone.
-    const bool timed = !tv_side && tr->ev_used * 2 + 1 < (int64_t)tr->ev.size();
+    // Render timing covers the batches whose render runs alone, or every
+    // batch when each carries the regulariser (dense grids).
+    const bool timed = (!tv_side || !tr->grid.sparse) &&
+                       tr->ev_used * 2 + 1 < (int64_t)tr->ev.size();
     int32_t st;
     if (tv_side) {
         if (hipEventRecord(tr->ev_step, tr->stream) != hipSuccess ||

@@ -630,8 +630,9 @@ pxl_trainer *pxl_trainer_create(const pxl_train_config *cfg, const pxl_grid *gri
 int32_t pxl_trainer_step(pxl_trainer *tr);
 
 /* Record HIP events around the render launch of the next max_steps
- * batches that render alone (batches whose variance loss overlaps the
- * forward are not timed); pxl_trainer_render_ms returns their average
+ * batches that render alone (on sparse grids, batches whose variance loss
+ * overlaps the forward are not timed; on dense grids every batch carries it
+ * and is timed with it); pxl_trainer_render_ms returns their average
  * duration and count. */
 int32_t pxl_trainer_time_render(pxl_trainer *tr, int32_t max_steps);
 int32_t pxl_trainer_render_ms(pxl_trainer *tr, double *avg_ms, int64_t *count);
