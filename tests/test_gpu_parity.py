@@ -590,17 +590,23 @@ def test_train_step_c1(c1):
     assert diff.max() <= 0.2 + 1e-6
 
 
-@pytest.mark.parametrize("log_samples,autotune", [(0, False), (512, False), (512, True)])
-def test_trainer_matches_stage_calls(c1, log_samples, autotune):
+@pytest.mark.parametrize("log_samples,autotune,tv_overlap",
+                         [(0, False, True), (512, False, True), (512, True, True),
+                          (512, False, False)])
+def test_trainer_matches_stage_calls(c1, log_samples, autotune, tv_overlap, monkeypatch):
     """The C++ batch loop (pxl_trainer_step) issues the same stages as the
     Python-driven sequence: dense grid (TV every batch), 3 batches, LR decay
     lr /= 2^(1/(batchCount (epoch+1))) per batch (Render.cu:499-500); with
-    log_samples the trainer runs the two-kernel logged step."""
+    log_samples the trainer runs the two-kernel logged step, with TV on its
+    second stream beside the forward unless PXL_TRAINER_TV_OVERLAP=0."""
     import math
 
     import torch
 
-    say(f"[trainer] 3 batches vs stage calls, log_samples={log_samples}, autotune={autotune}")
+    if not tv_overlap:
+        monkeypatch.setenv("PXL_TRAINER_TV_OVERLAP", "0")
+    say(f"[trainer] 3 batches vs stage calls, log_samples={log_samples}, autotune={autotune}, "
+        f"tv_overlap={tv_overlap}")
     dev = c1["dev"]
     B = 1024
     order = c1["si"]
