@@ -19,6 +19,31 @@ void clear_error();
 // Checks the launch that was just issued; records the HIP error text.
 int32_t check_launch(const char *what);
 
+// Library options (pxl_set_option): the eight-sample marcher for the fixed
+// rule (else one sample per wave), and the TV gathers' 64-bit address form.
+bool option_group_marcher();
+bool option_tv_addr64();
+
+// Extra arguments of the train entry points (pxl_train_rays_ws / _sched /
+// _prop), and the event the gradient kernels wait for (the trainer runs the
+// variance loss, which writes the same gradient planes, on another stream).
+struct TrainExtra {
+    void *workspace = nullptr;
+    size_t workspace_bytes = 0;
+    int32_t max_samples = 0;
+    const pxl_schedule *sched = nullptr;
+    const uint64_t *bin_mask = nullptr;
+    uint64_t *bin_targets = nullptr;
+    uint8_t *target_valid = nullptr;
+    double target_wmin = 0.0;
+    hipEvent_t grad_wait = nullptr;
+};
+int32_t train_rays_ex(const pxl_grid *grid, const pxl_optim *optim, const pxl_rays *rays,
+                      double *losses, const int32_t *sample_index, int64_t start, int64_t len,
+                      double mask_coef, float huber_delta, void *stream, const TrainExtra &x);
+// Whether the train step of this grid runs as the two-kernel logged step.
+bool logged_step_serves(const pxl_grid *g);
+
 #define PXL_REQUIRE(cond, ...)                 \
     do {                                       \
         if (!(cond)) {                         \

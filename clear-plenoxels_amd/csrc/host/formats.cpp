@@ -366,19 +366,21 @@ extern "C" int32_t pxl_grid_load_text(const char *dir, int32_t band, int32_t spa
     const int64_t nv = (int64_t)W * H * D, ncell = (int64_t)(W - 1) * (H - 1) * (D - 1);
     const int64_t n = sparse ? n_records : nv;
     PXL_REQUIRE(n >= 1 && n <= nv, "pxl_grid_load_text: %lld records", (long long)n);
+    // Everything is read and checked on the host before the first copy to
+    // the device, so a rejected directory leaves the device buffers as they
+    // were.
     std::vector<int> iv;
     std::vector<uint8_t> occ;
+    std::vector<uint32_t> bits;
     if (have_occupancy) {
         PXL_REQUIRE(read_ints(join(dir, "occupancy.txt"), ncell, iv) == ncell,
                     "pxl_grid_load_text: occupancy.txt of %s is short", dir);
         occ.resize((size_t)ncell);
-        std::vector<uint32_t> bits((size_t)((ncell + 31) / 32), 0u);
+        bits.assign((size_t)((ncell + 31) / 32), 0u);
         for (int64_t c = 0; c < ncell; c++) {
             occ[(size_t)c] = (uint8_t)iv[(size_t)c];
             if (occ[(size_t)c]) bits[(size_t)(c >> 5)] |= 1u << (c & 31);
         }
-        const int32_t rc = h2d(occupancy, bits.data(), bits.size() * 4, st);
-        if (rc != PXL_OK) return rc;
     }
     std::vector<float> rec((size_t)n * stride, 0.0f), fv;
     const std::string dens = sparse ? fmt("sparse_opacity_band%d.txt", band)
@@ -395,11 +397,12 @@ extern "C" int32_t pxl_grid_load_text(const char *dir, int32_t band, int32_t spa
             for (int64_t r = 0; r < n; r++)
                 rec[(size_t)r * stride + 1 + ch * sh_dim + i] = fv[(size_t)r];
         }
+    std::vector<uint8_t> cage;
+    std::vector<uint32_t> io, c2d;
     if (sparse) {
         // In-cage flags from their file when it holds them; a directory
         // the reference saved holds only the count there, and the flags
         // follow from the occupancy by the pruning rule.
-        std::vector<uint8_t> cage;
         const int64_t got = read_ints(join(dir, fmt("sparse_inCageInfo_band%d.txt", band)), nv, iv);
         if (got == nv) {
             cage.resize((size_t)nv);
@@ -409,7 +412,7 @@ extern "C" int32_t pxl_grid_load_text(const char *dir, int32_t band, int32_t spa
         }
         PXL_REQUIRE(read_ints(join(dir, "sparse_indexOffset.txt"), nv, iv) == nv,
                     "pxl_grid_load_text: sparse_indexOffset.txt of %s is short", dir);
-        std::vector<uint32_t> io((size_t)nv), c2d;
+        io.resize((size_t)nv);
         c2d.reserve((size_t)n);
         for (int64_t i = 0; i < nv; i++) {
             io[(size_t)i] = (uint32_t)iv[(size_t)i];
@@ -422,11 +425,24 @@ extern "C" int32_t pxl_grid_load_text(const char *dir, int32_t band, int32_t spa
             PXL_REQUIRE(io[c2d[(size_t)k]] == (uint32_t)k,
                         "pxl_grid_load_text: index of vertex %u is %u, expected %lld", c2d[(size_t)k],
                         io[c2d[(size_t)k]], (long long)k);
-        int32_t rc;
-        if ((rc = h2d(in_cage, cage.data(), cage.size(), st)) != PXL_OK ||
-            (rc = h2d(index_offsets, io.data(), io.size() * 4, st)) != PXL_OK ||
-            (rc = h2d(compact_to_dense, c2d.data(), c2d.size() * 4, st)) != PXL_OK)
-            return rc;
+        // Every other vertex (outside the cage, and the first one in it) has
+        // index 0 (getCagedNums, SceneUtils.cu:1106-1113); a file that says
+        // otherwise would send kernels to records that do not exist.
+        for (int64_t i = 0; i < nv; i++)
+            PXL_REQUIRE(cage[(size_t)i] || io[(size_t)i] == 0u,
+                        "pxl_grid_load_text: vertex %lld is outside the cage but has index %u",
+                        (long long)i, io[(size_t)i]);
+        PXL_REQUIRE(n == 0 || io[c2d[0]] == 0u,
+                    "pxl_grid_load_text: the first in-cage vertex has index %u, expected 0",
+                    io[c2d[0]]);
     }
+    int32_t rc;
+    if (have_occupancy && (rc = h2d(occupancy, bits.data(), bits.size() * 4, st)) != PXL_OK)
+        return rc;
+    if (sparse &&
+        ((rc = h2d(in_cage, cage.data(), cage.size(), st)) != PXL_OK ||
+         (rc = h2d(index_offsets, io.data(), io.size() * 4, st)) != PXL_OK ||
+         (rc = h2d(compact_to_dense, c2d.data(), c2d.size() * 4, st)) != PXL_OK))
+        return rc;
     return h2d(params, rec.data(), rec.size() * 4, st);
 }

@@ -9,12 +9,6 @@
 #include "pxl.h"
 
 namespace pxl {
-// The next logged launch on the calling thread makes its stream wait for
-// `ev` between the forward and the gradient kernels (nullptr: no wait).
-void set_grad_wait(hipEvent_t ev);
-// Whether that wait could not be enqueued.
-bool grad_wait_failed();
-
 namespace host {
 
 bool invert(const double *src, double *dst, int n);

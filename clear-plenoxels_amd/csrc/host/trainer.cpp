@@ -69,15 +69,17 @@ struct pxl_trainer {
     bool side_used;
     // Regulariser batches of the logged step: TV runs on the side stream
     // beside the forward kernel, after the previous batch's Adam (ev_step);
-    // the gradient kernels wait for it (ev_tv).  PXL_TRAINER_TV_OVERLAP=0
-    // keeps TV on `stream` after the render.
+    // the gradient kernels wait for it (ev_tv).  cfg.tv_inline keeps TV on
+    // `stream` after the render.
     hipEvent_t ev_step, ev_tv;
     bool tv_overlap;
     int64_t batch_count;
     int64_t t;
     double lr, sh_lr;
-    // Optional HIP events around each batch's render launch.
+    // Optional HIP events around each batch's render launch, and whether
+    // the variance loss ran beside that render.
     std::vector<hipEvent_t> ev;
+    std::vector<char> ev_beside;
     int64_t ev_used;
     // Sample-log workspace of the two-kernel fused step (owned).
     void *march_ws;
@@ -150,10 +152,7 @@ extern "C" pxl_trainer *pxl_trainer_create(const pxl_train_config *cfg, const px
         tr->side = nullptr;
         tr->ev_rendered = tr->ev_reduced = tr->ev_step = tr->ev_tv = nullptr;
     }
-    {
-        const char *e = std::getenv("PXL_TRAINER_TV_OVERLAP");
-        tr->tv_overlap = tr->side != nullptr && !(e && e[0] == '0');
-    }
+    tr->tv_overlap = tr->side != nullptr && !cfg->tv_inline;
     tr->tv_scratch = tv_scratch;
     tr->tv_scratch_bytes = tv_scratch_bytes;
     tr->batch_count = (n_rays + cfg->batch_size - 1) / cfg->batch_size;
@@ -220,7 +219,8 @@ extern "C" pxl_trainer *pxl_trainer_create(const pxl_train_config *cfg, const px
 // pruning) runs unpruned, returns targets and trains the proposal, the
 // others run with the proposal's bin masks.
 static int32_t render_batch(pxl_trainer *tr, int64_t start, int64_t len, int32_t order,
-                            const pxl_schedule &sched, bool use_prop = false) {
+                            const pxl_schedule &sched, bool use_prop = false,
+                            hipEvent_t grad_wait = nullptr) {
     const pxl_train_config &c = tr->cfg;
     const int32_t *si = tr->order;
     int64_t s0 = start;
@@ -232,10 +232,15 @@ static int32_t render_batch(pxl_trainer *tr, int64_t start, int64_t len, int32_t
         si = tr->order_buf;
         s0 = 0;
     }
+    TrainExtra x;
+    x.workspace = tr->march_ws;
+    x.workspace_bytes = tr->march_ws_bytes;
+    x.max_samples = c.log_samples;
+    x.sched = &sched;
+    x.grad_wait = grad_wait;
     if (!use_prop || !tr->prop_on)
-        return pxl_train_rays_sched(&tr->grid, &tr->optim, &tr->rays, tr->losses, si, s0, len,
-                                    c.mask_coefficient, c.huber_delta, tr->march_ws,
-                                    tr->march_ws_bytes, c.log_samples, &sched, tr->stream);
+        return train_rays_ex(&tr->grid, &tr->optim, &tr->rays, tr->losses, si, s0, len,
+                             c.mask_coefficient, c.huber_delta, tr->stream, x);
     int32_t st = pxl_proposal_features(&tr->grid, &tr->rays, si, s0, len, tr->prop_ws,
                                        tr->prop_ws_bytes, tr->stream);
     if (st != PXL_OK) return st;
@@ -371,6 +376,7 @@ extern "C" int32_t pxl_trainer_time_render(pxl_trainer *tr, int32_t max_steps) {
     PXL_REQUIRE(tr && max_steps >= 0, "pxl_trainer_time_render: bad argument");
     for (hipEvent_t e : tr->ev) (void)hipEventDestroy(e);
     tr->ev.assign((size_t)max_steps * 2, nullptr);
+    tr->ev_beside.assign((size_t)max_steps, 0);
     for (auto &e : tr->ev) {
         if (hipEventCreate(&e) != hipSuccess) {
             set_error("pxl_trainer_time_render: hipEventCreate failed");
@@ -384,30 +390,24 @@ extern "C" int32_t pxl_trainer_time_render(pxl_trainer *tr, int32_t max_steps) {
 // One batch (Render.cu:399-500).
 extern "C" int32_t pxl_trainer_step(pxl_trainer *tr) {
     PXL_REQUIRE(tr != nullptr, "pxl_trainer_step: null trainer");
+    clear_error();
     const pxl_train_config &c = tr->cfg;
     const int64_t b = tr->t % tr->batch_count;
     const int64_t epoch = tr->t / tr->batch_count;
     const int64_t start = b * c.batch_size;
     const int64_t len = std::min<int64_t>(c.batch_size, tr->n_rays - start);
-    // The previous batch's loss reduction reads the losses this render
-    // rewrites.
-    if (tr->side_used && hipStreamWaitEvent(tr->stream, tr->ev_reduced, 0) != hipSuccess) {
-        set_error("pxl_trainer_step: hipStreamWaitEvent failed");
-        return PXL_EHIP;
-    }
     // Regulariser every batch when dense, every interval-th batch when
     // sparse (Render.cu:469).
     const bool do_tv = !tr->grid.sparse || b % c.variance_interval == 0;
     // With the logged step (schedule already chosen, no proposal) TV goes to
-    // the side stream ahead of the render.  Its gradient then enters the
-    // planes before the render's instead of after: the same sum in another
-    // order.
+    // the side stream ahead of the render, and only the gradient kernels of
+    // that step wait for it; its gradient then enters the planes before the
+    // render's instead of after: the same sum in another order.  Grids the
+    // logged step does not serve write gradients from the forward kernel
+    // itself, so their TV stays on the trainer's stream.
     const bool tv_side = do_tv && tr->tv_overlap && c.fused && tr->march_ws && tr->tuned &&
-                         !tr->prop_on && len > 0;
-    // Render timing covers the batches whose render runs alone, or every
-    // batch when each carries the regulariser (dense grids).
-    const bool timed = (!tv_side || !tr->grid.sparse) &&
-                       tr->ev_used * 2 + 1 < (int64_t)tr->ev.size();
+                         !tr->prop_on && len > 0 && logged_step_serves(&tr->grid);
+    const bool timed = tr->ev_used * 2 + 1 < (int64_t)tr->ev.size();
     int32_t st;
     if (tv_side) {
         if (hipEventRecord(tr->ev_step, tr->stream) != hipSuccess ||
@@ -425,17 +425,15 @@ extern "C" int32_t pxl_trainer_step(pxl_trainer *tr) {
             return PXL_EHIP;
         }
         tr->side_used = true;
-        pxl::set_grad_wait(tr->ev_tv);
+    }
+    if (c.fused && tr->march_ws && !tr->tuned) {
+        st = autotune(tr, start, len);
+        if (st != PXL_OK) return st;
     }
     if (timed) (void)hipEventRecord(tr->ev[tr->ev_used * 2], tr->stream);
     if (c.fused && tr->march_ws) {
-        if (!tr->tuned) {
-            if (timed) (void)hipEventRecord(tr->ev[tr->ev_used * 2], tr->stream);
-            st = autotune(tr, start, len);
-            if (st != PXL_OK) return st;
-            if (timed) (void)hipEventRecord(tr->ev[tr->ev_used * 2], tr->stream);
-        }
-        st = render_batch(tr, start, len, tr->order_mode, tr->sched, true);
+        st = render_batch(tr, start, len, tr->order_mode, tr->sched, true,
+                          tv_side ? tr->ev_tv : nullptr);
     } else if (c.fused) {
         st = pxl_train_rays(&tr->grid, &tr->optim, &tr->rays, tr->losses, tr->order, start, len,
                             c.mask_coefficient, c.huber_delta, tr->stream);
@@ -447,19 +445,16 @@ extern "C" int32_t pxl_trainer_step(pxl_trainer *tr) {
                                                            c.mask_coefficient, c.huber_delta,
                                                            tr->stream);
     }
-    if (tv_side) {
-        // A launch path that did not take the wait (or failed to enqueue it)
-        // is still ordered after TV before anything else touches the planes.
-        const bool failed = pxl::grad_wait_failed();
-        pxl::set_grad_wait(nullptr);
-        if (failed || hipStreamWaitEvent(tr->stream, tr->ev_tv, 0) != hipSuccess) {
-            set_error("pxl_trainer_step: hipStreamWaitEvent failed");
-            return PXL_EHIP;
-        }
+    // Whatever the launch did, nothing after it on this stream touches the
+    // gradient planes before TV is done.
+    if (tv_side && hipStreamWaitEvent(tr->stream, tr->ev_tv, 0) != hipSuccess) {
+        set_error("pxl_trainer_step: hipStreamWaitEvent failed");
+        return PXL_EHIP;
     }
     if (st != PXL_OK) return st;
     if (timed) {
         (void)hipEventRecord(tr->ev[tr->ev_used * 2 + 1], tr->stream);
+        tr->ev_beside[tr->ev_used] = tv_side ? 1 : 0;
         tr->ev_used++;
     }
     // Batch loss on the GPU in FP64 instead of a D2H copy and a host loop
@@ -491,6 +486,14 @@ extern "C" int32_t pxl_trainer_step(pxl_trainer *tr) {
     const int32_t time_step = (int32_t)(epoch * tr->batch_count + b + 1);
     st = Optimizer::updateRGBScene(tr->optim, tr->grid, tr->lr, tr->sh_lr, time_step, tr->stream);
     if (st != PXL_OK) return st;
+    // The caller's stream holds all of this batch's work, the loss
+    // reduction included (it ran beside TV and Adam): the next render, a
+    // synchronisation of the stream or a read of losses / loss_out after it
+    // see the finished reduction.
+    if (tr->side && hipStreamWaitEvent(tr->stream, tr->ev_reduced, 0) != hipSuccess) {
+        set_error("pxl_trainer_step: hipStreamWaitEvent failed");
+        return PXL_EHIP;
+    }
     // LR decay (Render.cu:499-500).
     const double decay = std::pow(2.0, 1.0 / (double)tr->batch_count / ((double)epoch + 1.0));
     tr->lr /= decay;
@@ -500,9 +503,11 @@ extern "C" int32_t pxl_trainer_step(pxl_trainer *tr) {
 }
 
 // Average duration of the timed render launches; synchronises on them.
-extern "C" int32_t pxl_trainer_render_ms(pxl_trainer *tr, double *avg_ms, int64_t *count) {
+extern "C" int32_t pxl_trainer_render_ms_split(pxl_trainer *tr, double *avg_ms, int64_t *count,
+                                               double *alone_ms, int64_t *alone_count) {
     PXL_REQUIRE(tr && avg_ms && count, "pxl_trainer_render_ms: null argument");
-    double tot = 0.0;
+    double tot = 0.0, tot_alone = 0.0;
+    int64_t n_alone = 0;
     for (int64_t i = 0; i < tr->ev_used; i++) {
         if (hipEventSynchronize(tr->ev[2 * i + 1]) != hipSuccess) {
             set_error("pxl_trainer_render_ms: hipEventSynchronize failed");
@@ -514,10 +519,20 @@ extern "C" int32_t pxl_trainer_render_ms(pxl_trainer *tr, double *avg_ms, int64_
             return PXL_EHIP;
         }
         tot += ms;
+        if (!tr->ev_beside[i]) {
+            tot_alone += ms;
+            n_alone++;
+        }
     }
     *count = tr->ev_used;
     *avg_ms = tr->ev_used ? tot / (double)tr->ev_used : 0.0;
+    if (alone_ms) *alone_ms = n_alone ? tot_alone / (double)n_alone : 0.0;
+    if (alone_count) *alone_count = n_alone;
     return PXL_OK;
+}
+
+extern "C" int32_t pxl_trainer_render_ms(pxl_trainer *tr, double *avg_ms, int64_t *count) {
+    return pxl_trainer_render_ms_split(tr, avg_ms, count, nullptr, nullptr);
 }
 
 // Copies the last batch's FP64 MSE and PSNR to the host; PXL_ENAN when the
@@ -574,5 +589,98 @@ extern "C" int32_t pxl_trainer_schedule(const pxl_trainer *tr, int32_t *batch_or
     PXL_REQUIRE(tr != nullptr, "pxl_trainer_schedule: null trainer");
     if (batch_order) *batch_order = tr->order_mode;
     if (schedule) *schedule = tr->sched;
+    return PXL_OK;
+}
+
+// Band b of the per-band SH strategy (Render.cu:282-315): the colour uses
+// the coefficients below (b+1)^2; band mode restricts gradients and Adam to
+// the band's own coefficients; the optimiser state of the band starts at
+// zero (pxl_optim_init_band); learning rates and the time step restart.
+extern "C" int32_t pxl_trainer_set_band(pxl_trainer *tr, int32_t band, int32_t band_mode,
+                                        int32_t density_initialized) {
+    clear_error();
+    PXL_REQUIRE(tr != nullptr, "pxl_trainer_set_band: null trainer");
+    const int32_t S = tr->grid.sh_dim;
+    PXL_REQUIRE(band >= 0 && (band + 1) * (band + 1) <= S,
+                "pxl_trainer_set_band: band %d beyond the grid's %d coefficients", band, S);
+    int32_t st = pxl_optim_init_band(&tr->grid, &tr->optim, band, band_mode, density_initialized,
+                                     tr->stream);
+    if (st != PXL_OK) return st;
+    tr->grid.sh_active = (band + 1) * (band + 1);
+    tr->optim.sh_first = band_mode ? band * band : 0;
+    tr->lr = tr->cfg.learning_rate;
+    tr->sh_lr = tr->cfg.sh_learning_rate;
+    tr->t = 0;
+    return PXL_OK;
+}
+
+extern "C" int32_t pxl_trainer_learn_level(pxl_trainer *tr, const pxl_level_config *lc) {
+    clear_error();
+    PXL_REQUIRE(tr && lc, "pxl_trainer_learn_level: null argument");
+    PXL_REQUIRE(lc->epochs >= 0 && lc->sh_band >= 0 &&
+                    (lc->sh_band + 1) * (lc->sh_band + 1) == tr->grid.sh_dim,
+                "pxl_trainer_learn_level: sh_band %d does not match sh_dim %d", lc->sh_band,
+                tr->grid.sh_dim);
+    // First band (Render.cu:267-276).
+    int32_t start_band = lc->initial_from_file ? (lc->upsampled ? 0 : lc->available_sh_band + 1) : 0;
+    const bool below_target = lc->sh_for_target_only && lc->current_hierarchy != lc->minimum_hierarchy;
+    if (!lc->hierarchy_band_strategy) start_band = below_target ? 0 : lc->sh_band;
+    PXL_REQUIRE(start_band >= 0, "pxl_trainer_learn_level: bad start band %d", start_band);
+    const bool band_mode = lc->hierarchy_band_strategy || below_target;
+    for (int32_t band = start_band; band <= lc->sh_band; band++) {
+        const bool density_initialized = !(lc->initial_from_file && band == lc->available_sh_band + 1);
+        int32_t st;
+        if (band_mode) {
+            st = pxl_trainer_set_band(tr, band, 1, density_initialized ? 1 : 0);
+        } else {
+            // initialAll: every coefficient, every moment from zero.
+            st = pxl_trainer_set_band(tr, band, 0, 1);
+        }
+        if (st != PXL_OK) return st;
+        const int64_t steps = (int64_t)lc->epochs * tr->batch_count;
+        for (int64_t i = 0; i < steps; i++) {
+            st = pxl_trainer_step(tr);
+            if (st != PXL_OK) return st;
+            if (lc->check_nan || i + 1 == steps) {
+                double mse = 0.0, psnr = 0.0;
+                st = pxl_trainer_loss(tr, &mse, &psnr);
+                if (st != PXL_OK) return st;
+            }
+        }
+    }
+    return PXL_OK;
+}
+
+extern "C" int32_t pxl_hierarchy_plan(int32_t hierarchy, int32_t minimum_hierarchy,
+                                      int32_t epochs, int64_t batch_size, int32_t shuffle_rays,
+                                      int32_t fix_batch_size, int32_t to_sparse_hierarchy,
+                                      pxl_level *out, int32_t capacity, int32_t *count) {
+    clear_error();
+    PXL_REQUIRE(count != nullptr, "pxl_hierarchy_plan: null count");
+    PXL_REQUIRE(minimum_hierarchy <= hierarchy,
+                "pxl_hierarchy_plan: minimum_hierarchy %d exceeds hierarchy %d",
+                minimum_hierarchy, hierarchy);
+    PXL_REQUIRE(fix_batch_size || hierarchy <= 3,
+                "pxl_hierarchy_plan: batch doubling (batch << (3 - level)) needs levels <= 3");
+    PXL_REQUIRE(minimum_hierarchy >= 0 || fix_batch_size || 3 - minimum_hierarchy < 62,
+                "pxl_hierarchy_plan: level %d out of range", minimum_hierarchy);
+    const int32_t n = hierarchy - minimum_hierarchy + 1;
+    *count = n;
+    if (!out) return PXL_OK;
+    PXL_REQUIRE(capacity >= n, "pxl_hierarchy_plan: %d levels, capacity %d", n, capacity);
+    // VolumeFieldRender.cu:459, 476-495, 549-557.
+    int32_t ep = epochs + 3 - hierarchy;
+    for (int32_t k = 0; k < n; k++) {
+        const int32_t lvl = hierarchy - k;
+        ep = shuffle_rays ? ep + 1 : 10;
+        pxl_level &L = out[k];
+        L.level = lvl;
+        L.epochs = ep;
+        L.batch_size = fix_batch_size ? batch_size : batch_size << (3 - lvl);
+        L.upsample = lvl > minimum_hierarchy
+                         ? (lvl <= minimum_hierarchy - to_sparse_hierarchy ? 2 : 1)
+                         : 0;
+        L.reserved = 0;
+    }
     return PXL_OK;
 }

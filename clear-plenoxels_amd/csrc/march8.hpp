@@ -166,16 +166,17 @@ __device__ __forceinline__ void sample_grads(const RayResid &rr, double C0, doub
 // from a large share of all cells: those go to acc0, a per-lane register
 // added to the gradient once per ray (flush_record0), instead of each being
 // an atomic on the same cache line.
-template <int P>
+// `live`: this lane's parameter takes a gradient (the density, or an SH
+// coefficient of the optimised band); the other lanes hold zeros.
 __device__ __forceinline__ void retire_cell(const RenderArgs &a, float (&acc)[4],
-                                            const uint32_t (&rec)[4], float &acc0, int dx,
-                                            int dy, int dz, int lh, int pp) {
+                                            const uint32_t (&rec)[4], double &acc0, int dx,
+                                            int dy, int dz, int lh, int pp, bool live) {
     const auto in01 = [](int v) { return v == 0 || v == 1; };
 #pragma unroll
     for (int h = 0; h < 4; h++) {
         const bool kept = in01(lh - dx) && in01((h & 1) - dy) && in01((h >> 1) - dz);
-        if (!kept && rec[h] == 0u) acc0 += acc[h];
-        else if (!kept && pp < P)
+        if (!kept && rec[h] == 0u) acc0 += (double)acc[h];
+        else if (!kept && live)
             unsafeAtomicAdd(a.grad + (size_t)rec[h] * (size_t)a.gstride + pp, acc[h]);
     }
     // New corner j' = 2h' + lh takes old corner j' + d: register
@@ -195,11 +196,16 @@ __device__ __forceinline__ void retire_cell(const RenderArgs &a, float (&acc)[4]
     for (int h = 0; h < 4; h++) acc[h] = moved[h];
 }
 
-// Adds the ray's record-0 sum (both lane halves) to the gradient.
-template <int P>
-__device__ __forceinline__ void flush_record0(const RenderArgs &a, float acc0, int lh, int pp) {
-    const float v = acc0 + __shfl_xor(acc0, 32, 64);
-    if (lh == 0 && pp < P && v != 0.0f) unsafeAtomicAdd(a.grad + pp, v);
+// Adds the record-0 sum of batch entry b (both lane halves): into the
+// entry's FP64 slot of the logged step (a.rec0, folded into the gradient in
+// a fixed order afterwards, so the result does not depend on the order in
+// which rays finish), else with one float atomic.
+__device__ __forceinline__ void flush_record0(const RenderArgs &a, double acc0, int lh, int pp,
+                                              bool live, int64_t b) {
+    const double v = acc0 + __shfl_xor(acc0, 32, 64);
+    if (lh != 0 || !live || v == 0.0) return;
+    if (a.rec0 != nullptr) unsafeAtomicAdd(a.rec0 + (size_t)pp * (size_t)a.len + (size_t)b, v);
+    else unsafeAtomicAdd(a.grad + pp, (float)v);
 }
 
 // Batch entry of the next ray a wave of a persistent launch processes.  The
@@ -269,6 +275,9 @@ __device__ __forceinline__ void march8_ray(const RenderArgs &a, const int64_t b)
     // Second fused launch of the logged step: only the rays the log could
     // not hold.
     if (MODE == kFused && a.log_redo && uniform_i(a.log_count[b]) != -1) return;
+    // The entry's record-0 slots start at zero (every logged entry, whatever
+    // path its gradient takes afterwards).
+    if (logging && lane < 32) a.rec0[(size_t)lane * (size_t)a.len + (size_t)b] = 0.0;
     if (ray < 0 || (int64_t)ray >= a.rays.n) {
         if ((do_bwd || logging) && lane == 0) a.losses[b] = 0.0;
         if (logging && lane == 0) a.log_count[b] = 0;
@@ -302,9 +311,10 @@ __device__ __forceinline__ void march8_ray(const RenderArgs &a, const int64_t b)
         return;
     }
 
+    // Basis of the coefficients that enter the colour (k < sh_active).
     float Y[S];
 #pragma unroll
-    for (int k = 0; k < S; k++) Y[k] = sh_basis(k, fdx, fdy, fdz);
+    for (int k = 0; k < S; k++) Y[k] = k < a.sh_active ? sh_basis(k, fdx, fdy, fdz) : 0.0f;
     // Proposal mask of this batch entry (all bins when none is given).
     uint64_t bmask = ~0ull;
     if (a.bin_mask) {
@@ -319,9 +329,11 @@ __device__ __forceinline__ void march8_ray(const RenderArgs &a, const int64_t b)
     // the SH basis value parameter pp multiplies, pch its colour channel.
     const int lh = lane >> 5, pp = lane & 31;
     const int pch = pp == 0 ? -1 : (pp - 1) / S;
+    const bool live = pp == 0 || (pp < P && (pp - 1) % S >= a.sh_first && (pp - 1) % S < a.sh_active);
     float ysel = 0.0f;
 #pragma unroll
     for (int k = 0; k < S; k++) ysel = (pp >= 1 && (pp - 1) % S == k) ? Y[k] : ysel;
+    if (!live) ysel = 0.0f;
 
     // Backward coefficients (set after the forward / from the ray planes).
     double C0 = 0.0, C1 = 0.0, C2 = 0.0, A = 0.0;
@@ -353,7 +365,7 @@ __device__ __forceinline__ void march8_ray(const RenderArgs &a, const int64_t b)
         int cur_x = 0, cur_y = 0, cur_z = 0;
         uint32_t cur_rec[4] = {0u, 0u, 0u, 0u};
         float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-        float acc0 = 0.0f;
+        double acc0 = 0.0;
         uint32_t hcount = 0, hash = 2166136261u;
         int k0 = 0;
         uint64_t pending = 0;  // occupied positions of [k0, k0 + 64) not yet done
@@ -512,8 +524,8 @@ __device__ __forceinline__ void march8_ray(const RenderArgs &a, const int64_t b)
                     const int ny = __builtin_amdgcn_readlane(by, sb);
                     const int nz = __builtin_amdgcn_readlane(bz, sb);
                     if (!has_cur || nx != cur_x || ny != cur_y || nz != cur_z) {
-                        if (has_cur) retire_cell<P>(a, acc, cur_rec, acc0, nx - cur_x,
-                                                    ny - cur_y, nz - cur_z, lh, pp);
+                        if (has_cur) retire_cell(a, acc, cur_rec, acc0, nx - cur_x,
+                                                 ny - cur_y, nz - cur_z, lh, pp, live);
                         has_cur = true;
                         cur_x = nx;
                         cur_y = ny;
@@ -545,8 +557,8 @@ __device__ __forceinline__ void march8_ray(const RenderArgs &a, const int64_t b)
 
         if (bwd && has_cur) {
             // Retire the last cell: a displacement of 2 shares no corner.
-            retire_cell<P>(a, acc, cur_rec, acc0, 2, 2, 2, lh, pp);
-            flush_record0<P>(a, acc0, lh, pp);
+            retire_cell(a, acc, cur_rec, acc0, 2, 2, 2, lh, pp, live);
+            flush_record0(a, acc0, lh, pp, live, b);
         }
 
         if (!bwd) {
@@ -730,9 +742,11 @@ __device__ __forceinline__ void log_grad_ray(const RenderArgs &a, const int64_t 
     for (int k = 0; k < S; k++) Y[k] = sh_basis(k, fdx, fdy, fdz);
     const int lh = lane >> 5, pp = lane & 31;
     const int pch = pp == 0 ? -1 : (pp - 1) / S;
+    const bool live = pp == 0 || (pp < P && (pp - 1) % S >= a.sh_first && (pp - 1) % S < a.sh_active);
     float ysel = 0.0f;
 #pragma unroll
     for (int k = 0; k < S; k++) ysel = (pp >= 1 && (pp - 1) % S == k) ? Y[k] : ysel;
+    if (!live) ysel = 0.0f;
 
     double T = 1.0, Acc = 0.0, Cr = 0.0, Cg = 0.0, Cb = 0.0;
     if (chunk > 0) {
@@ -747,7 +761,7 @@ __device__ __forceinline__ void log_grad_ray(const RenderArgs &a, const int64_t 
     int cur_x = 0, cur_y = 0, cur_z = 0;
     uint32_t cur_rec[4] = {0u, 0u, 0u, 0u};
     float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-    float acc0 = 0.0f;
+    double acc0 = 0.0;
     uint64_t tmask = 0ull;  // bins holding weight (proposal targets)
     const double tfar = a.rays.far_t[ray];
     for (int c0 = c_begin; c0 < c_end; c0 += 64) {
@@ -799,8 +813,8 @@ __device__ __forceinline__ void log_grad_ray(const RenderArgs &a, const int64_t 
             const int nz = __builtin_amdgcn_readlane(bz, i);
             if (!has_cur || nx != cur_x || ny != cur_y || nz != cur_z) {
                 if (has_cur)
-                    retire_cell<P>(a, acc, cur_rec, acc0, nx - cur_x, ny - cur_y, nz - cur_z, lh,
-                                   pp);
+                    retire_cell(a, acc, cur_rec, acc0, nx - cur_x, ny - cur_y, nz - cur_z, lh, pp,
+                                live);
                 has_cur = true;
                 cur_x = nx;
                 cur_y = ny;
@@ -830,8 +844,8 @@ __device__ __forceinline__ void log_grad_ray(const RenderArgs &a, const int64_t 
         Cb = readlane_d(H2, 63);
     }
     if (has_cur) {
-        retire_cell<P>(a, acc, cur_rec, acc0, 2, 2, 2, lh, pp);
-        flush_record0<P>(a, acc0, lh, pp);
+        retire_cell(a, acc, cur_rec, acc0, 2, 2, 2, lh, pp, live);
+        flush_record0(a, acc0, lh, pp, live, b);
     }
     if (TGT && lane == 0) {
         a.bin_targets[b] = tmask;

@@ -14,11 +14,22 @@ struct TvArgs {
     const uint32_t *occ;
     const uint32_t *io;
     const uint32_t *c2d;
+    const uint8_t *cage;  // in-cage flag per dense vertex
     int W, H, D, stride, P;
+    int S, sh_lo, sh_hi;  // SH terms for coefficients [sh_lo, sh_hi) of each channel
     int sparse;
     int64_t n_vox;
     float lam_d, lam_sh;
 };
+
+// Coefficient of parameter p: the density's, or the SH one for the
+// coefficients of the band being optimised (startSHIdx .. (optimizingBand +
+// 1)^2, RenderUtils.cu:2480-2493, 2594), else 0.
+__device__ __forceinline__ float tv_lambda(const TvArgs &a, int p) {
+    if (p == 0) return a.lam_d;
+    const int k = (p - 1) % a.S;
+    return (k >= a.sh_lo && k < a.sh_hi) ? a.lam_sh : 0.0f;
+}
 
 template <int STORE>
 __device__ __forceinline__ float ld(const float *base, size_t idx) {
@@ -34,10 +45,14 @@ __device__ __forceinline__ bool cell_occupied(const TvArgs &a, int cx, int cy, i
     return (a.occ[cell >> 5] >> (cell & 31)) & 1u;
 }
 
+// Record of vertex (x, y, z); an index beyond the grid's records (a caged
+// index that does not belong to this grid) reads record 0.
 __device__ __forceinline__ uint32_t record_of(const TvArgs &a, int x, int y, int z) {
     const uint32_t d = (uint32_t)z * (uint32_t)a.W * (uint32_t)a.H + (uint32_t)y * (uint32_t)a.W +
                        (uint32_t)x;
-    return a.sparse ? a.io[d] : d;
+    if (!a.sparse) return d;
+    const uint32_t r = a.io[d];
+    return (int64_t)r < a.n_vox ? r : 0u;
 }
 
 // Cells around a vertex that contain its neighbour i = (dz+1)*9 + (dy+1)*3 +
@@ -70,7 +85,7 @@ __device__ __forceinline__ uint32_t tv_cells_of(int i) {
 template <int LPV, int STORE>
 __device__ __forceinline__ void tv_record(const TvArgs &a, int64_t r, int p, uint32_t nrec,
                                           uint32_t cmask) {
-    const float lam = p == 0 ? a.lam_d : a.lam_sh;
+    const float lam = tv_lambda(a, p);
     const bool mine = p < a.P && lam != 0.0f;
     // Every lane issues all 27 loads unconditionally, so they are in flight
     // together (a load under a branch is waited for at the branch's end).
@@ -157,7 +172,7 @@ __device__ __forceinline__ float ld_buf(__amdgpu_buffer_rsrc_t rs, uint32_t voff
 template <int STORE, bool OFF32>
 __device__ __forceinline__ void tv_interior(const TvArgs &a, int64_t r, int p) {
     constexpr uint32_t kElem = STORE == PXL_STORE_F32 ? 4u : 2u;
-    const float lam = p == 0 ? a.lam_d : a.lam_sh;
+    const float lam = tv_lambda(a, p);
     const bool mine = p < a.P && lam != 0.0f;
     const uint32_t q = mine ? (uint32_t)p : 0u;
     const int64_t plane = (int64_t)a.W * a.H;
@@ -220,7 +235,7 @@ __global__ __launch_bounds__(256) void k_tv_zmarch(const TvArgs a, int64_t n_col
     const int g = threadIdx.x / LPV, p = threadIdx.x % LPV;
     const int64_t col = (int64_t)xcd_slice_block(blockIdx.x, gridDim.x) * kGroups + g;
     if (col >= n_cols) return;
-    const float lam = p == 0 ? a.lam_d : a.lam_sh;
+    const float lam = tv_lambda(a, p);
     if (p >= a.P || lam == 0.0f) return;
     const int nx = a.W - 2, ny = a.H - 2;
     const int x = 1 + (int)(col % nx), y = 1 + (int)((col / nx) % ny);
@@ -376,7 +391,7 @@ __global__ __launch_bounds__(256) void k_tv_disparity_table(const TvArgs a,
     // A record index beyond the grid (scratch not prepared for this cage)
     // reads record 0 instead of faulting.
     if (p < 27 && (int64_t)e >= a.n_vox) e = 0u;
-    const float lam = p == 0 ? a.lam_d : a.lam_sh;
+    const float lam = tv_lambda(a, p);
     const bool mine = p < a.P && lam != 0.0f;
     // Every lane issues all 27 loads unconditionally, so they are in flight
     // together; a lane without a parameter reads element 0.
@@ -535,6 +550,7 @@ __global__ __launch_bounds__(256) void k_alias_weights(const TvArgs a, int64_t n
         const uint32_t d = (uint32_t)(z + jz) * (uint32_t)a.W * (uint32_t)a.H +
                            (uint32_t)(y + jy) * (uint32_t)a.W + (uint32_t)(x + jx);
         rec[j] = a.io[d];
+        if ((int64_t)rec[j] >= a.n_vox) rec[j] = 0u;  // not this grid's index: record 0
         if (rec[j] == 0u && d != d0) n++;
     }
     if (n == 0) return;
@@ -607,7 +623,7 @@ __global__ __launch_bounds__(256) void k_alias_final(const TvArgs a, const Alias
         __syncthreads();
     }
     if (t != 0 || p >= a.P) return;
-    const float lam = p == 0 ? a.lam_d : a.lam_sh;
+    const float lam = tv_lambda(a, p);
     if (lam == 0.0f || hdr->magic != kAliasMagic || hdr->n_vox != a.n_vox) return;
     const double u0 = (double)ld<STORE>(a.params, (size_t)p);
     a.grad[p] += (float)((double)lam * ((double)hdr->n_alias * u0 - red[0]));
@@ -623,20 +639,44 @@ __global__ void k_alias_init(AliasHeader *hdr, float *w, int64_t n_vox) {
     }
 }
 
-// totalVarianceLoss, 6-neighbour branch (RenderUtils.cu:2660-2763), dense
-// grids: interior vertices, per parameter plane
-// grad += lambda / (sqrt(sum (u - u_n)^2) / 6 + 1e-6) * sum (u - u_n);
-// a neighbour whose linear index is 0 counts as the vertex itself.
-template <int LPV, int STORE>
-__global__ __launch_bounds__(256) void k_tv_neighbour6(const TvArgs a) {
+// totalVarianceLoss, 6-neighbour branch (RenderUtils.cu:2660-2763): for an
+// interior vertex and each parameter plane,
+//   grad += lambda / (sqrt(sum (u - u_n)^2) / 6 + 1e-6) * sum (u - u_n)
+// over the 6 face neighbours; a neighbour whose index (record, when caged)
+// is 0 counts as the vertex itself.  One group of LPV lanes (one per
+// parameter) per record; a dense record is its vertex.  A caged record r > 0
+// belongs to one vertex, c2d[r]; the reference tests the in-cage flag array
+// (indexed by dense vertex) at the record number r (RenderUtils.cu:2669),
+// which is kept.  Record 0 (every vertex outside the cage, and the first one
+// in it) is k_tv_neighbour6_rec0's.  Only vertices whose column y W + x is
+// below col_limit take part (the launch of the reference's kernel covers
+// that many columns).
+__device__ __forceinline__ double tv6_term(float self, const float (&un)[6], float lam) {
+    double sq = 0.0, dis = 0.0;
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        const double d = (double)self - (double)un[i];
+        sq += d * d;
+        dis += d;
+    }
+    const double sd = sqrt(sq) / 6.0;
+    return (double)lam / (sd + 1e-6) * dis;
+}
+
+template <int LPV, int STORE, bool SPARSE>
+__global__ __launch_bounds__(256) void k_tv_neighbour6(const TvArgs a, int64_t col_limit) {
     constexpr int kGroups = 256 / LPV;
     const int g = threadIdx.x / LPV, p = threadIdx.x % LPV;
     const int64_t r = (int64_t)blockIdx.x * kGroups + g;
-    if (r >= a.n_vox || p >= a.P) return;
-    const float lam = p == 0 ? a.lam_d : a.lam_sh;
+    if (r >= a.n_vox || p >= a.P || (SPARSE && r == 0)) return;
+    const float lam = tv_lambda(a, p);
     if (lam == 0.0f) return;
-    const int x = (int)(r % a.W), y = (int)((r / a.W) % a.H), z = (int)(r / ((int64_t)a.W * a.H));
+    const uint32_t d = SPARSE ? a.c2d[r] : (uint32_t)r;
+    const int x = (int)(d % (uint32_t)a.W), y = (int)((d / (uint32_t)a.W) % (uint32_t)a.H);
+    const int z = (int)(d / ((uint32_t)a.W * (uint32_t)a.H));
     if (x == 0 || x == a.W - 1 || y == 0 || y == a.H - 1 || z == 0 || z == a.D - 1) return;
+    if ((int64_t)y * a.W + x >= col_limit) return;
+    if (SPARSE && !a.cage[r]) return;
     const float self = ld<STORE>(a.params, (size_t)r * a.stride + p);
     const int off[6][3] = {{0, 0, -1}, {0, -1, 0}, {-1, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
     // Unconditional loads (index 0 reads the vertex itself), all in flight
@@ -647,15 +687,44 @@ __global__ __launch_bounds__(256) void k_tv_neighbour6(const TvArgs a) {
         const uint32_t n = record_of(a, x + off[i][0], y + off[i][1], z + off[i][2]);
         un[i] = ld<STORE>(a.params, (size_t)(n != 0u ? n : (uint32_t)r) * a.stride + p);
     }
-    double sq = 0.0, dis = 0.0;
+    a.grad[(size_t)r * a.gstride + p] += (float)tv6_term(self, un, lam);
+}
+
+// Record 0 of a caged grid under the 6-neighbour term: every interior
+// vertex whose record is 0 adds its term, with u = record 0, when the
+// in-cage flag of dense vertex 0 is set (the reference's test at record
+// number 0).  Thread per dense vertex; a vertex whose 6 neighbours all read
+// record 0 adds 0.  The adds into record 0 are float atomics (the reference
+// races on them).
+template <int STORE>
+__global__ __launch_bounds__(256) void k_tv_neighbour6_rec0(const TvArgs a, int64_t col_limit) {
+    if (!a.cage[0]) return;
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nv = (int64_t)a.W * a.H * a.D;
+    if (v >= nv) return;
+    const int x = (int)(v % a.W), y = (int)((v / a.W) % a.H), z = (int)(v / ((int64_t)a.W * a.H));
+    if (x == 0 || x == a.W - 1 || y == 0 || y == a.H - 1 || z == 0 || z == a.D - 1) return;
+    if ((int64_t)y * a.W + x >= col_limit) return;
+    if (record_of(a, x, y, z) != 0u) return;
+    const int off[6][3] = {{0, 0, -1}, {0, -1, 0}, {-1, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    uint32_t nrec[6];
+    bool any = false;
 #pragma unroll
     for (int i = 0; i < 6; i++) {
-        const double d = (double)self - (double)un[i];
-        sq += d * d;
-        dis += d;
+        nrec[i] = record_of(a, x + off[i][0], y + off[i][1], z + off[i][2]);
+        any = any || nrec[i] != 0u;
     }
-    const double s = sqrt(sq) / 6.0;
-    a.grad[(size_t)r * a.gstride + p] += (float)((double)lam / (s + 1e-6) * dis);
+    if (!any) return;
+    for (int p = 0; p < a.P; p++) {
+        const float lam = tv_lambda(a, p);
+        if (lam == 0.0f) continue;
+        const float self = ld<STORE>(a.params, (size_t)p);
+        float un[6];
+#pragma unroll
+        for (int i = 0; i < 6; i++)
+            un[i] = nrec[i] != 0u ? ld<STORE>(a.params, (size_t)nrec[i] * a.stride + p) : self;
+        atomicAdd(a.grad + p, (float)tv6_term(self, un, lam));
+    }
 }
 
 // updateRGBSceneGPU_ADAM (Optimizer.cu:359-415).  One thread per 4
@@ -694,8 +763,9 @@ __global__ __launch_bounds__(256) void k_adam(float *__restrict__ prm, float *__
                                               float *__restrict__ mom, float *__restrict__ var,
                                               float *__restrict__ master,
                                               const float *__restrict__ alr, int64_t n_vox,
-                                              int pstride, int gstride, int P, int Q, double lr,
-                                              double sh_lr, double bc1, double bc2) {
+                                              int pstride, int gstride, int P, int Q, int S,
+                                              int sh_lo, int sh_hi, double lr, double sh_lr,
+                                              double bc1, double bc2) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_vox * Q) return;
     const int64_t r = i / Q;
@@ -727,6 +797,9 @@ __global__ __launch_bounds__(256) void k_adam(float *__restrict__ prm, float *__
     for (int e = 0; e < 4; e++) {
         const int p = 4 * q + e;
         if (p >= P) break;
+        // SH coefficients outside the optimised band keep their state
+        // (Optimizer.cu:393-412).
+        if (p > 0 && ((p - 1) % S < sh_lo || (p - 1) % S >= sh_hi)) continue;
         const float gg = gp[e];
         const float mm = (float)((double)(b1 * mp[e]) + (1.0 - (double)b1) * (double)gg);
         const float vv =
@@ -754,6 +827,25 @@ __global__ __launch_bounds__(256) void k_adam(float *__restrict__ prm, float *__
     }    *reinterpret_cast<float4 *>(grd + off) = g4;
     st_stream(mom + off, m4);
     st_stream(var + off, v4);
+}
+
+// Optimiser state of a new band (bandMemoryManage / initialAll,
+// Optimizer.cu:11-64, 161-187): element p of every record of grad, m and v
+// becomes 0 when `all`, or when it is the density and `density`, or an SH
+// coefficient k in [lo, hi) of any channel.
+__global__ __launch_bounds__(256) void k_init_band(float *grad, float *m, float *v,
+                                                   int64_t n_vox, int stride, int P, int S,
+                                                   int lo, int hi, int density, int all) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_vox * stride) return;
+    const int p = (int)(i % stride);
+    bool clear = all != 0;
+    if (p == 0) clear = clear || density != 0;
+    else if (p < P) clear = clear || ((p - 1) % S >= lo && (p - 1) % S < hi);
+    if (!clear) return;
+    grad[i] = 0.0f;
+    m[i] = 0.0f;
+    v[i] = 0.0f;
 }
 
 __global__ __launch_bounds__(1024) void k_loss_reduce(const double *__restrict__ l, int64_t n,
@@ -802,12 +894,10 @@ int32_t tv_check(const pxl_grid *g, const pxl_optim *o, const char *who) {
     return PXL_OK;
 }
 
-// PXL_TV_OFFSETS=64 forces the 64-bit address form of the TV gathers, which
-// otherwise serves only grids beyond 2^24 records or 4 GiB (both are tested).
-bool offsets64_forced() {
-    const char *e = getenv("PXL_TV_OFFSETS");
-    return e && strcmp(e, "64") == 0;
-}
+// The PXL_OPT_TV_ADDR64 option forces the 64-bit address form of the TV
+// gathers, which otherwise serves only grids beyond 2^24 records or 4 GiB
+// (both are tested).
+bool offsets64_forced() { return option_tv_addr64(); }
 
 template <int STORE>
 void tv_launch(const TvArgs &a, int type, void *scratch, hipStream_t s) {
@@ -884,9 +974,31 @@ void tv_launch(const TvArgs &a, int type, void *scratch, hipStream_t s) {
             hipLaunchKernelGGL(k_alias_final<STORE>, dim3((unsigned)a.P), dim3(256), 0, s, a, hdr,
                                part);
         }
+    }
+}
+
+// The 6-neighbour term over the columns y W + x < col_limit.
+template <int STORE>
+void tv6_launch(const TvArgs &a, int64_t col_limit, hipStream_t s) {
+    const bool wide = a.P > 32;
+    const int groups = wide ? 4 : 8;
+    const unsigned blocks = (unsigned)((a.n_vox + groups - 1) / groups);
+    if (a.sparse) {
+        if (wide)
+            hipLaunchKernelGGL((k_tv_neighbour6<64, STORE, true>), dim3(blocks), dim3(256), 0, s, a,
+                               col_limit);
+        else
+            hipLaunchKernelGGL((k_tv_neighbour6<32, STORE, true>), dim3(blocks), dim3(256), 0, s, a,
+                               col_limit);
+        const int64_t nv = (int64_t)a.W * a.H * a.D;
+        hipLaunchKernelGGL(k_tv_neighbour6_rec0<STORE>, dim3((unsigned)((nv + 255) / 256)),
+                           dim3(256), 0, s, a, col_limit);
+    } else if (wide) {
+        hipLaunchKernelGGL((k_tv_neighbour6<64, STORE, false>), dim3(blocks), dim3(256), 0, s, a,
+                           col_limit);
     } else {
-        if (wide) hipLaunchKernelGGL((k_tv_neighbour6<64, STORE>), dim3(blocks), dim3(256), 0, s, a);
-        else hipLaunchKernelGGL((k_tv_neighbour6<32, STORE>), dim3(blocks), dim3(256), 0, s, a);
+        hipLaunchKernelGGL((k_tv_neighbour6<32, STORE, false>), dim3(blocks), dim3(256), 0, s, a,
+                           col_limit);
     }
 }
 
@@ -907,11 +1019,15 @@ static void fill_tv_args(const pxl_grid *grid, const pxl_optim *optim, TvArgs &a
     a.occ = grid->occupancy;
     a.io = grid->index_offsets;
     a.c2d = grid->compact_to_dense;
+    a.cage = grid->in_cage;
     a.W = grid->W;
     a.H = grid->H;
     a.D = grid->D;
     a.stride = grid->stride;
     a.P = 1 + 3 * grid->sh_dim;
+    a.S = grid->sh_dim;
+    a.sh_lo = optim ? optim->sh_first : 0;
+    a.sh_hi = grid->sh_active > 0 ? grid->sh_active : grid->sh_dim;
     a.sparse = grid->sparse;
     a.n_vox = grid->n_vox;
     a.lam_d = 0.0f;
@@ -961,9 +1077,16 @@ extern "C" int32_t pxl_add_variance_loss(const pxl_grid *grid, const pxl_optim *
     clear_error();
     int32_t st = tv_check(grid, optim, "pxl_add_variance_loss");
     if (st != PXL_OK) return st;
-    PXL_REQUIRE(type == PXL_TV_DISPARITY || type == PXL_TV_NEIGHBOUR,
+    PXL_REQUIRE(type == PXL_TV_DISPARITY || type == PXL_TV_NEIGHBOUR || type == PXL_TV_BOTH,
                 "pxl_add_variance_loss: unsupported type %d", type);
-    if (type == PXL_TV_DISPARITY && grid->sparse) {
+    const bool disparity = type == PXL_TV_DISPARITY || type == PXL_TV_BOTH;
+    const bool nbr6 = type == PXL_TV_NEIGHBOUR || type == PXL_TV_BOTH;
+    if (nbr6) {
+        PXL_REQUIRE(neighbours == 6, "pxl_add_variance_loss: only the 6-neighbour TV is built");
+        PXL_REQUIRE(!grid->sparse || grid->in_cage,
+                    "pxl_add_variance_loss: the 6-neighbour TV of a caged grid needs in_cage");
+    }
+    if (disparity && grid->sparse) {
         const size_t need = pxl_tv_scratch_bytes(grid);
         if (!scratch || scratch_bytes < need || ((uintptr_t)scratch & 255)) {
             set_error("pxl_add_variance_loss: sparse grids need %zu bytes of scratch prepared "
@@ -973,17 +1096,35 @@ extern "C" int32_t pxl_add_variance_loss(const pxl_grid *grid, const pxl_optim *
     }
     TvArgs a;
     fill_tv_args(grid, optim, a);
+    PXL_REQUIRE(a.sh_hi <= a.S && a.sh_lo >= 0 && a.sh_lo < a.sh_hi,
+                "pxl_add_variance_loss: band [%d, %d) outside the grid's %d coefficients",
+                a.sh_lo, a.sh_hi, a.S);
     a.lam_d = (float)lambda_density;
     hipStream_t s = as_stream(stream);
-    if (type == PXL_TV_NEIGHBOUR) {
-        PXL_REQUIRE(neighbours == 6, "pxl_add_variance_loss: only the 6-neighbour TV is built");
-        PXL_REQUIRE(!grid->sparse, "pxl_add_variance_loss: 6-neighbour TV needs a dense grid");
+    // The 6-neighbour term first (VarianceType 2 runs it, then the disparity
+    // term; RenderUtils.cu:2840-2845), over the columns its launch covers:
+    // all W x H for type 0, ceil((W-1)/16) x ceil((H-1)/16) blocks of 256 for
+    // type 2.
+    if (nbr6) {
+        a.lam_sh = (float)lambda_sh;
+        const int64_t limit =
+            type == PXL_TV_NEIGHBOUR
+                ? (int64_t)grid->W * grid->H
+                : (int64_t)((grid->W - 1 + 15) / 16) * ((grid->H - 1 + 15) / 16) * 256;
+        switch (grid->storage) {
+            case PXL_STORE_F32: tv6_launch<PXL_STORE_F32>(a, limit, s); break;
+            case PXL_STORE_BF16: tv6_launch<PXL_STORE_BF16>(a, limit, s); break;
+            default: tv6_launch<PXL_STORE_F16>(a, limit, s); break;
+        }
     }
-    a.lam_sh = (type == PXL_TV_NEIGHBOUR || optimize_step <= 1) ? (float)lambda_sh : 0.0f;
-    switch (grid->storage) {
-        case PXL_STORE_F32: tv_launch<PXL_STORE_F32>(a, type, scratch, s); break;
-        case PXL_STORE_BF16: tv_launch<PXL_STORE_BF16>(a, type, scratch, s); break;
-        default: tv_launch<PXL_STORE_F16>(a, type, scratch, s); break;
+    if (disparity) {
+        // SH terms only for optimize_step <= 1 (RenderUtils.cu:2493).
+        a.lam_sh = optimize_step <= 1 ? (float)lambda_sh : 0.0f;
+        switch (grid->storage) {
+            case PXL_STORE_F32: tv_launch<PXL_STORE_F32>(a, PXL_TV_DISPARITY, scratch, s); break;
+            case PXL_STORE_BF16: tv_launch<PXL_STORE_BF16>(a, PXL_TV_DISPARITY, scratch, s); break;
+            default: tv_launch<PXL_STORE_F16>(a, PXL_TV_DISPARITY, scratch, s); break;
+        }
     }
     return check_launch("pxl_add_variance_loss");
 }
@@ -1012,6 +1153,10 @@ extern "C" int32_t pxl_adam_step(const pxl_grid *grid, const pxl_optim *optim, d
     PXL_REQUIRE(grid->n_vox >= 0, "pxl_adam_step: bad n_vox");
     if (grid->n_vox == 0) return PXL_OK;
     const int Q = (P + 3) / 4;
+    const int sh_hi = grid->sh_active > 0 ? grid->sh_active : grid->sh_dim;
+    PXL_REQUIRE(sh_hi <= grid->sh_dim && optim->sh_first >= 0 && optim->sh_first < sh_hi,
+                "pxl_adam_step: band [%d, %d) outside the grid's %d coefficients", optim->sh_first,
+                sh_hi, grid->sh_dim);
     // Bias corrections with float pow as in the reference (Optimizer.cu:384-385).
     const double bc1 = 1.0 - (double)powf(0.9f, (float)t);
     const double bc2 = 1.0 - (double)powf(0.999f, (float)t);
@@ -1022,8 +1167,8 @@ extern "C" int32_t pxl_adam_step(const pxl_grid *grid, const pxl_optim *optim, d
     hipStream_t s = as_stream(stream);
 #define PXL_ADAM(ST)                                                                            \
     hipLaunchKernelGGL(k_adam<ST>, gdim, dim3(256), 0, s, grid->params, optim->grad, optim->m, \
-                       optim->v, master, alr, grid->n_vox, pstride, gstride, P, Q, lr, sh_lr,  \
-                       bc1, bc2)
+                       optim->v, master, alr, grid->n_vox, pstride, gstride, P, Q, grid->sh_dim, \
+                       optim->sh_first, sh_hi, lr, sh_lr, bc1, bc2)
     switch (grid->storage) {
         case PXL_STORE_F32: PXL_ADAM(PXL_STORE_F32); break;
         case PXL_STORE_BF16: PXL_ADAM(PXL_STORE_BF16); break;
@@ -1040,4 +1185,29 @@ extern "C" int32_t pxl_loss_reduce_f64(const double *losses, int64_t len, double
     PXL_REQUIRE(len > 0, "pxl_loss_reduce_f64: empty batch");
     hipLaunchKernelGGL(k_loss_reduce, dim3(1), dim3(1024), 0, as_stream(stream), losses, len, out);
     return check_launch("pxl_loss_reduce_f64");
+}
+
+extern "C" int32_t pxl_optim_init_band(const pxl_grid *grid, const pxl_optim *optim,
+                                       int32_t band, int32_t band_mode,
+                                       int32_t density_initialized, void *stream) {
+    clear_error();
+    PXL_REQUIRE(grid && optim && optim->grad && optim->m && optim->v,
+                "pxl_optim_init_band: null argument");
+    PXL_REQUIRE(grid->sh_dim >= 1 && grid->n_vox >= 0, "pxl_optim_init_band: bad grid");
+    PXL_REQUIRE(band >= 0 && (band + 1) * (band + 1) <= grid->sh_dim,
+                "pxl_optim_init_band: band %d beyond the grid's %d coefficients", band,
+                grid->sh_dim);
+    const int P = 1 + 3 * grid->sh_dim;
+    const int stride = optim->stride > 0 ? optim->stride : grid->stride;
+    PXL_REQUIRE(stride >= P, "pxl_optim_init_band: stride %d below %d", stride, P);
+    PXL_REQUIRE(grid->storage == PXL_STORE_F32 || optim->stride > 0,
+                "pxl_optim_init_band: 16-bit grids need an explicit optimiser stride");
+    const int64_t total = grid->n_vox * (int64_t)stride;
+    if (total == 0) return PXL_OK;
+    const int all = band_mode ? 0 : 1;
+    const int density = band == 0 || !density_initialized ? 1 : 0;
+    hipLaunchKernelGGL(k_init_band, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                       as_stream(stream), optim->grad, optim->m, optim->v, grid->n_vox, stride, P,
+                       grid->sh_dim, band * band, (band + 1) * (band + 1), density, all);
+    return check_launch("pxl_optim_init_band");
 }

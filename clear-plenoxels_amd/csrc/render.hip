@@ -43,6 +43,9 @@ struct RenderArgs {
     const uint32_t *io;
     int W, H, D, stride;
     int gstride;  // floats per gradient record
+    // SH coefficients k < sh_active enter the colour; gradients go to the
+    // density and to k in [sh_first, sh_active).
+    int sh_active, sh_first;
     float step;
     int white_bg;
     pxl_rays rays;
@@ -72,6 +75,11 @@ struct RenderArgs {
     // Batch entries longer than the log, appended by k_march8<kLog>; their
     // count is work[kRedoCounter].
     int32_t *redo_list;
+    // Logged step: per parameter p and batch entry b the entry's FP64 sum
+    // into record 0 ([32][len], zeroed by the forward), folded into the
+    // gradient in a fixed order by k_rec0_fold.  Null: record 0 takes one
+    // float atomic per ray.
+    double *rec0;
     // Persistent launches: 8 claim counters per kernel (zeroed before launch).
     unsigned long long *work;
     // Learned proposal: per batch entry, the bins a ray may evaluate (null:
@@ -207,28 +215,34 @@ template <typename R> __device__ __forceinline__ R readlane(R v, int l);
 template <> __device__ __forceinline__ float readlane<float>(float v, int l) { return readlane_f(v, l); }
 template <> __device__ __forceinline__ double readlane<double>(double v, int l) { return readlane_d(v, l); }
 
-// Unnormalised real SH basis (RenderUtils.cu:1639-1670), FP32 as in the
-// reference; entry k of the first 16.
-__device__ __forceinline__ float sh_basis(int k, float x, float y, float z) {
+// Unnormalised real SH basis (RenderUtils.cu:1639-1670) of the FP32
+// direction; entry k of the first 16.  Evaluated in FP64 (once per ray):
+// entries such as 3 z^2 - 1 cancel, and their FP32 evaluation (the
+// reference's) carries an absolute error of ~1e-7 that the gradient sums
+// of many rays then pile up on entries that nearly cancel.
+__device__ __forceinline__ double sh_basis_d(int k, double x, double y, double z) {
     switch (k) {
-        case 0: return 1.0f;
+        case 0: return 1.0;
         case 1: return y;
         case 2: return z;
         case 3: return x;
         case 4: return x * y;
         case 5: return y * z;
-        case 6: return 3.0f * z * z - 1.0f;
+        case 6: return 3.0 * z * z - 1.0;
         case 7: return x * z;
         case 8: return x * x - y * y;
-        case 9: return y * (3.0f * x * x - y * y);
+        case 9: return y * (3.0 * x * x - y * y);
         case 10: return x * y * z;
-        case 11: return y * (5.0f * z * z - 1.0f);
-        case 12: return 5.0f * z * z * z - 3.0f * z;
-        case 13: return x * (5.0f * z * z - 1.0f);
+        case 11: return y * (5.0 * z * z - 1.0);
+        case 12: return 5.0 * z * z * z - 3.0 * z;
+        case 13: return x * (5.0 * z * z - 1.0);
         case 14: return (x * x - y * y) * z;
-        case 15: return x * (x * x - 3.0f * y * y);
-        default: return 0.0f;
+        case 15: return x * (x * x - 3.0 * y * y);
+        default: return 0.0;
     }
+}
+__device__ __forceinline__ float sh_basis(int k, float x, float y, float z) {
+    return (float)sh_basis_d(k, (double)x, (double)y, (double)z);
 }
 
 // One sample position: base cell, corner records and trilinear weights.
@@ -406,7 +420,9 @@ __global__ __launch_bounds__(256) void k_render(const RenderArgs a) {
     int p = -1;
     if (row < 3 && kk < S) p = 1 + row * S + kk;
     else if (row == 3 && kk == 0) p = 0;
-    const R ylane = row < 3 ? (R)sh_basis(kk, fdx, fdy, fdz) : (R)1;
+    // Coefficients from sh_active on do not enter the colour (the band limit
+    // of trilerpRGB, RenderUtils.cu:871).
+    const R ylane = row < 3 ? (kk < a.sh_active ? (R)sh_basis_d(kk, fdx, fdy, fdz) : (R)0) : (R)1;
 
     const bool do_fwd =
         MODE == kFwd || MODE == kFused || MODE == kTrace || MODE == kTouch || MODE == kImage;
@@ -577,7 +593,9 @@ __global__ __launch_bounds__(256) void k_render(const RenderArgs a) {
 
     double t = tnear, T = 1.0, Ai = 0.0, H0 = 0.0, H1 = 0.0, H2 = 0.0;
     int k = 0;
-    const bool active = p >= 0;
+    // Lanes whose parameter takes a gradient: the density and the SH
+    // coefficients of the optimised band (RenderUtils.cu:1117-1132).
+    const bool active = p >= 0 && (row == 3 || (kk >= a.sh_first && kk < a.sh_active));
     float acc[8];
     float acc0 = 0.0f;
     uint32_t cv[8];
@@ -689,11 +707,8 @@ __global__ __launch_bounds__(256) void k_render(const RenderArgs a) {
 // Marcher selection: the eight-samples-per-wave kernel serves the fixed rule
 // for S <= 9 in the forward, backward, fused and trace modes; the one-sample
 // kernel serves the adaptive rule, S = 16 and the touched-record count.
-// PXL_MARCHER=wave forces the one-sample kernel (both are tested).
-bool use_march8() {
-    const char *e = getenv("PXL_MARCHER");
-    return !(e && strcmp(e, "wave") == 0);
-}
+// The PXL_OPT_MARCHER option forces the one-sample kernel (both are tested).
+bool use_march8() { return option_group_marcher(); }
 
 template <int S, bool SPARSE, bool ADAPT, int MODE, int STORE>
 int32_t launch3(const RenderArgs &a, hipStream_t st) {
@@ -773,6 +788,8 @@ int32_t check_grid(const pxl_grid *g, const char *who) {
     PXL_REQUIRE(g->step > 0.0f, "%s: step must be positive", who);
     PXL_REQUIRE(g->march_mode == PXL_MARCH_FIXED || g->march_mode == PXL_MARCH_REF_ADAPTIVE,
                 "%s: bad march_mode %d", who, g->march_mode);
+    PXL_REQUIRE(g->sh_active >= 0 && g->sh_active <= g->sh_dim,
+                "%s: sh_active %d outside [0, %d]", who, g->sh_active, g->sh_dim);
     return PXL_OK;
 }
 
@@ -798,6 +815,8 @@ int32_t prepare(const pxl_grid *g, const pxl_rays *r, const int32_t *sample_inde
     a.D = g->D;
     a.stride = g->stride;
     a.gstride = g->stride;
+    a.sh_active = g->sh_active > 0 ? g->sh_active : g->sh_dim;
+    a.sh_first = 0;
     a.step = g->step;
     a.white_bg = r->white_bg;
     a.rays = *r;
@@ -872,6 +891,7 @@ extern "C" int32_t pxl_render_image(const pxl_grid *grid, const pxl_camera *cam,
     a.D = grid->D;
     a.stride = grid->stride;
     a.gstride = grid->stride;
+    a.sh_active = grid->sh_active > 0 ? grid->sh_active : grid->sh_dim;
     a.step = grid->step;
     a.white_bg = white_bg;
     a.len = (int64_t)width * (int64_t)height;
@@ -926,30 +946,13 @@ extern "C" int32_t pxl_render_fwd_trace(const pxl_grid *grid, const pxl_rays *ra
 }
 
 namespace pxl {
-// Event that the next logged launch on the calling thread waits for between
-// its forward kernel and its gradient kernels, then forgets.  The trainer runs
-// the regulariser on a second stream beside the forward; the regulariser
-// writes the gradient planes that the gradient kernels add into.
-static thread_local hipEvent_t t_grad_wait = nullptr;
-static thread_local bool t_grad_wait_failed = false;
-void set_grad_wait(hipEvent_t ev) {
-    t_grad_wait = ev;
-    t_grad_wait_failed = false;
-}
-bool grad_wait_failed() { return t_grad_wait_failed; }
-
 namespace {
-void wait_before_grad(hipStream_t st) {
-    if (!t_grad_wait) return;
-    if (hipStreamWaitEvent(st, t_grad_wait, 0) != hipSuccess) t_grad_wait_failed = true;
-    t_grad_wait = nullptr;
-}
-
-// Workspace of the two-kernel fused step: per batch entry 4 FP64 totals and a
-// count, per logged sample 8 records and 8 value words, per 64-sample chunk
-// of an entry's log 5 FP64 prefixes; 256-byte aligned.
+// Workspace of the two-kernel fused step: per batch entry 4 FP64 totals, a
+// count and 32 FP64 record-0 slots, per logged sample 8 records and 8 value
+// words, per 64-sample chunk of an entry's log 5 FP64 prefixes; 256-byte
+// aligned.
 struct LogLayout {
-    size_t work, tot, count, rec, val, pre, redo, total;
+    size_t work, tot, count, rec, val, pre, redo, rec0, total;
 };
 inline size_t align256(size_t x) { return (x + 255) / 256 * 256; }
 LogLayout log_layout(int64_t len, int32_t cap) {
@@ -969,6 +972,8 @@ LogLayout log_layout(int64_t len, int32_t cap) {
     o += align256((size_t)len * (size_t)((cap + 63) / 64) * 5 * sizeof(double));
     L.redo = o;
     o += align256((size_t)len * sizeof(int32_t));
+    L.rec0 = o;
+    o += align256((size_t)len * 32 * sizeof(double));
     L.total = o;
     return L;
 }
@@ -976,6 +981,32 @@ constexpr int32_t kMaxLogSamples = 1 << 16;
 
 __global__ void k_zero_u64(unsigned long long *p, int n) {
     if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
+}
+
+// Record 0 of the logged step: block p sums the batch entries' FP64 slots of
+// parameter p in a fixed order (thread t: entries t, t + 256, ...; then a
+// fixed tree over the threads) and adds the total to the gradient.  Every
+// corner outside the cage of a caged grid lands on record 0
+// (SceneUtils.cu:1106-1113), so this sum has ~10^5 largely cancelling terms.
+__global__ __launch_bounds__(256) void k_rec0_fold(const double *__restrict__ rec0, int64_t len,
+                                                   float *grad) {
+    __shared__ double red[256];
+    const int p = blockIdx.x, t = threadIdx.x;
+    const double *src = rec0 + (size_t)p * (size_t)len;
+    double s0 = 0.0, s1 = 0.0;
+    int64_t i = t;
+    for (; i + 256 < len; i += 512) {
+        s0 += src[i];
+        s1 += src[i + 256];
+    }
+    if (i < len) s0 += src[i];
+    red[t] = s0 + s1;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (t < h) red[t] += red[t + h];
+        __syncthreads();
+    }
+    if (t == 0 && red[0] != 0.0) grad[p] += (float)red[0];
 }
 
 // Blocks of `threads` threads of `kernel` that fit on the whole device at
@@ -997,8 +1028,14 @@ unsigned resident_blocks(K kernel, int threads) {
 // resident and let the waves claim batch entries.  The workspace's first
 // 17 words (claim counters, redo count) start at zero.
 template <int S, bool SPARSE, int STORE>
-void launch_logged(const RenderArgs &a, int wpb, int gwpb, bool persist, bool chunked,
-                   hipStream_t st) {
+int32_t launch_logged(const RenderArgs &a, int wpb, int gwpb, bool persist, bool chunked,
+                      hipEvent_t grad_wait, hipStream_t st) {
+    // The gradient kernels add into planes another stream may be writing
+    // (the trainer's variance loss): they start after grad_wait.
+    bool wait_ok = true;
+    const auto wait_before_grad = [&](hipStream_t s) {
+        if (grad_wait && hipStreamWaitEvent(s, grad_wait, 0) != hipSuccess) wait_ok = false;
+    };
     const int threads = 64 * wpb;
     const unsigned blocks = (unsigned)((a.len + wpb - 1) / wpb);
     // The gradient kernels' own workgroup shape.
@@ -1044,15 +1081,19 @@ void launch_logged(const RenderArgs &a, int wpb, int gwpb, bool persist, bool ch
     redo.log_redo = 1;
     hipLaunchKernelGGL((k_march8_redo<S, SPARSE, STORE>), dim3(std::min(blocks, 1024u)),
                        dim3(threads), 0, st, redo);
+    hipLaunchKernelGGL(k_rec0_fold, dim3(1 + 3 * S), dim3(256), 0, st, a.rec0, a.len, a.grad);
+    if (!wait_ok) {
+        set_error("render: hipStreamWaitEvent before the gradient kernels failed");
+        return PXL_EHIP;
+    }
+    return PXL_OK;
 }
 template <int S>
 int32_t dispatch_logged_s(bool sparse, int store, const RenderArgs &a, int wpb, int gwpb,
-                          bool persist, bool chunked, hipStream_t st) {
-#define PXL_LOGGED(SP, ST)                                              \
-    if (sparse == SP && store == ST) {                                  \
-        launch_logged<S, SP, ST>(a, wpb, gwpb, persist, chunked, st);   \
-        return PXL_OK;                                                  \
-    }
+                          bool persist, bool chunked, hipEvent_t grad_wait, hipStream_t st) {
+#define PXL_LOGGED(SP, ST)                                                        \
+    if (sparse == SP && store == ST)                                              \
+        return launch_logged<S, SP, ST>(a, wpb, gwpb, persist, chunked, grad_wait, st);
     PXL_LOGGED(true, PXL_STORE_F32)
     PXL_LOGGED(true, PXL_STORE_BF16)
     PXL_LOGGED(true, PXL_STORE_F16)
@@ -1064,11 +1105,11 @@ int32_t dispatch_logged_s(bool sparse, int store, const RenderArgs &a, int wpb, 
     return PXL_EINVAL;
 }
 int32_t dispatch_logged(int S, bool sparse, int store, const RenderArgs &a, int wpb, int gwpb,
-                        bool persist, bool chunked, hipStream_t st) {
+                        bool persist, bool chunked, hipEvent_t gw, hipStream_t st) {
     switch (S) {
-        case 1: return dispatch_logged_s<1>(sparse, store, a, wpb, gwpb, persist, chunked, st);
-        case 4: return dispatch_logged_s<4>(sparse, store, a, wpb, gwpb, persist, chunked, st);
-        case 9: return dispatch_logged_s<9>(sparse, store, a, wpb, gwpb, persist, chunked, st);
+        case 1: return dispatch_logged_s<1>(sparse, store, a, wpb, gwpb, persist, chunked, gw, st);
+        case 4: return dispatch_logged_s<4>(sparse, store, a, wpb, gwpb, persist, chunked, gw, st);
+        case 9: return dispatch_logged_s<9>(sparse, store, a, wpb, gwpb, persist, chunked, gw, st);
         default:
             set_error("render: the logged fused step supports sh_dim 1, 4 and 9");
             return PXL_EINVAL;
@@ -1077,14 +1118,20 @@ int32_t dispatch_logged(int S, bool sparse, int store, const RenderArgs &a, int 
 }  // namespace
 }  // namespace pxl
 
+namespace pxl {
+// Grids whose train step runs as the two-kernel logged step (forward + sample
+// log, then the gradient kernels): the fixed rule with the eight-sample
+// marcher, sh_dim 1, 4 or 9.
+bool logged_step_serves(const pxl_grid *g) {
+    return g && g->march_mode == PXL_MARCH_FIXED && g->sh_dim <= 9 && use_march8();
+}
+}  // namespace pxl
+
 static int32_t bwd_common(const pxl_grid *grid, const pxl_optim *optim, const pxl_rays *rays,
                           double *losses, const int32_t *sample_index, int64_t start,
                           int64_t len, double mask_coef, float huber_delta, void *stream,
-                          bool fused, void *workspace = nullptr, size_t workspace_bytes = 0,
-                          int32_t max_samples = 0, const pxl_schedule *sched = nullptr,
-                          const uint64_t *bin_mask = nullptr, uint64_t *bin_targets = nullptr,
-                          uint8_t *target_valid = nullptr, double target_wmin = 0.0) {
-    const char *who = workspace ? "pxl_train_rays_ws" : fused ? "pxl_train_rays" : "pxl_render_bwd_rgb";
+                          bool fused, const TrainExtra &x = TrainExtra()) {
+    const char *who = x.workspace ? "pxl_train_rays_ws" : fused ? "pxl_train_rays" : "pxl_render_bwd_rgb";
     clear_error();
     RenderArgs a;
     int32_t st = prepare(grid, rays, sample_index, start, len, who, a);
@@ -1095,9 +1142,12 @@ static int32_t bwd_common(const pxl_grid *grid, const pxl_optim *optim, const px
                 "%s: ground-truth planes must be non-null", who);
     PXL_REQUIRE(rays->acc_r && rays->acc_g && rays->acc_b && rays->acc_w && rays->acc_density,
                 "%s: forward planes must be non-null", who);
+    PXL_REQUIRE(optim->sh_first >= 0 && optim->sh_first < a.sh_active,
+                "%s: sh_first %d outside [0, %d)", who, optim->sh_first, a.sh_active);
     if (len == 0) return PXL_OK;
     a.grad = optim->grad;
     a.gstride = optim->stride > 0 ? optim->stride : grid->stride;
+    a.sh_first = optim->sh_first;
     PXL_REQUIRE(a.gstride >= 1 + 3 * grid->sh_dim && a.gstride % 4 == 0,
                 "%s: gradient stride %d must be a multiple of 4 and >= %d", who, a.gstride,
                 1 + 3 * grid->sh_dim);
@@ -1107,25 +1157,27 @@ static int32_t bwd_common(const pxl_grid *grid, const pxl_optim *optim, const px
     a.mask_coef = mask_coef;
     a.huber = huber_delta;
     const bool adapt = grid->march_mode == PXL_MARCH_REF_ADAPTIVE;
-    const bool group = !adapt && grid->sh_dim <= 9 && use_march8();
-    PXL_REQUIRE(!bin_mask || group,
+    const bool group = logged_step_serves(grid);
+    PXL_REQUIRE(!x.bin_mask || group,
                 "%s: proposal masks need the eight-sample marcher (fixed rule, sh_dim <= 9)", who);
-    PXL_REQUIRE(!bin_targets || (group && workspace && target_valid),
+    PXL_REQUIRE(!x.bin_targets || (group && x.workspace && x.target_valid),
                 "%s: proposal targets need the logged step and a validity array", who);
-    a.bin_mask = bin_mask;
-    a.bin_targets = bin_targets;
-    a.target_valid = target_valid;
-    a.target_wmin = target_wmin;
-    if (workspace && group) {
+    a.bin_mask = x.bin_mask;
+    a.bin_targets = x.bin_targets;
+    a.target_valid = x.target_valid;
+    a.target_wmin = x.target_wmin;
+    const pxl_schedule *sched = x.sched;
+    if (x.workspace && group) {
+        const int32_t max_samples = x.max_samples;
         PXL_REQUIRE(max_samples > 0 && max_samples <= kMaxLogSamples,
                     "%s: max_samples %d outside [1, %d]", who, max_samples, kMaxLogSamples);
         const LogLayout L = log_layout(len, max_samples);
-        PXL_REQUIRE(workspace_bytes >= L.total,
+        PXL_REQUIRE(x.workspace_bytes >= L.total,
                     "%s: workspace of %zu bytes, %zu needed for %lld rays x %d samples", who,
-                    workspace_bytes, L.total, (long long)len, max_samples);
-        PXL_REQUIRE(((uintptr_t)workspace & 255) == 0, "%s: workspace must be 256-byte aligned",
-                    who);
-        char *w = static_cast<char *>(workspace);
+                    x.workspace_bytes, L.total, (long long)len, max_samples);
+        PXL_REQUIRE(((uintptr_t)x.workspace & 255) == 0,
+                    "%s: workspace must be 256-byte aligned", who);
+        char *w = static_cast<char *>(x.workspace);
         a.log_tot = reinterpret_cast<double *>(w + L.tot);
         a.log_count = reinterpret_cast<int32_t *>(w + L.count);
         a.log_rec = reinterpret_cast<uint32_t *>(w + L.rec);
@@ -1133,6 +1185,7 @@ static int32_t bwd_common(const pxl_grid *grid, const pxl_optim *optim, const px
         a.log_cap = max_samples;
         a.log_pre = reinterpret_cast<double *>(w + L.pre);
         a.redo_list = reinterpret_cast<int32_t *>(w + L.redo);
+        a.rec0 = reinterpret_cast<double *>(w + L.rec0);
         a.log_chunks = (max_samples + 63) / 64;
         a.work = reinterpret_cast<unsigned long long *>(w + L.work);
         const int wpb = sched ? sched->waves_per_block : 4;
@@ -1144,9 +1197,14 @@ static int32_t bwd_common(const pxl_grid *grid, const pxl_optim *optim, const px
         PXL_REQUIRE(!sched || !(sched->persistent && sched->chunked),
                     "%s: the chunk-parallel gradient has no persistent form", who);
         st = dispatch_logged(grid->sh_dim, grid->sparse, grid->storage, a, wpb, gwpb,
-                             sched && sched->persistent, sched && sched->chunked,
+                             sched && sched->persistent, sched && sched->chunked, x.grad_wait,
                              as_stream(stream));
         return st != PXL_OK ? st : check_launch(who);
+    }
+    // One kernel writes the gradient: it starts after grad_wait.
+    if (x.grad_wait && hipStreamWaitEvent(as_stream(stream), x.grad_wait, 0) != hipSuccess) {
+        set_error("%s: hipStreamWaitEvent failed", who);
+        return PXL_EHIP;
     }
     st = fused ? dispatch<kFused>(grid->sh_dim, grid->sparse, adapt, grid->storage, a,
                                   as_stream(stream))
@@ -1154,6 +1212,15 @@ static int32_t bwd_common(const pxl_grid *grid, const pxl_optim *optim, const px
                                 as_stream(stream));
     return st != PXL_OK ? st : check_launch(who);
 }
+
+namespace pxl {
+int32_t train_rays_ex(const pxl_grid *grid, const pxl_optim *optim, const pxl_rays *rays,
+                      double *losses, const int32_t *sample_index, int64_t start, int64_t len,
+                      double mask_coef, float huber_delta, void *stream, const TrainExtra &x) {
+    return bwd_common(grid, optim, rays, losses, sample_index, start, len, mask_coef,
+                      huber_delta, stream, true, x);
+}
+}  // namespace pxl
 
 extern "C" int32_t pxl_render_bwd_rgb(const pxl_grid *grid, const pxl_optim *optim,
                                       const pxl_rays *rays, double *losses,
@@ -1186,8 +1253,12 @@ extern "C" int32_t pxl_train_rays_ws(const pxl_grid *grid, const pxl_optim *opti
         set_error("pxl_train_rays_ws: null workspace");
         return PXL_EINVAL;
     }
+    TrainExtra x;
+    x.workspace = workspace;
+    x.workspace_bytes = workspace_bytes;
+    x.max_samples = max_samples;
     return bwd_common(grid, optim, rays, losses, sample_index, start, len, mask_coef,
-                      huber_delta, stream, true, workspace, workspace_bytes, max_samples);
+                      huber_delta, stream, true, x);
 }
 
 extern "C" int32_t pxl_train_rays_prop(const pxl_grid *grid, const pxl_optim *optim,
@@ -1204,9 +1275,17 @@ extern "C" int32_t pxl_train_rays_prop(const pxl_grid *grid, const pxl_optim *op
         return PXL_EINVAL;
     }
     const pxl_schedule dflt = {4, 0, 0, 0};
+    TrainExtra x;
+    x.workspace = workspace;
+    x.workspace_bytes = workspace_bytes;
+    x.max_samples = max_samples;
+    x.sched = sched ? sched : &dflt;
+    x.bin_mask = bin_mask;
+    x.bin_targets = bin_targets;
+    x.target_valid = target_valid;
+    x.target_wmin = target_wmin;
     return bwd_common(grid, optim, rays, losses, sample_index, start, len, mask_coef,
-                      huber_delta, stream, true, workspace, workspace_bytes, max_samples,
-                      sched ? sched : &dflt, bin_mask, bin_targets, target_valid, target_wmin);
+                      huber_delta, stream, true, x);
 }
 
 extern "C" int32_t pxl_train_rays_sched(const pxl_grid *grid, const pxl_optim *optim,
@@ -1220,8 +1299,13 @@ extern "C" int32_t pxl_train_rays_sched(const pxl_grid *grid, const pxl_optim *o
         set_error("pxl_train_rays_sched: null workspace or schedule");
         return PXL_EINVAL;
     }
+    TrainExtra x;
+    x.workspace = workspace;
+    x.workspace_bytes = workspace_bytes;
+    x.max_samples = max_samples;
+    x.sched = sched;
     return bwd_common(grid, optim, rays, losses, sample_index, start, len, mask_coef,
-                      huber_delta, stream, true, workspace, workspace_bytes, max_samples, sched);
+                      huber_delta, stream, true, x);
 }
 
 namespace pxl {
@@ -1273,7 +1357,7 @@ extern "C" const char *pxl_render_kernel_name(const pxl_grid *grid, int32_t mode
     }
     static thread_local char name[96];
     const bool adapt = grid->march_mode == PXL_MARCH_REF_ADAPTIVE;
-    const bool group = !adapt && grid->sh_dim <= 9 && use_march8();
+    const bool group = logged_step_serves(grid);
     const char *modes[4] = {"fwd", "bwd", "fused", "fused"};
     const char *store = grid->storage == PXL_STORE_BF16  ? "bf16"
                         : grid->storage == PXL_STORE_F16 ? "f16"

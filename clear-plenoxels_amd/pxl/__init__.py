@@ -18,7 +18,9 @@ LIB_PATH = os.path.join(PKG_DIR, "libpxl.so")
 
 PXL_OK, PXL_EINVAL, PXL_ENOMEM, PXL_EHIP, PXL_ENAN = 0, 1, 2, 3, 4
 MARCH_FIXED, MARCH_REF_ADAPTIVE = 0, 1
-TV_NEIGHBOUR, TV_DISPARITY = 0, 1
+TV_NEIGHBOUR, TV_DISPARITY, TV_BOTH = 0, 1, 2
+OPT_MARCHER, OPT_TV_ADDR64 = 1, 2
+MARCHER_GROUP, MARCHER_WAVE = 0, 1
 STORE_F32, STORE_BF16, STORE_F16 = 0, 1, 2
 
 # Every symbol include/pxl.h declares.
@@ -41,6 +43,8 @@ EXPORTS = (
     "pxl_upsample_density", "pxl_upsample", "pxl_rays_init_box", "pxl_mask_bounds",
     "pxl_rays_permute", "pxl_shuffle_rank", "pxl_read_cameras", "pxl_read_image_text",
     "pxl_grid_save_text", "pxl_grid_text_records", "pxl_grid_load_text",
+    "pxl_set_option", "pxl_get_option", "pxl_trainer_render_ms_split", "pxl_optim_init_band",
+    "pxl_trainer_set_band", "pxl_trainer_learn_level", "pxl_hierarchy_plan",
 )
 
 PROP_IN, PROP_WIDTH, PROP_BINS = 128, 64, 64
@@ -72,9 +76,10 @@ def lib():
             raise ImportError(f"libpxl.so not built at {LIB_PATH}; run __graft_entry__.build()")
         L = ctypes.CDLL(LIB_PATH)
         _declare(L)
-        sizes = (ctypes.c_int64 * 6)()
-        L.pxl_abi_sizes(sizes, 6)
-        mine = [ctypes.sizeof(c) for c in (Camera, BBox, RaysC, GridC, OptimC, TrainConfigC)]
+        sizes = (ctypes.c_int64 * 8)()
+        L.pxl_abi_sizes(sizes, 8)
+        mine = [ctypes.sizeof(c) for c in (Camera, BBox, RaysC, GridC, OptimC, TrainConfigC,
+                                           LevelConfigC, LevelC)]
         if list(sizes) != mine:
             raise ImportError(f"libpxl.so struct sizes {list(sizes)} != binding {mine}")
         _lib = L
@@ -111,12 +116,12 @@ class GridC(ctypes.Structure):
                 ("sparse", _I32), ("n_vox", _I64), ("params", _P), ("occupancy", _P),
                 ("index_offsets", _P), ("in_cage", _P), ("compact_to_dense", _P),
                 ("brick_occupancy", _P), ("bbox", BBox), ("step", _F32),
-                ("march_mode", _I32), ("storage", _I32), ("reserved", _I32)]
+                ("march_mode", _I32), ("storage", _I32), ("sh_active", _I32)]
 
 
 class OptimC(ctypes.Structure):
     _fields_ = [("grad", _P), ("m", _P), ("v", _P), ("adaptive_lr", _P), ("master", _P),
-                ("stride", _I32), ("reserved", _I32)]
+                ("stride", _I32), ("sh_first", _I32)]
 
 
 class TrainConfigC(ctypes.Structure):
@@ -126,7 +131,20 @@ class TrainConfigC(ctypes.Structure):
                 ("variance_interval", _I32), ("optimize_step", _I32),
                 ("mask_coefficient", _F64), ("huber_delta", _F32), ("fused", _I32),
                 ("log_samples", _I32), ("autotune", _I32), ("batch_order", _I32),
-                ("waves_per_block", _I32), ("persistent", _I32), ("chunked", _I32)]
+                ("waves_per_block", _I32), ("persistent", _I32), ("chunked", _I32),
+                ("tv_inline", _I32), ("reserved", _I32)]
+
+
+class LevelConfigC(ctypes.Structure):
+    _fields_ = [(n, _I32) for n in ("epochs", "sh_band", "hierarchy_band_strategy",
+                                    "sh_for_target_only", "current_hierarchy",
+                                    "minimum_hierarchy", "initial_from_file", "upsampled",
+                                    "available_sh_band", "check_nan")]
+
+
+class LevelC(ctypes.Structure):
+    _fields_ = [("level", _I32), ("epochs", _I32), ("batch_size", _I64), ("upsample", _I32),
+                ("reserved", _I32)]
 
 
 class ProposalC(ctypes.Structure):
@@ -198,6 +216,15 @@ def _declare(L):
         "pxl_trainer_set_proposal": ([_P, ctypes.POINTER(ProposalC), _I32, _I32, _F32, _I32,
                                       _F64, _F32, _F64], _I32),
         "pxl_trainer_proposal_loss": ([_P, ctypes.POINTER(_F64)], _I32),
+        "pxl_set_option": ([_I32, _I64], _I32),
+        "pxl_get_option": ([_I32, ctypes.POINTER(_I64)], _I32),
+        "pxl_trainer_render_ms_split": ([_P, ctypes.POINTER(_F64), ctypes.POINTER(_I64),
+                                         ctypes.POINTER(_F64), ctypes.POINTER(_I64)], _I32),
+        "pxl_optim_init_band": ([G, O, _I32, _I32, _I32, _P], _I32),
+        "pxl_trainer_set_band": ([_P, _I32, _I32, _I32], _I32),
+        "pxl_trainer_learn_level": ([_P, ctypes.POINTER(LevelConfigC)], _I32),
+        "pxl_hierarchy_plan": ([_I32, _I32, _I32, _I64, _I32, _I32, _I32, ctypes.POINTER(LevelC),
+                                _I32, ctypes.POINTER(_I32)], _I32),
         "pxl_render_image": ([G, ctypes.POINTER(Camera), _I32, _I32, _I32, _P, _P, _P, _P, _P],
                              _I32),
         "pxl_upsample_density": ([G, _I32, _I32, _I32, _F32, _P, _P], _I32),
@@ -290,7 +317,7 @@ class Grid:
 
     def __init__(self, W, H, D, S, params, step=0.5, bbox=None, march_mode=MARCH_FIXED,
                  occupancy=None, index_offsets=None, in_cage=None, compact_to_dense=None,
-                 brick_occupancy=None, storage=STORE_F32):
+                 brick_occupancy=None, storage=STORE_F32, sh_active=0):
         self.W, self.H, self.D, self.S = int(W), int(H), int(D), int(S)
         self.params = params
         self.stride = int(params.shape[1])
@@ -304,6 +331,7 @@ class Grid:
         self.compact_to_dense = compact_to_dense
         self.brick_occupancy = brick_occupancy
         self.storage = int(storage)
+        self.sh_active = int(sh_active)
         if self.storage != STORE_F32:
             assert params.element_size() == 2, "16-bit storage needs a 2-byte tensor"
         self.sparse = occupancy is not None
@@ -329,6 +357,7 @@ class Grid:
         g.step = self.step
         g.march_mode = self.march_mode
         g.storage = self.storage
+        g.sh_active = self.sh_active
         return g
 
 
@@ -337,7 +366,7 @@ class Optim:
     `master` (FP32 parameters that Adam updates) is created from the
     given FP32 records when master_from is passed."""
 
-    def __init__(self, grid, adaptive_lr=None, master_from=None):
+    def __init__(self, grid, adaptive_lr=None, master_from=None, sh_first=0):
         import torch
 
         stride = grid.stride if grid.storage == STORE_F32 else record_stride(grid.S)
@@ -348,6 +377,7 @@ class Optim:
         self.m = torch.zeros(shape, dtype=torch.float32, device=dev)
         self.v = torch.zeros(shape, dtype=torch.float32, device=dev)
         self.adaptive_lr = adaptive_lr
+        self.sh_first = int(sh_first)
         self.master = None
         if master_from is not None:
             self.master = torch.zeros(shape, dtype=torch.float32, device=dev)
@@ -361,6 +391,7 @@ class Optim:
         o.adaptive_lr = None if self.adaptive_lr is None else self.adaptive_lr.data_ptr()
         o.master = None if self.master is None else self.master.data_ptr()
         o.stride = self.stride
+        o.sh_first = self.sh_first
         return o
 
 
@@ -658,12 +689,52 @@ def add_variance_loss(grid, optim, lambda_density, lambda_sh, tv_type=TV_DISPARI
                       neighbours=6, optimize_step=0, stream=None):
     """RenderUtils::addVarianceLoss (RenderUtils.cu:2815-2851)."""
     g, o = grid.c(), optim.c()
-    sc = tv_scratch(grid, stream) if tv_type == TV_DISPARITY else None
+    sc = tv_scratch(grid, stream) if tv_type in (TV_DISPARITY, TV_BOTH) else None
     buf, nbytes = (None, 0) if sc is None else sc
     _check("pxl_add_variance_loss",
            lib().pxl_add_variance_loss(ctypes.byref(g), ctypes.byref(o), lambda_density,
                                        lambda_sh, tv_type, neighbours, optimize_step,
                                        _ptr(buf), nbytes, _stream(stream)))
+
+
+def set_option(key, value):
+    """Library option (OPT_MARCHER: MARCHER_GROUP / MARCHER_WAVE;
+    OPT_TV_ADDR64: 0 / 1)."""
+    _check("pxl_set_option", lib().pxl_set_option(int(key), int(value)))
+
+
+def get_option(key):
+    v = _I64()
+    _check("pxl_get_option", lib().pxl_get_option(int(key), ctypes.byref(v)))
+    return v.value
+
+
+def optim_init_band(grid, optim, band, band_mode=True, density_initialized=True, stream=None):
+    """Optimiser state at the start of a band (optimizer::initialBand /
+    initialAll, Optimizer.cu:11-64, 139-187)."""
+    g, o = grid.c(), optim.c()
+    _check("pxl_optim_init_band",
+           lib().pxl_optim_init_band(ctypes.byref(g), ctypes.byref(o), band, int(band_mode),
+                                     int(density_initialized), _stream(stream)))
+
+
+def hierarchy_plan(hierarchy, minimum_hierarchy, epochs, batch_size, shuffle_rays=True,
+                   fix_batch_size=False, to_sparse_hierarchy=0):
+    """Levels of the coarse-to-fine loop (VolumeFieldRender.cu:456-577), by
+    the C++ host layer: list of dicts level, epochs, batch_size, upsample
+    (None after the last level, else {"to_sparse": bool})."""
+    n = _I32()
+    args = (int(hierarchy), int(minimum_hierarchy), int(epochs), int(batch_size),
+            int(shuffle_rays), int(fix_batch_size), int(to_sparse_hierarchy))
+    _check("pxl_hierarchy_plan", lib().pxl_hierarchy_plan(*args, None, 0, ctypes.byref(n)))
+    arr = (LevelC * max(1, n.value))()
+    _check("pxl_hierarchy_plan", lib().pxl_hierarchy_plan(*args, arr, n.value, ctypes.byref(n)))
+    out = []
+    for L in arr[:n.value]:
+        up = None if L.upsample == 0 else {"to_sparse": L.upsample == 2}
+        out.append({"level": L.level, "epochs": L.epochs, "batch_size": L.batch_size,
+                    "upsample": up})
+    return out
 
 
 def adam_step(grid, optim, lr, sh_lr, t, stream=None):
@@ -775,7 +846,8 @@ class Trainer:
                  sh_coefficient=1e-6, variance_type=TV_DISPARITY, neighbour_nums=6,
                  variance_interval=3, optimize_step=0, mask_coefficient=1.0, huber_delta=0.1,
                  fused=True, log_samples=0, autotune=False, batch_order=0,
-                 waves_per_block=0, persistent=False, chunked=False, stream=None):
+                 waves_per_block=0, persistent=False, chunked=False, tv_inline=False,
+                 stream=None):
         import torch
 
         self._keep = (grid, optim, rays, order)
@@ -786,10 +858,10 @@ class Trainer:
                            sh_coefficient, variance_type, neighbour_nums, variance_interval,
                            optimize_step, mask_coefficient, huber_delta, int(fused),
                            int(log_samples), int(autotune), int(batch_order),
-                           int(waves_per_block), int(persistent), int(chunked))
+                           int(waves_per_block), int(persistent), int(chunked), int(tv_inline), 0)
         self._g, self._o, self._r = grid.c(), optim.c(), rays.c()
         n_rays = order.numel() if n_rays is None else n_rays
-        sc = tv_scratch(grid, stream) if variance_type == TV_DISPARITY else None
+        sc = tv_scratch(grid, stream) if variance_type in (TV_DISPARITY, TV_BOTH) else None
         buf, nbytes = (None, 0) if sc is None else sc
         h = lib().pxl_trainer_create(ctypes.byref(cfg), ctypes.byref(self._g),
                                      ctypes.byref(self._o), ctypes.byref(self._r), _ptr(order),
@@ -810,6 +882,32 @@ class Trainer:
         _check("pxl_trainer_render_ms",
                lib().pxl_trainer_render_ms(self._h, ctypes.byref(avg), ctypes.byref(n)))
         return avg.value, n.value
+
+    def render_ms_split(self):
+        """(mean over every timed batch, count, mean over the batches whose
+        render ran without the variance loss beside it, count)."""
+        avg, n, al, na = ctypes.c_double(), ctypes.c_int64(), ctypes.c_double(), ctypes.c_int64()
+        _check("pxl_trainer_render_ms_split",
+               lib().pxl_trainer_render_ms_split(self._h, ctypes.byref(avg), ctypes.byref(n),
+                                                 ctypes.byref(al), ctypes.byref(na)))
+        return avg.value, n.value, al.value, na.value
+
+    def set_band(self, band, band_mode=True, density_initialized=True):
+        """Switch to SH band `band` (Render.cu:282-315)."""
+        _check("pxl_trainer_set_band",
+               lib().pxl_trainer_set_band(self._h, band, int(band_mode), int(density_initialized)))
+
+    def learn_level(self, epochs, sh_band, hierarchy_band_strategy=False,
+                    sh_for_target_only=True, current_hierarchy=0, minimum_hierarchy=0,
+                    initial_from_file=False, upsampled=False, available_sh_band=0,
+                    check_nan=False):
+        """One level of Render::objectSceneLearner (bands x epochs x batches),
+        run by the C++ host layer."""
+        lc = LevelConfigC(int(epochs), int(sh_band), int(hierarchy_band_strategy),
+                          int(sh_for_target_only), int(current_hierarchy),
+                          int(minimum_hierarchy), int(initial_from_file), int(upsampled),
+                          int(available_sh_band), int(check_nan))
+        _check("pxl_trainer_learn_level", lib().pxl_trainer_learn_level(self._h, ctypes.byref(lc)))
 
     def loss(self):
         mse, psnr = ctypes.c_double(), ctypes.c_double()

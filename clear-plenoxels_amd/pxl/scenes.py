@@ -76,8 +76,11 @@ def view_intrinsics(img, fov_deg):
     return f, f, img / 2.0, img / 2.0
 
 
-def shell_density(N, seed, n_shells=3, thickness=2.0, n_blobs=32, device="cuda"):
-    """Thin spherical shells plus blobs (the uneven-ray workload)."""
+def shell_density(N, seed, n_shells=3, thickness=1.0, n_blobs=32, radius=(0.12, 0.27),
+                  device="cuda"):
+    """Thin spherical shells (amplitude 40, radius U(radius) N, centres in the
+    central third) plus small blobs: the uneven-ray workload.  At N = 512
+    with the defaults, cells with a corner above 20 are 1.02 % of all."""
     import torch
 
     rng = np.random.default_rng(seed)
@@ -86,7 +89,7 @@ def shell_density(N, seed, n_shells=3, thickness=2.0, n_blobs=32, device="cuda")
                              indexing="ij")
     for _ in range(n_shells):
         c = rng.uniform(0.35 * N, 0.65 * N, 3)
-        rad = rng.uniform(0.2, 0.45) * N
+        rad = rng.uniform(radius[0], radius[1]) * N
         d = torch.sqrt((x - c[0]) ** 2 + (y - c[1]) ** 2 + (z - c[2]) ** 2)
         dens += 40.0 * (torch.abs(d - rad) < thickness / 2).float()
     return dens

@@ -57,8 +57,23 @@ enum {
  * (RenderUtils.cu:2815-2851). */
 enum {
     PXL_TV_NEIGHBOUR = 0,   /* totalVarianceLoss, 6-neighbour branch      */
-    PXL_TV_DISPARITY = 1    /* addVarianceLossGPU, per-cell mean disparity */
+    PXL_TV_DISPARITY = 1,   /* addVarianceLossGPU, per-cell mean disparity */
+    PXL_TV_BOTH = 2         /* totalVarianceLoss over the launch of type 1,
+                               then addVarianceLossGPU (RenderUtils.cu:2840-2845) */
 };
+
+/* Process-wide library options (pxl_set_option / pxl_get_option).  Each
+ * launch reads the current value; nothing is read from the environment. */
+enum {
+    /* Marcher of the fixed rule: PXL_MARCHER_GROUP (default, eight samples
+     * per wave) or PXL_MARCHER_WAVE (one sample per wave, as the adaptive
+     * rule's kernel).  Both give the reference's results. */
+    PXL_OPT_MARCHER = 1,
+    /* 1: the TV gathers always take their 64-bit address form (otherwise
+     * only for grids beyond 2^24 records or 4 GiB); default 0. */
+    PXL_OPT_TV_ADDR64 = 2
+};
+enum { PXL_MARCHER_GROUP = 0, PXL_MARCHER_WAVE = 1 };
 
 /* Pinhole camera, host memory.  Mirrors the fields of Camera that
  * initialRays reads (CameraUtils.cuh:4-84): c2wExternalMat, cameraCenter,
@@ -118,7 +133,10 @@ typedef struct pxl_grid {
     float step;                   /* marching step in voxels (StepSize) */
     int32_t march_mode;           /* PXL_MARCH_*                       */
     int32_t storage;              /* PXL_STORE_*: element type of params */
-    int32_t reserved;
+    /* SH coefficients k < sh_active of each channel enter the colour; the
+     * reference's (optimizingBand + 1)^2 (trilerpRGB, RenderUtils.cu:
+     * 859-1000).  0 = all sh_dim. */
+    int32_t sh_active;
 } pxl_grid;
 
 /* Optimiser state (Optimizer.cuh:5-37): FP32 records of `stride` floats
@@ -132,17 +150,29 @@ typedef struct pxl_optim {
     const float *adaptive_lr;     /* dense only: per-vertex LR factor or NULL */
     float *master;                /* 16-bit grids: FP32 parameters or NULL */
     int32_t stride;               /* floats per record, multiple of 4  */
-    int32_t reserved;
+    /* Gradients, variance losses and Adam touch the SH coefficients k in
+     * [sh_first, sh_active) of each channel, and the density: the
+     * reference's band of the per-band strategy (hierarchyStrategy with
+     * optimizingBand^2, RenderUtils.cu:1117-1132, Optimizer.cu:393-412);
+     * 0 = from the first. */
+    int32_t sh_first;
 } pxl_optim;
 
 /* Last error message of the calling thread ("" if none). */
 const char *pxl_last_error(void);
 
+/* Library options (PXL_OPT_*): PXL_EINVAL for an unknown key or value.
+ * Not tied to a stream: set them between calls, not during one.  No
+ * counterpart in the reference. */
+int32_t pxl_set_option(int32_t key, int64_t value);
+int32_t pxl_get_option(int32_t key, int64_t *value);
+
 /* Library version, (major << 16) | minor. */
 int32_t pxl_version(void);
 
-/* sizeof of pxl_camera, pxl_bbox, pxl_rays, pxl_grid, pxl_optim and
- * pxl_train_config, for bindings to check their mirrors; returns 6. */
+/* sizeof of pxl_camera, pxl_bbox, pxl_rays, pxl_grid, pxl_optim,
+ * pxl_train_config, pxl_level_config and pxl_level, for bindings to check
+ * their mirrors; returns 8. */
 int32_t pxl_abi_sizes(int64_t *out, int32_t n);
 
 /* Host-side camera of one pose (readFromMyBlender per view,
@@ -245,7 +275,19 @@ int32_t pxl_train_rays(const pxl_grid *grid, const pxl_optim *optim,
  *   computed race-free as a per-vertex gather; SH terms only when
  *   optimize_step <= 1 (RenderUtils.cu:2493).
  * type PXL_TV_NEIGHBOUR: totalVarianceLoss 6-neighbour branch
- *   (RenderUtils.cu:2660-2763); neighbours must be 6.
+ *   (RenderUtils.cu:2660-2763); neighbours must be 6.  Interior vertices
+ *   (no face of the grid); a neighbour whose index (dense, or record when
+ *   caged) is 0 counts as the vertex itself.  Caged grids keep the
+ *   reference's test of in_cage at the vertex's record number
+ *   (RenderUtils.cu:2669: the flag array is indexed by dense vertex); a
+ *   vertex whose record is 0 adds into record 0 atomically.
+ * type PXL_TV_BOTH: VarianceType 2 (RenderUtils.cu:2840-2845): the
+ *   6-neighbour term over the columns of a (W-1) x (H-1) launch rounded up
+ *   to 16 x 16 (columns y W + x below ceil((W-1)/16) ceil((H-1)/16) 256),
+ *   then the disparity term.
+ * SH terms cover the coefficients [optim->sh_first, grid->sh_active) of
+ * each channel (startSHIdx .. (optimizingBand+1)^2, RenderUtils.cu:2480,
+ * 2594).
  * scratch: sparse grids only - pxl_tv_scratch_bytes() bytes of device
  *   memory, 256-byte aligned, filled once by pxl_tv_prepare for the grid's
  *   cage (it holds the per-record weights of the corners that alias record
@@ -601,6 +643,9 @@ typedef struct pxl_train_config {
     int32_t persistent;      /* when not autotuned */
     int32_t chunked;         /* when not autotuned: chunk-parallel gradient
                                 (pxl_schedule), not with persistent */
+    int32_t tv_inline;       /* 1: the variance loss runs on the trainer's
+                                stream after the render (no overlap) */
+    int32_t reserved;
 } pxl_train_config;
 
 typedef struct pxl_trainer pxl_trainer;
@@ -623,19 +668,24 @@ pxl_trainer *pxl_trainer_create(const pxl_train_config *cfg, const pxl_grid *gri
  * (Render.cu:499-500).  Asynchronous.  With the logged step, once its
  * schedule is chosen and without the proposal, the variance loss runs on a
  * second stream beside the forward kernel and the gradient kernels wait for
- * it (the same sum into the gradient planes in another order);
- * PXL_TRAINER_TV_OVERLAP=0 in the environment at create keeps it after the
- * render on the trainer's stream.  The batch loss is reduced on that second
- * stream too; pxl_trainer_loss waits for it. */
+ * it (the same sum into the gradient planes in another order); tv_inline
+ * keeps it after the render on the trainer's stream.  The overlap is used
+ * only where the render runs as the two-kernel logged step (the fixed rule
+ * or the reference's adaptive rule, sh_dim 1, 4 or 9, the eight-sample
+ * marcher).  The batch loss is reduced on that second stream too; the
+ * trainer's stream waits for the reduction before the call returns its
+ * work, so synchronising that stream makes losses and loss_out readable. */
 int32_t pxl_trainer_step(pxl_trainer *tr);
 
 /* Record HIP events around the render launch of the next max_steps
- * batches that render alone (on sparse grids, batches whose variance loss
- * overlaps the forward are not timed; on dense grids every batch carries it
- * and is timed with it); pxl_trainer_render_ms returns their average
- * duration and count. */
+ * batches; pxl_trainer_render_ms returns the average duration and count of
+ * every timed batch and, when the pointers are not NULL, of the batches
+ * whose render ran without the variance loss beside it (alone_ms,
+ * alone_count; on dense grids every batch carries the variance loss). */
 int32_t pxl_trainer_time_render(pxl_trainer *tr, int32_t max_steps);
 int32_t pxl_trainer_render_ms(pxl_trainer *tr, double *avg_ms, int64_t *count);
+int32_t pxl_trainer_render_ms_split(pxl_trainer *tr, double *avg_ms, int64_t *count,
+                                    double *alone_ms, int64_t *alone_count);
 
 /* Last batch's FP64 MSE and PSNR (synchronises); PXL_ENAN if not finite. */
 int32_t pxl_trainer_loss(pxl_trainer *tr, double *mse, double *psnr);
@@ -662,6 +712,77 @@ int32_t pxl_trainer_schedule(const pxl_trainer *tr, int32_t *batch_order,
                              pxl_schedule *schedule);
 
 void pxl_trainer_destroy(pxl_trainer *tr);
+
+/* ------------------------------------------------------------------ */
+/* Per-band SH strategy and the coarse-to-fine level loop               */
+/* ------------------------------------------------------------------ */
+
+/* Optimiser state at the start of a band (optimizer::initialBand /
+ * bandMemoryManage and optimizer::initialAll, Optimizer.cu:11-64,
+ * 139-187), on the gradient and both Adam moments: band_mode 0 clears every
+ * element; band_mode 1 clears SH coefficients [band^2, (band+1)^2) of each
+ * channel, and the density too when band == 0 or !density_initialized (the
+ * other elements keep their values, which the band's updates never read). */
+int32_t pxl_optim_init_band(const pxl_grid *grid, const pxl_optim *optim, int32_t band,
+                            int32_t band_mode, int32_t density_initialized, void *stream);
+
+/* Switches a trainer to band `band` of its grid (Render.cu:282-315): the
+ * colour uses coefficients k < (band+1)^2 (grid sh_active), gradients and
+ * Adam touch [band^2, (band+1)^2) in band mode, else every coefficient
+ * (optim sh_first 0); pxl_optim_init_band; learning rates back to their
+ * configured values and the time step back to 1. */
+int32_t pxl_trainer_set_band(pxl_trainer *tr, int32_t band, int32_t band_mode,
+                             int32_t density_initialized);
+
+/* The band settings of one level of Render::objectSceneLearner
+ * (Render.cu:264-300; config keys SphericalHarmonicsBand,
+ * HierarchyBandStrategy, SHsForTargetResolutionOnly, InitialFromFile,
+ * AvailableSHBands, Utils.cu:765-932). */
+typedef struct pxl_level_config {
+    int32_t epochs;
+    int32_t sh_band;                   /* the grid's band: sh_dim = (sh_band+1)^2 */
+    int32_t hierarchy_band_strategy;
+    int32_t sh_for_target_only;
+    int32_t current_hierarchy;
+    int32_t minimum_hierarchy;
+    int32_t initial_from_file;
+    int32_t upsampled;                 /* the grid came from the level before */
+    int32_t available_sh_band;
+    int32_t check_nan;                 /* 1: stop with PXL_ENAN after a NaN batch loss */
+} pxl_level_config;
+
+/* One level of the coarse-to-fine loop on a trainer (the band loop, its
+ * epochs and batches of Render::objectSceneLearner, Render.cu:264-503):
+ * bands from the start band (0, the file's next band, or sh_band; Render.cu:
+ * 267-276) to sh_band; per band pxl_trainer_set_band with band mode when
+ * hierarchy_band_strategy, or when sh_for_target_only below the minimum
+ * hierarchy, else the whole optimiser cleared; epochs x batch_count
+ * pxl_trainer_step calls.  With check_nan the batch loss is read after
+ * every batch (one synchronisation each; the reference stops on NaN,
+ * Render.cu:450-457), otherwise once at the end.  The grid's sh_dim must be
+ * (sh_band+1)^2. */
+int32_t pxl_trainer_learn_level(pxl_trainer *tr, const pxl_level_config *lc);
+
+/* One level of the coarse-to-fine plan (trainSynthesisBlender,
+ * VolumeFieldRender.cu:456-577). */
+typedef struct pxl_level {
+    int32_t level;       /* hierarchy index, coarsest first            */
+    int32_t epochs;      /* epochs of the level (:459, 476-489)         */
+    int64_t batch_size;  /* batch_size << (3 - level) unless fixed (:491-495) */
+    int32_t upsample;    /* after the level: 0 none (last), 1 dense, 2 caged (:549-557) */
+    int32_t reserved;
+} pxl_level;
+
+/* Levels hierarchy .. minimum_hierarchy of the plan: epochs start at
+ * epochs + 3 - hierarchy and grow by one per level with shuffled rays (10
+ * without); the batch doubles per level below 3 unless fixed (levels above
+ * 3 with doubling are rejected: the reference's shift count would be
+ * negative); the upsampling after level i > minimum is caged when
+ * i <= minimum_hierarchy - to_sparse_hierarchy.  out == NULL: only *count. */
+int32_t pxl_hierarchy_plan(int32_t hierarchy, int32_t minimum_hierarchy, int32_t epochs,
+                           int64_t batch_size, int32_t shuffle_rays, int32_t fix_batch_size,
+                           int32_t to_sparse_hierarchy, pxl_level *out, int32_t capacity,
+                           int32_t *count);
 
 #ifdef __cplusplus
 }

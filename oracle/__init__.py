@@ -14,8 +14,14 @@ Parity status of each function (how it is pinned; see tests/test_oracle.py):
   build_index       - brute force / prefix-sum identity
   render_fwd        - closed forms (constant field, slab), testRender inputs
   render_bwd        - central finite differences of a differentiable restatement
+  render_bwd_cond   - the gradient equals render_bwd's; the magnitude sum
+                      bounds it and equals it for terms of one sign
+  render_bwd_masked - central finite differences of the masked forward
   tv_disparity      - closed forms (constant / affine field, single spike)
-  tv_neighbour6     - closed forms (constant field, single spike)
+  tv_neighbour6     - closed forms (constant field, single spike, affine
+                      field), brute force of the caged rule on a tiny grid
+  tv_both           - the column limit of the type-2 launch by hand; sum of
+                      the two terms
   adam              - closed form at t = 1 and zero-gradient decay
   render_fwd_masked - all-ones mask == plain march, one-bin masks on a constant
                       field (closed form)
@@ -60,6 +66,7 @@ class OrcGrid(ctypes.Structure):
         ("planes", ctypes.c_void_p), ("occupancy", ctypes.c_void_p),
         ("index_offsets", ctypes.c_void_p), ("bbox", ctypes.c_int32 * 6),
         ("step", ctypes.c_float), ("adaptive", ctypes.c_int32),
+        ("sh_active", ctypes.c_int32), ("sh_first", ctypes.c_int32), ("in_cage", ctypes.c_void_p),
     ]
 
 
@@ -89,6 +96,9 @@ def _declare(L):
     L.orc_render_bwd.argtypes = [ctypes.POINTER(OrcGrid), ctypes.POINTER(OrcRays), _P, _I64, _I64,
                                  _P, _P, _P, _P, _F64, _F32, _P, _P]
     L.orc_render_bwd.restype = None
+    L.orc_render_bwd_cond.argtypes = [ctypes.POINTER(OrcGrid), ctypes.POINTER(OrcRays), _P,
+                                      _I64, _I64, _P, _P, _P, _P, _F64, _F32, _P, _P, _P]
+    L.orc_render_bwd_cond.restype = None
     L.orc_render_fwd_masked.argtypes = [ctypes.POINTER(OrcGrid), ctypes.POINTER(OrcRays), _P, _I64,
                                         _I64, _P, ctypes.c_double, _P, _P, _P, _P, _P, _P, _P]
     L.orc_render_fwd_masked.restype = None
@@ -104,8 +114,13 @@ def _declare(L):
     L.orc_tv_disparity.restype = None
     L.orc_tv_neighbour6.argtypes = [ctypes.POINTER(OrcGrid), _F64, _F64, _P]
     L.orc_tv_neighbour6.restype = None
+    L.orc_tv_neighbour6_cols.argtypes = [ctypes.POINTER(OrcGrid), _F64, _F64, _I64, _P]
+    L.orc_tv_neighbour6_cols.restype = None
     L.orc_adam.argtypes = [_P, _P, _P, _P, _P, _I32, _I64, _F64, _F64, _I32]
     L.orc_adam.restype = None
+    L.orc_adam_band.argtypes = [_P, _P, _P, _P, _P, _I32, _I32, _I64, _F64, _F64, _I32, _I32,
+                                _I32]
+    L.orc_adam_band.restype = None
     L.orc_upsample_density.argtypes = [ctypes.POINTER(OrcGrid), _I32, _I32, _I32, _F32, _P]
     L.orc_upsample_density.restype = None
     L.orc_upsample.argtypes = [ctypes.POINTER(OrcGrid), _I32, _I32, _I32, _I32, _P, _P, _I64,
@@ -177,7 +192,7 @@ class Grid:
     """Grid for the oracle: planes float32[P, n_vox] (reference layout)."""
 
     def __init__(self, W, H, D, S, planes, step, bbox=None, occupancy=None,
-                 index_offsets=None, adaptive=False):
+                 index_offsets=None, adaptive=False, sh_active=0, sh_first=0, in_cage=None):
         self.W, self.H, self.D, self.S = W, H, D, S
         self.planes = _c(planes, np.float32).reshape(1 + 3 * S, -1)
         self.n_vox = self.planes.shape[1]
@@ -187,6 +202,9 @@ class Grid:
         self.step = step
         self.bbox = bbox if bbox is not None else (0, W - 1, 0, H - 1, 0, D - 1)
         self.adaptive = adaptive
+        self.sh_active = int(sh_active)
+        self.sh_first = int(sh_first)
+        self.in_cage = None if in_cage is None else _c(in_cage, np.uint8)
 
     def c(self):
         g = OrcGrid()
@@ -199,6 +217,9 @@ class Grid:
         g.bbox = (ctypes.c_int32 * 6)(*self.bbox)
         g.step = self.step
         g.adaptive = int(self.adaptive)
+        g.sh_active = self.sh_active
+        g.sh_first = self.sh_first
+        g.in_cage = _p(self.in_cage)
         return g
 
 
@@ -251,6 +272,25 @@ def render_bwd(grid, rays, sample_index, fwd, mask_coef=1.0, huber_delta=0.1, st
                          _p(f["r"]), _p(f["g"]), _p(f["b"]), _p(f["w"]), mask_coef,
                          huber_delta, _p(losses), _p(grad))
     return losses, grad
+
+
+def render_bwd_cond(grid, rays, sample_index, fwd, mask_coef=1.0, huber_delta=0.1, start=0,
+                    length=None):
+    """render_bwd that also returns the magnitude sum of every entry's terms:
+    (losses, grad, grad_abs)."""
+    si = _c(sample_index, np.int32)
+    if length is None:
+        length = len(si) - start
+    P = 1 + 3 * grid.S
+    grad = np.zeros((P, grid.n_vox), np.float64)
+    gabs = np.zeros((P, grid.n_vox), np.float64)
+    losses = np.zeros(length, np.float64)
+    f = {k: _c(fwd[k], np.float64) for k in ("r", "g", "b", "w")}
+    g, r = grid.c(), rays.c()
+    lib().orc_render_bwd_cond(ctypes.byref(g), ctypes.byref(r), _p(si), start, length,
+                              _p(f["r"]), _p(f["g"]), _p(f["b"]), _p(f["w"]), mask_coef,
+                              huber_delta, _p(losses), _p(grad), _p(gabs))
+    return losses, grad, gabs
 
 
 def render_fwd_masked(grid, rays, sample_index, bin_mask=None, target_wmin=None, start=0,
@@ -321,22 +361,39 @@ def tv_disparity(grid, lambda_density, lambda_sh, optimize_step=0, grad=None):
     return grad
 
 
-def tv_neighbour6(grid, lambda_density, lambda_sh, grad=None):
+def tv_neighbour6(grid, lambda_density, lambda_sh, grad=None, col_limit=None):
+    """totalVarianceLoss, 6 neighbours; col_limit: columns y W + x covered
+    (default W H, type 0; tv_both passes the type-2 launch's)."""
     P = 1 + 3 * grid.S
     if grad is None:
         grad = np.zeros((P, grid.n_vox), np.float64)
+    if grid.sparse and grid.in_cage is None:
+        raise ValueError("the 6-neighbour TV of a caged grid needs in_cage")
     g = grid.c()
-    lib().orc_tv_neighbour6(ctypes.byref(g), lambda_density, lambda_sh, _p(grad))
+    limit = grid.W * grid.H if col_limit is None else int(col_limit)
+    lib().orc_tv_neighbour6_cols(ctypes.byref(g), lambda_density, lambda_sh, limit, _p(grad))
     return grad
 
 
-def adam(params, grad, m, v, lr, sh_lr, t, adaptive_lr=None):
-    """In-place Adam on float32 planes [P, n] (params, grad, m, v)."""
+def tv_both(grid, lambda_density, lambda_sh, optimize_step=0, grad=None):
+    """VarianceType 2 (RenderUtils.cu:2840-2845): totalVarianceLoss over the
+    columns of a ceil((W-1)/16) x ceil((H-1)/16) launch of 16 x 16 blocks,
+    then addVarianceLossGPU."""
+    limit = ((grid.W - 1 + 15) // 16) * ((grid.H - 1 + 15) // 16) * 256
+    grad = tv_neighbour6(grid, lambda_density, lambda_sh, grad=grad, col_limit=limit)
+    return tv_disparity(grid, lambda_density, lambda_sh, optimize_step, grad=grad)
+
+
+def adam(params, grad, m, v, lr, sh_lr, t, adaptive_lr=None, band=None):
+    """In-place Adam on float32 planes [P, n] (params, grad, m, v); band =
+    (sh_lo, sh_hi) restricts the SH planes updated (the density always is)."""
     for a in (params, grad, m, v):
         assert a.dtype == np.float32 and a.flags.c_contiguous
     P, n = params.shape
     alr = None if adaptive_lr is None else _c(adaptive_lr, np.float32)
-    lib().orc_adam(_p(params), _p(grad), _p(m), _p(v), _p(alr), P, n, lr, sh_lr, t)
+    S, (lo, hi) = (0, (0, 0)) if band is None else ((P - 1) // 3, band)
+    lib().orc_adam_band(_p(params), _p(grad), _p(m), _p(v), _p(alr), P, S, n, lr, sh_lr, t, lo,
+                        hi)
 
 
 def mse(losses):

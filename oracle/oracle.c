@@ -415,9 +415,12 @@ static int locate(const orc_grid *g, double px, double py, double pz, sample *s)
 }
 
 /* trilerpRGB (RenderUtils.cu:826-1004): raw density and raw colours. */
+static int sh_active_of(const orc_grid *g) { return g->sh_active > 0 ? g->sh_active : g->S; }
+
 static void interp(const orc_grid *g, const sample *s, const double *Y, double *sraw,
                    double craw[3]) {
     const int S = g->S;
+    const int act = sh_active_of(g);
     const float *pl = g->planes;
     const int64_t n = g->n_vox;
     double sd = 0.0;
@@ -425,7 +428,7 @@ static void interp(const orc_grid *g, const sample *s, const double *Y, double *
     *sraw = sd;
     for (int ch = 0; ch < 3; ch++) {
         double c = 0.0;
-        for (int k = 0; k < S; k++) {
+        for (int k = 0; k < act; k++) {
             double shk = 0.0;
             const float *plane = pl + (int64_t)(1 + ch * S + k) * n;
             for (int j = 0; j < 8; j++) shk += s->w[j] * (double)plane[s->v[j]];
@@ -630,7 +633,7 @@ static void render_bwd(const orc_grid *g, const orc_rays *r, const int32_t *samp
                        int64_t start, int64_t len, const double *fwd_r, const double *fwd_g,
                        const double *fwd_b, const double *fwd_w, double mask_coef,
                        float huber_delta, double *losses, double *grad,
-                       const uint64_t *bin_mask) {
+                       const uint64_t *bin_mask, double *grad_abs) {
     const double step = (double)g->step;
     const int S = g->S;
     const int64_t n = g->n_vox;
@@ -696,16 +699,23 @@ static void render_bwd(const orc_grid *g, const orc_rays *r, const int32_t *samp
             lam += sigma * step;
             T = exp(-lam);
             const double ds = dsoftplus13(sraw);
-            double gs = 0.0;
-            for (int ch = 0; ch < 3; ch++) gs += res[ch] * (T * col[ch] - (Cf[ch] - Ch[ch]));
+            double gs = 0.0, gs_abs = 0.0;
+            for (int ch = 0; ch < 3; ch++) {
+                gs += res[ch] * (T * col[ch] - (Cf[ch] - Ch[ch]));
+                gs_abs += fabs(res[ch]) * (fabs(T * col[ch]) + fabs(Cf[ch] - Ch[ch]));
+            }
             gs *= kd * step * ds;
             gs += step * ds * (Af - Ai - T) * m;
+            gs_abs = kd * step * ds * gs_abs + step * ds * (fabs(Af - Ai) + T) * fabs(m);
             for (int j = 0; j < 8; j++) {
                 add_d(&grad[s.v[j]], gs * s.w[j]);
+                if (grad_abs) add_d(&grad_abs[s.v[j]], gs_abs * fabs(s.w[j]));
                 for (int ch = 0; ch < 3; ch++)
-                    for (int kk = 0; kk < S; kk++)
-                        add_d(&grad[(int64_t)(1 + ch * S + kk) * n + s.v[j]],
-                              gc[ch] * s.w[j] * Y[kk]);
+                    for (int kk = g->sh_first; kk < sh_active_of(g); kk++) {
+                        const double c = gc[ch] * s.w[j] * Y[kk];
+                        add_d(&grad[(int64_t)(1 + ch * S + kk) * n + s.v[j]], c);
+                        if (grad_abs) add_d(&grad_abs[(int64_t)(1 + ch * S + kk) * n + s.v[j]], fabs(c));
+                    }
             }
             ++k;
             if (g->adaptive) t += fabs(step - w);
@@ -719,7 +729,15 @@ void orc_render_bwd(const orc_grid *g, const orc_rays *r, const int32_t *sample_
                     const double *fwd_b, const double *fwd_w, double mask_coef,
                     float huber_delta, double *losses, double *grad) {
     render_bwd(g, r, sample_index, start, len, fwd_r, fwd_g, fwd_b, fwd_w, mask_coef, huber_delta,
-               losses, grad, NULL);
+               losses, grad, NULL, NULL);
+}
+
+void orc_render_bwd_cond(const orc_grid *g, const orc_rays *r, const int32_t *sample_index,
+                         int64_t start, int64_t len, const double *fwd_r, const double *fwd_g,
+                         const double *fwd_b, const double *fwd_w, double mask_coef,
+                         float huber_delta, double *losses, double *grad, double *grad_abs) {
+    render_bwd(g, r, sample_index, start, len, fwd_r, fwd_g, fwd_b, fwd_w, mask_coef, huber_delta,
+               losses, grad, NULL, grad_abs);
 }
 
 void orc_render_bwd_masked(const orc_grid *g, const orc_rays *r, const int32_t *sample_index,
@@ -728,7 +746,7 @@ void orc_render_bwd_masked(const orc_grid *g, const orc_rays *r, const int32_t *
                            const double *fwd_w, double mask_coef, float huber_delta,
                            double *losses, double *grad) {
     render_bwd(g, r, sample_index, start, len, fwd_r, fwd_g, fwd_b, fwd_w, mask_coef, huber_delta,
-               losses, grad, bin_mask);
+               losses, grad, bin_mask, NULL);
 }
 
 /* ------------------------------------------------------------------ */
@@ -755,7 +773,9 @@ void orc_tv_disparity(const orc_grid *g, double lambda_density, double lambda_sh
                 for (int p = 0; p < P; p++) {
                     double lam;
                     if (p == 0) lam = lambda_density;
-                    else if (optimize_step <= 1) lam = lambda_sh;
+                    else if (optimize_step <= 1 && (p - 1) % S >= g->sh_first &&
+                             (p - 1) % S < sh_active_of(g))
+                        lam = lambda_sh;
                     else continue;
                     const float *pl = g->planes + (int64_t)p * n;
                     double mean = 0.0;
@@ -767,32 +787,52 @@ void orc_tv_disparity(const orc_grid *g, double lambda_density, double lambda_sh
             }
 }
 
-void orc_tv_neighbour6(const orc_grid *g, double lambda_density, double lambda_sh,
-                       double *grad) {
+void orc_tv_neighbour6_cols(const orc_grid *g, double lambda_density, double lambda_sh,
+                            int64_t col_limit, double *grad) {
     const int W = g->W, H = g->H, D = g->D, S = g->S;
     const int64_t n = g->n_vox;
     const int P = 1 + 3 * S;
+    /* Neighbour order of the 27-index stencil entries 4, 10, 12, 14, 16, 22. */
     static const int off[6][3] = {{0, 0, -1}, {0, -1, 0}, {-1, 0, 0},
                                   {1, 0, 0},  {0, 1, 0},  {0, 0, 1}};
-    for (int z = 1; z < D - 1; z++)
-        for (int y = 1; y < H - 1; y++)
-            for (int x = 1; x < W - 1; x++) {
-                const uint32_t self = scene_index(x, y, z, W, H, D);
+    for (int y = 1; y < H - 1; y++)
+        for (int x = 1; x < W - 1; x++) {
+            if ((int64_t)y * W + x >= col_limit) continue;
+            for (int z = 1; z < D - 1; z++) {
+                uint32_t self = scene_index(x, y, z, W, H, D);
+                if (g->sparse) {
+                    self = g->index_offsets[self];
+                    if (!g->in_cage[self]) continue;
+                }
+                uint32_t nb[6];
+                for (int i = 0; i < 6; i++) {
+                    nb[i] = scene_index(x + off[i][0], y + off[i][1], z + off[i][2], W, H, D);
+                    if (g->sparse) nb[i] = g->index_offsets[nb[i]];
+                }
                 for (int p = 0; p < P; p++) {
-                    const double lam = p == 0 ? lambda_density : lambda_sh;
+                    double lam;
+                    if (p == 0) lam = lambda_density;
+                    else if ((p - 1) % S >= g->sh_first && (p - 1) % S < sh_active_of(g))
+                        lam = lambda_sh;
+                    else continue;
                     const float *pl = g->planes + (int64_t)p * n;
                     const double u = pl[self];
                     double sq = 0.0, dis = 0.0;
                     for (int i = 0; i < 6; i++) {
-                        const uint32_t nb = scene_index(x + off[i][0], y + off[i][1], z + off[i][2], W, H, D);
-                        const double un = nb != 0u ? (double)pl[nb] : u;
+                        const double un = nb[i] != 0u ? (double)pl[nb[i]] : u;
                         sq += (u - un) * (u - un);
                         dis += u - un;
                     }
-                    const double s = sqrt(sq) / 6.0;
-                    grad[(int64_t)p * n + self] += lam / (s + 1e-6) * dis;
+                    const double sd = sqrt(sq) / 6.0;
+                    grad[(int64_t)p * n + self] += lam / (sd + 1e-6) * dis;
                 }
             }
+        }
+}
+
+void orc_tv_neighbour6(const orc_grid *g, double lambda_density, double lambda_sh,
+                       double *grad) {
+    orc_tv_neighbour6_cols(g, lambda_density, lambda_sh, (int64_t)g->W * g->H, grad);
 }
 
 /* ------------------------------------------------------------------ */
@@ -801,12 +841,19 @@ void orc_tv_neighbour6(const orc_grid *g, double lambda_density, double lambda_s
 
 void orc_adam(float *params, float *grad, float *m, float *v, const float *adaptive_lr,
               int32_t P, int64_t n, double lr, double sh_lr, int32_t t) {
+    orc_adam_band(params, grad, m, v, adaptive_lr, P, 0, n, lr, sh_lr, t, 0, 0);
+}
+
+void orc_adam_band(float *params, float *grad, float *m, float *v, const float *adaptive_lr,
+                   int32_t P, int32_t S, int64_t n, double lr, double sh_lr, int32_t t,
+                   int32_t sh_lo, int32_t sh_hi) {
     const float b1 = 0.9f, b2 = 0.999f, eps = 1.e-6f;
     const double bc1 = 1.0 - (double)powf(b1, (float)t);
     const double bc2 = 1.0 - (double)powf(b2, (float)t);
 #pragma omp parallel for collapse(2) schedule(static) num_threads(g_threads) if (g_threads > 1)
     for (int p = 0; p < P; p++)
         for (int64_t i = 0; i < n; i++) {
+            if (S > 0 && p > 0 && ((p - 1) % S < sh_lo || (p - 1) % S >= sh_hi)) continue;
             const int64_t k = (int64_t)p * n + i;
             const float g = grad[k];
             const float mm = (float)((double)(b1 * m[k]) + (1.0 - (double)b1) * (double)g);

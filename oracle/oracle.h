@@ -29,6 +29,12 @@ typedef struct orc_grid {
     int32_t bbox[6];               /* x_min, x_max, y_min, y_max, z_min, z_max */
     float step;
     int32_t adaptive;              /* 0 fixed, 1 reference adaptive step */
+    /* SH band (RenderUtils.cu:859-1000, 1117-1132): coefficients k <
+     * sh_active enter the colour (0 = all S); gradients and the SH terms of
+     * the variance losses cover k in [sh_first, sh_active). */
+    int32_t sh_active;
+    int32_t sh_first;
+    const uint8_t *in_cage;        /* sparse: one byte per dense vertex (TV type 0) */
 } orc_grid;
 
 typedef struct orc_rays {
@@ -79,6 +85,16 @@ void orc_render_bwd(const orc_grid *g, const orc_rays *r, const int32_t *sample_
                     int64_t start, int64_t len, const double *fwd_r, const double *fwd_g,
                     const double *fwd_b, const double *fwd_w, double mask_coef,
                     float huber_delta, double *losses, double *grad);
+/* orc_render_bwd that also adds into grad_abs[P][n_vox] the magnitude of
+ * every term of every contribution: |colour contribution|, and for the
+ * density 0.6667 step sigma' sum_ch |r_ch| (|T c_ch| + |C_ch - H_ch|) +
+ * step sigma' (|A - A_i| + T) |m| times the corner weight.  A gradient entry
+ * computed in FP32 arithmetic carries an error of a few FP32 rounding units
+ * of this sum, which can exceed 1e-4 of the entry where its terms cancel. */
+void orc_render_bwd_cond(const orc_grid *g, const orc_rays *r, const int32_t *sample_index,
+                         int64_t start, int64_t len, const double *fwd_r, const double *fwd_g,
+                         const double *fwd_b, const double *fwd_w, double mask_coef,
+                         float huber_delta, double *losses, double *grad, double *grad_abs);
 /* Proposal variants: steps in bins whose bit of bin_mask[b] is clear are
  * skipped (bin k of a ray: floor(k step 64 / (t_far - t_near)), at most 63);
  * the forward also returns, per batch entry, the bins holding a sample of
@@ -105,14 +121,29 @@ void orc_render_bwd_masked(const orc_grid *g, const orc_rays *r, const int32_t *
 void orc_tv_disparity(const orc_grid *g, double lambda_density, double lambda_sh,
                       int32_t optimize_step, double *grad);
 
-/* totalVarianceLoss, 6-neighbour branch, dense grid (RenderUtils.cu:2660-2763). */
+/* totalVarianceLoss, 6-neighbour branch (RenderUtils.cu:2660-2763): the
+ * reference's loop over interior vertices whose column y W + x is below
+ * col_limit (W H for type 0; the launch of type 2, RenderUtils.cu:2840-2843,
+ * covers ceil((W-1)/16) ceil((H-1)/16) 256), serially.  Caged grids: the
+ * vertex's record r = index_offsets[v] is skipped unless in_cage[r] (the
+ * flag array is per dense vertex; the reference reads it at the record
+ * number, :2669); a neighbour whose index (record) is 0 reads the vertex
+ * itself. */
 void orc_tv_neighbour6(const orc_grid *g, double lambda_density, double lambda_sh,
                        double *grad);
+void orc_tv_neighbour6_cols(const orc_grid *g, double lambda_density, double lambda_sh,
+                            int64_t col_limit, double *grad);
 
 /* updateRGBSceneGPU_ADAM (Optimizer.cu:359-415) in the reference's own
  * float/double mix.  Planes [P][n]; grad is zeroed. */
 void orc_adam(float *params, float *grad, float *m, float *v, const float *adaptive_lr,
               int32_t P, int64_t n, double lr, double sh_lr, int32_t t);
+/* The same restricted, when S > 0 (P = 1 + 3 S), to the density and the SH
+ * coefficients [sh_lo, sh_hi) of each channel (hierarchyStrategy,
+ * Optimizer.cu:393-412); other planes keep their values. */
+void orc_adam_band(float *params, float *grad, float *m, float *v, const float *adaptive_lr,
+                   int32_t P, int32_t S, int64_t n, double lr, double sh_lr, int32_t t,
+                   int32_t sh_lo, int32_t sh_hi);
 
 /* getMSEError (Utils.cu:311-320). */
 /* upsampleScene + upsample_trilinear (SceneUtils.cu:262-485).

@@ -166,3 +166,39 @@ def to_16bit_planes(planes, storage):
     if storage == pxl.STORE_BF16:
         return widen_bf16(round_bf16(planes))
     return planes.astype(np.float16).astype(np.float32)
+
+
+def grad_close_cond(g, ref, absum, rel=1e-4, floor=1e-3, kappa=1e-6, rel0=None):
+    """grad_close, or within kappa of the entry's magnitude sum (absum, from
+    oracle.render_bwd_cond: the sum of the magnitudes of every term that
+    enters the entry).  An entry whose terms cancel is known to FP32
+    arithmetic only to a few rounding units (2^-24 each) of that sum, which
+    can exceed 1e-4 of the entry itself; kappa = 1e-6 allows ~16 of them.
+    rel0 (optional): record 0 within rel0 relative instead.  Returns the
+    worst ratio and the number of entries held by the magnitude term only."""
+    worst = 0.0
+    where = None
+    n_cond = 0
+    for p in range(ref.shape[0]):
+        scale = np.abs(ref[p]).max()
+        if scale == 0:
+            assert np.abs(g[p]).max() == 0, p
+            continue
+        tol_rel = rel * (np.abs(ref[p]) + floor * scale)
+        if rel0 is not None:
+            tol_rel[0] = rel0 * abs(ref[p][0]) + rel * floor * scale
+        err = np.abs(g[p] - ref[p])
+        n_cond += int(((err > tol_rel) & (err <= kappa * absum[p])).sum())
+        tol = np.maximum(tol_rel, kappa * absum[p])
+        if rel0 is not None:
+            tol[0] = tol_rel[0]
+        r = err / tol
+        i = int(np.argmax(r))
+        if r[i] > worst:
+            worst = float(r[i])
+            where = (p, i, float(g[p][i]), float(ref[p][i]), float(scale), float(absum[p][i]))
+    if worst > 1.0:
+        print(f"    worst at plane {where[0]} record {where[1]}: got {where[2]:.6g} "
+              f"ref {where[3]:.6g} (plane max {where[4]:.3g}, magnitude sum {where[5]:.3g})",
+              flush=True)
+    return worst, n_cond

@@ -104,3 +104,52 @@ def test_read_image_text(tmp_path):
     assert pxl.read_image_text(prefix, W, H)["bounds"] == [H - 1, 0, W - 1, 0]
     with pytest.raises(pxl.PxlError):
         pxl.read_image_text(prefix, W, H + 1)
+
+
+def _write_caged_dir(d, occ, cage, io, n):
+    """A band-0 caged directory in the reference's file layout
+    (objectScene::saveSelf, SceneUtils.cu:1246-1299)."""
+    def ints(name, a):
+        (d / name).write_text(" ".join(str(int(x)) for x in a) + "\n")
+
+    def floats(name, a):
+        (d / name).write_text(" ".join(f"{x:.6f}" for x in a) + "\n")
+
+    ints("occupancy.txt", occ)
+    ints("sparse_inCageInfo_band0.txt", cage)
+    ints("sparse_indexOffset.txt", io)
+    rng = np.random.default_rng(3)
+    floats("sparse_opacity_band0.txt", rng.uniform(0, 20, n))
+    for ch in ("red", "green", "blue"):
+        floats(f"{ch}_0_sparse.txt", rng.normal(0, 1, n))
+
+
+@pytest.mark.parametrize("corrupt", ["outside", "first"])
+def test_grid_load_text_rejects_bad_offsets(tmp_path, corrupt):
+    """A caged directory whose index file sends a vertex outside the cage
+    (or the first in-cage vertex) to a record other than 0 is rejected
+    before anything reaches the device: kernels would otherwise read and
+    write records that do not exist (getCagedNums gives those vertices 0,
+    SceneUtils.cu:1106-1113)."""
+    import ctypes
+
+    import oracle
+
+    N = 5
+    dens = np.zeros(N ** 3, np.float32)
+    dens[2 * N * N + 2 * N + 2] = 30.0
+    occ, cage, _ = oracle.build_occupancy(dens, N, N, N, 10.0)
+    io, n = oracle.build_index(cage, N, N, N)
+    io = io.astype(np.int64)
+    if corrupt == "outside":
+        io[int(np.flatnonzero(cage == 0)[-1])] = 10 ** 6
+    else:
+        io[int(np.flatnonzero(cage)[0])] = 1
+    _write_caged_dir(tmp_path, occ, cage, io, n)
+    L = pxl.lib()
+    dummy = ctypes.c_void_p(16)  # never reached: validation fails first
+    rc = L.pxl_grid_load_text(str(tmp_path).encode(), 0, 1, 1, N, N, N, 1, 4, n, dummy, dummy,
+                              dummy, dummy, dummy, None)
+    assert rc == pxl.PXL_EINVAL
+    msg = L.pxl_last_error()
+    assert (b"outside the cage" in msg) if corrupt == "outside" else (b"first in-cage" in msg), msg

@@ -6,13 +6,20 @@ four 800 x 800 views of the upper hemisphere, random 8-bit targets.
   at 256^3;
 - 128 rays of the batch against the FP64 oracle on the full grid (colours
   1e-5, losses 255 * 1e-5, gradients helpers.grad_close), fused and logged;
-- the whole batch: the fused kernel, the logged step, the persistent
-  one-wave schedule and the chunk-parallel gradient give the same per-ray
-  colours and losses, and gradients equal up to the order of FP32 atomics
-  (record 0, where every out-of-cage corner lands, within 1e-3 relative:
-  helpers.grad_close_alias);
+- the whole batch against the FP64 oracle (threaded on the host): the
+  fused kernel, the logged step, the persistent one-wave schedule and the
+  chunk-parallel gradient give the oracle's per-ray colours and losses, and
+  gradients within helpers.grad_close or, for entries whose terms cancel
+  (one entry here is 9.9e-7 from four rays' terms of 1e-3 in magnitude),
+  within 1e-6 of the entry's magnitude sum (helpers.grad_close_cond: FP32
+  arithmetic resolves such an entry to a few rounding units of that sum);
+  record 0, where every out-of-cage
+  corner lands (~10^5 largely cancelling terms), within 1e-4 relative on the
+  logged paths, which sum it per ray in FP64 and fold it in a fixed order;
+  the single fused kernel (no workspace) adds it with one float atomic per
+  ray and is held to 1e-3 there;
 - the whole batch's gradient is the sum of its two halves' (no cross-ray
-  interference, record 0 included);
+  interference, record 0 included, 1e-4);
 - TV over the full caged grid and one Adam step over all records against
   the oracle.
 """
@@ -23,7 +30,7 @@ import pytest
 
 import oracle
 import pxl
-from helpers import from_records, grad_close, grad_close_alias, pack_bits
+from helpers import from_records, grad_close, grad_close_alias, grad_close_cond, pack_bits
 
 pytestmark = pytest.mark.gpu
 
@@ -38,6 +45,11 @@ def say(*a):
 
 @pytest.fixture(scope="module")
 def c2(dev):
+    return build_c2(dev)
+
+
+def build_c2(dev):
+    """The 256^3 caged scene, its rays and the 65 536-ray batch."""
     import torch
 
     from pxl import scenes
@@ -96,6 +108,10 @@ def c2(dev):
 
 @pytest.fixture(scope="module")
 def c2_oracle(c2):
+    return build_c2_oracle(c2)
+
+
+def build_c2_oracle(c2):
     """The oracle's occupancy and index of the same density, its grid over
     the same records, and the rays as host arrays."""
     occ_o, cage_o, n_o = oracle.build_occupancy(c2["dens"].cpu().numpy(), N, N, N, c2["tau"])
@@ -167,21 +183,46 @@ def test_c2_sampled_rays(c2, c2_oracle, path):
     assert worst <= 1.0
 
 
-def test_c2_schedules_agree(c2):
+@pytest.fixture(scope="module")
+def c2_full_ref(c2, c2_oracle):
+    """The oracle's forward and backward of the whole 65 536-ray batch."""
+    import os
+    import time
+
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    oracle.set_threads(threads)
+    try:
+        t0 = time.time()
+        batch = c2["batch"]
+        fwd = oracle.render_fwd(c2_oracle["grid"], c2_oracle["rays"], batch)
+        l_ref, g_ref, g_abs = oracle.render_bwd_cond(c2_oracle["grid"], c2_oracle["rays"], batch,
+                                                     fwd, mask_coef=1.0, huber_delta=0.1)
+        say(f"[c2] oracle of the whole batch on {threads} threads: {time.time() - t0:.1f} s")
+    finally:
+        oracle.set_threads(1)
+    cols = np.stack([fwd[k][batch] for k in "rgbw"])
+    return cols, l_ref, g_ref, g_abs
+
+
+@pytest.mark.parametrize("path", ["fused", "logged", "persistent", "chunked"])
+def test_c2_full_batch_vs_oracle(c2, c2_full_ref, path):
     import torch
 
     si = torch.from_numpy(c2["batch"]).to(c2["dev"])
-    say("[c2] full batch: fused, logged, persistent, chunked")
-    o_f, l_f, c_f = _train(c2, si, "fused")
-    g_f = from_records(o_f.grad, P)
-    del o_f
-    for path in ("logged", "persistent", "chunked"):
-        o, l, c = _train(c2, si, path)
-        dc, dl = np.abs(c - c_f).max(), np.abs(l - l_f).max()
-        worst = grad_close_alias(from_records(o.grad, P), g_f)
-        say(f"    {path}: colour diff {dc:.3g}, loss diff {dl:.3g}, gradient / tolerance {worst:.3g}")
-        assert dc <= 1e-6 and dl <= 255 * 1e-6
-        assert worst <= 1.0
+    cols_ref, l_ref, g_ref, g_abs = c2_full_ref
+    say(f"[c2] full batch vs oracle: {path}")
+    o, l, c = _train(c2, si, path)
+    dc, dl = np.abs(c - cols_ref).max(), np.abs(l - l_ref).max()
+    g = from_records(o.grad, P)
+    del o
+    rel0 = 1e-3 if path == "fused" else 1e-4
+    worst, n_cond = grad_close_cond(g, g_ref, g_abs, rel0=rel0)
+    e0 = max(abs(g[p][0] - g_ref[p][0]) / max(1e-30, abs(g_ref[p][0])) for p in range(P))
+    say(f"    colour error {dc:.3g}, loss error {dl:.3g}, gradient / tolerance {worst:.3g} "
+        f"({n_cond} of {int((g_ref != 0).sum())} nonzero entries held by the magnitude term), "
+        f"record 0 relative error {e0:.3g} (bar {rel0:g})")
+    assert dc <= 1e-5 and dl <= 255 * 1e-5
+    assert worst <= 1.0
 
 
 def test_c2_gradient_additive(c2):
@@ -198,7 +239,7 @@ def test_c2_gradient_additive(c2):
     del o_a
     o_b, _, _ = _train(c2, si[h:].clone(), "logged")
     g_sum += from_records(o_b.grad, P)
-    worst = grad_close_alias(g_full, g_sum)
+    worst = grad_close_alias(g_full, g_sum, rel0=1e-4)
     say(f"    worst difference / tolerance = {worst:.3g}")
     assert worst <= 1.0
 

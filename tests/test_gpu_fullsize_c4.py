@@ -1,25 +1,21 @@
 """Parity at the uneven-ray workload's full size: the 512^3 caged grid of
-thin shells and blobs (cells occupied above half the shell amplitude, 5.1 M
-records) stored in bfloat16, and a 262 144-ray batch with origins on a
-radius-6 sphere aimed at uniform points of the central half-cube (0 to ~300
-samples per ray), random 8-bit targets.
+thin shells and blobs (1.02 % of the cells occupied) stored in bfloat16, and
+a 262 144-ray batch with origins on a radius-6 sphere aimed at uniform points
+of the central half-cube, random 8-bit targets.
 
-- 1024 rays of the batch against the FP64 oracle on the widened bfloat16 grid
-  (colours 1e-5, losses 255 * 1e-5; gradients within 1e-4 relative plus
-  1e-5 of the plane's largest entry, helpers.grad_close with floor 0.1).
-  The wider floor than the 128^3 tests' is for the density gradient of
-  long rays through the shells: its per-sample term T c - (C - H) nearly
-  cancels where a sample's colour matches what lies behind it, and the
-  FP32 per-sample arithmetic then differs from FP64 by up to ~3e-3 relative
-  on entries ~1e-3 of the plane maximum (~2e-6 of the maximum absolute);
-- the whole batch under every launch schedule the autotuner chooses from
-  (1 or 4 waves per block, persistent or not, the chunk-parallel gradient;
-  given, Morton and length + Morton order): identical per-ray colours and
-  losses, gradients equal up to the order of FP32 atomics (the bar of the
-  sampled-ray check: the same cancelling density entries also carry the
-  different rounding of differently grouped FP32 sums; record 0, the sum of
-  every out-of-cage corner of the batch, within 1e-3 relative:
-  helpers.grad_close_alias);
+- the per-ray sample distribution (marched positions and evaluated samples)
+  is reported;
+- the whole batch against the FP64 oracle on the widened bfloat16 grid,
+  under every launch schedule the autotuner chooses from (1 or 4 waves per
+  block, persistent or not, the chunk-parallel gradient; given, Morton and
+  length + Morton order): colours 1e-5, losses 255 * 1e-5 (identical across
+  schedules), gradients within helpers.grad_close_cond: 1e-4 relative with
+  a floor of 1e-3 of the plane maximum, or, where the entry's terms cancel,
+  1e-6 of their magnitude sum.  Long rays through the shells make density
+  entries whose per-sample term T c - (C - H) nearly cancels, which FP32
+  arithmetic resolves only to a few rounding units of the terms; record 0,
+  the sum of every out-of-cage corner of the batch, within 1e-4 relative
+  (the logged step sums it per ray in FP64 and folds it in a fixed order);
 - one Adam step over all records with the FP32 master copy against the
   oracle's FP32 Adam, the grid holding the master's round-to-nearest-even
   bfloat16 value.
@@ -31,7 +27,7 @@ import pytest
 
 import oracle
 import pxl
-from helpers import from_records, grad_close, grad_close_alias, round_bf16, widen_bf16
+from helpers import from_records, grad_close_cond, round_bf16, widen_bf16
 
 pytestmark = [pytest.mark.gpu, pytest.mark.timeout(600)]
 
@@ -129,8 +125,12 @@ def _train(c4, si, wpb=4, persistent=False, chunked=False, gwpb=0):
     return optim, losses.cpu().numpy(), cols, idx
 
 
-def test_c4_sampled_rays(c4):
-    import torch
+@pytest.fixture(scope="module")
+def c4_oracle(c4):
+    """The oracle's occupancy, index and grid of the widened bfloat16 records,
+    and its forward and backward of the whole batch (threaded on the host)."""
+    import os
+    import time
 
     occ_o, cage_o, _ = oracle.build_occupancy(c4["dens"].cpu().numpy(), N, N, N, c4["tau"])
     io_o, cnt_o = oracle.build_index(cage_o, N, N, N)
@@ -141,35 +141,36 @@ def test_c4_sampled_rays(c4):
     rays_o = oracle.Rays({k: getattr(r, k).cpu().numpy() for k in
                           ("ox", "oy", "oz", "dx", "dy", "dz", "near_t", "far_t", "gt_r", "gt_g",
                            "gt_b", "gt_mask")})
-    sub = np.sort(np.random.default_rng(44).choice(BATCH, 1024, replace=False)).astype(np.int32)
-    say("[c4] 1024 rays vs the oracle on the widened bfloat16 grid")
-    optim, losses, cols, _ = _train(c4, torch.from_numpy(sub).to(c4["dev"]))
-    fwd = oracle.render_fwd(grid_o, rays_o, sub)
-    l_ref, g_ref = oracle.render_bwd(grid_o, rays_o, sub, fwd, mask_coef=1.0, huber_delta=0.1)
-    err = max(np.abs(cols[i] - fwd[k][sub]).max() for i, k in enumerate("rgb"))
-    say(f"    max colour error {err:.3g}, max loss error {np.abs(losses - l_ref).max():.3g}")
-    assert err <= 1e-5
-    assert np.abs(losses - l_ref).max() <= 255 * 1e-5
-    g = from_records(optim.grad, P)
-    strict = sum(int((np.abs(g[p] - g_ref[p]) > 1e-4 * (np.abs(g_ref[p]) +
-                                                         1e-3 * np.abs(g_ref[p]).max())).sum())
-                 for p in range(P))
-    worst = grad_close(g, g_ref, floor=0.1)
-    say(f"    worst gradient error / tolerance = {worst:.3g}; {strict} of "
-        f"{int((g_ref != 0).sum())} nonzero entries outside the 1e-3-floor bar")
-    assert worst <= 1.0
+    batch = np.arange(BATCH, dtype=np.int32)
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    oracle.set_threads(threads)
+    try:
+        t0 = time.time()
+        fwd = oracle.render_fwd(grid_o, rays_o, batch, trace=True)
+        l_ref, g_ref, g_abs = oracle.render_bwd_cond(grid_o, rays_o, batch, fwd, mask_coef=1.0,
+                                                     huber_delta=0.1)
+        say(f"[c4] oracle of the whole batch on {threads} threads: {time.time() - t0:.1f} s")
+    finally:
+        oracle.set_threads(1)
+    cnt = fwd["trace"][:, 0].astype(np.int64)
+    tn, tf = rays_o.d["near_t"].astype(np.float64), rays_o.d["far_t"].astype(np.float64)
+    marched = np.where(tf >= tn, np.ceil((tf - tn) / 0.5), 0)
+    q = lambda a: ", ".join(f"{p}%: {int(np.percentile(a, p))}" for p in (50, 90, 99, 100))
+    say(f"[c4] marched positions per ray: mean {marched.mean():.0f} ({q(marched)}); evaluated "
+        f"samples per ray: mean {cnt.mean():.1f} ({q(cnt)}); rays with none "
+        f"{float((cnt == 0).mean()):.3f}")
+    cols = np.stack([fwd[k] for k in "rgb"])
+    return dict(cols=cols, losses=l_ref, grad=g_ref, gabs=g_abs)
 
 
-def test_c4_schedules_agree(c4):
+def test_c4_schedules_vs_oracle(c4, c4_oracle):
     import torch
 
     dev = c4["dev"]
     si = torch.arange(BATCH, dtype=torch.int32, device=dev)
-    say("[c4] full batch, 4 waves per block, given order")
-    o_ref, l_ref, c_ref, _ = _train(c4, si)
-    g_ref = from_records(o_ref.grad, P)
-    del o_ref
+    ref = c4_oracle
     grid = _grid(c4)
+    first = None
     for mode, name in ((0, "given"), (pxl.ORDER_MORTON, "Morton"),
                        (pxl.ORDER_LENGTH_MORTON, "length + Morton")):
         if mode:
@@ -180,24 +181,30 @@ def test_c4_schedules_agree(c4):
             order = si
         perm = order.cpu().numpy()
         assert np.array_equal(np.sort(perm), np.arange(BATCH))
-        for wpb, persist, chunked, gwpb in ((1, False, False, 0), (1, False, False, 4),
+        for wpb, persist, chunked, gwpb in ((4, False, False, 0), (1, False, False, 4),
                                             (4, True, False, 0), (1, True, False, 0),
                                             (4, False, True, 0)):
             o, l, c, idx = _train(c4, order, wpb, persist, chunked, gwpb)
             # losses are per batch position: map them back to ray ids
             l_by_ray = np.empty(BATCH)
             l_by_ray[idx] = l
-            c_by_ray = np.empty_like(c_ref)
+            c_by_ray = np.empty_like(c)
             c_by_ray[:, idx] = c
-            dc, dl = np.abs(c_by_ray - c_ref).max(), np.abs(l_by_ray - l_ref).max()
-            worst = grad_close_alias(from_records(o.grad, P), g_ref, floor=0.1)
+            dc = np.abs(c_by_ray - ref["cols"]).max()
+            dl = np.abs(l_by_ray - ref["losses"]).max()
+            worst, n_cond = grad_close_cond(from_records(o.grad, P), ref["grad"], ref["gabs"],
+                                            rel0=1e-4)
             say(f"    {name}, {wpb} waves/block (gradient {gwpb or wpb}), persistent {persist}, "
-                f"chunked {chunked}: "
-                f"colour diff {dc:.3g}, "
-                f"loss diff {dl:.3g}, gradient / tolerance {worst:.3g}")
+                f"chunked {chunked}: colour error {dc:.3g}, loss error {dl:.3g}, gradient / "
+                f"tolerance {worst:.3g} ({n_cond} of {int((ref['grad'] != 0).sum())} nonzero "
+                f"entries held by the magnitude term)")
             del o
-            assert dc == 0.0 and dl == 0.0
+            assert dc <= 1e-5 and dl <= 255 * 1e-5
             assert worst <= 1.0
+            if first is None:
+                first = (c_by_ray, l_by_ray)
+            else:
+                assert np.array_equal(c_by_ray, first[0]) and np.array_equal(l_by_ray, first[1])
 
 
 def test_c4_adam_master(c4):

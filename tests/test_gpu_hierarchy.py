@@ -3,13 +3,15 @@ over two levels: a dense 32^3 SH-2 grid trained at level 1, upsampled to a
 caged 64^3 grid (to_sparse_hierarchy -1), trained at level 0; 4 views of
 32 x 32 random 8-bit targets per level.
 
-- each level runs epochs * ceil(rays / batch) batches at the plan's batch
-  size (trainer time step), on a grid of the level's size;
-- the loop equals the same stages issued by hand (pxl.Trainer per level,
-  pxl.upsample_grid between): same caged record count, per-level batch
-  losses within 1e-6 relative, final parameters equal up to the order of
-  float atomics (at most 1e-3 of them beyond 1e-3 absolute after 26 Adam
-  steps).
+- each level runs epochs * ceil(rays / batch) batches per SH band at the
+  plan's batch size (trainer time step), on a grid of the level's size;
+  with SHsForTargetResolutionOnly (the default configuration) level 1 trains
+  bands 0, 1, 2 in turn and level 0 the whole band-2 grid (Render.cu:267-300);
+- the loop (pxl_hierarchy_plan + pxl_trainer_learn_level in the C++ host
+  layer) equals the same stages issued by hand (pxl.Trainer per level with
+  set_band per band, pxl.upsample_grid between): same caged record count,
+  per-level batch losses within 1e-6 relative, final parameters equal up to
+  the order of float atomics (at most 1e-3 of them beyond 1e-3 absolute).
 """
 import math
 import sys
@@ -99,8 +101,12 @@ def test_two_levels_match_hand_sequence(dev):
         rays, order = make(lv["level"], grid)
         tr = pxl.Trainer(grid, pxl.Optim(grid), rays, order, batch_size=lv["batch_size"],
                          optimize_step=lv["level"], **KW)
-        for _ in range(lv["epochs"] * math.ceil(order.numel() / lv["batch_size"])):
-            tr.step()
+        # Bands 0..2 in band mode above the last level, the whole grid at it.
+        bands = [(0, True), (1, True), (2, True)] if lv["level"] > 0 else [(2, False)]
+        for band, band_mode in bands:
+            tr.set_band(band, band_mode=band_mode)
+            for _ in range(lv["epochs"] * math.ceil(order.numel() / lv["batch_size"])):
+                tr.step()
         hand.append(tr.loss()[0])
         tr.close()
         if lv["upsample"]:

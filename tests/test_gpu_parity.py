@@ -47,11 +47,13 @@ def c1(dev):
 
 
 @pytest.fixture(params=["group", "wave"])
-def marcher(request, monkeypatch):
+def marcher(request):
     """Fixed-rule marcher under test: eight samples per wave (default) or
-    one sample per wave (PXL_MARCHER=wave)."""
-    monkeypatch.setenv("PXL_MARCHER", request.param)
-    return request.param
+    one sample per wave (library option PXL_OPT_MARCHER)."""
+    pxl.set_option(pxl.OPT_MARCHER,
+                   pxl.MARCHER_WAVE if request.param == "wave" else pxl.MARCHER_GROUP)
+    yield request.param
+    pxl.set_option(pxl.OPT_MARCHER, pxl.MARCHER_GROUP)
 
 
 def device_rays(rays_np, dev, white_bg=True):
@@ -225,7 +227,7 @@ def _bwd_case(c1, grid, grid_o, nr, path, huber=0.1):
                           huber_delta=huber)
         torch.cuda.synchronize()
         fwd = oracle.render_fwd(grid_o, rays_o, batch)
-        if (os.environ.get("PXL_MARCHER", "group") != "wave"
+        if (pxl.get_option(pxl.OPT_MARCHER) == pxl.MARCHER_GROUP
                 and grid.march_mode == pxl.MARCH_FIXED):
             # Per-ray counts follow 256 bytes of claim counters and the FP64
             # totals (32 bytes per ray, 256-byte aligned): both the logged
@@ -413,11 +415,11 @@ def test_tv_disparity(c1, sparse1, kind):
 
 @pytest.mark.parametrize("kind", ["f32", "bf16", "wide", "dense", "dense-wide"])
 @pytest.mark.parametrize("form", ["short", "64"])
-def test_tv_disparity_forms(c1, sparse1, kind, form, monkeypatch):
+def test_tv_disparity_forms(c1, sparse1, kind, form, request):
     """TV in both address forms: the short one (caged: 24-bit record
     offsets; dense: 32-bit offsets through a buffer resource) and the
     64-bit form that grids beyond 2^24 records or 4 GiB of records take
-    (forced here by PXL_TV_OFFSETS=64); on caged FP32 and bfloat16 records,
+    (forced here by the PXL_OPT_TV_ADDR64 option); on caged FP32 and bfloat16 records,
     on records wider than 32 parameters (SH degree 3: 49 parameters, one
     wave per record), and on dense grids (interior vertices take their own
     path, boundary vertices the general gather)."""
@@ -425,7 +427,8 @@ def test_tv_disparity_forms(c1, sparse1, kind, form, monkeypatch):
 
     from helpers import to_16bit_planes
 
-    monkeypatch.setenv("PXL_TV_OFFSETS", form)
+    pxl.set_option(pxl.OPT_TV_ADDR64, 1 if form == "64" else 0)
+    request.addfinalizer(lambda: pxl.set_option(pxl.OPT_TV_ADDR64, 0))
     dev = c1["dev"]
     if kind.startswith("dense"):
         N, S = (40, 16) if kind == "dense-wide" else (N1, S1)
@@ -593,18 +596,16 @@ def test_train_step_c1(c1):
 @pytest.mark.parametrize("log_samples,autotune,tv_overlap",
                          [(0, False, True), (512, False, True), (512, True, True),
                           (512, False, False)])
-def test_trainer_matches_stage_calls(c1, log_samples, autotune, tv_overlap, monkeypatch):
+def test_trainer_matches_stage_calls(c1, log_samples, autotune, tv_overlap):
     """The C++ batch loop (pxl_trainer_step) issues the same stages as the
     Python-driven sequence: dense grid (TV every batch), 3 batches, LR decay
     lr /= 2^(1/(batchCount (epoch+1))) per batch (Render.cu:499-500); with
     log_samples the trainer runs the two-kernel logged step, with TV on its
-    second stream beside the forward unless PXL_TRAINER_TV_OVERLAP=0."""
+    second stream beside the forward unless tv_inline."""
     import math
 
     import torch
 
-    if not tv_overlap:
-        monkeypatch.setenv("PXL_TRAINER_TV_OVERLAP", "0")
     say(f"[trainer] 3 batches vs stage calls, log_samples={log_samples}, autotune={autotune}, "
         f"tv_overlap={tv_overlap}")
     dev = c1["dev"]
@@ -622,7 +623,7 @@ def test_trainer_matches_stage_calls(c1, log_samples, autotune, tv_overlap, monk
     tr = pxl.Trainer(grids[0], optims[0], rays_a, order, batch_size=B, learning_rate=0.1,
                      sh_learning_rate=0.2, variance_coefficient=1e-5, sh_coefficient=1e-6,
                      variance_interval=3, fused=True, log_samples=log_samples,
-                     autotune=autotune)
+                     autotune=autotune, tv_inline=not tv_overlap)
     for _ in range(3):
         tr.step()
     losses = torch.zeros(B, dtype=torch.float64, device=dev)

@@ -28,7 +28,7 @@ def test_exports_every_declared_symbol(pxl_lib):
 
 def test_version_and_error_string(pxl_lib):
     assert pxl_lib.lib().pxl_version() >> 16 == 1
-    assert pxl_lib.lib().pxl_abi_sizes(None, 0) == 6
+    assert pxl_lib.lib().pxl_abi_sizes(None, 0) == 8
     assert isinstance(pxl_lib.lib().pxl_last_error(), bytes)
 
 
@@ -38,7 +38,9 @@ def test_struct_layouts_match_header(pxl_lib):
     assert ctypes.sizeof(pxl_lib.BBox) == 24
     assert ctypes.sizeof(pxl_lib.RaysC) == 17 * 8 + 8 + 8
     assert ctypes.sizeof(pxl_lib.GridC) == 24 + 8 + 6 * 8 + 24 + 16
-    assert ctypes.sizeof(pxl_lib.TrainConfigC) == 8 + 4 * 8 + 4 * 4 + 8 + 4 * 2 + 4 * 6
+    assert ctypes.sizeof(pxl_lib.TrainConfigC) == 8 + 4 * 8 + 4 * 4 + 8 + 4 * 2 + 4 * 8
+    assert ctypes.sizeof(pxl_lib.LevelConfigC) == 40
+    assert ctypes.sizeof(pxl_lib.LevelC) == 24
     assert ctypes.sizeof(pxl_lib.ScheduleC) == 16
     assert ctypes.sizeof(pxl_lib.OptimC) == 48
 
@@ -72,11 +74,11 @@ def test_argument_validation_without_device(pxl_lib):
     assert L.pxl_adam_step(None, None, 0.1, 0.1, 1, None) == pxl_lib.PXL_EINVAL
 
 
-def test_render_kernel_name(pxl_lib, monkeypatch):
+def test_render_kernel_name(pxl_lib):
     L = pxl_lib.lib()
     g = pxl_lib.GridC()
     g.sh_dim, g.sparse, g.storage = 9, 1, pxl_lib.STORE_F32
-    monkeypatch.delenv("PXL_MARCHER", raising=False)
+    assert pxl_lib.get_option(pxl_lib.OPT_MARCHER) == pxl_lib.MARCHER_GROUP
     assert L.pxl_render_kernel_name(ctypes.byref(g), 2) == b"k_march8<9,sparse,fused,f32>"
     g.storage = pxl_lib.STORE_BF16
     assert L.pxl_render_kernel_name(ctypes.byref(g), 0) == b"k_march8<9,sparse,fwd,bf16>"
@@ -85,10 +87,14 @@ def test_render_kernel_name(pxl_lib, monkeypatch):
     g.sh_dim, g.march_mode, g.storage = 9, pxl_lib.MARCH_REF_ADAPTIVE, pxl_lib.STORE_F32
     assert L.pxl_render_kernel_name(ctypes.byref(g), 2) == b"k_render<9,dense,adaptive,fused,f32>"
     g.march_mode = pxl_lib.MARCH_FIXED
-    monkeypatch.setenv("PXL_MARCHER", "wave")
-    assert L.pxl_render_kernel_name(ctypes.byref(g), 2) == b"k_render<9,dense,fixed,fused,f32>"
-    assert L.pxl_render_kernel_name(ctypes.byref(g), 3) == b"k_render<9,dense,fixed,fused,f32>"
-    monkeypatch.delenv("PXL_MARCHER")
+    pxl_lib.set_option(pxl_lib.OPT_MARCHER, pxl_lib.MARCHER_WAVE)
+    try:
+        assert L.pxl_render_kernel_name(ctypes.byref(g), 2) == b"k_render<9,dense,fixed,fused,f32>"
+        assert L.pxl_render_kernel_name(ctypes.byref(g), 3) == b"k_render<9,dense,fixed,fused,f32>"
+    finally:
+        pxl_lib.set_option(pxl_lib.OPT_MARCHER, pxl_lib.MARCHER_GROUP)
+    assert L.pxl_set_option(pxl_lib.OPT_MARCHER, 7) == pxl_lib.PXL_EINVAL
+    assert L.pxl_set_option(99, 0) == pxl_lib.PXL_EINVAL
     assert (L.pxl_render_kernel_name(ctypes.byref(g), 3) ==
             b"k_march8<9,dense,log,f32>+k_log_grad<9>")
     assert L.pxl_render_kernel_name(ctypes.byref(g), 4) is None
