@@ -32,7 +32,7 @@ import pytest
 import oracle
 import pxl
 from helpers import (c1_planes, cameras, device_sparse_grid, from_records, grad_close,
-                     oracle_rays, sparse_from_dense, to_records)
+                     grad_close_alias, oracle_rays, sparse_from_dense, to_records)
 
 pytestmark = pytest.mark.gpu
 
@@ -311,7 +311,9 @@ def test_nan_parameter(sc, kind, path):
     assert np.abs(losses[~hit] - l_ref[~hit]).max() <= 255 * 1e-5
     g = from_records(optim.grad, grid.P)
     assert np.isfinite(g).all()
-    assert grad_close(g, g_ref) <= 1.0
+    # Record 0 of a caged grid: the single fused kernel adds each ray's sum
+    # with a float atomic (1e-3 relative), the logged step folds FP64 sums.
+    assert grad_close_alias(g, g_ref, rel0=1e-3 if path == "fused" else 1e-4) <= 1.0
     out = torch.zeros(2, dtype=torch.float64, device=sc["dev"])
     pxl.loss_reduce(torch.from_numpy(losses).to(sc["dev"]), out)
     torch.cuda.synchronize()
