@@ -17,6 +17,7 @@
 // 8 x (1+3S) scattered atomicAdds per sample, RenderUtils.cu:1074-1364).
 #include <algorithm>
 #include <cstdlib>
+#include <mutex>
 #include <type_traits>
 
 #include "common.hpp"
@@ -1304,6 +1305,72 @@ extern "C" int32_t pxl_train_rays_sched(const pxl_grid *grid, const pxl_optim *o
     x.workspace_bytes = workspace_bytes;
     x.max_samples = max_samples;
     x.sched = sched;
+    return bwd_common(grid, optim, rays, losses, sample_index, start, len, mask_coef,
+                      huber_delta, stream, true, x);
+}
+
+// Sample log of pxl_train_rays_noalloc: one array per GPU, held by the
+// library until the process ends.  Its name is a host-side stand-in on the
+// host: kernels get the device address hipGetSymbolAddress returns for the
+// current device (scratch_of_device).
+static __device__ __align__(256) unsigned char scratch[PXL_NOALLOC_SCRATCH_BYTES];
+
+namespace pxl {
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up on the first
+// call on that device.
+int32_t scratch_of_device(void **out, const char *who) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+        set_error("%s: no current device (or device %d beyond %d)", who, dev, kMaxDevices);
+        return PXL_EHIP;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || p == nullptr) {
+            set_error("%s: hipGetSymbolAddress of the sample log failed on device %d: %s", who,
+                      dev, hipGetErrorString(e));
+            return PXL_EHIP;
+        }
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return PXL_OK;
+}
+}  // namespace
+}  // namespace pxl
+
+extern "C" int32_t pxl_train_rays_noalloc(const pxl_grid *grid, const pxl_optim *optim,
+                                          const pxl_rays *rays, double *losses,
+                                          const int32_t *sample_index, int64_t start,
+                                          int64_t len, double mask_coef, float huber_delta,
+                                          int32_t max_samples, const pxl_schedule *sched,
+                                          void *stream) {
+    clear_error();
+    const char *who = "pxl_train_rays_noalloc";
+    PXL_REQUIRE(len >= 0 && max_samples > 0 && max_samples <= kMaxLogSamples,
+                "%s: len %lld / max_samples %d outside [1, %d]", who, (long long)len, max_samples,
+                kMaxLogSamples);
+    const size_t need = log_layout(len, max_samples).total;
+    PXL_REQUIRE(need <= (size_t)PXL_NOALLOC_SCRATCH_BYTES,
+                "%s: %lld rays x %d samples need a %zu-byte sample log, beyond the library's "
+                "%zu bytes (PXL_NOALLOC_SCRATCH_BYTES); use pxl_train_rays_sched with a workspace",
+                who, (long long)len, max_samples, need, (size_t)PXL_NOALLOC_SCRATCH_BYTES);
+    void *ws = nullptr;
+    const int32_t st = scratch_of_device(&ws, who);
+    if (st != PXL_OK) return st;
+    const pxl_schedule dflt = {4, 0, 0, 0};
+    TrainExtra x;
+    x.workspace = ws;
+    x.workspace_bytes = (size_t)PXL_NOALLOC_SCRATCH_BYTES;
+    x.max_samples = max_samples;
+    x.sched = sched ? sched : &dflt;
     return bwd_common(grid, optim, rays, losses, sample_index, start, len, mask_coef,
                       huber_delta, stream, true, x);
 }

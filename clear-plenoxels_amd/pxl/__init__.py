@@ -45,7 +45,9 @@ EXPORTS = (
     "pxl_grid_save_text", "pxl_grid_text_records", "pxl_grid_load_text",
     "pxl_set_option", "pxl_get_option", "pxl_trainer_render_ms_split", "pxl_optim_init_band",
     "pxl_trainer_set_band", "pxl_trainer_learn_level", "pxl_hierarchy_plan",
+    "pxl_train_rays_noalloc",
 )
+NOALLOC_SCRATCH_BYTES = 64 << 20
 
 PROP_IN, PROP_WIDTH, PROP_BINS = 128, 64, 64
 PROP_WEIGHTS = PROP_IN * PROP_WIDTH + 2 * PROP_WIDTH * PROP_WIDTH
@@ -217,6 +219,8 @@ def _declare(L):
                                       _F64, _F32, _F64], _I32),
         "pxl_trainer_proposal_loss": ([_P, ctypes.POINTER(_F64)], _I32),
         "pxl_set_option": ([_I32, _I64], _I32),
+        "pxl_train_rays_noalloc": ([G, O, R, _P, _P, _I64, _I64, _F64, _F32, _I32,
+                                    ctypes.POINTER(ScheduleC), _P], _I32),
         "pxl_get_option": ([_I32, ctypes.POINTER(_I64)], _I32),
         "pxl_trainer_render_ms_split": ([_P, ctypes.POINTER(_F64), ctypes.POINTER(_I64),
                                          ctypes.POINTER(_F64), ctypes.POINTER(_I64)], _I32),

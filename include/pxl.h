@@ -499,6 +499,26 @@ int32_t pxl_train_rays_sched(const pxl_grid *grid, const pxl_optim *optim,
                              size_t workspace_bytes, int32_t max_samples,
                              const pxl_schedule *sched, void *stream);
 
+/* pxl_train_rays_sched without a workspace argument and without any
+ * allocation: the sample log lives in an array of PXL_NOALLOC_SCRATCH_BYTES
+ * (64 MiB) of device memory that the library holds on every GPU it runs on
+ * until the process ends (its address is looked up on the first call on a
+ * device).  No counterpart in the reference: for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops).  Rejects (PXL_EINVAL) a batch whose log would exceed the array:
+ * pxl_train_rays_workspace_bytes(len, max_samples) must be at most
+ * PXL_NOALLOC_SCRATCH_BYTES (for example 4096 rays x 240 samples).  Every
+ * call on one GPU uses the same array, which keeps the last call's
+ * contents: calls that overlap on one GPU (other streams, other threads)
+ * race and must be serialised by the caller.  sched may be NULL (4 waves per
+ * block).  Calls no hipMalloc / hipFree. */
+#define PXL_NOALLOC_SCRATCH_BYTES (64u << 20)
+int32_t pxl_train_rays_noalloc(const pxl_grid *grid, const pxl_optim *optim,
+                               const pxl_rays *rays, double *losses,
+                               const int32_t *sample_index, int64_t start, int64_t len,
+                               double mask_coef, float huber_delta, int32_t max_samples,
+                               const pxl_schedule *sched, void *stream);
+
 /* Batch orders (pxl_batch_order). */
 enum {
     PXL_ORDER_MORTON = 1,        /* Morton code of the grid entry point */
