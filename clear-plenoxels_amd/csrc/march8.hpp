@@ -139,17 +139,19 @@ __device__ __forceinline__ bool ray_residuals(const RenderArgs &a, int ray, doub
 // channel (g0..g2, before the SH basis and trilinear weight) and dL/draw-
 // density (gs, before the trilinear weight).  ws = T_i alpha_i, Tn =
 // T_{i+1}, Ai and H* the opacity / colour prefixes including sample i.
+// R: the type the activations were evaluated in (float under the fixed
+// rule, double under the adaptive rule).
+template <typename R>
 __device__ __forceinline__ void sample_grads(const RayResid &rr, double C0, double C1,
                                              double C2, double A, double step, double ws,
                                              double Tn, double Ai, double H0, double H1,
-                                             double H2, float col0, float col1, float col2,
-                                             float sraw, float &g0, float &g1, float &g2,
-                                             double &gs) {
+                                             double H2, R col0, R col1, R col2, R sraw,
+                                             float &g0, float &g1, float &g2, double &gs) {
     const double kc = (double)0.6667f;
-    g0 = (float)(kc * ws * rr.r0 * (double)(0.1f * col0 * (1.0f - col0)));
-    g1 = (float)(kc * ws * rr.r1 * (double)(0.1f * col1 * (1.0f - col1)));
-    g2 = (float)(kc * ws * rr.r2 * (double)(0.1f * col2 * (1.0f - col2)));
-    const double ds = (double)dsoftplus_shift<float>(sraw);
+    g0 = (float)(kc * ws * rr.r0 * (double)((R)0.1 * col0 * ((R)1 - col0)));
+    g1 = (float)(kc * ws * rr.r1 * (double)((R)0.1 * col1 * ((R)1 - col1)));
+    g2 = (float)(kc * ws * rr.r2 * (double)((R)0.1 * col2 * ((R)1 - col2)));
+    const double ds = (double)dsoftplus_shift<R>(sraw);
     gs = rr.r0 * (Tn * (double)col0 - (C0 - H0)) + rr.r1 * (Tn * (double)col1 - (C1 - H1)) +
          rr.r2 * (Tn * (double)col2 - (C2 - H2));
     gs *= 0.6667 * step * ds;
@@ -638,22 +640,62 @@ __device__ __forceinline__ double wave_scan_mul64(double v, int lane) {
 // samples, so the atomics never hold up the wave.  chunk >= 0: only the
 // logged samples 64 chunk .. 64 chunk + 63, starting from the prefix
 // k_log_prefix stored for them (chunk-parallel schedule).
-template <int S, bool TGT>
+// ADAPT: the log of the adaptive rule's marcher (k_adapt8): per sample 8
+// FP64 words (raw density, three raw colours, the position t), activations
+// in FP64.
+template <int S, bool TGT, bool ADAPT>
 __device__ __forceinline__ void log_grad_ray(const RenderArgs &a, const int64_t b,
                                              const int chunk = -1);
 
+// The logged raw values of sample row i of entry b and the activations of
+// the rule: sigma / alpha / colours (float under the fixed rule, FP64 under
+// the adaptive one), and the sample's position t.
+template <bool ADAPT>
+struct LoggedSample {
+    using R = typename std::conditional<ADAPT, double, float>::type;
+    R sraw, col0, col1, col2, alpha;
+    double t;
+    int k;
+};
+template <bool ADAPT>
+__device__ __forceinline__ LoggedSample<ADAPT> read_logged(const RenderArgs &a, size_t row,
+                                                           bool act, double step, double tnear) {
+    using R = typename LoggedSample<ADAPT>::R;
+    LoggedSample<ADAPT> o;
+    if (ADAPT) {
+        const double *v = reinterpret_cast<const double *>(a.log_val) + row * 8;
+        o.sraw = (R)v[0];
+        o.col0 = (R)sigmoid01<double>(v[1]);
+        o.col1 = (R)sigmoid01<double>(v[2]);
+        o.col2 = (R)sigmoid01<double>(v[3]);
+        o.t = v[4];
+        o.k = 0;
+    } else {
+        const float4 v = *reinterpret_cast<const float4 *>(a.log_val + row * 8);
+        o.k = __float_as_int(a.log_val[row * 8 + 4]);
+        o.sraw = (R)v.x;
+        o.col0 = (R)v.y;
+        o.col1 = (R)v.z;
+        o.col2 = (R)v.w;
+        o.t = fma((double)(act ? o.k : 0), step, tnear);
+    }
+    const R sigma = act ? softplus_shift<R>(o.sraw) : (R)0;
+    o.alpha = act ? one_minus_exp_neg<R>(sigma * (R)step) : (R)0;
+    return o;
+}
+
 // TGT: also write the proposal targets (a.bin_targets, a.target_valid).
-template <int S, bool PERSIST = false, bool TGT = false>
+template <int S, bool PERSIST = false, bool TGT = false, bool ADAPT = false>
 __global__ __launch_bounds__(256) void k_log_grad(const RenderArgs a) {
     if (PERSIST) {
         int slice = blockIdx.x & 7, visited = 0;
         for (int64_t b = claim_ray(a.work + 8, a.len, slice, visited); b < a.len;
              b = claim_ray(a.work + 8, a.len, slice, visited))
-            log_grad_ray<S, TGT>(a, b);
+            log_grad_ray<S, TGT, ADAPT>(a, b);
     } else {
         const int64_t b = (int64_t)xcd_slice_block(blockIdx.x, gridDim.x) * (blockDim.x >> 6) +
                           (threadIdx.x >> 6);
-        if (b < a.len) log_grad_ray<S, TGT>(a, b);
+        if (b < a.len) log_grad_ray<S, TGT, ADAPT>(a, b);
     }
 }
 
@@ -663,29 +705,27 @@ __global__ __launch_bounds__(256) void k_log_grad(const RenderArgs a) {
 // in the same order, as log_grad_ray, so every chunk starts from the values
 // the one-wave-per-ray kernel carries into it.  One wave per batch entry;
 // entries with at most 64 logged samples have nothing to store.
-template <int S>
+template <int S, bool ADAPT = false>
 __global__ __launch_bounds__(256) void k_log_prefix(const RenderArgs a) {
     const int64_t b = (int64_t)xcd_slice_block(blockIdx.x, gridDim.x) * (blockDim.x >> 6) +
                       (threadIdx.x >> 6);
     if (b >= a.len) return;
     const int lane = threadIdx.x & 63;
     const int cnt = uniform_i(a.log_count[b]);
-    const float fstep = a.step;
+    const double step = (double)a.step;
     double T = 1.0, Acc = 0.0, Cr = 0.0, Cg = 0.0, Cb = 0.0;
     for (int c0 = 0; c0 + 64 < cnt; c0 += 64) {
-        const size_t at = ((size_t)b * (size_t)a.log_cap + (size_t)(c0 + lane)) * 8;
-        const float4 v = *reinterpret_cast<const float4 *>(a.log_val + at);
-        const float sigma = softplus_shift<float>(v.x);
-        const float alpha = one_minus_exp_neg<float>(sigma * fstep);
-        const double keep = 1.0 - (double)alpha;
+        const size_t row = (size_t)b * (size_t)a.log_cap + (size_t)(c0 + lane);
+        const LoggedSample<ADAPT> v = read_logged<ADAPT>(a, row, true, step, 0.0);
+        const double keep = 1.0 - (double)v.alpha;
         const double incl = wave_scan_mul64(keep, lane);
         double excl = __shfl_up(incl, 1, 64);
         if (lane == 0) excl = 1.0;
-        const double ws = T * excl * (double)alpha;
+        const double ws = T * excl * (double)v.alpha;
         const double Ai = Acc + wave_scan_add64(ws, lane);
-        const double H0 = Cr + wave_scan_add64(ws * (double)v.y, lane);
-        const double H1 = Cg + wave_scan_add64(ws * (double)v.z, lane);
-        const double H2 = Cb + wave_scan_add64(ws * (double)v.w, lane);
+        const double H0 = Cr + wave_scan_add64(ws * (double)v.col0, lane);
+        const double H1 = Cg + wave_scan_add64(ws * (double)v.col1, lane);
+        const double H2 = Cb + wave_scan_add64(ws * (double)v.col2, lane);
         T *= readlane_d(incl, 63);
         Acc = readlane_d(Ai, 63);
         Cr = readlane_d(H0, 63);
@@ -701,15 +741,15 @@ __global__ __launch_bounds__(256) void k_log_prefix(const RenderArgs a) {
 // log), entry = item / log_chunks; the corner accumulators are flushed at
 // the end of each chunk.  For batches too small to fill the chip, where the
 // longest rays' chunks would otherwise run one after another.
-template <int S>
+template <int S, bool ADAPT = false>
 __global__ __launch_bounds__(256) void k_log_grad_chunked(const RenderArgs a, int64_t n_items) {
     const int64_t item = (int64_t)xcd_slice_block(blockIdx.x, gridDim.x) * (blockDim.x >> 6) +
                          (threadIdx.x >> 6);
     if (item >= n_items) return;
-    log_grad_ray<S, false>(a, item / a.log_chunks, (int)(item % a.log_chunks));
+    log_grad_ray<S, false, ADAPT>(a, item / a.log_chunks, (int)(item % a.log_chunks));
 }
 
-template <int S, bool TGT>
+template <int S, bool TGT, bool ADAPT>
 __device__ __forceinline__ void log_grad_ray(const RenderArgs &a, const int64_t b,
                                              const int chunk) {
     constexpr int P = 1 + 3 * S;
@@ -731,7 +771,6 @@ __device__ __forceinline__ void log_grad_ray(const RenderArgs &a, const int64_t 
     const double dx = fdx, dy = fdy, dz = fdz;
     const double tnear = a.rays.near_t[ray];
     const double step = (double)a.step;
-    const float fstep = a.step;
     const double C0 = a.log_tot[4 * b + 0], C1 = a.log_tot[4 * b + 1],
                  C2 = a.log_tot[4 * b + 2], A = a.log_tot[4 * b + 3];
     RayResid rr;
@@ -767,16 +806,15 @@ __device__ __forceinline__ void log_grad_ray(const RenderArgs &a, const int64_t 
     for (int c0 = c_begin; c0 < c_end; c0 += 64) {
         const int m = min(64, c_end - c0);
         const bool act = lane < m;
-        const size_t at = ((size_t)b * (size_t)a.log_cap + (size_t)(c0 + (act ? lane : 0))) * 8;
-        const float4 v = *reinterpret_cast<const float4 *>(a.log_val + at);
-        const int k = __float_as_int(a.log_val[at + 4]);
+        const size_t row = (size_t)b * (size_t)a.log_cap + (size_t)(c0 + (act ? lane : 0));
+        const size_t at = row * 8;
+        const LoggedSample<ADAPT> v = read_logged<ADAPT>(a, row, act, step, tnear);
+        const int k = v.k;
         const uint4 q0 = *reinterpret_cast<const uint4 *>(a.log_rec + at);
         const uint4 q1 = *reinterpret_cast<const uint4 *>(a.log_rec + at + 4);
         const uint32_t recs[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
-        const float sraw = v.x, col0 = v.y, col1 = v.z, col2 = v.w;
-
-        const float sigma = act ? softplus_shift<float>(sraw) : 0.0f;
-        const float alpha = act ? one_minus_exp_neg<float>(sigma * fstep) : 0.0f;
+        const auto sraw = v.sraw, col0 = v.col0, col1 = v.col1, col2 = v.col2;
+        const auto alpha = v.alpha;
         const double keep = 1.0 - (double)alpha;
         const double incl = wave_scan_mul64(keep, lane);
         double excl = __shfl_up(incl, 1, 64);
@@ -800,8 +838,8 @@ __device__ __forceinline__ void log_grad_ray(const RenderArgs &a, const int64_t 
                 bit |= (uint64_t)__shfl_xor((unsigned long long)bit, off, 64);
             tmask |= bit;
         }
-        // Position of sample k exactly as the marcher computed it.
-        const double t = fma((double)(act ? k : 0), step, tnear);
+        // Position of the sample exactly as the marcher computed it.
+        const double t = v.t;
         const double px = fma(t, dx, ox), py = fma(t, dy, oy), pz = fma(t, dz, oz);
         const int bx = base_of(px), by = base_of(py), bz = base_of(pz);
         const float fx = (float)(px - (double)bx), fy = (float)(py - (double)by),

@@ -372,6 +372,9 @@ __global__ __launch_bounds__(256) void k_render(const RenderArgs a) {
     const int64_t b =
         (int64_t)xcd_slice_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
     if (b >= a.len) return;
+    // Second launch of the adaptive rule's logged step: only the rays the
+    // log could not hold.
+    if (MODE == kFused && a.log_redo && uniform_i(a.log_count[b]) != -1) return;
     const int ray = MODE == kImage ? (int)b : uniform_i(a.sample_index[a.start + b]);
     if (MODE != kImage && (ray < 0 || (int64_t)ray >= a.rays.n)) {
         // Out-of-range sample index: no outputs, zero loss.
@@ -598,7 +601,7 @@ __global__ __launch_bounds__(256) void k_render(const RenderArgs a) {
     // coefficients of the optimised band (RenderUtils.cu:1117-1132).
     const bool active = p >= 0 && (row == 3 || (kk >= a.sh_first && kk < a.sh_active));
     float acc[8];
-    float acc0 = 0.0f;
+    double acc0 = 0.0;
     uint32_t cv[8];
     int cbx = -1, cby = -1, cbz = -1;
 #pragma unroll
@@ -672,7 +675,7 @@ __global__ __launch_bounds__(256) void k_render(const RenderArgs a) {
                     const bool kept = (unsigned)jx < 2u && (unsigned)jy < 2u && (unsigned)jz < 2u;
                     // Record 0 (also every corner outside the cage) is summed
                     // in acc0 and added once per ray.
-                    if (!kept && cv[i] == 0u) acc0 += acc[i];
+                    if (!kept && cv[i] == 0u) acc0 += (double)acc[i];
                     else if (!kept && active)
                         unsafeAtomicAdd(gbase + (size_t)cv[i] * stride, acc[i]);
                 }
@@ -696,14 +699,20 @@ __global__ __launch_bounds__(256) void k_render(const RenderArgs a) {
     if (cbx >= 0) {
 #pragma unroll
         for (int j = 0; j < 8; j++) {
-            if (cv[j] == 0u) acc0 += acc[j];
+            if (cv[j] == 0u) acc0 += (double)acc[j];
             else if (active) unsafeAtomicAdd(gbase + (size_t)cv[j] * stride, acc[j]);
         }
     }
-    if (active && acc0 != 0.0f) unsafeAtomicAdd(gbase, acc0);
+    // Record 0: into the entry's FP64 slot of the logged step, else one
+    // float atomic per ray.
+    if (active && acc0 != 0.0) {
+        if (a.rec0 != nullptr) unsafeAtomicAdd(a.rec0 + (size_t)p * (size_t)a.len + (size_t)b, acc0);
+        else unsafeAtomicAdd(gbase, (float)acc0);
+    }
 }
 
 #include "march8.hpp"
+#include "adapt8.hpp"
 
 // Marcher selection: the eight-samples-per-wave kernel serves the fixed rule
 // for S <= 9 in the forward, backward, fused and trace modes; the one-sample
@@ -714,6 +723,13 @@ bool use_march8() { return option_group_marcher(); }
 template <int S, bool SPARSE, bool ADAPT, int MODE, int STORE>
 int32_t launch3(const RenderArgs &a, hipStream_t st) {
     const unsigned blocks = (unsigned)((a.len + 3) / 4);
+    // The adaptive rule's forward: eight rays per wave.
+    constexpr bool kAdapt8 = ADAPT && S <= 9 && (MODE == kFwd || MODE == kTrace);
+    if (kAdapt8 && use_march8()) {
+        hipLaunchKernelGGL((k_adapt8<(S <= 9 ? S : 9), SPARSE, MODE>),
+                           dim3((unsigned)((a.len + 31) / 32)), dim3(256), 0, st, a);
+        return PXL_OK;
+    }
     constexpr bool kGroup = !ADAPT && S <= 9 && MODE != kTouch;
     if (kGroup && use_march8()) {
         hipLaunchKernelGGL((k_march8<(S <= 9 ? S : 9), SPARSE, (MODE == kTouch ? kFwd : MODE),
@@ -949,7 +965,7 @@ extern "C" int32_t pxl_render_fwd_trace(const pxl_grid *grid, const pxl_rays *ra
 namespace pxl {
 namespace {
 // Workspace of the two-kernel fused step: per batch entry 4 FP64 totals, a
-// count and 32 FP64 record-0 slots, per logged sample 8 records and 8 value
+// count and 32 FP64 record-0 slots, per logged sample 8 records and 16 value
 // words, per 64-sample chunk of an entry's log 5 FP64 prefixes; 256-byte
 // aligned.
 struct LogLayout {
@@ -967,8 +983,8 @@ LogLayout log_layout(int64_t len, int32_t cap) {
     o += align256((size_t)len * sizeof(int32_t));
     L.rec = o;
     o += align256((size_t)len * (size_t)cap * 8 * sizeof(uint32_t));
-    L.val = o;
-    o += align256((size_t)len * (size_t)cap * 8 * sizeof(float));
+    L.val = o;  // 8 words per sample (fixed rule) or 8 FP64 words (adaptive rule)
+    o += align256((size_t)len * (size_t)cap * 16 * sizeof(float));
     L.pre = o;
     o += align256((size_t)len * (size_t)((cap + 63) / 64) * 5 * sizeof(double));
     L.redo = o;
@@ -1089,6 +1105,47 @@ int32_t launch_logged(const RenderArgs &a, int wpb, int gwpb, bool persist, bool
     }
     return PXL_OK;
 }
+// The adaptive rule's logged step: k_adapt8 (forward + log, eight rays per
+// wave, wpb waves per block), the gradient kernels over the FP64 log, the
+// one-sample fused kernel for the rays longer than the log, the record-0
+// fold.  A persistent schedule applies to the gradient kernel only.
+template <int S, bool SPARSE>
+int32_t launch_logged_adapt(const RenderArgs &a, int wpb, int gwpb, bool persist, bool chunked,
+                            hipEvent_t grad_wait, hipStream_t st) {
+    bool wait_ok = true;
+    const int threads = 64 * wpb;
+    const unsigned blocks = (unsigned)((a.len + 8 * wpb - 1) / (8 * wpb));
+    const int gthreads = 64 * gwpb;
+    const unsigned gblocks = (unsigned)((a.len + gwpb - 1) / gwpb);
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, a.work, kRedoCounter + 1);
+    hipLaunchKernelGGL((k_adapt8<S, SPARSE, kLog>), dim3(blocks), dim3(threads), 0, st, a);
+    if (grad_wait && hipStreamWaitEvent(st, grad_wait, 0) != hipSuccess) wait_ok = false;
+    if (persist) {
+        const unsigned gb =
+            std::min(gblocks, resident_blocks(k_log_grad<S, true, false, true>, gthreads));
+        hipLaunchKernelGGL((k_log_grad<S, true, false, true>), dim3(gb), dim3(gthreads), 0, st, a);
+    } else if (chunked) {
+        hipLaunchKernelGGL((k_log_prefix<S, true>), dim3(gblocks), dim3(gthreads), 0, st, a);
+        const int64_t items = a.len * (int64_t)a.log_chunks;
+        hipLaunchKernelGGL((k_log_grad_chunked<S, true>),
+                           dim3((unsigned)((items + gwpb - 1) / gwpb)), dim3(gthreads), 0, st, a,
+                           items);
+    } else {
+        hipLaunchKernelGGL((k_log_grad<S, false, false, true>), dim3(gblocks), dim3(gthreads), 0,
+                           st, a);
+    }
+    RenderArgs redo = a;
+    redo.log_redo = 1;
+    hipLaunchKernelGGL((k_render<S, SPARSE, true, kFused, PXL_STORE_F32>),
+                       dim3((unsigned)((a.len + 3) / 4)), dim3(256), 0, st, redo);
+    hipLaunchKernelGGL(k_rec0_fold, dim3(1 + 3 * S), dim3(256), 0, st, a.rec0, a.len, a.grad);
+    if (!wait_ok) {
+        set_error("render: hipStreamWaitEvent before the gradient kernels failed");
+        return PXL_EHIP;
+    }
+    return PXL_OK;
+}
+
 template <int S>
 int32_t dispatch_logged_s(bool sparse, int store, const RenderArgs &a, int wpb, int gwpb,
                           bool persist, bool chunked, hipEvent_t grad_wait, hipStream_t st) {
@@ -1106,7 +1163,27 @@ int32_t dispatch_logged_s(bool sparse, int store, const RenderArgs &a, int wpb, 
     return PXL_EINVAL;
 }
 int32_t dispatch_logged(int S, bool sparse, int store, const RenderArgs &a, int wpb, int gwpb,
-                        bool persist, bool chunked, hipEvent_t gw, hipStream_t st) {
+                        bool persist, bool chunked, hipEvent_t gw, hipStream_t st,
+                        bool adapt = false) {
+    if (adapt) {
+        if (store != PXL_STORE_F32) {
+            set_error("render: the adaptive marching rule needs FP32 storage");
+            return PXL_EINVAL;
+        }
+#define PXL_ADAPT(SV)                                                                         \
+    case SV:                                                                                  \
+        return sparse ? launch_logged_adapt<SV, true>(a, wpb, gwpb, persist, chunked, gw, st) \
+                      : launch_logged_adapt<SV, false>(a, wpb, gwpb, persist, chunked, gw, st);
+        switch (S) {
+            PXL_ADAPT(1)
+            PXL_ADAPT(4)
+            PXL_ADAPT(9)
+            default:
+                set_error("render: the logged step supports sh_dim 1, 4 and 9");
+                return PXL_EINVAL;
+        }
+#undef PXL_ADAPT
+    }
     switch (S) {
         case 1: return dispatch_logged_s<1>(sparse, store, a, wpb, gwpb, persist, chunked, gw, st);
         case 4: return dispatch_logged_s<4>(sparse, store, a, wpb, gwpb, persist, chunked, gw, st);
@@ -1124,7 +1201,9 @@ namespace pxl {
 // log, then the gradient kernels): the fixed rule with the eight-sample
 // marcher, sh_dim 1, 4 or 9.
 bool logged_step_serves(const pxl_grid *g) {
-    return g && g->march_mode == PXL_MARCH_FIXED && g->sh_dim <= 9 && use_march8();
+    if (!g || g->sh_dim > 9 || !use_march8()) return false;
+    return g->march_mode == PXL_MARCH_FIXED ||
+           (g->march_mode == PXL_MARCH_REF_ADAPTIVE && g->storage == PXL_STORE_F32);
 }
 }  // namespace pxl
 
@@ -1159,9 +1238,9 @@ static int32_t bwd_common(const pxl_grid *grid, const pxl_optim *optim, const px
     a.huber = huber_delta;
     const bool adapt = grid->march_mode == PXL_MARCH_REF_ADAPTIVE;
     const bool group = logged_step_serves(grid);
-    PXL_REQUIRE(!x.bin_mask || group,
+    PXL_REQUIRE(!x.bin_mask || (group && !adapt),
                 "%s: proposal masks need the eight-sample marcher (fixed rule, sh_dim <= 9)", who);
-    PXL_REQUIRE(!x.bin_targets || (group && x.workspace && x.target_valid),
+    PXL_REQUIRE(!x.bin_targets || (group && !adapt && x.workspace && x.target_valid),
                 "%s: proposal targets need the logged step and a validity array", who);
     a.bin_mask = x.bin_mask;
     a.bin_targets = x.bin_targets;
@@ -1199,7 +1278,7 @@ static int32_t bwd_common(const pxl_grid *grid, const pxl_optim *optim, const px
                     "%s: the chunk-parallel gradient has no persistent form", who);
         st = dispatch_logged(grid->sh_dim, grid->sparse, grid->storage, a, wpb, gwpb,
                              sched && sched->persistent, sched && sched->chunked, x.grad_wait,
-                             as_stream(stream));
+                             as_stream(stream), adapt);
         return st != PXL_OK ? st : check_launch(who);
     }
     // One kernel writes the gradient: it starts after grad_wait.
@@ -1429,10 +1508,16 @@ extern "C" const char *pxl_render_kernel_name(const pxl_grid *grid, int32_t mode
     const char *store = grid->storage == PXL_STORE_BF16  ? "bf16"
                         : grid->storage == PXL_STORE_F16 ? "f16"
                                                          : "f32";
-    if (group && mode == 3)
+    if (group && adapt && mode == 3)
+        snprintf(name, sizeof(name), "k_adapt8<%d,%s,log>+k_log_grad<%d,adaptive>",
+                 grid->sh_dim, grid->sparse ? "sparse" : "dense", grid->sh_dim);
+    else if (group && adapt && mode == 0)
+        snprintf(name, sizeof(name), "k_adapt8<%d,%s,fwd>", grid->sh_dim,
+                 grid->sparse ? "sparse" : "dense");
+    else if (group && mode == 3)
         snprintf(name, sizeof(name), "k_march8<%d,%s,log,%s>+k_log_grad<%d>", grid->sh_dim,
                  grid->sparse ? "sparse" : "dense", store, grid->sh_dim);
-    else if (group)
+    else if (group && !adapt)
         snprintf(name, sizeof(name), "k_march8<%d,%s,%s,%s>", grid->sh_dim,
                  grid->sparse ? "sparse" : "dense", modes[mode], store);
     else

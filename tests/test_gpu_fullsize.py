@@ -20,6 +20,8 @@ four 800 x 800 views of the upper hemisphere, random 8-bit targets.
   ray and is held to 1e-3 there;
 - the whole batch's gradient is the sum of its two halves' (no cross-ray
   interference, record 0 included, 1e-4);
+- 128 rays of the batch under the reference's adaptive rule (fused and
+  logged) against the oracle;
 - TV over the full caged grid and one Adam step over all records against
   the oracle.
 """
@@ -180,6 +182,41 @@ def test_c2_sampled_rays(c2, c2_oracle, path):
     assert np.abs(losses - l_ref).max() <= 255 * 1e-5
     worst = grad_close(from_records(optim.grad, P), g_ref)
     say(f"    worst gradient error / tolerance = {worst:.3g}")
+    assert worst <= 1.0
+
+
+@pytest.mark.parametrize("path", ["fused", "logged"])
+def test_c2_sampled_rays_adaptive(c2, c2_oracle, path):
+    """128 rays of the batch under the reference's adaptive marching rule
+    (t += |step - w|) against the oracle on the full 256^3 grid."""
+    import torch
+
+    sub = c2["batch"][np.random.default_rng(16).choice(BATCH, 128, replace=False)]
+    say(f"[c2] 128 rays, adaptive rule, {path}")
+    grid = pxl.Grid(N, N, N, S, c2["rec"], march_mode=pxl.MARCH_REF_ADAPTIVE, **c2["grid_kw"])
+    optim = pxl.Optim(grid)
+    losses = torch.zeros(128, dtype=torch.float64, device=c2["dev"])
+    si = torch.from_numpy(sub).to(c2["dev"])
+    if path == "fused":
+        pxl.train_rays(grid, optim, c2["rays"], losses, si)
+    else:
+        ws = pxl.train_workspace(128, 512, c2["dev"])
+        pxl.train_rays_ws(grid, optim, c2["rays"], losses, si, ws, 512)
+    torch.cuda.synchronize()
+    g_o = c2_oracle["grid"]
+    grid_o = oracle.Grid(N, N, N, S, g_o.planes, 0.5, occupancy=g_o.occupancy,
+                         index_offsets=g_o.index_offsets, adaptive=True)
+    fwd = oracle.render_fwd(grid_o, c2_oracle["rays"], sub)
+    l_ref, g_ref = oracle.render_bwd(grid_o, c2_oracle["rays"], sub, fwd, mask_coef=1.0,
+                                     huber_delta=0.1)
+    r = c2["rays"]
+    err = max(np.abs(getattr(r, k).cpu().numpy()[sub] - fwd[o][sub]).max()
+              for k, o in (("acc_r", "r"), ("acc_g", "g"), ("acc_b", "b"), ("acc_w", "w")))
+    dl = np.abs(losses.cpu().numpy() - l_ref).max()
+    worst = grad_close(from_records(optim.grad, P), g_ref)
+    say(f"    max colour / opacity error {err:.3g}, loss error {dl:.3g}, gradient / tolerance "
+        f"{worst:.3g}")
+    assert err <= 1e-5 and dl <= 255 * 1e-5
     assert worst <= 1.0
 
 
