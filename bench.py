@@ -84,6 +84,14 @@ def parse():
                          "the same consecutive chunks of one seeded shuffle)")
     ap.add_argument("--log-samples", type=int, default=512,
                     help="samples per ray the two-kernel fused step logs (0: one fused kernel)")
+    ap.add_argument("--march", choices=("fixed", "adaptive"), default="fixed",
+                    help="marching rule of the trained grid: fixed step (the benchmark "
+                         "configurations) or the reference's t += |step - w|")
+    ap.add_argument("--fast-path", type=int, default=None,
+                    help="1: after the plain measurement, measure the learned proposal with "
+                         "BF16 records on the same rays (default on for the plain256 workload)")
+    ap.add_argument("--fast-pretrain", type=int, default=1000,
+                    help="fast path: untimed steps with the proposal learning on every batch")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="run one device-to-device copy of this many MiB first (a kernel of "
                          "known traffic for calibrating profiler byte counters)")
@@ -99,6 +107,9 @@ def parse():
         a.log_samples = 1024
     if a.pretrain is None:
         a.pretrain = 2000 if a.proposal else 0
+    if a.fast_path is None:
+        a.fast_path = int(a.workload == "plain256" and not a.proposal and a.storage == "f32"
+                          and a.march == "fixed")
     return a
 
 
@@ -398,6 +409,9 @@ class Workload:
         log(f"rays {self.n_rays} built with targets ({time.time() - t0:.1f}s)")
 
         # Trained grid: same caged structure, InitialDensity 0, InitialColor 0.
+        self.grid_kw = grid_kw
+        if args.march == "adaptive":
+            grid_kw = dict(grid_kw, march_mode=pxl.MARCH_REF_ADAPTIVE)
         self.rec = torch.zeros_like(gt_rec)
         if args.storage == "bf16":
             zero32 = self.rec
@@ -542,19 +556,21 @@ def profiled_traffic(kernel, args):
     return None
 
 
-def proposal_quality(w, prop, dev, args):
+def proposal_quality(w, prop, dev, args, grid=None, fp32_grid=None):
     """Held-out batch (the last of the shuffle, never trained on): sample
     counts and PSNR against the targets of the plain and the pruned render
-    of the current grid."""
+    of the current grid (w.grid, or `grid`); with fp32_grid (the FP32 master
+    records of a 16-bit grid) also the plain FP32 render's PSNR."""
     import torch
 
     import pxl
 
+    grid = w.grid if grid is None else grid
     B = args.batch
     start = (w.n_rays // B - 1) * B
     idx = w.order[start:start + B]
     pws = pxl.proposal_workspace(B, dev)
-    pxl.proposal_features(w.grid, w.rays, w.order, pws, start=start, length=B)
+    pxl.proposal_features(grid, w.rays, w.order, pws, start=start, length=B)
     logits = torch.zeros((B, 64), dtype=torch.float32, device=dev)
     pxl.proposal_forward(prop, B, logits, pws)
     masks = torch.zeros(B, dtype=torch.int64, device=dev)
@@ -565,9 +581,12 @@ def proposal_quality(w, prop, dev, args):
     # (the reference's convention); PSNR is over the rays that enter it.
     hit = w.rays.near_t[idx.long()] <= w.rays.far_t[idx.long()]
     res = {}
-    for name, m in (("plain", None), ("proposal", masks)):
+    runs = [("plain", grid, None), ("proposal", grid, masks)]
+    if fp32_grid is not None:
+        runs.append(("plain_fp32", fp32_grid, None))
+    for name, g, m in runs:
         tr = torch.zeros(2 * B, dtype=torch.int32, device=dev)
-        pxl.render_fwd_masked(w.grid, w.rays, w.order, m, start=start, length=B, trace=tr)
+        pxl.render_fwd_masked(g, w.rays, w.order, m, start=start, length=B, trace=tr)
         c = torch.stack([w.rays.acc_r[idx.long()], w.rays.acc_g[idx.long()],
                          w.rays.acc_b[idx.long()]], 1).double()
         torch.cuda.synchronize()
@@ -576,11 +595,70 @@ def proposal_quality(w, prop, dev, args):
                      "psnr": round(10.0 * math.log10(1.0 / mse), 4)}
     res["pruned_frac"] = round(1.0 - res["proposal"]["samples"] / max(1, res["plain"]["samples"]), 4)
     res["psnr_delta"] = round(res["proposal"]["psnr"] - res["plain"]["psnr"], 4)
+    if fp32_grid is not None:
+        res["psnr_delta_vs_fp32"] = round(res["proposal"]["psnr"] - res["plain_fp32"]["psnr"], 4)
     res["interval"] = args.prop_interval
     res["threshold"] = args.prop_threshold
     res["dilate"] = args.prop_dilate
     log(f"proposal on held-out batch: {res}")
     return res
+
+
+def fast_path(w, dev, args, plain_ms):
+    """The north-star fast path on the plain workload's rays and schedule:
+    BF16 records with an FP32 master copy, the learned proposal learning on
+    every batch for fast_pretrain untimed steps, then every prop_interval-th
+    batch unpruned (learning) and the others pruned; warmup + `steps` timed
+    steps.  Quality on the held-out batch: PSNR of the pruned BF16 render
+    against the unpruned BF16 render and against the FP32 render of the
+    master records (the plain FP32 path's values)."""
+    import torch
+
+    import pxl
+
+    N, S = w.N, w.S
+    zero32 = torch.zeros_like(w.rec)
+    rec16 = pxl.records_to_16bit(zero32, pxl.STORE_BF16)
+    grid = pxl.Grid(N, N, N, S, rec16, storage=pxl.STORE_BF16, **w.grid_kw)
+    optim = pxl.Optim(grid, master_from=zero32)
+    del zero32
+    tr = pxl.Trainer(grid, optim, w.rays, w.order, batch_size=args.batch, learning_rate=0.3,
+                     sh_learning_rate=1.0, variance_coefficient=1e-5, sh_coefficient=1e-6,
+                     variance_type=pxl.TV_DISPARITY, neighbour_nums=6, variance_interval=3,
+                     mask_coefficient=1.0, huber_delta=0.1, fused=True,
+                     log_samples=args.log_samples, autotune=bool(args.autotune))
+    prop = pxl.Proposal(dev, seed=1)
+    tr.set_proposal(prop, interval=1, prune=False)
+    t0 = time.time()
+    for _ in range(args.fast_pretrain):
+        tr.step()
+    torch.cuda.synchronize()
+    log(f"fast path: {args.fast_pretrain} untimed steps with the proposal learning "
+        f"({time.time() - t0:.1f}s), batch PSNR {tr.loss()[1]:.2f}")
+    tr.set_proposal(prop, interval=args.prop_interval, prune=True, threshold=args.prop_threshold,
+                    dilate=args.prop_dilate)
+    for _ in range(args.warmup):
+        tr.step()
+    K = args.steps
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start.record()
+    for _ in range(K):
+        tr.step()
+    end.record()
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(end) / K
+    fp32 = pxl.Grid(N, N, N, S, optim.master, **w.grid_kw)
+    q = proposal_quality(w, prop, dev, args, grid=grid, fp32_grid=fp32)
+    out = {"config": "learned proposal + BF16 records (FP32 master), "
+                     f"{args.fast_pretrain} untimed steps learning first, then every "
+                     f"{args.prop_interval}th batch unpruned",
+           "ms_per_step": round(ms, 4), "value": round(args.batch / (ms / 1e3), 1),
+           "unit": "rays/s", "ratio_vs_plain": round(plain_ms / ms, 4),
+           "held_out": q, "target_ratio": 3.0, "target_psnr_loss_db": 0.1}
+    log(f"fast path: {ms:.3f} ms/step ({plain_ms / ms:.3f}x the plain path), held out {q}")
+    tr.close()
+    return out
 
 
 def copy_bandwidth(dev, mib=1024, reps=5):
@@ -653,8 +731,10 @@ def main():
 
     K = args.steps
     # HIP events around each batch's render launch, recorded by the trainer
-    # on the stream it launches on; batches whose regulariser runs beside the
-    # forward on the trainer's second stream are left out of this mean.
+    # on the stream it launches on.  The roofline uses the mean over every
+    # timed batch (on regulariser batches of a sparse grid the variance loss
+    # runs beside the forward on the trainer's second stream, inside that
+    # window); the mean over the other batches is reported beside it.
     w.trainer.time_render(K)
     start = torch.cuda.Event(enable_timing=True)
     end = torch.cuda.Event(enable_timing=True)
@@ -667,7 +747,7 @@ def main():
     torch.cuda.synchronize()
     wall = time.time() - wall0
     ms = start.elapsed_time(end)
-    kern_ms, nk = w.trainer.render_ms()
+    kern_ms, nk, alone_ms, n_alone = w.trainer.render_ms_split()
     assert 0 < nk <= K
     ms_step = ms / K
     rays_per_s = args.batch * K / (ms / 1e3)
@@ -696,6 +776,8 @@ def main():
             "frac_of_measured_copy": round(achieved / copy_gbs, 5),
             "traffic": profiled_traffic(kname, args),
             "kernel": kname, "kernel_ms": round(kern_ms, 4), "kernel_batches_timed": int(nk),
+            "kernel_ms_without_tv_beside": round(alone_ms, 4) if n_alone else None,
+            "kernel_batches_without_tv_beside": int(n_alone),
             "touched_records": int(vt), "alg_bytes": int(alg),
             "step_alg_bytes": int(step_alg),
             "step_frac": round(step_alg / (ms_step / 1e3) / 1e9 / HBM_PEAK_GBS, 5)}
@@ -736,6 +818,11 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    if args.march == "adaptive":
+        out["metric"] += ", reference adaptive marching rule"
+        out["config"]["march"] = "adaptive: t += |step - w| (RenderUtils.cu:1704)"
+    if args.fast_path:
+        out["fast_path"] = fast_path(w, dev, args, ms_step)
     if prop is not None:
         out["metric"] += (" with learned proposal"
                           + (", BF16 grid" if args.storage == "bf16" else ""))

@@ -7,11 +7,13 @@
 // sample), a skipped one (empty cell) by step.  Each position depends on the
 // previous sample's weight, so one ray's samples are sequential; the work
 // mapping therefore puts eight rays in a wave instead of eight samples of
-// one ray: lane = (ray slot g, corner j), g = lane / 8.  Per iteration each
-// slot places its ray's next position, tests the cell (the occupancy and
-// the corner records are kept while the ray stays in one cell), and for an
-// evaluated sample lane (g, j) loads corner j's record with 16-byte loads
-// and forms its share of the raw density and raw colours; three xor
+// one ray: lane = (ray slot g, corner j), g = lane / 8.  Per iteration the
+// 8 lanes of a slot test the ray's next 8 positions at once (lane j reaches
+// its position from t by j additions of step, the sequence the rule takes
+// while it skips), and the slot moves to the first occupied one, or past
+// all 8.  For an evaluated sample lane (g, j) loads corner j's record with
+// 16-byte loads (the corner records are kept while the ray stays in one
+// cell) and forms its share of the raw density and raw colours; three xor
 // shuffles sum the 8 corners.  Weights, interpolation, activations and the
 // compositing scalars are FP64, as in the one-sample kernel k_render, so
 // the positions follow the FP64 oracle's.
@@ -28,6 +30,75 @@ __device__ __forceinline__ double corner_sum_d(double v) {
     return v;
 }
 
+// Results of a slot's finished batch entry b (lane j == 0 writes): kind 1
+// an out-of-range ray index, 2 a ray that missed the grid box, 0 a marched
+// ray (colour totals, trace, and for the logged step the loss and log count,
+// or the redo list when the ray outgrew the log).
+template <int MODE>
+__device__ __forceinline__ void adapt8_finish(const RenderArgs &a, int64_t b, int ray, int kind,
+                                              double C0, double C1, double C2, double A,
+                                              double sig, int logged, uint32_t hcount,
+                                              uint32_t hash) {
+    const bool logging = MODE == kLog;
+    if (kind != 0) {
+        if (kind == 2) {
+            a.rays.acc_r[ray] = 0.0f;
+            a.rays.acc_g[ray] = 0.0f;
+            a.rays.acc_b[ray] = 0.0f;
+            a.rays.acc_w[ray] = 0.0f;
+            a.rays.acc_density[ray] = 0.0f;
+        }
+        if (MODE == kTrace) {
+            a.trace[2 * b] = 0u;
+            a.trace[2 * b + 1] = 2166136261u;
+        }
+        if (logging) {
+            a.losses[b] = 0.0;
+            a.log_count[b] = 0;
+        }
+        return;
+    }
+    if (a.white_bg) {
+        C0 += 1.0 - A;
+        C1 += 1.0 - A;
+        C2 += 1.0 - A;
+    }
+    a.rays.acc_r[ray] = (float)C0;
+    a.rays.acc_g[ray] = (float)C1;
+    a.rays.acc_b[ray] = (float)C2;
+    a.rays.acc_w[ray] = (float)A;
+    a.rays.acc_density[ray] = (float)sig;
+    if (MODE == kTrace) {
+        a.trace[2 * b] = hcount;
+        a.trace[2 * b + 1] = hash;
+    }
+    if (logging) {
+        if (logged <= a.log_cap) {
+            RayResid rr;
+            const bool ok = ray_residuals(a, ray, C0, C1, C2, A, rr);
+            a.losses[b] = ok ? rr.loss : 255.0;
+            a.log_count[b] = ok ? logged : 0;
+            a.log_tot[4 * b + 0] = C0;
+            a.log_tot[4 * b + 1] = C1;
+            a.log_tot[4 * b + 2] = C2;
+            a.log_tot[4 * b + 3] = A;
+        } else {
+            // Longer than the log: the one-sample fused kernel redoes it.
+            a.log_count[b] = -1;
+            a.redo_list[atomicAdd(a.work + kRedoCounter, 1ull)] = (int32_t)b;
+        }
+    }
+}
+
+// Slots and work: the sample counts of one batch's rays spread widely (at
+// 256^3 a ray evaluates 24 samples on average, the longest of 8 rays 83), so
+// a slot whose ray has ended takes the next batch entry instead of idling
+// until the wave's longest ray is done.  With claim counters (a.work[0..7],
+// zeroed before the launch: the logged step) a wave claims entries for its
+// free slots with one atomic add on the counter of its block's slice of the
+// batch (block b runs on XCD b mod 8), moving to the next slice when one is
+// used up; without them (forward and trace launches) each slot marches the
+// one entry of its position in the grid.
 template <int S, bool SPARSE, int MODE>
 __global__ __launch_bounds__(256) void k_adapt8(const RenderArgs a) {
     constexpr int P = 1 + 3 * S;
@@ -35,101 +106,194 @@ __global__ __launch_bounds__(256) void k_adapt8(const RenderArgs a) {
     const int lane = threadIdx.x & 63;
     const int g = lane >> 3, j = lane & 7;
     const int jx = j & 1, jy = (j >> 1) & 1, jz = j >> 2;
+    const int gbase = 8 * g;
     const int64_t wave =
         (int64_t)xcd_slice_block(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    const int64_t b = wave * 8 + g;
-    bool live = b < a.len;
-    const int ray = live ? a.sample_index[a.start + b] : -1;
-    if (logging && live) {
-        // The entry's record-0 slots start at zero (4 per lane of the slot).
-#pragma unroll
-        for (int q = 0; q < 4; q++) a.rec0[(size_t)(4 * j + q) * (size_t)a.len + (size_t)b] = 0.0;
-    }
-    if (live && (ray < 0 || (int64_t)ray >= a.rays.n)) {
-        if (j == 0) {
-            if (logging) {
-                a.losses[b] = 0.0;
-                a.log_count[b] = 0;
-            }
-            if (MODE == kTrace) {
-                a.trace[2 * b] = 0u;
-                a.trace[2 * b + 1] = 2166136261u;
-            }
-        }
-        live = false;
-    }
-    double ox = 0.0, oy = 0.0, oz = 0.0, dx = 0.0, dy = 0.0, dz = 0.0, tnear = 0.0, tfar = 0.0;
-    float fdx = 0.0f, fdy = 0.0f, fdz = 0.0f;
-    if (live) {
-        ox = a.rays.ox[ray];
-        oy = a.rays.oy[ray];
-        oz = a.rays.oz[ray];
-        fdx = a.rays.dx[ray];
-        fdy = a.rays.dy[ray];
-        fdz = a.rays.dz[ray];
-        dx = fdx;
-        dy = fdy;
-        dz = fdz;
-        tnear = a.rays.near_t[ray];
-        tfar = a.rays.far_t[ray];
-    }
+    const bool claiming = a.work != nullptr;
+    int slice = (int)(blockIdx.x & 7u), visited = 0;
+    bool first = true;
     const double step = (double)a.step;
+
+    // The slot's batch entry and its ray.
+    bool live = false, exhausted = false;
+    int64_t b = 0;
+    int ray = 0, kind = 0;
+    double ox = 0.0, oy = 0.0, oz = 0.0, dx = 0.0, dy = 0.0, dz = 0.0, tnear = 0.0, tfar = 0.0;
     double Y[S];
 #pragma unroll
-    for (int k = 0; k < S; k++) Y[k] = k < a.sh_active ? sh_basis_d(k, fdx, fdy, fdz) : 0.0;
-
-    double t = tnear, T = 1.0, A = 0.0, C0 = 0.0, C1 = 0.0, C2 = 0.0, sig = 0.0;
+    for (int q = 0; q < S; q++) Y[q] = 0.0;
+    double t = 0.0, T = 1.0, A = 0.0, C0 = 0.0, C1 = 0.0, C2 = 0.0, sig = 0.0;
     int k = 0, logged = 0;
     uint32_t hcount = 0, hash = 2166136261u;
-    // The cell of the last position and what it holds: occupied, and this
-    // lane's corner record.
+    // The cell of the last evaluated sample and this lane's corner record
+    // (grid facts: they stay valid from one ray to the next).
     int cbx = INT32_MIN, cby = 0, cbz = 0;
-    bool cocc = false;
     uint32_t crec = 0u;
-    // Last brick found empty (sparse grids), so positions in it skip the
-    // cell lookups.
+    // Per lane: the last brick this lane found empty (sparse grids).
     uint32_t empty_brick = 0xFFFFFFFFu;
     while (true) {
+        if (live && !(t < tfar && k < kMaxSteps)) {
+            if (j == 0)
+                adapt8_finish<MODE>(a, b, ray, kind, C0, C1, C2, A, sig, logged, hcount, hash);
+            live = false;
+        }
+        // Free slots take the next batch entries (rank: the slot's place
+        // among the wave's free slots).
+        const uint64_t freem = __ballot(!live && !exhausted && j == 0);
+        if (freem != 0ull) {
+            const bool is_free = (freem >> gbase) & 1ull;
+            const int rank = __builtin_popcountll(freem & ((1ull << gbase) - 1ull));
+            int64_t nb = -1;
+            if (claiming) {
+                const int nfree = __builtin_popcountll(freem);
+                int done = 0;
+                while (done < nfree && visited < 8) {
+                    const int64_t lo = a.len * slice / 8, hi = a.len * (slice + 1) / 8;
+                    unsigned long long got = 0ull;
+                    if (lane == 0) got = atomicAdd(a.work + slice, (unsigned long long)(nfree - done));
+                    const uint32_t glo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)got);
+                    const uint32_t ghi =
+                        (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(got >> 32));
+                    const int64_t base = lo + (int64_t)(((uint64_t)ghi << 32) | glo);
+                    const int64_t room = hi - base;
+                    const int avail =
+                        room <= 0 ? 0 : (room < (int64_t)(nfree - done) ? (int)room : nfree - done);
+                    if (is_free && rank >= done && rank < done + avail) nb = base + (rank - done);
+                    done += avail;
+                    if (done < nfree) {
+                        slice = (slice + 1) & 7;
+                        visited++;
+                    }
+                }
+            } else if (first && wave * 8 + g < a.len) {
+                nb = wave * 8 + g;
+            }
+            if (is_free) {
+                if (nb < 0) {
+                    exhausted = true;
+                } else {
+                    b = nb;
+                    live = true;
+                    ray = a.sample_index[a.start + b];
+                    if (logging) {
+                        // The entry's record-0 slots start at zero (4 per lane).
+#pragma unroll
+                        for (int q = 0; q < 4; q++)
+                            a.rec0[(size_t)(4 * j + q) * (size_t)a.len + (size_t)b] = 0.0;
+                    }
+                    kind = 0;
+                    tnear = 0.0;
+                    tfar = -1.0;
+                    if (ray < 0 || (int64_t)ray >= a.rays.n) {
+                        kind = 1;
+                    } else {
+                        ox = a.rays.ox[ray];
+                        oy = a.rays.oy[ray];
+                        oz = a.rays.oz[ray];
+                        const float fdx = a.rays.dx[ray], fdy = a.rays.dy[ray],
+                                    fdz = a.rays.dz[ray];
+                        dx = fdx;
+                        dy = fdy;
+                        dz = fdz;
+                        tnear = a.rays.near_t[ray];
+                        tfar = a.rays.far_t[ray];
+                        if (tnear > tfar) kind = 2;
+#pragma unroll
+                        for (int q = 0; q < S; q++)
+                            Y[q] = q < a.sh_active ? sh_basis_d(q, fdx, fdy, fdz) : 0.0;
+                    }
+                    t = tnear;
+                    T = 1.0;
+                    A = C0 = C1 = C2 = sig = 0.0;
+                    k = logged = 0;
+                    hcount = 0u;
+                    hash = 2166136261u;
+                }
+            }
+        }
+        first = false;
+        if (__ballot(live) == 0ull) break;
         const bool act = live && t < tfar && k < kMaxSteps;
-        if (__ballot(act) == 0ull) break;
-        const double px = fma(t, dx, ox), py = fma(t, dy, oy), pz = fma(t, dz, oz);
-        const int bx = base_of(px), by = base_of(py), bz = base_of(pz);
-        if (act && (bx != cbx || by != cby || bz != cbz)) {
-            cbx = bx;
-            cby = by;
-            cbz = bz;
-            cocc = true;
-            if (SPARSE) {
-                const int cx = clampi(bx, 0, a.W - 2), cy = clampi(by, 0, a.H - 2),
-                          cz = clampi(bz, 0, a.D - 2);
+        if (__ballot(act) == 0ull) continue;
+        // Each active slot finds its next occupied position (or the end of
+        // its ray) before any slot evaluates, so the slots evaluate their
+        // samples together.  Probe: lane (g, j) tests the ray's j-th next
+        // position, reached from t by j additions of step, exactly the
+        // sequence the rule produces while it skips; tn = one more step.
+        bool searching = act, eval = false;
+        double t_at = t;
+        int sbx = 0, sby = 0, sbz = 0;
+        while (__ballot(searching) != 0ull) {
+            double tj = t;
+#pragma unroll
+            for (int q = 0; q < 7; q++)
+                if (q < j) tj += step;
+            const double tn = tj + step;
+            const bool valid = searching && tj < tfar && k + j < kMaxSteps;
+            const double qx = fma(tj, dx, ox), qy = fma(tj, dy, oy), qz = fma(tj, dz, oz);
+            const int qbx = base_of(qx), qby = base_of(qy), qbz = base_of(qz);
+            bool occ = valid;
+            if (SPARSE && valid) {
+                const int cx = clampi(qbx, 0, a.W - 2), cy = clampi(qby, 0, a.H - 2),
+                          cz = clampi(qbz, 0, a.D - 2);
                 if (a.bricks != nullptr) {
                     const uint32_t br = (uint32_t)(cz >> 3) * (uint32_t)a.nbx * (uint32_t)a.nby +
                                         (uint32_t)(cy >> 3) * (uint32_t)a.nbx + (uint32_t)(cx >> 3);
                     if (br == empty_brick) {
-                        cocc = false;
+                        occ = false;
                     } else {
-                        cocc = (a.bricks[br >> 5] >> (br & 31)) & 1u;
-                        if (!cocc) empty_brick = br;
+                        occ = (a.bricks[br >> 5] >> (br & 31)) & 1u;
+                        if (!occ) empty_brick = br;
                     }
                 }
-                if (cocc) {
+                if (occ) {
                     const uint32_t cell = (uint32_t)cz * (uint32_t)(a.W - 1) * (uint32_t)(a.H - 1) +
                                           (uint32_t)cy * (uint32_t)(a.W - 1) + (uint32_t)cx;
-                    cocc = (a.occ[cell >> 5] >> (cell & 31)) & 1u;
+                    occ = (a.occ[cell >> 5] >> (cell & 31)) & 1u;
                 }
             }
-            if (cocc) {
-                crec = scene_index(bx + jx, by + jy, bz + jz, a.W, a.H, a.D);
-                if (SPARSE) crec = min(a.io[crec], (uint32_t)(a.n_vox - 1));
+            const uint32_t omask = (uint32_t)(__ballot(occ) >> gbase) & 0xFFu;
+            const uint32_t vmask = (uint32_t)(__ballot(valid) >> gbase) & 0xFFu;
+            // First occupied position of the slot (8: none), the valid ones
+            // (a prefix: positions grow).
+            const int f = omask ? __builtin_ctz(omask) : 8;
+            const int nvalid = __builtin_popcount(vmask);
+            const int src = gbase + (f < 8 ? f : (nvalid < 8 ? nvalid : 7));
+            const double tsrc = __shfl(f < 8 || nvalid < 8 ? tj : tn, src, 64);
+            const int xb = __shfl(qbx, src, 64), yb = __shfl(qby, src, 64), zb = __shfl(qbz, src, 64);
+            if (searching) {
+                if (f < 8) {
+                    // The sample: f skipped positions before it.
+                    eval = true;
+                    searching = false;
+                    t_at = tsrc;
+                    sbx = xb;
+                    sby = yb;
+                    sbz = zb;
+                    k += f;
+                } else {
+                    // Every valid position skipped: on past them.
+                    t = tsrc;
+                    k += nvalid;
+                    searching = t < tfar && k < kMaxSteps;
+                }
             }
         }
-        const bool eval = act && cocc;
-        // This lane's corner share (zero for lanes without a sample).
+        if (__ballot(eval) == 0ull) continue;
+        if (eval && (sbx != cbx || sby != cby || sbz != cbz)) {
+            cbx = sbx;
+            cby = sby;
+            cbz = sbz;
+            crec = scene_index(sbx + jx, sby + jy, sbz + jz, a.W, a.H, a.D);
+            if (SPARSE) crec = min(a.io[crec], (uint32_t)(a.n_vox - 1));
+        }
+        // This lane's corner share of the sample (zero for lanes without one).
         double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
         if (eval) {
             float th[P];
             load_record<P, PXL_STORE_F32>(a.params, crec, a.stride, th);
-            const double fx = px - (double)bx, fy = py - (double)by, fz = pz - (double)bz;
+            const double px = fma(t_at, dx, ox), py = fma(t_at, dy, oy), pz = fma(t_at, dz, oz);
+            const double fx = px - (double)sbx, fy = py - (double)sby, fz = pz - (double)sbz;
             const double w = (jx ? fx : 1.0 - fx) * (jy ? fy : 1.0 - fy) * (jz ? fz : 1.0 - fz);
             double c[3];
 #pragma unroll
@@ -146,24 +310,35 @@ __global__ __launch_bounds__(256) void k_adapt8(const RenderArgs a) {
         }
         const double sraw = corner_sum_d(p0);
         const double cr0 = corner_sum_d(p1), cr1 = corner_sum_d(p2), cr2 = corner_sum_d(p3);
+        // Activations spread over the slot's lanes, one exp each: lane 0 the
+        // density, softplus(sraw - 13) = [x > 30: x] + log1p(exp(+-x)), lanes
+        // 1-3 the colours, 1 / (1 + exp(-0.1 raw)) (the same operations as
+        // softplus_shift<double> and sigmoid01<double>).
+        const double xs = sraw - 13.0;
+        const double craw = j == 1 ? cr0 : (j == 2 ? cr1 : cr2);
+        const double e = exp(j == 0 ? (xs > 30.0 ? -xs : xs) : -0.1 * craw);
+        const double act_v = j == 0 ? (xs > 30.0 ? xs + log1p(e) : log1p(e)) : 1.0 / (1.0 + e);
+        const double sigma_s = __shfl(act_v, gbase, 64);
+        const double col0_s = __shfl(act_v, gbase + 1, 64), col1_s = __shfl(act_v, gbase + 2, 64),
+                     col2_s = __shfl(act_v, gbase + 3, 64);
         if (eval) {
             if (MODE == kTrace) {
                 hcount++;
                 hash = fnv_mix(hash, (uint32_t)k);
-                hash = fnv_mix(hash, (uint32_t)bx);
-                hash = fnv_mix(hash, (uint32_t)by);
-                hash = fnv_mix(hash, (uint32_t)bz);
+                hash = fnv_mix(hash, (uint32_t)sbx);
+                hash = fnv_mix(hash, (uint32_t)sby);
+                hash = fnv_mix(hash, (uint32_t)sbz);
             }
-            const double sigma = softplus_shift<double>(sraw);
-            const double col0 = sigmoid01<double>(cr0), col1 = sigmoid01<double>(cr1),
-                         col2 = sigmoid01<double>(cr2);
+            const double sigma = sigma_s;
+            const double col0 = col0_s, col1 = col1_s, col2 = col2_s;
             const double alpha = one_minus_exp_neg<double>(sigma * step);
             const double w = T * alpha;
             if (logging && logged < a.log_cap) {
                 const size_t row = (size_t)b * (size_t)a.log_cap + (size_t)logged;
                 a.log_rec[row * 8 + j] = crec;
                 if (j < 5) {
-                    const double v = j == 0 ? sraw : (j == 1 ? cr0 : (j == 2 ? cr1 : (j == 3 ? cr2 : t)));
+                    const double v =
+                        j == 0 ? sraw : (j == 1 ? cr0 : (j == 2 ? cr1 : (j == 3 ? cr2 : t_at)));
                     reinterpret_cast<double *>(a.log_val)[row * 8 + j] = v;
                 }
             }
@@ -174,62 +349,8 @@ __global__ __launch_bounds__(256) void k_adapt8(const RenderArgs a) {
             C2 += w * col2;
             sig += sigma;
             T = T * (1.0 - alpha);
-            t += fabs(step - w);
-        } else if (act) {
-            t += step;
-        }
-        if (act) ++k;
-    }
-    if (!live) return;
-    if (tnear > tfar) {
-        // Missed the grid box: nothing marched (the loop did not run).
-        if (j == 0) {
-            a.rays.acc_r[ray] = 0.0f;
-            a.rays.acc_g[ray] = 0.0f;
-            a.rays.acc_b[ray] = 0.0f;
-            a.rays.acc_w[ray] = 0.0f;
-            a.rays.acc_density[ray] = 0.0f;
-            if (MODE == kTrace) {
-                a.trace[2 * b] = 0u;
-                a.trace[2 * b + 1] = 2166136261u;
-            }
-            if (logging) {
-                a.losses[b] = 0.0;
-                a.log_count[b] = 0;
-            }
-        }
-        return;
-    }
-    if (a.white_bg) {
-        C0 += 1.0 - A;
-        C1 += 1.0 - A;
-        C2 += 1.0 - A;
-    }
-    if (j == 0) {
-        a.rays.acc_r[ray] = (float)C0;
-        a.rays.acc_g[ray] = (float)C1;
-        a.rays.acc_b[ray] = (float)C2;
-        a.rays.acc_w[ray] = (float)A;
-        a.rays.acc_density[ray] = (float)sig;
-        if (MODE == kTrace) {
-            a.trace[2 * b] = hcount;
-            a.trace[2 * b + 1] = hash;
-        }
-    }
-    if (logging && j == 0) {
-        if (logged <= a.log_cap) {
-            RayResid rr;
-            const bool ok = ray_residuals(a, ray, C0, C1, C2, A, rr);
-            a.losses[b] = ok ? rr.loss : 255.0;
-            a.log_count[b] = ok ? logged : 0;
-            a.log_tot[4 * b + 0] = C0;
-            a.log_tot[4 * b + 1] = C1;
-            a.log_tot[4 * b + 2] = C2;
-            a.log_tot[4 * b + 3] = A;
-        } else {
-            // Longer than the log: the one-sample fused kernel redoes it.
-            a.log_count[b] = -1;
-            a.redo_list[atomicAdd(a.work + kRedoCounter, 1ull)] = (int32_t)b;
+            t = t_at + fabs(step - w);
+            ++k;
         }
     }
 }

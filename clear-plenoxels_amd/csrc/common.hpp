@@ -23,6 +23,7 @@ int32_t check_launch(const char *what);
 // rule (else one sample per wave), and the TV gathers' 64-bit address form.
 bool option_group_marcher();
 bool option_tv_addr64();
+int64_t option_claim_grid();
 
 // Extra arguments of the train entry points (pxl_train_rays_ws / _sched /
 // _prop), and the event the gradient kernels wait for (the trainer runs the

@@ -22,9 +22,11 @@ void clear_error() { g_err[0] = '\0'; }
 // Library options (pxl_set_option): process-wide, read by each launch.
 static std::atomic<int> g_marcher{PXL_MARCHER_GROUP};
 static std::atomic<int> g_tv_addr64{0};
+static std::atomic<int64_t> g_claim_grid{0};
 
 bool option_group_marcher() { return g_marcher.load(std::memory_order_relaxed) == PXL_MARCHER_GROUP; }
 bool option_tv_addr64() { return g_tv_addr64.load(std::memory_order_relaxed) != 0; }
+int64_t option_claim_grid() { return g_claim_grid.load(std::memory_order_relaxed); }
 
 int32_t check_launch(const char *what) {
     const hipError_t e = hipGetLastError();
@@ -64,6 +66,12 @@ extern "C" int32_t pxl_set_option(int32_t key, int64_t value) {
                         (long long)value);
             pxl::g_tv_addr64.store((int)value);
             return PXL_OK;
+        case PXL_OPT_CLAIM_GRID:
+            PXL_REQUIRE(value >= 0 && value <= (int64_t)1 << 30,
+                        "pxl_set_option: claim grid %lld (0: resident, else 1 .. 2^30 blocks)",
+                        (long long)value);
+            pxl::g_claim_grid.store(value);
+            return PXL_OK;
         default:
             pxl::set_error("pxl_set_option: unknown option %d", key);
             return PXL_EINVAL;
@@ -76,6 +84,7 @@ extern "C" int32_t pxl_get_option(int32_t key, int64_t *value) {
     switch (key) {
         case PXL_OPT_MARCHER: *value = pxl::g_marcher.load(); return PXL_OK;
         case PXL_OPT_TV_ADDR64: *value = pxl::g_tv_addr64.load(); return PXL_OK;
+        case PXL_OPT_CLAIM_GRID: *value = pxl::g_claim_grid.load(); return PXL_OK;
         default:
             pxl::set_error("pxl_get_option: unknown option %d", key);
             return PXL_EINVAL;

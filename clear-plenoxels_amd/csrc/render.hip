@@ -1027,7 +1027,7 @@ __global__ __launch_bounds__(256) void k_rec0_fold(const double *__restrict__ re
 }
 
 // Blocks of `threads` threads of `kernel` that fit on the whole device at
-// once (the grid of a persistent launch).
+// once (the grid of a persistent launch), or PXL_OPT_CLAIM_GRID if smaller.
 template <typename K>
 unsigned resident_blocks(K kernel, int threads) {
     int dev = 0, cus = 0, per_cu = 0;
@@ -1035,8 +1035,10 @@ unsigned resident_blocks(K kernel, int threads) {
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
         hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess ||
         cus <= 0 || per_cu <= 0)
-        return 1024;
-    return (unsigned)cus * (unsigned)per_cu;
+        cus = 1024, per_cu = 1;
+    const unsigned resident = (unsigned)cus * (unsigned)per_cu;
+    const int64_t cap = option_claim_grid();
+    return cap > 0 && cap < (int64_t)resident ? (unsigned)cap : resident;
 }
 
 // Forward + log, gradient of the logged rays, then the fused marcher again
@@ -1114,7 +1116,10 @@ int32_t launch_logged_adapt(const RenderArgs &a, int wpb, int gwpb, bool persist
                             hipEvent_t grad_wait, hipStream_t st) {
     bool wait_ok = true;
     const int threads = 64 * wpb;
-    const unsigned blocks = (unsigned)((a.len + 8 * wpb - 1) / (8 * wpb));
+    // The forward's slots claim batch entries (a.work[0..7]): a resident grid.
+    const unsigned blocks =
+        std::min((unsigned)((a.len + 8 * wpb - 1) / (8 * wpb)),
+                 resident_blocks(k_adapt8<S, SPARSE, kLog>, threads));
     const int gthreads = 64 * gwpb;
     const unsigned gblocks = (unsigned)((a.len + gwpb - 1) / gwpb);
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, a.work, kRedoCounter + 1);

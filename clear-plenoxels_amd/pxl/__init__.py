@@ -19,7 +19,7 @@ LIB_PATH = os.path.join(PKG_DIR, "libpxl.so")
 PXL_OK, PXL_EINVAL, PXL_ENOMEM, PXL_EHIP, PXL_ENAN = 0, 1, 2, 3, 4
 MARCH_FIXED, MARCH_REF_ADAPTIVE = 0, 1
 TV_NEIGHBOUR, TV_DISPARITY, TV_BOTH = 0, 1, 2
-OPT_MARCHER, OPT_TV_ADDR64 = 1, 2
+OPT_MARCHER, OPT_TV_ADDR64, OPT_CLAIM_GRID = 1, 2, 3
 MARCHER_GROUP, MARCHER_WAVE = 0, 1
 STORE_F32, STORE_BF16, STORE_F16 = 0, 1, 2
 
@@ -703,7 +703,8 @@ def add_variance_loss(grid, optim, lambda_density, lambda_sh, tv_type=TV_DISPARI
 
 def set_option(key, value):
     """Library option (OPT_MARCHER: MARCHER_GROUP / MARCHER_WAVE;
-    OPT_TV_ADDR64: 0 / 1)."""
+    OPT_TV_ADDR64: 0 / 1; OPT_CLAIM_GRID: 0 resident, else the largest grid
+    of the launches whose waves claim batch entries)."""
     _check("pxl_set_option", lib().pxl_set_option(int(key), int(value)))
 
 

@@ -71,7 +71,12 @@ enum {
     PXL_OPT_MARCHER = 1,
     /* 1: the TV gathers always take their 64-bit address form (otherwise
      * only for grids beyond 2^24 records or 4 GiB); default 0. */
-    PXL_OPT_TV_ADDR64 = 2
+    PXL_OPT_TV_ADDR64 = 2,
+    /* Largest grid (blocks) of the launches whose waves claim batch entries
+     * (the persistent schedules and the adaptive rule's logged forward);
+     * 0 (default): what is resident on the device.  A small cap makes every
+     * claim path run at small batch sizes; results do not depend on it. */
+    PXL_OPT_CLAIM_GRID = 3
 };
 enum { PXL_MARCHER_GROUP = 0, PXL_MARCHER_WAVE = 1 };
 

@@ -14,7 +14,9 @@ whose corners exceed 12), 8 views of 64 x 64 on a radius-4 sphere, random
   the redo launch), its chunk-parallel and persistent gradient schedules;
   clipped (Huber delta 0.1) and unclipped (10) residuals; losses within
   255 * 1e-5, gradients helpers.grad_close (record 0 of the caged grid 1e-4
-  relative on the logged paths, 1e-3 on the single fused kernel).
+  relative on the logged paths, 1e-3 on the single fused kernel);
+- the logged forward with its claim grid capped at 3 blocks, so that slots
+  march many batch entries each and claims cross every slice.
 """
 import sys
 
@@ -122,6 +124,45 @@ def test_adaptive_backward(sc, kind, path, huber):
     rel0 = 1e-3 if path == "fused" else 1e-4
     worst = grad_close_alias(from_records(optim.grad, grid.P), g_ref, rel0=rel0)
     say(f"[adaptive] {kind} {path} huber {huber}: loss error {dl:.3g}, gradient / tolerance "
+        f"{worst:.3g}")
+    assert dl <= 255 * 1e-5
+    assert worst <= 1.0
+
+
+@pytest.mark.parametrize("kind", ["dense", "caged"])
+@pytest.mark.parametrize("path", ["logged", "logged_24", "persistent"])
+def test_adaptive_backward_small_claim_grid(sc, kind, path):
+    """The logged forward's slots claim batch entries: with the claim grid
+    capped at 3 blocks (12 or 3 waves for 1024 entries) every slot marches
+    many entries and the waves move across all 8 slices of the batch; the
+    result is the oracle's as with the resident grid."""
+    import torch
+
+    dev = sc["dev"]
+    grid, go = _grids(sc, kind)
+    optim = pxl.Optim(grid)
+    rays = pxl.Rays.from_arrays(sc["rays_np"], device=dev)
+    losses = torch.zeros(NR, dtype=torch.float64, device=dev)
+    cap = 24 if path == "logged_24" else 512
+    ws = pxl.train_workspace(NR, cap, dev)
+    ws.fill_(0xAB)
+    pxl.set_option(pxl.OPT_CLAIM_GRID, 3)
+    try:
+        pxl.train_rays_sched(grid, optim, rays, losses, sc["si"], ws, cap,
+                             waves_per_block=1 if path == "persistent" else 4,
+                             persistent=path == "persistent", huber_delta=0.1)
+        torch.cuda.synchronize()
+    finally:
+        pxl.set_option(pxl.OPT_CLAIM_GRID, 0)
+    off = 256 + (NR * 32 + 255) // 256 * 256
+    cnt = ws[off:off + 4 * NR].view(torch.int32).cpu().numpy()
+    assert int((cnt > 0).sum()) > 0 and (int((cnt == -1).sum()) > 0) == (cap == 24)
+    ro = oracle.Rays(sc["rays_np"])
+    fwd = oracle.render_fwd(go, ro, sc["batch"])
+    l_ref, g_ref = oracle.render_bwd(go, ro, sc["batch"], fwd, 1.0, 0.1)
+    dl = np.abs(losses.cpu().numpy() - l_ref).max()
+    worst = grad_close_alias(from_records(optim.grad, grid.P), g_ref, rel0=1e-4)
+    say(f"[adaptive] {kind} {path}, claim grid 3: loss error {dl:.3g}, gradient / tolerance "
         f"{worst:.3g}")
     assert dl <= 255 * 1e-5
     assert worst <= 1.0

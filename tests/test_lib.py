@@ -95,6 +95,13 @@ def test_render_kernel_name(pxl_lib):
         pxl_lib.set_option(pxl_lib.OPT_MARCHER, pxl_lib.MARCHER_GROUP)
     assert L.pxl_set_option(pxl_lib.OPT_MARCHER, 7) == pxl_lib.PXL_EINVAL
     assert L.pxl_set_option(99, 0) == pxl_lib.PXL_EINVAL
+    assert pxl_lib.get_option(pxl_lib.OPT_CLAIM_GRID) == 0
+    assert L.pxl_set_option(pxl_lib.OPT_CLAIM_GRID, -1) == pxl_lib.PXL_EINVAL
+    pxl_lib.set_option(pxl_lib.OPT_CLAIM_GRID, 3)
+    try:
+        assert pxl_lib.get_option(pxl_lib.OPT_CLAIM_GRID) == 3
+    finally:
+        pxl_lib.set_option(pxl_lib.OPT_CLAIM_GRID, 0)
     assert (L.pxl_render_kernel_name(ctypes.byref(g), 3) ==
             b"k_march8<9,dense,log,f32>+k_log_grad<9>")
     assert L.pxl_render_kernel_name(ctypes.byref(g), 4) is None
