@@ -11,10 +11,11 @@
 // 8 lanes of a slot test the ray's next 8 positions at once (lane j reaches
 // its position from t by j additions of step, the sequence the rule takes
 // while it skips), and the slot moves to the first occupied one, or past
-// all 8.  For an evaluated sample lane (g, j) loads corner j's record with
-// 16-byte loads (the corner records are kept while the ray stays in one
-// cell) and forms its share of the raw density and raw colours; three xor
-// shuffles sum the 8 corners.  Weights, interpolation, activations and the
+// all 8; positions in the cell of the ray's last sample need no occupancy
+// lookup.  When a sample enters a new cell, lane (g, j) loads corner j's
+// record with 16-byte loads and keeps its density and its three colour sums
+// sum_k Y_k theta_k while the ray stays in the cell; each sample then weighs
+// them trilinearly and three xor shuffles sum the 8 corners.  Weights, interpolation, activations and the
 // compositing scalars are FP64, as in the one-sample kernel k_render, so
 // the positions follow the FP64 oracle's.
 //
@@ -125,10 +126,12 @@ __global__ __launch_bounds__(256) void k_adapt8(const RenderArgs a) {
     double t = 0.0, T = 1.0, A = 0.0, C0 = 0.0, C1 = 0.0, C2 = 0.0, sig = 0.0;
     int k = 0, logged = 0;
     uint32_t hcount = 0, hash = 2166136261u;
-    // The cell of the last evaluated sample and this lane's corner record
-    // (grid facts: they stay valid from one ray to the next).
+    // The cell of the ray's last evaluated sample, this lane's corner record
+    // and its density and three SH colour sums (the colour sums depend on
+    // the ray's direction: a new ray starts with no cell).
     int cbx = INT32_MIN, cby = 0, cbz = 0;
     uint32_t crec = 0u;
+    double q0 = 0.0, q1 = 0.0, q2 = 0.0, q3 = 0.0;
     // Per lane: the last brick this lane found empty (sparse grids).
     uint32_t empty_brick = 0xFFFFFFFFu;
     while (true) {
@@ -202,6 +205,7 @@ __global__ __launch_bounds__(256) void k_adapt8(const RenderArgs a) {
                         for (int q = 0; q < S; q++)
                             Y[q] = q < a.sh_active ? sh_basis_d(q, fdx, fdy, fdz) : 0.0;
                     }
+                    cbx = INT32_MIN;
                     t = tnear;
                     T = 1.0;
                     A = C0 = C1 = C2 = sig = 0.0;
@@ -232,24 +236,33 @@ __global__ __launch_bounds__(256) void k_adapt8(const RenderArgs a) {
             const bool valid = searching && tj < tfar && k + j < kMaxSteps;
             const double qx = fma(tj, dx, ox), qy = fma(tj, dy, oy), qz = fma(tj, dz, oz);
             const int qbx = base_of(qx), qby = base_of(qy), qbz = base_of(qz);
-            bool occ = valid;
-            if (SPARSE && valid) {
+            // A position in the cell of the ray's last sample is occupied;
+            // only the positions in front of the first such one look the
+            // occupancy up, the brick word and the cell word together.
+            const bool incell = valid && qbx == cbx && qby == cby && qbz == cbz;
+            const uint32_t inmask = (uint32_t)(__ballot(incell) >> gbase) & 0xFFu;
+            const int fin = inmask ? __builtin_ctz(inmask) : 8;
+            bool occ = valid && j <= fin;
+            if (SPARSE && valid && j < fin) {
                 const int cx = clampi(qbx, 0, a.W - 2), cy = clampi(qby, 0, a.H - 2),
                           cz = clampi(qbz, 0, a.D - 2);
+                const uint32_t cell = (uint32_t)cz * (uint32_t)(a.W - 1) * (uint32_t)(a.H - 1) +
+                                      (uint32_t)cy * (uint32_t)(a.W - 1) + (uint32_t)cx;
+                uint32_t br = 0u;
+                bool skip = false;
                 if (a.bricks != nullptr) {
-                    const uint32_t br = (uint32_t)(cz >> 3) * (uint32_t)a.nbx * (uint32_t)a.nby +
-                                        (uint32_t)(cy >> 3) * (uint32_t)a.nbx + (uint32_t)(cx >> 3);
-                    if (br == empty_brick) {
-                        occ = false;
-                    } else {
-                        occ = (a.bricks[br >> 5] >> (br & 31)) & 1u;
-                        if (!occ) empty_brick = br;
-                    }
+                    br = (uint32_t)(cz >> 3) * (uint32_t)a.nbx * (uint32_t)a.nby +
+                         (uint32_t)(cy >> 3) * (uint32_t)a.nbx + (uint32_t)(cx >> 3);
+                    skip = br == empty_brick;
                 }
-                if (occ) {
-                    const uint32_t cell = (uint32_t)cz * (uint32_t)(a.W - 1) * (uint32_t)(a.H - 1) +
-                                          (uint32_t)cy * (uint32_t)(a.W - 1) + (uint32_t)cx;
-                    occ = (a.occ[cell >> 5] >> (cell & 31)) & 1u;
+                if (skip) {
+                    occ = false;
+                } else {
+                    const uint32_t ow = a.occ[cell >> 5];
+                    const uint32_t bw = a.bricks != nullptr ? a.bricks[br >> 5] : 0xFFFFFFFFu;
+                    const bool bocc = (bw >> (br & 31)) & 1u;
+                    if (!bocc) empty_brick = br;
+                    occ = bocc && ((ow >> (cell & 31)) & 1u);
                 }
             }
             const uint32_t omask = (uint32_t)(__ballot(occ) >> gbase) & 0xFFu;
@@ -281,20 +294,15 @@ __global__ __launch_bounds__(256) void k_adapt8(const RenderArgs a) {
         }
         if (__ballot(eval) == 0ull) continue;
         if (eval && (sbx != cbx || sby != cby || sbz != cbz)) {
+            // A new cell: this lane's corner record, its density and its
+            // three colour sums sum_k Y_k theta_k.
             cbx = sbx;
             cby = sby;
             cbz = sbz;
             crec = scene_index(sbx + jx, sby + jy, sbz + jz, a.W, a.H, a.D);
             if (SPARSE) crec = min(a.io[crec], (uint32_t)(a.n_vox - 1));
-        }
-        // This lane's corner share of the sample (zero for lanes without one).
-        double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
-        if (eval) {
             float th[P];
             load_record<P, PXL_STORE_F32>(a.params, crec, a.stride, th);
-            const double px = fma(t_at, dx, ox), py = fma(t_at, dy, oy), pz = fma(t_at, dz, oz);
-            const double fx = px - (double)sbx, fy = py - (double)sby, fz = pz - (double)sbz;
-            const double w = (jx ? fx : 1.0 - fx) * (jy ? fy : 1.0 - fy) * (jz ? fz : 1.0 - fz);
             double c[3];
 #pragma unroll
             for (int ch = 0; ch < 3; ch++) {
@@ -303,10 +311,21 @@ __global__ __launch_bounds__(256) void k_adapt8(const RenderArgs a) {
                 for (int q = 0; q < S; q++) v = fma(Y[q], (double)th[1 + ch * S + q], v);
                 c[ch] = v;
             }
-            p0 = w * (double)th[0];
-            p1 = w * c[0];
-            p2 = w * c[1];
-            p3 = w * c[2];
+            q0 = (double)th[0];
+            q1 = c[0];
+            q2 = c[1];
+            q3 = c[2];
+        }
+        // This lane's corner share of the sample (zero for lanes without one).
+        double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
+        if (eval) {
+            const double px = fma(t_at, dx, ox), py = fma(t_at, dy, oy), pz = fma(t_at, dz, oz);
+            const double fx = px - (double)sbx, fy = py - (double)sby, fz = pz - (double)sbz;
+            const double w = (jx ? fx : 1.0 - fx) * (jy ? fy : 1.0 - fy) * (jz ? fz : 1.0 - fz);
+            p0 = w * q0;
+            p1 = w * q1;
+            p2 = w * q2;
+            p3 = w * q3;
         }
         const double sraw = corner_sum_d(p0);
         const double cr0 = corner_sum_d(p1), cr1 = corner_sum_d(p2), cr2 = corner_sum_d(p3);
