@@ -22,6 +22,41 @@ struct TvArgs {
     float lam_d, lam_sh;
 };
 
+// The Adam step fused into a TV pass (pxl_tv_adam_step): the records the
+// TV reads (TvArgs::params) stay as they are and the updated ones go to
+// `out` (same layout), so no lane reads a neighbour another lane has
+// already stepped.  Optimiser records (grad, m, v, master) hold ostride
+// floats; q4 = 4 ceil(P / 4), the elements the Adam kernel visits.
+struct AdamArgs {
+    void *out;
+    float *m, *v, *master;
+    const float *alr;
+    int ostride, q4;
+    double lr, sh_lr, bc1, bc2;
+};
+
+// updateRGBSceneGPU_ADAM's update of element p (Optimizer.cu:380-412), the
+// float/double mix expression by expression: false (nothing changes) for an
+// SH coefficient outside the optimised band [sh_lo, sh_hi).
+__device__ __forceinline__ bool adam_elem(int p, int S, int sh_lo, int sh_hi, double lr_d,
+                                          double sh_lr, double bc1, double bc2, float &th,
+                                          float &g, float &m, float &v) {
+    if (p > 0 && ((p - 1) % S < sh_lo || (p - 1) % S >= sh_hi)) return false;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1.e-6f;
+    const float gg = g;
+    const float mm = (float)((double)(b1 * m) + (1.0 - (double)b1) * (double)gg);
+    const float vv = (float)((double)(b2 * v) + (1.0 - (double)b2) * (double)gg * (double)gg);
+    const float cm = (float)((double)mm / bc1);
+    const float cv = (float)((double)vv / bc2);
+    const double upd = (p == 0 ? lr_d : sh_lr) * (double)cm / (double)(sqrtf(cv) + eps);
+    const double nt = (double)th - upd;
+    th = p == 0 ? (nt < 0.0 ? 0.0f : (float)nt) : (float)nt;
+    m = mm;
+    v = vv;
+    g = 0.0f;
+    return true;
+}
+
 // Coefficient of parameter p: the density's, or the SH one for the
 // coefficients of the band being optimised (startSHIdx .. (optimizingBand +
 // 1)^2, RenderUtils.cu:2480-2493, 2594), else 0.
@@ -37,6 +72,91 @@ __device__ __forceinline__ float ld(const float *base, size_t idx) {
     const uint16_t h = reinterpret_cast<const uint16_t *>(base)[idx];
     if (STORE == PXL_STORE_BF16) return __uint_as_float((uint32_t)h << 16);
     return __half2float(__ushort_as_half(h));
+}
+
+__device__ __forceinline__ uint16_t to_bf16(float f) {
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+template <int STORE>
+__device__ __forceinline__ float widen16(uint16_t h) {
+    if (STORE == PXL_STORE_BF16) return __uint_as_float((uint32_t)h << 16);
+    return __half2float(__ushort_as_half(h));
+}
+template <int STORE>
+__device__ __forceinline__ uint16_t narrow16(float v) {
+    if (STORE == PXL_STORE_BF16) return to_bf16(v);
+    return __half_as_ushort(__float2half_rn(v));
+}
+
+// Streaming store (non-temporal hint) of the Adam moments: no kernel reads
+// them before the next Adam step, so they need not stay in the caches.
+typedef float f4v __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void st_stream(float *p, const float4 &v) {
+    const f4v w = {v.x, v.y, v.z, v.w};
+    __builtin_nontemporal_store(w, reinterpret_cast<f4v *>(p));
+}
+
+__device__ __forceinline__ void st_stream1(float *p, float v) { __builtin_nontemporal_store(v, p); }
+
+// One element of the fused Adam step: lane p of record r loads it before
+// the TV gathers (AdamElem: parameter, gradient, moments) and stores it once
+// the TV sum is known.  Elements the Adam kernel does not update keep their
+// value in `out`: SH coefficients outside the band, padding in [P, q4)
+// (rounded through the FP32 value, as the Adam kernel writes it back) and
+// the rest of the record (copied bit for bit).
+struct AdamElem {
+    float th, g, m, v;
+};
+
+template <int STORE>
+__device__ __forceinline__ AdamElem adam_load(const TvArgs &a, const AdamArgs &ad, int64_t r,
+                                              int p) {
+    AdamElem e{0.0f, 0.0f, 0.0f, 0.0f};
+    const size_t po = (size_t)r * (size_t)a.stride + (size_t)p;
+    const size_t o = (size_t)r * (size_t)ad.ostride + (size_t)p;
+    if (p < a.P) {
+        e.g = a.grad[o];
+        e.m = ad.m[o];
+        e.v = ad.v[o];
+    }
+    if (STORE == PXL_STORE_F32) {
+        e.th = a.params[po];
+    } else if (p < ad.q4) {
+        e.th = ad.master != nullptr ? ad.master[o] : ld<STORE>(a.params, po);
+    } else {
+        e.th = __uint_as_float((uint32_t)reinterpret_cast<const uint16_t *>(a.params)[po]);
+    }
+    return e;
+}
+
+template <int STORE>
+__device__ __forceinline__ void adam_store(const TvArgs &a, const AdamArgs &ad, int64_t r, int p,
+                                           AdamElem e, bool tvl, float lam, double tsum) {
+    const size_t po = (size_t)r * (size_t)a.stride + (size_t)p;
+    if (p < a.P) {
+        const size_t o = (size_t)r * (size_t)ad.ostride + (size_t)p;
+        if (tvl) e.g += (float)((double)lam * tsum);
+        const double lr_d = ad.alr != nullptr ? ad.lr * (double)ad.alr[r] : ad.lr;
+        if (adam_elem(p, a.S, a.sh_lo, a.sh_hi, lr_d, ad.sh_lr, ad.bc1, ad.bc2, e.th, e.g, e.m,
+                      e.v)) {
+            a.grad[o] = e.g;
+            st_stream1(ad.m + o, e.m);
+            st_stream1(ad.v + o, e.v);
+            if (STORE != PXL_STORE_F32 && ad.master != nullptr) ad.master[o] = e.th;
+        } else if (tvl) {
+            a.grad[o] = e.g;
+        }
+    }
+    if (STORE == PXL_STORE_F32)
+        reinterpret_cast<float *>(ad.out)[po] = e.th;
+    else if (p < ad.q4)
+        reinterpret_cast<uint16_t *>(ad.out)[po] = narrow16<STORE>(e.th);
+    else
+        reinterpret_cast<uint16_t *>(ad.out)[po] = (uint16_t)__float_as_uint(e.th);
 }
 
 __device__ __forceinline__ bool cell_occupied(const TvArgs &a, int cx, int cy, int cz) {
@@ -79,19 +199,16 @@ __device__ __forceinline__ uint32_t tv_cells_of(int i) {
 // one group of LPV lanes per record, one lane per parameter; a neighbour
 // value is loaded only when an occupied cell around the vertex contains it.
 //
-// Gradient of one record from its 27 neighbour records (lane i of the group
+// TV sum of one record from its 27 neighbour records (lane i of the group
 // holds neighbour i's record in nrec) and the mask of occupied cells around
-// it (bit c = cell (x - (c&1), y - (c>>1&1), z - (c>>2))).
+// it (bit c = cell (x - (c&1), y - (c>>1&1), z - (c>>2))); lane q's
+// element.  The gradient takes lambda times it.
 template <int LPV, int STORE>
-__device__ __forceinline__ void tv_record(const TvArgs &a, int64_t r, int p, uint32_t nrec,
-                                          uint32_t cmask) {
-    const float lam = tv_lambda(a, p);
-    const bool mine = p < a.P && lam != 0.0f;
+__device__ __forceinline__ double tv_record_sum(const TvArgs &a, int64_t r, int q, uint32_t nrec,
+                                                uint32_t cmask) {
     // Every lane issues all 27 loads unconditionally, so they are in flight
     // together (a load under a branch is waited for at the branch's end).
-    // A neighbour outside the occupied cells reads the record itself, and a
-    // lane without a parameter reads element 0: both lines are read anyway.
-    const int q = mine ? p : 0;
+    // A neighbour outside the occupied cells reads the record itself.
     float v[27];
 #pragma unroll
     for (int i = 0; i < 27; i++) {
@@ -100,7 +217,6 @@ __device__ __forceinline__ void tv_record(const TvArgs &a, int64_t r, int p, uin
         const bool used = (cells & cmask) != 0u;
         v[i] = ld<STORE>(a.params, (size_t)(used ? ri : (uint32_t)r) * (size_t)a.stride + q);
     }
-    if (!mine) return;
     // sum over occupied cells c of (u - mean_c) = n u - (1/8) sum_i m_i u_i,
     // n the occupied cells around the vertex, m_i those that also contain
     // neighbour i (m_13 = n).  Sums run in FP64: a vertex value and the mean
@@ -109,8 +225,7 @@ __device__ __forceinline__ void tv_record(const TvArgs &a, int64_t r, int p, uin
 #pragma unroll
     for (int i = 0; i < 27; i++)
         msum = fma((double)__builtin_popcount(tv_cells_of(i) & cmask), (double)v[i], msum);
-    const double tsum = (double)__builtin_popcount(cmask) * (double)v[13] - 0.125 * msum;
-    a.grad[(size_t)r * (size_t)a.gstride + p] += (float)((double)lam * tsum);
+    return (double)__builtin_popcount(cmask) * (double)v[13] - 0.125 * msum;
 }
 
 // Neighbour records and cell mask of record r, computed by its group: lane
@@ -160,7 +275,7 @@ __device__ __forceinline__ float ld_buf(__amdgpu_buffer_rsrc_t rs, uint32_t voff
     return __half2float(__ushort_as_half(h));
 }
 
-// TV gradient of an interior vertex of a dense grid: all 8 cells around it
+// TV sum of an interior vertex of a dense grid (lane q's element): all 8 cells around it
 // exist, so m_i = 2^(zero components of (dx, dy, dz)) and
 //   n u - (1/8) sum_i m_i u_i = 7 u - F / 2 - E / 4 - C / 8
 // with F, E, C the sums over the 6 face, 12 edge and 8 corner neighbours.
@@ -170,11 +285,8 @@ __device__ __forceinline__ float ld_buf(__amdgpu_buffer_rsrc_t rs, uint32_t voff
 // offset from r0 in a vector register, shared by all 27 loads, and the
 // neighbour's offset from r0 in a scalar register.
 template <int STORE, bool OFF32>
-__device__ __forceinline__ void tv_interior(const TvArgs &a, int64_t r, int p) {
+__device__ __forceinline__ double tv_interior_sum(const TvArgs &a, int64_t r, int q) {
     constexpr uint32_t kElem = STORE == PXL_STORE_F32 ? 4u : 2u;
-    const float lam = tv_lambda(a, p);
-    const bool mine = p < a.P && lam != 0.0f;
-    const uint32_t q = mine ? (uint32_t)p : 0u;
     const int64_t plane = (int64_t)a.W * a.H;
     const int64_t r0 = r - (1 + a.W + plane);
     float v[27];
@@ -183,7 +295,7 @@ __device__ __forceinline__ void tv_interior(const TvArgs &a, int64_t r, int p) {
         const uint32_t bytes = (uint32_t)a.n_vox * rec_bytes;
         const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
             const_cast<float *>(a.params), (short)0, (int)bytes, 0x00020000);
-        const uint32_t lane_off = (uint32_t)r0 * rec_bytes + q * kElem;
+        const uint32_t lane_off = (uint32_t)r0 * rec_bytes + (uint32_t)q * kElem;
 #pragma unroll
         for (int i = 0; i < 27; i++) {
             const uint32_t off = (uint32_t)(i % 3) + (uint32_t)((i / 3) % 3) * (uint32_t)a.W +
@@ -198,7 +310,6 @@ __device__ __forceinline__ void tv_interior(const TvArgs &a, int64_t r, int p) {
             v[i] = ld<STORE>(a.params, (size_t)(r0 + off) * (size_t)a.stride + q);
         }
     }
-    if (!mine) return;
     double f = 0.0, e = 0.0, c = 0.0;
 #pragma unroll
     for (int i = 0; i < 27; i++) {
@@ -207,8 +318,7 @@ __device__ __forceinline__ void tv_interior(const TvArgs &a, int64_t r, int p) {
         else if (nz == 2) e += (double)v[i];
         else if (nz == 3) c += (double)v[i];
     }
-    const double tsum = 7.0 * (double)v[13] - (0.5 * f + 0.25 * e + 0.125 * c);
-    a.grad[(size_t)r * (size_t)a.gstride + p] += (float)((double)lam * tsum);
+    return 7.0 * (double)v[13] - (0.5 * f + 0.25 * e + 0.125 * c);
 }
 
 // Interior vertices of a dense grid, marched along z.  With w(0) = 2 and
@@ -219,7 +329,8 @@ __device__ __forceinline__ void tv_interior(const TvArgs &a, int64_t r, int p) {
 // Bxy of the planes below and at z: 9 loads per vertex instead of 27.
 // Columns are numbered x fastest, then y, then the z chunk.  Loads go
 // through a buffer resource (the records span less than 4 GiB): lane offset
-// in a vector register, neighbour offset in a scalar one.
+// in a vector register, neighbour offset in a scalar one.  FUSED: each
+// vertex's record then takes its Adam step into ad.out.
 constexpr int kTvChunk = 24;
 
 __device__ __forceinline__ double tv_blur_xy(const float (&v)[9]) {
@@ -228,15 +339,19 @@ __device__ __forceinline__ double tv_blur_xy(const float (&v)[9]) {
     return 4.0 * (double)v[4] + 2.0 * e + c;
 }
 
-template <int LPV, int STORE>
-__global__ __launch_bounds__(256) void k_tv_zmarch(const TvArgs a, int64_t n_cols) {
+template <int LPV, int STORE, bool FUSED>
+__global__ __launch_bounds__(256) void k_tv_zmarch(const TvArgs a, const AdamArgs ad,
+                                                   int64_t n_cols) {
     constexpr int kGroups = 256 / LPV;
     constexpr uint32_t kElem = STORE == PXL_STORE_F32 ? 4u : 2u;
     const int g = threadIdx.x / LPV, p = threadIdx.x % LPV;
     const int64_t col = (int64_t)xcd_slice_block(blockIdx.x, gridDim.x) * kGroups + g;
     if (col >= n_cols) return;
     const float lam = tv_lambda(a, p);
-    if (p >= a.P || lam == 0.0f) return;
+    const bool tvl = p < a.P && lam != 0.0f;
+    // Alone, only the lanes with a TV term work; fused with Adam, every lane
+    // of the record.
+    if (FUSED ? p >= a.stride : !tvl) return;
     const int nx = a.W - 2, ny = a.H - 2;
     const int x = 1 + (int)(col % nx), y = 1 + (int)((col / nx) % ny);
     const int z0 = 1 + (int)(col / ((int64_t)nx * ny)) * kTvChunk;
@@ -254,12 +369,13 @@ __global__ __launch_bounds__(256) void k_tv_zmarch(const TvArgs a, int64_t n_col
     for (int i = 0; i < 9; i++) nb[i] = ((uint32_t)(i / 3) * (uint32_t)a.W + (uint32_t)(i % 3)) * rec_bytes;
     const size_t gplane = (size_t)a.W * a.H * (size_t)a.gstride;
     float *const gcol = a.grad + ((size_t)y * a.W + x) * (size_t)a.gstride + p;
+    const int64_t rcol = (int64_t)y * a.W + x, rplane = (int64_t)a.W * a.H;
     auto plane = [&](int zz, float (&v)[9]) {
 #pragma unroll
         for (int i = 0; i < 9; i++) v[i] = ld_buf<STORE>(rs, lane0 + (uint32_t)zz * plane_bytes, nb[i]);
     };
-    double b_lo, b_mid, u;
-    {
+    double b_lo = 0.0, b_mid = 0.0, u = 0.0;
+    if (tvl) {
         float v[9];
         plane(z0 - 1, v);
         b_lo = tv_blur_xy(v);
@@ -268,24 +384,36 @@ __global__ __launch_bounds__(256) void k_tv_zmarch(const TvArgs a, int64_t n_col
         u = (double)v[4];
     }
     // Ring of three slots: the slot of vertex s holds plane s + 1 and the
-    // gradient of s, loaded three steps before it is used (indices clamped
-    // to the chunk: in bounds, unused).  A step uses its slot, then refills
-    // it for vertex s + 3, so two steps' loads stay in flight behind it.
+    // gradient of s (fused: the Adam element of s), loaded three steps
+    // before it is used (indices clamped to the chunk: in bounds, unused).
+    // A step uses its slot, then refills it for vertex s + 3, so two steps'
+    // loads stay in flight behind it.
     float pv[3][9], pg[3];
+    AdamElem pa[3];
     auto fill = [&](int k, int s) {
-        pg[k] = gcol[(size_t)min(s, z1 - 1) * gplane];
-        plane(min(s + 1, z1), pv[k]);
+        const int sc = min(s, z1 - 1);
+        if (FUSED)
+            pa[k] = adam_load<STORE>(a, ad, (int64_t)sc * rplane + rcol, p);
+        else
+            pg[k] = gcol[(size_t)sc * gplane];
+        if (tvl) plane(min(s + 1, z1), pv[k]);
     };
     fill(0, z0);
     fill(1, z0 + 1);
     fill(2, z0 + 2);
     auto step = [&](int k, int s, bool refill) {
-        const double b_hi = tv_blur_xy(pv[k]);
-        const double tsum = 8.0 * u - 0.125 * (b_lo + 2.0 * b_mid + b_hi);
-        gcol[(size_t)s * gplane] = pg[k] + (float)((double)lam * tsum);
-        b_lo = b_mid;
-        b_mid = b_hi;
-        u = (double)pv[k][4];
+        double tsum = 0.0;
+        if (tvl) {
+            const double b_hi = tv_blur_xy(pv[k]);
+            tsum = 8.0 * u - 0.125 * (b_lo + 2.0 * b_mid + b_hi);
+            b_lo = b_mid;
+            b_mid = b_hi;
+            u = (double)pv[k][4];
+        }
+        if (FUSED)
+            adam_store<STORE>(a, ad, (int64_t)s * rplane + rcol, p, pa[k], tvl, lam, tsum);
+        else
+            gcol[(size_t)s * gplane] = pg[k] + (float)((double)lam * tsum);
         if (refill) fill(k, s + 3);
     };
     int s = z0;
@@ -317,31 +445,51 @@ __device__ __forceinline__ int64_t tv_boundary_record(const TvArgs &a, int64_t b
 
 // addVarianceLossGPU as a gather, looking the neighbours up on the fly
 // (dense grids, where a record's neighbours are arithmetic).  All vertices
-// (n_items = n_vox): interior ones take tv_interior, the rest the general
-// gather.  boundary_only (n_items boundary vertices, tv_boundary_record):
-// the general gather for the vertices k_tv_zmarch leaves.
-template <int LPV, int STORE, bool OFF32>
-__global__ __launch_bounds__(256) void k_tv_disparity(const TvArgs a, int64_t n_items,
-                                                      int boundary_only) {
+// (n_items = n_vox): interior ones take tv_interior_sum, the rest the
+// general gather.  boundary_only (n_items boundary vertices,
+// tv_boundary_record): the general gather for the vertices k_tv_zmarch
+// leaves.  FUSED: the group then runs the Adam step of its record.
+template <int LPV, int STORE, bool OFF32, bool FUSED>
+__global__ __launch_bounds__(256) void k_tv_disparity(const TvArgs a, const AdamArgs ad,
+                                                      int64_t n_items, int boundary_only) {
     constexpr int kGroups = 256 / LPV;
     const int g = threadIdx.x / LPV, p = threadIdx.x % LPV;
     const int64_t item = (int64_t)xcd_slice_block(blockIdx.x, gridDim.x) * kGroups + g;
     if (item >= n_items) return;  // the whole group leaves together
     const int64_t r = boundary_only ? tv_boundary_record(a, item) : item;
+    const float lam = tv_lambda(a, p);
+    const bool mine = p < a.P && lam != 0.0f;
+    // A lane without a TV term reads element 0 (the line is read anyway).
+    const int q = mine ? p : 0;
+    AdamElem ae{0.0f, 0.0f, 0.0f, 0.0f};
+    if (FUSED && p < a.stride) ae = adam_load<STORE>(a, ad, r, p);
+    bool interior = false;
     if (!boundary_only) {
         const uint32_t d = (uint32_t)r;
         const uint32_t x = d % (uint32_t)a.W, y = (d / (uint32_t)a.W) % (uint32_t)a.H;
         const uint32_t z = d / ((uint32_t)a.W * (uint32_t)a.H);
-        if (x - 1u < (uint32_t)a.W - 2u && y - 1u < (uint32_t)a.H - 2u &&
-            z - 1u < (uint32_t)a.D - 2u) {
-            tv_interior<STORE, OFF32>(a, r, p);
-            return;
+        interior = x - 1u < (uint32_t)a.W - 2u && y - 1u < (uint32_t)a.H - 2u &&
+                   z - 1u < (uint32_t)a.D - 2u;
+    }
+    double tsum = 0.0;
+    bool tvl = false;
+    if (interior) {
+        tsum = tv_interior_sum<STORE, OFF32>(a, r, q);
+        tvl = mine;
+    } else {
+        uint32_t cmask;
+        const uint32_t nrec = tv_neighbours<LPV>(a, r, p, cmask);
+        // No occupied cell around the vertex: no TV term.
+        if (cmask != 0u) {
+            tsum = tv_record_sum<LPV, STORE>(a, r, q, nrec, cmask);
+            tvl = mine;
         }
     }
-    uint32_t cmask;
-    const uint32_t nrec = tv_neighbours<LPV>(a, r, p, cmask);
-    if (cmask == 0u) return;  // no occupied cell around this vertex
-    tv_record<LPV, STORE>(a, r, p, nrec, cmask);
+    if (!FUSED) {
+        if (tvl) a.grad[(size_t)r * (size_t)a.gstride + p] += (float)((double)lam * tsum);
+        return;
+    }
+    if (p < a.stride) adam_store<STORE>(a, ad, r, p, ae, tvl, lam, tsum);
 }
 
 // Caged grids: the neighbour records and the cell mask depend only on the
@@ -377,49 +525,62 @@ __global__ __launch_bounds__(256) void k_tv_table_build(const TvArgs a, uint32_t
 // TV gradient of record r from its prepared table row.  OFF24: every record
 // index is below 2^24 and every byte offset into the records below 2^32, so
 // an address is one 24-bit multiply-add on the records' base pointer.
-template <int LPV, int STORE, bool OFF24>
-__global__ __launch_bounds__(256) void k_tv_disparity_table(const TvArgs a,
+// FUSED: the group then runs the Adam step of record r into ad.out.
+template <int LPV, int STORE, bool OFF24, bool FUSED>
+__global__ __launch_bounds__(256) void k_tv_disparity_table(const TvArgs a, const AdamArgs ad,
                                                             const uint32_t *__restrict__ table) {
     constexpr int kGroups = 256 / LPV;
     constexpr uint32_t kElem = STORE == PXL_STORE_F32 ? 4u : 2u;
     const int g = threadIdx.x / LPV, p = threadIdx.x % LPV;
     const int64_t r = (int64_t)xcd_slice_block(blockIdx.x, gridDim.x) * kGroups + g;
     if (r >= a.n_vox) return;
+    AdamElem ae{0.0f, 0.0f, 0.0f, 0.0f};
+    if (FUSED && p < a.stride) ae = adam_load<STORE>(a, ad, r, p);
     uint32_t e = p < kTableWords ? table[(size_t)r * kTableWords + p] : 0u;
     const uint32_t cmask = (uint32_t)__shfl((int)e, 27, LPV) & 0xFFu;
-    if (cmask == 0u) return;
-    // A record index beyond the grid (scratch not prepared for this cage)
-    // reads record 0 instead of faulting.
-    if (p < 27 && (int64_t)e >= a.n_vox) e = 0u;
     const float lam = tv_lambda(a, p);
     const bool mine = p < a.P && lam != 0.0f;
-    // Every lane issues all 27 loads unconditionally, so they are in flight
-    // together; a lane without a parameter reads element 0.
-    const uint32_t q = mine ? (uint32_t)p : 0u;
-    const char *base = reinterpret_cast<const char *>(a.params);
-    const uint32_t rec_bytes = (uint32_t)a.stride * kElem;
-    float v[27];
+    double tsum = 0.0;
+    bool tvl = false;
+    // No occupied cell around the vertex: no TV term.
+    if (cmask != 0u) {
+        // A record index beyond the grid (scratch not prepared for this
+        // cage) reads record 0 instead of faulting.
+        if (p < 27 && (int64_t)e >= a.n_vox) e = 0u;
+        // Every lane issues all 27 loads unconditionally, so they are in
+        // flight together; a lane without a parameter reads element 0.
+        const uint32_t q = mine ? (uint32_t)p : 0u;
+        const char *base = reinterpret_cast<const char *>(a.params);
+        const uint32_t rec_bytes = (uint32_t)a.stride * kElem;
+        float v[27];
 #pragma unroll
-    for (int i = 0; i < 27; i++) {
-        const uint32_t ri = (uint32_t)__shfl((int)e, i, LPV);
-        if (OFF24)
-            v[i] = ld_at<STORE>(base, (uint32_t)__umul24(ri, rec_bytes) + q * kElem);
-        else
-            v[i] = ld<STORE>(a.params, (size_t)ri * (size_t)a.stride + q);
+        for (int i = 0; i < 27; i++) {
+            const uint32_t ri = (uint32_t)__shfl((int)e, i, LPV);
+            if (OFF24)
+                v[i] = ld_at<STORE>(base, (uint32_t)__umul24(ri, rec_bytes) + q * kElem);
+            else
+                v[i] = ld<STORE>(a.params, (size_t)ri * (size_t)a.stride + q);
+        }
+        uint32_t mw[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) mw[j] = (uint32_t)__shfl((int)e, 28 + j, LPV);
+        if (mine) {
+            // sum over occupied cells c of (u - mean_c) = n u - (1/8) sum_i
+            // m_i u_i (m_13 = n), in FP64: a vertex value and the mean of a
+            // cell around it nearly cancel.
+            double msum = 0.0;
+#pragma unroll
+            for (int i = 0; i < 27; i++)
+                msum = fma((double)((mw[i >> 3] >> (4 * (i & 7))) & 15u), (double)v[i], msum);
+            tsum = (double)__builtin_popcount(cmask) * (double)v[13] - 0.125 * msum;
+            tvl = true;
+        }
     }
-    uint32_t mw[4];
-#pragma unroll
-    for (int j = 0; j < 4; j++) mw[j] = (uint32_t)__shfl((int)e, 28 + j, LPV);
-    if (!mine) return;
-    // sum over occupied cells c of (u - mean_c) = n u - (1/8) sum_i m_i u_i
-    // (m_13 = n), in FP64: a vertex value and the mean of a cell around it
-    // nearly cancel.
-    double msum = 0.0;
-#pragma unroll
-    for (int i = 0; i < 27; i++)
-        msum = fma((double)((mw[i >> 3] >> (4 * (i & 7))) & 15u), (double)v[i], msum);
-    const double tsum = (double)__builtin_popcount(cmask) * (double)v[13] - 0.125 * msum;
-    a.grad[(size_t)r * (size_t)a.gstride + p] += (float)((double)lam * tsum);
+    if (!FUSED) {
+        if (tvl) a.grad[(size_t)r * (size_t)a.gstride + p] += (float)((double)lam * tsum);
+        return;
+    }
+    if (p < a.stride) adam_store<STORE>(a, ad, r, p, ae, tvl, lam, tsum);
 }
 
 // Caged grids: a cell corner that is not in cage reads and writes record 0
@@ -732,32 +893,6 @@ __global__ __launch_bounds__(256) void k_tv_neighbour6_rec0(const TvArgs a, int6
 // follows the reference expression by expression.  With 16-bit storage the
 // update runs on the FP32 master copy when there is one (else on the
 // widened stored value) and the grid receives the rounded result.
-__device__ __forceinline__ uint16_t to_bf16(float f) {
-    uint32_t u = __float_as_uint(f);
-    if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
-}
-
-template <int STORE>
-__device__ __forceinline__ float widen16(uint16_t h) {
-    if (STORE == PXL_STORE_BF16) return __uint_as_float((uint32_t)h << 16);
-    return __half2float(__ushort_as_half(h));
-}
-template <int STORE>
-__device__ __forceinline__ uint16_t narrow16(float v) {
-    if (STORE == PXL_STORE_BF16) return to_bf16(v);
-    return __half_as_ushort(__float2half_rn(v));
-}
-
-// Streaming store (non-temporal hint) of the Adam moments: no kernel reads
-// them before the next Adam step, so they need not stay in the caches.
-typedef float f4v __attribute__((ext_vector_type(4)));
-__device__ __forceinline__ void st_stream(float *p, const float4 &v) {
-    const f4v w = {v.x, v.y, v.z, v.w};
-    __builtin_nontemporal_store(w, reinterpret_cast<f4v *>(p));
-}
-
 template <int STORE>
 __global__ __launch_bounds__(256) void k_adam(float *__restrict__ prm, float *__restrict__ grd,
                                               float *__restrict__ mom, float *__restrict__ var,
@@ -791,7 +926,6 @@ __global__ __launch_bounds__(256) void k_adam(float *__restrict__ prm, float *__
     float4 m4 = *reinterpret_cast<const float4 *>(mom + off);
     float4 v4 = *reinterpret_cast<const float4 *>(var + off);
     float *tp = &th.x, *gp = &g4.x, *mp = &m4.x, *vp = &v4.x;
-    const float b1 = 0.9f, b2 = 0.999f, eps = 1.e-6f;
     const double lr_d = alr ? lr * (double)alr[r] : lr;
 #pragma unroll
     for (int e = 0; e < 4; e++) {
@@ -799,19 +933,7 @@ __global__ __launch_bounds__(256) void k_adam(float *__restrict__ prm, float *__
         if (p >= P) break;
         // SH coefficients outside the optimised band keep their state
         // (Optimizer.cu:393-412).
-        if (p > 0 && ((p - 1) % S < sh_lo || (p - 1) % S >= sh_hi)) continue;
-        const float gg = gp[e];
-        const float mm = (float)((double)(b1 * mp[e]) + (1.0 - (double)b1) * (double)gg);
-        const float vv =
-            (float)((double)(b2 * vp[e]) + (1.0 - (double)b2) * (double)gg * (double)gg);
-        const float cm = (float)((double)mm / bc1);
-        const float cv = (float)((double)vv / bc2);
-        const double upd = (p == 0 ? lr_d : sh_lr) * (double)cm / (double)(sqrtf(cv) + eps);
-        const double nt = (double)tp[e] - upd;
-        tp[e] = p == 0 ? (nt < 0.0 ? 0.0f : (float)nt) : (float)nt;
-        mp[e] = mm;
-        vp[e] = vv;
-        gp[e] = 0.0f;
+        (void)adam_elem(p, S, sh_lo, sh_hi, lr_d, sh_lr, bc1, bc2, tp[e], gp[e], mp[e], vp[e]);
     }
     if (STORE == PXL_STORE_F32) {
         *reinterpret_cast<float4 *>(prm + poff) = th;
@@ -824,7 +946,8 @@ __global__ __launch_bounds__(256) void k_adam(float *__restrict__ prm, float *__
         h.x = (uint32_t)narrow16<STORE>(th.x) | ((uint32_t)narrow16<STORE>(th.y) << 16);
         h.y = (uint32_t)narrow16<STORE>(th.z) | ((uint32_t)narrow16<STORE>(th.w) << 16);
         *reinterpret_cast<uint2 *>(reinterpret_cast<uint16_t *>(prm) + poff) = h;
-    }    *reinterpret_cast<float4 *>(grd + off) = g4;
+    }
+    *reinterpret_cast<float4 *>(grd + off) = g4;
     st_stream(mom + off, m4);
     st_stream(var + off, v4);
 }
@@ -899,82 +1022,88 @@ int32_t tv_check(const pxl_grid *g, const pxl_optim *o, const char *who) {
 // (both are tested).
 bool offsets64_forced() { return option_tv_addr64(); }
 
-template <int STORE>
-void tv_launch(const TvArgs &a, int type, void *scratch, hipStream_t s) {
+// The disparity TV of the grid into the gradient, or (ad != nullptr) fused
+// with the Adam step into ad->out.  Caged grids add the aliased corners'
+// sum into record 0 first (k_alias_partial / k_alias_final), so that both
+// forms add record 0's two terms in the same order.
+template <int STORE, bool FUSED>
+void tv_launch_as(const TvArgs &a, const AdamArgs &ad, void *scratch, hipStream_t s) {
     const bool wide = a.P > 32;
     const int groups = wide ? 4 : 8;
     const unsigned blocks = (unsigned)((a.n_vox + groups - 1) / groups);
-    if (type == PXL_TV_DISPARITY) {
-        if (a.sparse) {
-            const uint32_t *table = reinterpret_cast<const uint32_t *>(
-                reinterpret_cast<char *>(scratch) + tv_table_off(a.n_vox));
-            constexpr uint64_t kElem = STORE == PXL_STORE_F32 ? 4u : 2u;
-            const bool off24 = !offsets64_forced() && a.n_vox <= (int64_t)(1u << 24) &&
-                               (uint64_t)a.stride * kElem < (1u << 24) &&
-                               (uint64_t)a.n_vox * (uint64_t)a.stride * kElem <= 0xFFFFFFFFull;
-            if (wide && off24)
-                hipLaunchKernelGGL((k_tv_disparity_table<64, STORE, true>), dim3(blocks),
-                                   dim3(256), 0, s, a, table);
-            else if (wide)
-                hipLaunchKernelGGL((k_tv_disparity_table<64, STORE, false>), dim3(blocks),
-                                   dim3(256), 0, s, a, table);
-            else if (off24)
-                hipLaunchKernelGGL((k_tv_disparity_table<32, STORE, true>), dim3(blocks),
-                                   dim3(256), 0, s, a, table);
-            else
-                hipLaunchKernelGGL((k_tv_disparity_table<32, STORE, false>), dim3(blocks),
-                                   dim3(256), 0, s, a, table);
-        } else {
-            constexpr uint64_t kElem = STORE == PXL_STORE_F32 ? 4u : 2u;
-            const bool off32 = !offsets64_forced() &&
-                               (uint64_t)a.n_vox * (uint64_t)a.stride * kElem <= 0xFFFFFFFFull;
-            // 32-bit offsets: interior vertices marched along z, the rest
-            // gathered one by one; otherwise every vertex gathered.
-            const int skip = off32 && a.W >= 3 && a.H >= 3 && a.D >= 3;
-            if (skip) {
-                const int64_t n_cols = (int64_t)(a.W - 2) * (a.H - 2) *
-                                       ((a.D - 2 + kTvChunk - 1) / kTvChunk);
-                const unsigned cb = (unsigned)((n_cols + groups - 1) / groups);
-                if (wide)
-                    hipLaunchKernelGGL((k_tv_zmarch<64, STORE>), dim3(cb), dim3(256), 0, s, a,
-                                       n_cols);
-                else
-                    hipLaunchKernelGGL((k_tv_zmarch<32, STORE>), dim3(cb), dim3(256), 0, s, a,
-                                       n_cols);
-            }
-            const int64_t items =
-                skip ? 2 * ((int64_t)a.W * a.H + (int64_t)a.W * (a.D - 2) +
-                            (int64_t)(a.H - 2) * (a.D - 2))
-                     : a.n_vox;
-            const unsigned ib = (unsigned)((items + groups - 1) / groups);
-            if (wide && off32)
-                hipLaunchKernelGGL((k_tv_disparity<64, STORE, true>), dim3(ib), dim3(256), 0, s,
-                                   a, items, skip);
-            else if (wide)
-                hipLaunchKernelGGL((k_tv_disparity<64, STORE, false>), dim3(ib), dim3(256), 0, s,
-                                   a, items, 0);
-            else if (off32)
-                hipLaunchKernelGGL((k_tv_disparity<32, STORE, true>), dim3(ib), dim3(256), 0, s,
-                                   a, items, skip);
-            else
-                hipLaunchKernelGGL((k_tv_disparity<32, STORE, false>), dim3(ib), dim3(256), 0, s,
-                                   a, items, 0);
-        }
-        if (a.sparse) {
-            char *base = reinterpret_cast<char *>(scratch);
-            const AliasHeader *hdr = reinterpret_cast<const AliasHeader *>(base);
-            const uint2 *list = reinterpret_cast<const uint2 *>(base + alias_list_off(a.n_vox));
-            double *part = reinterpret_cast<double *>(base + alias_partials_off(a.n_vox));
-            if (wide)
-                hipLaunchKernelGGL((k_alias_partial<64, STORE>), dim3(kAliasBlocks), dim3(256), 0,
-                                   s, a, hdr, list, part);
-            else
-                hipLaunchKernelGGL((k_alias_partial<32, STORE>), dim3(kAliasBlocks), dim3(256), 0,
-                                   s, a, hdr, list, part);
-            hipLaunchKernelGGL(k_alias_final<STORE>, dim3((unsigned)a.P), dim3(256), 0, s, a, hdr,
-                               part);
-        }
+    if (a.sparse) {
+        char *base = reinterpret_cast<char *>(scratch);
+        const AliasHeader *hdr = reinterpret_cast<const AliasHeader *>(base);
+        const uint2 *list = reinterpret_cast<const uint2 *>(base + alias_list_off(a.n_vox));
+        double *part = reinterpret_cast<double *>(base + alias_partials_off(a.n_vox));
+        if (wide)
+            hipLaunchKernelGGL((k_alias_partial<64, STORE>), dim3(kAliasBlocks), dim3(256), 0, s,
+                               a, hdr, list, part);
+        else
+            hipLaunchKernelGGL((k_alias_partial<32, STORE>), dim3(kAliasBlocks), dim3(256), 0, s,
+                               a, hdr, list, part);
+        hipLaunchKernelGGL(k_alias_final<STORE>, dim3((unsigned)a.P), dim3(256), 0, s, a, hdr,
+                           part);
+        const uint32_t *table = reinterpret_cast<const uint32_t *>(base + tv_table_off(a.n_vox));
+        constexpr uint64_t kElem = STORE == PXL_STORE_F32 ? 4u : 2u;
+        const bool off24 = !offsets64_forced() && a.n_vox <= (int64_t)(1u << 24) &&
+                           (uint64_t)a.stride * kElem < (1u << 24) &&
+                           (uint64_t)a.n_vox * (uint64_t)a.stride * kElem <= 0xFFFFFFFFull;
+        if (wide && off24)
+            hipLaunchKernelGGL((k_tv_disparity_table<64, STORE, true, FUSED>), dim3(blocks),
+                               dim3(256), 0, s, a, ad, table);
+        else if (wide)
+            hipLaunchKernelGGL((k_tv_disparity_table<64, STORE, false, FUSED>), dim3(blocks),
+                               dim3(256), 0, s, a, ad, table);
+        else if (off24)
+            hipLaunchKernelGGL((k_tv_disparity_table<32, STORE, true, FUSED>), dim3(blocks),
+                               dim3(256), 0, s, a, ad, table);
+        else
+            hipLaunchKernelGGL((k_tv_disparity_table<32, STORE, false, FUSED>), dim3(blocks),
+                               dim3(256), 0, s, a, ad, table);
+        return;
     }
+    constexpr uint64_t kElem = STORE == PXL_STORE_F32 ? 4u : 2u;
+    const bool off32 = !offsets64_forced() &&
+                       (uint64_t)a.n_vox * (uint64_t)a.stride * kElem <= 0xFFFFFFFFull;
+    // 32-bit offsets: interior vertices marched along z, the rest gathered
+    // one by one; otherwise every vertex gathered.
+    const int skip = off32 && a.W >= 3 && a.H >= 3 && a.D >= 3;
+    if (skip) {
+        const int64_t n_cols =
+            (int64_t)(a.W - 2) * (a.H - 2) * ((a.D - 2 + kTvChunk - 1) / kTvChunk);
+        const unsigned cb = (unsigned)((n_cols + groups - 1) / groups);
+        if (wide)
+            hipLaunchKernelGGL((k_tv_zmarch<64, STORE, FUSED>), dim3(cb), dim3(256), 0, s, a, ad,
+                               n_cols);
+        else
+            hipLaunchKernelGGL((k_tv_zmarch<32, STORE, FUSED>), dim3(cb), dim3(256), 0, s, a, ad,
+                               n_cols);
+    }
+    const int64_t items = skip ? 2 * ((int64_t)a.W * a.H + (int64_t)a.W * (a.D - 2) +
+                                      (int64_t)(a.H - 2) * (a.D - 2))
+                               : a.n_vox;
+    const unsigned ib = (unsigned)((items + groups - 1) / groups);
+    if (wide && off32)
+        hipLaunchKernelGGL((k_tv_disparity<64, STORE, true, FUSED>), dim3(ib), dim3(256), 0, s, a,
+                           ad, items, skip);
+    else if (wide)
+        hipLaunchKernelGGL((k_tv_disparity<64, STORE, false, FUSED>), dim3(ib), dim3(256), 0, s,
+                           a, ad, items, 0);
+    else if (off32)
+        hipLaunchKernelGGL((k_tv_disparity<32, STORE, true, FUSED>), dim3(ib), dim3(256), 0, s, a,
+                           ad, items, skip);
+    else
+        hipLaunchKernelGGL((k_tv_disparity<32, STORE, false, FUSED>), dim3(ib), dim3(256), 0, s,
+                           a, ad, items, 0);
+}
+
+template <int STORE>
+void tv_launch(const TvArgs &a, const AdamArgs *ad, void *scratch, hipStream_t s) {
+    if (ad != nullptr)
+        tv_launch_as<STORE, true>(a, *ad, scratch, s);
+    else
+        tv_launch_as<STORE, false>(a, AdamArgs{}, scratch, s);
 }
 
 // The 6-neighbour term over the columns y W + x < col_limit.
@@ -1121,9 +1250,9 @@ extern "C" int32_t pxl_add_variance_loss(const pxl_grid *grid, const pxl_optim *
         // SH terms only for optimize_step <= 1 (RenderUtils.cu:2493).
         a.lam_sh = optimize_step <= 1 ? (float)lambda_sh : 0.0f;
         switch (grid->storage) {
-            case PXL_STORE_F32: tv_launch<PXL_STORE_F32>(a, PXL_TV_DISPARITY, scratch, s); break;
-            case PXL_STORE_BF16: tv_launch<PXL_STORE_BF16>(a, PXL_TV_DISPARITY, scratch, s); break;
-            default: tv_launch<PXL_STORE_F16>(a, PXL_TV_DISPARITY, scratch, s); break;
+            case PXL_STORE_F32: tv_launch<PXL_STORE_F32>(a, nullptr, scratch, s); break;
+            case PXL_STORE_BF16: tv_launch<PXL_STORE_BF16>(a, nullptr, scratch, s); break;
+            default: tv_launch<PXL_STORE_F16>(a, nullptr, scratch, s); break;
         }
     }
     return check_launch("pxl_add_variance_loss");
@@ -1176,6 +1305,65 @@ extern "C" int32_t pxl_adam_step(const pxl_grid *grid, const pxl_optim *optim, d
     }
 #undef PXL_ADAM
     return check_launch("pxl_adam_step");
+}
+
+extern "C" int32_t pxl_tv_adam_step(const pxl_grid *grid, const pxl_optim *optim,
+                                    double lambda_density, double lambda_sh,
+                                    int32_t optimize_step, void *scratch, size_t scratch_bytes,
+                                    double lr, double sh_lr, int32_t t, void *params_out,
+                                    void *stream) {
+    clear_error();
+    int32_t st = tv_check(grid, optim, "pxl_tv_adam_step");
+    if (st != PXL_OK) return st;
+    PXL_REQUIRE(optim->m && optim->v && params_out, "pxl_tv_adam_step: null argument");
+    PXL_REQUIRE(t >= 1, "pxl_tv_adam_step: time step %d must be >= 1", t);
+    const int P = 1 + 3 * grid->sh_dim;
+    const size_t elem = grid->storage == PXL_STORE_F32 ? 4u : 2u;
+    const size_t bytes = (size_t)grid->n_vox * (size_t)grid->stride * elem;
+    const uintptr_t in0 = (uintptr_t)grid->params, out0 = (uintptr_t)params_out;
+    PXL_REQUIRE(out0 + bytes <= in0 || in0 + bytes <= out0,
+                "pxl_tv_adam_step: params_out overlaps the grid's records");
+    PXL_REQUIRE(grid->stride <= (P > 32 ? 64 : 32),
+                "pxl_tv_adam_step: record stride %d above %d", grid->stride, P > 32 ? 64 : 32);
+    const int ostride = optim->stride > 0 ? optim->stride : grid->stride;
+    PXL_REQUIRE(ostride >= P && ostride % 4 == 0, "pxl_tv_adam_step: bad optimiser stride %d",
+                ostride);
+    if (grid->sparse) {
+        const size_t need = pxl_tv_scratch_bytes(grid);
+        if (!scratch || scratch_bytes < need || ((uintptr_t)scratch & 255)) {
+            set_error("pxl_tv_adam_step: sparse grids need %zu bytes of scratch prepared by "
+                      "pxl_tv_prepare, got %zu", need, scratch_bytes);
+            return PXL_ENOMEM;
+        }
+    }
+    TvArgs a;
+    fill_tv_args(grid, optim, a);
+    PXL_REQUIRE(a.sh_hi <= a.S && a.sh_lo >= 0 && a.sh_lo < a.sh_hi,
+                "pxl_tv_adam_step: band [%d, %d) outside the grid's %d coefficients", a.sh_lo,
+                a.sh_hi, a.S);
+    a.lam_d = (float)lambda_density;
+    // SH terms only for optimize_step <= 1 (RenderUtils.cu:2493).
+    a.lam_sh = optimize_step <= 1 ? (float)lambda_sh : 0.0f;
+    AdamArgs ad;
+    ad.out = params_out;
+    ad.m = optim->m;
+    ad.v = optim->v;
+    ad.master = grid->storage == PXL_STORE_F32 ? nullptr : optim->master;
+    ad.alr = grid->sparse ? nullptr : optim->adaptive_lr;
+    ad.ostride = a.gstride;
+    ad.q4 = (P + 3) / 4 * 4;
+    ad.lr = lr;
+    ad.sh_lr = sh_lr;
+    // Bias corrections with float pow as in the reference (Optimizer.cu:384-385).
+    ad.bc1 = 1.0 - (double)powf(0.9f, (float)t);
+    ad.bc2 = 1.0 - (double)powf(0.999f, (float)t);
+    hipStream_t s = as_stream(stream);
+    switch (grid->storage) {
+        case PXL_STORE_F32: tv_launch<PXL_STORE_F32>(a, &ad, scratch, s); break;
+        case PXL_STORE_BF16: tv_launch<PXL_STORE_BF16>(a, &ad, scratch, s); break;
+        default: tv_launch<PXL_STORE_F16>(a, &ad, scratch, s); break;
+    }
+    return check_launch("pxl_tv_adam_step");
 }
 
 extern "C" int32_t pxl_loss_reduce_f64(const double *losses, int64_t len, double *out,

@@ -27,7 +27,7 @@ STORE_F32, STORE_BF16, STORE_F16 = 0, 1, 2
 EXPORTS = (
     "pxl_last_error", "pxl_version", "pxl_abi_sizes", "pxl_camera_from_pose", "pxl_rays_init",
     "pxl_render_fwd", "pxl_render_fwd_trace", "pxl_render_bwd_rgb", "pxl_train_rays",
-    "pxl_tv_scratch_bytes", "pxl_add_variance_loss", "pxl_adam_step",
+    "pxl_tv_scratch_bytes", "pxl_add_variance_loss", "pxl_adam_step", "pxl_tv_adam_step",
     "pxl_build_occupancy", "pxl_sparse_index_scratch_bytes", "pxl_build_sparse_index",
     "pxl_grid_compact", "pxl_grid_import_planes", "pxl_grid_export_planes",
     "pxl_loss_reduce_f64", "pxl_count_touched", "pxl_trainer_create", "pxl_trainer_step",
@@ -175,6 +175,7 @@ def _declare(L):
         "pxl_tv_scratch_bytes": ([G], _SZ),
         "pxl_add_variance_loss": ([G, O, _F64, _F64, _I32, _I32, _I32, _P, _SZ, _P], _I32),
         "pxl_adam_step": ([G, O, _F64, _F64, _I32, _P], _I32),
+        "pxl_tv_adam_step": ([G, O, _F64, _F64, _I32, _P, _SZ, _F64, _F64, _I32, _P, _P], _I32),
         "pxl_build_occupancy": ([_P, _I32, _I32, _I32, _F32, _P, _P, _P, _P], _I32),
         "pxl_sparse_index_scratch_bytes": ([_I64], _SZ),
         "pxl_build_sparse_index": ([_P, _I32, _I32, _I32, _P, _P, _P, _P, _SZ, _P], _I32),
@@ -366,14 +367,17 @@ class Grid:
 
 
 class Optim:
-    """Gradient and Adam moments (FP32 records).  For a 16-bit grid,
-    `master` (FP32 parameters that Adam updates) is created from the
+    """Gradient and Adam moments (FP32 records of `stride` floats: by
+    default the grid's stride, or 32 for a 16-bit grid; any multiple of 4
+    from P up, e.g. 28 for S = 9, which streams no padding).  For a 16-bit
+    grid, `master` (FP32 parameters that Adam updates) is created from the
     given FP32 records when master_from is passed."""
 
-    def __init__(self, grid, adaptive_lr=None, master_from=None, sh_first=0):
+    def __init__(self, grid, adaptive_lr=None, master_from=None, sh_first=0, stride=None):
         import torch
 
-        stride = grid.stride if grid.storage == STORE_F32 else record_stride(grid.S)
+        if stride is None:
+            stride = grid.stride if grid.storage == STORE_F32 else record_stride(grid.S)
         shape = (grid.n_vox, stride)
         dev = grid.params.device
         self.stride = stride
@@ -740,6 +744,21 @@ def hierarchy_plan(hierarchy, minimum_hierarchy, epochs, batch_size, shuffle_ray
         out.append({"level": L.level, "epochs": L.epochs, "batch_size": L.batch_size,
                     "upsample": up})
     return out
+
+
+def tv_adam_step(grid, optim, lambda_density, lambda_sh, lr, sh_lr, t, params_out,
+                 optimize_step=0, stream=None):
+    """addVarianceLossGPU (type 1) and updateRGBSceneGPU_ADAM in one pass
+    (RenderUtils.cu:2469-2547, Optimizer.cu:359-415): the stepped records go
+    to params_out (a tensor shaped and typed as grid.params); grid.params is
+    left as it was."""
+    assert params_out.shape == grid.params.shape and params_out.dtype == grid.params.dtype
+    g, o = grid.c(), optim.c()
+    buf, nbytes = tv_scratch(grid, stream) if grid.sparse else (None, 0)
+    _check("pxl_tv_adam_step",
+           lib().pxl_tv_adam_step(ctypes.byref(g), ctypes.byref(o), lambda_density, lambda_sh,
+                                  optimize_step, _ptr(buf), nbytes, lr, sh_lr, t,
+                                  _ptr(params_out), _stream(stream)))
 
 
 def adam_step(grid, optim, lr, sh_lr, t, stream=None):

@@ -316,6 +316,27 @@ int32_t pxl_add_variance_loss(const pxl_grid *grid, const pxl_optim *optim,
 int32_t pxl_adam_step(const pxl_grid *grid, const pxl_optim *optim,
                       double lr, double sh_lr, int32_t t, void *stream);
 
+/* pxl_add_variance_loss of type PXL_TV_DISPARITY followed by pxl_adam_step,
+ * as one pass over the records: the TV sums of the records in
+ * grid->params enter each record's gradient and its Adam step, and the
+ * stepped records go to params_out, so that no lane reads a neighbour that
+ * has already been stepped.  Fuses addVarianceLossGPU (RenderUtils.cu:
+ * 2469-2547) and updateRGBSceneGPU_ADAM (Optimizer.cu:359-415), which the
+ * reference runs one after the other (Render.cu:471, 492).
+ * params_out: n_vox x stride elements of the grid's storage type, not
+ *   overlapping grid->params; on return (stream-ordered) it holds what
+ *   grid->params would hold after the two separate calls, bit for bit,
+ *   and grid->params is unchanged.  The caller then uses params_out as the
+ *   grid's records (the trainer swaps two buffers).
+ * grad := 0 and m, v (and the FP32 master of 16-bit grids) are updated in
+ * place as by pxl_adam_step.  Records of at most 32 elements (64 when
+ * sh_dim > 10); scratch as for pxl_add_variance_loss. */
+int32_t pxl_tv_adam_step(const pxl_grid *grid, const pxl_optim *optim,
+                         double lambda_density, double lambda_sh,
+                         int32_t optimize_step, void *scratch,
+                         size_t scratch_bytes, double lr, double sh_lr,
+                         int32_t t, void *params_out, void *stream);
+
 /* Cell occupancy and in-cage flags from a dense raw-density plane
  * (objectScene::pruneSelf / pruneScene / occupiedVoxel,
  * SceneUtils.cu:800-913).  occupancy: ceil((W-1)(H-1)(D-1)/32) words.
