@@ -248,7 +248,9 @@ class UnevenWorkload:
         zero32 = torch.zeros_like(rec32)
         self.rec = pxl.records_to_16bit(zero32, pxl.STORE_BF16)
         self.grid = pxl.Grid(N, N, N, S, self.rec, storage=pxl.STORE_BF16, **grid_kw)
-        self.optim = pxl.Optim(self.grid, master_from=zero32 if args.bf16_master else None)
+        ostride = (P + 3) // 4 * 4
+        self.optim = pxl.Optim(self.grid, stride=ostride,
+                               master_from=zero32[:, :ostride] if args.bf16_master else None)
         del rec32
         self.trainer = pxl.Trainer(self.grid, self.optim, self.rays, self.order, batch_size=B,
                                    learning_rate=0.3, sh_learning_rate=1.0,
@@ -413,14 +415,18 @@ class Workload:
         if args.march == "adaptive":
             grid_kw = dict(grid_kw, march_mode=pxl.MARCH_REF_ADAPTIVE)
         self.rec = torch.zeros_like(gt_rec)
+        # Optimiser records of P floats (gradient, moments, master): the Adam
+        # pass streams no padding.
+        ostride = (self.P + 3) // 4 * 4
         if args.storage == "bf16":
             zero32 = self.rec
             self.rec = pxl.records_to_16bit(zero32, pxl.STORE_BF16)
             self.grid = pxl.Grid(N, N, N, S, self.rec, storage=pxl.STORE_BF16, **grid_kw)
-            self.optim = pxl.Optim(self.grid, master_from=zero32 if args.bf16_master else None)
+            self.optim = pxl.Optim(self.grid, stride=ostride,
+                                   master_from=zero32[:, :ostride] if args.bf16_master else None)
         else:
             self.grid = pxl.Grid(N, N, N, S, self.rec, **grid_kw)
-            self.optim = pxl.Optim(self.grid)
+            self.optim = pxl.Optim(self.grid, stride=ostride)
         # config_linux.txt defaults: LearningRate 0.3, SHLearningRate 1,
         # VarianceCoefficient 1e-5, ColorDisparityCoefficient 1e-6,
         # VarianceType 1, VarianceLossInterval 3, MaskRayCoefficient 1,
@@ -620,7 +626,8 @@ def fast_path(w, dev, args, plain_ms):
     zero32 = torch.zeros_like(w.rec)
     rec16 = pxl.records_to_16bit(zero32, pxl.STORE_BF16)
     grid = pxl.Grid(N, N, N, S, rec16, storage=pxl.STORE_BF16, **w.grid_kw)
-    optim = pxl.Optim(grid, master_from=zero32)
+    ostride = (w.P + 3) // 4 * 4
+    optim = pxl.Optim(grid, stride=ostride, master_from=zero32[:, :ostride])
     del zero32
     tr = pxl.Trainer(grid, optim, w.rays, w.order, batch_size=args.batch, learning_rate=0.3,
                      sh_learning_rate=1.0, variance_coefficient=1e-5, sh_coefficient=1e-6,
@@ -648,6 +655,7 @@ def fast_path(w, dev, args, plain_ms):
     end.record()
     torch.cuda.synchronize()
     ms = start.elapsed_time(end) / K
+    tr.sync()
     fp32 = pxl.Grid(N, N, N, S, optim.master, **w.grid_kw)
     q = proposal_quality(w, prop, dev, args, grid=grid, fp32_grid=fp32)
     out = {"config": "learned proposal + BF16 records (FP32 master), "
@@ -749,6 +757,9 @@ def main():
     ms = start.elapsed_time(end)
     kern_ms, nk, alone_ms, n_alone = w.trainer.render_ms_split()
     assert 0 < nk <= K
+    # The fused TV + Adam pass leaves the current records in either of two
+    # buffers; the measurements below read the grid's.
+    w.trainer.sync()
     ms_step = ms / K
     rays_per_s = args.batch * K / (ms / 1e3)
     mse, psnr = w.trainer.loss()

@@ -81,6 +81,12 @@ struct pxl_trainer {
     std::vector<hipEvent_t> ev;
     std::vector<char> ev_beside;
     int64_t ev_used;
+    // Fused TV + Adam: the caller's records (params_home) and a second,
+    // owned buffer of params_bytes; grid.params names the current one.
+    bool tv_fused;
+    float *params_home;
+    void *params_alt;
+    size_t params_bytes;
     // Sample-log workspace of the two-kernel fused step (owned).
     void *march_ws;
     size_t march_ws_bytes;
@@ -152,7 +158,22 @@ extern "C" pxl_trainer *pxl_trainer_create(const pxl_train_config *cfg, const px
         tr->side = nullptr;
         tr->ev_rendered = tr->ev_reduced = tr->ev_step = tr->ev_tv = nullptr;
     }
-    tr->tv_overlap = tr->side != nullptr && !cfg->tv_inline;
+    tr->params_home = grid->params;
+    tr->params_alt = nullptr;
+    tr->params_bytes = (size_t)grid->n_vox * (size_t)grid->stride *
+                       (grid->storage == PXL_STORE_F32 ? 4u : 2u);
+    const int32_t P = 1 + 3 * grid->sh_dim;
+    tr->tv_fused = !cfg->tv_separate && cfg->variance_type == PXL_TV_DISPARITY &&
+                   grid->stride <= (P > 32 ? 64 : 32) && tr->params_bytes > 0;
+    if (tr->tv_fused && hipMalloc(&tr->params_alt, tr->params_bytes) != hipSuccess) {
+        tr->params_alt = nullptr;
+        set_error("pxl_trainer_create: cannot allocate a second %zu-byte record buffer for the "
+                  "fused TV + Adam pass (tv_separate = 1 runs without it)",
+                  tr->params_bytes);
+        pxl_trainer_destroy(tr);
+        return nullptr;
+    }
+    tr->tv_overlap = tr->side != nullptr && !cfg->tv_inline && !tr->tv_fused;
     tr->tv_scratch = tv_scratch;
     tr->tv_scratch_bytes = tv_scratch_bytes;
     tr->batch_count = (n_rays + cfg->batch_size - 1) / cfg->batch_size;
@@ -476,16 +497,29 @@ extern "C" int32_t pxl_trainer_step(pxl_trainer *tr) {
         st = pxl_loss_reduce_f64(tr->losses, len, tr->loss_out, tr->stream);
         if (st != PXL_OK) return st;
     }
-    if (do_tv && !tv_side) {
-        st = RenderUtils::addVarianceLoss(tr->grid, tr->optim, c.variance_coefficient,
-                                          c.sh_coefficient, c.variance_type, c.neighbour_nums,
-                                          c.optimize_step, tr->tv_scratch,
-                                          tr->tv_scratch_bytes, tr->stream);
+    const int32_t time_step = (int32_t)(epoch * tr->batch_count + b + 1);
+    if (do_tv && tr->tv_fused) {
+        // One pass: the stepped records go to the other buffer, which then
+        // becomes the grid's.
+        void *other = tr->grid.params == tr->params_home ? tr->params_alt
+                                                         : (void *)tr->params_home;
+        st = pxl_tv_adam_step(&tr->grid, &tr->optim, c.variance_coefficient, c.sh_coefficient,
+                              c.optimize_step, tr->tv_scratch, tr->tv_scratch_bytes, tr->lr,
+                              tr->sh_lr, time_step, other, tr->stream);
+        if (st != PXL_OK) return st;
+        tr->grid.params = reinterpret_cast<float *>(other);
+    } else {
+        if (do_tv && !tv_side) {
+            st = RenderUtils::addVarianceLoss(tr->grid, tr->optim, c.variance_coefficient,
+                                              c.sh_coefficient, c.variance_type,
+                                              c.neighbour_nums, c.optimize_step, tr->tv_scratch,
+                                              tr->tv_scratch_bytes, tr->stream);
+            if (st != PXL_OK) return st;
+        }
+        st = Optimizer::updateRGBScene(tr->optim, tr->grid, tr->lr, tr->sh_lr, time_step,
+                                       tr->stream);
         if (st != PXL_OK) return st;
     }
-    const int32_t time_step = (int32_t)(epoch * tr->batch_count + b + 1);
-    st = Optimizer::updateRGBScene(tr->optim, tr->grid, tr->lr, tr->sh_lr, time_step, tr->stream);
-    if (st != PXL_OK) return st;
     // The caller's stream holds all of this batch's work, the loss
     // reduction included (it ran beside TV and Adam): the next render, a
     // synchronisation of the stream or a read of losses / loss_out after it
@@ -565,8 +599,28 @@ extern "C" int32_t pxl_trainer_state(const pxl_trainer *tr, int64_t *t, double *
     return PXL_OK;
 }
 
+extern "C" int32_t pxl_trainer_sync(pxl_trainer *tr) {
+    PXL_REQUIRE(tr != nullptr, "pxl_trainer_sync: null trainer");
+    clear_error();
+    if (tr->params_alt == nullptr || tr->grid.params == tr->params_home) return PXL_OK;
+    if (hipMemcpyAsync(tr->params_home, tr->grid.params, tr->params_bytes,
+                       hipMemcpyDeviceToDevice, tr->stream) != hipSuccess) {
+        set_error("pxl_trainer_sync: hipMemcpyAsync of %zu bytes failed", tr->params_bytes);
+        return PXL_EHIP;
+    }
+    tr->grid.params = tr->params_home;
+    return PXL_OK;
+}
+
+extern "C" int32_t pxl_trainer_grid(const pxl_trainer *tr, pxl_grid *out) {
+    PXL_REQUIRE(tr != nullptr && out != nullptr, "pxl_trainer_grid: null argument");
+    *out = tr->grid;
+    return PXL_OK;
+}
+
 extern "C" void pxl_trainer_destroy(pxl_trainer *tr) {
     if (!tr) return;
+    (void)pxl_trainer_sync(tr);
     for (hipEvent_t e : tr->ev) (void)hipEventDestroy(e);
     (void)hipStreamSynchronize(tr->stream);
     if (tr->side) {
@@ -575,6 +629,7 @@ extern "C" void pxl_trainer_destroy(pxl_trainer *tr) {
             (void)hipEventDestroy(e);
         (void)hipStreamDestroy(tr->side);
     }
+    if (tr->params_alt) (void)hipFree(tr->params_alt);
     if (tr->march_ws) (void)hipFree(tr->march_ws);
     if (tr->order_buf) (void)hipFree(tr->order_buf);
     if (tr->order_ws) (void)hipFree(tr->order_ws);
@@ -648,7 +703,9 @@ extern "C" int32_t pxl_trainer_learn_level(pxl_trainer *tr, const pxl_level_conf
             }
         }
     }
-    return PXL_OK;
+    // The level's records end in the caller's buffer (upsampling and
+    // export read them there).
+    return pxl_trainer_sync(tr);
 }
 
 extern "C" int32_t pxl_hierarchy_plan(int32_t hierarchy, int32_t minimum_hierarchy,

@@ -44,6 +44,7 @@ EXPORTS = (
     "pxl_rays_permute", "pxl_shuffle_rank", "pxl_read_cameras", "pxl_read_image_text",
     "pxl_grid_save_text", "pxl_grid_text_records", "pxl_grid_load_text",
     "pxl_set_option", "pxl_get_option", "pxl_trainer_render_ms_split", "pxl_optim_init_band",
+    "pxl_trainer_sync", "pxl_trainer_grid",
     "pxl_trainer_set_band", "pxl_trainer_learn_level", "pxl_hierarchy_plan",
     "pxl_train_rays_noalloc",
 )
@@ -134,7 +135,7 @@ class TrainConfigC(ctypes.Structure):
                 ("mask_coefficient", _F64), ("huber_delta", _F32), ("fused", _I32),
                 ("log_samples", _I32), ("autotune", _I32), ("batch_order", _I32),
                 ("waves_per_block", _I32), ("persistent", _I32), ("chunked", _I32),
-                ("tv_inline", _I32), ("reserved", _I32)]
+                ("tv_inline", _I32), ("tv_separate", _I32)]
 
 
 class LevelConfigC(ctypes.Structure):
@@ -188,6 +189,8 @@ def _declare(L):
                                 _SZ, _P], _P),
         "pxl_tv_prepare": ([G, _P, _SZ, _P], _I32),
         "pxl_trainer_step": ([_P], _I32),
+        "pxl_trainer_sync": ([_P], _I32),
+        "pxl_trainer_grid": ([_P, G], _I32),
         "pxl_trainer_time_render": ([_P, _I32], _I32),
         "pxl_trainer_render_ms": ([_P, ctypes.POINTER(_F64), ctypes.POINTER(_I64)], _I32),
         "pxl_trainer_loss": ([_P, ctypes.POINTER(_F64), ctypes.POINTER(_F64)], _I32),
@@ -871,7 +874,7 @@ class Trainer:
                  variance_interval=3, optimize_step=0, mask_coefficient=1.0, huber_delta=0.1,
                  fused=True, log_samples=0, autotune=False, batch_order=0,
                  waves_per_block=0, persistent=False, chunked=False, tv_inline=False,
-                 stream=None):
+                 tv_separate=False, stream=None):
         import torch
 
         self._keep = (grid, optim, rays, order)
@@ -882,7 +885,8 @@ class Trainer:
                            sh_coefficient, variance_type, neighbour_nums, variance_interval,
                            optimize_step, mask_coefficient, huber_delta, int(fused),
                            int(log_samples), int(autotune), int(batch_order),
-                           int(waves_per_block), int(persistent), int(chunked), int(tv_inline), 0)
+                           int(waves_per_block), int(persistent), int(chunked), int(tv_inline),
+                           int(tv_separate))
         self._g, self._o, self._r = grid.c(), optim.c(), rays.c()
         n_rays = order.numel() if n_rays is None else n_rays
         sc = tv_scratch(grid, stream) if variance_type in (TV_DISPARITY, TV_BOTH) else None
@@ -897,6 +901,18 @@ class Trainer:
 
     def step(self):
         _check("pxl_trainer_step", lib().pxl_trainer_step(self._h))
+
+    def sync(self):
+        """Bring the current records into grid.params (the fused TV + Adam
+        pass alternates them with a second, trainer-owned buffer); call it
+        before reading grid.params between steps."""
+        _check("pxl_trainer_sync", lib().pxl_trainer_sync(self._h))
+
+    def grid_c(self):
+        """The trainer's grid descriptor (GridC), current records included."""
+        g = GridC()
+        _check("pxl_trainer_grid", lib().pxl_trainer_grid(self._h, ctypes.byref(g)))
+        return g
 
     def time_render(self, max_steps):
         _check("pxl_trainer_time_render", lib().pxl_trainer_time_render(self._h, max_steps))

@@ -691,7 +691,11 @@ typedef struct pxl_train_config {
                                 (pxl_schedule), not with persistent */
     int32_t tv_inline;       /* 1: the variance loss runs on the trainer's
                                 stream after the render (no overlap) */
-    int32_t reserved;
+    int32_t tv_separate;     /* 1: the disparity variance loss and Adam run
+                                as two passes; 0: one fused pass
+                                (pxl_tv_adam_step) on the batches that take
+                                the variance loss, the trainer keeping a
+                                second record buffer (pxl_trainer_sync) */
 } pxl_train_config;
 
 typedef struct pxl_trainer pxl_trainer;
@@ -709,12 +713,14 @@ pxl_trainer *pxl_trainer_create(const pxl_train_config *cfg, const pxl_grid *gri
 
 /* One batch: render + backward (fused or the two reference launches), FP64
  * batch loss, variance loss (every batch when dense, every
- * variance_interval-th when sparse, Render.cu:469), Adam with time step
+ * variance_interval-th when sparse, Render.cu:469) fused with Adam or
+ * before it (tv_separate), Adam with time step
  * epoch*batchCount + batch + 1, LR decay lr /= 2^(1/(batchCount (epoch+1)))
- * (Render.cu:499-500).  Asynchronous.  With the logged step, once its
- * schedule is chosen and without the proposal, the variance loss runs on a
- * second stream beside the forward kernel and the gradient kernels wait for
- * it (the same sum into the gradient planes in another order); tv_inline
+ * (Render.cu:499-500).  Asynchronous.  A variance loss not fused with Adam:
+ * with the logged step, once its schedule is chosen and without the
+ * proposal, it runs on a second stream beside the forward kernel and the
+ * gradient kernels wait for it (the same sum into the gradient planes in
+ * another order); tv_inline
  * keeps it after the render on the trainer's stream.  The overlap is used
  * only where the render runs as the two-kernel logged step (the fixed rule
  * or the reference's adaptive rule, sh_dim 1, 4 or 9, the eight-sample
@@ -722,6 +728,20 @@ pxl_trainer *pxl_trainer_create(const pxl_train_config *cfg, const pxl_grid *gri
  * trainer's stream waits for the reduction before the call returns its
  * work, so synchronising that stream makes losses and loss_out readable. */
 int32_t pxl_trainer_step(pxl_trainer *tr);
+
+/* With the fused TV + Adam pass (tv_separate 0, variance type
+ * PXL_TV_DISPARITY, records of at most 32 elements, 64 when sh_dim > 10)
+ * the trainer allocates a second record buffer at creation and the current
+ * records alternate between it and the caller's grid->params after each
+ * regulariser batch.  pxl_trainer_sync copies the current records into the
+ * caller's buffer when they are in the other one (on the trainer's stream)
+ * and makes the caller's current again; pxl_trainer_grid writes the
+ * trainer's grid descriptor, current records included, to *out.  The
+ * trainer syncs at the end of pxl_trainer_learn_level and in
+ * pxl_trainer_destroy.  No counterpart in the reference, whose Adam steps
+ * its planes in place. */
+int32_t pxl_trainer_sync(pxl_trainer *tr);
+int32_t pxl_trainer_grid(const pxl_trainer *tr, pxl_grid *out);
 
 /* Record HIP events around the render launch of the next max_steps
  * batches; pxl_trainer_render_ms returns the average duration and count of

@@ -324,13 +324,16 @@ def test_nan_parameter(sc, kind, path):
     assert math.isnan(out[0].item())
 
 
-@pytest.mark.parametrize("case", ["adaptive", "sh16", "caged", "caged-inline"])
+@pytest.mark.parametrize("case", ["adaptive", "sh16", "caged", "caged-inline", "caged-fused",
+                                  "adaptive-fused"])
 def test_trainer_overlap_matches_stage_calls(sc, case):
     """The trainer with the logged step and the variance loss on its second
     stream equals the stage calls (render, variance loss, Adam per batch):
     on the adaptive rule and SH degree 3 (whose render writes gradients in
     the forward kernel, so the variance loss must not run beside it) and on
-    a caged grid with an interval of 3 (side stream on, and off)."""
+    a caged grid with an interval of 3 (side stream on, and off); and with
+    the variance loss fused into Adam (caged, adaptive), read back through
+    pxl_trainer_sync."""
     import torch
 
     dev = sc["dev"]
@@ -340,7 +343,7 @@ def test_trainer_overlap_matches_stage_calls(sc, case):
         S16 = 16
         planes = c1_planes(N, S16, n_blobs=8, seed=31)
         mk = lambda: pxl.Grid(N, N, N, S16, to_records(planes, dev), step=0.5)
-    elif case == "adaptive":
+    elif case.startswith("adaptive"):
         mk = lambda: _grids(sc, "dense", march=pxl.MARCH_REF_ADAPTIVE)[0]
     else:
         mk = lambda: _grids(sc, "caged")[0]
@@ -350,9 +353,11 @@ def test_trainer_overlap_matches_stage_calls(sc, case):
     kw = dict(learning_rate=0.1, sh_learning_rate=0.2, variance_coefficient=1e-3,
               sh_coefficient=1e-4, variance_interval=3)
     tr = pxl.Trainer(g0, o0, _rays(sc), order, batch_size=B, fused=True, log_samples=512,
-                     tv_inline=case == "caged-inline", **kw)
+                     tv_inline=case == "caged-inline", tv_separate=not case.endswith("fused"),
+                     **kw)
     for _ in range(steps):
         tr.step()
+    tr.sync()
     rays = _rays(sc)
     losses = torch.zeros(B, dtype=torch.float64, device=dev)
     ws = pxl.train_workspace(B, 512, dev)

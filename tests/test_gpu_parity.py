@@ -593,21 +593,24 @@ def test_train_step_c1(c1):
     assert diff.max() <= 0.2 + 1e-6
 
 
-@pytest.mark.parametrize("log_samples,autotune,tv_overlap",
-                         [(0, False, True), (512, False, True), (512, True, True),
-                          (512, False, False)])
-def test_trainer_matches_stage_calls(c1, log_samples, autotune, tv_overlap):
+@pytest.mark.parametrize("log_samples,autotune,tv_mode",
+                         [(0, False, "fused"), (512, False, "fused"), (512, True, "overlap"),
+                          (512, False, "overlap"), (512, False, "inline")])
+def test_trainer_matches_stage_calls(c1, log_samples, autotune, tv_mode):
     """The C++ batch loop (pxl_trainer_step) issues the same stages as the
     Python-driven sequence: dense grid (TV every batch), 3 batches, LR decay
     lr /= 2^(1/(batchCount (epoch+1))) per batch (Render.cu:499-500); with
-    log_samples the trainer runs the two-kernel logged step, with TV on its
-    second stream beside the forward unless tv_inline."""
+    log_samples the trainer runs the two-kernel logged step.  TV modes: fused
+    with Adam (the trainer alternates two record buffers: after 3 batches the
+    current records are in its own, pxl_trainer_sync brings them back), or
+    separate, on the second stream beside the forward (overlap) or after the
+    render (inline)."""
     import math
 
     import torch
 
     say(f"[trainer] 3 batches vs stage calls, log_samples={log_samples}, autotune={autotune}, "
-        f"tv_overlap={tv_overlap}")
+        f"tv {tv_mode}")
     dev = c1["dev"]
     B = 1024
     order = c1["si"]
@@ -623,9 +626,14 @@ def test_trainer_matches_stage_calls(c1, log_samples, autotune, tv_overlap):
     tr = pxl.Trainer(grids[0], optims[0], rays_a, order, batch_size=B, learning_rate=0.1,
                      sh_learning_rate=0.2, variance_coefficient=1e-5, sh_coefficient=1e-6,
                      variance_interval=3, fused=True, log_samples=log_samples,
-                     autotune=autotune, tv_inline=not tv_overlap)
+                     autotune=autotune, tv_inline=tv_mode == "inline",
+                     tv_separate=tv_mode != "fused")
     for _ in range(3):
         tr.step()
+    current = tr.grid_c().params
+    assert (current != grids[0].params.data_ptr()) == (tv_mode == "fused")
+    tr.sync()
+    assert tr.grid_c().params == grids[0].params.data_ptr()
     losses = torch.zeros(B, dtype=torch.float64, device=dev)
     lr, sl = 0.1, 0.2
     bc = (n + B - 1) // B
