@@ -321,17 +321,76 @@ __device__ __forceinline__ double tv_interior_sum(const TvArgs &a, int64_t r, in
     return 7.0 * (double)v[13] - (0.5 * f + 0.25 * e + 0.125 * c);
 }
 
+// Record of the b-th boundary vertex of a dense grid (W, H, D >= 3): the
+// planes z = 0 and z = D-1, then the rows y = 0 and y = H-1 of the planes
+// between, then the columns x = 0 and x = W-1 of the rows between.
+__device__ __forceinline__ int64_t tv_boundary_record(const TvArgs &a, int64_t b) {
+    const int64_t W = a.W, H = a.H, D = a.D;
+    const int64_t fz = W * H, fy = W * (D - 2), fx = (H - 2) * (D - 2);
+    if (b < 2 * fz) return (b < fz ? 0 : (D - 1) * fz) + b % fz;
+    b -= 2 * fz;
+    if (b < 2 * fy) {
+        const int64_t y = b < fy ? 0 : H - 1, e = b % fy;
+        return (1 + e / W) * fz + y * W + e % W;
+    }
+    b -= 2 * fy;
+    const int64_t x = b < fx ? 0 : W - 1, e = b % fx;
+    return (1 + e / (H - 2)) * fz + (1 + e % (H - 2)) * W + x;
+}
+
+// One vertex of addVarianceLossGPU's gather (item: a record, or with
+// boundary_only the item-th boundary vertex) by its group of LPV lanes.
+template <int LPV, int STORE, bool OFF32, bool FUSED>
+__device__ __forceinline__ void tv_general_item(const TvArgs &a, const AdamArgs &ad, int64_t item,
+                                                int g, int p, int boundary_only) {
+    const int64_t r = boundary_only ? tv_boundary_record(a, item) : item;
+    const float lam = tv_lambda(a, p);
+    const bool mine = p < a.P && lam != 0.0f;
+    // A lane without a TV term reads element 0 (the line is read anyway).
+    const int q = mine ? p : 0;
+    AdamElem ae{0.0f, 0.0f, 0.0f, 0.0f};
+    if (FUSED && p < a.stride) ae = adam_load<STORE>(a, ad, r, p);
+    bool interior = false;
+    if (!boundary_only) {
+        const uint32_t d = (uint32_t)r;
+        const uint32_t x = d % (uint32_t)a.W, y = (d / (uint32_t)a.W) % (uint32_t)a.H;
+        const uint32_t z = d / ((uint32_t)a.W * (uint32_t)a.H);
+        interior = x - 1u < (uint32_t)a.W - 2u && y - 1u < (uint32_t)a.H - 2u &&
+                   z - 1u < (uint32_t)a.D - 2u;
+    }
+    double tsum = 0.0;
+    bool tvl = false;
+    if (interior) {
+        tsum = tv_interior_sum<STORE, OFF32>(a, r, q);
+        tvl = mine;
+    } else {
+        uint32_t cmask;
+        const uint32_t nrec = tv_neighbours<LPV>(a, r, p, cmask);
+        // No occupied cell around the vertex: no TV term.
+        if (cmask != 0u) {
+            tsum = tv_record_sum<LPV, STORE>(a, r, q, nrec, cmask);
+            tvl = mine;
+        }
+    }
+    if (!FUSED) {
+        if (tvl) a.grad[(size_t)r * (size_t)a.gstride + p] += (float)((double)lam * tsum);
+        return;
+    }
+    if (p < a.stride) adam_store<STORE>(a, ad, r, p, ae, tvl, lam, tsum);
+}
+
 // Interior vertices of a dense grid, marched along z.  With w(0) = 2 and
 // w(+-1) = 1, m_i = w(dx) w(dy) w(dz), so the interior gradient is
 //   8 u - B / 8,  B = sum_z' w(z' - z) Bxy(z'),  Bxy(z') = sum w(dx) w(dy) u
 // over the 3 x 3 vertices around (x, y) in plane z'.  One group of LPV lanes
 // (one per parameter) walks column (x, y) over kTvChunk vertices, keeping
 // Bxy of the planes below and at z: 9 loads per vertex instead of 27.
-// Columns are numbered x fastest, then y, then the z chunk.  Loads go
+// Columns are numbered x fastest, then y, then the z chunk; the blocks past
+// the columns' gather the boundary vertices in the same launch.  Loads go
 // through a buffer resource (the records span less than 4 GiB): lane offset
 // in a vector register, neighbour offset in a scalar one.  FUSED: each
 // vertex's record then takes its Adam step into ad.out.
-constexpr int kTvChunk = 24;
+constexpr int kTvChunk = 128;
 
 __device__ __forceinline__ double tv_blur_xy(const float (&v)[9]) {
     const double e = ((double)v[1] + (double)v[3]) + ((double)v[5] + (double)v[7]);
@@ -341,11 +400,20 @@ __device__ __forceinline__ double tv_blur_xy(const float (&v)[9]) {
 
 template <int LPV, int STORE, bool FUSED>
 __global__ __launch_bounds__(256) void k_tv_zmarch(const TvArgs a, const AdamArgs ad,
-                                                   int64_t n_cols) {
+                                                   int64_t n_cols, int64_t col_blocks,
+                                                   int64_t n_boundary) {
     constexpr int kGroups = 256 / LPV;
     constexpr uint32_t kElem = STORE == PXL_STORE_F32 ? 4u : 2u;
     const int g = threadIdx.x / LPV, p = threadIdx.x % LPV;
-    const int64_t col = (int64_t)xcd_slice_block(blockIdx.x, gridDim.x) * kGroups + g;
+    if ((int64_t)blockIdx.x >= col_blocks) {
+        // The blocks past the columns' (dispatched after them) gather the
+        // boundary vertices.
+        const int64_t item = ((int64_t)blockIdx.x - col_blocks) * kGroups + g;
+        if (item < n_boundary) tv_general_item<LPV, STORE, true, FUSED>(a, ad, item, g, p, 1);
+        return;
+    }
+    // The column blocks alone are sliced over the XCDs.
+    const int64_t col = (int64_t)xcd_slice_block(blockIdx.x, (unsigned)col_blocks) * kGroups + g;
     if (col >= n_cols) return;
     const float lam = tv_lambda(a, p);
     const bool tvl = p < a.P && lam != 0.0f;
@@ -426,70 +494,20 @@ __global__ __launch_bounds__(256) void k_tv_zmarch(const TvArgs a, const AdamArg
     if (s + 1 < z1) step(1, s + 1, false);
 }
 
-// Record of the b-th boundary vertex of a dense grid (W, H, D >= 3): the
-// planes z = 0 and z = D-1, then the rows y = 0 and y = H-1 of the planes
-// between, then the columns x = 0 and x = W-1 of the rows between.
-__device__ __forceinline__ int64_t tv_boundary_record(const TvArgs &a, int64_t b) {
-    const int64_t W = a.W, H = a.H, D = a.D;
-    const int64_t fz = W * H, fy = W * (D - 2), fx = (H - 2) * (D - 2);
-    if (b < 2 * fz) return (b < fz ? 0 : (D - 1) * fz) + b % fz;
-    b -= 2 * fz;
-    if (b < 2 * fy) {
-        const int64_t y = b < fy ? 0 : H - 1, e = b % fy;
-        return (1 + e / W) * fz + y * W + e % W;
-    }
-    b -= 2 * fy;
-    const int64_t x = b < fx ? 0 : W - 1, e = b % fx;
-    return (1 + e / (H - 2)) * fz + (1 + e % (H - 2)) * W + x;
-}
-
 // addVarianceLossGPU as a gather, looking the neighbours up on the fly
-// (dense grids, where a record's neighbours are arithmetic).  All vertices
-// (n_items = n_vox): interior ones take tv_interior_sum, the rest the
-// general gather.  boundary_only (n_items boundary vertices,
-// tv_boundary_record): the general gather for the vertices k_tv_zmarch
-// leaves.  FUSED: the group then runs the Adam step of its record.
+// (dense grids, where a record's neighbours are arithmetic), over all
+// n_items = n_vox vertices: interior ones take tv_interior_sum, the rest the
+// general gather (grids k_tv_zmarch does not serve: 64-bit offsets, or a
+// side shorter than 3).  FUSED: the group then runs the Adam step of its
+// record.
 template <int LPV, int STORE, bool OFF32, bool FUSED>
 __global__ __launch_bounds__(256) void k_tv_disparity(const TvArgs a, const AdamArgs ad,
-                                                      int64_t n_items, int boundary_only) {
+                                                      int64_t n_items) {
     constexpr int kGroups = 256 / LPV;
     const int g = threadIdx.x / LPV, p = threadIdx.x % LPV;
     const int64_t item = (int64_t)xcd_slice_block(blockIdx.x, gridDim.x) * kGroups + g;
     if (item >= n_items) return;  // the whole group leaves together
-    const int64_t r = boundary_only ? tv_boundary_record(a, item) : item;
-    const float lam = tv_lambda(a, p);
-    const bool mine = p < a.P && lam != 0.0f;
-    // A lane without a TV term reads element 0 (the line is read anyway).
-    const int q = mine ? p : 0;
-    AdamElem ae{0.0f, 0.0f, 0.0f, 0.0f};
-    if (FUSED && p < a.stride) ae = adam_load<STORE>(a, ad, r, p);
-    bool interior = false;
-    if (!boundary_only) {
-        const uint32_t d = (uint32_t)r;
-        const uint32_t x = d % (uint32_t)a.W, y = (d / (uint32_t)a.W) % (uint32_t)a.H;
-        const uint32_t z = d / ((uint32_t)a.W * (uint32_t)a.H);
-        interior = x - 1u < (uint32_t)a.W - 2u && y - 1u < (uint32_t)a.H - 2u &&
-                   z - 1u < (uint32_t)a.D - 2u;
-    }
-    double tsum = 0.0;
-    bool tvl = false;
-    if (interior) {
-        tsum = tv_interior_sum<STORE, OFF32>(a, r, q);
-        tvl = mine;
-    } else {
-        uint32_t cmask;
-        const uint32_t nrec = tv_neighbours<LPV>(a, r, p, cmask);
-        // No occupied cell around the vertex: no TV term.
-        if (cmask != 0u) {
-            tsum = tv_record_sum<LPV, STORE>(a, r, q, nrec, cmask);
-            tvl = mine;
-        }
-    }
-    if (!FUSED) {
-        if (tvl) a.grad[(size_t)r * (size_t)a.gstride + p] += (float)((double)lam * tsum);
-        return;
-    }
-    if (p < a.stride) adam_store<STORE>(a, ad, r, p, ae, tvl, lam, tsum);
+    tv_general_item<LPV, STORE, OFF32, FUSED>(a, ad, item, g, p, 0);
 }
 
 // Caged grids: the neighbour records and the cell mask depend only on the
@@ -1066,36 +1084,36 @@ void tv_launch_as(const TvArgs &a, const AdamArgs &ad, void *scratch, hipStream_
     constexpr uint64_t kElem = STORE == PXL_STORE_F32 ? 4u : 2u;
     const bool off32 = !offsets64_forced() &&
                        (uint64_t)a.n_vox * (uint64_t)a.stride * kElem <= 0xFFFFFFFFull;
-    // 32-bit offsets: interior vertices marched along z, the rest gathered
-    // one by one; otherwise every vertex gathered.
-    const int skip = off32 && a.W >= 3 && a.H >= 3 && a.D >= 3;
-    if (skip) {
+    // 32-bit offsets: interior vertices marched along z and the rest
+    // gathered one by one, in one launch; otherwise every vertex gathered.
+    if (off32 && a.W >= 3 && a.H >= 3 && a.D >= 3) {
         const int64_t n_cols =
             (int64_t)(a.W - 2) * (a.H - 2) * ((a.D - 2 + kTvChunk - 1) / kTvChunk);
-        const unsigned cb = (unsigned)((n_cols + groups - 1) / groups);
+        const int64_t cb = (n_cols + groups - 1) / groups;
+        const int64_t nbd = 2 * ((int64_t)a.W * a.H + (int64_t)a.W * (a.D - 2) +
+                                 (int64_t)(a.H - 2) * (a.D - 2));
+        const unsigned nb = (unsigned)(cb + (nbd + groups - 1) / groups);
         if (wide)
-            hipLaunchKernelGGL((k_tv_zmarch<64, STORE, FUSED>), dim3(cb), dim3(256), 0, s, a, ad,
-                               n_cols);
+            hipLaunchKernelGGL((k_tv_zmarch<64, STORE, FUSED>), dim3(nb), dim3(256), 0, s, a, ad,
+                               n_cols, cb, nbd);
         else
-            hipLaunchKernelGGL((k_tv_zmarch<32, STORE, FUSED>), dim3(cb), dim3(256), 0, s, a, ad,
-                               n_cols);
+            hipLaunchKernelGGL((k_tv_zmarch<32, STORE, FUSED>), dim3(nb), dim3(256), 0, s, a, ad,
+                               n_cols, cb, nbd);
+        return;
     }
-    const int64_t items = skip ? 2 * ((int64_t)a.W * a.H + (int64_t)a.W * (a.D - 2) +
-                                      (int64_t)(a.H - 2) * (a.D - 2))
-                               : a.n_vox;
-    const unsigned ib = (unsigned)((items + groups - 1) / groups);
+    const unsigned ib = (unsigned)((a.n_vox + groups - 1) / groups);
     if (wide && off32)
         hipLaunchKernelGGL((k_tv_disparity<64, STORE, true, FUSED>), dim3(ib), dim3(256), 0, s, a,
-                           ad, items, skip);
+                           ad, a.n_vox);
     else if (wide)
         hipLaunchKernelGGL((k_tv_disparity<64, STORE, false, FUSED>), dim3(ib), dim3(256), 0, s,
-                           a, ad, items, 0);
+                           a, ad, a.n_vox);
     else if (off32)
         hipLaunchKernelGGL((k_tv_disparity<32, STORE, true, FUSED>), dim3(ib), dim3(256), 0, s, a,
-                           ad, items, skip);
+                           ad, a.n_vox);
     else
         hipLaunchKernelGGL((k_tv_disparity<32, STORE, false, FUSED>), dim3(ib), dim3(256), 0, s,
-                           a, ad, items, 0);
+                           a, ad, a.n_vox);
 }
 
 template <int STORE>
