@@ -59,6 +59,9 @@ def parse():
                     help="1: the trainer times every batch order x schedule on the first "
                          "(untimed) batch and keeps the fastest; 0: given order, 4 waves per "
                          "block")
+    ap.add_argument("--optim-stride", type=int, default=0,
+                    help="floats per optimiser record (gradient, moments, master); 0: P rounded "
+                         "up to 4 (28 for SH-2), which streams no padding")
     ap.add_argument("--bf16-master", type=int, default=1,
                     help="with --storage bf16: Adam updates an FP32 master copy (1) or the "
                          "widened BF16 values themselves (0)")
@@ -248,7 +251,7 @@ class UnevenWorkload:
         zero32 = torch.zeros_like(rec32)
         self.rec = pxl.records_to_16bit(zero32, pxl.STORE_BF16)
         self.grid = pxl.Grid(N, N, N, S, self.rec, storage=pxl.STORE_BF16, **grid_kw)
-        ostride = (P + 3) // 4 * 4
+        ostride = args.optim_stride or (P + 3) // 4 * 4
         self.optim = pxl.Optim(self.grid, stride=ostride,
                                master_from=zero32[:, :ostride] if args.bf16_master else None)
         del rec32
@@ -417,7 +420,7 @@ class Workload:
         self.rec = torch.zeros_like(gt_rec)
         # Optimiser records of P floats (gradient, moments, master): the Adam
         # pass streams no padding.
-        ostride = (self.P + 3) // 4 * 4
+        ostride = args.optim_stride or (self.P + 3) // 4 * 4
         if args.storage == "bf16":
             zero32 = self.rec
             self.rec = pxl.records_to_16bit(zero32, pxl.STORE_BF16)
@@ -626,7 +629,7 @@ def fast_path(w, dev, args, plain_ms):
     zero32 = torch.zeros_like(w.rec)
     rec16 = pxl.records_to_16bit(zero32, pxl.STORE_BF16)
     grid = pxl.Grid(N, N, N, S, rec16, storage=pxl.STORE_BF16, **w.grid_kw)
-    ostride = (w.P + 3) // 4 * 4
+    ostride = args.optim_stride or (w.P + 3) // 4 * 4
     optim = pxl.Optim(grid, stride=ostride, master_from=zero32[:, :ostride])
     del zero32
     tr = pxl.Trainer(grid, optim, w.rays, w.order, batch_size=args.batch, learning_rate=0.3,

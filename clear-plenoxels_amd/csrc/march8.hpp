@@ -210,6 +210,168 @@ __device__ __forceinline__ void flush_record0(const RenderArgs &a, double acc0, 
     else unsafeAtomicAdd(a.grad + pp, (float)v);
 }
 
+// Gradient walk of one ray's logged samples (log_grad_ray).  Lane (hf, ch,
+// j) = lane bits (5, 4..3, 2..0) holds, for corner j of the current cell and
+// channel ch (density, R, G, B), half hf's sum of weight x channel
+// coefficient over the samples in the cell (the halves take alternate
+// samples of a run).  A chunk's samples are first staged in LDS, kWalkRows
+// at a time: per sample row its 32 (corner, channel) products, written by
+// the sample's lane, and its 8 corner records; the walk then sums row after
+// row with one LDS read per lane, and on a cell change flushes the corners
+// the new cell does not share (parameter p of a corner takes its channel's
+// sum times Y_k) and moves the shared ones (retire).
+constexpr int kWalkRows = 32;
+constexpr int kWalkStride = 33;  // floats per row: the sample lanes' writes hit distinct banks
+constexpr size_t kWalkLdsBytes = (size_t)kWalkRows * kWalkStride * 4 + (size_t)kWalkRows * 8 * 4;
+
+// A cell change by d = (dx, dy, dz) (each clamped to [-2, 2]; 2, 2, 2: the
+// end of the walk) for the gradient walk: bits 0-7 the old corners that
+// leave (j - d outside {0,1}^3), bits 8-15 the new corners taken from an
+// old one (j + d inside {0,1}^3), bits 16-20 the lane offset dx + 2 dy +
+// 4 dz + 16 of that old corner.
+__device__ __forceinline__ uint32_t cell_move(int dx, int dy, int dz) {
+    const auto in01 = [](int v) { return v == 0 || v == 1; };
+    uint32_t m = (uint32_t)(dx + 2 * dy + 4 * dz + 16) << 16;
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+        const int cx = c & 1, cy = (c >> 1) & 1, cz = c >> 2;
+        if (!(in01(cx - dx) && in01(cy - dy) && in01(cz - dz))) m |= 1u << c;
+        if (in01(cx + dx) && in01(cy + dy) && in01(cz + dz)) m |= 1u << (8 + c);
+    }
+    return m;
+}
+
+struct GradWalk {
+    float *q;      // [kWalkRows][kWalkStride]
+    uint32_t *r;   // [kWalkRows][8]
+    float acc;     // this lane's half sum for (corner, channel)
+    uint32_t rec;  // record of this lane's corner of the current cell
+    bool has_cur;
+    int cx, cy, cz;  // the current cell (wave-uniform)
+
+    __device__ __forceinline__ void init(float *lq, uint32_t *lr) {
+        q = lq;
+        r = lr;
+        acc = 0.0f;
+        rec = 0u;
+        has_cur = false;
+        cx = cy = cz = 0;
+    }
+
+    // Leaves the current cell for one displaced by d: move = cell_move(d)
+    // (wave-uniform).  Old corner j stays as corner j - d of the new cell
+    // when that is in {0,1}^3; the others (bits of rmask) are added to the
+    // gradient, two corners per pass (half hf takes the pair's corner hf,
+    // lane pp its parameter).  Record 0 (where every corner outside the cage
+    // of a caged grid lands, SceneUtils.cu:1106-1113) collects into acc0,
+    // added once per ray (flush_record0).
+    __device__ __forceinline__ void retire(const RenderArgs &a, uint32_t move, int cj, int cch,
+                                           int hf, int pp, int pch, bool live, float ysel, int P,
+                                           double &acc0) {
+        const float tot = acc + __shfl_xor(acc, 32, 64);
+        uint32_t rmask = move & 0xFFu;
+        while (rmask != 0u) {
+            const int ja = __builtin_ctz(rmask);
+            rmask &= rmask - 1u;
+            const int jb = rmask != 0u ? __builtin_ctz(rmask) : -1;
+            if (jb >= 0) rmask &= rmask - 1u;
+            const int jsel = hf ? jb : ja;
+            const float v = __shfl(tot, (jsel < 0 ? 0 : jsel) + 8 * pch, 64);
+            const uint32_t ra = (uint32_t)__builtin_amdgcn_readlane((int)rec, ja);
+            const uint32_t rb = (uint32_t)__builtin_amdgcn_readlane((int)rec, jb < 0 ? ja : jb);
+            const uint32_t rc = hf ? rb : ra;
+            if (jsel >= 0 && pp < P) {
+                const float val = pp == 0 ? v : v * ysel;
+                if (rc == 0u) acc0 += (double)val;
+                else if (live) unsafeAtomicAdd(a.grad + (size_t)rc * (size_t)a.gstride + pp, val);
+            }
+        }
+        // New corner cj is old corner cj + d (lane cj + dx + 2 dy + 4 dz)
+        // when bit cj of the keep mask is set.
+        const bool ok = (move >> (8 + cj)) & 1u;
+        const int src = cj + (int)((move >> 16) & 31u) - 16 + 8 * cch;
+        const float mv = __shfl(tot, ok ? src : 0, 64);
+        acc = (ok && hf == 0) ? mv : 0.0f;
+    }
+
+    // The m (<= 64) samples of one chunk, sample i in lane i: its cell, the
+    // fractions within it, the four channel coefficients and its 8 corner
+    // records (q0, q1).
+    __device__ __forceinline__ void walk(const RenderArgs &a, int lane, int m, bool act, int bx,
+                                         int by, int bz, float fx, float fy, float fz, float gs,
+                                         float g0, float g1, float g2, const uint4 &q0,
+                                         const uint4 &q1, int cj, int cch, int hf, int pp,
+                                         int pch, bool live, float ysel, int P, double &acc0) {
+        // A sample starts a new cell when its cell differs from the previous
+        // sample's (for the chunk's first, the cell carried in).
+        int ux = __shfl_up(bx, 1, 64), uy = __shfl_up(by, 1, 64), uz = __shfl_up(bz, 1, 64);
+        bool prev = true;
+        if (lane == 0) {
+            ux = cx;
+            uy = cy;
+            uz = cz;
+            prev = has_cur;
+        }
+        const bool changed = act && !(prev && bx == ux && by == uy && bz == uz);
+        const uint32_t move = cell_move(clampi(bx - ux, -2, 2), clampi(by - uy, -2, 2),
+                                        clampi(bz - uz, -2, 2));
+        const uint64_t chg = __ballot(changed);
+        float wj[8];
+        {
+            const float x0 = 1.0f - fx, y0 = 1.0f - fy, z0 = 1.0f - fz;
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                const float wyz = ((j >> 1) & 1 ? fy : y0) * ((j >> 2) ? fz : z0);
+                wj[j] = ((j & 1) ? fx : x0) * wyz;
+            }
+        }
+        const float cc[4] = {gs, g0, g1, g2};
+        for (int base = 0; base < m; base += kWalkRows) {
+            const int n = min(kWalkRows, m - base);
+            if (act && lane >= base && lane < base + kWalkRows) {
+                const int row = lane - base;
+#pragma unroll
+                for (int ch = 0; ch < 4; ch++)
+#pragma unroll
+                    for (int j = 0; j < 8; j++) q[row * kWalkStride + 8 * ch + j] = wj[j] * cc[ch];
+                *reinterpret_cast<uint4 *>(r + row * 8) = q0;
+                *reinterpret_cast<uint4 *>(r + row * 8 + 4) = q1;
+            }
+            // The rows are read by other lanes of this wave.
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const uint64_t rc = (chg >> base) & ((1ull << n) - 1ull);
+            int s = 0;
+            while (s < n) {
+                if ((rc >> s) & 1ull) {
+                    if (has_cur)
+                        retire(a, (uint32_t)__builtin_amdgcn_readlane((int)move, base + s), cj,
+                               cch, hf, pp, pch, live, ysel, P, acc0);
+                    has_cur = true;
+                    cx = __builtin_amdgcn_readlane(bx, base + s);
+                    cy = __builtin_amdgcn_readlane(by, base + s);
+                    cz = __builtin_amdgcn_readlane(bz, base + s);
+                    rec = r[s * 8 + cj];
+                }
+                const uint64_t later = rc >> (s + 1);
+                const int e = later != 0ull ? min(n, s + 1 + __builtin_ctzll(later)) : n;
+                // Two rows per step in flight (each half takes every other row).
+                int t = s + hf;
+                for (; t + 2 < e; t += 4)
+                    acc += q[t * kWalkStride + (lane & 31)] + q[(t + 2) * kWalkStride + (lane & 31)];
+                if (t < e) acc += q[t * kWalkStride + (lane & 31)];
+                s = e;
+            }
+            // Every lane's reads of these rows are done before the next
+            // rows overwrite them.
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+    }
+};
+
 // Batch entry of the next ray a wave of a persistent launch processes.  The
 // batch is cut into 8 slices, one per XCD (a block's XCD taken as blockIdx
 // mod 8, the dispatch order); a wave claims entries of its XCD's slice from
@@ -645,7 +807,7 @@ __device__ __forceinline__ double wave_scan_mul64(double v, int lane) {
 // in FP64.
 template <int S, bool TGT, bool ADAPT>
 __device__ __forceinline__ void log_grad_ray(const RenderArgs &a, const int64_t b,
-                                             const int chunk = -1);
+                                             float *lds_q, uint32_t *lds_r, const int chunk = -1);
 
 // The logged raw values of sample row i of entry b and the activations of
 // the rule: sigma / alpha / colours (float under the fixed rule, FP64 under
@@ -685,17 +847,29 @@ __device__ __forceinline__ LoggedSample<ADAPT> read_logged(const RenderArgs &a, 
 }
 
 // TGT: also write the proposal targets (a.bin_targets, a.target_valid).
+// Each wave's gradient-walk rows in the block's dynamic LDS (launches pass
+// kWalkLdsBytes per wave).
+__device__ __forceinline__ void walk_lds(float *&q, uint32_t *&r) {
+    extern __shared__ __align__(16) unsigned char walk_lds_base[];
+    unsigned char *w = walk_lds_base + (size_t)(threadIdx.x >> 6) * kWalkLdsBytes;
+    q = reinterpret_cast<float *>(w);
+    r = reinterpret_cast<uint32_t *>(w + (size_t)kWalkRows * kWalkStride * 4);
+}
+
 template <int S, bool PERSIST = false, bool TGT = false, bool ADAPT = false>
 __global__ __launch_bounds__(256) void k_log_grad(const RenderArgs a) {
+    float *lq;
+    uint32_t *lr;
+    walk_lds(lq, lr);
     if (PERSIST) {
         int slice = blockIdx.x & 7, visited = 0;
         for (int64_t b = claim_ray(a.work + 8, a.len, slice, visited); b < a.len;
              b = claim_ray(a.work + 8, a.len, slice, visited))
-            log_grad_ray<S, TGT, ADAPT>(a, b);
+            log_grad_ray<S, TGT, ADAPT>(a, b, lq, lr);
     } else {
         const int64_t b = (int64_t)xcd_slice_block(blockIdx.x, gridDim.x) * (blockDim.x >> 6) +
                           (threadIdx.x >> 6);
-        if (b < a.len) log_grad_ray<S, TGT, ADAPT>(a, b);
+        if (b < a.len) log_grad_ray<S, TGT, ADAPT>(a, b, lq, lr);
     }
 }
 
@@ -746,12 +920,15 @@ __global__ __launch_bounds__(256) void k_log_grad_chunked(const RenderArgs a, in
     const int64_t item = (int64_t)xcd_slice_block(blockIdx.x, gridDim.x) * (blockDim.x >> 6) +
                          (threadIdx.x >> 6);
     if (item >= n_items) return;
-    log_grad_ray<S, false, ADAPT>(a, item / a.log_chunks, (int)(item % a.log_chunks));
+    float *lq;
+    uint32_t *lr;
+    walk_lds(lq, lr);
+    log_grad_ray<S, false, ADAPT>(a, item / a.log_chunks, lq, lr, (int)(item % a.log_chunks));
 }
 
 template <int S, bool TGT, bool ADAPT>
 __device__ __forceinline__ void log_grad_ray(const RenderArgs &a, const int64_t b,
-                                             const int chunk) {
+                                             float *lds_q, uint32_t *lds_r, const int chunk) {
     constexpr int P = 1 + 3 * S;
     const int lane = threadIdx.x & 63;
     const int cnt = uniform_i(a.log_count[b]);
@@ -779,13 +956,23 @@ __device__ __forceinline__ void log_grad_ray(const RenderArgs &a, const int64_t 
     float Y[S];
 #pragma unroll
     for (int k = 0; k < S; k++) Y[k] = sh_basis(k, fdx, fdy, fdz);
-    const int lh = lane >> 5, pp = lane & 31;
-    const int pch = pp == 0 ? -1 : (pp - 1) / S;
+    // Flush lanes (hf, pp): parameter pp of the corner a flush pair gives
+    // half hf; its channel (density, R, G, B) and SH factor.  An SH
+    // coefficient's gradient is its channel's coefficient times Y_k, Y_k
+    // fixed along the ray, so the walk sums per corner and channel only and
+    // multiplies by Y_k when a record is flushed.
+    const int hf = lane >> 5, pp = lane & 31;
+    const int pch = pp == 0 ? 0 : min(3, 1 + (pp - 1) / S);
     const bool live = pp == 0 || (pp < P && (pp - 1) % S >= a.sh_first && (pp - 1) % S < a.sh_active);
-    float ysel = 0.0f;
+    float ysel = 1.0f;
 #pragma unroll
     for (int k = 0; k < S; k++) ysel = (pp >= 1 && (pp - 1) % S == k) ? Y[k] : ysel;
     if (!live) ysel = 0.0f;
+    // Accumulator lanes: corner cj = (jx, jy, jz) and channel ch of lane & 31;
+    // the two halves sum alternate samples of a run.
+    const int cj = lane & 7, cch = (lane >> 3) & 3;
+    GradWalk w;
+    w.init(lds_q, lds_r);
 
     double T = 1.0, Acc = 0.0, Cr = 0.0, Cg = 0.0, Cb = 0.0;
     if (chunk > 0) {
@@ -796,10 +983,6 @@ __device__ __forceinline__ void log_grad_ray(const RenderArgs &a, const int64_t 
         Cg = pre[3];
         Cb = pre[4];
     }
-    bool has_cur = false;
-    int cur_x = 0, cur_y = 0, cur_z = 0;
-    uint32_t cur_rec[4] = {0u, 0u, 0u, 0u};
-    float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     double acc0 = 0.0;
     uint64_t tmask = 0ull;  // bins holding weight (proposal targets)
     const double tfar = a.rays.far_t[ray];
@@ -812,7 +995,6 @@ __device__ __forceinline__ void log_grad_ray(const RenderArgs &a, const int64_t 
         const int k = v.k;
         const uint4 q0 = *reinterpret_cast<const uint4 *>(a.log_rec + at);
         const uint4 q1 = *reinterpret_cast<const uint4 *>(a.log_rec + at + 4);
-        const uint32_t recs[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
         const auto sraw = v.sraw, col0 = v.col0, col1 = v.col1, col2 = v.col2;
         const auto alpha = v.alpha;
         const double keep = 1.0 - (double)alpha;
@@ -830,7 +1012,6 @@ __device__ __forceinline__ void log_grad_ray(const RenderArgs &a, const int64_t 
         double gs;
         sample_grads(rr, C0, C1, C2, A, step, ws, Tn, Ai, H0, H1, H2, col0, col1, col2, sraw, g0,
                      g1, g2, gs);
-        const float gsf = (float)gs;
         if (TGT) {
             uint64_t bit = act && ws > a.target_wmin ? 1ull << bin_of(k, step, tnear, tfar) : 0ull;
 #pragma unroll
@@ -844,46 +1025,17 @@ __device__ __forceinline__ void log_grad_ray(const RenderArgs &a, const int64_t 
         const int bx = base_of(px), by = base_of(py), bz = base_of(pz);
         const float fx = (float)(px - (double)bx), fy = (float)(py - (double)by),
                     fz = (float)(pz - (double)bz);
-
-        for (int i = 0; i < m; i++) {
-            const int nx = __builtin_amdgcn_readlane(bx, i);
-            const int ny = __builtin_amdgcn_readlane(by, i);
-            const int nz = __builtin_amdgcn_readlane(bz, i);
-            if (!has_cur || nx != cur_x || ny != cur_y || nz != cur_z) {
-                if (has_cur)
-                    retire_cell(a, acc, cur_rec, acc0, nx - cur_x, ny - cur_y, nz - cur_z, lh, pp,
-                                live);
-                has_cur = true;
-                cur_x = nx;
-                cur_y = ny;
-                cur_z = nz;
-#pragma unroll
-                for (int h = 0; h < 4; h++) {
-                    const uint32_t e = (uint32_t)__builtin_amdgcn_readlane((int)recs[2 * h], i);
-                    const uint32_t o = (uint32_t)__builtin_amdgcn_readlane((int)recs[2 * h + 1], i);
-                    cur_rec[h] = lh ? o : e;
-                }
-            }
-            const float sfx = readlane_f(fx, i), sfy = readlane_f(fy, i), sfz = readlane_f(fz, i);
-            const float sgs = readlane_f(gsf, i), sg0 = readlane_f(g0, i),
-                        sg1 = readlane_f(g1, i), sg2 = readlane_f(g2, i);
-            const float coef = pch < 0 ? sgs : (pch == 0 ? sg0 : (pch == 1 ? sg1 : sg2)) * ysel;
-            const float wx = lh ? sfx : 1.0f - sfx;
-#pragma unroll
-            for (int h = 0; h < 4; h++) {
-                const float wyz = ((h & 1) ? sfy : 1.0f - sfy) * ((h >> 1) ? sfz : 1.0f - sfz);
-                acc[h] = fmaf(wx * wyz, coef, acc[h]);
-            }
-        }
+        w.walk(a, lane, m, act, bx, by, bz, fx, fy, fz, (float)gs, g0, g1, g2, q0, q1, cj, cch,
+               hf, pp, pch, live, ysel, P, acc0);
         T *= readlane_d(incl, 63);
         Acc = readlane_d(Ai, 63);
         Cr = readlane_d(H0, 63);
         Cg = readlane_d(H1, 63);
         Cb = readlane_d(H2, 63);
     }
-    if (has_cur) {
-        retire_cell(a, acc, cur_rec, acc0, 2, 2, 2, lh, pp, live);
-        flush_record0(a, acc0, lh, pp, live, b);
+    if (w.has_cur) {
+        w.retire(a, 0xFFu | (16u << 16), cj, cch, hf, pp, pch, live, ysel, P, acc0);
+        flush_record0(a, acc0, hf, pp, live, b);
     }
     if (TGT && lane == 0) {
         a.bin_targets[b] = tmask;

@@ -1029,11 +1029,11 @@ __global__ __launch_bounds__(256) void k_rec0_fold(const double *__restrict__ re
 // Blocks of `threads` threads of `kernel` that fit on the whole device at
 // once (the grid of a persistent launch), or PXL_OPT_CLAIM_GRID if smaller.
 template <typename K>
-unsigned resident_blocks(K kernel, int threads) {
+unsigned resident_blocks(K kernel, int threads, size_t lds = 0) {
     int dev = 0, cus = 0, per_cu = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, lds) != hipSuccess ||
         cus <= 0 || per_cu <= 0)
         cus = 1024, per_cu = 1;
     const unsigned resident = (unsigned)cus * (unsigned)per_cu;
@@ -1059,6 +1059,7 @@ int32_t launch_logged(const RenderArgs &a, int wpb, int gwpb, bool persist, bool
     const unsigned blocks = (unsigned)((a.len + wpb - 1) / wpb);
     // The gradient kernels' own workgroup shape.
     const int gthreads = 64 * gwpb;
+    const size_t glds = (size_t)gwpb * kWalkLdsBytes;  // the gradient walk's rows
     const unsigned gblocks = (unsigned)((a.len + gwpb - 1) / gwpb);
     const bool tgt = a.bin_targets != nullptr;
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, a.work, kRedoCounter + 1);
@@ -1070,11 +1071,12 @@ int32_t launch_logged(const RenderArgs &a, int wpb, int gwpb, bool persist, bool
         wait_before_grad(st);
         if (tgt) {
             const unsigned gb =
-                std::min(gblocks, resident_blocks(k_log_grad<S, true, true>, gthreads));
-            hipLaunchKernelGGL((k_log_grad<S, true, true>), dim3(gb), dim3(gthreads), 0, st, a);
+                std::min(gblocks, resident_blocks(k_log_grad<S, true, true>, gthreads, glds));
+            hipLaunchKernelGGL((k_log_grad<S, true, true>), dim3(gb), dim3(gthreads), glds, st, a);
         } else {
-            const unsigned gb = std::min(gblocks, resident_blocks(k_log_grad<S, true>, gthreads));
-            hipLaunchKernelGGL((k_log_grad<S, true>), dim3(gb), dim3(gthreads), 0, st, a);
+            const unsigned gb =
+                std::min(gblocks, resident_blocks(k_log_grad<S, true>, gthreads, glds));
+            hipLaunchKernelGGL((k_log_grad<S, true>), dim3(gb), dim3(gthreads), glds, st, a);
         }
     } else if (chunked && !tgt) {
         // Chunk-parallel gradient (the proposal's target words need the
@@ -1085,16 +1087,16 @@ int32_t launch_logged(const RenderArgs &a, int wpb, int gwpb, bool persist, bool
         hipLaunchKernelGGL((k_log_prefix<S>), dim3(gblocks), dim3(gthreads), 0, st, a);
         const int64_t items = a.len * (int64_t)a.log_chunks;
         hipLaunchKernelGGL((k_log_grad_chunked<S>), dim3((unsigned)((items + gwpb - 1) / gwpb)),
-                           dim3(gthreads), 0, st, a, items);
+                           dim3(gthreads), glds, st, a, items);
     } else {
         hipLaunchKernelGGL((k_march8<S, SPARSE, kLog, STORE>), dim3(blocks), dim3(threads), 0, st,
                            a);
         wait_before_grad(st);
         if (tgt)
-            hipLaunchKernelGGL((k_log_grad<S, false, true>), dim3(gblocks), dim3(gthreads), 0, st,
-                               a);
+            hipLaunchKernelGGL((k_log_grad<S, false, true>), dim3(gblocks), dim3(gthreads), glds,
+                               st, a);
         else
-            hipLaunchKernelGGL((k_log_grad<S>), dim3(gblocks), dim3(gthreads), 0, st, a);
+            hipLaunchKernelGGL((k_log_grad<S>), dim3(gblocks), dim3(gthreads), glds, st, a);
     }
     RenderArgs redo = a;
     redo.log_redo = 1;
@@ -1121,23 +1123,25 @@ int32_t launch_logged_adapt(const RenderArgs &a, int wpb, int gwpb, bool persist
         std::min((unsigned)((a.len + 8 * wpb - 1) / (8 * wpb)),
                  resident_blocks(k_adapt8<S, SPARSE, kLog>, threads));
     const int gthreads = 64 * gwpb;
+    const size_t glds = (size_t)gwpb * kWalkLdsBytes;  // the gradient walk's rows
     const unsigned gblocks = (unsigned)((a.len + gwpb - 1) / gwpb);
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, a.work, kRedoCounter + 1);
     hipLaunchKernelGGL((k_adapt8<S, SPARSE, kLog>), dim3(blocks), dim3(threads), 0, st, a);
     if (grad_wait && hipStreamWaitEvent(st, grad_wait, 0) != hipSuccess) wait_ok = false;
     if (persist) {
         const unsigned gb =
-            std::min(gblocks, resident_blocks(k_log_grad<S, true, false, true>, gthreads));
-        hipLaunchKernelGGL((k_log_grad<S, true, false, true>), dim3(gb), dim3(gthreads), 0, st, a);
+            std::min(gblocks, resident_blocks(k_log_grad<S, true, false, true>, gthreads, glds));
+        hipLaunchKernelGGL((k_log_grad<S, true, false, true>), dim3(gb), dim3(gthreads), glds, st,
+                           a);
     } else if (chunked) {
         hipLaunchKernelGGL((k_log_prefix<S, true>), dim3(gblocks), dim3(gthreads), 0, st, a);
         const int64_t items = a.len * (int64_t)a.log_chunks;
         hipLaunchKernelGGL((k_log_grad_chunked<S, true>),
-                           dim3((unsigned)((items + gwpb - 1) / gwpb)), dim3(gthreads), 0, st, a,
+                           dim3((unsigned)((items + gwpb - 1) / gwpb)), dim3(gthreads), glds, st, a,
                            items);
     } else {
-        hipLaunchKernelGGL((k_log_grad<S, false, false, true>), dim3(gblocks), dim3(gthreads), 0,
-                           st, a);
+        hipLaunchKernelGGL((k_log_grad<S, false, false, true>), dim3(gblocks), dim3(gthreads),
+                           glds, st, a);
     }
     RenderArgs redo = a;
     redo.log_redo = 1;
