@@ -86,6 +86,22 @@ __device__ __forceinline__ int uniform_i(int v) {
     return __builtin_amdgcn_readfirstlane(v);
 }
 
+// The two wave halves' values through gfx950's v_permlane32_swap (a VALU
+// operation, no LDS round trip): lo = v of lane L & 31, hi = v of lane
+// (L & 31) + 32, in every lane L.
+__device__ __forceinline__ void halves_f(float v, float &lo, float &hi) {
+    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(v), false,
+                                                    false);
+    lo = __uint_as_float(r[0]);
+    hi = __uint_as_float(r[1]);
+}
+// v of lane L ^ 32.
+__device__ __forceinline__ float xor32_f(float v) {
+    float lo, hi;
+    halves_f(v, lo, hi);
+    return (threadIdx.x & 32) ? lo : hi;
+}
+
 // DPP row rotate (16-lane rows): ctrl 0x120 + n = row_ror:n.
 template <int N>
 __device__ __forceinline__ float row_ror_f(float v) {

@@ -191,7 +191,7 @@ __device__ __forceinline__ void retire_cell(const RenderArgs &a, float (&acc)[4]
         const bool yz_ok = in01(yy) && in01(zz);
         const int src = yz_ok ? yy + 2 * zz : 0;
         float v = src == 0 ? acc[0] : (src == 1 ? acc[1] : (src == 2 ? acc[2] : acc[3]));
-        if (dx != 0) v = __shfl_xor(v, 32, 64);
+        if (dx != 0) v = xor32_f(v);
         moved[h] = (yz_ok && x_ok) ? v : 0.0f;
     }
 #pragma unroll
@@ -268,7 +268,10 @@ struct GradWalk {
     __device__ __forceinline__ void retire(const RenderArgs &a, uint32_t move, int cj, int cch,
                                            int hf, int pp, int pch, bool live, float ysel, int P,
                                            double &acc0) {
-        const float tot = acc + __shfl_xor(acc, 32, 64);
+        // The corner's sum over both halves (the same in both).
+        float lo, hi;
+        halves_f(acc, lo, hi);
+        const float tot = lo + hi;
         uint32_t rmask = move & 0xFFu;
         while (rmask != 0u) {
             const int ja = __builtin_ctz(rmask);
