@@ -79,8 +79,8 @@ def parse():
                     help="bin probability below which the proposal prunes the bin")
     ap.add_argument("--prop-dilate", type=int, default=1,
                     help="bins kept on each side of a bin the proposal keeps")
-    ap.add_argument("--cpu-rays", type=int, default=8192,
-                    help="rays of the bounded CPU-baseline sample (0: skip)")
+    ap.add_argument("--cpu-rays", type=int, default=0,
+                    help="rays of the batch the CPU baseline marches (0: the whole batch)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batch-order", choices=("shuffled", "sorted"), default="sorted",
                     help="order of the rays inside each batch (the batches themselves are "
@@ -481,29 +481,46 @@ def samples_per_batch(w, dev, nb=2):
 
 
 def cpu_cores():
-    """Host threads for the CPU baseline: the cores this process may run on,
-    at most 16 (the host share of one GPU)."""
+    """Host threads for the CPU baseline: every core this process may run on,
+    at most the OMP_NUM_THREADS the host sets (the host share of one GPU;
+    16 when unset)."""
     try:
         n = len(os.sched_getaffinity(0))
     except AttributeError:
         n = os.cpu_count() or 1
-    return max(1, min(16, n))
+    try:
+        cap = int(os.environ.get("OMP_NUM_THREADS", "16"))
+    except ValueError:
+        cap = 16
+    return max(1, min(n, cap if cap > 0 else 16))
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def cpu_baseline(w, dev, nrays):
-    """The oracle's train step on the host cores (OpenMP over rays, cells and
-    parameters): a bounded sample of nrays rays (forward + backward) plus the
-    per-step TV and Adam over the full caged grid, scaled to rays/s of a
-    65536-ray step."""
+    """The train step in the reference's arithmetic types (oracle/refprec.c:
+    FP32 parameters, float interpolation and activations, FP64 t, T, weights
+    and loss, float gradient atomics) on the host cores: forward + backward
+    of nrays rays of the batch (0: the whole batch), the variance loss over
+    the grid (every batch dense, every 3rd caged) and Adam over every
+    record; each part once untimed, then the median of 5 timed runs."""
     import numpy as np
-    import torch
 
     import oracle
 
     cores = cpu_cores()
-    oracle.set_threads(cores)
-
     P = w.P
+    B = w.args.batch
+    n = B if nrays <= 0 else min(nrays, B)
     planes = w.rec[:, :P].T.contiguous().cpu().numpy()
     if w.dense:
         occ = io = None
@@ -514,35 +531,42 @@ def cpu_baseline(w, dev, nrays):
         io = w.grid.index_offsets.cpu().numpy().view(np.uint32)
     keys = ("ox", "oy", "oz", "dx", "dy", "dz", "near_t", "far_t", "gt_r", "gt_g", "gt_b",
             "gt_mask")
-    idx = w.order[:nrays].long()
+    idx = w.order[:n].long()
     sub = {k: getattr(w.rays, k)[idx].cpu().numpy() for k in keys}
-    rays_o = oracle.Rays(sub, white_bg=True)
-    grid_o = oracle.Grid(w.N, w.N, w.N, w.S, planes, 0.5, occupancy=occ, index_offsets=io)
-    si = np.arange(nrays, dtype=np.int32)
-    t0 = oracle.now()
-    fwd = oracle.render_fwd(grid_o, rays_o, si)
-    oracle.render_bwd(grid_o, rays_o, si, fwd, 1.0, 0.1)
-    t_rays = (oracle.now() - t0) / nrays
-    t0 = oracle.now()
-    oracle.tv_disparity(grid_o, 1e-5, 1e-6, 0)
-    t_tv = oracle.now() - t0
-    p32 = planes.copy()
-    g = np.zeros_like(p32)
-    m = np.zeros_like(p32)
-    v = np.zeros_like(p32)
-    t0 = oracle.now()
-    oracle.adam(p32, g, m, v, 0.3, 1.0, 1)
-    t_adam = oracle.now() - t0
-    B = w.args.batch
-    oracle.set_threads(1)
+    rays_o = oracle.Rays(sub, white_bg=bool(w.rays.white_bg))
+    grid_o = oracle.Grid(w.N, w.N, w.N, w.S, planes, 0.5, occupancy=occ, index_offsets=io,
+                         adaptive=w.args.march == "adaptive")
+    si = np.arange(n, dtype=np.int32)
+    grad = np.zeros_like(planes)
+    params = planes.copy()
+    m = np.zeros_like(planes)
+    v = np.zeros_like(planes)
+
+    def median5(fn):
+        fn()
+        times = []
+        for _ in range(5):
+            t0 = oracle.rp_now()
+            fn()
+            times.append(oracle.rp_now() - t0)
+        return sorted(times)[2]
+
+    t_rays = median5(lambda: oracle.rp_render_train(grid_o, rays_o, si, grad, 1.0, 0.1,
+                                                    threads=cores)) / n
+    t_tv = median5(lambda: oracle.rp_tv_disparity(grid_o, 1e-5, 1e-6, 0, grad, threads=cores))
+    t_adam = median5(lambda: oracle.rp_adam(params, grad, m, v, 0.3, 1.0, 1, threads=cores))
     t_step = B * t_rays + (t_tv if w.dense else t_tv / 3.0) + t_adam
-    sample = (f"{nrays} rays of the workload (FP64 forward + backward, "
-              f"{t_rays * 1e6:.2f} us/ray on {cores} threads) + TV over the "
-              f"{'dense' if w.dense else 'caged'} grid ({t_tv:.2f} s, "
-              f"{'every' if w.dense else 'every 3rd'} step) + Adam over {w.n_vox} records "
-              f"({t_adam:.2f} s); step = {B} rays")
-    return {"value": B / t_step, "unit": "rays/s", "cores": cores, "kind": "port",
-            "sample": sample}
+    nproc = os.cpu_count()
+    model = cpu_model()
+    sample = (f"the train step in the reference's arithmetic types (oracle/refprec.c), "
+              f"{cores} OpenMP threads on {model} (nproc {nproc}); forward + backward of "
+              f"{n} rays of the batch ({t_rays * 1e6:.2f} us/ray), variance loss over the "
+              f"{'dense' if w.dense else 'caged'} grid ({t_tv:.3f} s, "
+              f"{'every' if w.dense else 'every 3rd'} step), Adam over {w.n_vox} records "
+              f"({t_adam:.3f} s); each part: 1 untimed run, median of 5; step = {B} rays")
+    log(f"CPU baseline: {B / t_step:.1f} rays/s; {sample}")
+    return {"value": round(B / t_step, 1), "unit": "rays/s", "cores": cores, "kind": "port",
+            "sample": sample, "host": {"nproc": nproc, "cpu_model": model}}
 
 
 TRAFFIC_FILE = os.path.join(REPO, "profiles", "train_step_256_sparse_f32.md")
@@ -803,9 +827,9 @@ def main():
                         "GBps": round(n_samp * per_sample / (kern_ms / 1e3) / 1e9, 1)}
 
     cpu = None
-    if (not args.no_cpu_baseline and args.cpu_rays > 0 and not uneven and not args.proposal
+    if (not args.no_cpu_baseline and not uneven and not args.proposal
             and args.storage == "f32" and not args.pretrain):
-        log("CPU baseline (oracle) ...")
+        log("CPU baseline (reference-precision restatement) ...")
         cpu = cpu_baseline(w, dev, args.cpu_rays)
     out = {
         "metric": f"train rays/sec (fwd+bwd+TV+Adam) {args.grid}^3 SH2"

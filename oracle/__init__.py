@@ -31,6 +31,10 @@ Parity status of each function (how it is pinned; see tests/test_oracle.py):
   upsample          - even vertices copy the source, affine fields reproduced
                       away from the clamped upper faces, the clamp itself,
                       unoccupied source cells, the threshold rule, band masks
+
+rp_* (librefprec.so, oracle/refprec.c) is the benchmark's CPU baseline: the
+train step in the reference's arithmetic types, pinned against render_fwd /
+render_bwd / tv_disparity / adam above (tests/test_refprec.py).
 """
 import ctypes
 import os
@@ -432,3 +436,65 @@ def upsample(grid, W2, H2, D2, S2, threshold, low_band=False, in_cage=None, inde
 def set_threads(n):
     """Threads for the oracle's loops (1 = reproducible sums, the default)."""
     lib().orc_set_threads(int(n))
+
+
+# ------------------------------------------------------------ refprec --
+# The train step in the reference's arithmetic types (refprec.c): the
+# benchmark's CPU baseline, pinned against the FP64 functions above by
+# tests/test_refprec.py.
+
+_RP_PATH = os.path.join(_HERE, "librefprec.so")
+_rp = None
+
+
+def rp_lib():
+    global _rp
+    if _rp is None:
+        if not os.path.exists(_RP_PATH):
+            build()
+        L = ctypes.CDLL(_RP_PATH)
+        L.rp_render_train.argtypes = [ctypes.POINTER(OrcGrid), ctypes.POINTER(OrcRays), _P, _I64,
+                                      _F64, _F32, _P, _P, _P, _I32]
+        L.rp_render_train.restype = None
+        L.rp_tv_disparity.argtypes = [ctypes.POINTER(OrcGrid), _F64, _F64, _I32, _P, _I32]
+        L.rp_tv_disparity.restype = None
+        L.rp_adam.argtypes = [_P, _P, _P, _P, _I32, _I64, _F64, _F64, _I32, _I32]
+        L.rp_adam.restype = None
+        L.rp_now.argtypes = []
+        L.rp_now.restype = _F64
+        _rp = L
+    return _rp
+
+
+def rp_render_train(grid, rays, sample_index, grad, mask_coef=1.0, huber_delta=0.1, threads=1):
+    """Forward + backward of the batch in the reference's types: returns
+    (fwd float32[len, 5] = colour, opacity, density sum; losses float64[len]);
+    gradients added into grad (float32[P, n_vox])."""
+    si = _c(sample_index, np.int32)
+    assert grad.dtype == np.float32 and grad.flags.c_contiguous
+    fwd = np.zeros((len(si), 5), np.float32)
+    losses = np.zeros(len(si), np.float64)
+    g, r = grid.c(), rays.c()
+    rp_lib().rp_render_train(ctypes.byref(g), ctypes.byref(r), _p(si), len(si), mask_coef,
+                             huber_delta, _p(fwd), _p(losses), _p(grad), int(threads))
+    return fwd, losses
+
+
+def rp_tv_disparity(grid, lambda_density, lambda_sh, optimize_step, grad, threads=1):
+    """addVarianceLossGPU in the reference's types into grad (float32[P, n_vox])."""
+    assert grad.dtype == np.float32 and grad.flags.c_contiguous
+    g = grid.c()
+    rp_lib().rp_tv_disparity(ctypes.byref(g), lambda_density, lambda_sh, optimize_step, _p(grad),
+                             int(threads))
+
+
+def rp_adam(params, grad, m, v, lr, sh_lr, t, threads=1):
+    """updateRGBSceneGPU_ADAM over float32[P, n] arrays (in place)."""
+    for a in (params, grad, m, v):
+        assert a.dtype == np.float32 and a.flags.c_contiguous
+    P, n = params.shape
+    rp_lib().rp_adam(_p(params), _p(grad), _p(m), _p(v), P, n, lr, sh_lr, t, int(threads))
+
+
+def rp_now():
+    return rp_lib().rp_now()
