@@ -81,6 +81,7 @@ struct RenderArgs {
     // gradient in a fixed order by k_rec0_fold.  Null: record 0 takes one
     // float atomic per ray.
     double *rec0;
+    double *rec0_part;  // [32][kFoldParts] partial sums of the fold
     // Persistent launches: 8 claim counters per kernel (zeroed before launch).
     unsigned long long *work;
     // Learned proposal: per batch entry, the bins a ray may evaluate (null:
@@ -966,11 +967,13 @@ namespace pxl {
 namespace {
 // Workspace of the two-kernel fused step: per batch entry 4 FP64 totals, a
 // count and 32 FP64 record-0 slots, per logged sample 8 records and 16 value
-// words, per 64-sample chunk of an entry's log 5 FP64 prefixes; 256-byte
-// aligned.
+// words, per 64-sample chunk of an entry's log 5 FP64 prefixes, and the
+// record-0 fold's 32 x kFoldParts partial sums; 256-byte aligned.
 struct LogLayout {
-    size_t work, tot, count, rec, val, pre, redo, rec0, total;
+    size_t work, tot, count, rec, val, pre, redo, rec0, part, total;
 };
+// Slices of the batch the record-0 fold's first pass sums separately.
+constexpr int kFoldParts = 64;
 inline size_t align256(size_t x) { return (x + 255) / 256 * 256; }
 LogLayout log_layout(int64_t len, int32_t cap) {
     LogLayout L;
@@ -991,6 +994,8 @@ LogLayout log_layout(int64_t len, int32_t cap) {
     o += align256((size_t)len * sizeof(int32_t));
     L.rec0 = o;
     o += align256((size_t)len * 32 * sizeof(double));
+    L.part = o;
+    o += align256((size_t)32 * kFoldParts * sizeof(double));
     L.total = o;
     return L;
 }
@@ -1000,30 +1005,44 @@ __global__ void k_zero_u64(unsigned long long *p, int n) {
     if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
 }
 
-// Record 0 of the logged step: block p sums the batch entries' FP64 slots of
-// parameter p in a fixed order (thread t: entries t, t + 256, ...; then a
-// fixed tree over the threads) and adds the total to the gradient.  Every
-// corner outside the cage of a caged grid lands on record 0
-// (SceneUtils.cu:1106-1113), so this sum has ~10^5 largely cancelling terms.
-__global__ __launch_bounds__(256) void k_rec0_fold(const double *__restrict__ rec0, int64_t len,
-                                                   float *grad) {
+// Record 0 of the logged step, summed in a fixed order in two passes:
+// block (c, p) of k_rec0_part sums parameter p's FP64 slots of the c-th of
+// kFoldParts equal slices of the batch (thread t: entries t, t + 256, ... of
+// the slice, then a fixed tree over the threads); k_rec0_fold (one wave per
+// parameter) sums the slices with a fixed butterfly and adds the total to
+// the gradient.  Every corner outside the cage of a caged grid lands on
+// record 0 (SceneUtils.cu:1106-1113), so this sum has ~10^5 largely
+// cancelling terms.
+__global__ __launch_bounds__(256) void k_rec0_part(const double *__restrict__ rec0, int64_t len,
+                                                   double *__restrict__ part) {
     __shared__ double red[256];
-    const int p = blockIdx.x, t = threadIdx.x;
+    const int c = blockIdx.x, p = blockIdx.y, t = threadIdx.x;
+    const int64_t per = (len + kFoldParts - 1) / kFoldParts;
+    const int64_t lo = (int64_t)c * per, hi = lo + per < len ? lo + per : len;
     const double *src = rec0 + (size_t)p * (size_t)len;
-    double s0 = 0.0, s1 = 0.0;
-    int64_t i = t;
-    for (; i + 256 < len; i += 512) {
-        s0 += src[i];
-        s1 += src[i + 256];
-    }
-    if (i < len) s0 += src[i];
-    red[t] = s0 + s1;
+    double s = 0.0;
+    for (int64_t i = lo + t; i < hi; i += 256) s += src[i];
+    red[t] = s;
     __syncthreads();
     for (int h = 128; h > 0; h >>= 1) {
         if (t < h) red[t] += red[t + h];
         __syncthreads();
     }
-    if (t == 0 && red[0] != 0.0) grad[p] += (float)red[0];
+    if (t == 0) part[p * kFoldParts + c] = red[0];
+}
+
+__global__ __launch_bounds__(64) void k_rec0_fold(const double *__restrict__ part, float *grad) {
+    static_assert(kFoldParts == 64, "one slice per lane");
+    const int p = blockIdx.x;
+    const double v = wave_sum_d(part[p * kFoldParts + threadIdx.x]);
+    if (threadIdx.x == 0 && v != 0.0) grad[p] += (float)v;
+}
+
+// The two passes over the P parameters' slots.
+inline void launch_rec0_fold(const RenderArgs &a, int P, hipStream_t st) {
+    hipLaunchKernelGGL(k_rec0_part, dim3(kFoldParts, (unsigned)P), dim3(256), 0, st, a.rec0,
+                       a.len, a.rec0_part);
+    hipLaunchKernelGGL(k_rec0_fold, dim3((unsigned)P), dim3(64), 0, st, a.rec0_part, a.grad);
 }
 
 // Blocks of `threads` threads of `kernel` that fit on the whole device at
@@ -1102,7 +1121,7 @@ int32_t launch_logged(const RenderArgs &a, int wpb, int gwpb, bool persist, bool
     redo.log_redo = 1;
     hipLaunchKernelGGL((k_march8_redo<S, SPARSE, STORE>), dim3(std::min(blocks, 1024u)),
                        dim3(threads), 0, st, redo);
-    hipLaunchKernelGGL(k_rec0_fold, dim3(1 + 3 * S), dim3(256), 0, st, a.rec0, a.len, a.grad);
+    launch_rec0_fold(a, 1 + 3 * S, st);
     if (!wait_ok) {
         set_error("render: hipStreamWaitEvent before the gradient kernels failed");
         return PXL_EHIP;
@@ -1147,7 +1166,7 @@ int32_t launch_logged_adapt(const RenderArgs &a, int wpb, int gwpb, bool persist
     redo.log_redo = 1;
     hipLaunchKernelGGL((k_render<S, SPARSE, true, kFused, PXL_STORE_F32>),
                        dim3((unsigned)((a.len + 3) / 4)), dim3(256), 0, st, redo);
-    hipLaunchKernelGGL(k_rec0_fold, dim3(1 + 3 * S), dim3(256), 0, st, a.rec0, a.len, a.grad);
+    launch_rec0_fold(a, 1 + 3 * S, st);
     if (!wait_ok) {
         set_error("render: hipStreamWaitEvent before the gradient kernels failed");
         return PXL_EHIP;
@@ -1275,6 +1294,7 @@ static int32_t bwd_common(const pxl_grid *grid, const pxl_optim *optim, const px
         a.log_pre = reinterpret_cast<double *>(w + L.pre);
         a.redo_list = reinterpret_cast<int32_t *>(w + L.redo);
         a.rec0 = reinterpret_cast<double *>(w + L.rec0);
+        a.rec0_part = reinterpret_cast<double *>(w + L.part);
         a.log_chunks = (max_samples + 63) / 64;
         a.work = reinterpret_cast<unsigned long long *>(w + L.work);
         const int wpb = sched ? sched->waves_per_block : 4;

@@ -5,8 +5,9 @@ Optimizer.cu:359-415), on the same inputs: stepped records, gradient, both
 moments and the FP32 master equal bit for bit, over two consecutive steps
 (the second from the first's output, as the trainer swaps its buffers).
 
-Cases: dense grids with odd sizes (the z-marched interior in chunks of 24
-plus a tail, and the boundary gather), a grid too thin for the z-march, the
+Cases: dense grids with odd sizes (the z-marched interior, in chunks of 128
+plus a tail on a tall grid, and the boundary gather in the same launch), a
+grid too thin for the z-march, the
 64-bit address forms (library option), caged grids (prepared table + the
 record-0 alias term), S = 16 (64 lanes per record), BF16 with an FP32 master,
 F16 without one, an SH band (sh_first 1, sh_active 4), optimize_step 2 (no
@@ -123,6 +124,7 @@ DENSE = {
     "f32_odd_o28": (21, 19, 33, 9, pxl.STORE_F32, {"ostride": 28}),
     "f32_alr_step2": (21, 19, 33, 9, pxl.STORE_F32, {"alr": True, "opt_step": 2}),
     "f32_thin": (2, 7, 9, 9, pxl.STORE_F32, {}),
+    "f32_tall": (5, 4, 300, 9, pxl.STORE_F32, {}),
     "f32_addr64": (21, 19, 33, 9, pxl.STORE_F32, {"addr64": True}),
     "f32_band": (21, 19, 33, 9, pxl.STORE_F32, {"sh_active": 4, "sh_first": 1}),
     "bf16_master": (21, 19, 33, 9, pxl.STORE_BF16, {}),
