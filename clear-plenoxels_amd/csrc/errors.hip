@@ -44,13 +44,14 @@ extern "C" const char *pxl_last_error(void) { return pxl::g_err; }
 extern "C" int32_t pxl_version(void) { return (1 << 16) | 2; }
 
 extern "C" int32_t pxl_abi_sizes(int64_t *out, int32_t n) {
-    const int64_t sz[8] = {(int64_t)sizeof(pxl_camera),       (int64_t)sizeof(pxl_bbox),
+    const int64_t sz[9] = {(int64_t)sizeof(pxl_camera),       (int64_t)sizeof(pxl_bbox),
                            (int64_t)sizeof(pxl_rays),         (int64_t)sizeof(pxl_grid),
                            (int64_t)sizeof(pxl_optim),        (int64_t)sizeof(pxl_train_config),
-                           (int64_t)sizeof(pxl_level_config), (int64_t)sizeof(pxl_level)};
-    if (!out) return 8;
-    for (int i = 0; i < n && i < 8; i++) out[i] = sz[i];
-    return 8;
+                           (int64_t)sizeof(pxl_level_config), (int64_t)sizeof(pxl_level),
+                           (int64_t)sizeof(pxl_schedule)};
+    if (!out) return 9;
+    for (int i = 0; i < n && i < 9; i++) out[i] = sz[i];
+    return 9;
 }
 
 extern "C" int32_t pxl_set_option(int32_t key, int64_t value) {

@@ -97,6 +97,7 @@ struct pxl_trainer {
     int32_t order_mode;
     pxl_schedule sched;
     bool tuned;
+    bool binned_ok;  // the workspace holds the brick-binned gradient's arrays
     // Learned proposal (caller-owned network; trainer-owned buffers).
     bool prop_on;
     pxl_proposal prop;
@@ -183,14 +184,16 @@ extern "C" pxl_trainer *pxl_trainer_create(const pxl_train_config *cfg, const px
     tr->ev_used = 0;
     tr->march_ws = nullptr;
     tr->march_ws_bytes = 0;
+    tr->binned_ok = false;
     tr->order_buf = nullptr;
     tr->order_ws = nullptr;
     tr->order_ws_bytes = 0;
     tr->order_mode = cfg->batch_order;
     tr->sched.waves_per_block = cfg->waves_per_block > 0 ? cfg->waves_per_block : 4;
     tr->sched.persistent = cfg->persistent;
-    tr->sched.chunked = cfg->chunked;
+    tr->sched.gradient = cfg->gradient;
     tr->sched.grad_waves_per_block = 0;
+    tr->sched.slices = cfg->slices;
     tr->tuned = !cfg->autotune;
     tr->prop_on = false;
     tr->prop_ws = nullptr;
@@ -199,20 +202,35 @@ extern "C" pxl_trainer *pxl_trainer_create(const pxl_train_config *cfg, const px
     tr->prop_valid = nullptr;
     tr->prop_loss = nullptr;
     const bool logged = cfg->fused && cfg->log_samples > 0;
-    if (!logged && (cfg->autotune || cfg->batch_order || cfg->persistent || cfg->chunked ||
+    if (!logged && (cfg->autotune || cfg->batch_order || cfg->persistent || cfg->gradient ||
                     (cfg->waves_per_block && cfg->waves_per_block != 4))) {
         set_error("pxl_trainer_create: batch orders and schedules need the logged step "
                   "(fused with log_samples > 0)");
         pxl_trainer_destroy(tr);
         return nullptr;
     }
-    if (cfg->chunked && cfg->persistent) {
+    if (cfg->gradient == PXL_GRAD_CHUNKED && cfg->persistent) {
         set_error("pxl_trainer_create: the chunk-parallel gradient has no persistent form");
         pxl_trainer_destroy(tr);
         return nullptr;
     }
+    if (cfg->slices < 0 || cfg->slices > 8 || (!logged && cfg->slices > 1)) {
+        set_error("pxl_trainer_create: slices %d (0 .. 8, the logged step only)", cfg->slices);
+        pxl_trainer_destroy(tr);
+        return nullptr;
+    }
+    if (cfg->gradient < PXL_GRAD_PER_RAY || cfg->gradient > PXL_GRAD_BINNED) {
+        set_error("pxl_trainer_create: unknown gradient schedule %d", cfg->gradient);
+        pxl_trainer_destroy(tr);
+        return nullptr;
+    }
     if (logged) {
-        tr->march_ws_bytes = pxl_train_rays_workspace_bytes(cfg->batch_size, cfg->log_samples);
+        // Sized for every gradient schedule, the brick-binned one included
+        // where the grid allows it.
+        tr->march_ws_bytes =
+            pxl_train_rays_workspace_bytes_grid(grid, cfg->batch_size, cfg->log_samples);
+        tr->binned_ok = tr->march_ws_bytes >
+                        pxl_train_rays_workspace_bytes(cfg->batch_size, cfg->log_samples);
         if (tr->march_ws_bytes == 0 ||
             hipMalloc(&tr->march_ws, tr->march_ws_bytes) != hipSuccess) {
             set_error("pxl_trainer_create: cannot allocate a %zu-byte sample log for %d samples "
@@ -348,13 +366,21 @@ static int32_t autotune(pxl_trainer *tr, int64_t start, int64_t len) {
     std::vector<Cand> cands;
     // Waves per workgroup of the forward and (separately) of the gradient
     // kernel; persistent launches share one shape; the chunk-parallel
-    // gradient with 4 waves per workgroup.
+    // gradient with 4 waves per workgroup; the brick-binned gradient with 1
+    // or 4 waves per workgroup of its per-ray passes.
+    // Each with the batch in 1 slice or (overlapping one slice's gradient
+    // with the next slice's forward) in 2 or 4.
     for (int32_t order : {0, (int32_t)PXL_ORDER_MORTON, (int32_t)PXL_ORDER_LENGTH_MORTON})
-        for (int32_t wpb : {1, 2, 4}) {
-            for (int32_t gwpb : {1, 2, 4}) cands.push_back({order, {wpb, 0, 0, gwpb}});
-            cands.push_back({order, {wpb, 1, 0, 0}});
-            cands.push_back({order, {wpb, 0, 1, 4}});
-        }
+        for (int32_t wpb : {1, 2, 4})
+            for (int32_t sl : {1, 2, 4}) {
+                for (int32_t gwpb : {1, 2, 4})
+                    cands.push_back({order, {wpb, 0, PXL_GRAD_PER_RAY, gwpb, sl}});
+                cands.push_back({order, {wpb, 1, PXL_GRAD_PER_RAY, 0, sl}});
+                cands.push_back({order, {wpb, 0, PXL_GRAD_CHUNKED, 4, sl}});
+                if (tr->binned_ok)
+                    for (int32_t gwpb : {1, 4})
+                        cands.push_back({order, {wpb, 0, PXL_GRAD_BINNED, gwpb, sl}});
+            }
     constexpr int kReps = 2;
     std::vector<hipEvent_t> ev(cands.size() * kReps * 2, nullptr);
     int32_t st = PXL_OK;

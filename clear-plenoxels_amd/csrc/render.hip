@@ -82,6 +82,14 @@ struct RenderArgs {
     // float atomic per ray.
     double *rec0;
     double *rec0_part;  // [32][kFoldParts] partial sums of the fold
+    // Brick-binned gradient (schedule gradient PXL_GRAD_BINNED): per bin key
+    // (brick of 8^3 vertices x 8 classes) the sample count and the fill
+    // pointer into gb_list (the log rows in key order), the bricks with
+    // work (count in work[kTargetCounter]), the scan's block sums, per batch
+    // entry the ray's SH basis; vertex bricks per axis.
+    uint32_t *gb_count, *gb_fill, *gb_bricks, *gb_list, *gb_blk;
+    float *gb_y;
+    int vbx, vby, vbz;
     // Persistent launches: 8 claim counters per kernel (zeroed before launch).
     unsigned long long *work;
     // Learned proposal: per batch entry, the bins a ray may evaluate (null:
@@ -714,6 +722,7 @@ __global__ __launch_bounds__(256) void k_render(const RenderArgs a) {
 
 #include "march8.hpp"
 #include "adapt8.hpp"
+#include "binned.hpp"
 
 // Marcher selection: the eight-samples-per-wave kernel serves the fixed rule
 // for S <= 9 in the forward, backward, fused and trace modes; the one-sample
@@ -971,11 +980,14 @@ namespace {
 // record-0 fold's 32 x kFoldParts partial sums; 256-byte aligned.
 struct LogLayout {
     size_t work, tot, count, rec, val, pre, redo, rec0, part, total;
+    // Brick-binned gradient (only with nkeys > 0): counts and fill pointers
+    // per key, scan block sums, target bricks, per-entry SH basis.
+    size_t gb_count, gb_fill, gb_blk, gb_bricks, gb_y;
 };
 // Slices of the batch the record-0 fold's first pass sums separately.
 constexpr int kFoldParts = 64;
 inline size_t align256(size_t x) { return (x + 255) / 256 * 256; }
-LogLayout log_layout(int64_t len, int32_t cap) {
+LogLayout log_layout(int64_t len, int32_t cap, int64_t nkeys = 0) {
     LogLayout L;
     size_t o = 0;
     L.work = o;  // 16 claim counters of the persistent launches
@@ -996,8 +1008,30 @@ LogLayout log_layout(int64_t len, int32_t cap) {
     o += align256((size_t)len * 32 * sizeof(double));
     L.part = o;
     o += align256((size_t)32 * kFoldParts * sizeof(double));
+    L.gb_count = L.gb_fill = L.gb_blk = L.gb_bricks = L.gb_y = 0;
+    if (nkeys > 0) {
+        const int64_t nblk = (nkeys + kBinScanBlockKeys - 1) / kBinScanBlockKeys;
+        L.gb_count = o;
+        o += align256((size_t)nkeys * sizeof(uint32_t));
+        L.gb_fill = o;
+        o += align256((size_t)nkeys * sizeof(uint32_t));
+        L.gb_blk = o;
+        o += align256((size_t)nblk * sizeof(uint32_t));
+        L.gb_bricks = o;
+        o += align256((size_t)(nkeys / 8) * sizeof(uint32_t));
+        L.gb_y = o;
+        o += align256((size_t)len * kBinYStride * sizeof(float));
+    }
     L.total = o;
     return L;
+}
+// Bin keys of the brick-binned gradient for a grid: 8 per brick of 8^3
+// vertices (0 for grids it does not serve: each side 2 .. 1025 vertices,
+// so a cell coordinate fits in 10 bits).
+int64_t bin_keys(const pxl_grid *g) {
+    if (!g || g->W < 2 || g->H < 2 || g->D < 2 || g->W > 1025 || g->H > 1025 || g->D > 1025)
+        return 0;
+    return (int64_t)((g->W + 7) / 8) * ((g->H + 7) / 8) * ((g->D + 7) / 8) * 8;
 }
 constexpr int32_t kMaxLogSamples = 1 << 16;
 
@@ -1060,34 +1094,74 @@ unsigned resident_blocks(K kernel, int threads, size_t lds = 0) {
     return cap > 0 && cap < (int64_t)resident ? (unsigned)cap : resident;
 }
 
-// Forward + log, gradient of the logged rays, then the fused marcher again
-// for the rays longer than the log (the list the forward appended to).
-// wpb waves per block; persistent launches size the grid to what is
-// resident and let the waves claim batch entries.  The workspace's first
-// 17 words (claim counters, redo count) start at zero.
+// The brick-binned gradient after the forward + log kernel (and after
+// grad_wait): per-sample scalars and bin counts, the scan and the target
+// bricks, the scatter into key order, the LDS accumulation per brick.
+// gwpb: waves per workgroup of the per-ray kernels.  The bin counts are
+// cleared here; the target counter with the claim counters.
+template <int S, bool SPARSE, bool TGT, bool ADAPT>
+void launch_binned(const RenderArgs &a, int gwpb, hipStream_t st) {
+    const int64_t nkeys = (int64_t)a.vbx * a.vby * a.vbz * 8;
+    const unsigned nblk = (unsigned)((nkeys + kBinScanBlockKeys - 1) / kBinScanBlockKeys);
+    const int gthreads = 64 * gwpb;
+    const unsigned gblocks = (unsigned)((a.len + gwpb - 1) / gwpb);
+    hipLaunchKernelGGL(k_bin_zero, dim3((unsigned)((nkeys + 255) / 256)), dim3(256), 0, st,
+                       a.gb_count, nkeys);
+    hipLaunchKernelGGL((k_bin_samples<S, TGT, ADAPT>), dim3(gblocks), dim3(gthreads), 0, st, a);
+    hipLaunchKernelGGL(k_bin_scan_sum, dim3(nblk), dim3(kBinScanThreads), 0, st, a.gb_count,
+                       nkeys, a.gb_blk);
+    hipLaunchKernelGGL(k_bin_scan_blocks, dim3(1), dim3(kBinScanThreads), 0, st, a.gb_blk,
+                       (int64_t)nblk);
+    hipLaunchKernelGGL(k_bin_scan_final, dim3(nblk), dim3(kBinScanThreads), 0, st, a, nkeys);
+    hipLaunchKernelGGL(k_bin_scatter, dim3(gblocks), dim3(gthreads), 0, st, a);
+    constexpr size_t lds = bin_accum_lds<S>();
+    static std::once_flag attr_once;
+    std::call_once(attr_once, [] {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_bin_accum<S, SPARSE>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    });
+    const unsigned nb = (unsigned)(nkeys / 8);
+    const unsigned ga = std::min(nb, resident_blocks(k_bin_accum<S, SPARSE>, 256, lds));
+    hipLaunchKernelGGL((k_bin_accum<S, SPARSE>), dim3(ga), dim3(256), lds, st, a);
+}
+
+// The logged step's kernels in two halves, so that a batch cut into slices
+// can run slice i's gradient beside slice i+1's forward.
+//
+// Forward half: the claim / redo / target counters cleared, then the
+// forward + log kernel (wpb waves per block; persistent launches size the
+// grid to what is resident and let the waves claim batch entries).
 template <int S, bool SPARSE, int STORE>
-int32_t launch_logged(const RenderArgs &a, int wpb, int gwpb, bool persist, bool chunked,
-                      hipEvent_t grad_wait, hipStream_t st) {
-    // The gradient kernels add into planes another stream may be writing
-    // (the trainer's variance loss): they start after grad_wait.
-    bool wait_ok = true;
-    const auto wait_before_grad = [&](hipStream_t s) {
-        if (grad_wait && hipStreamWaitEvent(s, grad_wait, 0) != hipSuccess) wait_ok = false;
-    };
+void fwd_logged(const RenderArgs &a, int wpb, bool persist, hipStream_t st) {
     const int threads = 64 * wpb;
     const unsigned blocks = (unsigned)((a.len + wpb - 1) / wpb);
-    // The gradient kernels' own workgroup shape.
-    const int gthreads = 64 * gwpb;
-    const size_t glds = (size_t)gwpb * kWalkLdsBytes;  // the gradient walk's rows
-    const unsigned gblocks = (unsigned)((a.len + gwpb - 1) / gwpb);
-    const bool tgt = a.bin_targets != nullptr;
-    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, a.work, kRedoCounter + 1);
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, a.work, kTargetCounter + 1);
     if (persist) {
         const unsigned ga =
             std::min(blocks, resident_blocks(k_march8<S, SPARSE, kLog, STORE, true>, threads));
         hipLaunchKernelGGL((k_march8<S, SPARSE, kLog, STORE, true>), dim3(ga), dim3(threads), 0,
                            st, a);
-        wait_before_grad(st);
+    } else {
+        hipLaunchKernelGGL((k_march8<S, SPARSE, kLog, STORE>), dim3(blocks), dim3(threads), 0, st,
+                           a);
+    }
+}
+// Gradient half: the gradient of the logged rays under the schedule's
+// gradient mode, the fused marcher again for the rays longer than the log
+// (the list the forward appended to), the record-0 fold.
+template <int S, bool SPARSE, int STORE>
+void grad_logged(const RenderArgs &a, int wpb, int gwpb, bool persist, int gmode,
+                 hipStream_t st) {
+    const int threads = 64 * wpb;
+    const unsigned blocks = (unsigned)((a.len + wpb - 1) / wpb);
+    const int gthreads = 64 * gwpb;
+    const size_t glds = (size_t)gwpb * kWalkLdsBytes;  // the gradient walk's rows
+    const unsigned gblocks = (unsigned)((a.len + gwpb - 1) / gwpb);
+    const bool tgt = a.bin_targets != nullptr;
+    if (gmode == PXL_GRAD_BINNED) {
+        if (tgt) launch_binned<S, SPARSE, true, false>(a, gwpb, st);
+        else launch_binned<S, SPARSE, false, false>(a, gwpb, st);
+    } else if (persist) {
         if (tgt) {
             const unsigned gb =
                 std::min(gblocks, resident_blocks(k_log_grad<S, true, true>, gthreads, glds));
@@ -1097,62 +1171,54 @@ int32_t launch_logged(const RenderArgs &a, int wpb, int gwpb, bool persist, bool
                 std::min(gblocks, resident_blocks(k_log_grad<S, true>, gthreads, glds));
             hipLaunchKernelGGL((k_log_grad<S, true>), dim3(gb), dim3(gthreads), glds, st, a);
         }
-    } else if (chunked && !tgt) {
+    } else if (gmode == PXL_GRAD_CHUNKED && !tgt) {
         // Chunk-parallel gradient (the proposal's target words need the
         // whole ray in one wave, so learning batches keep the per-ray kernel).
-        hipLaunchKernelGGL((k_march8<S, SPARSE, kLog, STORE>), dim3(blocks), dim3(threads), 0, st,
-                           a);
-        wait_before_grad(st);
         hipLaunchKernelGGL((k_log_prefix<S>), dim3(gblocks), dim3(gthreads), 0, st, a);
         const int64_t items = a.len * (int64_t)a.log_chunks;
         hipLaunchKernelGGL((k_log_grad_chunked<S>), dim3((unsigned)((items + gwpb - 1) / gwpb)),
                            dim3(gthreads), glds, st, a, items);
-    } else {
-        hipLaunchKernelGGL((k_march8<S, SPARSE, kLog, STORE>), dim3(blocks), dim3(threads), 0, st,
+    } else if (tgt) {
+        hipLaunchKernelGGL((k_log_grad<S, false, true>), dim3(gblocks), dim3(gthreads), glds, st,
                            a);
-        wait_before_grad(st);
-        if (tgt)
-            hipLaunchKernelGGL((k_log_grad<S, false, true>), dim3(gblocks), dim3(gthreads), glds,
-                               st, a);
-        else
-            hipLaunchKernelGGL((k_log_grad<S>), dim3(gblocks), dim3(gthreads), glds, st, a);
+    } else {
+        hipLaunchKernelGGL((k_log_grad<S>), dim3(gblocks), dim3(gthreads), glds, st, a);
     }
     RenderArgs redo = a;
     redo.log_redo = 1;
     hipLaunchKernelGGL((k_march8_redo<S, SPARSE, STORE>), dim3(std::min(blocks, 1024u)),
                        dim3(threads), 0, st, redo);
     launch_rec0_fold(a, 1 + 3 * S, st);
-    if (!wait_ok) {
-        set_error("render: hipStreamWaitEvent before the gradient kernels failed");
-        return PXL_EHIP;
-    }
-    return PXL_OK;
 }
-// The adaptive rule's logged step: k_adapt8 (forward + log, eight rays per
-// wave, wpb waves per block), the gradient kernels over the FP64 log, the
-// one-sample fused kernel for the rays longer than the log, the record-0
-// fold.  A persistent schedule applies to the gradient kernel only.
+// The adaptive rule's forward half: k_adapt8 (forward + log, eight rays per
+// wave, wpb waves per block, slots claiming batch entries from a resident
+// grid).
 template <int S, bool SPARSE>
-int32_t launch_logged_adapt(const RenderArgs &a, int wpb, int gwpb, bool persist, bool chunked,
-                            hipEvent_t grad_wait, hipStream_t st) {
-    bool wait_ok = true;
+void fwd_logged_adapt(const RenderArgs &a, int wpb, hipStream_t st) {
     const int threads = 64 * wpb;
-    // The forward's slots claim batch entries (a.work[0..7]): a resident grid.
     const unsigned blocks =
         std::min((unsigned)((a.len + 8 * wpb - 1) / (8 * wpb)),
                  resident_blocks(k_adapt8<S, SPARSE, kLog>, threads));
-    const int gthreads = 64 * gwpb;
-    const size_t glds = (size_t)gwpb * kWalkLdsBytes;  // the gradient walk's rows
-    const unsigned gblocks = (unsigned)((a.len + gwpb - 1) / gwpb);
-    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, a.work, kRedoCounter + 1);
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, a.work, kTargetCounter + 1);
     hipLaunchKernelGGL((k_adapt8<S, SPARSE, kLog>), dim3(blocks), dim3(threads), 0, st, a);
-    if (grad_wait && hipStreamWaitEvent(st, grad_wait, 0) != hipSuccess) wait_ok = false;
-    if (persist) {
+}
+// The adaptive rule's gradient half: the gradient kernels over the FP64
+// log (a persistent schedule applies to the gradient kernel only), the
+// one-sample fused kernel for the rays longer than the log, the record-0
+// fold.
+template <int S, bool SPARSE>
+void grad_logged_adapt(const RenderArgs &a, int gwpb, bool persist, int gmode, hipStream_t st) {
+    const int gthreads = 64 * gwpb;
+    const size_t glds = (size_t)gwpb * kWalkLdsBytes;
+    const unsigned gblocks = (unsigned)((a.len + gwpb - 1) / gwpb);
+    if (gmode == PXL_GRAD_BINNED) {
+        launch_binned<S, SPARSE, false, true>(a, gwpb, st);
+    } else if (persist) {
         const unsigned gb =
             std::min(gblocks, resident_blocks(k_log_grad<S, true, false, true>, gthreads, glds));
         hipLaunchKernelGGL((k_log_grad<S, true, false, true>), dim3(gb), dim3(gthreads), glds, st,
                            a);
-    } else if (chunked) {
+    } else if (gmode == PXL_GRAD_CHUNKED) {
         hipLaunchKernelGGL((k_log_prefix<S, true>), dim3(gblocks), dim3(gthreads), 0, st, a);
         const int64_t items = a.len * (int64_t)a.log_chunks;
         hipLaunchKernelGGL((k_log_grad_chunked<S, true>),
@@ -1167,19 +1233,158 @@ int32_t launch_logged_adapt(const RenderArgs &a, int wpb, int gwpb, bool persist
     hipLaunchKernelGGL((k_render<S, SPARSE, true, kFused, PXL_STORE_F32>),
                        dim3((unsigned)((a.len + 3) / 4)), dim3(256), 0, st, redo);
     launch_rec0_fold(a, 1 + 3 * S, st);
-    if (!wait_ok) {
-        set_error("render: hipStreamWaitEvent before the gradient kernels failed");
+}
+
+// A batch of the logged step as `slices` consecutive slices, each with its
+// own part of the workspace.  One slice: forward, then (after grad_wait)
+// the gradient half, all on the caller's stream.  Several: the forwards run
+// in order on the caller's stream; the gradient halves run in order on the
+// library's second stream of the device, slice i's after slice i's forward
+// (and the first after grad_wait and everything the caller's stream held
+// at the call), so that slice i's gradient kernels overlap slice i+1's
+// forward; the caller's stream then waits for the last gradient half.
+struct LogSlices {
+    char *base;          // the workspace
+    size_t bytes;        // bytes per slice (LogLayout::total of one slice)
+    LogLayout L;         // layout of one slice
+    int64_t per;         // batch entries per slice (the last may hold fewer)
+    int count;           // slices
+    int64_t nkeys;       // bin keys (brick-binned gradient) or 0
+};
+RenderArgs slice_args(const RenderArgs &a, const LogSlices &ls, int i) {
+    RenderArgs r = a;
+    const int64_t off = (int64_t)i * ls.per;
+    r.start = a.start + off;
+    r.len = std::min(ls.per, a.len - off);
+    r.losses = a.losses + off;
+    if (a.bin_mask) r.bin_mask = a.bin_mask + off;
+    if (a.bin_targets) r.bin_targets = a.bin_targets + off;
+    if (a.target_valid) r.target_valid = a.target_valid + off;
+    char *w = ls.base + (size_t)i * ls.bytes;
+    const LogLayout &L = ls.L;
+    r.work = reinterpret_cast<unsigned long long *>(w + L.work);
+    r.log_tot = reinterpret_cast<double *>(w + L.tot);
+    r.log_count = reinterpret_cast<int32_t *>(w + L.count);
+    r.log_rec = reinterpret_cast<uint32_t *>(w + L.rec);
+    r.log_val = reinterpret_cast<float *>(w + L.val);
+    r.log_pre = reinterpret_cast<double *>(w + L.pre);
+    r.redo_list = reinterpret_cast<int32_t *>(w + L.redo);
+    r.rec0 = reinterpret_cast<double *>(w + L.rec0);
+    r.rec0_part = reinterpret_cast<double *>(w + L.part);
+    if (ls.nkeys > 0) {
+        r.gb_count = reinterpret_cast<uint32_t *>(w + L.gb_count);
+        r.gb_fill = reinterpret_cast<uint32_t *>(w + L.gb_fill);
+        r.gb_blk = reinterpret_cast<uint32_t *>(w + L.gb_blk);
+        r.gb_bricks = reinterpret_cast<uint32_t *>(w + L.gb_bricks);
+        r.gb_y = reinterpret_cast<float *>(w + L.gb_y);
+        // The log rows in key order take the value words the per-sample
+        // pass has read (one u32 per logged sample).
+        r.gb_list = reinterpret_cast<uint32_t *>(w + L.val);
+    }
+    return r;
+}
+
+// The library's second stream and events of a device for the sliced step,
+// created on first use and kept until the process ends; `mu` is held while
+// one call enqueues its work, so that calls from several host threads do
+// not interleave their event records.
+constexpr int kMaxSlices = 8;
+struct SliceStream {
+    std::mutex mu;
+    hipStream_t side = nullptr;
+    hipEvent_t ev[kMaxSlices + 1] = {};
+};
+SliceStream g_slice[64];
+std::mutex g_slice_init;
+int32_t slice_stream(SliceStream **out) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) {
+        set_error("render: no current device for the sliced step");
+        return PXL_EHIP;
+    }
+    std::lock_guard<std::mutex> lock(g_slice_init);
+    SliceStream &s = g_slice[dev];
+    if (!s.side) {
+        if (hipStreamCreateWithFlags(&s.side, hipStreamNonBlocking) != hipSuccess) {
+            s.side = nullptr;
+            set_error("render: cannot create the sliced step's stream");
+            return PXL_EHIP;
+        }
+        for (hipEvent_t &e : s.ev)
+            if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+                set_error("render: cannot create the sliced step's events");
+                return PXL_EHIP;
+            }
+    }
+    *out = &s;
+    return PXL_OK;
+}
+
+template <typename FwdF, typename GradF>
+int32_t run_slices(const RenderArgs &a, const LogSlices &ls, hipEvent_t grad_wait,
+                   hipStream_t st, FwdF fwd, GradF grad) {
+    if (ls.count <= 1) {
+        const RenderArgs a0 = slice_args(a, ls, 0);
+        fwd(a0, st);
+        if (grad_wait && hipStreamWaitEvent(st, grad_wait, 0) != hipSuccess) {
+            set_error("render: hipStreamWaitEvent before the gradient kernels failed");
+            return PXL_EHIP;
+        }
+        grad(a0, st);
+        return PXL_OK;
+    }
+    SliceStream *ss = nullptr;
+    int32_t rc = slice_stream(&ss);
+    if (rc != PXL_OK) return rc;
+    std::lock_guard<std::mutex> lock(ss->mu);
+    bool ok = hipEventRecord(ss->ev[kMaxSlices], st) == hipSuccess &&
+              hipStreamWaitEvent(ss->side, ss->ev[kMaxSlices], 0) == hipSuccess;
+    if (ok && grad_wait) ok = hipStreamWaitEvent(ss->side, grad_wait, 0) == hipSuccess;
+    for (int i = 0; ok && i < ls.count; i++) {
+        const RenderArgs ai = slice_args(a, ls, i);
+        if (ai.len <= 0) break;
+        fwd(ai, st);
+        ok = hipEventRecord(ss->ev[i], st) == hipSuccess &&
+             hipStreamWaitEvent(ss->side, ss->ev[i], 0) == hipSuccess;
+        if (ok) grad(ai, ss->side);
+    }
+    ok = ok && hipEventRecord(ss->ev[kMaxSlices], ss->side) == hipSuccess &&
+         hipStreamWaitEvent(st, ss->ev[kMaxSlices], 0) == hipSuccess;
+    if (!ok) {
+        set_error("render: event ordering of the sliced step failed");
         return PXL_EHIP;
     }
     return PXL_OK;
 }
 
+template <int S, bool SPARSE, int STORE>
+int32_t launch_logged(const RenderArgs &a, const LogSlices &ls, int wpb, int gwpb, bool persist,
+                      int gmode, hipEvent_t grad_wait, hipStream_t st) {
+    return run_slices(
+        a, ls, grad_wait, st,
+        [&](const RenderArgs &ai, hipStream_t s) { fwd_logged<S, SPARSE, STORE>(ai, wpb, persist, s); },
+        [&](const RenderArgs &ai, hipStream_t s) {
+            grad_logged<S, SPARSE, STORE>(ai, wpb, gwpb, persist, gmode, s);
+        });
+}
+template <int S, bool SPARSE>
+int32_t launch_logged_adapt(const RenderArgs &a, const LogSlices &ls, int wpb, int gwpb,
+                            bool persist, int gmode, hipEvent_t grad_wait, hipStream_t st) {
+    return run_slices(
+        a, ls, grad_wait, st,
+        [&](const RenderArgs &ai, hipStream_t s) { fwd_logged_adapt<S, SPARSE>(ai, wpb, s); },
+        [&](const RenderArgs &ai, hipStream_t s) {
+            grad_logged_adapt<S, SPARSE>(ai, gwpb, persist, gmode, s);
+        });
+}
+
 template <int S>
-int32_t dispatch_logged_s(bool sparse, int store, const RenderArgs &a, int wpb, int gwpb,
-                          bool persist, bool chunked, hipEvent_t grad_wait, hipStream_t st) {
+int32_t dispatch_logged_s(bool sparse, int store, const RenderArgs &a, const LogSlices &ls,
+                          int wpb, int gwpb, bool persist, int gmode, hipEvent_t grad_wait,
+                          hipStream_t st) {
 #define PXL_LOGGED(SP, ST)                                                        \
     if (sparse == SP && store == ST)                                              \
-        return launch_logged<S, SP, ST>(a, wpb, gwpb, persist, chunked, grad_wait, st);
+        return launch_logged<S, SP, ST>(a, ls, wpb, gwpb, persist, gmode, grad_wait, st);
     PXL_LOGGED(true, PXL_STORE_F32)
     PXL_LOGGED(true, PXL_STORE_BF16)
     PXL_LOGGED(true, PXL_STORE_F16)
@@ -1190,9 +1395,9 @@ int32_t dispatch_logged_s(bool sparse, int store, const RenderArgs &a, int wpb, 
     set_error("render: unknown storage %d", store);
     return PXL_EINVAL;
 }
-int32_t dispatch_logged(int S, bool sparse, int store, const RenderArgs &a, int wpb, int gwpb,
-                        bool persist, bool chunked, hipEvent_t gw, hipStream_t st,
-                        bool adapt = false) {
+int32_t dispatch_logged(int S, bool sparse, int store, const RenderArgs &a, const LogSlices &ls,
+                        int wpb, int gwpb, bool persist, int gmode, hipEvent_t gw,
+                        hipStream_t st, bool adapt = false) {
     if (adapt) {
         if (store != PXL_STORE_F32) {
             set_error("render: the adaptive marching rule needs FP32 storage");
@@ -1200,8 +1405,9 @@ int32_t dispatch_logged(int S, bool sparse, int store, const RenderArgs &a, int 
         }
 #define PXL_ADAPT(SV)                                                                         \
     case SV:                                                                                  \
-        return sparse ? launch_logged_adapt<SV, true>(a, wpb, gwpb, persist, chunked, gw, st) \
-                      : launch_logged_adapt<SV, false>(a, wpb, gwpb, persist, chunked, gw, st);
+        return sparse                                                                          \
+                   ? launch_logged_adapt<SV, true>(a, ls, wpb, gwpb, persist, gmode, gw, st)  \
+                   : launch_logged_adapt<SV, false>(a, ls, wpb, gwpb, persist, gmode, gw, st);
         switch (S) {
             PXL_ADAPT(1)
             PXL_ADAPT(4)
@@ -1213,9 +1419,9 @@ int32_t dispatch_logged(int S, bool sparse, int store, const RenderArgs &a, int 
 #undef PXL_ADAPT
     }
     switch (S) {
-        case 1: return dispatch_logged_s<1>(sparse, store, a, wpb, gwpb, persist, chunked, gw, st);
-        case 4: return dispatch_logged_s<4>(sparse, store, a, wpb, gwpb, persist, chunked, gw, st);
-        case 9: return dispatch_logged_s<9>(sparse, store, a, wpb, gwpb, persist, chunked, gw, st);
+        case 1: return dispatch_logged_s<1>(sparse, store, a, ls, wpb, gwpb, persist, gmode, gw, st);
+        case 4: return dispatch_logged_s<4>(sparse, store, a, ls, wpb, gwpb, persist, gmode, gw, st);
+        case 9: return dispatch_logged_s<9>(sparse, store, a, ls, wpb, gwpb, persist, gmode, gw, st);
         default:
             set_error("render: the logged fused step supports sh_dim 1, 4 and 9");
             return PXL_EINVAL;
@@ -1279,34 +1485,53 @@ static int32_t bwd_common(const pxl_grid *grid, const pxl_optim *optim, const px
         const int32_t max_samples = x.max_samples;
         PXL_REQUIRE(max_samples > 0 && max_samples <= kMaxLogSamples,
                     "%s: max_samples %d outside [1, %d]", who, max_samples, kMaxLogSamples);
-        const LogLayout L = log_layout(len, max_samples);
-        PXL_REQUIRE(x.workspace_bytes >= L.total,
-                    "%s: workspace of %zu bytes, %zu needed for %lld rays x %d samples", who,
-                    x.workspace_bytes, L.total, (long long)len, max_samples);
+        const int gmode = sched ? sched->gradient : PXL_GRAD_PER_RAY;
+        PXL_REQUIRE(gmode == PXL_GRAD_PER_RAY || gmode == PXL_GRAD_CHUNKED ||
+                        gmode == PXL_GRAD_BINNED,
+                    "%s: unknown gradient schedule %d", who, gmode);
+        const int64_t nkeys = gmode == PXL_GRAD_BINNED ? bin_keys(grid) : 0;
+        PXL_REQUIRE(gmode != PXL_GRAD_BINNED || nkeys > 0,
+                    "%s: the brick-binned gradient needs 2 to 1025 vertices per side (grid "
+                    "%d x %d x %d)",
+                    who, grid->W, grid->H, grid->D);
+        const int slices = sched && sched->slices > 1 ? sched->slices : 1;
+        PXL_REQUIRE(slices <= kMaxSlices, "%s: %d slices (at most %d)", who, slices, kMaxSlices);
+        const int64_t per = (len + slices - 1) / slices;
+        const LogLayout L = log_layout(per, max_samples, nkeys);
+        const size_t need = (size_t)slices * L.total;
+        PXL_REQUIRE(x.workspace_bytes >= need,
+                    "%s: workspace of %zu bytes, %zu needed for %lld rays x %d samples in %d "
+                    "slice(s)%s",
+                    who, x.workspace_bytes, need, (long long)len, max_samples, slices,
+                    nkeys > 0 ? " with the brick-binned gradient "
+                                "(pxl_train_rays_workspace_bytes_grid)"
+                              : "");
         PXL_REQUIRE(((uintptr_t)x.workspace & 255) == 0,
                     "%s: workspace must be 256-byte aligned", who);
-        char *w = static_cast<char *>(x.workspace);
-        a.log_tot = reinterpret_cast<double *>(w + L.tot);
-        a.log_count = reinterpret_cast<int32_t *>(w + L.count);
-        a.log_rec = reinterpret_cast<uint32_t *>(w + L.rec);
-        a.log_val = reinterpret_cast<float *>(w + L.val);
+        LogSlices ls;
+        ls.base = static_cast<char *>(x.workspace);
+        ls.bytes = L.total;
+        ls.L = L;
+        ls.per = per;
+        ls.count = (int)((len + per - 1) / per);
+        ls.nkeys = nkeys;
         a.log_cap = max_samples;
-        a.log_pre = reinterpret_cast<double *>(w + L.pre);
-        a.redo_list = reinterpret_cast<int32_t *>(w + L.redo);
-        a.rec0 = reinterpret_cast<double *>(w + L.rec0);
-        a.rec0_part = reinterpret_cast<double *>(w + L.part);
         a.log_chunks = (max_samples + 63) / 64;
-        a.work = reinterpret_cast<unsigned long long *>(w + L.work);
+        if (nkeys > 0) {
+            a.vbx = (grid->W + 7) / 8;
+            a.vby = (grid->H + 7) / 8;
+            a.vbz = (grid->D + 7) / 8;
+        }
         const int wpb = sched ? sched->waves_per_block : 4;
         PXL_REQUIRE(wpb == 1 || wpb == 2 || wpb == 4, "%s: waves_per_block %d (1, 2 or 4)", who,
                     wpb);
         const int gwpb = sched && sched->grad_waves_per_block ? sched->grad_waves_per_block : wpb;
         PXL_REQUIRE(gwpb == 1 || gwpb == 2 || gwpb == 4,
                     "%s: grad_waves_per_block %d (0, 1, 2 or 4)", who, gwpb);
-        PXL_REQUIRE(!sched || !(sched->persistent && sched->chunked),
+        PXL_REQUIRE(!sched || !(sched->persistent && gmode == PXL_GRAD_CHUNKED),
                     "%s: the chunk-parallel gradient has no persistent form", who);
-        st = dispatch_logged(grid->sh_dim, grid->sparse, grid->storage, a, wpb, gwpb,
-                             sched && sched->persistent, sched && sched->chunked, x.grad_wait,
+        st = dispatch_logged(grid->sh_dim, grid->sparse, grid->storage, a, ls, wpb, gwpb,
+                             sched && sched->persistent, gmode, x.grad_wait,
                              as_stream(stream), adapt);
         return st != PXL_OK ? st : check_launch(who);
     }
@@ -1347,9 +1572,27 @@ extern "C" int32_t pxl_train_rays(const pxl_grid *grid, const pxl_optim *optim,
                       huber_delta, stream, true);
 }
 
+namespace pxl {
+namespace {
+// Workspace of a batch under every slice count 1 .. kMaxSlices.
+size_t log_bytes_any(int64_t len, int32_t cap, int64_t nkeys) {
+    size_t most = 0;
+    for (int k = 1; k <= kMaxSlices; k++)
+        most = std::max(most, (size_t)k * log_layout((len + k - 1) / k, cap, nkeys).total);
+    return most;
+}
+}  // namespace
+}  // namespace pxl
+
 extern "C" size_t pxl_train_rays_workspace_bytes(int64_t len, int32_t max_samples) {
     if (len <= 0 || max_samples <= 0 || max_samples > kMaxLogSamples) return 0;
-    return log_layout(len, max_samples).total;
+    return log_bytes_any(len, max_samples, 0);
+}
+
+extern "C" size_t pxl_train_rays_workspace_bytes_grid(const pxl_grid *grid, int64_t len,
+                                                      int32_t max_samples) {
+    if (len <= 0 || max_samples <= 0 || max_samples > kMaxLogSamples) return 0;
+    return log_bytes_any(len, max_samples, bin_keys(grid));
 }
 
 extern "C" int32_t pxl_train_rays_ws(const pxl_grid *grid, const pxl_optim *optim,
@@ -1383,7 +1626,7 @@ extern "C" int32_t pxl_train_rays_prop(const pxl_grid *grid, const pxl_optim *op
         set_error("pxl_train_rays_prop: null workspace");
         return PXL_EINVAL;
     }
-    const pxl_schedule dflt = {4, 0, 0, 0};
+    const pxl_schedule dflt = {4, 0, 0, 0, 0};
     TrainExtra x;
     x.workspace = workspace;
     x.workspace_bytes = workspace_bytes;
@@ -1465,7 +1708,7 @@ extern "C" int32_t pxl_train_rays_noalloc(const pxl_grid *grid, const pxl_optim 
     PXL_REQUIRE(len >= 0 && max_samples > 0 && max_samples <= kMaxLogSamples,
                 "%s: len %lld / max_samples %d outside [1, %d]", who, (long long)len, max_samples,
                 kMaxLogSamples);
-    const size_t need = log_layout(len, max_samples).total;
+    const size_t need = log_bytes_any(len, max_samples, 0);
     PXL_REQUIRE(need <= (size_t)PXL_NOALLOC_SCRATCH_BYTES,
                 "%s: %lld rays x %d samples need a %zu-byte sample log, beyond the library's "
                 "%zu bytes (PXL_NOALLOC_SCRATCH_BYTES); use pxl_train_rays_sched with a workspace",
@@ -1473,7 +1716,7 @@ extern "C" int32_t pxl_train_rays_noalloc(const pxl_grid *grid, const pxl_optim 
     void *ws = nullptr;
     const int32_t st = scratch_of_device(&ws, who);
     if (st != PXL_OK) return st;
-    const pxl_schedule dflt = {4, 0, 0, 0};
+    const pxl_schedule dflt = {4, 0, 0, 0, 0};
     TrainExtra x;
     x.workspace = ws;
     x.workspace_bytes = (size_t)PXL_NOALLOC_SCRATCH_BYTES;

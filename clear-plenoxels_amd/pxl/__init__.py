@@ -34,7 +34,8 @@ EXPORTS = (
     "pxl_trainer_time_render", "pxl_trainer_render_ms", "pxl_trainer_loss",
     "pxl_trainer_state", "pxl_trainer_destroy", "pxl_brick_words",
     "pxl_build_brick_occupancy", "pxl_records_to_16bit", "pxl_tv_prepare",
-    "pxl_render_kernel_name", "pxl_train_rays_workspace_bytes", "pxl_train_rays_ws",
+    "pxl_render_kernel_name", "pxl_train_rays_workspace_bytes",
+    "pxl_train_rays_workspace_bytes_grid", "pxl_train_rays_ws",
     "pxl_train_rays_sched", "pxl_batch_order_scratch_bytes", "pxl_batch_order",
     "pxl_trainer_schedule", "pxl_proposal_workspace_bytes", "pxl_proposal_init",
     "pxl_proposal_features", "pxl_proposal_forward", "pxl_proposal_train_step",
@@ -79,10 +80,10 @@ def lib():
             raise ImportError(f"libpxl.so not built at {LIB_PATH}; run __graft_entry__.build()")
         L = ctypes.CDLL(LIB_PATH)
         _declare(L)
-        sizes = (ctypes.c_int64 * 8)()
-        L.pxl_abi_sizes(sizes, 8)
+        sizes = (ctypes.c_int64 * 9)()
+        L.pxl_abi_sizes(sizes, 9)
         mine = [ctypes.sizeof(c) for c in (Camera, BBox, RaysC, GridC, OptimC, TrainConfigC,
-                                           LevelConfigC, LevelC)]
+                                           LevelConfigC, LevelC, ScheduleC)]
         if list(sizes) != mine:
             raise ImportError(f"libpxl.so struct sizes {list(sizes)} != binding {mine}")
         _lib = L
@@ -134,8 +135,8 @@ class TrainConfigC(ctypes.Structure):
                 ("variance_interval", _I32), ("optimize_step", _I32),
                 ("mask_coefficient", _F64), ("huber_delta", _F32), ("fused", _I32),
                 ("log_samples", _I32), ("autotune", _I32), ("batch_order", _I32),
-                ("waves_per_block", _I32), ("persistent", _I32), ("chunked", _I32),
-                ("tv_inline", _I32), ("tv_separate", _I32)]
+                ("waves_per_block", _I32), ("persistent", _I32), ("gradient", _I32),
+                ("tv_inline", _I32), ("tv_separate", _I32), ("slices", _I32)]
 
 
 class LevelConfigC(ctypes.Structure):
@@ -155,8 +156,19 @@ class ProposalC(ctypes.Structure):
 
 
 class ScheduleC(ctypes.Structure):
-    _fields_ = [("waves_per_block", _I32), ("persistent", _I32), ("chunked", _I32),
-                ("grad_waves_per_block", _I32)]
+    _fields_ = [("waves_per_block", _I32), ("persistent", _I32), ("gradient", _I32),
+                ("grad_waves_per_block", _I32), ("slices", _I32)]
+
+
+# Gradient schedules of the logged step (pxl_schedule.gradient).
+GRAD_PER_RAY, GRAD_CHUNKED, GRAD_BINNED = 0, 1, 2
+GRAD_NAMES = {GRAD_PER_RAY: "per_ray", GRAD_CHUNKED: "chunked", GRAD_BINNED: "binned"}
+
+
+def _grad_mode(chunked=False, binned=False):
+    if chunked and binned:
+        raise ValueError("chunked and binned are two different gradient schedules")
+    return GRAD_BINNED if binned else (GRAD_CHUNKED if chunked else GRAD_PER_RAY)
 
 
 def _declare(L):
@@ -202,6 +214,7 @@ def _declare(L):
         "pxl_records_to_16bit": ([_P, _I32, _I64, _P, _I32, _I32, _P], _I32),
         "pxl_render_kernel_name": ([G, _I32], ctypes.c_char_p),
         "pxl_train_rays_workspace_bytes": ([_I64, _I32], _SZ),
+        "pxl_train_rays_workspace_bytes_grid": ([G, _I64, _I32], _SZ),
         "pxl_train_rays_ws": ([G, O, R, _P, _P, _I64, _I64, _F64, _F32, _P, _SZ, _I32, _P],
                               _I32),
         "pxl_train_rays_sched": ([G, O, R, _P, _P, _I64, _I64, _F64, _F32, _P, _SZ, _I32,
@@ -513,12 +526,17 @@ def train_rays(grid, optim, rays, losses, sample_index, start=0, length=None, ma
                                 _stream(stream)))
 
 
-def train_workspace(length, max_samples, device="cuda"):
+def train_workspace(length, max_samples, device="cuda", grid=None):
     """Device workspace for train_rays_ws over `length` rays logging up to
-    `max_samples` samples per ray."""
+    `max_samples` samples per ray; with `grid`, also for the brick-binned
+    gradient schedule on that grid."""
     import torch
 
-    nbytes = lib().pxl_train_rays_workspace_bytes(length, max_samples)
+    if grid is None:
+        nbytes = lib().pxl_train_rays_workspace_bytes(length, max_samples)
+    else:
+        g = grid.c()
+        nbytes = lib().pxl_train_rays_workspace_bytes_grid(ctypes.byref(g), length, max_samples)
     if nbytes == 0:
         raise PxlError("pxl_train_rays_workspace_bytes", PXL_EINVAL,
                        f"bad length {length} / max_samples {max_samples}")
@@ -543,13 +561,18 @@ ORDER_NONE, ORDER_MORTON, ORDER_LENGTH_MORTON = 0, 1, 2
 
 def train_rays_sched(grid, optim, rays, losses, sample_index, workspace, max_samples,
                      waves_per_block=4, persistent=False, chunked=False, grad_waves_per_block=0,
-                     start=0, length=None, mask_coef=1.0, huber_delta=0.1, stream=None):
+                     start=0, length=None, mask_coef=1.0, huber_delta=0.1, stream=None,
+                     binned=False, slices=1):
     """train_rays_ws with an explicit launch schedule (chunked: one gradient
-    wave per 64-sample chunk of each ray's log; grad_waves_per_block: the
-    gradient kernel's waves per workgroup, 0 = waves_per_block)."""
+    wave per 64-sample chunk of each ray's log; binned: the brick-binned
+    gradient, workspace from train_workspace(..., grid=grid);
+    grad_waves_per_block: the per-ray gradient kernels' waves per workgroup,
+    0 = waves_per_block; slices: the batch in that many slices, one slice's
+    gradient beside the next slice's forward)."""
     length = sample_index.numel() - start if length is None else length
     g, o, r = grid.c(), optim.c(), rays.c()
-    sc = ScheduleC(waves_per_block, int(persistent), int(chunked), int(grad_waves_per_block))
+    sc = ScheduleC(waves_per_block, int(persistent), _grad_mode(chunked, binned),
+                   int(grad_waves_per_block), int(slices))
     _check("pxl_train_rays_sched",
            lib().pxl_train_rays_sched(ctypes.byref(g), ctypes.byref(o), ctypes.byref(r),
                                       _ptr(losses), _ptr(sample_index), start, length, mask_coef,
@@ -665,12 +688,14 @@ def render_fwd_masked(grid, rays, sample_index, bin_mask=None, start=0, length=N
 def train_rays_prop(grid, optim, rays, losses, sample_index, workspace, max_samples,
                     bin_mask=None, bin_targets=None, target_valid=None, target_wmin=1e-4,
                     waves_per_block=4, persistent=False, chunked=False, start=0, length=None,
-                    mask_coef=1.0, huber_delta=0.1, stream=None):
+                    mask_coef=1.0, huber_delta=0.1, stream=None, binned=False, slices=1):
     """The two-kernel step with an optional proposal mask and target output
-    (batches with targets run the gradient one wave per ray)."""
+    (batches with targets run the chunked schedule's gradient one wave per
+    ray)."""
     length = sample_index.numel() - start if length is None else length
     g, o, r = grid.c(), optim.c(), rays.c()
-    sc = ScheduleC(waves_per_block, int(persistent), int(chunked), 0)
+    sc = ScheduleC(waves_per_block, int(persistent), _grad_mode(chunked, binned), 0,
+                   int(slices))
     _check("pxl_train_rays_prop",
            lib().pxl_train_rays_prop(ctypes.byref(g), ctypes.byref(o), ctypes.byref(r),
                                      _ptr(losses), _ptr(sample_index), start, length, mask_coef,
@@ -874,7 +899,7 @@ class Trainer:
                  variance_interval=3, optimize_step=0, mask_coefficient=1.0, huber_delta=0.1,
                  fused=True, log_samples=0, autotune=False, batch_order=0,
                  waves_per_block=0, persistent=False, chunked=False, tv_inline=False,
-                 tv_separate=False, stream=None):
+                 tv_separate=False, stream=None, binned=False, slices=0):
         import torch
 
         self._keep = (grid, optim, rays, order)
@@ -885,8 +910,8 @@ class Trainer:
                            sh_coefficient, variance_type, neighbour_nums, variance_interval,
                            optimize_step, mask_coefficient, huber_delta, int(fused),
                            int(log_samples), int(autotune), int(batch_order),
-                           int(waves_per_block), int(persistent), int(chunked), int(tv_inline),
-                           int(tv_separate))
+                           int(waves_per_block), int(persistent), _grad_mode(chunked, binned),
+                           int(tv_inline), int(tv_separate), int(slices))
         self._g, self._o, self._r = grid.c(), optim.c(), rays.c()
         n_rays = order.numel() if n_rays is None else n_rays
         sc = tv_scratch(grid, stream) if variance_type in (TV_DISPARITY, TV_BOTH) else None
@@ -978,13 +1003,13 @@ class Trainer:
         return v.value
 
     def schedule(self):
-        """(batch order, waves per block, persistent, chunked, gradient waves
-        per block) the trainer runs."""
+        """(batch order, waves per block, persistent, gradient schedule
+        GRAD_*, gradient waves per block, slices) the trainer runs."""
         order, sc = _I32(), ScheduleC()
         _check("pxl_trainer_schedule",
                lib().pxl_trainer_schedule(self._h, ctypes.byref(order), ctypes.byref(sc)))
-        return (order.value, sc.waves_per_block, bool(sc.persistent), bool(sc.chunked),
-                sc.grad_waves_per_block or sc.waves_per_block)
+        return (order.value, sc.waves_per_block, bool(sc.persistent), sc.gradient,
+                sc.grad_waves_per_block or sc.waves_per_block, max(1, sc.slices))
 
     def close(self):
         if getattr(self, "_h", None):

@@ -176,8 +176,8 @@ int32_t pxl_get_option(int32_t key, int64_t *value);
 int32_t pxl_version(void);
 
 /* sizeof of pxl_camera, pxl_bbox, pxl_rays, pxl_grid, pxl_optim,
- * pxl_train_config, pxl_level_config and pxl_level, for bindings to check
- * their mirrors; returns 8. */
+ * pxl_train_config, pxl_level_config, pxl_level and pxl_schedule, for
+ * bindings to check their mirrors; returns 9. */
 int32_t pxl_abi_sizes(int64_t *out, int32_t n);
 
 /* Host-side camera of one pose (readFromMyBlender per view,
@@ -488,28 +488,59 @@ int32_t pxl_count_touched(const pxl_grid *grid, const pxl_rays *rays,
  * max_samples evaluated samples is differentiated by the marcher itself.
  * Grids the eight-sample marcher does not serve (adaptive rule, S = 16) run
  * as pxl_train_rays.  workspace: 256-byte aligned device memory of at least
- * pxl_train_rays_workspace_bytes(len, max_samples) bytes; calls sharing a
- * workspace must not overlap.  No counterpart in the reference (an
+ * pxl_train_rays_workspace_bytes(len, max_samples) bytes (enough for every
+ * schedule below with up to 8 slices, except the brick-binned gradient);
+ * calls sharing a workspace must not overlap.  No counterpart in the reference (an
  * implementation of renderRays_withGradients_RGB, RenderUtils.cu:2257-2278,
  * fused with renderRays). */
 size_t pxl_train_rays_workspace_bytes(int64_t len, int32_t max_samples);
+/* Workspace bytes of the logged step on `grid` under every gradient
+ * schedule, the brick-binned one (PXL_GRAD_BINNED) included: its bin arrays
+ * (8 bytes per bin key, 8 keys per brick of 8^3 vertices) and one SH basis
+ * of 9 floats per batch entry come on top of the sample log.  0 for bad
+ * sizes.  No counterpart in the reference. */
+size_t pxl_train_rays_workspace_bytes_grid(const pxl_grid *grid, int64_t len,
+                                           int32_t max_samples);
 
 /* Launch schedule of the two-kernel step: waves (rays) per workgroup, 1, 2
  * or 4, of the forward + log kernel and (grad_waves_per_block, 0 = the same)
- * of the gradient kernel; persistent != 0 launches as many workgroups as fit on the device at
- * once and lets each wave claim batch entries (XCD-sliced claim counters in
- * the workspace) until the batch is done, for batches of very uneven ray
- * lengths; chunked != 0 (not with persistent) runs the gradient kernel as one
- * wave per 64-sample chunk of each ray's log after a pass that stores each
- * chunk's FP64 prefix, for batches too small to fill the device, where the
- * longest rays would otherwise set the kernel's duration (batches with
- * proposal targets keep one wave per ray).  The results do not depend on the
- * schedule beyond the order of float atomics. */
+ * of the per-ray gradient kernels; persistent != 0 launches as many
+ * workgroups as fit on the device at once and lets each wave claim batch
+ * entries (XCD-sliced claim counters in the workspace) until the batch is
+ * done, for batches of very uneven ray lengths.  gradient selects how the
+ * gradient of the logged samples reaches the planes:
+ *   PXL_GRAD_PER_RAY  one wave per ray walks its samples and adds the
+ *                     corners each cell change leaves with float atomics;
+ *   PXL_GRAD_CHUNKED  (not with persistent) the same walk as one wave per
+ *                     64-sample chunk of each ray's log after a pass that
+ *                     stores each chunk's FP64 prefix, for batches too small
+ *                     to fill the device (batches with proposal targets keep
+ *                     one wave per ray);
+ *   PXL_GRAD_BINNED   the samples' gradient scalars are binned by brick of
+ *                     8^3 vertices and one workgroup per brick sums them in
+ *                     LDS and updates each touched record once, without
+ *                     float atomics (workspace:
+ *                     pxl_train_rays_workspace_bytes_grid; grids of 2 to
+ *                     1025 vertices per side).
+ * The results do not depend on the schedule beyond the order of float
+ * additions. */
+enum {
+    PXL_GRAD_PER_RAY = 0,
+    PXL_GRAD_CHUNKED = 1,
+    PXL_GRAD_BINNED = 2
+};
 typedef struct pxl_schedule {
     int32_t waves_per_block;
     int32_t persistent;
-    int32_t chunked;
+    int32_t gradient;            /* PXL_GRAD_* */
     int32_t grad_waves_per_block;
+    /* 0 or 1: each kernel once over the batch; 2 .. 8: the batch in that
+     * many consecutive slices, each with its part of the workspace; the
+     * forwards run in order on `stream`, the gradient kernels of slice i on
+     * the library's second stream of the device once slice i's forward is
+     * done, beside slice i + 1's forward; `stream` then waits for the last
+     * of them. */
+    int32_t slices;
 } pxl_schedule;
 int32_t pxl_train_rays_ws(const pxl_grid *grid, const pxl_optim *optim,
                           const pxl_rays *rays, double *losses,
@@ -687,8 +718,8 @@ typedef struct pxl_train_config {
     int32_t batch_order;     /* 0 as given, PXL_ORDER_* (when not autotuned) */
     int32_t waves_per_block; /* 1, 2 or 4 (when not autotuned; 0 = 4) */
     int32_t persistent;      /* when not autotuned */
-    int32_t chunked;         /* when not autotuned: chunk-parallel gradient
-                                (pxl_schedule), not with persistent */
+    int32_t gradient;        /* when not autotuned: PXL_GRAD_* (pxl_schedule);
+                                PXL_GRAD_CHUNKED not with persistent */
     int32_t tv_inline;       /* 1: the variance loss runs on the trainer's
                                 stream after the render (no overlap) */
     int32_t tv_separate;     /* 1: the disparity variance loss and Adam run
@@ -696,6 +727,7 @@ typedef struct pxl_train_config {
                                 (pxl_tv_adam_step) on the batches that take
                                 the variance loss, the trainer keeping a
                                 second record buffer (pxl_trainer_sync) */
+    int32_t slices;          /* when not autotuned: pxl_schedule slices */
 } pxl_train_config;
 
 typedef struct pxl_trainer pxl_trainer;

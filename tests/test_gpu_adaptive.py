@@ -11,7 +11,8 @@ whose corners exceed 12), 8 views of 64 x 64 on a radius-4 sphere, random
   and caged;
 - backward on every path: the one-sample fused kernel, the two-kernel
   logged step (log holding every ray, or 24 samples so that long rays take
-  the redo launch), its chunk-parallel and persistent gradient schedules;
+  the redo launch), its chunk-parallel, persistent and brick-binned
+  gradient schedules, the batch in 3 slices;
   clipped (Huber delta 0.1) and unclipped (10) residuals; losses within
   255 * 1e-5, gradients helpers.grad_close (record 0 of the caged grid 1e-4
   relative on the logged paths, 1e-3 on the single fused kernel);
@@ -91,7 +92,8 @@ def test_adaptive_forward(sc, kind):
 
 
 @pytest.mark.parametrize("kind", ["dense", "caged"])
-@pytest.mark.parametrize("path", ["fused", "logged", "logged_24", "chunked", "persistent"])
+@pytest.mark.parametrize("path", ["fused", "logged", "logged_24", "chunked", "persistent",
+                                  "binned", "binned_24", "sliced"])
 @pytest.mark.parametrize("huber", [0.1, 10.0])
 def test_adaptive_backward(sc, kind, path, huber):
     import torch
@@ -104,18 +106,21 @@ def test_adaptive_backward(sc, kind, path, huber):
     if path == "fused":
         pxl.train_rays(grid, optim, rays, losses, sc["si"], huber_delta=huber)
     else:
-        cap = 24 if path == "logged_24" else 512
-        ws = pxl.train_workspace(NR, cap, dev)
+        cap = 24 if path.endswith("_24") else 512
+        binned = path.startswith("binned")
+        ws = pxl.train_workspace(NR, cap, dev, grid=grid if binned else None)
         ws.fill_(0xAB)  # stale contents must not leak into the result
         pxl.train_rays_sched(grid, optim, rays, losses, sc["si"], ws, cap,
                              waves_per_block=1 if path == "persistent" else 4,
                              persistent=path == "persistent", chunked=path == "chunked",
-                             huber_delta=huber)
+                             binned=binned, huber_delta=huber,
+                             slices=3 if path == "sliced" else 1)
         torch.cuda.synchronize()
-        off = 256 + (NR * 32 + 255) // 256 * 256
-        cnt = ws[off:off + 4 * NR].view(torch.int32).cpu().numpy()
-        n_redo = int((cnt == -1).sum())
-        assert int((cnt > 0).sum()) > 0 and (n_redo > 0) == (cap == 24), n_redo
+        if path != "sliced":  # one slice: the counts follow the forward totals
+            off = 256 + (NR * 32 + 255) // 256 * 256
+            cnt = ws[off:off + 4 * NR].view(torch.int32).cpu().numpy()
+            n_redo = int((cnt == -1).sum())
+            assert int((cnt > 0).sum()) > 0 and (n_redo > 0) == (cap == 24), n_redo
     torch.cuda.synchronize()
     ro = oracle.Rays(sc["rays_np"])
     fwd = oracle.render_fwd(go, ro, sc["batch"])

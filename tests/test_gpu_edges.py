@@ -1,6 +1,7 @@
 """Edge cases of the train step against the oracle, on every launch path
 (fused kernel, logged two-kernel step, logged step with rays redone past a
-short log, persistent single-wave blocks, chunk-parallel gradient):
+short log, persistent single-wave blocks, chunk-parallel gradient,
+brick-binned gradient with a full and a short log, the batch in 3 slices):
 
 - ragged batch sizes: 1, 63, 65 and 257 rays (partial waves and blocks);
 - rays that miss the grid (near > far: outputs zeroed and no sample,
@@ -29,7 +30,7 @@ from helpers import (c1_planes, cameras, device_sparse_grid, from_records, grad_
 pytestmark = pytest.mark.gpu
 
 N, S, IMG = 48, 9, 48
-PATHS = ("fused", "logged", "redo", "persistent", "chunked")
+PATHS = ("fused", "logged", "redo", "persistent", "chunked", "binned", "binned_redo", "sliced")
 
 
 def say(*a):
@@ -68,13 +69,18 @@ def _run(grid, rays, si, path, dev):
     if path == "fused":
         pxl.train_rays(grid, optim, rays, losses, si)
     else:
-        cap = 8 if path == "redo" else 512
-        ws = pxl.train_workspace(max(nr, 1), cap, dev)
-        if path == "persistent":
+        cap = 8 if path in ("redo", "binned_redo") else 512
+        binned = path.startswith("binned")
+        ws = pxl.train_workspace(max(nr, 1), cap, dev, grid=grid if binned else None)
+        if binned:
+            pxl.train_rays_sched(grid, optim, rays, losses, si, ws, cap, binned=True)
+        elif path == "persistent":
             pxl.train_rays_sched(grid, optim, rays, losses, si, ws, cap, waves_per_block=1,
                                  persistent=True)
         elif path == "chunked":
             pxl.train_rays_sched(grid, optim, rays, losses, si, ws, cap, chunked=True)
+        elif path == "sliced":
+            pxl.train_rays_sched(grid, optim, rays, losses, si, ws, cap, slices=3)
         else:
             pxl.train_rays_ws(grid, optim, rays, losses, si, ws, cap)
     torch.cuda.synchronize()

@@ -7,8 +7,9 @@ four 800 x 800 views of the upper hemisphere, random 8-bit targets.
 - 128 rays of the batch against the FP64 oracle on the full grid (colours
   1e-5, losses 255 * 1e-5, gradients helpers.grad_close), fused and logged;
 - the whole batch against the FP64 oracle (threaded on the host): the
-  fused kernel, the logged step, the persistent one-wave schedule and the
-  chunk-parallel gradient give the oracle's per-ray colours and losses, and
+  fused kernel, the logged step, the persistent one-wave schedule, the
+  chunk-parallel gradient, the brick-binned gradient and the batch in four
+  slices give the oracle's per-ray colours and losses, and
   gradients within helpers.grad_close or, for entries whose terms cancel
   (one entry here is 9.9e-7 from four rays' terms of 1e-3 in magnitude),
   within 1e-6 of the entry's magnitude sum (helpers.grad_close_cond: FP32
@@ -20,8 +21,8 @@ four 800 x 800 views of the upper hemisphere, random 8-bit targets.
   ray and is held to 1e-3 there;
 - the whole batch's gradient is the sum of its two halves' (no cross-ray
   interference, record 0 included, 1e-4);
-- 128 rays of the batch under the reference's adaptive rule (fused and
-  logged) against the oracle;
+- 128 rays of the batch under the reference's adaptive rule (fused, logged
+  and brick-binned) against the oracle;
 - TV over the full caged grid and one Adam step over all records against
   the oracle.
 """
@@ -142,12 +143,18 @@ def _train(c2, si, path):
     if path == "fused":
         pxl.train_rays(grid, optim, c2["rays"], losses, si)
     else:
-        ws = pxl.train_workspace(n, 512, c2["dev"])
+        ws = pxl.train_workspace(n, 512, c2["dev"], grid=grid if path == "binned" else None)
         if path == "persistent":
             pxl.train_rays_sched(grid, optim, c2["rays"], losses, si, ws, 512, waves_per_block=1,
                                  persistent=True)
+        elif path == "binned":
+            pxl.train_rays_sched(grid, optim, c2["rays"], losses, si, ws, 512, waves_per_block=1,
+                                 binned=True, grad_waves_per_block=4)
         elif path == "chunked":
             pxl.train_rays_sched(grid, optim, c2["rays"], losses, si, ws, 512, chunked=True)
+        elif path == "sliced":
+            pxl.train_rays_sched(grid, optim, c2["rays"], losses, si, ws, 512, waves_per_block=1,
+                                 grad_waves_per_block=4, slices=4)
         else:
             pxl.train_rays_ws(grid, optim, c2["rays"], losses, si, ws, 512)
     torch.cuda.synchronize()
@@ -185,7 +192,7 @@ def test_c2_sampled_rays(c2, c2_oracle, path):
     assert worst <= 1.0
 
 
-@pytest.mark.parametrize("path", ["fused", "logged"])
+@pytest.mark.parametrize("path", ["fused", "logged", "binned"])
 def test_c2_sampled_rays_adaptive(c2, c2_oracle, path):
     """128 rays of the batch under the reference's adaptive marching rule
     (t += |step - w|) against the oracle on the full 256^3 grid."""
@@ -199,6 +206,9 @@ def test_c2_sampled_rays_adaptive(c2, c2_oracle, path):
     si = torch.from_numpy(sub).to(c2["dev"])
     if path == "fused":
         pxl.train_rays(grid, optim, c2["rays"], losses, si)
+    elif path == "binned":
+        ws = pxl.train_workspace(128, 512, c2["dev"], grid=grid)
+        pxl.train_rays_sched(grid, optim, c2["rays"], losses, si, ws, 512, binned=True)
     else:
         ws = pxl.train_workspace(128, 512, c2["dev"])
         pxl.train_rays_ws(grid, optim, c2["rays"], losses, si, ws, 512)
@@ -241,7 +251,8 @@ def c2_full_ref(c2, c2_oracle):
     return cols, l_ref, g_ref, g_abs
 
 
-@pytest.mark.parametrize("path", ["fused", "logged", "persistent", "chunked"])
+@pytest.mark.parametrize("path", ["fused", "logged", "persistent", "chunked", "binned",
+                                  "sliced"])
 def test_c2_full_batch_vs_oracle(c2, c2_full_ref, path):
     import torch
 

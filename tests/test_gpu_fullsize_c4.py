@@ -7,7 +7,8 @@ of the central half-cube, random 8-bit targets.
   is reported;
 - the whole batch against the FP64 oracle on the widened bfloat16 grid,
   under every launch schedule the autotuner chooses from (1 or 4 waves per
-  block, persistent or not, the chunk-parallel gradient; given, Morton and
+  block, persistent or not, the chunk-parallel and the brick-binned
+  gradient; given, Morton and
   length + Morton order): colours 1e-5, losses 255 * 1e-5 (identical across
   schedules), gradients within helpers.grad_close_cond: 1e-4 relative with
   a floor of 1e-3 of the plane maximum, or, where the entry's terms cancel,
@@ -108,16 +109,18 @@ def _grid(c4):
     return pxl.Grid(N, N, N, S, c4["rec16"], storage=pxl.STORE_BF16, **c4["grid_kw"])
 
 
-def _train(c4, si, wpb=4, persistent=False, chunked=False, gwpb=0):
+def _train(c4, si, wpb=4, persistent=False, gmode=0, gwpb=0):
     import torch
 
     grid = _grid(c4)
     optim = pxl.Optim(grid, master_from=c4["rec32"])
     n = si.numel()
     losses = torch.zeros(n, dtype=torch.float64, device=c4["dev"])
-    ws = pxl.train_workspace(n, 512, c4["dev"])
+    binned = gmode == pxl.GRAD_BINNED
+    ws = pxl.train_workspace(n, 512, c4["dev"], grid=grid if binned else None)
     pxl.train_rays_sched(grid, optim, c4["rays"], losses, si, ws, 512, waves_per_block=wpb,
-                         persistent=persistent, chunked=chunked, grad_waves_per_block=gwpb)
+                         persistent=persistent, chunked=gmode == pxl.GRAD_CHUNKED,
+                         binned=binned, grad_waves_per_block=gwpb)
     torch.cuda.synchronize()
     r = c4["rays"]
     idx = si.cpu().numpy()
@@ -181,10 +184,10 @@ def test_c4_schedules_vs_oracle(c4, c4_oracle):
             order = si
         perm = order.cpu().numpy()
         assert np.array_equal(np.sort(perm), np.arange(BATCH))
-        for wpb, persist, chunked, gwpb in ((4, False, False, 0), (1, False, False, 4),
-                                            (4, True, False, 0), (1, True, False, 0),
-                                            (4, False, True, 0)):
-            o, l, c, idx = _train(c4, order, wpb, persist, chunked, gwpb)
+        for wpb, persist, gmode, gwpb in ((4, False, 0, 0), (1, False, 0, 4), (4, True, 0, 0),
+                                          (1, True, 0, 0), (4, False, 1, 0), (1, False, 2, 4),
+                                          (1, True, 2, 1)):
+            o, l, c, idx = _train(c4, order, wpb, persist, gmode, gwpb)
             # losses are per batch position: map them back to ray ids
             l_by_ray = np.empty(BATCH)
             l_by_ray[idx] = l
@@ -195,7 +198,8 @@ def test_c4_schedules_vs_oracle(c4, c4_oracle):
             worst, n_cond = grad_close_cond(from_records(o.grad, P), ref["grad"], ref["gabs"],
                                             rel0=1e-4)
             say(f"    {name}, {wpb} waves/block (gradient {gwpb or wpb}), persistent {persist}, "
-                f"chunked {chunked}: colour error {dc:.3g}, loss error {dl:.3g}, gradient / "
+                f"gradient {pxl.GRAD_NAMES[gmode]}: colour error {dc:.3g}, loss error {dl:.3g}, "
+                f"gradient / "
                 f"tolerance {worst:.3g} ({n_cond} of {int((ref['grad'] != 0).sum())} nonzero "
                 f"entries held by the magnitude term)")
             del o

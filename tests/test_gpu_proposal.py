@@ -185,10 +185,12 @@ def test_masked_render_matches_oracle(scene):
 
 
 @pytest.mark.parametrize("cap", [512, 64])
-def test_masked_train_and_targets(scene, cap):
+@pytest.mark.parametrize("binned", [False, True])
+def test_masked_train_and_targets(scene, cap, binned):
     import torch
 
-    say(f"[prop] masked train step + targets vs oracle, 1024 rays, log {cap}")
+    say(f"[prop] masked train step + targets vs oracle, 1024 rays, log {cap}, "
+        f"{'brick-binned' if binned else 'per-ray'} gradient")
     dev = scene["dev"]
     n = 1024
     grid = pxl.Grid(N1, N1, N1, S1, scene["rec"], step=0.5)
@@ -196,13 +198,13 @@ def test_masked_train_and_targets(scene, cap):
     optim = pxl.Optim(grid)
     masks = _random_masks(np.random.default_rng(5), n)
     losses = torch.zeros(n, dtype=torch.float64, device=dev)
-    ws = pxl.train_workspace(n, cap, dev)
+    ws = pxl.train_workspace(n, cap, dev, grid=grid if binned else None)
     tg = torch.zeros(n, dtype=torch.int64, device=dev)
     tv = torch.zeros(n, dtype=torch.uint8, device=dev)
     wmin = 1e-4
     pxl.train_rays_prop(grid, optim, rays, losses, scene["si"], ws, cap,
                         bin_mask=torch.from_numpy(masks.view(np.int64)).to(dev), bin_targets=tg,
-                        target_valid=tv, target_wmin=wmin, length=n)
+                        target_valid=tv, target_wmin=wmin, length=n, binned=binned)
     torch.cuda.synchronize()
     grid_o = oracle.Grid(N1, N1, N1, S1, scene["planes"], 0.5)
     rays_o = oracle.Rays(scene["rays_np"])
