@@ -869,11 +869,11 @@ def main():
     out["config"]["storage"] = args.storage
     if args.storage == "bf16" or args.workload == "uneven512":
         out["config"]["fp32_master"] = bool(args.bf16_master)
-    order, wpb, pers, gmode, gwpb, nsl = w.trainer.schedule()
+    order, wpb, pers, gmode, gwpb = w.trainer.schedule()
     out["config"]["schedule"] = {"batch_order": ["given", "morton", "length+morton"][order],
                                  "waves_per_block": wpb, "grad_waves_per_block": gwpb,
                                  "persistent": pers, "gradient": pxl.GRAD_NAMES[gmode],
-                                 "slices": nsl, "autotuned": bool(args.autotune)}
+                                 "autotuned": bool(args.autotune)}
     if uneven:
         out["metric"] = "train rays/sec (fwd+bwd+TV+Adam) 512^3 uneven SH2 BF16"
         out["config"]["workload"] = (

@@ -193,7 +193,6 @@ extern "C" pxl_trainer *pxl_trainer_create(const pxl_train_config *cfg, const px
     tr->sched.persistent = cfg->persistent;
     tr->sched.gradient = cfg->gradient;
     tr->sched.grad_waves_per_block = 0;
-    tr->sched.slices = cfg->slices;
     tr->tuned = !cfg->autotune;
     tr->prop_on = false;
     tr->prop_ws = nullptr;
@@ -211,11 +210,6 @@ extern "C" pxl_trainer *pxl_trainer_create(const pxl_train_config *cfg, const px
     }
     if (cfg->gradient == PXL_GRAD_CHUNKED && cfg->persistent) {
         set_error("pxl_trainer_create: the chunk-parallel gradient has no persistent form");
-        pxl_trainer_destroy(tr);
-        return nullptr;
-    }
-    if (cfg->slices < 0 || cfg->slices > 8 || (!logged && cfg->slices > 1)) {
-        set_error("pxl_trainer_create: slices %d (0 .. 8, the logged step only)", cfg->slices);
         pxl_trainer_destroy(tr);
         return nullptr;
     }
@@ -368,19 +362,16 @@ static int32_t autotune(pxl_trainer *tr, int64_t start, int64_t len) {
     // kernel; persistent launches share one shape; the chunk-parallel
     // gradient with 4 waves per workgroup; the brick-binned gradient with 1
     // or 4 waves per workgroup of its per-ray passes.
-    // Each with the batch in 1 slice or (overlapping one slice's gradient
-    // with the next slice's forward) in 2 or 4.
     for (int32_t order : {0, (int32_t)PXL_ORDER_MORTON, (int32_t)PXL_ORDER_LENGTH_MORTON})
-        for (int32_t wpb : {1, 2, 4})
-            for (int32_t sl : {1, 2, 4}) {
-                for (int32_t gwpb : {1, 2, 4})
-                    cands.push_back({order, {wpb, 0, PXL_GRAD_PER_RAY, gwpb, sl}});
-                cands.push_back({order, {wpb, 1, PXL_GRAD_PER_RAY, 0, sl}});
-                cands.push_back({order, {wpb, 0, PXL_GRAD_CHUNKED, 4, sl}});
-                if (tr->binned_ok)
-                    for (int32_t gwpb : {1, 4})
-                        cands.push_back({order, {wpb, 0, PXL_GRAD_BINNED, gwpb, sl}});
-            }
+        for (int32_t wpb : {1, 2, 4}) {
+            for (int32_t gwpb : {1, 2, 4})
+                cands.push_back({order, {wpb, 0, PXL_GRAD_PER_RAY, gwpb}});
+            cands.push_back({order, {wpb, 1, PXL_GRAD_PER_RAY, 0}});
+            cands.push_back({order, {wpb, 0, PXL_GRAD_CHUNKED, 4}});
+            if (tr->binned_ok)
+                for (int32_t gwpb : {1, 4})
+                    cands.push_back({order, {wpb, 0, PXL_GRAD_BINNED, gwpb}});
+        }
     constexpr int kReps = 2;
     std::vector<hipEvent_t> ev(cands.size() * kReps * 2, nullptr);
     int32_t st = PXL_OK;

@@ -1125,8 +1125,7 @@ void launch_binned(const RenderArgs &a, int gwpb, hipStream_t st) {
     hipLaunchKernelGGL((k_bin_accum<S, SPARSE>), dim3(ga), dim3(256), lds, st, a);
 }
 
-// The logged step's kernels in two halves, so that a batch cut into slices
-// can run slice i's gradient beside slice i+1's forward.
+// The logged step's kernels in two halves.
 //
 // Forward half: the claim / redo / target counters cleared, then the
 // forward + log kernel (wpb waves per block; persistent launches size the
@@ -1235,33 +1234,8 @@ void grad_logged_adapt(const RenderArgs &a, int gwpb, bool persist, int gmode, h
     launch_rec0_fold(a, 1 + 3 * S, st);
 }
 
-// A batch of the logged step as `slices` consecutive slices, each with its
-// own part of the workspace.  One slice: forward, then (after grad_wait)
-// the gradient half, all on the caller's stream.  Several: the forwards run
-// in order on the caller's stream; the gradient halves run in order on the
-// library's second stream of the device, slice i's after slice i's forward
-// (and the first after grad_wait and everything the caller's stream held
-// at the call), so that slice i's gradient kernels overlap slice i+1's
-// forward; the caller's stream then waits for the last gradient half.
-struct LogSlices {
-    char *base;          // the workspace
-    size_t bytes;        // bytes per slice (LogLayout::total of one slice)
-    LogLayout L;         // layout of one slice
-    int64_t per;         // batch entries per slice (the last may hold fewer)
-    int count;           // slices
-    int64_t nkeys;       // bin keys (brick-binned gradient) or 0
-};
-RenderArgs slice_args(const RenderArgs &a, const LogSlices &ls, int i) {
-    RenderArgs r = a;
-    const int64_t off = (int64_t)i * ls.per;
-    r.start = a.start + off;
-    r.len = std::min(ls.per, a.len - off);
-    r.losses = a.losses + off;
-    if (a.bin_mask) r.bin_mask = a.bin_mask + off;
-    if (a.bin_targets) r.bin_targets = a.bin_targets + off;
-    if (a.target_valid) r.target_valid = a.target_valid + off;
-    char *w = ls.base + (size_t)i * ls.bytes;
-    const LogLayout &L = ls.L;
+// The workspace pointers of a batch's sample log (layout L at w).
+void set_log_args(RenderArgs &r, char *w, const LogLayout &L, int64_t nkeys) {
     r.work = reinterpret_cast<unsigned long long *>(w + L.work);
     r.log_tot = reinterpret_cast<double *>(w + L.tot);
     r.log_count = reinterpret_cast<int32_t *>(w + L.count);
@@ -1271,7 +1245,7 @@ RenderArgs slice_args(const RenderArgs &a, const LogSlices &ls, int i) {
     r.redo_list = reinterpret_cast<int32_t *>(w + L.redo);
     r.rec0 = reinterpret_cast<double *>(w + L.rec0);
     r.rec0_part = reinterpret_cast<double *>(w + L.part);
-    if (ls.nkeys > 0) {
+    if (nkeys > 0) {
         r.gb_count = reinterpret_cast<uint32_t *>(w + L.gb_count);
         r.gb_fill = reinterpret_cast<uint32_t *>(w + L.gb_fill);
         r.gb_blk = reinterpret_cast<uint32_t *>(w + L.gb_blk);
@@ -1281,97 +1255,36 @@ RenderArgs slice_args(const RenderArgs &a, const LogSlices &ls, int i) {
         // pass has read (one u32 per logged sample).
         r.gb_list = reinterpret_cast<uint32_t *>(w + L.val);
     }
-    return r;
 }
 
-// The library's second stream and events of a device for the sliced step,
-// created on first use and kept until the process ends; `mu` is held while
-// one call enqueues its work, so that calls from several host threads do
-// not interleave their event records.
-constexpr int kMaxSlices = 8;
-struct SliceStream {
-    std::mutex mu;
-    hipStream_t side = nullptr;
-    hipEvent_t ev[kMaxSlices + 1] = {};
-};
-SliceStream g_slice[64];
-std::mutex g_slice_init;
-int32_t slice_stream(SliceStream **out) {
-    int dev = -1;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) {
-        set_error("render: no current device for the sliced step");
-        return PXL_EHIP;
-    }
-    std::lock_guard<std::mutex> lock(g_slice_init);
-    SliceStream &s = g_slice[dev];
-    if (!s.side) {
-        if (hipStreamCreateWithFlags(&s.side, hipStreamNonBlocking) != hipSuccess) {
-            s.side = nullptr;
-            set_error("render: cannot create the sliced step's stream");
-            return PXL_EHIP;
-        }
-        for (hipEvent_t &e : s.ev)
-            if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
-                set_error("render: cannot create the sliced step's events");
-                return PXL_EHIP;
-            }
-    }
-    *out = &s;
-    return PXL_OK;
-}
-
+// Forward, then (after grad_wait) the gradient half, on one stream.
 template <typename FwdF, typename GradF>
-int32_t run_slices(const RenderArgs &a, const LogSlices &ls, hipEvent_t grad_wait,
-                   hipStream_t st, FwdF fwd, GradF grad) {
-    if (ls.count <= 1) {
-        const RenderArgs a0 = slice_args(a, ls, 0);
-        fwd(a0, st);
-        if (grad_wait && hipStreamWaitEvent(st, grad_wait, 0) != hipSuccess) {
-            set_error("render: hipStreamWaitEvent before the gradient kernels failed");
-            return PXL_EHIP;
-        }
-        grad(a0, st);
-        return PXL_OK;
-    }
-    SliceStream *ss = nullptr;
-    int32_t rc = slice_stream(&ss);
-    if (rc != PXL_OK) return rc;
-    std::lock_guard<std::mutex> lock(ss->mu);
-    bool ok = hipEventRecord(ss->ev[kMaxSlices], st) == hipSuccess &&
-              hipStreamWaitEvent(ss->side, ss->ev[kMaxSlices], 0) == hipSuccess;
-    if (ok && grad_wait) ok = hipStreamWaitEvent(ss->side, grad_wait, 0) == hipSuccess;
-    for (int i = 0; ok && i < ls.count; i++) {
-        const RenderArgs ai = slice_args(a, ls, i);
-        if (ai.len <= 0) break;
-        fwd(ai, st);
-        ok = hipEventRecord(ss->ev[i], st) == hipSuccess &&
-             hipStreamWaitEvent(ss->side, ss->ev[i], 0) == hipSuccess;
-        if (ok) grad(ai, ss->side);
-    }
-    ok = ok && hipEventRecord(ss->ev[kMaxSlices], ss->side) == hipSuccess &&
-         hipStreamWaitEvent(st, ss->ev[kMaxSlices], 0) == hipSuccess;
-    if (!ok) {
-        set_error("render: event ordering of the sliced step failed");
+int32_t run_logged(const RenderArgs &a, hipEvent_t grad_wait, hipStream_t st, FwdF fwd,
+                   GradF grad) {
+    fwd(a, st);
+    if (grad_wait && hipStreamWaitEvent(st, grad_wait, 0) != hipSuccess) {
+        set_error("render: hipStreamWaitEvent before the gradient kernels failed");
         return PXL_EHIP;
     }
+    grad(a, st);
     return PXL_OK;
 }
 
 template <int S, bool SPARSE, int STORE>
-int32_t launch_logged(const RenderArgs &a, const LogSlices &ls, int wpb, int gwpb, bool persist,
-                      int gmode, hipEvent_t grad_wait, hipStream_t st) {
-    return run_slices(
-        a, ls, grad_wait, st,
+int32_t launch_logged(const RenderArgs &a, int wpb, int gwpb, bool persist, int gmode,
+                      hipEvent_t grad_wait, hipStream_t st) {
+    return run_logged(
+        a, grad_wait, st,
         [&](const RenderArgs &ai, hipStream_t s) { fwd_logged<S, SPARSE, STORE>(ai, wpb, persist, s); },
         [&](const RenderArgs &ai, hipStream_t s) {
             grad_logged<S, SPARSE, STORE>(ai, wpb, gwpb, persist, gmode, s);
         });
 }
 template <int S, bool SPARSE>
-int32_t launch_logged_adapt(const RenderArgs &a, const LogSlices &ls, int wpb, int gwpb,
-                            bool persist, int gmode, hipEvent_t grad_wait, hipStream_t st) {
-    return run_slices(
-        a, ls, grad_wait, st,
+int32_t launch_logged_adapt(const RenderArgs &a, int wpb, int gwpb, bool persist, int gmode,
+                            hipEvent_t grad_wait, hipStream_t st) {
+    return run_logged(
+        a, grad_wait, st,
         [&](const RenderArgs &ai, hipStream_t s) { fwd_logged_adapt<S, SPARSE>(ai, wpb, s); },
         [&](const RenderArgs &ai, hipStream_t s) {
             grad_logged_adapt<S, SPARSE>(ai, gwpb, persist, gmode, s);
@@ -1379,12 +1292,11 @@ int32_t launch_logged_adapt(const RenderArgs &a, const LogSlices &ls, int wpb, i
 }
 
 template <int S>
-int32_t dispatch_logged_s(bool sparse, int store, const RenderArgs &a, const LogSlices &ls,
-                          int wpb, int gwpb, bool persist, int gmode, hipEvent_t grad_wait,
-                          hipStream_t st) {
+int32_t dispatch_logged_s(bool sparse, int store, const RenderArgs &a, int wpb, int gwpb,
+                          bool persist, int gmode, hipEvent_t grad_wait, hipStream_t st) {
 #define PXL_LOGGED(SP, ST)                                                        \
     if (sparse == SP && store == ST)                                              \
-        return launch_logged<S, SP, ST>(a, ls, wpb, gwpb, persist, gmode, grad_wait, st);
+        return launch_logged<S, SP, ST>(a, wpb, gwpb, persist, gmode, grad_wait, st);
     PXL_LOGGED(true, PXL_STORE_F32)
     PXL_LOGGED(true, PXL_STORE_BF16)
     PXL_LOGGED(true, PXL_STORE_F16)
@@ -1395,9 +1307,9 @@ int32_t dispatch_logged_s(bool sparse, int store, const RenderArgs &a, const Log
     set_error("render: unknown storage %d", store);
     return PXL_EINVAL;
 }
-int32_t dispatch_logged(int S, bool sparse, int store, const RenderArgs &a, const LogSlices &ls,
-                        int wpb, int gwpb, bool persist, int gmode, hipEvent_t gw,
-                        hipStream_t st, bool adapt = false) {
+int32_t dispatch_logged(int S, bool sparse, int store, const RenderArgs &a, int wpb, int gwpb,
+                        bool persist, int gmode, hipEvent_t gw, hipStream_t st,
+                        bool adapt = false) {
     if (adapt) {
         if (store != PXL_STORE_F32) {
             set_error("render: the adaptive marching rule needs FP32 storage");
@@ -1405,9 +1317,8 @@ int32_t dispatch_logged(int S, bool sparse, int store, const RenderArgs &a, cons
         }
 #define PXL_ADAPT(SV)                                                                         \
     case SV:                                                                                  \
-        return sparse                                                                          \
-                   ? launch_logged_adapt<SV, true>(a, ls, wpb, gwpb, persist, gmode, gw, st)  \
-                   : launch_logged_adapt<SV, false>(a, ls, wpb, gwpb, persist, gmode, gw, st);
+        return sparse ? launch_logged_adapt<SV, true>(a, wpb, gwpb, persist, gmode, gw, st) \
+                      : launch_logged_adapt<SV, false>(a, wpb, gwpb, persist, gmode, gw, st);
         switch (S) {
             PXL_ADAPT(1)
             PXL_ADAPT(4)
@@ -1419,9 +1330,9 @@ int32_t dispatch_logged(int S, bool sparse, int store, const RenderArgs &a, cons
 #undef PXL_ADAPT
     }
     switch (S) {
-        case 1: return dispatch_logged_s<1>(sparse, store, a, ls, wpb, gwpb, persist, gmode, gw, st);
-        case 4: return dispatch_logged_s<4>(sparse, store, a, ls, wpb, gwpb, persist, gmode, gw, st);
-        case 9: return dispatch_logged_s<9>(sparse, store, a, ls, wpb, gwpb, persist, gmode, gw, st);
+        case 1: return dispatch_logged_s<1>(sparse, store, a, wpb, gwpb, persist, gmode, gw, st);
+        case 4: return dispatch_logged_s<4>(sparse, store, a, wpb, gwpb, persist, gmode, gw, st);
+        case 9: return dispatch_logged_s<9>(sparse, store, a, wpb, gwpb, persist, gmode, gw, st);
         default:
             set_error("render: the logged fused step supports sh_dim 1, 4 and 9");
             return PXL_EINVAL;
@@ -1494,27 +1405,16 @@ static int32_t bwd_common(const pxl_grid *grid, const pxl_optim *optim, const px
                     "%s: the brick-binned gradient needs 2 to 1025 vertices per side (grid "
                     "%d x %d x %d)",
                     who, grid->W, grid->H, grid->D);
-        const int slices = sched && sched->slices > 1 ? sched->slices : 1;
-        PXL_REQUIRE(slices <= kMaxSlices, "%s: %d slices (at most %d)", who, slices, kMaxSlices);
-        const int64_t per = (len + slices - 1) / slices;
-        const LogLayout L = log_layout(per, max_samples, nkeys);
-        const size_t need = (size_t)slices * L.total;
-        PXL_REQUIRE(x.workspace_bytes >= need,
-                    "%s: workspace of %zu bytes, %zu needed for %lld rays x %d samples in %d "
-                    "slice(s)%s",
-                    who, x.workspace_bytes, need, (long long)len, max_samples, slices,
+        const LogLayout L = log_layout(len, max_samples, nkeys);
+        PXL_REQUIRE(x.workspace_bytes >= L.total,
+                    "%s: workspace of %zu bytes, %zu needed for %lld rays x %d samples%s", who,
+                    x.workspace_bytes, L.total, (long long)len, max_samples,
                     nkeys > 0 ? " with the brick-binned gradient "
                                 "(pxl_train_rays_workspace_bytes_grid)"
                               : "");
         PXL_REQUIRE(((uintptr_t)x.workspace & 255) == 0,
                     "%s: workspace must be 256-byte aligned", who);
-        LogSlices ls;
-        ls.base = static_cast<char *>(x.workspace);
-        ls.bytes = L.total;
-        ls.L = L;
-        ls.per = per;
-        ls.count = (int)((len + per - 1) / per);
-        ls.nkeys = nkeys;
+        set_log_args(a, static_cast<char *>(x.workspace), L, nkeys);
         a.log_cap = max_samples;
         a.log_chunks = (max_samples + 63) / 64;
         if (nkeys > 0) {
@@ -1530,7 +1430,7 @@ static int32_t bwd_common(const pxl_grid *grid, const pxl_optim *optim, const px
                     "%s: grad_waves_per_block %d (0, 1, 2 or 4)", who, gwpb);
         PXL_REQUIRE(!sched || !(sched->persistent && gmode == PXL_GRAD_CHUNKED),
                     "%s: the chunk-parallel gradient has no persistent form", who);
-        st = dispatch_logged(grid->sh_dim, grid->sparse, grid->storage, a, ls, wpb, gwpb,
+        st = dispatch_logged(grid->sh_dim, grid->sparse, grid->storage, a, wpb, gwpb,
                              sched && sched->persistent, gmode, x.grad_wait,
                              as_stream(stream), adapt);
         return st != PXL_OK ? st : check_launch(who);
@@ -1572,27 +1472,15 @@ extern "C" int32_t pxl_train_rays(const pxl_grid *grid, const pxl_optim *optim,
                       huber_delta, stream, true);
 }
 
-namespace pxl {
-namespace {
-// Workspace of a batch under every slice count 1 .. kMaxSlices.
-size_t log_bytes_any(int64_t len, int32_t cap, int64_t nkeys) {
-    size_t most = 0;
-    for (int k = 1; k <= kMaxSlices; k++)
-        most = std::max(most, (size_t)k * log_layout((len + k - 1) / k, cap, nkeys).total);
-    return most;
-}
-}  // namespace
-}  // namespace pxl
-
 extern "C" size_t pxl_train_rays_workspace_bytes(int64_t len, int32_t max_samples) {
     if (len <= 0 || max_samples <= 0 || max_samples > kMaxLogSamples) return 0;
-    return log_bytes_any(len, max_samples, 0);
+    return log_layout(len, max_samples).total;
 }
 
 extern "C" size_t pxl_train_rays_workspace_bytes_grid(const pxl_grid *grid, int64_t len,
                                                       int32_t max_samples) {
     if (len <= 0 || max_samples <= 0 || max_samples > kMaxLogSamples) return 0;
-    return log_bytes_any(len, max_samples, bin_keys(grid));
+    return log_layout(len, max_samples, bin_keys(grid)).total;
 }
 
 extern "C" int32_t pxl_train_rays_ws(const pxl_grid *grid, const pxl_optim *optim,
@@ -1626,7 +1514,7 @@ extern "C" int32_t pxl_train_rays_prop(const pxl_grid *grid, const pxl_optim *op
         set_error("pxl_train_rays_prop: null workspace");
         return PXL_EINVAL;
     }
-    const pxl_schedule dflt = {4, 0, 0, 0, 0};
+    const pxl_schedule dflt = {4, 0, 0, 0};
     TrainExtra x;
     x.workspace = workspace;
     x.workspace_bytes = workspace_bytes;
@@ -1708,7 +1596,7 @@ extern "C" int32_t pxl_train_rays_noalloc(const pxl_grid *grid, const pxl_optim 
     PXL_REQUIRE(len >= 0 && max_samples > 0 && max_samples <= kMaxLogSamples,
                 "%s: len %lld / max_samples %d outside [1, %d]", who, (long long)len, max_samples,
                 kMaxLogSamples);
-    const size_t need = log_bytes_any(len, max_samples, 0);
+    const size_t need = log_layout(len, max_samples).total;
     PXL_REQUIRE(need <= (size_t)PXL_NOALLOC_SCRATCH_BYTES,
                 "%s: %lld rays x %d samples need a %zu-byte sample log, beyond the library's "
                 "%zu bytes (PXL_NOALLOC_SCRATCH_BYTES); use pxl_train_rays_sched with a workspace",
@@ -1716,7 +1604,7 @@ extern "C" int32_t pxl_train_rays_noalloc(const pxl_grid *grid, const pxl_optim 
     void *ws = nullptr;
     const int32_t st = scratch_of_device(&ws, who);
     if (st != PXL_OK) return st;
-    const pxl_schedule dflt = {4, 0, 0, 0, 0};
+    const pxl_schedule dflt = {4, 0, 0, 0};
     TrainExtra x;
     x.workspace = ws;
     x.workspace_bytes = (size_t)PXL_NOALLOC_SCRATCH_BYTES;

@@ -136,7 +136,7 @@ class TrainConfigC(ctypes.Structure):
                 ("mask_coefficient", _F64), ("huber_delta", _F32), ("fused", _I32),
                 ("log_samples", _I32), ("autotune", _I32), ("batch_order", _I32),
                 ("waves_per_block", _I32), ("persistent", _I32), ("gradient", _I32),
-                ("tv_inline", _I32), ("tv_separate", _I32), ("slices", _I32)]
+                ("tv_inline", _I32), ("tv_separate", _I32)]
 
 
 class LevelConfigC(ctypes.Structure):
@@ -157,7 +157,7 @@ class ProposalC(ctypes.Structure):
 
 class ScheduleC(ctypes.Structure):
     _fields_ = [("waves_per_block", _I32), ("persistent", _I32), ("gradient", _I32),
-                ("grad_waves_per_block", _I32), ("slices", _I32)]
+                ("grad_waves_per_block", _I32)]
 
 
 # Gradient schedules of the logged step (pxl_schedule.gradient).
@@ -562,17 +562,16 @@ ORDER_NONE, ORDER_MORTON, ORDER_LENGTH_MORTON = 0, 1, 2
 def train_rays_sched(grid, optim, rays, losses, sample_index, workspace, max_samples,
                      waves_per_block=4, persistent=False, chunked=False, grad_waves_per_block=0,
                      start=0, length=None, mask_coef=1.0, huber_delta=0.1, stream=None,
-                     binned=False, slices=1):
+                     binned=False):
     """train_rays_ws with an explicit launch schedule (chunked: one gradient
     wave per 64-sample chunk of each ray's log; binned: the brick-binned
     gradient, workspace from train_workspace(..., grid=grid);
     grad_waves_per_block: the per-ray gradient kernels' waves per workgroup,
-    0 = waves_per_block; slices: the batch in that many slices, one slice's
-    gradient beside the next slice's forward)."""
+    0 = waves_per_block)."""
     length = sample_index.numel() - start if length is None else length
     g, o, r = grid.c(), optim.c(), rays.c()
     sc = ScheduleC(waves_per_block, int(persistent), _grad_mode(chunked, binned),
-                   int(grad_waves_per_block), int(slices))
+                   int(grad_waves_per_block))
     _check("pxl_train_rays_sched",
            lib().pxl_train_rays_sched(ctypes.byref(g), ctypes.byref(o), ctypes.byref(r),
                                       _ptr(losses), _ptr(sample_index), start, length, mask_coef,
@@ -688,14 +687,13 @@ def render_fwd_masked(grid, rays, sample_index, bin_mask=None, start=0, length=N
 def train_rays_prop(grid, optim, rays, losses, sample_index, workspace, max_samples,
                     bin_mask=None, bin_targets=None, target_valid=None, target_wmin=1e-4,
                     waves_per_block=4, persistent=False, chunked=False, start=0, length=None,
-                    mask_coef=1.0, huber_delta=0.1, stream=None, binned=False, slices=1):
+                    mask_coef=1.0, huber_delta=0.1, stream=None, binned=False):
     """The two-kernel step with an optional proposal mask and target output
     (batches with targets run the chunked schedule's gradient one wave per
     ray)."""
     length = sample_index.numel() - start if length is None else length
     g, o, r = grid.c(), optim.c(), rays.c()
-    sc = ScheduleC(waves_per_block, int(persistent), _grad_mode(chunked, binned), 0,
-                   int(slices))
+    sc = ScheduleC(waves_per_block, int(persistent), _grad_mode(chunked, binned), 0)
     _check("pxl_train_rays_prop",
            lib().pxl_train_rays_prop(ctypes.byref(g), ctypes.byref(o), ctypes.byref(r),
                                      _ptr(losses), _ptr(sample_index), start, length, mask_coef,
@@ -899,7 +897,7 @@ class Trainer:
                  variance_interval=3, optimize_step=0, mask_coefficient=1.0, huber_delta=0.1,
                  fused=True, log_samples=0, autotune=False, batch_order=0,
                  waves_per_block=0, persistent=False, chunked=False, tv_inline=False,
-                 tv_separate=False, stream=None, binned=False, slices=0):
+                 tv_separate=False, stream=None, binned=False):
         import torch
 
         self._keep = (grid, optim, rays, order)
@@ -911,7 +909,7 @@ class Trainer:
                            optimize_step, mask_coefficient, huber_delta, int(fused),
                            int(log_samples), int(autotune), int(batch_order),
                            int(waves_per_block), int(persistent), _grad_mode(chunked, binned),
-                           int(tv_inline), int(tv_separate), int(slices))
+                           int(tv_inline), int(tv_separate))
         self._g, self._o, self._r = grid.c(), optim.c(), rays.c()
         n_rays = order.numel() if n_rays is None else n_rays
         sc = tv_scratch(grid, stream) if variance_type in (TV_DISPARITY, TV_BOTH) else None
@@ -1004,12 +1002,12 @@ class Trainer:
 
     def schedule(self):
         """(batch order, waves per block, persistent, gradient schedule
-        GRAD_*, gradient waves per block, slices) the trainer runs."""
+        GRAD_*, gradient waves per block) the trainer runs."""
         order, sc = _I32(), ScheduleC()
         _check("pxl_trainer_schedule",
                lib().pxl_trainer_schedule(self._h, ctypes.byref(order), ctypes.byref(sc)))
         return (order.value, sc.waves_per_block, bool(sc.persistent), sc.gradient,
-                sc.grad_waves_per_block or sc.waves_per_block, max(1, sc.slices))
+                sc.grad_waves_per_block or sc.waves_per_block)
 
     def close(self):
         if getattr(self, "_h", None):

@@ -488,9 +488,9 @@ int32_t pxl_count_touched(const pxl_grid *grid, const pxl_rays *rays,
  * max_samples evaluated samples is differentiated by the marcher itself.
  * Grids the eight-sample marcher does not serve (adaptive rule, S = 16) run
  * as pxl_train_rays.  workspace: 256-byte aligned device memory of at least
- * pxl_train_rays_workspace_bytes(len, max_samples) bytes (enough for every
- * schedule below with up to 8 slices, except the brick-binned gradient);
- * calls sharing a workspace must not overlap.  No counterpart in the reference (an
+ * pxl_train_rays_workspace_bytes(len, max_samples) bytes (every schedule
+ * below except the brick-binned gradient); calls sharing a workspace must
+ * not overlap.  No counterpart in the reference (an
  * implementation of renderRays_withGradients_RGB, RenderUtils.cu:2257-2278,
  * fused with renderRays). */
 size_t pxl_train_rays_workspace_bytes(int64_t len, int32_t max_samples);
@@ -534,13 +534,6 @@ typedef struct pxl_schedule {
     int32_t persistent;
     int32_t gradient;            /* PXL_GRAD_* */
     int32_t grad_waves_per_block;
-    /* 0 or 1: each kernel once over the batch; 2 .. 8: the batch in that
-     * many consecutive slices, each with its part of the workspace; the
-     * forwards run in order on `stream`, the gradient kernels of slice i on
-     * the library's second stream of the device once slice i's forward is
-     * done, beside slice i + 1's forward; `stream` then waits for the last
-     * of them. */
-    int32_t slices;
 } pxl_schedule;
 int32_t pxl_train_rays_ws(const pxl_grid *grid, const pxl_optim *optim,
                           const pxl_rays *rays, double *losses,
@@ -727,7 +720,6 @@ typedef struct pxl_train_config {
                                 (pxl_tv_adam_step) on the batches that take
                                 the variance loss, the trainer keeping a
                                 second record buffer (pxl_trainer_sync) */
-    int32_t slices;          /* when not autotuned: pxl_schedule slices */
 } pxl_train_config;
 
 typedef struct pxl_trainer pxl_trainer;

@@ -12,7 +12,7 @@ whose corners exceed 12), 8 views of 64 x 64 on a radius-4 sphere, random
 - backward on every path: the one-sample fused kernel, the two-kernel
   logged step (log holding every ray, or 24 samples so that long rays take
   the redo launch), its chunk-parallel, persistent and brick-binned
-  gradient schedules, the batch in 3 slices;
+  gradient schedules;
   clipped (Huber delta 0.1) and unclipped (10) residuals; losses within
   255 * 1e-5, gradients helpers.grad_close (record 0 of the caged grid 1e-4
   relative on the logged paths, 1e-3 on the single fused kernel);
@@ -93,7 +93,7 @@ def test_adaptive_forward(sc, kind):
 
 @pytest.mark.parametrize("kind", ["dense", "caged"])
 @pytest.mark.parametrize("path", ["fused", "logged", "logged_24", "chunked", "persistent",
-                                  "binned", "binned_24", "sliced"])
+                                  "binned", "binned_24"])
 @pytest.mark.parametrize("huber", [0.1, 10.0])
 def test_adaptive_backward(sc, kind, path, huber):
     import torch
@@ -113,14 +113,12 @@ def test_adaptive_backward(sc, kind, path, huber):
         pxl.train_rays_sched(grid, optim, rays, losses, sc["si"], ws, cap,
                              waves_per_block=1 if path == "persistent" else 4,
                              persistent=path == "persistent", chunked=path == "chunked",
-                             binned=binned, huber_delta=huber,
-                             slices=3 if path == "sliced" else 1)
+                             binned=binned, huber_delta=huber)
         torch.cuda.synchronize()
-        if path != "sliced":  # one slice: the counts follow the forward totals
-            off = 256 + (NR * 32 + 255) // 256 * 256
-            cnt = ws[off:off + 4 * NR].view(torch.int32).cpu().numpy()
-            n_redo = int((cnt == -1).sum())
-            assert int((cnt > 0).sum()) > 0 and (n_redo > 0) == (cap == 24), n_redo
+        off = 256 + (NR * 32 + 255) // 256 * 256
+        cnt = ws[off:off + 4 * NR].view(torch.int32).cpu().numpy()
+        n_redo = int((cnt == -1).sum())
+        assert int((cnt > 0).sum()) > 0 and (n_redo > 0) == (cap == 24), n_redo
     torch.cuda.synchronize()
     ro = oracle.Rays(sc["rays_np"])
     fwd = oracle.render_fwd(go, ro, sc["batch"])

@@ -1,7 +1,7 @@
 """Edge cases of the train step against the oracle, on every launch path
 (fused kernel, logged two-kernel step, logged step with rays redone past a
 short log, persistent single-wave blocks, chunk-parallel gradient,
-brick-binned gradient with a full and a short log, the batch in 3 slices):
+brick-binned gradient with a full and a short log):
 
 - ragged batch sizes: 1, 63, 65 and 257 rays (partial waves and blocks);
 - rays that miss the grid (near > far: outputs zeroed and no sample,
@@ -30,7 +30,7 @@ from helpers import (c1_planes, cameras, device_sparse_grid, from_records, grad_
 pytestmark = pytest.mark.gpu
 
 N, S, IMG = 48, 9, 48
-PATHS = ("fused", "logged", "redo", "persistent", "chunked", "binned", "binned_redo", "sliced")
+PATHS = ("fused", "logged", "redo", "persistent", "chunked", "binned", "binned_redo")
 
 
 def say(*a):
@@ -79,8 +79,6 @@ def _run(grid, rays, si, path, dev):
                                  persistent=True)
         elif path == "chunked":
             pxl.train_rays_sched(grid, optim, rays, losses, si, ws, cap, chunked=True)
-        elif path == "sliced":
-            pxl.train_rays_sched(grid, optim, rays, losses, si, ws, cap, slices=3)
         else:
             pxl.train_rays_ws(grid, optim, rays, losses, si, ws, cap)
     torch.cuda.synchronize()

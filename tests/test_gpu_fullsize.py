@@ -8,8 +8,8 @@ four 800 x 800 views of the upper hemisphere, random 8-bit targets.
   1e-5, losses 255 * 1e-5, gradients helpers.grad_close), fused and logged;
 - the whole batch against the FP64 oracle (threaded on the host): the
   fused kernel, the logged step, the persistent one-wave schedule, the
-  chunk-parallel gradient, the brick-binned gradient and the batch in four
-  slices give the oracle's per-ray colours and losses, and
+  chunk-parallel gradient and the brick-binned gradient give the oracle's
+  per-ray colours and losses, and
   gradients within helpers.grad_close or, for entries whose terms cancel
   (one entry here is 9.9e-7 from four rays' terms of 1e-3 in magnitude),
   within 1e-6 of the entry's magnitude sum (helpers.grad_close_cond: FP32
@@ -152,9 +152,6 @@ def _train(c2, si, path):
                                  binned=True, grad_waves_per_block=4)
         elif path == "chunked":
             pxl.train_rays_sched(grid, optim, c2["rays"], losses, si, ws, 512, chunked=True)
-        elif path == "sliced":
-            pxl.train_rays_sched(grid, optim, c2["rays"], losses, si, ws, 512, waves_per_block=1,
-                                 grad_waves_per_block=4, slices=4)
         else:
             pxl.train_rays_ws(grid, optim, c2["rays"], losses, si, ws, 512)
     torch.cuda.synchronize()
@@ -251,8 +248,7 @@ def c2_full_ref(c2, c2_oracle):
     return cols, l_ref, g_ref, g_abs
 
 
-@pytest.mark.parametrize("path", ["fused", "logged", "persistent", "chunked", "binned",
-                                  "sliced"])
+@pytest.mark.parametrize("path", ["fused", "logged", "persistent", "chunked", "binned"])
 def test_c2_full_batch_vs_oracle(c2, c2_full_ref, path):
     import torch
 

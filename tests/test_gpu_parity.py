@@ -336,25 +336,22 @@ def test_batch_order(c1, mode):
     assert np.all(keys[1:] >= keys[:-1])
 
 
-@pytest.mark.parametrize("sched", [(1, False, 0, 0, 0, 1), (2, True, 1, 0, 4, 1),
-                                   (4, True, 2, 0, 0, 1), (4, False, 1, 0, 1, 1),
-                                   (1, False, 0, 0, 4, 1), (4, False, 0, 1, 0, 1),
-                                   (1, False, 2, 1, 2, 1), (1, False, 0, 2, 1, 1),
-                                   (4, True, 1, 2, 4, 1), (2, False, 2, 2, 4, 1),
-                                   (1, False, 0, 0, 4, 2), (4, True, 1, 0, 0, 3),
-                                   (1, False, 2, 1, 4, 4), (1, False, 0, 2, 4, 8)])
+@pytest.mark.parametrize("sched", [(1, False, 0, 0, 0), (2, True, 1, 0, 4),
+                                   (4, True, 2, 0, 0), (4, False, 1, 0, 1),
+                                   (1, False, 0, 0, 4), (4, False, 0, 1, 0),
+                                   (1, False, 2, 1, 2), (1, False, 0, 2, 1),
+                                   (4, True, 1, 2, 4), (2, False, 2, 2, 4)])
 def test_logged_schedules(c1, sparse1, sched):
     """The two-kernel step under each schedule and batch order matches the
     oracle (sparse 128^3, 1024 rays, log of 64 samples: some rays redone;
     the chunk-parallel gradient with a log of 256: rays of up to 4 chunks;
-    the brick-binned gradient with a log of 64; the batch in 2, 3 (uneven
-    slices), 4 or 8 slices)."""
+    the brick-binned gradient with a log of 64)."""
     import torch
 
-    wpb, persist, mode, gmode, gwpb, nsl = sched
+    wpb, persist, mode, gmode, gwpb = sched
     chunked, binned = gmode == pxl.GRAD_CHUNKED, gmode == pxl.GRAD_BINNED
     say(f"[sched] waves/block {wpb} (gradient {gwpb or wpb}), persistent {persist}, "
-        f"order {mode}, gradient {pxl.GRAD_NAMES[gmode]}, slices {nsl}")
+        f"order {mode}, gradient {pxl.GRAD_NAMES[gmode]}")
     s = sparse1
     dev = c1["dev"]
     grid = device_sparse_grid(N1, S1, s["compact"], s["occ"], s["io"], s["cage"], s["c2d"], dev)
@@ -375,7 +372,7 @@ def test_logged_schedules(c1, sparse1, sched):
     ws = pxl.train_workspace(nr, cap, dev, grid=grid if binned else None)
     pxl.train_rays_sched(grid, optim, rays, losses, si, ws, cap, waves_per_block=wpb,
                          chunked=chunked, binned=binned, persistent=persist,
-                         grad_waves_per_block=gwpb, slices=nsl)
+                         grad_waves_per_block=gwpb)
     torch.cuda.synchronize()
     rays_o = oracle.Rays(c1["rays_np"])
     fwd = oracle.render_fwd(grid_o, rays_o, batch)
@@ -652,9 +649,9 @@ def test_trainer_matches_stage_calls(c1, log_samples, autotune, tv_mode):
         lr, sl = lr / d, sl / d
     torch.cuda.synchronize()
     if autotune:
-        order_, wpb_, pers_, grad_, gwpb_, sl_ = tr.schedule()
+        order_, wpb_, pers_, grad_, gwpb_ = tr.schedule()
         say(f"    autotuned: order {order_}, waves/block {wpb_}, persistent {pers_}, "
-            f"gradient {pxl.GRAD_NAMES[grad_]}, slices {sl_}")
+            f"gradient {pxl.GRAD_NAMES[grad_]}")
         assert order_ in (0, 1, 2) and wpb_ in (1, 2, 4)
     t_, lr_, sl_ = tr.state()
     assert t_ == 3 and abs(lr_ - lr) < 1e-15 and abs(sl_ - sl) < 1e-15

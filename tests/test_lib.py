@@ -38,10 +38,10 @@ def test_struct_layouts_match_header(pxl_lib):
     assert ctypes.sizeof(pxl_lib.BBox) == 24
     assert ctypes.sizeof(pxl_lib.RaysC) == 17 * 8 + 8 + 8
     assert ctypes.sizeof(pxl_lib.GridC) == 24 + 8 + 6 * 8 + 24 + 16
-    assert ctypes.sizeof(pxl_lib.TrainConfigC) == 8 + 4 * 8 + 4 * 4 + 8 + 4 * 2 + 4 * 9 + 4
+    assert ctypes.sizeof(pxl_lib.TrainConfigC) == 8 + 4 * 8 + 4 * 4 + 8 + 4 * 2 + 4 * 8
     assert ctypes.sizeof(pxl_lib.LevelConfigC) == 40
     assert ctypes.sizeof(pxl_lib.LevelC) == 24
-    assert ctypes.sizeof(pxl_lib.ScheduleC) == 20
+    assert ctypes.sizeof(pxl_lib.ScheduleC) == 16
     assert ctypes.sizeof(pxl_lib.OptimC) == 48
 
 
