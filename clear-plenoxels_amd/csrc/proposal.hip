@@ -8,9 +8,18 @@
 // and of the direction, 126 used) -> 64 -> 64 -> 64 bin logits, ReLU between
 // layers.  Weights are BF16 with an FP32 master copy; activations and
 // backward signals are rounded to BF16 for the matrix cores, products
-// accumulate in FP32 (v_mfma_f32_16x16x16_bf16).  Training is binary cross
-// entropy per bin against the bins that the grid's own samples fill, with an
-// FP32 Adam.
+// accumulate in FP32 (gfx950's v_mfma_f32_16x16x32_bf16, K = 32 per
+// instruction).  Training is binary cross entropy per bin against the bins
+// that the grid's own samples fill, with an FP32 Adam.
+//
+// Two fused kernels carry the network: k_prop_fwd (inference: the three
+// layers of a 64-row tile per workgroup, logits out) and k_prop_train (the
+// forward, the loss and the backward of a tile, each workgroup summing its
+// tiles' weight and bias gradients in registers; k_prop_reduce then adds the
+// workgroups' partial gradients in a fixed order).  Both are persistent
+// grids of resident workgroups that stage the weights in LDS once.
+#include <algorithm>
+
 #include "common.hpp"
 
 namespace pxl {
@@ -76,139 +85,380 @@ __global__ void k_prop_features(const pxl_rays r, const int32_t *__restrict__ sa
     }
 }
 
-// ----------------------------------------------------------------- GEMM --
+// ------------------------------------------------------------ network --
 
-// C[M][64] = op(A)[M][K] x op(B)[K][64] in FP32, BF16 operands.  A is
-// [M][K] row-major, or [K][M] when TA; B is [K][64] row-major, or [64][K]
-// when TB.  A workgroup of 4 waves owns 64 rows; wave w owns rows
-// 16w .. 16w+15 and all 64 columns (4 MFMA accumulators).  blockIdx.y
-// selects a K chunk of kchunk; its partial product goes to
-// C + blockIdx.y * M * 64 (split-K, summed by k_sum_partials).
-// MFMA 16x16x16 operand layout: lane l holds A[row l%16][k 4(l/16)..+3],
-// B[k 4(l/16)..+3][col l%16], and gets D[rows 4(l/16)..+3][col l%16].
-template <bool TA, bool TB>
-__global__ __launch_bounds__(256) void k_gemm_bf16(const uint16_t *__restrict__ A,
-                                                   const uint16_t *__restrict__ B,
-                                                   float *__restrict__ C, int M, int K,
-                                                   int kchunk) {
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int row0 = blockIdx.x * 64 + wave * 16;
-    if (row0 >= M) return;
-    const int kb = blockIdx.y * kchunk, ke = min(K, kb + kchunk);
-    const int lr = lane & 15, lk = (lane >> 4) * 4;
-    v4f acc[4];
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+
+// MFMA 16x16x32 BF16 operand layout (lane l, g = l / 16): A[row l % 16]
+// [k 8g .. 8g+7], B[k 8g .. 8g+7][col l % 16] as 16-byte vectors; the
+// accumulator holds D[rows 4g .. 4g+3][col l % 16].
+__device__ __forceinline__ v8bf ld8(const uint16_t *p) {
+    return __builtin_bit_cast(v8bf, *reinterpret_cast<const uint4 *>(p));
+}
+__device__ __forceinline__ v4f mma32(const v8bf &a, const v8bf &b, v4f c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// LDS row strides (BF16 elements): 8 pad elements per row so that the 16
+// lanes of a lane group, reading 16 rows at one column offset, hit distinct
+// banks; row starts stay 16-byte aligned.
+constexpr int kS128 = kIn + 8;   // rows of 128
+constexpr int kS64 = kW + 8;     // rows of 64
+constexpr int kTile = 64;        // rows per workgroup tile (4 waves x 16)
+
+// Forward weights transposed in LDS (B operands: B[k][n] = W[k][n] read as
+// WT[n][k .. k+7]) and the biases.
+struct PropLds {
+    uint16_t w1t[kW * kS128];
+    uint16_t w2t[kW * kS64];
+    uint16_t w3t[kBins * kS64];
+    float b[3 * kW];
+};
+__device__ __forceinline__ void stage_weights(const uint16_t *__restrict__ w,
+                                              const float *__restrict__ bias, PropLds &L) {
+    for (int i = threadIdx.x; i < kW1; i += blockDim.x) {
+        const int k = i / kW, n = i % kW;
+        L.w1t[n * kS128 + k] = w[i];
+    }
+    for (int i = threadIdx.x; i < kW2; i += blockDim.x) {
+        const int k = i / kW, n = i % kW;
+        L.w2t[n * kS64 + k] = w[kW1 + i];
+        L.w3t[n * kS64 + k] = w[kW1 + kW2 + i];
+    }
+    for (int i = threadIdx.x; i < 3 * kW; i += blockDim.x) L.b[i] = bias[i];
+}
+
+// One layer of a wave's 16 rows: acc[n] (output columns 16n .. 16n+15) =
+// A (16 x K, row-major at a, row stride sa) x W (K x 64, transposed at wt).
+template <int K>
+__device__ __forceinline__ void layer(const uint16_t *a, int sa, const uint16_t *wt, int swt,
+                                      v4f (&acc)[4]) {
+    const int lane = threadIdx.x & 63, lr = lane & 15, g = lane >> 4;
 #pragma unroll
     for (int n = 0; n < 4; n++) acc[n] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
-    for (int k0 = kb; k0 < ke; k0 += 16) {
-        v4s a;
+#pragma unroll
+    for (int kb = 0; kb < K / 32; kb++) {
+        const v8bf av = ld8(a + lr * sa + 32 * kb + 8 * g);
+#pragma unroll
+        for (int n = 0; n < 4; n++)
+            acc[n] = mma32(av, ld8(wt + (16 * n + lr) * swt + 32 * kb + 8 * g), acc[n]);
+    }
+}
+
+// relu(acc + bias) as BF16 into a wave's 16 rows at h (row stride sh), and
+// (when ht) transposed into ht[col][rows] (row stride sht, first row r0);
+// rows of the tile at or beyond len are zero.
+__device__ __forceinline__ void relu_out(const v4f (&acc)[4], const float *bias, int64_t row0,
+                                         int64_t len, uint16_t *h, int sh, uint16_t *ht, int sht,
+                                         int r0) {
+    const int lane = threadIdx.x & 63, lr = lane & 15, g = lane >> 4;
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+        const int col = 16 * n + lr;
+        uint16_t v[4];
 #pragma unroll
         for (int j = 0; j < 4; j++) {
-            const int m = row0 + lr, k = k0 + lk + j;
-            a[j] = (short)(TA ? A[(size_t)k * M + m] : A[(size_t)m * K + k]);
+            const int row = 4 * g + j;
+            const float x = row0 + row < len ? fmaxf(acc[n][j] + bias[col], 0.0f) : 0.0f;
+            v[j] = f2bf(x);
+            h[row * sh + col] = v[j];
         }
-#pragma unroll
-        for (int n = 0; n < 4; n++) {
-            v4s bb;
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const int col = n * 16 + lr, k = k0 + lk + j;
-                bb[j] = (short)(TB ? B[(size_t)col * K + k] : B[(size_t)k * 64 + col]);
-            }
-            acc[n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bb, acc[n], 0, 0, 0);
+        if (ht) {
+            const uint2 pk = make_uint2((uint32_t)v[0] | ((uint32_t)v[1] << 16),
+                                        (uint32_t)v[2] | ((uint32_t)v[3] << 16));
+            *reinterpret_cast<uint2 *>(ht + col * sht + r0 + 4 * g) = pk;
         }
     }
-    float *Cp = C + (size_t)blockIdx.y * (size_t)M * 64;
+}
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Inference: logits Z[rows][64] (FP32) of the featured rows X[rows][128],
+// one 64-row tile per workgroup at a time (wave w: rows 16w .. 16w+15).
+__global__ __launch_bounds__(256) void k_prop_fwd(const uint16_t *__restrict__ X,
+                                                  const uint16_t *__restrict__ w,
+                                                  const float *__restrict__ bias, int64_t len,
+                                                  int64_t rows, float *__restrict__ Z) {
+    __shared__ PropLds L;
+    __shared__ __align__(16) uint16_t hbuf[4][16 * kS64];
+    stage_weights(w, bias, L);
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, lr = lane & 15, g = lane >> 4;
+    uint16_t *h = hbuf[wave];
+    const int64_t tiles = rows / kTile;
+    for (int64_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+        const int64_t row0 = t * kTile + 16 * wave;
+        v4f acc[4];
+        layer<kIn>(X + row0 * kIn, kIn, L.w1t, kS128, acc);
+        relu_out(acc, L.b, row0, len, h, kS64, nullptr, 0, 0);
+        wave_sync();
+        layer<kW>(h, kS64, L.w2t, kS64, acc);
+        wave_sync();  // every lane has read h before it is overwritten
+        relu_out(acc, L.b + kW, row0, len, h, kS64, nullptr, 0, 0);
+        wave_sync();
+        layer<kW>(h, kS64, L.w3t, kS64, acc);
+        wave_sync();
 #pragma unroll
-    for (int n = 0; n < 4; n++)
+        for (int n = 0; n < 4; n++) {
+            const int col = 16 * n + lr;
 #pragma unroll
-        for (int j = 0; j < 4; j++)
-            Cp[(size_t)(row0 + lk + j) * 64 + n * 16 + lr] = acc[n][j];
+            for (int j = 0; j < 4; j++) {
+                const int64_t r = row0 + 4 * g + j;
+                if (r < len) Z[r * kBins + col] = acc[n][j] + L.b[2 * kW + col];
+            }
+        }
+    }
 }
 
-// out[i] = sum over c < chunks of part[c * n + i] (+ out[i] when accumulate).
-__global__ void k_sum_partials(const float *__restrict__ part, int chunks, int n,
-                               float *__restrict__ out) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    float s = 0.0f;
-    for (int c = 0; c < chunks; c++) s += part[(size_t)c * n + i];
-    out[i] = s;
+// Training.  Per tile: the forward (H1, H2 kept in LDS transposed for the
+// weight gradients), the loss and dZ = bf16(scale w (sigmoid(z) - y)), then
+// dW3 += H2^T dZ, G2 = dZ W3^T, dH2 = bf16(G2 [H2 > 0]), dW2 += H1^T dH2,
+// G1 = dH2 W2^T, dH1 = bf16(G1 [H1 > 0]), dW1 += X^T dH1, and the bias
+// gradients as column sums of dZ, dH2, dH1.  Weight gradient blocks of
+// 16 x 16 are split over the waves (dW1: wave w owns rows 32w .. 32w+31,
+// dW2 and dW3 rows 16w .. 16w+15) and accumulate over the workgroup's tiles
+// in registers; at the end each workgroup writes its partial gradient and
+// loss to part[blockIdx.x].
+struct TrainLds {
+    uint16_t xt[kIn * (kTile + 8)];    // X^T [feature][row]
+    uint16_t h1t[kW * (kTile + 8)];    // H1^T [unit][row]
+    uint16_t h2t[kW * (kTile + 8)];    // H2^T
+    uint16_t rm[kTile * kS64];         // row-major tile: H1, H2, dZ, dH2 in turn
+    uint16_t cm[kW * (kTile + 8)];     // column-major tile: dZ, dH2, dH1 in turn
+    float db[3 * kW];                  // bias gradients of this workgroup's tiles
+};
+constexpr int kST = kTile + 8;  // row stride of the transposed tiles
+
+__global__ __launch_bounds__(256) void k_prop_train(
+    const uint16_t *__restrict__ X, const uint16_t *__restrict__ w,
+    const float *__restrict__ bias, int64_t len, int64_t rows,
+    const uint64_t *__restrict__ targets, const uint8_t *__restrict__ valid, float pos_weight,
+    float scale, float *__restrict__ part, double *__restrict__ part_loss) {
+    __shared__ PropLds L;
+    __shared__ TrainLds T;
+    stage_weights(w, bias, L);
+    for (int i = threadIdx.x; i < 3 * kW; i += blockDim.x) T.db[i] = 0.0f;
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, lr = lane & 15, g = lane >> 4;
+    const uint16_t *W2 = w + kW1, *W3 = w + kW1 + kW2;  // row-major [in][out]
+    v4f dw1[2][4], dw2[4], dw3[4];
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+        dw1[0][n] = dw1[1][n] = dw2[n] = dw3[n] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+    }
+    double loss = 0.0;
+    const int64_t tiles = rows / kTile;
+    const int r0 = 16 * wave;  // this wave's first row within the tile
+    for (int64_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+        const int64_t row0 = t * kTile + r0;
+        // X rows of this wave, transposed into LDS for dW1.
+#pragma unroll
+        for (int kb = 0; kb < kIn / 32; kb++) {
+            const uint4 u = *reinterpret_cast<const uint4 *>(X + (row0 + lr) * kIn + 32 * kb + 8 * g);
+            const uint32_t e[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+            for (int q = 0; q < 8; q++)
+                T.xt[(32 * kb + 8 * g + q) * kST + r0 + lr] = (uint16_t)(e[q >> 1] >> (16 * (q & 1)));
+        }
+        v4f acc[4];
+        uint16_t *rm = T.rm + r0 * kS64;
+        layer<kIn>(X + row0 * kIn, kIn, L.w1t, kS128, acc);
+        relu_out(acc, L.b, row0, len, rm, kS64, T.h1t, kST, r0);
+        wave_sync();
+        layer<kW>(rm, kS64, L.w2t, kS64, acc);
+        wave_sync();
+        relu_out(acc, L.b + kW, row0, len, rm, kS64, T.h2t, kST, r0);
+        wave_sync();
+        layer<kW>(rm, kS64, L.w3t, kS64, acc);
+        wave_sync();
+        // Loss and dZ (k_logits_bce's arithmetic).
+        double lrow[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int n = 0; n < 4; n++) {
+            const int col = 16 * n + lr;
+            uint16_t v[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int64_t r = row0 + 4 * g + j;
+                const bool live = r < len;
+                const float z = live ? acc[n][j] + L.b[2 * kW + col] : 0.0f;
+                const bool use = live && (valid == nullptr || valid[r] != 0);
+                const float y = use ? (float)((targets[r] >> col) & 1ull) : 0.0f;
+                const float p = 1.0f / (1.0f + expf(-z));
+                const float wt = y > 0.0f ? pos_weight : 1.0f;
+                v[j] = f2bf(use ? scale * wt * (p - y) : 0.0f);
+                rm[(4 * g + j) * kS64 + col] = v[j];
+                if (use)
+                    lrow[j] += (double)wt * ((double)fmaxf(z, 0.0f) - (double)z * y +
+                                             (double)log1pf(expf(-fabsf(z))));
+            }
+            *reinterpret_cast<uint2 *>(T.cm + col * kST + r0 + 4 * g) =
+                make_uint2((uint32_t)v[0] | ((uint32_t)v[1] << 16),
+                           (uint32_t)v[2] | ((uint32_t)v[3] << 16));
+        }
+        loss += lrow[0] + lrow[1] + lrow[2] + lrow[3];
+        __syncthreads();  // dZ (cm) and H2^T of every wave are in LDS
+        // dW3 rows 16w..16w+15 (hidden units), all 64 bins; db3.
+#pragma unroll
+        for (int ks = 0; ks < kTile / 32; ks++) {
+            const v8bf a = ld8(T.h2t + (16 * wave + lr) * kST + 32 * ks + 8 * g);
+#pragma unroll
+            for (int n = 0; n < 4; n++)
+                dw3[n] = mma32(a, ld8(T.cm + (16 * n + lr) * kST + 32 * ks + 8 * g), dw3[n]);
+        }
+        if (threadIdx.x < kBins) {
+            float s = 0.0f;
+            for (int r = 0; r < kTile; r++) s += bf2f(T.cm[threadIdx.x * kST + r]);
+            T.db[2 * kW + threadIdx.x] += s;
+        }
+        // G2 = dZ W3^T (this wave's rows), B[k = bin][n = unit] = W3[unit][bin].
+        v4f gacc[4];
+#pragma unroll
+        for (int n = 0; n < 4; n++) gacc[n] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int kb = 0; kb < kBins / 32; kb++) {
+            const v8bf a = ld8(rm + lr * kS64 + 32 * kb + 8 * g);
+#pragma unroll
+            for (int n = 0; n < 4; n++)
+                gacc[n] = mma32(a, ld8(W3 + (16 * n + lr) * kBins + 32 * kb + 8 * g), gacc[n]);
+        }
+        __syncthreads();  // dZ in cm and rm fully read
+        // dH2 = bf16(G2 [H2 > 0]) into rm (row-major) and cm (column-major).
+#pragma unroll
+        for (int n = 0; n < 4; n++) {
+            const int col = 16 * n + lr;
+            uint16_t v[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const uint16_t hv = T.h2t[col * kST + r0 + 4 * g + j];
+                v[j] = f2bf(hv != 0 && (hv & 0x8000u) == 0 ? gacc[n][j] : 0.0f);
+                rm[(4 * g + j) * kS64 + col] = v[j];
+            }
+            *reinterpret_cast<uint2 *>(T.cm + col * kST + r0 + 4 * g) =
+                make_uint2((uint32_t)v[0] | ((uint32_t)v[1] << 16),
+                           (uint32_t)v[2] | ((uint32_t)v[3] << 16));
+        }
+        __syncthreads();
+        // dW2 rows 16w..16w+15, db2.
+#pragma unroll
+        for (int ks = 0; ks < kTile / 32; ks++) {
+            const v8bf a = ld8(T.h1t + (16 * wave + lr) * kST + 32 * ks + 8 * g);
+#pragma unroll
+            for (int n = 0; n < 4; n++)
+                dw2[n] = mma32(a, ld8(T.cm + (16 * n + lr) * kST + 32 * ks + 8 * g), dw2[n]);
+        }
+        if (threadIdx.x < kW) {
+            float s = 0.0f;
+            for (int r = 0; r < kTile; r++) s += bf2f(T.cm[threadIdx.x * kST + r]);
+            T.db[kW + threadIdx.x] += s;
+        }
+        // G1 = dH2 W2^T, B[k = out][n = in] = W2[in][out].
+#pragma unroll
+        for (int n = 0; n < 4; n++) gacc[n] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int kb = 0; kb < kW / 32; kb++) {
+            const v8bf a = ld8(rm + lr * kS64 + 32 * kb + 8 * g);
+#pragma unroll
+            for (int n = 0; n < 4; n++)
+                gacc[n] = mma32(a, ld8(W2 + (16 * n + lr) * kW + 32 * kb + 8 * g), gacc[n]);
+        }
+        __syncthreads();  // dH2 in cm fully read
+        // dH1 = bf16(G1 [H1 > 0]) into cm.
+#pragma unroll
+        for (int n = 0; n < 4; n++) {
+            const int col = 16 * n + lr;
+            uint16_t v[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const uint16_t hv = T.h1t[col * kST + r0 + 4 * g + j];
+                v[j] = f2bf(hv != 0 && (hv & 0x8000u) == 0 ? gacc[n][j] : 0.0f);
+            }
+            *reinterpret_cast<uint2 *>(T.cm + col * kST + r0 + 4 * g) =
+                make_uint2((uint32_t)v[0] | ((uint32_t)v[1] << 16),
+                           (uint32_t)v[2] | ((uint32_t)v[3] << 16));
+        }
+        __syncthreads();
+        // dW1 rows 32w..32w+31 (features), db1.
+#pragma unroll
+        for (int ks = 0; ks < kTile / 32; ks++) {
+#pragma unroll
+            for (int m = 0; m < 2; m++) {
+                const v8bf a = ld8(T.xt + (32 * wave + 16 * m + lr) * kST + 32 * ks + 8 * g);
+#pragma unroll
+                for (int n = 0; n < 4; n++)
+                    dw1[m][n] = mma32(a, ld8(T.cm + (16 * n + lr) * kST + 32 * ks + 8 * g),
+                                      dw1[m][n]);
+            }
+        }
+        if (threadIdx.x < kW) {
+            float s = 0.0f;
+            for (int r = 0; r < kTile; r++) s += bf2f(T.cm[threadIdx.x * kST + r]);
+            T.db[threadIdx.x] += s;
+        }
+        __syncthreads();  // the next tile overwrites every LDS tile
+    }
+    // This workgroup's partial gradient, in the gradient's layout.
+    float *pg = part + (size_t)blockIdx.x * kParams;
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+        const int col = 16 * n + lr;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int r = 4 * g + j;
+            pg[(32 * wave + r) * kW + col] = dw1[0][n][j];
+            pg[(32 * wave + 16 + r) * kW + col] = dw1[1][n][j];
+            pg[kW1 + (16 * wave + r) * kW + col] = dw2[n][j];
+            pg[kW1 + kW2 + (16 * wave + r) * kBins + col] = dw3[n][j];
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < 3 * kW; i += blockDim.x) pg[kWeights + i] = T.db[i];
+    // Loss: per-wave sums (each row's 64 bins over 16 lanes and 4 n), then
+    // the waves in order.
+    __shared__ double lw[4];
+    loss = wave_sum_d(loss);
+    if (lane == 0) lw[wave] = loss;
+    __syncthreads();
+    if (threadIdx.x == 0) part_loss[blockIdx.x] = lw[0] + lw[1] + lw[2] + lw[3];
 }
 
-// ------------------------------------------------------------ epilogues --
-
-// H = bf16(relu(C + bias)); rows >= len (tile padding) are zero.
-__global__ void k_bias_relu(const float *__restrict__ Cm, const float *__restrict__ bias,
-                            int64_t len, int64_t rows, uint16_t *__restrict__ H) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= rows * 64) return;
-    const int64_t r = i / 64;
-    const float v = r < len ? fmaxf(Cm[i] + bias[i & 63], 0.0f) : 0.0f;
-    H[i] = f2bf(v);
-}
-
-// Logits z = C + bias (FP32).  Training: p = sigmoid(z), target bit y of
-// the row's mask, weight pos_weight for y = 1; dZ = bf16(scale w (p - y)),
-// scale = 1 / (valid rows * 64); the row's BCE summed into loss_rows.
-__global__ void k_logits_bce(const float *__restrict__ Cm, const float *__restrict__ bias,
-                             int64_t len, int64_t rows, float *__restrict__ Z,
-                             const uint64_t *__restrict__ targets,
-                             const uint8_t *__restrict__ valid, float pos_weight, float scale,
-                             uint16_t *__restrict__ dZ, double *__restrict__ loss_rows) {
-    const int64_t r = (int64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x / 64);
-    const int j = threadIdx.x & 63;
-    if (r >= rows) return;
-    const int64_t i = r * 64 + j;
-    const bool live = r < len;
-    const float z = live ? Cm[i] + bias[j] : 0.0f;
-    if (Z && live) Z[i] = z;
-    if (!dZ) return;
-    const bool use = live && (valid == nullptr || valid[r] != 0);
-    const float y = use ? (float)((targets[r] >> j) & 1ull) : 0.0f;
-    const float p = 1.0f / (1.0f + expf(-z));
-    const float w = y > 0.0f ? pos_weight : 1.0f;
-    dZ[i] = f2bf(use ? scale * w * (p - y) : 0.0f);
-    // BCE with logits: max(z,0) - z y + log1p(exp(-|z|)), weighted.
-    double l = use ? (double)w * ((double)fmaxf(z, 0.0f) - (double)z * y +
-                                  (double)log1pf(expf(-fabsf(z))))
-                   : 0.0;
-    for (int off = 32; off > 0; off >>= 1) l += __shfl_xor(l, off, 64);
-    if (j == 0 && loss_rows) loss_rows[r] = l;
-}
-
-// dH = bf16(G * [H > 0]).
-__global__ void k_relu_back(const float *__restrict__ G, const uint16_t *__restrict__ H,
-                            int64_t n, uint16_t *__restrict__ dH) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    dH[i] = f2bf(H[i] != 0 && (H[i] & 0x8000u) == 0 ? G[i] : 0.0f);
-}
-
-// Column sums of a [rows][64] BF16 matrix (bias gradients): block c sums
-// rows [c * chunk_rows, (c+1) * chunk_rows) into part[c][64]; k_sum_partials
-// adds the chunks.
-__global__ void k_colsum(const uint16_t *__restrict__ D, int64_t rows, int chunk_rows,
-                         float *__restrict__ part) {
+// Gradient = sum of the workgroups' partials; loss too.  A block of 256
+// threads covers 64 parameters: thread (q, i) sums the partials of
+// workgroups q, q + 4, ... of parameter i, then the four sums are added in
+// order (a fixed order for a given number of workgroups).
+__global__ __launch_bounds__(256) void k_prop_reduce(const float *__restrict__ part,
+                                                     const double *__restrict__ part_loss,
+                                                     int groups, float *__restrict__ grad,
+                                                     double *__restrict__ loss_out) {
     __shared__ float red[4][64];
-    const int j = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int64_t r0 = (int64_t)blockIdx.x * chunk_rows;
-    const int64_t r1 = min(rows, r0 + chunk_rows);
+    const int q = threadIdx.x >> 6, il = threadIdx.x & 63;
+    const int i = blockIdx.x * 64 + il;
     float s = 0.0f;
-    for (int64_t r = r0 + w; r < r1; r += 4) s += bf2f(D[r * 64 + j]);
-    red[w][j] = s;
+    if (i < kParams) {
+        int c = q;
+        for (; c + 12 < groups; c += 16) {
+            const float a = part[(size_t)c * kParams + i], b = part[(size_t)(c + 4) * kParams + i],
+                        d = part[(size_t)(c + 8) * kParams + i],
+                        e = part[(size_t)(c + 12) * kParams + i];
+            s += a;
+            s += b;
+            s += d;
+            s += e;
+        }
+        for (; c < groups; c += 4) s += part[(size_t)c * kParams + i];
+    }
+    red[q][il] = s;
     __syncthreads();
-    if (w == 0) part[(size_t)blockIdx.x * 64 + j] = red[0][j] + red[1][j] + red[2][j] + red[3][j];
-}
-
-// Sum of per-row losses (one block, FP64).
-__global__ void k_sum_rows(const double *__restrict__ l, int64_t n, double *__restrict__ out) {
-    __shared__ double part[4];
-    double s = 0.0;
-    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) s += l[i];
-    s = wave_sum_d(s);
-    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) out[0] = part[0] + part[1] + part[2] + part[3];
+    if (q == 0 && i < kParams) grad[i] = ((red[0][il] + red[1][il]) + red[2][il]) + red[3][il];
+    if (blockIdx.x == 0 && threadIdx.x == 0 && loss_out) {
+        double l = 0.0;
+        for (int c = 0; c < groups; c++) l += part_loss[c];
+        loss_out[0] = l;
+    }
 }
 
 // Adam on the FP32 master (weights then biases), BF16 copy of the weights.
@@ -267,81 +517,52 @@ __global__ void k_prop_masks(const float *__restrict__ Z, int64_t len, float thr
 
 // -------------------------------------------------------------- host side --
 
+// Proposal workspace: feature rows X [rows][128] BF16 (rows = len rounded
+// up to the 64-row tile), logits [rows][64] FP32, the training kernel's
+// per-workgroup partial gradients and losses.
 struct PropLayout {
-    size_t X, C, H1, H2, Z, dZ, dH, part, loss, total;
+    size_t X, Z, part, ploss, total;
     int64_t rows;
-    int chunks;
 };
 inline size_t a256(size_t x) { return (x + 255) / 256 * 256; }
-constexpr int kChunkRows = 1024;  // split-K chunk of the weight-gradient GEMMs
+constexpr int kMaxGroups = 512;  // workgroups of the persistent kernels, at most
 
 PropLayout prop_layout(int64_t len) {
     PropLayout L;
-    L.rows = (len + 63) / 64 * 64;
-    L.chunks = (int)((L.rows + kChunkRows - 1) / kChunkRows);
+    L.rows = (len + kTile - 1) / kTile * kTile;
     size_t o = 0;
     L.X = o;
     o += a256((size_t)L.rows * kIn * 2);
-    L.C = o;
-    o += a256((size_t)L.rows * 64 * 4);
-    L.H1 = o;
-    o += a256((size_t)L.rows * 64 * 2);
-    L.H2 = o;
-    o += a256((size_t)L.rows * 64 * 2);
     L.Z = o;
-    o += a256((size_t)L.rows * 64 * 4);
-    L.dZ = o;
-    o += a256((size_t)L.rows * 64 * 2);
-    L.dH = o;
-    o += a256((size_t)L.rows * 64 * 2);
+    o += a256((size_t)L.rows * kBins * 4);
     L.part = o;
-    o += a256((size_t)L.chunks * kIn * 64 * 4);
-    L.loss = o;
-    o += a256((size_t)L.rows * 8 + 8);
+    o += a256((size_t)kMaxGroups * kParams * 4);
+    L.ploss = o;
+    o += a256((size_t)kMaxGroups * 8);
     L.total = o;
     return L;
 }
 
-void gemm(bool ta, bool tb, const uint16_t *A, const uint16_t *B, float *C, int M, int K,
-          int chunks, hipStream_t st) {
-    // Chunks are whole 64-row tiles, so no 16-deep K step crosses a chunk.
-    const int kchunk = ((K + chunks - 1) / chunks + 63) / 64 * 64;
-    const dim3 grid((unsigned)((M + 63) / 64), (unsigned)chunks);
-    if (!ta && !tb) hipLaunchKernelGGL((k_gemm_bf16<false, false>), grid, dim3(256), 0, st, A, B, C, M, K, kchunk);
-    else if (!ta && tb) hipLaunchKernelGGL((k_gemm_bf16<false, true>), grid, dim3(256), 0, st, A, B, C, M, K, kchunk);
-    else hipLaunchKernelGGL((k_gemm_bf16<true, false>), grid, dim3(256), 0, st, A, B, C, M, K, kchunk);
-}
-
-// Bias gradient: column sums of D in chunks of kChunkRows rows, then added.
-void colsum(const uint16_t *D, const PropLayout &L, float *part, float *out, hipStream_t st) {
-    hipLaunchKernelGGL(k_colsum, dim3((unsigned)L.chunks), dim3(256), 0, st, D, L.rows,
-                       kChunkRows, part);
-    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, st, part, L.chunks, 64, out);
+// Workgroups of a persistent kernel: those resident on the device at once,
+// at most one per tile and kMaxGroups.
+template <typename K>
+unsigned prop_groups(K kernel, int64_t tiles) {
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0) != hipSuccess ||
+        cus <= 0 || per_cu <= 0)
+        cus = 256, per_cu = 1;
+    int64_t g = (int64_t)cus * per_cu;
+    g = std::min<int64_t>(g, tiles);
+    g = std::min<int64_t>(g, kMaxGroups);
+    return (unsigned)std::max<int64_t>(g, 1);
 }
 
 int32_t check_prop(const pxl_proposal *p, const char *who) {
     PXL_REQUIRE(p && p->weights && p->master && p->m && p->v && p->grad, "%s: null proposal buffer",
                 who);
     return PXL_OK;
-}
-
-// Forward through the three layers; leaves H1, H2 (BF16) and the last
-// layer's FP32 product in C.
-void forward(const pxl_proposal *p, const char *w, const PropLayout &L, int64_t len,
-             hipStream_t st) {
-    const uint16_t *X = (const uint16_t *)(w + L.X);
-    float *C = (float *)(w + L.C);
-    uint16_t *H1 = (uint16_t *)(w + L.H1), *H2 = (uint16_t *)(w + L.H2);
-    const uint16_t *W1 = p->weights, *W2 = W1 + kW1, *W3 = W2 + kW2;
-    const float *b1 = p->master + kWeights, *b2 = b1 + kW, *b3 = b2 + kW;
-    (void)b3;
-    const int M = (int)L.rows;
-    const unsigned eb = (unsigned)((L.rows * 64 + 255) / 256);
-    gemm(false, false, X, W1, C, M, kIn, 1, st);
-    hipLaunchKernelGGL(k_bias_relu, dim3(eb), dim3(256), 0, st, C, b1, len, L.rows, H1);
-    gemm(false, false, H1, W2, C, M, kW, 1, st);
-    hipLaunchKernelGGL(k_bias_relu, dim3(eb), dim3(256), 0, st, C, b2, len, L.rows, H2);
-    gemm(false, false, H2, W3, C, M, kW, 1, st);
 }
 
 }  // namespace
@@ -392,12 +613,11 @@ extern "C" int32_t pxl_proposal_forward(const pxl_proposal *p, int64_t len, floa
     PXL_REQUIRE(workspace_bytes >= L.total, "pxl_proposal_forward: workspace too small");
     char *w = static_cast<char *>(workspace);
     const hipStream_t s = as_stream(stream);
-    forward(p, w, L, len, s);
-    const float *b3 = p->master + kWeights + 2 * kW;
     float *Z = logits ? logits : (float *)(w + L.Z);
-    hipLaunchKernelGGL(k_logits_bce, dim3((unsigned)((L.rows + 3) / 4)), dim3(256), 0, s,
-                       (const float *)(w + L.C), b3, len, L.rows, Z, nullptr, nullptr, 1.0f,
-                       1.0f, nullptr, nullptr);
+    const int64_t tiles = L.rows / kTile;
+    hipLaunchKernelGGL(k_prop_fwd, dim3(prop_groups(k_prop_fwd, tiles)), dim3(256), 0, s,
+                       (const uint16_t *)(w + L.X), (const uint16_t *)p->weights,
+                       (const float *)(p->master + kWeights), len, L.rows, Z);
     return check_launch("pxl_proposal_forward");
 }
 
@@ -415,42 +635,16 @@ extern "C" int32_t pxl_proposal_train_step(const pxl_proposal *p, int64_t len,
     PXL_REQUIRE(workspace_bytes >= L.total, "pxl_proposal_train_step: workspace too small");
     char *w = static_cast<char *>(workspace);
     const hipStream_t s = as_stream(stream);
-    forward(p, w, L, len, s);
-    const int M = (int)L.rows;
-    const uint16_t *X = (const uint16_t *)(w + L.X);
-    float *C = (float *)(w + L.C);
-    uint16_t *H1 = (uint16_t *)(w + L.H1), *H2 = (uint16_t *)(w + L.H2);
-    uint16_t *dZ = (uint16_t *)(w + L.dZ), *dH = (uint16_t *)(w + L.dH);
-    float *part = (float *)(w + L.part);
-    double *lrow = (double *)(w + L.loss);
-    const uint16_t *W2 = p->weights + kW1, *W3 = W2 + kW2;
-    const float *b3 = p->master + kWeights + 2 * kW;
-    float *g = p->grad;
     const float scale = 1.0f / (float)((n_valid > 0 ? n_valid : len) * 64);
-    hipLaunchKernelGGL(k_logits_bce, dim3((unsigned)((L.rows + 3) / 4)), dim3(256), 0, s, C, b3,
-                       len, L.rows, (float *)(w + L.Z), targets, valid, pos_weight, scale, dZ,
-                       lrow);
-    const unsigned n64 = (unsigned)((L.rows * 64 + 255) / 256);
-    // Layer 3: dW3 = H2^T dZ, db3 = colsum(dZ), G = dZ W3^T.
-    gemm(true, false, H2, dZ, part, kW, M, L.chunks, s);
-    hipLaunchKernelGGL(k_sum_partials, dim3(kW3 / 256), dim3(256), 0, s, part, L.chunks, kW3,
-                       g + kW1 + kW2);
-    colsum(dZ, L, part, g + kWeights + 2 * kW, s);
-    gemm(false, true, dZ, W3, C, M, kBins, 1, s);
-    hipLaunchKernelGGL(k_relu_back, dim3(n64), dim3(256), 0, s, C, H2, L.rows * 64, dH);
-    // Layer 2: dW2 = H1^T dH2, db2, G = dH2 W2^T.
-    gemm(true, false, H1, dH, part, kW, M, L.chunks, s);
-    hipLaunchKernelGGL(k_sum_partials, dim3(kW2 / 256), dim3(256), 0, s, part, L.chunks, kW2,
-                       g + kW1);
-    colsum(dH, L, part, g + kWeights + kW, s);
-    gemm(false, true, dH, W2, C, M, kW, 1, s);
-    hipLaunchKernelGGL(k_relu_back, dim3(n64), dim3(256), 0, s, C, H1, L.rows * 64, dZ);
-    // Layer 1: dW1 = X^T dH1, db1.
-    gemm(true, false, X, dZ, part, kIn, M, L.chunks, s);
-    hipLaunchKernelGGL(k_sum_partials, dim3(kW1 / 256), dim3(256), 0, s, part, L.chunks, kW1, g);
-    colsum(dZ, L, part, g + kWeights, s);
-    if (loss_out)
-        hipLaunchKernelGGL(k_sum_rows, dim3(1), dim3(256), 0, s, lrow, L.rows, loss_out);
+    const int64_t tiles = L.rows / kTile;
+    const unsigned groups = prop_groups(k_prop_train, tiles);
+    float *part = (float *)(w + L.part);
+    double *ploss = (double *)(w + L.ploss);
+    hipLaunchKernelGGL(k_prop_train, dim3(groups), dim3(256), 0, s, (const uint16_t *)(w + L.X),
+                       (const uint16_t *)p->weights, (const float *)(p->master + kWeights), len,
+                       L.rows, targets, valid, pos_weight, scale, part, ploss);
+    hipLaunchKernelGGL(k_prop_reduce, dim3((kParams + 63) / 64), dim3(256), 0, s, part, ploss,
+                       (int)groups, p->grad, loss_out);
     if (lr > 0.0) {
         const double c1 = 1.0 - pow(0.9, (double)t), c2 = 1.0 - pow(0.999, (double)t);
         hipLaunchKernelGGL(k_prop_adam, dim3((kParams + 255) / 256), dim3(256), 0, s, p->master,
