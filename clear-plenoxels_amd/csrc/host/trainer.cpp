@@ -225,6 +225,12 @@ extern "C" pxl_trainer *pxl_trainer_create(const pxl_train_config *cfg, const px
             pxl_train_rays_workspace_bytes_grid(grid, cfg->batch_size, cfg->log_samples);
         tr->binned_ok = tr->march_ws_bytes >
                         pxl_train_rays_workspace_bytes(cfg->batch_size, cfg->log_samples);
+        if (cfg->gradient == PXL_GRAD_BINNED && !tr->binned_ok) {
+            set_error("pxl_trainer_create: the brick-binned gradient needs 2 to 1025 vertices "
+                      "per side");
+            pxl_trainer_destroy(tr);
+            return nullptr;
+        }
         if (tr->march_ws_bytes == 0 ||
             hipMalloc(&tr->march_ws, tr->march_ws_bytes) != hipSuccess) {
             set_error("pxl_trainer_create: cannot allocate a %zu-byte sample log for %d samples "
@@ -360,17 +366,15 @@ static int32_t autotune(pxl_trainer *tr, int64_t start, int64_t len) {
     std::vector<Cand> cands;
     // Waves per workgroup of the forward and (separately) of the gradient
     // kernel; persistent launches share one shape; the chunk-parallel
-    // gradient with 4 waves per workgroup; the brick-binned gradient with 1
-    // or 4 waves per workgroup of its per-ray passes.
+    // gradient with 4 waves per workgroup.  The brick-binned gradient is not
+    // a candidate: it measured 3x (256^3 caged) to 10x (dense 128^3) slower
+    // than the per-ray walk; cfg.gradient selects it without autotuning.
     for (int32_t order : {0, (int32_t)PXL_ORDER_MORTON, (int32_t)PXL_ORDER_LENGTH_MORTON})
         for (int32_t wpb : {1, 2, 4}) {
             for (int32_t gwpb : {1, 2, 4})
                 cands.push_back({order, {wpb, 0, PXL_GRAD_PER_RAY, gwpb}});
             cands.push_back({order, {wpb, 1, PXL_GRAD_PER_RAY, 0}});
             cands.push_back({order, {wpb, 0, PXL_GRAD_CHUNKED, 4}});
-            if (tr->binned_ok)
-                for (int32_t gwpb : {1, 4})
-                    cands.push_back({order, {wpb, 0, PXL_GRAD_BINNED, gwpb}});
         }
     constexpr int kReps = 2;
     std::vector<hipEvent_t> ev(cands.size() * kReps * 2, nullptr);

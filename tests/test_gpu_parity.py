@@ -599,7 +599,8 @@ def test_train_step_c1(c1):
 
 @pytest.mark.parametrize("log_samples,autotune,tv_mode",
                          [(0, False, "fused"), (512, False, "fused"), (512, True, "overlap"),
-                          (512, False, "overlap"), (512, False, "inline")])
+                          (512, False, "overlap"), (512, False, "inline"),
+                          (512, False, "binned")])
 def test_trainer_matches_stage_calls(c1, log_samples, autotune, tv_mode):
     """The C++ batch loop (pxl_trainer_step) issues the same stages as the
     Python-driven sequence: dense grid (TV every batch), 3 batches, LR decay
@@ -608,7 +609,8 @@ def test_trainer_matches_stage_calls(c1, log_samples, autotune, tv_mode):
     with Adam (the trainer alternates two record buffers: after 3 batches the
     current records are in its own, pxl_trainer_sync brings them back), or
     separate, on the second stream beside the forward (overlap) or after the
-    render (inline)."""
+    render (inline); "binned": fused TV + Adam with the brick-binned
+    gradient schedule."""
     import math
 
     import torch
@@ -631,11 +633,12 @@ def test_trainer_matches_stage_calls(c1, log_samples, autotune, tv_mode):
                      sh_learning_rate=0.2, variance_coefficient=1e-5, sh_coefficient=1e-6,
                      variance_interval=3, fused=True, log_samples=log_samples,
                      autotune=autotune, tv_inline=tv_mode == "inline",
-                     tv_separate=tv_mode != "fused")
+                     tv_separate=tv_mode not in ("fused", "binned"),
+                     binned=tv_mode == "binned")
     for _ in range(3):
         tr.step()
     current = tr.grid_c().params
-    assert (current != grids[0].params.data_ptr()) == (tv_mode == "fused")
+    assert (current != grids[0].params.data_ptr()) == (tv_mode in ("fused", "binned"))
     tr.sync()
     assert tr.grid_c().params == grids[0].params.data_ptr()
     losses = torch.zeros(B, dtype=torch.float64, device=dev)
