@@ -62,6 +62,9 @@ def parse():
     ap.add_argument("--optim-stride", type=int, default=0,
                     help="floats per optimiser record (gradient, moments, master); 0: P rounded "
                          "up to 4 (28 for SH-2), which streams no padding")
+    ap.add_argument("--grad-stride", type=int, default=0,
+                    help="floats per gradient record; 0: the FP32 record stride (32 up to 32 "
+                         "parameters), so that the gradient walk flushes whole 128-byte lines")
     ap.add_argument("--bf16-master", type=int, default=1,
                     help="with --storage bf16: Adam updates an FP32 master copy (1) or the "
                          "widened BF16 values themselves (0)")
@@ -253,6 +256,7 @@ class UnevenWorkload:
         self.grid = pxl.Grid(N, N, N, S, self.rec, storage=pxl.STORE_BF16, **grid_kw)
         ostride = args.optim_stride or (P + 3) // 4 * 4
         self.optim = pxl.Optim(self.grid, stride=ostride,
+                               grad_stride=args.grad_stride or pxl.record_stride(S),
                                master_from=zero32[:, :ostride] if args.bf16_master else None)
         del rec32
         self.trainer = pxl.Trainer(self.grid, self.optim, self.rays, self.order, batch_size=B,
@@ -418,18 +422,21 @@ class Workload:
         if args.march == "adaptive":
             grid_kw = dict(grid_kw, march_mode=pxl.MARCH_REF_ADAPTIVE)
         self.rec = torch.zeros_like(gt_rec)
-        # Optimiser records of P floats (gradient, moments, master): the Adam
-        # pass streams no padding.
+        # Moment and master records of P floats (the Adam pass streams no
+        # padding); gradient records of 32 floats, one 128-byte line each,
+        # which the gradient walk's atomic flushes fill whole.
         ostride = args.optim_stride or (self.P + 3) // 4 * 4
         if args.storage == "bf16":
             zero32 = self.rec
             self.rec = pxl.records_to_16bit(zero32, pxl.STORE_BF16)
             self.grid = pxl.Grid(N, N, N, S, self.rec, storage=pxl.STORE_BF16, **grid_kw)
             self.optim = pxl.Optim(self.grid, stride=ostride,
+                                   grad_stride=args.grad_stride or pxl.record_stride(S),
                                    master_from=zero32[:, :ostride] if args.bf16_master else None)
         else:
             self.grid = pxl.Grid(N, N, N, S, self.rec, **grid_kw)
-            self.optim = pxl.Optim(self.grid, stride=ostride)
+            self.optim = pxl.Optim(self.grid, stride=ostride,
+                                   grad_stride=args.grad_stride or pxl.record_stride(S))
         # config_linux.txt defaults: LearningRate 0.3, SHLearningRate 1,
         # VarianceCoefficient 1e-5, ColorDisparityCoefficient 1e-6,
         # VarianceType 1, VarianceLossInterval 3, MaskRayCoefficient 1,
@@ -654,7 +661,8 @@ def fast_path(w, dev, args, plain_ms):
     rec16 = pxl.records_to_16bit(zero32, pxl.STORE_BF16)
     grid = pxl.Grid(N, N, N, S, rec16, storage=pxl.STORE_BF16, **w.grid_kw)
     ostride = args.optim_stride or (w.P + 3) // 4 * 4
-    optim = pxl.Optim(grid, stride=ostride, master_from=zero32[:, :ostride])
+    optim = pxl.Optim(grid, stride=ostride, grad_stride=args.grad_stride or pxl.record_stride(S),
+                      master_from=zero32[:, :ostride])
     del zero32
     tr = pxl.Trainer(grid, optim, w.rays, w.order, batch_size=args.batch, learning_rate=0.3,
                      sh_learning_rate=1.0, variance_coefficient=1e-5, sh_coefficient=1e-6,
@@ -869,6 +877,8 @@ def main():
     out["config"]["storage"] = args.storage
     if args.storage == "bf16" or args.workload == "uneven512":
         out["config"]["fp32_master"] = bool(args.bf16_master)
+    out["config"]["record_floats"] = {"moments": w.optim.stride,
+                                      "gradient": w.optim.grad_stride}
     order, wpb, pers, gmode, gwpb = w.trainer.schedule()
     out["config"]["schedule"] = {"batch_order": ["given", "morton", "length+morton"][order],
                                  "waves_per_block": wpb, "grad_waves_per_block": gwpb,

@@ -25,6 +25,15 @@ bool option_group_marcher();
 bool option_tv_addr64();
 int64_t option_claim_grid();
 
+// Floats per record of the optimiser's moments / master and of its
+// gradient (pxl_optim: 0 = the grid's stride, resp. the moments' stride).
+inline int optim_stride(const pxl_grid *g, const pxl_optim *o) {
+    return o && o->stride > 0 ? o->stride : g->stride;
+}
+inline int grad_stride(const pxl_grid *g, const pxl_optim *o) {
+    return o && o->grad_stride > 0 ? o->grad_stride : optim_stride(g, o);
+}
+
 // Extra arguments of the train entry points (pxl_train_rays_ws / _sched /
 // _prop), and the event the gradient kernels wait for (the trainer runs the
 // variance loss, which writes the same gradient planes, on another stream).

@@ -400,7 +400,7 @@ static int32_t autotune(pxl_trainer *tr, int64_t start, int64_t len) {
             if (best[i] < best[k]) k = i;
         tr->order_mode = cands[k].order;
         tr->sched = cands[k].s;
-        const int stride = tr->optim.stride > 0 ? tr->optim.stride : tr->grid.stride;
+        const int stride = grad_stride(&tr->grid, &tr->optim);
         if (hipMemsetAsync(tr->optim.grad, 0,
                            (size_t)tr->grid.n_vox * (size_t)stride * sizeof(float),
                            tr->stream) != hipSuccess)

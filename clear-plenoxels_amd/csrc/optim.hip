@@ -119,7 +119,7 @@ __device__ __forceinline__ AdamElem adam_load(const TvArgs &a, const AdamArgs &a
     const size_t po = (size_t)r * (size_t)a.stride + (size_t)p;
     const size_t o = (size_t)r * (size_t)ad.ostride + (size_t)p;
     if (p < a.P) {
-        e.g = a.grad[o];
+        e.g = a.grad[(size_t)r * (size_t)a.gstride + (size_t)p];
         e.m = ad.m[o];
         e.v = ad.v[o];
     }
@@ -139,16 +139,17 @@ __device__ __forceinline__ void adam_store(const TvArgs &a, const AdamArgs &ad, 
     const size_t po = (size_t)r * (size_t)a.stride + (size_t)p;
     if (p < a.P) {
         const size_t o = (size_t)r * (size_t)ad.ostride + (size_t)p;
+        const size_t go = (size_t)r * (size_t)a.gstride + (size_t)p;
         if (tvl) e.g += (float)((double)lam * tsum);
         const double lr_d = ad.alr != nullptr ? ad.lr * (double)ad.alr[r] : ad.lr;
         if (adam_elem(p, a.S, a.sh_lo, a.sh_hi, lr_d, ad.sh_lr, ad.bc1, ad.bc2, e.th, e.g, e.m,
                       e.v)) {
-            a.grad[o] = e.g;
+            a.grad[go] = e.g;
             st_stream1(ad.m + o, e.m);
             st_stream1(ad.v + o, e.v);
             if (STORE != PXL_STORE_F32 && ad.master != nullptr) ad.master[o] = e.th;
         } else if (tvl) {
-            a.grad[o] = e.g;
+            a.grad[go] = e.g;
         }
     }
     if (STORE == PXL_STORE_F32)
@@ -916,14 +917,15 @@ __global__ __launch_bounds__(256) void k_adam(float *__restrict__ prm, float *__
                                               float *__restrict__ mom, float *__restrict__ var,
                                               float *__restrict__ master,
                                               const float *__restrict__ alr, int64_t n_vox,
-                                              int pstride, int gstride, int P, int Q, int S,
-                                              int sh_lo, int sh_hi, double lr, double sh_lr,
-                                              double bc1, double bc2) {
+                                              int pstride, int ostride, int gstride, int P,
+                                              int Q, int S, int sh_lo, int sh_hi, double lr,
+                                              double sh_lr, double bc1, double bc2) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_vox * Q) return;
     const int64_t r = i / Q;
     const int q = (int)(i % Q);
-    const size_t off = (size_t)r * gstride + 4 * q;
+    const size_t off = (size_t)r * ostride + 4 * q;   // moments, master
+    const size_t goff = (size_t)r * gstride + 4 * q;  // gradient
     const size_t poff = (size_t)r * pstride + 4 * q;
     float4 th;
     if (STORE == PXL_STORE_F32) {
@@ -940,7 +942,7 @@ __global__ __launch_bounds__(256) void k_adam(float *__restrict__ prm, float *__
         th.z = widen16<STORE>((uint16_t)(h.y & 0xFFFFu));
         th.w = widen16<STORE>((uint16_t)(h.y >> 16));
     }
-    float4 g4 = *reinterpret_cast<const float4 *>(grd + off);
+    float4 g4 = *reinterpret_cast<const float4 *>(grd + goff);
     float4 m4 = *reinterpret_cast<const float4 *>(mom + off);
     float4 v4 = *reinterpret_cast<const float4 *>(var + off);
     float *tp = &th.x, *gp = &g4.x, *mp = &m4.x, *vp = &v4.x;
@@ -965,7 +967,7 @@ __global__ __launch_bounds__(256) void k_adam(float *__restrict__ prm, float *__
         h.y = (uint32_t)narrow16<STORE>(th.z) | ((uint32_t)narrow16<STORE>(th.w) << 16);
         *reinterpret_cast<uint2 *>(reinterpret_cast<uint16_t *>(prm) + poff) = h;
     }
-    *reinterpret_cast<float4 *>(grd + off) = g4;
+    *reinterpret_cast<float4 *>(grd + goff) = g4;
     st_stream(mom + off, m4);
     st_stream(var + off, v4);
 }
@@ -974,19 +976,21 @@ __global__ __launch_bounds__(256) void k_adam(float *__restrict__ prm, float *__
 // Optimizer.cu:11-64, 161-187): element p of every record of grad, m and v
 // becomes 0 when `all`, or when it is the density and `density`, or an SH
 // coefficient k in [lo, hi) of any channel.
+// Element (record r, parameter p < P) per thread; padding is left alone.
 __global__ __launch_bounds__(256) void k_init_band(float *grad, float *m, float *v,
-                                                   int64_t n_vox, int stride, int P, int S,
-                                                   int lo, int hi, int density, int all) {
+                                                   int64_t n_vox, int stride, int gstride, int P,
+                                                   int S, int lo, int hi, int density, int all) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n_vox * stride) return;
-    const int p = (int)(i % stride);
+    if (i >= n_vox * P) return;
+    const int64_t r = i / P;
+    const int p = (int)(i % P);
     bool clear = all != 0;
     if (p == 0) clear = clear || density != 0;
-    else if (p < P) clear = clear || ((p - 1) % S >= lo && (p - 1) % S < hi);
+    else clear = clear || ((p - 1) % S >= lo && (p - 1) % S < hi);
     if (!clear) return;
-    grad[i] = 0.0f;
-    m[i] = 0.0f;
-    v[i] = 0.0f;
+    grad[(size_t)r * gstride + p] = 0.0f;
+    m[(size_t)r * stride + p] = 0.0f;
+    v[(size_t)r * stride + p] = 0.0f;
 }
 
 __global__ __launch_bounds__(1024) void k_loss_reduce(const double *__restrict__ l, int64_t n,
@@ -1027,6 +1031,8 @@ int32_t tv_check(const pxl_grid *g, const pxl_optim *o, const char *who) {
                 "%s: bad stride %d", who, g->stride);
     PXL_REQUIRE(g->storage == PXL_STORE_F32 || o->stride > 0,
                 "%s: 16-bit grids need an explicit optimiser stride", who);
+    PXL_REQUIRE(grad_stride(g, o) >= 1 + 3 * g->sh_dim && grad_stride(g, o) % 4 == 0,
+                "%s: bad gradient stride %d", who, grad_stride(g, o));
     PXL_REQUIRE(g->sparse || g->n_vox == (int64_t)g->W * g->H * g->D,
                 "%s: dense grid needs n_vox = W*H*D", who);
     PXL_REQUIRE(!g->sparse || (g->occupancy && g->index_offsets && g->compact_to_dense),
@@ -1162,7 +1168,7 @@ extern "C" size_t pxl_tv_scratch_bytes(const pxl_grid *grid) {
 static void fill_tv_args(const pxl_grid *grid, const pxl_optim *optim, TvArgs &a) {
     a.params = grid->params;
     a.grad = optim ? optim->grad : nullptr;
-    a.gstride = optim && optim->stride > 0 ? optim->stride : grid->stride;
+    a.gstride = grad_stride(grid, optim);
     a.occ = grid->occupancy;
     a.io = grid->index_offsets;
     a.c2d = grid->compact_to_dense;
@@ -1288,12 +1294,12 @@ extern "C" int32_t pxl_adam_step(const pxl_grid *grid, const pxl_optim *optim, d
                 "pxl_adam_step: unknown storage %d", grid->storage);
     const int P = 1 + 3 * grid->sh_dim;
     const int pstride = grid->stride;
-    const int gstride = optim->stride > 0 ? optim->stride : grid->stride;
+    const int ostride = optim_stride(grid, optim), gstride = grad_stride(grid, optim);
     PXL_REQUIRE(grid->storage == PXL_STORE_F32 || optim->stride > 0,
                 "pxl_adam_step: 16-bit grids need an explicit optimiser stride");
-    PXL_REQUIRE(pstride >= P && pstride % (grid->storage ? 8 : 4) == 0 && gstride >= P &&
-                    gstride % 4 == 0,
-                "pxl_adam_step: bad strides %d / %d", pstride, gstride);
+    PXL_REQUIRE(pstride >= P && pstride % (grid->storage ? 8 : 4) == 0 && ostride >= P &&
+                    ostride % 4 == 0 && gstride >= P && gstride % 4 == 0,
+                "pxl_adam_step: bad strides %d / %d / %d", pstride, ostride, gstride);
     PXL_REQUIRE(((uintptr_t)grid->params | (uintptr_t)optim->grad | (uintptr_t)optim->m |
                  (uintptr_t)optim->v | (uintptr_t)optim->master) % 16 == 0,
                 "pxl_adam_step: parameter and state buffers must be 16-byte aligned");
@@ -1314,7 +1320,8 @@ extern "C" int32_t pxl_adam_step(const pxl_grid *grid, const pxl_optim *optim, d
     hipStream_t s = as_stream(stream);
 #define PXL_ADAM(ST)                                                                            \
     hipLaunchKernelGGL(k_adam<ST>, gdim, dim3(256), 0, s, grid->params, optim->grad, optim->m, \
-                       optim->v, master, alr, grid->n_vox, pstride, gstride, P, Q, grid->sh_dim, \
+                       optim->v, master, alr, grid->n_vox, pstride, ostride, gstride, P, Q,     \
+                       grid->sh_dim,                                                            \
                        optim->sh_first, sh_hi, lr, sh_lr, bc1, bc2)
     switch (grid->storage) {
         case PXL_STORE_F32: PXL_ADAM(PXL_STORE_F32); break;
@@ -1343,9 +1350,9 @@ extern "C" int32_t pxl_tv_adam_step(const pxl_grid *grid, const pxl_optim *optim
                 "pxl_tv_adam_step: params_out overlaps the grid's records");
     PXL_REQUIRE(grid->stride <= (P > 32 ? 64 : 32),
                 "pxl_tv_adam_step: record stride %d above %d", grid->stride, P > 32 ? 64 : 32);
-    const int ostride = optim->stride > 0 ? optim->stride : grid->stride;
-    PXL_REQUIRE(ostride >= P && ostride % 4 == 0, "pxl_tv_adam_step: bad optimiser stride %d",
-                ostride);
+    const int ostride = optim_stride(grid, optim), gs = grad_stride(grid, optim);
+    PXL_REQUIRE(ostride >= P && ostride % 4 == 0 && gs >= P && gs % 4 == 0,
+                "pxl_tv_adam_step: bad optimiser strides %d / %d", ostride, gs);
     if (grid->sparse) {
         const size_t need = pxl_tv_scratch_bytes(grid);
         if (!scratch || scratch_bytes < need || ((uintptr_t)scratch & 255)) {
@@ -1368,7 +1375,7 @@ extern "C" int32_t pxl_tv_adam_step(const pxl_grid *grid, const pxl_optim *optim
     ad.v = optim->v;
     ad.master = grid->storage == PXL_STORE_F32 ? nullptr : optim->master;
     ad.alr = grid->sparse ? nullptr : optim->adaptive_lr;
-    ad.ostride = a.gstride;
+    ad.ostride = ostride;
     ad.q4 = (P + 3) / 4 * 4;
     ad.lr = lr;
     ad.sh_lr = sh_lr;
@@ -1404,16 +1411,18 @@ extern "C" int32_t pxl_optim_init_band(const pxl_grid *grid, const pxl_optim *op
                 "pxl_optim_init_band: band %d beyond the grid's %d coefficients", band,
                 grid->sh_dim);
     const int P = 1 + 3 * grid->sh_dim;
-    const int stride = optim->stride > 0 ? optim->stride : grid->stride;
-    PXL_REQUIRE(stride >= P, "pxl_optim_init_band: stride %d below %d", stride, P);
+    const int stride = optim_stride(grid, optim), gstride = grad_stride(grid, optim);
+    PXL_REQUIRE(stride >= P && gstride >= P, "pxl_optim_init_band: strides %d / %d below %d",
+                stride, gstride, P);
     PXL_REQUIRE(grid->storage == PXL_STORE_F32 || optim->stride > 0,
                 "pxl_optim_init_band: 16-bit grids need an explicit optimiser stride");
-    const int64_t total = grid->n_vox * (int64_t)stride;
+    const int64_t total = grid->n_vox * (int64_t)P;
     if (total == 0) return PXL_OK;
     const int all = band_mode ? 0 : 1;
     const int density = band == 0 || !density_initialized ? 1 : 0;
     hipLaunchKernelGGL(k_init_band, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
-                       as_stream(stream), optim->grad, optim->m, optim->v, grid->n_vox, stride, P,
-                       grid->sh_dim, band * band, (band + 1) * (band + 1), density, all);
+                       as_stream(stream), optim->grad, optim->m, optim->v, grid->n_vox, stride,
+                       gstride, P, grid->sh_dim, band * band, (band + 1) * (band + 1), density,
+                       all);
     return check_launch("pxl_optim_init_band");
 }

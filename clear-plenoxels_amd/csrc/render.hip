@@ -1371,7 +1371,7 @@ static int32_t bwd_common(const pxl_grid *grid, const pxl_optim *optim, const px
                 "%s: sh_first %d outside [0, %d)", who, optim->sh_first, a.sh_active);
     if (len == 0) return PXL_OK;
     a.grad = optim->grad;
-    a.gstride = optim->stride > 0 ? optim->stride : grid->stride;
+    a.gstride = grad_stride(grid, optim);
     a.sh_first = optim->sh_first;
     PXL_REQUIRE(a.gstride >= 1 + 3 * grid->sh_dim && a.gstride % 4 == 0,
                 "%s: gradient stride %d must be a multiple of 4 and >= %d", who, a.gstride,

@@ -125,7 +125,7 @@ class GridC(ctypes.Structure):
 
 class OptimC(ctypes.Structure):
     _fields_ = [("grad", _P), ("m", _P), ("v", _P), ("adaptive_lr", _P), ("master", _P),
-                ("stride", _I32), ("sh_first", _I32)]
+                ("stride", _I32), ("sh_first", _I32), ("grad_stride", _I32), ("_pad", _I32)]
 
 
 class TrainConfigC(ctypes.Structure):
@@ -387,17 +387,24 @@ class Optim:
     default the grid's stride, or 32 for a 16-bit grid; any multiple of 4
     from P up, e.g. 28 for S = 9, which streams no padding).  For a 16-bit
     grid, `master` (FP32 parameters that Adam updates) is created from the
-    given FP32 records when master_from is passed."""
+    given FP32 records when master_from is passed.  The gradient may use its
+    own record stride (`grad_stride`, a multiple of 4 from P up; default
+    `stride`), e.g. 32 so that every gradient record is one 128-byte line
+    while the moments stream 28 floats."""
 
-    def __init__(self, grid, adaptive_lr=None, master_from=None, sh_first=0, stride=None):
+    def __init__(self, grid, adaptive_lr=None, master_from=None, sh_first=0, stride=None,
+                 grad_stride=None):
         import torch
 
         if stride is None:
             stride = grid.stride if grid.storage == STORE_F32 else record_stride(grid.S)
+        if grad_stride is None:
+            grad_stride = stride
         shape = (grid.n_vox, stride)
         dev = grid.params.device
         self.stride = stride
-        self.grad = torch.zeros(shape, dtype=torch.float32, device=dev)
+        self.grad_stride = grad_stride
+        self.grad = torch.zeros((grid.n_vox, grad_stride), dtype=torch.float32, device=dev)
         self.m = torch.zeros(shape, dtype=torch.float32, device=dev)
         self.v = torch.zeros(shape, dtype=torch.float32, device=dev)
         self.adaptive_lr = adaptive_lr
@@ -416,6 +423,7 @@ class Optim:
         o.master = None if self.master is None else self.master.data_ptr()
         o.stride = self.stride
         o.sh_first = self.sh_first
+        o.grad_stride = self.grad_stride
         return o
 
 

@@ -147,13 +147,16 @@ typedef struct pxl_grid {
 /* Optimiser state (Optimizer.cuh:5-37): FP32 records of `stride` floats
  * (0 = the grid's stride, FP32 grids only).  With 16-bit grid storage,
  * `master` (optional) holds the FP32 parameters Adam updates; the grid
- * then receives their rounded copy. */
+ * then receives their rounded copy.  The gradient may use its own record
+ * stride (`grad_stride`, 0 = `stride`): 32 floats per record put each row
+ * the gradient kernels add to the planes in one 128-byte cache line, while
+ * the moments Adam streams stay unpadded. */
 typedef struct pxl_optim {
-    float *grad;                  /* [n_vox][stride] gradient          */
+    float *grad;                  /* [n_vox][grad_stride] gradient     */
     float *m;                     /* [n_vox][stride] Adam first moment */
     float *v;                     /* [n_vox][stride] Adam second moment*/
     const float *adaptive_lr;     /* dense only: per-vertex LR factor or NULL */
-    float *master;                /* 16-bit grids: FP32 parameters or NULL */
+    float *master;                /* 16-bit grids: FP32 parameters [n_vox][stride] or NULL */
     int32_t stride;               /* floats per record, multiple of 4  */
     /* Gradients, variance losses and Adam touch the SH coefficients k in
      * [sh_first, sh_active) of each channel, and the density: the
@@ -161,6 +164,7 @@ typedef struct pxl_optim {
      * optimizingBand^2, RenderUtils.cu:1117-1132, Optimizer.cu:393-412);
      * 0 = from the first. */
     int32_t sh_first;
+    int32_t grad_stride;          /* floats per gradient record, multiple of 4; 0 = stride */
 } pxl_optim;
 
 /* Last error message of the calling thread ("" if none). */

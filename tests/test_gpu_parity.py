@@ -205,7 +205,10 @@ def test_render_fwd_adaptive(c1):
 # marcher, and the two-kernel logged step with a log that holds every ray
 # (512 samples, several 64-sample chunks per ray) or only short rays (24
 # samples: the longer rays go through the fused marcher's redo launch).
-BWD_PATHS = ["separate", "fused", "logged", "logged_24"]
+# The logged step also runs with optimiser records of 28 floats and with
+# gradient records of 36 floats beside moments of 28 (the gradient walk
+# flushes rows at the gradient's own stride).
+BWD_PATHS = ["separate", "fused", "logged", "logged_24", "logged_o28", "logged_g36"]
 
 
 def _bwd_case(c1, grid, grid_o, nr, path, huber=0.1):
@@ -216,11 +219,15 @@ def _bwd_case(c1, grid, grid_o, nr, path, huber=0.1):
     rays = device_rays(c1["rays_np"], dev)
     si = c1["si"][:nr]
     batch = c1["batch"][:nr]
-    optim = pxl.Optim(grid)
+    if path in ("logged_o28", "logged_g36"):
+        optim = pxl.Optim(grid, stride=28, grad_stride=36 if path == "logged_g36" else None)
+        optim.grad.fill_(0.0)
+    else:
+        optim = pxl.Optim(grid)
     losses = torch.zeros(nr, dtype=torch.float64, device=dev)
     rays_o = oracle.Rays(c1["rays_np"])
     if path.startswith("logged"):
-        cap = 512 if path == "logged" else 24
+        cap = 24 if path == "logged_24" else 512
         ws = pxl.train_workspace(nr, cap, dev)
         ws.fill_(0xAB)  # stale contents must not leak into the result
         pxl.train_rays_ws(grid, optim, rays, losses, si, ws, cap, mask_coef=1.0,
@@ -492,17 +499,22 @@ def test_tv_neighbour6_dense(dev):
     assert grad_close(from_records(optim.grad, grid.P), ref) <= 1.0
 
 
-def test_adam_matches_oracle(dev):
+@pytest.mark.parametrize("gstride", [None, 36])
+def test_adam_matches_oracle(dev, gstride):
+    """Adam against the oracle; with a gradient stride of 36 the gradient
+    records are wider than the parameter and moment records (28)."""
     import torch
 
-    say("[adam] 5 steps, 100k records")
+    say(f"[adam] 5 steps, 100k records, gradient stride {gstride or 'default'}")
     rng = np.random.default_rng(11)
     P, n = 28, 100_000
     planes = rng.normal(size=(P, n)).astype(np.float32)
     planes[0] = np.abs(planes[0])
     alr = rng.uniform(0.5, 1.0, n).astype(np.float32)
     grid = pxl.Grid(n, 1, 1, 9, to_records(planes, dev))  # dense: n_vox = W*H*D
-    optim = pxl.Optim(grid, adaptive_lr=torch.from_numpy(alr).to(dev))
+    optim = pxl.Optim(grid, adaptive_lr=torch.from_numpy(alr).to(dev), grad_stride=gstride)
+    if gstride:
+        optim.grad[:, P:] = 7.0  # padding of the gradient records is neither read nor cleared
     m = np.zeros_like(planes)
     v = np.zeros_like(planes)
     p_o = planes.copy()
@@ -519,7 +531,9 @@ def test_adam_matches_oracle(dev):
         i = np.unravel_index(np.argmax(err), err.shape)
         say(f"    {name}: worst rel err {err[i]:.3g} at {i}: gpu {g[i]!r} oracle {exp[i]!r}")
         np.testing.assert_allclose(g, exp, rtol=1e-6, atol=1e-7 if name == "params" else 1e-20)
-    assert float(optim.grad.abs().max()) == 0.0
+    assert float(optim.grad[:, :P].abs().max()) == 0.0
+    if gstride:
+        assert bool((optim.grad[:, P:] == 7.0).all())
 
 
 def test_loss_reduce(dev):

@@ -12,7 +12,8 @@ grid too thin for the z-march, the
 record-0 alias term), S = 16 (64 lanes per record), BF16 with an FP32 master,
 F16 without one, an SH band (sh_first 1, sh_active 4), optimize_step 2 (no
 SH terms), per-vertex adaptive LR, optimiser records of 28 floats (no
-padding streamed) and of the grid's stride.  The separate calls are pinned
+padding streamed) and of the grid's stride, gradient records of another
+stride than the moments (32 or 36 floats beside 28 or 32).  The separate calls are pinned
 to the FP64 oracle by the TV and Adam parity tests.
 """
 import sys
@@ -36,15 +37,17 @@ def _state(grid, optim, seed, dev):
     import torch
 
     gen = torch.Generator(device="cpu").manual_seed(seed)
-    n, w = optim.grad.shape
-    optim.grad.copy_(torch.randn(n, w, generator=gen).mul_(0.05))
+    n, w = optim.m.shape
+    optim.grad.copy_(torch.randn(n, optim.grad_stride, generator=gen).mul_(0.05))
     optim.m.copy_(torch.randn(n, w, generator=gen).mul_(0.01))
     optim.v.copy_(torch.rand(n, w, generator=gen).mul_(1e-3))
 
 
-def _clone_optim(grid, optim, stride):
-    o2 = pxl.Optim(grid, adaptive_lr=optim.adaptive_lr, sh_first=optim.sh_first, stride=stride)
-    o2.grad.copy_(optim.grad)
+def _clone_optim(grid, optim, stride, grad_stride=None):
+    o2 = pxl.Optim(grid, adaptive_lr=optim.adaptive_lr, sh_first=optim.sh_first, stride=stride,
+                   grad_stride=grad_stride)
+    w = min(o2.grad_stride, optim.grad_stride)
+    o2.grad[:, :w].copy_(optim.grad[:, :w])
     o2.m.copy_(optim.m)
     o2.v.copy_(optim.v)
     if optim.master is not None:
@@ -61,8 +64,10 @@ def _same(name, a, b):
     assert bad == 0, f"{name}: {bad} elements differ"
 
 
-def _run(grid, optim, ostride, lam=(0.03, 0.002), opt_step=0):
-    """Two steps, separate vs fused; compares everything after each."""
+def _run(grid, optim, ostride, lam=(0.03, 0.002), opt_step=0, fus_gstride=None):
+    """Two steps, separate vs fused; compares everything after each.  The
+    fused pass may keep its gradient at another record stride
+    (`fus_gstride`) than the separate calls."""
     import torch
 
     g_sep = grid
@@ -72,7 +77,7 @@ def _run(grid, optim, ostride, lam=(0.03, 0.002), opt_step=0):
                      in_cage=grid.in_cage, compact_to_dense=grid.compact_to_dense,
                      brick_occupancy=grid.brick_occupancy, storage=grid.storage,
                      sh_active=grid.sh_active)
-    o_fus = _clone_optim(g_fus, optim, ostride)
+    o_fus = _clone_optim(g_fus, optim, ostride, fus_gstride)
     spare = torch.empty_like(grid.params)
     first = grid.params.clone()
     for t in (1, 2):
@@ -122,6 +127,11 @@ DENSE = {
     # case: (W, H, D, S, storage, options)
     "f32_odd": (21, 19, 33, 9, pxl.STORE_F32, {}),
     "f32_odd_o28": (21, 19, 33, 9, pxl.STORE_F32, {"ostride": 28}),
+    # Gradient records of 32 floats beside moments of 28: in the separate
+    # calls (TV scatter + Adam) or in the fused pass.
+    "f32_odd_o28_g32sep": (21, 19, 33, 9, pxl.STORE_F32, {"ostride": 28, "gsep": 32}),
+    "f32_odd_o28_g32fus": (21, 19, 33, 9, pxl.STORE_F32, {"ostride": 28, "gfus": 32}),
+    "bf16_g36fus": (21, 19, 33, 9, pxl.STORE_BF16, {"gfus": 36}),
     "f32_alr_step2": (21, 19, 33, 9, pxl.STORE_F32, {"alr": True, "opt_step": 2}),
     "f32_thin": (2, 7, 9, 9, pxl.STORE_F32, {}),
     "f32_tall": (5, 4, 300, 9, pxl.STORE_F32, {}),
@@ -142,21 +152,24 @@ def test_tv_adam_dense(dev, case):
     grid, optim = _dense(dev, W, H, D, S, storage=storage, alr=opt.get("alr", False),
                          sh_active=opt.get("sh_active", 0), master=opt.get("master", True))
     optim.sh_first = opt.get("sh_first", 0)
-    if "ostride" in opt:
+    if "ostride" in opt or "gsep" in opt:
         optim = pxl.Optim(grid, adaptive_lr=optim.adaptive_lr, sh_first=optim.sh_first,
-                          stride=opt["ostride"])
+                          stride=opt.get("ostride", optim.stride), grad_stride=opt.get("gsep"),
+                          master_from=None if optim.master is None else optim.master)
     _state(grid, optim, 5, dev)
     say(f"[tv+adam] dense {W}x{H}x{D} S={S} {case}")
     if opt.get("addr64"):
         pxl.set_option(pxl.OPT_TV_ADDR64, 1)
     try:
-        _run(grid, optim, optim.stride, opt_step=opt.get("opt_step", 0))
+        _run(grid, optim, optim.stride, opt_step=opt.get("opt_step", 0),
+             fus_gstride=opt.get("gfus"))
     finally:
         pxl.set_option(pxl.OPT_TV_ADDR64, 0)
     torch.cuda.synchronize()
 
 
-@pytest.mark.parametrize("case", ["f32", "f32_o28", "f32_addr64", "bf16_master", "s16_f32"])
+@pytest.mark.parametrize("case", ["f32", "f32_o28", "f32_o28_g32", "f32_addr64", "bf16_master",
+                                  "s16_f32"])
 def test_tv_adam_caged(dev, case):
     import torch
 
@@ -173,14 +186,14 @@ def test_tv_adam_caged(dev, case):
                         in_cage=grid.in_cage, compact_to_dense=grid.compact_to_dense,
                         brick_occupancy=grid.brick_occupancy, storage=pxl.STORE_BF16)
         optim_master = rec32
-    ostride = 28 if case == "f32_o28" else None
+    ostride = 28 if case.startswith("f32_o28") else None
     optim = pxl.Optim(grid, master_from=optim_master, stride=ostride)
     _state(grid, optim, 9, dev)
     say(f"[tv+adam] caged {N}^3 S={S}, {grid.n_vox} records, {case}")
     if case == "f32_addr64":
         pxl.set_option(pxl.OPT_TV_ADDR64, 1)
     try:
-        _run(grid, optim, optim.stride)
+        _run(grid, optim, optim.stride, fus_gstride=32 if case.endswith("_g32") else None)
     finally:
         pxl.set_option(pxl.OPT_TV_ADDR64, 0)
     torch.cuda.synchronize()

@@ -42,7 +42,7 @@ def test_struct_layouts_match_header(pxl_lib):
     assert ctypes.sizeof(pxl_lib.LevelConfigC) == 40
     assert ctypes.sizeof(pxl_lib.LevelC) == 24
     assert ctypes.sizeof(pxl_lib.ScheduleC) == 16
-    assert ctypes.sizeof(pxl_lib.OptimC) == 48
+    assert ctypes.sizeof(pxl_lib.OptimC) == 56
 
 
 def test_argument_validation_without_device(pxl_lib):
