@@ -733,11 +733,14 @@ bool use_march8() { return option_group_marcher(); }
 template <int S, bool SPARSE, bool ADAPT, int MODE, int STORE>
 int32_t launch3(const RenderArgs &a, hipStream_t st) {
     const unsigned blocks = (unsigned)((a.len + 3) / 4);
-    // The adaptive rule's forward: eight rays per wave.
+    // The adaptive rule's forward: 64 / kAdaptLanes rays per wave, four
+    // waves per block.
     constexpr bool kAdapt8 = ADAPT && S <= 9 && (MODE == kFwd || MODE == kTrace);
     if (kAdapt8 && use_march8()) {
+        constexpr int64_t per_block = 4 * (64 / kAdaptLanes);
         hipLaunchKernelGGL((k_adapt8<(S <= 9 ? S : 9), SPARSE, MODE>),
-                           dim3((unsigned)((a.len + 31) / 32)), dim3(256), 0, st, a);
+                           dim3((unsigned)((a.len + per_block - 1) / per_block)), dim3(256), 0,
+                           st, a);
         return PXL_OK;
     }
     constexpr bool kGroup = !ADAPT && S <= 9 && MODE != kTouch;
@@ -1189,14 +1192,15 @@ void grad_logged(const RenderArgs &a, int wpb, int gwpb, bool persist, int gmode
                        dim3(threads), 0, st, redo);
     launch_rec0_fold(a, 1 + 3 * S, st);
 }
-// The adaptive rule's forward half: k_adapt8 (forward + log, eight rays per
-// wave, wpb waves per block, slots claiming batch entries from a resident
-// grid).
+// The adaptive rule's forward half: k_adapt8 (forward + log, 64 /
+// kAdaptLanes rays per wave, wpb waves per block, slots claiming batch
+// entries from a resident grid).
 template <int S, bool SPARSE>
 void fwd_logged_adapt(const RenderArgs &a, int wpb, hipStream_t st) {
     const int threads = 64 * wpb;
+    const int64_t per_block = (int64_t)wpb * (64 / kAdaptLanes);
     const unsigned blocks =
-        std::min((unsigned)((a.len + 8 * wpb - 1) / (8 * wpb)),
+        std::min((unsigned)((a.len + per_block - 1) / per_block),
                  resident_blocks(k_adapt8<S, SPARSE, kLog>, threads));
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, a.work, kTargetCounter + 1);
     hipLaunchKernelGGL((k_adapt8<S, SPARSE, kLog>), dim3(blocks), dim3(threads), 0, st, a);
