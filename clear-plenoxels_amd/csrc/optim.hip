@@ -976,21 +976,26 @@ __global__ __launch_bounds__(256) void k_adam(float *__restrict__ prm, float *__
 // Optimizer.cu:11-64, 161-187): element p of every record of grad, m and v
 // becomes 0 when `all`, or when it is the density and `density`, or an SH
 // coefficient k in [lo, hi) of any channel.
-// Element (record r, parameter p < P) per thread; padding is left alone.
+// Element (record r, element p < width = max(stride, gstride)) per thread:
+// moments of `stride` floats, gradient of `gstride`; padding (p >= P) is
+// cleared with the whole records only.
 __global__ __launch_bounds__(256) void k_init_band(float *grad, float *m, float *v,
-                                                   int64_t n_vox, int stride, int gstride, int P,
-                                                   int S, int lo, int hi, int density, int all) {
+                                                   int64_t n_vox, int stride, int gstride,
+                                                   int width, int P, int S, int lo, int hi,
+                                                   int density, int all) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n_vox * P) return;
-    const int64_t r = i / P;
-    const int p = (int)(i % P);
+    if (i >= n_vox * width) return;
+    const int64_t r = i / width;
+    const int p = (int)(i % width);
     bool clear = all != 0;
     if (p == 0) clear = clear || density != 0;
-    else clear = clear || ((p - 1) % S >= lo && (p - 1) % S < hi);
+    else if (p < P) clear = clear || ((p - 1) % S >= lo && (p - 1) % S < hi);
     if (!clear) return;
-    grad[(size_t)r * gstride + p] = 0.0f;
-    m[(size_t)r * stride + p] = 0.0f;
-    v[(size_t)r * stride + p] = 0.0f;
+    if (p < gstride) grad[(size_t)r * gstride + p] = 0.0f;
+    if (p < stride) {
+        m[(size_t)r * stride + p] = 0.0f;
+        v[(size_t)r * stride + p] = 0.0f;
+    }
 }
 
 __global__ __launch_bounds__(1024) void k_loss_reduce(const double *__restrict__ l, int64_t n,
@@ -1416,13 +1421,14 @@ extern "C" int32_t pxl_optim_init_band(const pxl_grid *grid, const pxl_optim *op
                 stride, gstride, P);
     PXL_REQUIRE(grid->storage == PXL_STORE_F32 || optim->stride > 0,
                 "pxl_optim_init_band: 16-bit grids need an explicit optimiser stride");
-    const int64_t total = grid->n_vox * (int64_t)P;
+    const int width = stride > gstride ? stride : gstride;
+    const int64_t total = grid->n_vox * (int64_t)width;
     if (total == 0) return PXL_OK;
     const int all = band_mode ? 0 : 1;
     const int density = band == 0 || !density_initialized ? 1 : 0;
     hipLaunchKernelGGL(k_init_band, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
                        as_stream(stream), optim->grad, optim->m, optim->v, grid->n_vox, stride,
-                       gstride, P, grid->sh_dim, band * band, (band + 1) * (band + 1), density,
-                       all);
+                       gstride, width, P, grid->sh_dim, band * band, (band + 1) * (band + 1),
+                       density, all);
     return check_launch("pxl_optim_init_band");
 }

@@ -225,6 +225,33 @@ def test_adam_band_and_init(dev):
     assert float(optim.v.abs().max()) == 0.0
 
 
+def test_init_band_two_strides(dev):
+    """pxl_optim_init_band with gradient records (36 floats) wider than the
+    moment records (28): each tensor is cleared at its own stride, the band's
+    columns only in band mode, whole records (padding included) otherwise."""
+    import torch
+
+    S, n = 9, 5000
+    P = 1 + 3 * S
+    planes = np.random.default_rng(5).normal(size=(P, n)).astype(np.float32)
+    grid = pxl.Grid(n, 1, 1, S, to_records(planes, dev), sh_active=9)
+    optim = pxl.Optim(grid, stride=28, grad_stride=36)
+    for t_ in (optim.grad, optim.m, optim.v):
+        t_.fill_(1.0)
+    pxl.optim_init_band(grid, optim, 1, band_mode=True, density_initialized=True)
+    torch.cuda.synchronize()
+    for t_ in (optim.grad, optim.m, optim.v):
+        a = t_.cpu().numpy()
+        cleared = np.zeros(a.shape[1], bool)
+        for ch in range(3):
+            cleared[1 + ch * S + 1:1 + ch * S + 4] = True
+        assert np.all(a[:, cleared] == 0.0) and np.all(a[:, ~cleared] == 1.0)
+    pxl.optim_init_band(grid, optim, 1, band_mode=False)
+    torch.cuda.synchronize()
+    for t_ in (optim.grad, optim.m, optim.v):
+        assert float(t_.abs().max()) == 0.0
+
+
 def test_learn_level_matches_stage_calls(sc):
     """pxl_trainer_learn_level with SHsForTargetResolutionOnly above the
     minimum hierarchy: bands 0, 1, 2 in turn, each from a cleared band state
