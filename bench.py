@@ -60,7 +60,7 @@ def parse():
                          "(untimed) batch and keeps the fastest; 0: given order, 4 waves per "
                          "block")
     ap.add_argument("--optim-stride", type=int, default=0,
-                    help="floats per optimiser record (gradient, moments, master); 0: P rounded "
+                    help="floats per moment and master record; 0: P rounded "
                          "up to 4 (28 for SH-2), which streams no padding")
     ap.add_argument("--grad-stride", type=int, default=0,
                     help="floats per gradient record; 0: the FP32 record stride (32 up to 32 "
@@ -111,6 +111,9 @@ def parse():
     a.occupancy = a.occupancy if a.occupancy is not None else (0.01 if uneven else 0.10)
     if uneven and a.log_samples == 512:
         a.log_samples = 1024
+    if a.log_samples == 0:
+        # The single fused kernel has one fixed schedule: nothing to tune.
+        a.autotune = 0
     if a.pretrain is None:
         a.pretrain = 2000 if a.proposal else 0
     if a.fast_path is None:

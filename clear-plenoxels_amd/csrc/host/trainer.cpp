@@ -5,6 +5,7 @@
 #include <cstdlib>
 #include <cstdio>
 #include <cstring>
+#include <string>
 #include <new>
 #include <vector>
 
@@ -116,6 +117,15 @@ struct pxl_trainer {
 using namespace pxl;
 using namespace pxl::host;
 
+// Releases a trainer whose creation failed and keeps the error text that
+// says why (the release itself clears the error state).
+static pxl_trainer *create_failed(pxl_trainer *tr) {
+    const std::string why = pxl_last_error();
+    pxl_trainer_destroy(tr);
+    set_error("%s", why.c_str());
+    return nullptr;
+}
+
 extern "C" pxl_trainer *pxl_trainer_create(const pxl_train_config *cfg, const pxl_grid *grid,
                                            const pxl_optim *optim, const pxl_rays *rays,
                                            const int32_t *order, int64_t n_rays,
@@ -171,8 +181,7 @@ extern "C" pxl_trainer *pxl_trainer_create(const pxl_train_config *cfg, const px
         set_error("pxl_trainer_create: cannot allocate a second %zu-byte record buffer for the "
                   "fused TV + Adam pass (tv_separate = 1 runs without it)",
                   tr->params_bytes);
-        pxl_trainer_destroy(tr);
-        return nullptr;
+        return create_failed(tr);
     }
     tr->tv_overlap = tr->side != nullptr && !cfg->tv_inline && !tr->tv_fused;
     tr->tv_scratch = tv_scratch;
@@ -205,18 +214,15 @@ extern "C" pxl_trainer *pxl_trainer_create(const pxl_train_config *cfg, const px
                     (cfg->waves_per_block && cfg->waves_per_block != 4))) {
         set_error("pxl_trainer_create: batch orders and schedules need the logged step "
                   "(fused with log_samples > 0)");
-        pxl_trainer_destroy(tr);
-        return nullptr;
+        return create_failed(tr);
     }
     if (cfg->gradient == PXL_GRAD_CHUNKED && cfg->persistent) {
         set_error("pxl_trainer_create: the chunk-parallel gradient has no persistent form");
-        pxl_trainer_destroy(tr);
-        return nullptr;
+        return create_failed(tr);
     }
     if (cfg->gradient < PXL_GRAD_PER_RAY || cfg->gradient > PXL_GRAD_BINNED) {
         set_error("pxl_trainer_create: unknown gradient schedule %d", cfg->gradient);
-        pxl_trainer_destroy(tr);
-        return nullptr;
+        return create_failed(tr);
     }
     if (logged) {
         // Sized for every gradient schedule, the brick-binned one included
@@ -228,16 +234,14 @@ extern "C" pxl_trainer *pxl_trainer_create(const pxl_train_config *cfg, const px
         if (cfg->gradient == PXL_GRAD_BINNED && !tr->binned_ok) {
             set_error("pxl_trainer_create: the brick-binned gradient needs 2 to 1025 vertices "
                       "per side");
-            pxl_trainer_destroy(tr);
-            return nullptr;
+            return create_failed(tr);
         }
         if (tr->march_ws_bytes == 0 ||
             hipMalloc(&tr->march_ws, tr->march_ws_bytes) != hipSuccess) {
             set_error("pxl_trainer_create: cannot allocate a %zu-byte sample log for %d samples "
                       "per ray",
                       tr->march_ws_bytes, cfg->log_samples);
-            pxl_trainer_destroy(tr);
-            return nullptr;
+            return create_failed(tr);
         }
         if (cfg->autotune || cfg->batch_order) {
             tr->order_ws_bytes = pxl_batch_order_scratch_bytes(cfg->batch_size);
@@ -245,8 +249,7 @@ extern "C" pxl_trainer *pxl_trainer_create(const pxl_train_config *cfg, const px
                     hipSuccess ||
                 hipMalloc(&tr->order_ws, tr->order_ws_bytes) != hipSuccess) {
                 set_error("pxl_trainer_create: cannot allocate the batch-order buffers");
-                pxl_trainer_destroy(tr);
-                return nullptr;
+                return create_failed(tr);
             }
         }
     }

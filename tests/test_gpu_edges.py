@@ -153,3 +153,22 @@ def test_all_miss_and_empty_batches(scene, path):
     pxl.render_fwd(grid, rays, empty)
     pxl.batch_order(grid, rays, empty, torch.zeros(0, dtype=torch.int32, device=dev))
     torch.cuda.synchronize()
+
+
+def test_trainer_create_reports_why(scene):
+    """A trainer that cannot be built says why (the release of the partly
+    built trainer keeps the error text): schedules without the logged step,
+    the chunk-parallel gradient in persistent form."""
+    import torch
+
+    grid, _ = _grids(scene, "sparse")
+    dev = scene["dev"]
+    rays = pxl.Rays.from_arrays(scene["rays_np"], device=dev)
+    order = torch.arange(256, dtype=torch.int32, device=dev)
+    with pytest.raises(pxl.PxlError, match="need the logged step"):
+        pxl.Trainer(grid, pxl.Optim(grid), rays, order, batch_size=128, log_samples=0,
+                    autotune=True)
+    with pytest.raises(pxl.PxlError, match="no persistent form"):
+        pxl.Trainer(grid, pxl.Optim(grid), rays, order, batch_size=128, log_samples=64,
+                    persistent=True, chunked=True)
+    torch.cuda.synchronize()
