@@ -460,6 +460,8 @@ __device__ __forceinline__ void march8_ray(const RenderArgs &a, const int64_t b)
     const double tnear = a.rays.near_t[ray], tfar = a.rays.far_t[ray];
     const double step = (double)a.step;
     const float fstep = a.step;
+    // Positions per coarse probe step: (cstride - 1) * step < 8 cells.
+    const int cstride = step > 0.0 && step < 8.0 ? (int)(8.0 / step) : 1;
 
     if (tnear > tfar) {
         if (do_fwd && lane == 0) {
@@ -537,9 +539,40 @@ __device__ __forceinline__ void march8_ray(const RenderArgs &a, const int64_t b)
         int k0 = 0;
         uint64_t pending = 0;  // occupied positions of [k0, k0 + 64) not yet done
         bool more = true;      // positions beyond k0 + 64 may remain
+        // At the ray's start and after a probe that found nothing, a coarse
+        // probe runs first: lane l tests position k0 + cstride * l against
+        // the dilated brick map.  The positions from there to the next
+        // coarse one lie within 8 cells, so inside that position's brick or
+        // a neighbour of it: a clear dilated bit clears them all, and the
+        // fine probe starts at the first coarse position whose bit is set.
+        bool coarse = SPARSE && a.dbricks != nullptr;
         while (true) {
             if (pending == 0) {
                 if (!more) break;
+                if (SPARSE && coarse) {
+                    const int kc = k0 + cstride * lane;
+                    const double tc = fma((double)kc, step, tnear);
+                    const bool cvalid = tc < tfar && kc < kMaxSteps;
+                    bool hit = false;
+                    if (cvalid) {
+                        const int cx = clampi(base_of(fma(tc, dx, ox)), 0, a.W - 2),
+                                  cy = clampi(base_of(fma(tc, dy, oy)), 0, a.H - 2),
+                                  cz = clampi(base_of(fma(tc, dz, oz)), 0, a.D - 2);
+                        const uint32_t br = (uint32_t)(cz >> 3) * (uint32_t)a.nbx * (uint32_t)a.nby +
+                                            (uint32_t)(cy >> 3) * (uint32_t)a.nbx + (uint32_t)(cx >> 3);
+                        hit = (a.dbricks[br >> 5] >> (br & 31)) & 1u;
+                    }
+                    const uint64_t hits = __ballot(hit);
+                    if (hits == 0ull) {
+                        // Every position up to the ray's end is empty, or on
+                        // to the next 64 coarse positions.
+                        if (__ballot(cvalid) != ~0ull) break;
+                        k0 += 64 * cstride;
+                        continue;
+                    }
+                    k0 += cstride * __builtin_ctzll(hits);
+                    coarse = false;
+                }
                 // Probe 64 positions, one per lane.
                 const int k = k0 + lane;
                 const double t = fma((double)k, step, tnear);
@@ -566,6 +599,7 @@ __device__ __forceinline__ void march8_ray(const RenderArgs &a, const int64_t b)
                 more = __ballot(valid) == ~0ull;
                 if (pending == 0) {
                     k0 += 64;
+                    coarse = SPARSE && a.dbricks != nullptr;
                     continue;
                 }
             }

@@ -40,6 +40,7 @@ struct RenderArgs {
     int64_t n_vox;
     const uint32_t *occ;
     const uint32_t *bricks;
+    const uint32_t *dbricks;  // the brick map dilated by one brick (after the map)
     int nbx, nby;
     const uint32_t *io;
     int W, H, D, stride;
@@ -109,6 +110,13 @@ struct RenderArgs {
     uint8_t *img_rgb8, *img_mask;
     float *img_disp;
 };
+
+// The dilated half of a brick map written by pxl_build_brick_occupancy.
+inline const uint32_t *dilated_bricks(const uint32_t *bricks, const pxl_grid *g) {
+    if (bricks == nullptr) return nullptr;
+    const int64_t nb = (int64_t)((g->W - 1 + 7) / 8) * ((g->H - 1 + 7) / 8) * ((g->D - 1 + 7) / 8);
+    return bricks + (nb + 31) / 32;
+}
 
 // Eval ray of pixel (row, col) (getBound, RenderUtils.cu:34-97): the pixel
 // centre through the camera, c2w x - centre, normalised, all FP64 with the
@@ -839,6 +847,7 @@ int32_t prepare(const pxl_grid *g, const pxl_rays *r, const int32_t *sample_inde
     a.bricks = g->sparse ? g->brick_occupancy : nullptr;
     a.nbx = (g->W - 1 + 7) / 8;
     a.nby = (g->H - 1 + 7) / 8;
+    a.dbricks = dilated_bricks(a.bricks, g);
     a.io = g->index_offsets;
     a.W = g->W;
     a.H = g->H;
@@ -915,6 +924,7 @@ extern "C" int32_t pxl_render_image(const pxl_grid *grid, const pxl_camera *cam,
     a.bricks = grid->sparse ? grid->brick_occupancy : nullptr;
     a.nbx = (grid->W - 1 + 7) / 8;
     a.nby = (grid->H - 1 + 7) / 8;
+    a.dbricks = dilated_bricks(a.bricks, grid);
     a.io = grid->index_offsets;
     a.W = grid->W;
     a.H = grid->H;

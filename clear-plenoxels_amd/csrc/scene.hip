@@ -78,6 +78,36 @@ __global__ __launch_bounds__(256) void k_bricks(const uint32_t *__restrict__ occ
     }
 }
 
+// The brick map dilated by one brick: bit b set when brick b or any of its
+// 26 neighbours (within the grid) has an occupied cell.  A ray segment
+// shorter than a brick side that starts in brick b stays inside b and its
+// neighbours, so a clear dilated bit clears the whole segment.
+__global__ __launch_bounds__(256) void k_bricks_dilate(const uint32_t *__restrict__ bricks,
+                                                       int nbx, int nby, int nbz,
+                                                       int64_t nbricks, int64_t nwords,
+                                                       uint32_t *__restrict__ dilated) {
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool any = false;
+    if (b < nbricks) {
+        const int bx = (int)(b % nbx), by = (int)((b / nbx) % nby), bz = (int)(b / ((int64_t)nbx * nby));
+        for (int z = max(bz - 1, 0); z <= min(bz + 1, nbz - 1) && !any; z++)
+            for (int y = max(by - 1, 0); y <= min(by + 1, nby - 1) && !any; y++)
+                for (int x = max(bx - 1, 0); x <= min(bx + 1, nbx - 1); x++) {
+                    const int64_t n = ((int64_t)z * nby + y) * nbx + x;
+                    if ((bricks[n >> 5] >> (n & 31)) & 1u) {
+                        any = true;
+                        break;
+                    }
+                }
+    }
+    const uint64_t bal = __ballot(any);
+    if ((threadIdx.x & 63) == 0) {
+        const int64_t w0 = b >> 5;
+        if (w0 < nwords) dilated[w0] = (uint32_t)bal;
+        if (w0 + 1 < nwords) dilated[w0 + 1] = (uint32_t)(bal >> 32);
+    }
+}
+
 constexpr int kScanThreads = 256;
 constexpr int kScanPerThread = 16;
 constexpr int64_t kScanTile = (int64_t)kScanThreads * kScanPerThread;
@@ -266,7 +296,7 @@ extern "C" int32_t pxl_build_occupancy(const float *density, int32_t W, int32_t 
 extern "C" int64_t pxl_brick_words(int32_t W, int32_t H, int32_t D) {
     if (W < 2 || H < 2 || D < 2) return 0;
     const int64_t n = (int64_t)((W - 1 + 7) / 8) * ((H - 1 + 7) / 8) * ((D - 1 + 7) / 8);
-    return (n + 31) / 32;
+    return 2 * ((n + 31) / 32);
 }
 
 extern "C" int32_t pxl_build_brick_occupancy(const uint32_t *occupancy, int32_t W, int32_t H,
@@ -279,6 +309,8 @@ extern "C" int32_t pxl_build_brick_occupancy(const uint32_t *occupancy, int32_t 
     const int64_t nwords = (nb + 31) / 32;
     hipLaunchKernelGGL(k_bricks, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0,
                        as_stream(stream), occupancy, W, H, D, nbx, nby, nb, nwords, bricks);
+    hipLaunchKernelGGL(k_bricks_dilate, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0,
+                       as_stream(stream), bricks, nbx, nby, nbz, nb, nwords, bricks + nwords);
     return check_launch("pxl_build_brick_occupancy");
 }
 

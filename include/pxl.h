@@ -118,9 +118,10 @@ typedef struct pxl_rays {
  * records, `index_offsets` maps a dense vertex to its record,
  * `occupancy` is one bit per cell, `in_cage` one byte per dense vertex and
  * `compact_to_dense` maps a record back to its dense vertex.
- * `brick_occupancy` (optional, sparse only, from pxl_build_brick_occupancy)
- * has one bit per brick of 8x8x8 cells, set when any cell in it is
- * occupied; marching then skips empty bricks without reading `occupancy`. */
+ * `brick_occupancy` (optional, sparse only, as written by
+ * pxl_build_brick_occupancy) has one bit per brick of 8x8x8 cells, set when
+ * any cell in it is occupied, followed by its dilation; marching then skips
+ * empty bricks without reading `occupancy`. */
 typedef struct pxl_grid {
     int32_t W, H, D;              /* vertices per axis                 */
     int32_t sh_dim;               /* S = (band+1)^2: 1, 4, 9 or 16     */
@@ -353,7 +354,11 @@ int32_t pxl_build_occupancy(const float *density, int32_t W, int32_t H,
 
 /* Brick map of a cell bitfield: bit b of brick (bx, by, bz) =
  * bz*nbx*nby + by*nbx + bx, nbx = ceil((W-1)/8) (likewise y, z), set when
- * any of its cells is occupied.  bricks: pxl_brick_words(W, H, D) words. */
+ * any of its cells is occupied.  bricks: pxl_brick_words(W, H, D) words =
+ * 2 * ceil(nbx*nby*nbz / 32): the map, then the map dilated by one brick
+ * (bit b set when brick b or one of its 26 neighbours has an occupied
+ * cell), from which the fixed-rule marcher skips empty space 8 cells per
+ * position probed. */
 int64_t pxl_brick_words(int32_t W, int32_t H, int32_t D);
 int32_t pxl_build_brick_occupancy(const uint32_t *occupancy, int32_t W, int32_t H,
                                   int32_t D, uint32_t *bricks, void *stream);

@@ -90,15 +90,25 @@ def device_sparse_grid(N, S, compact, occ, io, cage, c2d, dev, step=0.5, march=0
                     brick_occupancy=br)
 
 
-def brick_bits(occ_bytes, W, H, D):
-    """Reference brick map from cell bytes: any occupied cell per 8^3 brick."""
+def brick_bits(occ_bytes, W, H, D, dilated=False):
+    """Reference brick map from cell bytes: any occupied cell per 8^3 brick;
+    dilated: any occupied cell in the brick or one of its 26 neighbours."""
     cw, ch, cd = W - 1, H - 1, D - 1
     c = occ_bytes.reshape(cd, ch, cw) != 0
     nbx, nby, nbz = -(-cw // 8), -(-ch // 8), -(-cd // 8)
     pad = np.zeros((nbz * 8, nby * 8, nbx * 8), bool)
     pad[:cd, :ch, :cw] = c
-    b = pad.reshape(nbz, 8, nby, 8, nbx, 8).any(axis=(1, 3, 5)).reshape(-1)
-    return pack_bits(b.astype(np.uint8))
+    b = pad.reshape(nbz, 8, nby, 8, nbx, 8).any(axis=(1, 3, 5))
+    if dilated:
+        p = np.zeros((nbz + 2, nby + 2, nbx + 2), bool)
+        p[1:-1, 1:-1, 1:-1] = b
+        d = np.zeros_like(b)
+        for z in range(3):
+            for y in range(3):
+                for x in range(3):
+                    d |= p[z:z + nbz, y:y + nby, x:x + nbx]
+        b = d
+    return pack_bits(b.reshape(-1).astype(np.uint8))
 
 
 def grad_close_alias(g, ref, rel=1e-4, floor=1e-3, rel0=1e-3):

@@ -126,7 +126,11 @@ def test_occupancy_and_index_bit_exact(dev, c1, case):
     br = pxl.build_brick_occupancy(occ, W, H, D)
     torch.cuda.synchronize()
     exp = brick_bits(occ_o, W, H, D)
-    assert np.array_equal(br.cpu().numpy().view(np.uint32)[:len(exp)], exp)
+    got = br.cpu().numpy().view(np.uint32)
+    assert len(got) == 2 * len(exp) == pxl.lib().pxl_brick_words(W, H, D)
+    assert np.array_equal(got[:len(exp)], exp)
+    # The second half: the map dilated by one brick.
+    assert np.array_equal(got[len(exp):], brick_bits(occ_o, W, H, D, dilated=True))
 
 
 def _fwd_compare(grid_o, rays_np, batch, got, trace_gpu, white=True):
