@@ -583,16 +583,12 @@ __device__ __forceinline__ void march8_ray(const RenderArgs &a, const int64_t b)
                               bz = base_of(fma(t, dz, oz));
                     const int cx = clampi(bx, 0, a.W - 2), cy = clampi(by, 0, a.H - 2),
                               cz = clampi(bz, 0, a.D - 2);
-                    if (a.bricks != nullptr) {
-                        const uint32_t br = (uint32_t)(cz >> 3) * (uint32_t)a.nbx * (uint32_t)a.nby +
-                                            (uint32_t)(cy >> 3) * (uint32_t)a.nbx + (uint32_t)(cx >> 3);
-                        occ = (a.bricks[br >> 5] >> (br & 31)) & 1u;
-                    }
-                    if (occ) {
-                        const uint32_t cell = (uint32_t)cz * (uint32_t)(a.W - 1) * (uint32_t)(a.H - 1) +
-                                              (uint32_t)cy * (uint32_t)(a.W - 1) + (uint32_t)cx;
-                        occ = (a.occ[cell >> 5] >> (cell & 31)) & 1u;
-                    }
+                    // The cell bit alone: the coarse probe has already
+                    // stepped over the empty bricks, and one load here keeps
+                    // a probe to one round trip.
+                    const uint32_t cell = (uint32_t)cz * (uint32_t)(a.W - 1) * (uint32_t)(a.H - 1) +
+                                          (uint32_t)cy * (uint32_t)(a.W - 1) + (uint32_t)cx;
+                    occ = (a.occ[cell >> 5] >> (cell & 31)) & 1u;
                 }
                 if (bmask != ~0ull && occ) occ = (bmask >> bin_of(k, step, tnear, tfar)) & 1ull;
                 pending = __ballot(occ);
