@@ -194,6 +194,23 @@ def test_render_fwd_sparse(c1, sparse1, bricks, marcher):
     _fwd_compare(grid_o, c1["rays_np"], c1["batch"], got, tr)
 
 
+@pytest.mark.parametrize("step", [0.3, 1.0, 2.5])
+def test_render_fwd_sparse_steps(c1, sparse1, step):
+    """Other step sizes on the caged grid with the brick map: the coarse
+    probe's stride (positions per 8 cells: 26, 8, 3) changes with the step,
+    the evaluated samples must not."""
+    s = sparse1
+    say(f"[fwd] sparse 128^3, step {step}, brick map on")
+    dev = c1["dev"]
+    grid = device_sparse_grid(N1, S1, s["compact"], s["occ"], s["io"], s["cage"], s["c2d"], dev,
+                              step=step)
+    rays = device_rays(c1["rays_np"], dev)
+    got, tr = _gpu_fwd(grid, rays, c1["si"], dev)
+    grid_o = oracle.Grid(N1, N1, N1, S1, s["compact"], step, occupancy=s["occ"],
+                         index_offsets=s["io"])
+    _fwd_compare(grid_o, c1["rays_np"], c1["batch"], got, tr)
+
+
 def test_render_fwd_adaptive(c1):
     say("[fwd] dense 128^3 adaptive step, 1024 rays")
     dev = c1["dev"]
