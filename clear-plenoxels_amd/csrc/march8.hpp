@@ -405,8 +405,12 @@ constexpr int kRedoCounter = 16;
 // One wave per ray: either the block's waves take consecutive batch entries
 // of an XCD-sliced block order, or (a.persist) a fixed grid of waves claims
 // entries until the batch is done.
+// The forward + log kernel waits on gathers: five waves per SIMD (at most
+// 96 VGPRs, met without spills) keep more rays in flight; the backward
+// modes keep their registers.
 template <int S, bool SPARSE, int MODE, int STORE, bool PERSIST = false>
-__global__ __launch_bounds__(256) void k_march8(const RenderArgs a) {
+__global__ __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(MODE == kLog ? 5 : 1))) void k_march8(const RenderArgs a) {
     if (PERSIST) {
         int slice = blockIdx.x & 7, visited = 0;
         for (int64_t b = claim_ray(a.work, a.len, slice, visited); b < a.len;
