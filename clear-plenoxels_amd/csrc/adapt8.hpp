@@ -40,9 +40,9 @@ constexpr int kProbePerLane = 1;
 // Sum over the L lanes of a ray slot.
 template <int L>
 __device__ __forceinline__ double slot_sum_d(double v) {
-    v += shfl_xor64(v, 1);
-    v += shfl_xor64(v, 2);
-    if (L == 8) v += shfl_xor64(v, 4);
+    v += dpp_d<kDppXor1>(v);
+    v += dpp_d<kDppXor2>(v);
+    if (L == 8) v += dpp_d<kDppMirror8>(v);
     return v;
 }
 

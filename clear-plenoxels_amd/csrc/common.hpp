@@ -111,6 +111,48 @@ __device__ __forceinline__ float xor32_f(float v) {
     return (threadIdx.x & 32) ? lo : hi;
 }
 
+// Lane moves by DPP (VALU, no LDS round trip) that pair lane i with lane
+// i ^ 1, i ^ 2 (quad permutes), 7 - i within each 8 (row_half_mirror: a
+// lane of the other quad of the 8) and i ^ 8 (row_ror:8 in 16-lane rows);
+// gfx950's v_permlane16_swap / v_permlane32_swap pair i with i ^ 16 and
+// i ^ 32.  Butterfly sums built on them add the same pairs as the xor
+// shuffles, so every lane holds the same bits.
+constexpr int kDppXor1 = 0xB1;     // quad_perm [1, 0, 3, 2]
+constexpr int kDppXor2 = 0x4E;     // quad_perm [2, 3, 0, 1]
+constexpr int kDppMirror8 = 0x141; // row_half_mirror
+constexpr int kDppXor8 = 0x128;    // row_ror:8
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v) {
+    const uint64_t u = (uint64_t)__double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(
+        (long long)(((uint64_t)(uint32_t)hi << 32) | (uint64_t)(uint32_t)lo));
+}
+// v of lane L ^ 16 and L ^ 32.
+__device__ __forceinline__ uint32_t xor16_u(uint32_t v) {
+    const auto r = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+    return (threadIdx.x & 16) ? r[0] : r[1];
+}
+__device__ __forceinline__ uint32_t xor32_u(uint32_t v) {
+    const auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+    return (threadIdx.x & 32) ? r[0] : r[1];
+}
+__device__ __forceinline__ double xor16_d(double v) {
+    const uint64_t u = (uint64_t)__double_as_longlong(v);
+    const uint64_t r = ((uint64_t)xor16_u((uint32_t)(u >> 32)) << 32) | xor16_u((uint32_t)u);
+    return __longlong_as_double((long long)r);
+}
+__device__ __forceinline__ double xor32_d(double v) {
+    const uint64_t u = (uint64_t)__double_as_longlong(v);
+    const uint64_t r = ((uint64_t)xor32_u((uint32_t)(u >> 32)) << 32) | xor32_u((uint32_t)u);
+    return __longlong_as_double((long long)r);
+}
+
 // DPP row rotate (16-lane rows): ctrl 0x120 + n = row_ror:n.
 template <int N>
 __device__ __forceinline__ float row_ror_f(float v) {

@@ -40,16 +40,16 @@ __device__ __forceinline__ T shfl_xor64(T v, int off) {
 
 // Sum over the 8 corner lanes of a slot (lanes 8s .. 8s+7); all get it.
 __device__ __forceinline__ float corner_sum(float v) {
-    v += shfl_xor64(v, 1);
-    v += shfl_xor64(v, 2);
-    v += shfl_xor64(v, 4);
+    v += dpp_f<kDppXor1>(v);
+    v += dpp_f<kDppXor2>(v);
+    v += dpp_f<kDppMirror8>(v);
     return v;
 }
 // Sum over the 8 slots of values replicated across each slot's lanes.
 __device__ __forceinline__ double slot_sum(double v) {
-    v += shfl_xor64(v, 8);
-    v += shfl_xor64(v, 16);
-    v += shfl_xor64(v, 32);
+    v += dpp_d<kDppXor8>(v);
+    v += xor16_d(v);
+    v += xor32_d(v);
     return v;
 }
 // Inclusive scans over slots (stride 8 lanes) in FP64.
