@@ -137,15 +137,14 @@ __device__ __forceinline__ void bin_samples_ray(const RenderArgs &a, const int64
         const size_t row = (size_t)b * (size_t)a.log_cap + (size_t)(c0 + (act ? lane : 0));
         const LoggedSample<ADAPT> v = read_logged<ADAPT>(a, row, act, step, tnear);
         const double keep = 1.0 - (double)v.alpha;
-        const double incl = wave_scan_mul64(keep, lane);
-        double excl = __shfl_up(incl, 1, 64);
-        if (lane == 0) excl = 1.0;
+        const double incl = wave_scan_mul64(keep);
+        const double excl = wave_shr1_d(incl, 1.0);
         const double Ts = T * excl;
         const double ws = Ts * (double)v.alpha;
-        const double Ai = Acc + wave_scan_add64(ws, lane);
-        const double H0 = Cr + wave_scan_add64(ws * (double)v.col0, lane);
-        const double H1 = Cg + wave_scan_add64(ws * (double)v.col1, lane);
-        const double H2 = Cb + wave_scan_add64(ws * (double)v.col2, lane);
+        const double Ai = Acc + wave_scan_add64(ws);
+        const double H0 = Cr + wave_scan_add64(ws * (double)v.col0);
+        const double H1 = Cg + wave_scan_add64(ws * (double)v.col1);
+        const double H2 = Cb + wave_scan_add64(ws * (double)v.col2);
         const double Tn = Ts * keep;
         float g0, g1, g2;
         double gs;

@@ -133,6 +133,48 @@ __device__ __forceinline__ double dpp_d(double v) {
     return __longlong_as_double(
         (long long)(((uint64_t)(uint32_t)hi << 32) | (uint64_t)(uint32_t)lo));
 }
+// DPP move whose lanes without a source in range (or in rows outside ROWS)
+// receive `old`: row_shr:n = 0x110 + n (within 16-lane rows), wave_shr:1 =
+// 0x138, row_bcast15 = 0x142 (lane 15 of each row to the next row),
+// row_bcast31 = 0x143 (lane 31 to rows 2 and 3).
+template <int CTRL, int ROWS = 0xF>
+__device__ __forceinline__ double dpp_or_d(double v, double old) {
+    const uint64_t u = (uint64_t)__double_as_longlong(v), o = (uint64_t)__double_as_longlong(old);
+    const int lo = __builtin_amdgcn_update_dpp((int)(uint32_t)o, (int)(uint32_t)u, CTRL, ROWS, 0xF,
+                                               false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(uint32_t)(o >> 32), (int)(uint32_t)(u >> 32),
+                                               CTRL, ROWS, 0xF, false);
+    return __longlong_as_double(
+        (long long)(((uint64_t)(uint32_t)hi << 32) | (uint64_t)(uint32_t)lo));
+}
+
+// Inclusive FP64 scans over the 64 lanes of a wave: Hillis-Steele steps of
+// 1, 2, 4, 8 lanes inside each 16-lane row, then the row totals carried to
+// the rows after them (row_bcast15 into rows 1 and 3, row_bcast31 into
+// rows 2 and 3).  Lanes without a partner take the identity.
+__device__ __forceinline__ double wave_scan_add64(double v) {
+    v += dpp_or_d<0x111>(v, 0.0);
+    v += dpp_or_d<0x112>(v, 0.0);
+    v += dpp_or_d<0x114>(v, 0.0);
+    v += dpp_or_d<0x118>(v, 0.0);
+    v += dpp_or_d<0x142, 0xA>(v, 0.0);
+    v += dpp_or_d<0x143, 0xC>(v, 0.0);
+    return v;
+}
+__device__ __forceinline__ double wave_scan_mul64(double v) {
+    v *= dpp_or_d<0x111>(v, 1.0);
+    v *= dpp_or_d<0x112>(v, 1.0);
+    v *= dpp_or_d<0x114>(v, 1.0);
+    v *= dpp_or_d<0x118>(v, 1.0);
+    v *= dpp_or_d<0x142, 0xA>(v, 1.0);
+    v *= dpp_or_d<0x143, 0xC>(v, 1.0);
+    return v;
+}
+// v of lane L - 1 (lane 0: `first`).
+__device__ __forceinline__ double wave_shr1_d(double v, double first) {
+    return dpp_or_d<0x138>(v, first);
+}
+
 // v of lane L ^ 16 and L ^ 32.
 __device__ __forceinline__ uint32_t xor16_u(uint32_t v) {
     const auto r = __builtin_amdgcn_permlane16_swap(v, v, false, false);

@@ -52,24 +52,27 @@ __device__ __forceinline__ double slot_sum(double v) {
     v += xor32_d(v);
     return v;
 }
-// Inclusive scans over slots (stride 8 lanes) in FP64.
-__device__ __forceinline__ double slot_scan_add(double v, int lane) {
-    double y = shfl_up64(v, 8);
-    if (lane >= 8) v += y;
-    y = shfl_up64(v, 16);
-    if (lane >= 16) v += y;
-    y = shfl_up64(v, 32);
-    if (lane >= 32) v += y;
+// Inclusive scans over slots of values replicated across each slot's 8
+// lanes, in FP64: the second half of each 16-lane row adds the first
+// (row_shr:8), then the row totals are carried as in wave_scan_add64.
+__device__ __forceinline__ double slot_scan_add(double v) {
+    v += dpp_or_d<0x118>(v, 0.0);
+    v += dpp_or_d<0x142, 0xA>(v, 0.0);
+    v += dpp_or_d<0x143, 0xC>(v, 0.0);
     return v;
 }
-__device__ __forceinline__ double slot_scan_mul(double v, int lane) {
-    double y = shfl_up64(v, 8);
-    if (lane >= 8) v *= y;
-    y = shfl_up64(v, 16);
-    if (lane >= 16) v *= y;
-    y = shfl_up64(v, 32);
-    if (lane >= 32) v *= y;
+__device__ __forceinline__ double slot_scan_mul(double v) {
+    v *= dpp_or_d<0x118>(v, 1.0);
+    v *= dpp_or_d<0x142, 0xA>(v, 1.0);
+    v *= dpp_or_d<0x143, 0xC>(v, 1.0);
     return v;
+}
+// v of the slot before (slot 0: `first`): the first half of each row from
+// the previous row's lane 15, the second half from the first half.
+__device__ __forceinline__ double slot_prev(double v, double first, int lane) {
+    const double from_half = dpp_or_d<0x118>(v, first);
+    const double from_row = dpp_or_d<0x142, 0xE>(v, first);
+    return (lane & 8) ? from_half : from_row;
 }
 
 // Record of P parameters as floats, loaded with 16-byte accesses.
@@ -667,12 +670,11 @@ __device__ __forceinline__ void march8_ray(const RenderArgs &a, const int64_t b)
             const float alpha = active ? one_minus_exp_neg<float>(sigma * fstep) : 0.0f;
             // Transmittance before each slot: exclusive product-scan.
             const double keep = 1.0 - (double)alpha;
-            const double incl = slot_scan_mul(keep, lane);
-            double excl = shfl_up64(incl, 8);
-            if (lane < 8) excl = 1.0;
+            const double incl = slot_scan_mul(keep);
+            const double excl = slot_prev(incl, 1.0, lane);
             const double Ts = T * excl;
             const double ws = Ts * (double)alpha;
-            const double total_keep = __shfl(incl, 63, 64);
+            const double total_keep = readlane_d(incl, 63);
 
             if (!bwd) {
                 if (logging) {
@@ -701,10 +703,10 @@ __device__ __forceinline__ void march8_ray(const RenderArgs &a, const int64_t b)
                 T *= total_keep;
             } else {
                 // Prefix sums up to and including each slot.
-                const double Ai = Acc + slot_scan_add(ws, lane);
-                const double H0 = Cr + slot_scan_add(ws * (double)col0, lane);
-                const double H1 = Cg + slot_scan_add(ws * (double)col1, lane);
-                const double H2 = Cb + slot_scan_add(ws * (double)col2, lane);
+                const double Ai = Acc + slot_scan_add(ws);
+                const double H0 = Cr + slot_scan_add(ws * (double)col0);
+                const double H1 = Cg + slot_scan_add(ws * (double)col1);
+                const double H2 = Cb + slot_scan_add(ws * (double)col2);
                 const double Tn = Ts * keep;
                 float g0, g1, g2;
                 double gs;
@@ -748,10 +750,10 @@ __device__ __forceinline__ void march8_ray(const RenderArgs &a, const int64_t b)
                         acc[h] = fmaf(wx * wyz, coef, acc[h]);
                     }
                 }
-                Acc = __shfl(Ai, 63, 64);
-                Cr = __shfl(H0, 63, 64);
-                Cg = __shfl(H1, 63, 64);
-                Cb = __shfl(H2, 63, 64);
+                Acc = readlane_d(Ai, 63);
+                Cr = readlane_d(H0, 63);
+                Cg = readlane_d(H1, 63);
+                Cb = readlane_d(H2, 63);
                 T *= total_keep;
             }
         }
@@ -808,24 +810,6 @@ __device__ __forceinline__ void march8_ray(const RenderArgs &a, const int64_t b)
             }
         }
     }
-}
-
-// Inclusive scans over the 64 lanes of a wave in FP64.
-__device__ __forceinline__ double wave_scan_add64(double v, int lane) {
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const double y = __shfl_up(v, off, 64);
-        if (lane >= off) v += y;
-    }
-    return v;
-}
-__device__ __forceinline__ double wave_scan_mul64(double v, int lane) {
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const double y = __shfl_up(v, off, 64);
-        if (lane >= off) v *= y;
-    }
-    return v;
 }
 
 // Gradient of the rays k_march8 logged (second kernel of the two-kernel
@@ -929,14 +913,13 @@ __global__ __launch_bounds__(256) void k_log_prefix(const RenderArgs a) {
         const size_t row = (size_t)b * (size_t)a.log_cap + (size_t)(c0 + lane);
         const LoggedSample<ADAPT> v = read_logged<ADAPT>(a, row, true, step, 0.0);
         const double keep = 1.0 - (double)v.alpha;
-        const double incl = wave_scan_mul64(keep, lane);
-        double excl = __shfl_up(incl, 1, 64);
-        if (lane == 0) excl = 1.0;
+        const double incl = wave_scan_mul64(keep);
+        const double excl = wave_shr1_d(incl, 1.0);
         const double ws = T * excl * (double)v.alpha;
-        const double Ai = Acc + wave_scan_add64(ws, lane);
-        const double H0 = Cr + wave_scan_add64(ws * (double)v.col0, lane);
-        const double H1 = Cg + wave_scan_add64(ws * (double)v.col1, lane);
-        const double H2 = Cb + wave_scan_add64(ws * (double)v.col2, lane);
+        const double Ai = Acc + wave_scan_add64(ws);
+        const double H0 = Cr + wave_scan_add64(ws * (double)v.col0);
+        const double H1 = Cg + wave_scan_add64(ws * (double)v.col1);
+        const double H2 = Cb + wave_scan_add64(ws * (double)v.col2);
         T *= readlane_d(incl, 63);
         Acc = readlane_d(Ai, 63);
         Cr = readlane_d(H0, 63);
@@ -1035,15 +1018,14 @@ __device__ __forceinline__ void log_grad_ray(const RenderArgs &a, const int64_t 
         const auto sraw = v.sraw, col0 = v.col0, col1 = v.col1, col2 = v.col2;
         const auto alpha = v.alpha;
         const double keep = 1.0 - (double)alpha;
-        const double incl = wave_scan_mul64(keep, lane);
-        double excl = __shfl_up(incl, 1, 64);
-        if (lane == 0) excl = 1.0;
+        const double incl = wave_scan_mul64(keep);
+        const double excl = wave_shr1_d(incl, 1.0);
         const double Ts = T * excl;
         const double ws = Ts * (double)alpha;
-        const double Ai = Acc + wave_scan_add64(ws, lane);
-        const double H0 = Cr + wave_scan_add64(ws * (double)col0, lane);
-        const double H1 = Cg + wave_scan_add64(ws * (double)col1, lane);
-        const double H2 = Cb + wave_scan_add64(ws * (double)col2, lane);
+        const double Ai = Acc + wave_scan_add64(ws);
+        const double H0 = Cr + wave_scan_add64(ws * (double)col0);
+        const double H1 = Cg + wave_scan_add64(ws * (double)col1);
+        const double H2 = Cb + wave_scan_add64(ws * (double)col2);
         const double Tn = Ts * keep;
         float g0, g1, g2;
         double gs;
