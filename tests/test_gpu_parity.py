@@ -635,7 +635,7 @@ def test_train_step_c1(c1):
 @pytest.mark.parametrize("log_samples,autotune,tv_mode",
                          [(0, False, "fused"), (512, False, "fused"), (512, True, "overlap"),
                           (512, False, "overlap"), (512, False, "inline"),
-                          (512, False, "binned")])
+                          (512, False, "binned"), (512, True, "fused_o28g32")])
 def test_trainer_matches_stage_calls(c1, log_samples, autotune, tv_mode):
     """The C++ batch loop (pxl_trainer_step) issues the same stages as the
     Python-driven sequence: dense grid (TV every batch), 3 batches, LR decay
@@ -645,7 +645,9 @@ def test_trainer_matches_stage_calls(c1, log_samples, autotune, tv_mode):
     current records are in its own, pxl_trainer_sync brings them back), or
     separate, on the second stream beside the forward (overlap) or after the
     render (inline); "binned": fused TV + Adam with the brick-binned
-    gradient schedule."""
+    gradient schedule; "fused_o28g32": the bench's optimiser layout on the
+    trainer's side (moments of 28 floats, gradient rows of 32), autotuned,
+    against the stage calls on records of 32."""
     import math
 
     import torch
@@ -661,19 +663,23 @@ def test_trainer_matches_stage_calls(c1, log_samples, autotune, tv_mode):
         rec = c1["rec"].clone()
         g = pxl.Grid(N1, N1, N1, S1, rec, step=0.5)
         grids.append(g)
-        optims.append(pxl.Optim(g))
+        if tv_mode == "fused_o28g32" and not optims:
+            optims.append(pxl.Optim(g, stride=28, grad_stride=32))
+        else:
+            optims.append(pxl.Optim(g))
     rays_a = device_rays(c1["rays_np"], dev)
     rays_b = device_rays(c1["rays_np"], dev)
     tr = pxl.Trainer(grids[0], optims[0], rays_a, order, batch_size=B, learning_rate=0.1,
                      sh_learning_rate=0.2, variance_coefficient=1e-5, sh_coefficient=1e-6,
                      variance_interval=3, fused=True, log_samples=log_samples,
                      autotune=autotune, tv_inline=tv_mode == "inline",
-                     tv_separate=tv_mode not in ("fused", "binned"),
+                     tv_separate=tv_mode not in ("fused", "binned", "fused_o28g32"),
                      binned=tv_mode == "binned")
     for _ in range(3):
         tr.step()
     current = tr.grid_c().params
-    assert (current != grids[0].params.data_ptr()) == (tv_mode in ("fused", "binned"))
+    assert (current != grids[0].params.data_ptr()) == (tv_mode in ("fused", "binned",
+                                                                    "fused_o28g32"))
     tr.sync()
     assert tr.grid_c().params == grids[0].params.data_ptr()
     losses = torch.zeros(B, dtype=torch.float64, device=dev)
