@@ -137,7 +137,10 @@ def _train(c2, si, path):
     import torch
 
     grid = _grid(c2)
-    optim = pxl.Optim(grid)
+    # "logged_g32": the bench's layout, gradient rows of 32 floats beside
+    # moments of 28.
+    optim = (pxl.Optim(grid, stride=28, grad_stride=32) if path == "logged_g32"
+             else pxl.Optim(grid))
     n = si.numel()
     losses = torch.zeros(n, dtype=torch.float64, device=c2["dev"])
     if path == "fused":
@@ -248,7 +251,8 @@ def c2_full_ref(c2, c2_oracle):
     return cols, l_ref, g_ref, g_abs
 
 
-@pytest.mark.parametrize("path", ["fused", "logged", "persistent", "chunked", "binned"])
+@pytest.mark.parametrize("path", ["fused", "logged", "logged_g32", "persistent", "chunked",
+                                  "binned"])
 def test_c2_full_batch_vs_oracle(c2, c2_full_ref, path):
     import torch
 
