@@ -322,43 +322,22 @@ __device__ __forceinline__ double tv_interior_sum(const TvArgs &a, int64_t r, in
     return 7.0 * (double)v[13] - (0.5 * f + 0.25 * e + 0.125 * c);
 }
 
-// Record of the b-th boundary vertex of a dense grid (W, H, D >= 3): the
-// planes z = 0 and z = D-1, then the rows y = 0 and y = H-1 of the planes
-// between, then the columns x = 0 and x = W-1 of the rows between.
-__device__ __forceinline__ int64_t tv_boundary_record(const TvArgs &a, int64_t b) {
-    const int64_t W = a.W, H = a.H, D = a.D;
-    const int64_t fz = W * H, fy = W * (D - 2), fx = (H - 2) * (D - 2);
-    if (b < 2 * fz) return (b < fz ? 0 : (D - 1) * fz) + b % fz;
-    b -= 2 * fz;
-    if (b < 2 * fy) {
-        const int64_t y = b < fy ? 0 : H - 1, e = b % fy;
-        return (1 + e / W) * fz + y * W + e % W;
-    }
-    b -= 2 * fy;
-    const int64_t x = b < fx ? 0 : W - 1, e = b % fx;
-    return (1 + e / (H - 2)) * fz + (1 + e % (H - 2)) * W + x;
-}
-
-// One vertex of addVarianceLossGPU's gather (item: a record, or with
-// boundary_only the item-th boundary vertex) by its group of LPV lanes.
+// One vertex of addVarianceLossGPU's gather (record r) by its group of LPV
+// lanes.
 template <int LPV, int STORE, bool OFF32, bool FUSED>
-__device__ __forceinline__ void tv_general_item(const TvArgs &a, const AdamArgs &ad, int64_t item,
-                                                int g, int p, int boundary_only) {
-    const int64_t r = boundary_only ? tv_boundary_record(a, item) : item;
+__device__ __forceinline__ void tv_general_item(const TvArgs &a, const AdamArgs &ad, int64_t r,
+                                                int g, int p) {
     const float lam = tv_lambda(a, p);
     const bool mine = p < a.P && lam != 0.0f;
     // A lane without a TV term reads element 0 (the line is read anyway).
     const int q = mine ? p : 0;
     AdamElem ae{0.0f, 0.0f, 0.0f, 0.0f};
     if (FUSED && p < a.stride) ae = adam_load<STORE>(a, ad, r, p);
-    bool interior = false;
-    if (!boundary_only) {
-        const uint32_t d = (uint32_t)r;
-        const uint32_t x = d % (uint32_t)a.W, y = (d / (uint32_t)a.W) % (uint32_t)a.H;
-        const uint32_t z = d / ((uint32_t)a.W * (uint32_t)a.H);
-        interior = x - 1u < (uint32_t)a.W - 2u && y - 1u < (uint32_t)a.H - 2u &&
-                   z - 1u < (uint32_t)a.D - 2u;
-    }
+    const uint32_t d = (uint32_t)r;
+    const uint32_t x = d % (uint32_t)a.W, y = (d / (uint32_t)a.W) % (uint32_t)a.H;
+    const uint32_t z = d / ((uint32_t)a.W * (uint32_t)a.H);
+    const bool interior = x - 1u < (uint32_t)a.W - 2u && y - 1u < (uint32_t)a.H - 2u &&
+                          z - 1u < (uint32_t)a.D - 2u;
     double tsum = 0.0;
     bool tvl = false;
     if (interior) {
@@ -380,17 +359,24 @@ __device__ __forceinline__ void tv_general_item(const TvArgs &a, const AdamArgs 
     if (p < a.stride) adam_store<STORE>(a, ad, r, p, ae, tvl, lam, tsum);
 }
 
-// Interior vertices of a dense grid, marched along z.  With w(0) = 2 and
-// w(+-1) = 1, m_i = w(dx) w(dy) w(dz), so the interior gradient is
-//   8 u - B / 8,  B = sum_z' w(z' - z) Bxy(z'),  Bxy(z') = sum w(dx) w(dy) u
-// over the 3 x 3 vertices around (x, y) in plane z'.  One group of LPV lanes
+// Dense grids, marched along z.  Along each axis a vertex has a cell below
+// (x >= 1) and one above (x <= W - 2); the cells around it that contain a
+// neighbour at offset (dx, dy, dz) number m = wx(dx) wy(dy) wz(dz) with
+// w(-1) = [cell below], w(+1) = [cell above], w(0) = their sum, and
+// n = wx(0) wy(0) wz(0) cells surround the vertex, so its gradient is
+//   n u - B / 8,  B = sum_z' wz(z' - z) Bxy(z'),  Bxy(z') = sum wx wy u
+// over the 3 x 3 vertices around (x, y) in plane z'.  Inside the grid the
+// weights are [1, 2, 1] on every axis and n = 8.  One group of LPV lanes
 // (one per parameter) walks column (x, y) over kTvChunk vertices, keeping
 // Bxy of the planes below and at z: 9 loads per vertex instead of 27.
-// Columns are numbered x fastest, then y, then the z chunk; the blocks past
-// the columns' gather the boundary vertices in the same launch.  Loads go
-// through a buffer resource (the records span less than 4 GiB): lane offset
-// in a vector register, neighbour offset in a scalar one.  FUSED: each
-// vertex's record then takes its Adam step into ad.out.
+// Columns are numbered x fastest, then y, then the z chunk; every column of
+// the grid (W, H >= 3), its faces included, is one of them.  The 3 x 3
+// block loaded per plane stays inside the grid: on a face it is shifted one
+// column (row) inwards, and the neighbour it then holds past the vertex
+// takes weight 0.  Loads go through a buffer resource (the records span
+// less than 4 GiB): lane offset in a vector register, neighbour offset in a
+// scalar one.  FUSED: each vertex's record then takes its Adam step into
+// ad.out.
 constexpr int kTvChunk = 128;
 
 __device__ __forceinline__ double tv_blur_xy(const float (&v)[9]) {
@@ -399,40 +385,52 @@ __device__ __forceinline__ double tv_blur_xy(const float (&v)[9]) {
     return 4.0 * (double)v[4] + 2.0 * e + c;
 }
 
+// Bxy of a column on a face of the grid from its block v (row-major, x
+// fastest), shifted by sx (sy): -1 on the face x = 0 (y = 0), +1 on the
+// face x = W-1 (y = H-1), 0 inside, so the vertex's own column (row) in
+// the block is 1 + sx (1 + sy).  u: the vertex's own value.
+__device__ __forceinline__ double tv_blur_face(const float (&v)[9], int sx, int sy, double &u) {
+    double c[3], r[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const double a0 = (double)v[3 * i], a1 = (double)v[3 * i + 1], a2 = (double)v[3 * i + 2];
+        c[i] = sx < 0 ? a0 : (sx > 0 ? a2 : a1);
+        r[i] = sx == 0 ? 2.0 * a1 + (a0 + a2) : c[i] + a1;
+    }
+    u = sy < 0 ? c[0] : (sy > 0 ? c[2] : c[1]);
+    if (sy == 0) return 2.0 * r[1] + (r[0] + r[2]);
+    return (sy < 0 ? r[0] : r[2]) + r[1];
+}
+
 template <int LPV, int STORE, bool FUSED>
 __global__ __launch_bounds__(256) void k_tv_zmarch(const TvArgs a, const AdamArgs ad,
-                                                   int64_t n_cols, int64_t col_blocks,
-                                                   int64_t n_boundary) {
+                                                   int64_t n_cols) {
     constexpr int kGroups = 256 / LPV;
     constexpr uint32_t kElem = STORE == PXL_STORE_F32 ? 4u : 2u;
     const int g = threadIdx.x / LPV, p = threadIdx.x % LPV;
-    if ((int64_t)blockIdx.x >= col_blocks) {
-        // The blocks past the columns' (dispatched after them) gather the
-        // boundary vertices.
-        const int64_t item = ((int64_t)blockIdx.x - col_blocks) * kGroups + g;
-        if (item < n_boundary) tv_general_item<LPV, STORE, true, FUSED>(a, ad, item, g, p, 1);
-        return;
-    }
-    // The column blocks alone are sliced over the XCDs.
-    const int64_t col = (int64_t)xcd_slice_block(blockIdx.x, (unsigned)col_blocks) * kGroups + g;
+    const int64_t col = (int64_t)xcd_slice_block(blockIdx.x, gridDim.x) * kGroups + g;
     if (col >= n_cols) return;
     const float lam = tv_lambda(a, p);
     const bool tvl = p < a.P && lam != 0.0f;
     // Alone, only the lanes with a TV term work; fused with Adam, every lane
     // of the record.
     if (FUSED ? p >= a.stride : !tvl) return;
-    const int nx = a.W - 2, ny = a.H - 2;
-    const int x = 1 + (int)(col % nx), y = 1 + (int)((col / nx) % ny);
-    const int z0 = 1 + (int)(col / ((int64_t)nx * ny)) * kTvChunk;
-    const int z1 = min(z0 + kTvChunk, a.D - 1);  // vertices z0 .. z1-1, planes to z1
+    const int x = (int)(col % a.W), y = (int)((col / a.W) % a.H);
+    const int z0 = (int)(col / ((int64_t)a.W * a.H)) * kTvChunk;
+    const int z1 = min(z0 + kTvChunk, a.D);  // vertices z0 .. z1-1
+    const int sx = x == 0 ? -1 : (x == a.W - 1 ? 1 : 0);
+    const int sy = y == 0 ? -1 : (y == a.H - 1 ? 1 : 0);
+    const bool inner = sx == 0 && sy == 0;
     const uint32_t rec_bytes = (uint32_t)a.stride * kElem;
     const uint32_t plane_bytes = (uint32_t)a.W * (uint32_t)a.H * rec_bytes;
     const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
         const_cast<float *>(a.params), (short)0, (int)((uint32_t)a.n_vox * rec_bytes),
         0x00020000);
-    // Byte offset of (x - 1, y - 1, z') and of the 3 x 3 neighbours from it.
-    const uint32_t lane0 = ((uint32_t)(y - 1) * (uint32_t)a.W + (uint32_t)(x - 1)) * rec_bytes +
-                           (uint32_t)p * kElem;
+    // Byte offset of the block's first vertex (x - 1 - sx, y - 1 - sy, z')
+    // and of the 3 x 3 block from it.
+    const uint32_t lane0 =
+        ((uint32_t)(y - 1 - sy) * (uint32_t)a.W + (uint32_t)(x - 1 - sx)) * rec_bytes +
+        (uint32_t)p * kElem;
     uint32_t nb[9];
 #pragma unroll
     for (int i = 0; i < 9; i++) nb[i] = ((uint32_t)(i / 3) * (uint32_t)a.W + (uint32_t)(i % 3)) * rec_bytes;
@@ -443,14 +441,23 @@ __global__ __launch_bounds__(256) void k_tv_zmarch(const TvArgs a, const AdamArg
 #pragma unroll
         for (int i = 0; i < 9; i++) v[i] = ld_buf<STORE>(rs, lane0 + (uint32_t)zz * plane_bytes, nb[i]);
     };
+    // Bxy of a plane's block, and the vertex's own value in it.
+    auto blur = [&](const float (&v)[9], double &c) {
+        if (inner) {
+            c = (double)v[4];
+            return tv_blur_xy(v);
+        }
+        return tv_blur_face(v, sx, sy, c);
+    };
     double b_lo = 0.0, b_mid = 0.0, u = 0.0;
     if (tvl) {
         float v[9];
-        plane(z0 - 1, v);
-        b_lo = tv_blur_xy(v);
+        if (z0 >= 1) {
+            plane(z0 - 1, v);
+            b_lo = blur(v, u);
+        }
         plane(z0, v);
-        b_mid = tv_blur_xy(v);
-        u = (double)v[4];
+        b_mid = blur(v, u);
     }
     // Ring of three slots: the slot of vertex s holds plane s + 1 and the
     // gradient of s (fused: the Adam element of s), loaded three steps
@@ -465,19 +472,28 @@ __global__ __launch_bounds__(256) void k_tv_zmarch(const TvArgs a, const AdamArg
             pa[k] = adam_load<STORE>(a, ad, (int64_t)sc * rplane + rcol, p);
         else
             pg[k] = gcol[(size_t)sc * gplane];
-        if (tvl) plane(min(s + 1, z1), pv[k]);
+        if (tvl) plane(min(s + 1, a.D - 1), pv[k]);
     };
     fill(0, z0);
     fill(1, z0 + 1);
     fill(2, z0 + 2);
+    // Cells around the vertex in its xy plane's two directions.
+    const double nxy = (double)((sx == 0 ? 2 : 1) * (sy == 0 ? 2 : 1));
     auto step = [&](int k, int s, bool refill) {
         double tsum = 0.0;
         if (tvl) {
-            const double b_hi = tv_blur_xy(pv[k]);
-            tsum = 8.0 * u - 0.125 * (b_lo + 2.0 * b_mid + b_hi);
+            // Cells below and above along z (plane s + 1 exists when zp).
+            const int zm = s >= 1, zp = s <= a.D - 2;
+            double u_hi = 0.0;
+            const double b_hi = zp ? blur(pv[k], u_hi) : 0.0;
+            if (inner && zm && zp)
+                tsum = 8.0 * u - 0.125 * (b_lo + 2.0 * b_mid + b_hi);
+            else
+                tsum = nxy * (double)(zm + zp) * u -
+                       0.125 * ((double)(zm + zp) * b_mid + ((zm ? b_lo : 0.0) + b_hi));
             b_lo = b_mid;
             b_mid = b_hi;
-            u = (double)pv[k][4];
+            u = u_hi;
         }
         if (FUSED)
             adam_store<STORE>(a, ad, (int64_t)s * rplane + rcol, p, pa[k], tvl, lam, tsum);
@@ -508,7 +524,7 @@ __global__ __launch_bounds__(256) void k_tv_disparity(const TvArgs a, const Adam
     const int g = threadIdx.x / LPV, p = threadIdx.x % LPV;
     const int64_t item = (int64_t)xcd_slice_block(blockIdx.x, gridDim.x) * kGroups + g;
     if (item >= n_items) return;  // the whole group leaves together
-    tv_general_item<LPV, STORE, OFF32, FUSED>(a, ad, item, g, p, 0);
+    tv_general_item<LPV, STORE, OFF32, FUSED>(a, ad, item, g, p);
 }
 
 // Caged grids: the neighbour records and the cell mask depend only on the
@@ -1095,21 +1111,17 @@ void tv_launch_as(const TvArgs &a, const AdamArgs &ad, void *scratch, hipStream_
     constexpr uint64_t kElem = STORE == PXL_STORE_F32 ? 4u : 2u;
     const bool off32 = !offsets64_forced() &&
                        (uint64_t)a.n_vox * (uint64_t)a.stride * kElem <= 0xFFFFFFFFull;
-    // 32-bit offsets: interior vertices marched along z and the rest
-    // gathered one by one, in one launch; otherwise every vertex gathered.
+    // 32-bit offsets: every vertex marched along z; otherwise every vertex
+    // gathered.
     if (off32 && a.W >= 3 && a.H >= 3 && a.D >= 3) {
-        const int64_t n_cols =
-            (int64_t)(a.W - 2) * (a.H - 2) * ((a.D - 2 + kTvChunk - 1) / kTvChunk);
-        const int64_t cb = (n_cols + groups - 1) / groups;
-        const int64_t nbd = 2 * ((int64_t)a.W * a.H + (int64_t)a.W * (a.D - 2) +
-                                 (int64_t)(a.H - 2) * (a.D - 2));
-        const unsigned nb = (unsigned)(cb + (nbd + groups - 1) / groups);
+        const int64_t n_cols = (int64_t)a.W * a.H * ((a.D + kTvChunk - 1) / kTvChunk);
+        const unsigned nb = (unsigned)((n_cols + groups - 1) / groups);
         if (wide)
             hipLaunchKernelGGL((k_tv_zmarch<64, STORE, FUSED>), dim3(nb), dim3(256), 0, s, a, ad,
-                               n_cols, cb, nbd);
+                               n_cols);
         else
             hipLaunchKernelGGL((k_tv_zmarch<32, STORE, FUSED>), dim3(nb), dim3(256), 0, s, a, ad,
-                               n_cols, cb, nbd);
+                               n_cols);
         return;
     }
     const unsigned ib = (unsigned)((a.n_vox + groups - 1) / groups);
