@@ -135,6 +135,9 @@ DENSE = {
     "f32_alr_step2": (21, 19, 33, 9, pxl.STORE_F32, {"alr": True, "opt_step": 2}),
     "f32_thin": (2, 7, 9, 9, pxl.STORE_F32, {}),
     "f32_tall": (5, 4, 300, 9, pxl.STORE_F32, {}),
+    # The narrowest grid marched along z: every column on a face, its 3 x 3
+    # block the whole xy slab; the last z chunk holds 2 vertices.
+    "f32_3x3": (3, 3, 130, 9, pxl.STORE_F32, {}),
     "f32_addr64": (21, 19, 33, 9, pxl.STORE_F32, {"addr64": True}),
     "f32_band": (21, 19, 33, 9, pxl.STORE_F32, {"sh_active": 4, "sh_first": 1}),
     "bf16_master": (21, 19, 33, 9, pxl.STORE_BF16, {}),
